@@ -1,0 +1,285 @@
+/*
+ * physicell_hip.h -- C-ABI of the MI355X (gfx950) step hot path.
+ *
+ * Two layers, both plain C (no torch or HIP types in any signature):
+ *
+ *   hc_*  hip_context: the drop-in for the reference's MetalContext
+ *         (src/metal_context.h:14-100).  Device, stream, buffers and one
+ *         dispatch call per reference dispatch, plus the kernels the reference
+ *         runs on the CPU (Dirichlet, secretion, voxel keys).
+ *
+ *   pc_*  the host API above it (Microenvironment / CellMechanics / CellData
+ *         and the step loop of src/main.mm:275-415, written in C++ in this
+ *         library) exposed for callers that drive a whole simulation.
+ *
+ * Conventions (shared by every entry point):
+ *   - return 0 on success, a negative hc_status on failure; the message is
+ *     kept in hc_last_error(ctx) (or pc_last_error(sim)).  The reference
+ *     prints and aborts on setup errors (src/metal_context.mm:22-35); the C++
+ *     host classes in this library keep that behaviour, the C-ABI reports.
+ *   - one in-order HIP stream per context; every dispatch is asynchronous and
+ *     ordered after the previous one; hc_wait() is the only blocking call
+ *     (src/metal_context.mm:448-461).
+ *   - device memory is discrete: a host read or write of a buffer is legal
+ *     only through hc_download / hc_upload after hc_wait (the reference shares
+ *     memory with the GPU, src/metal_context.mm:52-60).
+ *   - all fields, positions and forces are IEEE FP64 (the reference is FP32;
+ *     FP64 is the parity form).
+ */
+#ifndef PHYSICELL_HIP_H
+#define PHYSICELL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum hc_status {
+    HC_OK = 0,
+    HC_ERR_ARG = -1,       /* invalid argument (null, size, range)         */
+    HC_ERR_HIP = -2,       /* a HIP runtime call failed                    */
+    HC_ERR_NOMEM = -3,     /* device allocation failed                     */
+    HC_ERR_STATE = -4,     /* call out of order (e.g. kernels not loaded)  */
+    HC_ERR_CAPACITY = -5   /* a fixed capacity would be exceeded            */
+} hc_status;
+
+typedef struct hc_ctx hc_ctx;   /* opaque: device, stream, timing state      */
+typedef struct hc_buf hc_buf;   /* opaque: device allocation + host mirror   */
+
+/* Diffusion grid; FP64 form of GridParams (shaders/types.h:23-38). */
+typedef struct hc_grid {
+    uint32_t nx, ny, nz, n_substrates;
+    double dx, dy, dz, dt;
+    double x_min, y_min, z_min;
+} hc_grid;
+
+/* Per-substrate parameters; FP64 form of SubstrateParams (shaders/types.h:41-49). */
+typedef struct hc_substrate {
+    double diffusion_coefficient;
+    double decay_rate;
+    double supply_rate;
+    double target_value;
+    uint32_t dirichlet_enabled;
+    uint32_t _pad;
+} hc_substrate;
+
+/* Mechanics parameters; FP64 form of MechanicsParams (shaders/types.h:52-63). */
+typedef struct hc_mech {
+    double dt;
+    double cell_cell_repulsion;       /* carried, not read by the kernels (as in the reference) */
+    double cell_cell_adhesion;        /* idem                                                   */
+    double max_interaction_distance;  /* 30 um cutoff                                           */
+    double mechanics_voxel_size;      /* 30 um                                                  */
+    uint32_t mech_grid_nx, mech_grid_ny, mech_grid_nz, _pad;
+} hc_mech;
+
+/* ---- cell SoA layout (device buffer of hc_cells_bytes(max_cells)) ----------
+ * The reference packs 28 four-byte fields x max_cells (src/cell_data.cpp:101-180).
+ * Here fields 0..22 are FP64 arrays of max_cells and fields 23..27 are uint32
+ * arrays of max_cells that follow them; the field numbers are the reference's
+ * CELL_FIELD_* (shaders/types.h:74-102).  Field f (< 23) starts at byte
+ * f*8*max_cells; field f (>= 23) at 23*8*max_cells + (f-23)*4*max_cells.     */
+enum {
+    HC_F_POS_X = 0, HC_F_POS_Y = 1, HC_F_POS_Z = 2,
+    HC_F_VEL_X = 3, HC_F_VEL_Y = 4, HC_F_VEL_Z = 5,
+    HC_F_PREV_VX = 6, HC_F_PREV_VY = 7, HC_F_PREV_VZ = 8,
+    HC_F_RADIUS = 9, HC_F_NUCLEAR_RAD = 10, HC_F_TOTAL_VOL = 11, HC_F_NUCLEAR_VOL = 12,
+    HC_F_REPULSION = 13, HC_F_ADHESION = 14, HC_F_REL_MAX_ADH = 15,
+    HC_F_MOT_SPEED = 16, HC_F_MOT_BIAS = 17, HC_F_MOT_DIR_X = 18, HC_F_MOT_DIR_Y = 19,
+    HC_F_MOT_DIR_Z = 20, HC_F_ONCOPROTEIN = 21, HC_F_PRESSURE = 22,
+    HC_F_CELL_TYPE = 23, HC_F_CUR_PHASE = 24, HC_F_IS_ALIVE = 25,
+    HC_F_VOXEL_IDX = 26, HC_F_MECH_VOXEL = 27,
+    HC_NUM_F64_FIELDS = 23, HC_NUM_U32_FIELDS = 5, HC_NUM_CELL_FIELDS = 28
+};
+size_t hc_cells_bytes(uint32_t max_cells);
+size_t hc_cells_field_offset(uint32_t field, uint32_t max_cells);
+
+/* Per-substrate rate buffer: 4 FP64 arrays [cell*ns + s] of max_cells*ns each,
+ * in the order secretion, uptake, saturation, net export
+ * (src/cell_data.h:148-151). */
+size_t hc_rates_bytes(uint32_t max_cells, uint32_t n_substrates);
+
+/* ---- context and buffers ------------------------------------------------ */
+
+/* MetalContext() (src/metal_context.mm:13-36): opens the device, creates the stream. */
+int  hc_create(hc_ctx** out, int device_ordinal);
+/* ~MetalContext (src/metal_context.mm:41-47): waits for queued work, then frees. */
+void hc_destroy(hc_ctx* ctx);
+/* loadShaders (src/metal_context.mm:66-168).  The gfx950 code object is linked
+ * into this library; the call checks that every kernel is loadable on the
+ * device.  code_object_or_null must be NULL (no external code objects). */
+int  hc_load_kernels(hc_ctx* ctx, const char* code_object_or_null);
+/* allocateBuffer (src/metal_context.mm:52-60); device memory, zero-filled. */
+int  hc_alloc(hc_ctx* ctx, size_t bytes, hc_buf** out);
+void hc_free(hc_buf* buf);
+/* Host staging copy of the buffer (pinned, allocated on first use).  Stands
+ * in for the reference's [buffer contents] view. */
+void* hc_host_mirror(hc_buf* buf);
+void* hc_device_ptr(hc_buf* buf);
+size_t hc_buf_size(const hc_buf* buf);
+/* Explicit copies between the mirror and the device at the reference's sync
+ * points; asynchronous on the context stream (hc_wait completes them). */
+int  hc_upload(hc_ctx* ctx, hc_buf* buf, size_t offset, size_t bytes);
+int  hc_download(hc_ctx* ctx, hc_buf* buf, size_t offset, size_t bytes);
+/* waitForCompletion (src/metal_context.mm:448-461). */
+int  hc_wait(hc_ctx* ctx);
+const char* hc_last_error(const hc_ctx* ctx);
+
+/* ---- diffusion ----------------------------------------------------------- */
+
+/* Size of the line-independent Thomas tables (c' and denominator per
+ * substrate, axis and node); and their construction (host compute, upload). */
+size_t hc_thomas_table_bytes(const hc_grid* grid);
+int hc_build_thomas_tables(hc_ctx* ctx, const hc_grid* grid, const hc_substrate* subs,
+                           hc_buf* tables);
+/* dispatchDiffusionX / dispatchDiffusionY (src/metal_context.mm:223-272),
+ * plus axis 2 (z).  Solves every line of every substrate along `axis` in
+ * place (shaders/diffusion_2d.metal:28-292; edge rows, n<=2 and n==3 cases
+ * as there).  fuse_dirichlet != 0 also re-applies Dirichlet values on every
+ * face (valid only for the last sweep of a substep). */
+int hc_diffusion_sweep(hc_ctx* ctx, int axis, hc_buf* density, const hc_grid* grid,
+                       const hc_substrate* subs, hc_buf* tables, int fuse_dirichlet);
+/* Microenvironment::applyDirichletBC (src/microenvironment.mm:127-152), all
+ * faces (z faces included when nz > 1). */
+int hc_dirichlet(hc_ctx* ctx, hc_buf* density, const hc_grid* grid, const hc_substrate* subs);
+/* One substep: x, y, [z] sweeps with Dirichlet fused into the last sweep
+ * (src/microenvironment.mm:106-115 + src/main.mm:343-347). */
+int hc_diffusion_substep(hc_ctx* ctx, hc_buf* density, const hc_grid* grid,
+                         const hc_substrate* subs, hc_buf* tables);
+
+/* ---- cells: voxel keys and secretion ------------------------------------ */
+
+/* Cell -> diffusion voxel, 3-D formula (src/microenvironment.mm:176-184);
+ * writes field HC_F_VOXEL_IDX.  Replaces the host refresh of
+ * src/cell_phenotype.cpp:778-789. */
+int hc_voxel_keys(hc_ctx* ctx, hc_buf* cells, const hc_grid* grid, uint32_t n, uint32_t max_cells);
+
+/* Secretion plan: live cells with a valid voxel grouped by voxel (ascending
+ * cell index inside a group) and their folded coefficients.  Rebuild it when
+ * voxel keys, volumes or rates change. */
+size_t hc_secretion_plan_bytes(const hc_grid* grid, uint32_t max_cells);
+int hc_secretion_plan(hc_ctx* ctx, hc_buf* cells, hc_buf* rates, const hc_grid* grid,
+                      double dt, uint32_t n, uint32_t max_cells, hc_buf* plan);
+/* processSecretion for every planned cell (src/cell_phenotype.cpp:526-564
+ * called from src/main.mm:352-363): within a voxel, cells are applied one
+ * after another in ascending index; no floating-point atomics. */
+int hc_secretion_apply(hc_ctx* ctx, hc_buf* density, hc_buf* plan, const hc_grid* grid);
+/* plan + apply in one call (the per-substep call of the reference). */
+int hc_secretion(hc_ctx* ctx, hc_buf* density, hc_buf* cells, hc_buf* rates,
+                 const hc_grid* grid, double dt, uint32_t n, uint32_t max_cells, hc_buf* plan);
+
+/* ---- mechanics ----------------------------------------------------------- */
+
+/* Hash workspace: counting-sort bins over n_mvox mech voxels (offsets,
+ * ascending-index cell list, hash-sorted cell records). */
+size_t hc_hash_bytes(uint32_t n_mvox, uint32_t max_cells);
+/* clear_spatial_hash + build_spatial_hash (shaders/mechanics.metal:27-85) as
+ * a counting sort: key (writes HC_F_MECH_VOXEL) -> wave-aggregated histogram
+ * -> prefix scan -> stable scatter.  No 64-per-voxel cap. */
+int hc_build_hash(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech,
+                  const hc_grid* mgrid, uint32_t n, uint32_t max_cells, uint32_t n_mvox);
+/* compute_forces (shaders/mechanics.metal:91-216) over the hash: writes
+ * velocity = force and simple pressure. */
+int hc_forces(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech,
+              uint32_t n, uint32_t max_cells, uint32_t n_mvox);
+/* dispatchForcesOnlyPipeline (src/metal_context.mm:390-443): hash + forces. */
+int hc_forces_only_pipeline(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech,
+                            const hc_grid* mgrid, uint32_t n, uint32_t max_cells, uint32_t n_mvox);
+/* dispatchIntegrate (src/metal_context.mm:360-383): AB2, clamp to the
+ * mech-grid extent [min, min + n*voxel], v_prev <- v (shaders/integrate.metal:31-80). */
+int hc_integrate(hc_ctx* ctx, hc_buf* cells, const hc_grid* mgrid, double dt,
+                 uint32_t n, uint32_t max_cells);
+/* dispatchMechanicsPipeline (src/metal_context.mm:468-541): hash, forces, AB2. */
+int hc_mechanics_pipeline(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech,
+                          const hc_grid* mgrid, double dt, uint32_t n, uint32_t max_cells,
+                          uint32_t n_mvox);
+/* Hash accessors of CellMechanics (src/cell_mechanics.h:53-62): downloads
+ * the bin offsets (n_mvox+1) and the ascending-index cell list (n) so that
+ * voxelCellCount(v) = off[v+1]-off[v] and voxelCells(v) = &list[off[v]]. */
+int hc_hash_download(hc_ctx* ctx, hc_buf* hash, uint32_t n_mvox, uint32_t max_cells, uint32_t n,
+                     uint32_t* offsets_out, uint32_t* cells_out);
+/* Largest bin population of the last hash build (the reference drops cells
+ * beyond 64 per bucket, shaders/mechanics.metal:82-84; here none are dropped
+ * and this reports how far past 64 a bucket went). */
+int hc_hash_max_bin(hc_ctx* ctx, hc_buf* hash, uint32_t n_mvox, uint32_t* max_out);
+/* Neighbour lists through the hash: for each cell the candidates passing the
+ * force kernel's predicates (same or adjacent mech voxel, d^2 < cutoff^2,
+ * d^2 >= 1e-12), in visiting order.  counts[n] (device), idx[n*cap] (device);
+ * entries past cap are counted but not stored. */
+int hc_neighbor_lists(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech,
+                      uint32_t n, uint32_t max_cells, uint32_t n_mvox, uint32_t cap,
+                      hc_buf* counts, hc_buf* idx);
+
+/* ---- timing ------------------------------------------------------------- */
+/* HIP events on the context stream around every launch of the selected
+ * kernels (bit mask of hc_kernel_id); accumulated per kernel. */
+typedef enum hc_kernel_id {
+    HC_K_SWEEP_X = 0, HC_K_SWEEP_Y = 1, HC_K_SWEEP_Z = 2, HC_K_SWEEP_SMALL = 3,
+    HC_K_DIRICHLET = 4, HC_K_VOXEL_KEYS = 5, HC_K_SEC_PLAN = 6, HC_K_SEC_APPLY = 7,
+    HC_K_HASH = 8, HC_K_FORCES = 9, HC_K_INTEGRATE = 10, HC_K_COUNT = 11
+} hc_kernel_id;
+int hc_timing_enable(hc_ctx* ctx, uint32_t kernel_mask);
+/* Waits, then returns the summed event time (ms) and launch count of `kernel`. */
+int hc_timing_get(hc_ctx* ctx, int kernel, double* total_ms, int64_t* launches);
+int hc_timing_reset(hc_ctx* ctx);
+
+/* ==== pc_*: the host simulation (Microenvironment + CellMechanics + CellData
+ * + the step loop), C++ in this library ==================================== */
+
+typedef struct pc_sim pc_sim;
+
+typedef struct pc_substrate_cfg {
+    double diffusion_coeff, decay_rate, initial_value, dirichlet_value;
+    uint32_t dirichlet_enabled, _pad;
+} pc_substrate_cfg;
+
+/* Subset of SimConfig (src/config_parser.h) that the path reads. */
+typedef struct pc_config {
+    double x_min, x_max, y_min, y_max, z_min, z_max;
+    double dx, dy, dz;
+    uint32_t use_2D, n_substrates;
+    double dt_diffusion, dt_mechanics, dt_phenotype, max_time;
+    uint32_t max_cells, _pad;
+    const pc_substrate_cfg* substrates;
+} pc_config;
+
+int  pc_sim_create(pc_sim** out, const pc_config* cfg, int device_ordinal);
+void pc_sim_destroy(pc_sim* sim);
+const char* pc_last_error(const pc_sim* sim);
+/* Places n cells (CellData::addCell + the fields the path reads); rate arrays
+ * are [cell*ns + s]; alive may be NULL (all alive).  Sets voxel keys with the
+ * 3-D formula, uploads, and plans secretion. */
+int pc_sim_set_cells(pc_sim* sim, uint32_t n, const double* x, const double* y, const double* z,
+                     const double* radius, const double* volume, const double* repulsion,
+                     const double* adhesion, const double* rel_max_adhesion,
+                     const double* secretion, const double* uptake, const double* saturation,
+                     const double* net_export, const uint32_t* alive);
+/* One diffusion substep: sweeps + Dirichlet + secretion. */
+int pc_sim_diffusion_substep(pc_sim* sim);
+/* One mechanics step: hash + forces + AB2 (dt = dt_mechanics). */
+int pc_sim_mechanics_step(pc_sim* sim);
+/* Phenotype-cadence hook on the path: voxel-key refresh + secretion re-plan. */
+int pc_sim_phenotype_step(pc_sim* sim);
+/* The step loop of src/main.mm:275-415 up to max_time (Appendix B order);
+ * counts may be NULL (int64[3]: substeps, mechanics, phenotype). */
+int pc_sim_run(pc_sim* sim, double max_time, int64_t* counts);
+/* Run `periods` x (steps_per_mech diffusion substeps + 1 mechanics step);
+ * use_graph != 0 captures one period as a hipGraph and replays it. */
+int pc_sim_run_periods(pc_sim* sim, int periods, int use_graph);
+int pc_sim_wait(pc_sim* sim);
+/* Copies out (waits first).  density: ns*nx*ny*nz FP64. */
+int pc_sim_get_density(pc_sim* sim, double* out);
+int pc_sim_set_density(pc_sim* sim, const double* in);
+/* field: an HC_F_* number; out has n entries (double or uint32 by field). */
+int pc_sim_get_field(pc_sim* sim, uint32_t field, void* out);
+int pc_sim_get_hash(pc_sim* sim, uint32_t* offsets, uint32_t* cells);
+int pc_sim_info(const pc_sim* sim, hc_grid* grid, hc_mech* mech, uint32_t* n_cells);
+hc_ctx* pc_sim_context(pc_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHYSICELL_HIP_H */

@@ -1,0 +1,235 @@
+"""ctypes binding of the FP64 oracle (liboracle.so) and of the reference's own
+processSecretion build (oracle/_ref/libref_path.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, where it is the checker / the timed CPU
+baseline.  The product path never imports this module.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_LIB_PATH = os.path.join(HERE, "_ref", "libref_path.so")
+
+u32p = ctypes.POINTER(ctypes.c_uint32)
+f64p = ctypes.POINTER(ctypes.c_double)
+f32p = ctypes.POINTER(ctypes.c_float)
+
+
+class OrGrid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_uint32), ("ny", ctypes.c_uint32), ("nz", ctypes.c_uint32),
+                ("ns", ctypes.c_uint32), ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+                ("dz", ctypes.c_double), ("dt", ctypes.c_double), ("x_min", ctypes.c_double),
+                ("y_min", ctypes.c_double), ("z_min", ctypes.c_double)]
+
+
+class OrSub(ctypes.Structure):
+    _fields_ = [("D", ctypes.c_double), ("decay", ctypes.c_double), ("target", ctypes.c_double),
+                ("dirichlet", ctypes.c_int32), ("_pad", ctypes.c_int32)]
+
+
+class OrMech(ctypes.Structure):
+    _fields_ = [("mnx", ctypes.c_uint32), ("mny", ctypes.c_uint32), ("mnz", ctypes.c_uint32),
+                ("_pad", ctypes.c_uint32), ("voxel", ctypes.c_double), ("max_dist", ctypes.c_double),
+                ("x_min", ctypes.c_double), ("y_min", ctypes.c_double), ("z_min", ctypes.c_double)]
+
+
+class OrState(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint32)] + [
+        (k, f64p) for k in ("x", "y", "z", "vx", "vy", "vz", "pvx", "pvy", "pvz", "radius",
+                            "volume", "rep", "adh", "rel_adh", "pressure", "sec", "upt",
+                            "sat", "net")] + [
+        ("alive", u32p), ("voxel", u32p), ("mech_key", u32p), ("rho", f64p)]
+
+
+class OrSchedule(ctypes.Structure):
+    _fields_ = [("dt_diff", ctypes.c_double), ("dt_mech", ctypes.c_double),
+                ("dt_pheno", ctypes.c_double), ("max_time", ctypes.c_double)]
+
+
+class OrCounts(ctypes.Structure):
+    _fields_ = [("substeps", ctypes.c_int64), ("mech_steps", ctypes.c_int64),
+                ("pheno_steps", ctypes.c_int64)]
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+        _lib = ctypes.CDLL(LIB_PATH)
+    return _lib
+
+
+def ref_lib():
+    """The reference's own processSecretion, or None when it was not built."""
+    global _ref
+    if _ref is None and os.path.exists(REF_LIB_PATH):
+        _ref = ctypes.CDLL(REF_LIB_PATH)
+    return _ref
+
+
+def P(a, t=f64p):
+    return a.ctypes.data_as(t)
+
+
+def grid(spec):
+    """spec: physicell_amd.scenarios.Scenario (or anything with the same fields)."""
+    return OrGrid(spec.nx, spec.ny, spec.nz, len(spec.substrates), spec.dx, spec.dy, spec.dz,
+                  spec.dt_diff, spec.x_min, spec.y_min, spec.z_min)
+
+
+def subs(spec):
+    arr = (OrSub * max(1, len(spec.substrates)))()
+    for i, s in enumerate(spec.substrates):
+        arr[i] = OrSub(s.D, s.decay, s.dirichlet_value, 1 if s.dirichlet else 0, 0)
+    return arr
+
+
+def mech(spec):
+    return OrMech(spec.mnx, spec.mny, spec.mnz, 0, spec.mech_voxel, spec.mech_voxel,
+                  spec.x_min, spec.y_min, spec.z_min)
+
+
+def sweep(axis, rho, g, s, nthreads=1):
+    lib().or_sweep(ctypes.c_int(axis), P(rho), ctypes.byref(g), s, ctypes.c_int(nthreads))
+
+
+def dirichlet(rho, g, s):
+    lib().or_dirichlet(P(rho), ctypes.byref(g), s)
+
+
+def diffusion_substep(rho, g, s, nthreads=1):
+    lib().or_diffusion_substep(P(rho), ctypes.byref(g), s, ctypes.c_int(nthreads))
+
+
+def voxel_keys(x, y, z, g):
+    out = np.zeros(len(x), np.uint32)
+    lib().or_voxel_keys(P(x), P(y), P(z), ctypes.c_uint32(len(x)), ctypes.byref(g), P(out, u32p))
+    return out
+
+
+def secretion(rho, g, dt, voxel, volume, sec, upt, sat, net, alive=None, grouped=False, nthreads=1):
+    n = len(voxel)
+    al = P(alive, u32p) if alive is not None else None
+    if grouped:
+        lib().or_secretion_grouped(P(rho), ctypes.byref(g), ctypes.c_double(dt), ctypes.c_uint32(n),
+                                   P(voxel, u32p), P(volume), P(sec), P(upt), P(sat), P(net), al,
+                                   ctypes.c_int(nthreads))
+    else:
+        lib().or_secretion(P(rho), ctypes.byref(g), ctypes.c_double(dt), ctypes.c_uint32(n),
+                           P(voxel, u32p), P(volume), P(sec), P(upt), P(sat), P(net), al)
+
+
+def mech_keys(x, y, z, m):
+    out = np.zeros(len(x), np.uint32)
+    lib().or_mech_keys(P(x), P(y), P(z), ctypes.c_uint32(len(x)), ctypes.byref(m), P(out, u32p))
+    return out
+
+
+def hash_sort(keys, nbins):
+    off = np.zeros(nbins + 1, np.uint32)
+    srt = np.zeros(len(keys), np.uint32)
+    lib().or_hash(P(keys, u32p), ctypes.c_uint32(len(keys)), ctypes.c_uint32(nbins),
+                  P(off, u32p), P(srt, u32p))
+    return off, srt
+
+
+def forces(x, y, z, radius, rep, adh, rel, keys, off, srt, m, nthreads=1):
+    n = len(x)
+    vx, vy, vz, pr = (np.zeros(n) for _ in range(4))
+    lib().or_forces(P(x), P(y), P(z), P(radius), P(rep), P(adh), P(rel), ctypes.c_uint32(n),
+                    P(keys, u32p), P(off, u32p), P(srt, u32p), ctypes.byref(m),
+                    P(vx), P(vy), P(vz), P(pr), ctypes.c_int(nthreads))
+    return vx, vy, vz, pr
+
+
+def neighbors(x, y, z, keys, off, srt, m, cap=128):
+    n = len(x)
+    counts = np.zeros(n, np.uint32)
+    idx = np.zeros(n * cap, np.uint32)
+    lib().or_neighbors(P(x), P(y), P(z), ctypes.c_uint32(n), P(keys, u32p), P(off, u32p),
+                       P(srt, u32p), ctypes.byref(m), ctypes.c_uint32(cap), P(counts, u32p),
+                       P(idx, u32p))
+    return counts, idx.reshape(n, cap)
+
+
+def integrate(x, y, z, vx, vy, vz, pvx, pvy, pvz, m, dt, nthreads=1):
+    lib().or_integrate(P(x), P(y), P(z), P(vx), P(vy), P(vz), P(pvx), P(pvy), P(pvz),
+                       ctypes.c_uint32(len(x)), ctypes.byref(m), ctypes.c_double(dt),
+                       ctypes.c_int(nthreads))
+
+
+class OracleSim:
+    """Whole-simulation state driven by or_run / or_mech_step (numpy-owned arrays)."""
+
+    def __init__(self, spec, cells):
+        self.spec = spec
+        self.g = grid(spec)
+        self.s = subs(spec)
+        self.m = mech(spec)
+        n = cells.n
+        ns = len(spec.substrates)
+        self.arr = {}
+        for k in ("x", "y", "z"):
+            self.arr[k] = np.array(getattr(cells, k), dtype=np.float64, copy=True)
+        for k in ("vx", "vy", "vz", "pvx", "pvy", "pvz", "pressure"):
+            self.arr[k] = np.zeros(n)
+        for k in ("radius", "volume", "rep", "adh", "rel_adh"):
+            self.arr[k] = np.array(getattr(cells, k), dtype=np.float64, copy=True)
+        for k in ("sec", "upt", "sat", "net"):
+            self.arr[k] = np.ascontiguousarray(np.array(getattr(cells, k), dtype=np.float64).reshape(n * ns if ns else 0))
+        self.arr["alive"] = np.array(cells.alive, dtype=np.uint32, copy=True)
+        self.arr["mech_key"] = np.zeros(n, np.uint32)
+        self.arr["rho"] = spec.initial_density().copy()
+        self.arr["voxel"] = voxel_keys(self.arr["x"], self.arr["y"], self.arr["z"], self.g)
+        self.off = np.zeros(spec.mnx * spec.mny * spec.mnz + 1, np.uint32)
+        self.srt = np.zeros(max(1, n), np.uint32)
+        st = OrState()
+        st.n = n
+        for k, _t in OrState._fields_[1:]:
+            a = self.arr[k]
+            setattr(st, k, P(a, u32p) if a.dtype == np.uint32 else P(a))
+        self.st = st
+
+    def run(self, max_time, nthreads=1):
+        sch = OrSchedule(self.spec.dt_diff, self.spec.dt_mech, self.spec.dt_pheno, max_time)
+        c = OrCounts()
+        lib().or_run(ctypes.byref(self.st), ctypes.byref(self.g), self.s, ctypes.byref(self.m),
+                     ctypes.byref(sch), P(self.off, u32p), P(self.srt, u32p),
+                     ctypes.c_int(nthreads), ctypes.byref(c))
+        return c.substeps, c.mech_steps, c.pheno_steps
+
+    def substep(self, nthreads=1):
+        diffusion_substep(self.arr["rho"], self.g, self.s, nthreads)
+        a = self.arr
+        secretion(a["rho"], self.g, self.spec.dt_diff, a["voxel"], a["volume"], a["sec"], a["upt"],
+                  a["sat"], a["net"], a["alive"], grouped=nthreads > 1, nthreads=nthreads)
+
+    def mech_step(self, nthreads=1):
+        lib().or_mech_step(ctypes.byref(self.st), ctypes.byref(self.m),
+                           ctypes.c_double(self.spec.dt_mech), P(self.off, u32p),
+                           P(self.srt, u32p), ctypes.c_int(nthreads))
+
+    def refresh_voxels(self):
+        a = self.arr
+        a["voxel"][:] = voxel_keys(a["x"], a["y"], a["z"], self.g)
+
+
+def ref_secretion(n, ns, nx, ny, nz, dx, dy, dz, dt, voxel, volume_f32, sec, upt, sat, net, density_f32):
+    """Run the reference's own processSecretion (FP32 density and volume)."""
+    r = ref_lib()
+    if r is None:
+        return None
+    r.ref_secretion(ctypes.c_uint32(n), ctypes.c_uint32(ns), ctypes.c_uint32(nx), ctypes.c_uint32(ny),
+                    ctypes.c_uint32(nz), ctypes.c_float(dx), ctypes.c_float(dy), ctypes.c_float(dz),
+                    ctypes.c_double(dt), P(voxel, u32p), P(volume_f32, f32p), P(sec), P(upt),
+                    P(sat), P(net), P(density_f32, f32p))
+    return density_f32

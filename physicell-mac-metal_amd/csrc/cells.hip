@@ -1,0 +1,838 @@
+// cells.hip -- agent-side kernels for gfx950 (FP64): voxel keys, secretion,
+// the spatial hash as a counting sort, repulsion/adhesion forces and AB2.
+//
+// Hash (replaces the atomic 64-slot buckets of shaders/mechanics.metal:27-85):
+//   k_mech_keys      key = clamp(trunc((p - min)/30)) per agent, written to
+//                    field 27; the histogram is built with wave-aggregated
+//                    atomics (lanes holding the same key are found with
+//                    ballots; one atomic per distinct key per wave) and each
+//                    lane gets its slot inside the bin.
+//   scan             exclusive prefix sum of the bin counts (wavefront scans).
+//   k_scatter        provisional placement: list[offset[key] + slot] = agent.
+//   k_stable_gather  rank = number of agents in the same bin with a smaller
+//                    index; writes the ascending-index list and a 64-byte
+//                    record (position, radius, repulsion, adhesion,
+//                    relative adhesion distance, index, key) per agent in
+//                    hash-sorted order, so the force kernel reads neighbours
+//                    from contiguous memory.
+// Forces (shaders/mechanics.metal:91-216) run one agent per lane in
+// hash-sorted order and visit the 27 (9 in 2-D) bins in the reference's
+// (dk, dj, di) order, agents of a bin in ascending index.
+#include "hc_internal.h"
+
+namespace hc {
+
+struct __attribute__((aligned(16))) CellRec {
+    double x, y, z, r, rep, adh, rel;
+    uint32_t idx, key;
+};
+static_assert(sizeof(CellRec) == 64, "record must be 64 bytes");
+
+struct CellPtrs {
+    double* f[HC_NUM_F64_FIELDS];
+    uint32_t* u[HC_NUM_U32_FIELDS];
+};
+
+static CellPtrs cell_ptrs(void* base, uint32_t max_cells) {
+    CellPtrs p;
+    for (int f = 0; f < HC_NUM_F64_FIELDS; f++) p.f[f] = (double*)base + (size_t)f * max_cells;
+    uint32_t* ub = (uint32_t*)((double*)base + (size_t)HC_NUM_F64_FIELDS * max_cells);
+    for (int f = 0; f < HC_NUM_U32_FIELDS; f++) p.u[f] = ub + (size_t)f * max_cells;
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// wavefront primitives
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Ballot binning: returns the slot of this lane's key inside its bin.
+// One atomicAdd per distinct key in the wave.
+__device__ inline uint32_t ballot_bin(bool valid, uint32_t key, uint32_t* counts) {
+    const int lane = threadIdx.x & 63;
+    uint64_t pending = __ballot(valid);
+    uint32_t slot = 0;
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lk = __shfl(key, leader, 64);
+        const uint64_t same = __ballot(valid && key == lk) & pending;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&counts[lk], (uint32_t)__popcll(same));
+        base = __shfl(base, leader, 64);
+        if ((same >> lane) & 1ull) slot = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        pending &= ~same;
+    }
+    return slot;
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan (3 kernels): 1024 values per block of 256 threads
+// ---------------------------------------------------------------------------
+static constexpr int SCAN_BS = 256, SCAN_PER = 4, SCAN_TILE = SCAN_BS * SCAN_PER;
+
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
+    __shared__ uint32_t wsum[SCAN_BS / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t inc = wave_inclusive_scan(v);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+    for (int k = 0; k < SCAN_BS / 64; k++) {
+        if (k < w) off += wsum[k];
+        tot += wsum[k];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - v;
+}
+
+__global__ __launch_bounds__(SCAN_BS) void k_scan_local(const uint32_t* __restrict__ in,
+                                                        uint32_t* __restrict__ out, uint64_t n,
+                                                        uint32_t* __restrict__ partial) {
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_PER;
+    uint32_t v[SCAN_PER];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; k++) {
+        v[k] = (base + k < n) ? in[base + k] : 0u;
+        sum += v[k];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(sum, &total);
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(SCAN_BS) void k_scan_partials(uint32_t* __restrict__ partial, uint32_t nb) {
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < nb; c0 += SCAN_BS) {
+        const uint32_t i = c0 + threadIdx.x;
+        const uint32_t v = i < nb ? partial[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan(v, &total);
+        if (i < nb) partial[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) partial[nb] = carry;
+}
+
+__global__ __launch_bounds__(SCAN_BS) void k_scan_add(uint32_t* __restrict__ out, uint64_t n,
+                                                      const uint32_t* __restrict__ partial, uint32_t nb) {
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_PER;
+    const uint32_t add = partial[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; k++)
+        if (base + k < n) out[base + k] += add;
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = partial[nb];
+}
+
+size_t scan_partial_bytes(uint64_t n) {
+    const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    return (size_t)(nb + 1) * sizeof(uint32_t);
+}
+
+int scan_exclusive(hc_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* partial) {
+    const uint32_t nb = (uint32_t)((n + SCAN_TILE - 1) / SCAN_TILE);
+    if (nb == 0) {
+        hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(SCAN_BS), 0, ctx->stream, partial, 0u);
+        hipLaunchKernelGGL(k_scan_add, dim3(1), dim3(SCAN_BS), 0, ctx->stream, out, (uint64_t)0, partial, 0u);
+        return check_launch(ctx, "scan(empty)");
+    }
+    hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(SCAN_BS), 0, ctx->stream, in, out, n, partial);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(SCAN_BS), 0, ctx->stream, partial, nb);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(SCAN_BS), 0, ctx->stream, out, n, partial, nb);
+    return check_launch(ctx, "scan");
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+static int fill_u32(hc_ctx* ctx, uint32_t* p, uint64_t n, uint32_t v) {
+    if (!n) return 0;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, p, n, v);
+    return check_launch(ctx, "fill");
+}
+
+// ---------------------------------------------------------------------------
+// cell -> diffusion voxel (src/microenvironment.mm:176-184)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_voxel_keys(const double* __restrict__ x, const double* __restrict__ y,
+                                                    const double* __restrict__ z, uint32_t* __restrict__ out,
+                                                    uint32_t n, hc_grid g) {
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    int i = (int)((x[c] - g.x_min) / g.dx);
+    int j = (int)((y[c] - g.y_min) / g.dy);
+    int k = (int)((z[c] - g.z_min) / g.dz);
+    i = (i < 0) ? 0 : (i >= (int)g.nx ? (int)g.nx - 1 : i);
+    j = (j < 0) ? 0 : (j >= (int)g.ny ? (int)g.ny - 1 : j);
+    k = (k < 0) ? 0 : (k >= (int)g.nz ? (int)g.nz - 1 : k);
+    out[c] = (uint32_t)k * g.nx * g.ny + (uint32_t)j * g.nx + (uint32_t)i;
+}
+
+// ---------------------------------------------------------------------------
+// mechanics hash
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mech_keys(const double* __restrict__ x, const double* __restrict__ y,
+                                                   const double* __restrict__ z, uint32_t* __restrict__ key_out,
+                                                   uint32_t* __restrict__ counts, uint32_t* __restrict__ slot,
+                                                   uint32_t n, double x_min, double y_min, double z_min,
+                                                   double voxel, uint32_t mnx, uint32_t mny, uint32_t mnz) {
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = c < n;
+    uint32_t key = 0;
+    if (valid) {
+        // shaders/mechanics.metal:61-72
+        int vi = (int)((x[c] - x_min) / voxel);
+        int vj = (int)((y[c] - y_min) / voxel);
+        int vk = (int)((z[c] - z_min) / voxel);
+        vi = vi < 0 ? 0 : (vi > (int)mnx - 1 ? (int)mnx - 1 : vi);
+        vj = vj < 0 ? 0 : (vj > (int)mny - 1 ? (int)mny - 1 : vj);
+        vk = vk < 0 ? 0 : (vk > (int)mnz - 1 ? (int)mnz - 1 : vk);
+        key = (uint32_t)vk * mny * mnx + (uint32_t)vj * mnx + (uint32_t)vi;
+        key_out[c] = key;
+    }
+    const uint32_t s = ballot_bin(valid, key, counts);
+    if (valid) slot[c] = s;
+}
+
+__global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ key, const uint32_t* __restrict__ slot,
+                                                 const uint32_t* __restrict__ offsets, uint32_t* __restrict__ tmp,
+                                                 uint32_t n) {
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    tmp[offsets[key[c]] + slot[c]] = c;
+}
+
+// rank inside the bin = number of bin members with a smaller index
+__device__ inline uint32_t stable_rank(const uint32_t* __restrict__ tmp, uint32_t beg, uint32_t end, uint32_t c) {
+    uint32_t r = 0;
+    for (uint32_t t = beg; t < end; t++) r += tmp[t] < c ? 1u : 0u;
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_stable_gather(CellPtrs cp, const uint32_t* __restrict__ key,
+                                                       const uint32_t* __restrict__ offsets,
+                                                       const uint32_t* __restrict__ tmp,
+                                                       uint32_t* __restrict__ sorted, CellRec* __restrict__ rec,
+                                                       uint32_t n) {
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t b = key[c];
+    const uint32_t beg = offsets[b], end = offsets[b + 1];
+    const uint32_t pos = beg + stable_rank(tmp, beg, end, c);
+    sorted[pos] = c;
+    CellRec r;
+    r.x = cp.f[HC_F_POS_X][c];
+    r.y = cp.f[HC_F_POS_Y][c];
+    r.z = cp.f[HC_F_POS_Z][c];
+    r.r = cp.f[HC_F_RADIUS][c];
+    r.rep = cp.f[HC_F_REPULSION][c];
+    r.adh = cp.f[HC_F_ADHESION][c];
+    r.rel = cp.f[HC_F_REL_MAX_ADH][c];
+    r.idx = c;
+    r.key = b;
+    rec[pos] = r;
+}
+
+// ---------------------------------------------------------------------------
+// forces (shaders/mechanics.metal:91-216)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_forces(const CellRec* __restrict__ rec, const uint32_t* __restrict__ offsets,
+                                                double* __restrict__ vx, double* __restrict__ vy,
+                                                double* __restrict__ vz, double* __restrict__ pressure,
+                                                uint32_t n, uint32_t mnx, uint32_t mny, uint32_t mnz,
+                                                double max_dist) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    const CellRec me = rec[t];
+    const double px = me.x, py = me.y, pz = me.z;
+    const double radius_i = me.r, rep_i = me.rep, adh_i = me.adh, rel_max_adh = me.rel;
+    const uint32_t my_voxel = me.key;
+    double fx = 0.0, fy = 0.0, fz = 0.0, local_pressure = 0.0;
+    const uint32_t vi = my_voxel % mnx;
+    const uint32_t vj = (my_voxel / mnx) % mny;
+    const uint32_t vk = my_voxel / (mnx * mny);
+    const int dk_min = (mnz > 1) ? -1 : 0;
+    const int dk_max = (mnz > 1) ? 1 : 0;
+    const double cut2 = max_dist * max_dist;
+    for (int dk = dk_min; dk <= dk_max; dk++) {
+        const int nk = (int)vk + dk;
+        if (nk < 0 || nk >= (int)mnz) continue;
+        for (int dj = -1; dj <= 1; dj++) {
+            const int nj = (int)vj + dj;
+            if (nj < 0 || nj >= (int)mny) continue;
+            for (int di = -1; di <= 1; di++) {
+                const int ni = (int)vi + di;
+                if (ni < 0 || ni >= (int)mnx) continue;
+                const uint32_t nb = (uint32_t)nk * mny * mnx + (uint32_t)nj * mnx + (uint32_t)ni;
+                const uint32_t beg = offsets[nb], end = offsets[nb + 1];
+                for (uint32_t u = beg; u < end; u++) {
+                    if (u == t) continue;
+                    const double ddx = px - rec[u].x;
+                    const double ddy = py - rec[u].y;
+                    const double ddz = pz - rec[u].z;
+                    const double dist_sq = ddx * ddx + ddy * ddy + ddz * ddz;
+                    if (dist_sq >= cut2) continue;
+                    if (dist_sq < 1e-12) continue;
+                    const double dist = sqrt(dist_sq);
+                    const double inv_dist = 1.0 / dist;
+                    const double nxv = ddx * inv_dist;
+                    const double nyv = ddy * inv_dist;
+                    const double nzv = ddz * inv_dist;
+                    const double R_sum = radius_i + rec[u].r;
+                    double force_mag = 0.0;
+                    if (dist < R_sum) {
+                        const double overlap = 1.0 - dist / R_sum;
+                        force_mag += overlap * overlap * sqrt(rep_i * rec[u].rep);
+                        local_pressure += overlap * overlap / 0.027288820670331;
+                    }
+                    const double adhesion_distance = rel_max_adh * R_sum;
+                    if (dist < adhesion_distance) {
+                        const double t2 = 1.0 - dist / adhesion_distance;
+                        force_mag -= t2 * t2 * sqrt(adh_i * rec[u].adh);
+                    }
+                    fx += force_mag * nxv;
+                    fy += force_mag * nyv;
+                    fz += force_mag * nzv;
+                }
+            }
+        }
+    }
+    const uint32_t c = me.idx;
+    vx[c] = fx;
+    vy[c] = fy;
+    vz[c] = fz;
+    pressure[c] = local_pressure;
+}
+
+__global__ __launch_bounds__(256) void k_neighbors(const CellRec* __restrict__ rec, const uint32_t* __restrict__ offsets,
+                                                   uint32_t* __restrict__ counts, uint32_t* __restrict__ idx,
+                                                   uint32_t n, uint32_t mnx, uint32_t mny, uint32_t mnz,
+                                                   double max_dist, uint32_t cap) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    const CellRec me = rec[t];
+    const uint32_t vi = me.key % mnx, vj = (me.key / mnx) % mny, vk = me.key / (mnx * mny);
+    const int dk_min = (mnz > 1) ? -1 : 0, dk_max = (mnz > 1) ? 1 : 0;
+    const double cut2 = max_dist * max_dist;
+    uint32_t cnt = 0;
+    for (int dk = dk_min; dk <= dk_max; dk++) {
+        const int nk = (int)vk + dk;
+        if (nk < 0 || nk >= (int)mnz) continue;
+        for (int dj = -1; dj <= 1; dj++) {
+            const int nj = (int)vj + dj;
+            if (nj < 0 || nj >= (int)mny) continue;
+            for (int di = -1; di <= 1; di++) {
+                const int ni = (int)vi + di;
+                if (ni < 0 || ni >= (int)mnx) continue;
+                const uint32_t nb = (uint32_t)nk * mny * mnx + (uint32_t)nj * mnx + (uint32_t)ni;
+                for (uint32_t u = offsets[nb]; u < offsets[nb + 1]; u++) {
+                    if (u == t) continue;
+                    const double ddx = me.x - rec[u].x, ddy = me.y - rec[u].y, ddz = me.z - rec[u].z;
+                    const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                    if (d2 >= cut2) continue;
+                    if (d2 < 1e-12) continue;
+                    if (cnt < cap) idx[(uint64_t)me.idx * cap + cnt] = rec[u].idx;
+                    cnt++;
+                }
+            }
+        }
+    }
+    counts[me.idx] = cnt;
+}
+
+__global__ __launch_bounds__(256) void k_max_bin(const uint32_t* __restrict__ offsets, uint32_t nb,
+                                                 uint32_t* __restrict__ out) {
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    uint32_t v = b < nb ? offsets[b + 1] - offsets[b] : 0u;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t o = __shfl_xor(v, d, 64);
+        v = o > v ? o : v;
+    }
+    if ((threadIdx.x & 63) == 0 && v) atomicMax(out, v);
+}
+
+// ---------------------------------------------------------------------------
+// AB2 (shaders/integrate.metal:56-79)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_integrate(CellPtrs cp, uint32_t n, double dt, double x_min,
+                                                   double y_min, double z_min, double x_max,
+                                                   double y_max, double z_max) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double px = cp.f[HC_F_POS_X][i], py = cp.f[HC_F_POS_Y][i], pz = cp.f[HC_F_POS_Z][i];
+    const double vx = cp.f[HC_F_VEL_X][i], vy = cp.f[HC_F_VEL_Y][i], vz = cp.f[HC_F_VEL_Z][i];
+    px += dt * (1.5 * vx - 0.5 * cp.f[HC_F_PREV_VX][i]);
+    py += dt * (1.5 * vy - 0.5 * cp.f[HC_F_PREV_VY][i]);
+    pz += dt * (1.5 * vz - 0.5 * cp.f[HC_F_PREV_VZ][i]);
+    px = px < x_min ? x_min : (px > x_max ? x_max : px);
+    py = py < y_min ? y_min : (py > y_max ? y_max : py);
+    pz = pz < z_min ? z_min : (pz > z_max ? z_max : pz);
+    cp.f[HC_F_POS_X][i] = px;
+    cp.f[HC_F_POS_Y][i] = py;
+    cp.f[HC_F_POS_Z][i] = pz;
+    cp.f[HC_F_PREV_VX][i] = vx;
+    cp.f[HC_F_PREV_VY][i] = vy;
+    cp.f[HC_F_PREV_VZ][i] = vz;
+}
+
+// ---------------------------------------------------------------------------
+// secretion (src/cell_phenotype.cpp:526-564), grouped by voxel
+// ---------------------------------------------------------------------------
+struct PlanHeader {
+    uint32_t n_elig, ngroups, has_net, n;
+};
+
+__global__ __launch_bounds__(256) void k_sec_keys(const uint32_t* __restrict__ voxel, const uint32_t* __restrict__ alive,
+                                                  uint32_t* __restrict__ key_out, uint32_t* __restrict__ counts,
+                                                  uint32_t* __restrict__ slot, uint32_t n, uint32_t V) {
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = c < n;
+    uint32_t key = V;                                   // V = "not secreting" bin
+    if (valid) {
+        const uint32_t v = voxel[c];
+        if (alive[c] && v < V) key = v;                // src/main.mm:360, src/cell_phenotype.cpp:534
+        key_out[c] = key;
+    }
+    const uint32_t s = ballot_bin(valid, key, counts);
+    if (valid) slot[c] = s;
+}
+
+// group-start flags over hash-sorted positions
+__global__ __launch_bounds__(256) void k_sec_flags(const uint32_t* __restrict__ key, const uint32_t* __restrict__ sorted,
+                                                   const uint32_t* __restrict__ offsets, uint32_t* __restrict__ flag,
+                                                   uint32_t n, uint32_t V) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t n_elig = offsets[V];
+    uint32_t f = 0;
+    if (t < n_elig) f = (t == 0 || key[sorted[t]] != key[sorted[t - 1]]) ? 1u : 0u;
+    flag[t] = f;
+}
+
+__global__ __launch_bounds__(256) void k_sec_groups(CellPtrs cp, const double* __restrict__ rates,
+                                                    const uint32_t* __restrict__ key, const uint32_t* __restrict__ sorted,
+                                                    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ gid,
+                                                    uint32_t* __restrict__ gvox, uint32_t* __restrict__ gbegin,
+                                                    double* __restrict__ coef, PlanHeader* __restrict__ hdr,
+                                                    uint32_t n, uint32_t max_cells, uint32_t ns, uint32_t V,
+                                                    double dt, double V_voxel) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t n_elig = offsets[V];
+    if (t == 0) {
+        hdr->n_elig = n_elig;
+        hdr->ngroups = gid[n];
+        hdr->n = n;
+        gbegin[gid[n]] = n_elig;
+    }
+    if (t >= n_elig) return;
+    const uint32_t c = sorted[t];
+    const uint32_t v = key[c];
+    if (t == 0 || key[sorted[t - 1]] != v) {
+        const uint32_t g = gid[t];
+        gvox[g] = v;
+        gbegin[g] = t;
+    }
+    const double* sec = rates;
+    const double* upt = rates + (size_t)max_cells * ns;
+    const double* sat = rates + (size_t)2 * max_cells * ns;
+    const double* net = rates + (size_t)3 * max_cells * ns;
+    const double V_cell = cp.f[HC_F_TOTAL_VOL][c];
+    const double internal_constant = dt * V_cell / V_voxel;
+    bool any_net = false;
+    for (uint32_t s = 0; s < ns; s++) {
+        const size_t off = (size_t)c * ns + s;
+        const double a = dt * net[off] / V_voxel;
+        const double temp1 = internal_constant * sec[off] * sat[off];
+        const double temp2 = 1.0 + internal_constant * (sec[off] + upt[off]);
+        coef[((size_t)s * 3 + 0) * max_cells + t] = a;
+        coef[((size_t)s * 3 + 1) * max_cells + t] = temp1;
+        coef[((size_t)s * 3 + 2) * max_cells + t] = temp2;
+        any_net |= (a != 0.0);
+    }
+    if (any_net) atomicOr(&hdr->has_net, 1u);
+}
+
+__global__ __launch_bounds__(256) void k_sec_apply(double* __restrict__ rho, const uint32_t* __restrict__ gvox,
+                                                   const uint32_t* __restrict__ gbegin,
+                                                   const double* __restrict__ coef,
+                                                   const PlanHeader* __restrict__ hdr, uint32_t max_cells,
+                                                   uint32_t ns, uint64_t V) {
+    const uint32_t g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= hdr->ngroups) return;
+    const bool has_net = hdr->has_net != 0;
+    const uint32_t v = gvox[g];
+    const uint32_t b = gbegin[g], e = gbegin[g + 1];
+    for (uint32_t s = 0; s < ns; s++) {
+        const double* ca = coef + ((size_t)s * 3 + 0) * max_cells;
+        const double* c1 = coef + ((size_t)s * 3 + 1) * max_cells;
+        const double* c2 = coef + ((size_t)s * 3 + 2) * max_cells;
+        double r = rho[(uint64_t)s * V + v];
+        for (uint32_t t = b; t < e; t++) {
+            if (has_net) r += ca[t];
+            r = (r + c1[t]) / c2[t];
+            if (r < 0.0) r = 0.0;
+        }
+        rho[(uint64_t)s * V + v] = r;
+    }
+}
+
+int probe_kernels_cells(hc_ctx* ctx) {
+    hipFuncAttributes attr;
+    const void* fns[] = {(const void*)k_scan_local, (const void*)k_scan_partials, (const void*)k_scan_add,
+                         (const void*)k_fill_u32, (const void*)k_voxel_keys, (const void*)k_mech_keys,
+                         (const void*)k_scatter, (const void*)k_stable_gather, (const void*)k_forces,
+                         (const void*)k_neighbors, (const void*)k_max_bin, (const void*)k_integrate,
+                         (const void*)k_sec_keys, (const void*)k_sec_flags, (const void*)k_sec_groups,
+                         (const void*)k_sec_apply};
+    for (const void* f : fns) {
+        hipError_t e = hipFuncGetAttributes(&attr, f);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: cell kernel");
+    }
+    return 0;
+}
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+HashLayout hash_layout(uint32_t n_mvox, uint32_t max_cells) {
+    HashLayout L;
+    size_t o = 0;
+    L.counts = o;  o = align256(o + ((size_t)n_mvox + 1) * 4);
+    L.offsets = o; o = align256(o + ((size_t)n_mvox + 2) * 4);
+    L.slot = o;    o = align256(o + (size_t)max_cells * 4);
+    L.tmp = o;     o = align256(o + (size_t)max_cells * 4);
+    L.sorted = o;  o = align256(o + (size_t)max_cells * 4);
+    L.rec = o;     o = align256(o + (size_t)max_cells * sizeof(CellRec));
+    L.partial = o; o = align256(o + scan_partial_bytes((uint64_t)n_mvox + 1));
+    L.total = o;
+    return L;
+}
+
+PlanLayout plan_layout(const hc_grid* g, uint32_t max_cells) {
+    PlanLayout L;
+    const uint64_t V = (uint64_t)g->nx * g->ny * g->nz;
+    size_t o = 0;
+    L.header = o;  o = align256(o + sizeof(PlanHeader));
+    L.counts = o;  o = align256(o + (V + 1) * 4);
+    L.offsets = o; o = align256(o + (V + 2) * 4);
+    L.slot = o;    o = align256(o + (size_t)max_cells * 4);
+    L.tmp = o;     o = align256(o + (size_t)max_cells * 4);
+    L.sorted = o;  o = align256(o + (size_t)max_cells * 4);
+    L.gflag = o;   o = align256(o + ((size_t)max_cells + 1) * 4);   // flags, then group ids (in place)
+    L.gvox = o;    o = align256(o + ((size_t)max_cells + 1) * 4);
+    L.gbegin = o;  o = align256(o + ((size_t)max_cells + 2) * 4);
+    L.coef = o;    o = align256(o + (size_t)3 * g->n_substrates * max_cells * sizeof(double));
+    size_t pb = scan_partial_bytes(V + 1);
+    const size_t pb2 = scan_partial_bytes(max_cells);
+    if (pb2 > pb) pb = pb2;
+    L.partial = o; o = align256(o + pb);
+    L.total = o;
+    return L;
+}
+
+}  // namespace hc
+
+using namespace hc;
+
+static int check_cells(hc_ctx* ctx, hc_buf* cells, uint32_t n, uint32_t max_cells, const char* who) {
+    if (!ctx || !cells) return fail(ctx, HC_ERR_ARG, std::string(who) + ": null argument");
+    if (n > max_cells) return fail(ctx, HC_ERR_ARG, std::string(who) + ": n > max_cells");
+    if (hc_buf_size(cells) < hc_cells_bytes(max_cells))
+        return fail(ctx, HC_ERR_ARG, std::string(who) + ": cell buffer smaller than hc_cells_bytes(max_cells)");
+    return 0;
+}
+
+extern "C" {
+
+int hc_voxel_keys(hc_ctx* ctx, hc_buf* cells, const hc_grid* g, uint32_t n, uint32_t max_cells) {
+    int r = check_cells(ctx, cells, n, max_cells, "hc_voxel_keys");
+    if (r) return r;
+    if (!g || !g->nx || !g->ny || !g->nz) return fail(ctx, HC_ERR_ARG, "hc_voxel_keys: bad grid");
+    if (!n) return 0;
+    CellPtrs cp = cell_ptrs(cells->dptr, max_cells);
+    KernelTimer tm(ctx, HC_K_VOXEL_KEYS);
+    hipLaunchKernelGGL(k_voxel_keys, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream,
+                       cp.f[HC_F_POS_X], cp.f[HC_F_POS_Y], cp.f[HC_F_POS_Z],
+                       cp.u[HC_F_VOXEL_IDX - HC_NUM_F64_FIELDS], n, *g);
+    return check_launch(ctx, "k_voxel_keys");
+}
+
+size_t hc_secretion_plan_bytes(const hc_grid* g, uint32_t max_cells) {
+    return g ? plan_layout(g, max_cells).total : 0;
+}
+
+
+}  // extern "C"
+
+namespace hc {
+
+__global__ __launch_bounds__(256) void k_stable_sort(const uint32_t* __restrict__ key,
+                                                     const uint32_t* __restrict__ offsets,
+                                                     const uint32_t* __restrict__ tmp,
+                                                     uint32_t* __restrict__ sorted, uint32_t n) {
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t b = key[c];
+    const uint32_t beg = offsets[b], end = offsets[b + 1];
+    sorted[beg + stable_rank(tmp, beg, end, c)] = c;
+}
+
+int probe_kernels_cells2(hc_ctx* ctx);
+int probe_kernels_cells2(hc_ctx* ctx) {
+    hipFuncAttributes attr;
+    hipError_t e = hipFuncGetAttributes(&attr, (const void*)k_stable_sort);
+    return e == hipSuccess ? 0 : hip_fail(ctx, e, "hc_load_kernels: k_stable_sort");
+}
+
+struct HashPtrs {
+    uint32_t *counts, *offsets, *slot, *tmp, *sorted, *partial;
+    CellRec* rec;
+};
+
+static HashPtrs hash_ptrs(hc_buf* hash, uint32_t n_mvox, uint32_t max_cells) {
+    const HashLayout L = hash_layout(n_mvox, max_cells);
+    uint8_t* P = (uint8_t*)hash->dptr;
+    HashPtrs h;
+    h.counts = (uint32_t*)(P + L.counts);
+    h.offsets = (uint32_t*)(P + L.offsets);
+    h.slot = (uint32_t*)(P + L.slot);
+    h.tmp = (uint32_t*)(P + L.tmp);
+    h.sorted = (uint32_t*)(P + L.sorted);
+    h.partial = (uint32_t*)(P + L.partial);
+    h.rec = (CellRec*)(P + L.rec);
+    return h;
+}
+
+}  // namespace hc
+
+extern "C" {
+
+int hc_secretion_plan(hc_ctx* ctx, hc_buf* cells, hc_buf* rates, const hc_grid* g, double dt,
+                      uint32_t n, uint32_t max_cells, hc_buf* plan) {
+    int r = check_cells(ctx, cells, n, max_cells, "hc_secretion_plan");
+    if (r) return r;
+    if (!g || !rates || !plan) return fail(ctx, HC_ERR_ARG, "hc_secretion_plan: null argument");
+    const uint64_t V = (uint64_t)g->nx * g->ny * g->nz;
+    if (V >= 0xFFFFFFFFull) return fail(ctx, HC_ERR_CAPACITY, "hc_secretion_plan: grid too large for 32-bit voxel keys");
+    const PlanLayout L = plan_layout(g, max_cells);
+    if (hc_buf_size(plan) < L.total) return fail(ctx, HC_ERR_ARG, "hc_secretion_plan: plan buffer too small");
+    if (hc_buf_size(rates) < hc_rates_bytes(max_cells, g->n_substrates))
+        return fail(ctx, HC_ERR_ARG, "hc_secretion_plan: rates buffer too small");
+    uint8_t* P = (uint8_t*)plan->dptr;
+    uint32_t* counts = (uint32_t*)(P + L.counts);
+    uint32_t* offsets = (uint32_t*)(P + L.offsets);
+    uint32_t* slot = (uint32_t*)(P + L.slot);
+    uint32_t* tmp = (uint32_t*)(P + L.tmp);
+    uint32_t* sorted = (uint32_t*)(P + L.sorted);
+    uint32_t* gflag = (uint32_t*)(P + L.gflag);
+    uint32_t* gvox = (uint32_t*)(P + L.gvox);
+    uint32_t* gbegin = (uint32_t*)(P + L.gbegin);
+    uint32_t* partial = (uint32_t*)(P + L.partial);
+    double* coef = (double*)(P + L.coef);
+    PlanHeader* hdr = (PlanHeader*)(P + L.header);
+    // per-cell sort keys live in gbegin until the groups are written
+    uint32_t* key = gbegin;
+    CellPtrs cp = cell_ptrs(cells->dptr, max_cells);
+    plan->aux = max_cells;
+    KernelTimer tm(ctx, HC_K_SEC_PLAN);
+    if ((r = fill_u32(ctx, counts, V + 1, 0))) return r;
+    if ((r = fill_u32(ctx, (uint32_t*)hdr, sizeof(PlanHeader) / 4, 0))) return r;
+    const unsigned nb = blocks_for(n ? n : 1, 256);
+    if (n) {
+        hipLaunchKernelGGL(k_sec_keys, dim3(nb), dim3(256), 0, ctx->stream,
+                           cp.u[HC_F_VOXEL_IDX - HC_NUM_F64_FIELDS], cp.u[HC_F_IS_ALIVE - HC_NUM_F64_FIELDS],
+                           key, counts, slot, n, (uint32_t)V);
+        if ((r = check_launch(ctx, "k_sec_keys"))) return r;
+    }
+    if ((r = scan_exclusive(ctx, counts, offsets, V + 1, partial))) return r;
+    if (n) {
+        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, ctx->stream, key, slot, offsets, tmp, n);
+        hipLaunchKernelGGL(k_stable_sort, dim3(nb), dim3(256), 0, ctx->stream, key, offsets, tmp, sorted, n);
+        hipLaunchKernelGGL(k_sec_flags, dim3(nb), dim3(256), 0, ctx->stream, key, sorted, offsets, gflag, n,
+                           (uint32_t)V);
+        if ((r = check_launch(ctx, "secretion plan sort"))) return r;
+    }
+    if ((r = scan_exclusive(ctx, gflag, gflag, n, partial))) return r;
+    // keys move out of gbegin before the group table is written: copy them to tmp
+    // (tmp is free after the stable sort)
+    if (n) {
+        hipError_t e = hipMemcpyAsync(tmp, key, (size_t)n * 4, hipMemcpyDeviceToDevice, ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hc_secretion_plan: key copy");
+    }
+    hipLaunchKernelGGL(k_sec_groups, dim3(nb), dim3(256), 0, ctx->stream, cp, (const double*)rates->dptr,
+                       tmp, sorted, offsets, gflag, gvox, gbegin, coef, hdr, n, max_cells,
+                       g->n_substrates, (uint32_t)V, dt, g->dx * g->dy * g->dz);
+    return check_launch(ctx, "k_sec_groups");
+}
+
+int hc_secretion_apply(hc_ctx* ctx, hc_buf* density, hc_buf* plan, const hc_grid* g) {
+    if (!ctx || !density || !plan || !g) return fail(ctx, HC_ERR_ARG, "hc_secretion_apply: null argument");
+    const uint64_t V = (uint64_t)g->nx * g->ny * g->nz;
+    if (hc_buf_size(density) < V * g->n_substrates * sizeof(double))
+        return fail(ctx, HC_ERR_ARG, "hc_secretion_apply: density buffer too small");
+    const uint8_t* P = (const uint8_t*)plan->dptr;
+    const uint32_t max_cells = (uint32_t)plan->aux;
+    if (!max_cells) return fail(ctx, HC_ERR_STATE, "hc_secretion_apply: plan was never built (call hc_secretion_plan)");
+    if (plan_layout(g, max_cells).total > hc_buf_size(plan))
+        return fail(ctx, HC_ERR_ARG, "hc_secretion_apply: grid does not match the plan");
+    const PlanLayout L = plan_layout(g, max_cells);
+    if (!g->n_substrates || !max_cells) return 0;
+    KernelTimer tm(ctx, HC_K_SEC_APPLY);
+    hipLaunchKernelGGL(k_sec_apply, dim3(blocks_for(max_cells, 256)), dim3(256), 0, ctx->stream,
+                       (double*)density->dptr, (const uint32_t*)(P + L.gvox), (const uint32_t*)(P + L.gbegin),
+                       (const double*)(P + L.coef), (const PlanHeader*)(P + L.header), max_cells,
+                       g->n_substrates, V);
+    return check_launch(ctx, "k_sec_apply");
+}
+
+int hc_secretion(hc_ctx* ctx, hc_buf* density, hc_buf* cells, hc_buf* rates, const hc_grid* g,
+                 double dt, uint32_t n, uint32_t max_cells, hc_buf* plan) {
+    int r = hc_secretion_plan(ctx, cells, rates, g, dt, n, max_cells, plan);
+    if (r) return r;
+    return hc_secretion_apply(ctx, density, plan, g);
+}
+
+size_t hc_hash_bytes(uint32_t n_mvox, uint32_t max_cells) { return hash_layout(n_mvox, max_cells).total; }
+
+static int check_hash(hc_ctx* ctx, hc_buf* hash, uint32_t n_mvox, uint32_t max_cells, const char* who) {
+    if (!hash || n_mvox == 0) return fail(ctx, HC_ERR_ARG, std::string(who) + ": bad hash argument");
+    if (hc_buf_size(hash) < hc_hash_bytes(n_mvox, max_cells))
+        return fail(ctx, HC_ERR_ARG, std::string(who) + ": hash buffer smaller than hc_hash_bytes");
+    return 0;
+}
+
+int hc_build_hash(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech, const hc_grid* mgrid,
+                  uint32_t n, uint32_t max_cells, uint32_t n_mvox) {
+    int r = check_cells(ctx, cells, n, max_cells, "hc_build_hash");
+    if (r || (r = check_hash(ctx, hash, n_mvox, max_cells, "hc_build_hash"))) return r;
+    if (!mech || !mgrid) return fail(ctx, HC_ERR_ARG, "hc_build_hash: null parameters");
+    if ((uint64_t)mech->mech_grid_nx * mech->mech_grid_ny * mech->mech_grid_nz != n_mvox)
+        return fail(ctx, HC_ERR_ARG, "hc_build_hash: n_mvox != mech grid size");
+    HashPtrs h = hash_ptrs(hash, n_mvox, max_cells);
+    CellPtrs cp = cell_ptrs(cells->dptr, max_cells);
+    KernelTimer tm(ctx, HC_K_HASH);
+    if ((r = fill_u32(ctx, h.counts, n_mvox, 0))) return r;
+    const unsigned nb = blocks_for(n ? n : 1, 256);
+    if (n) {
+        hipLaunchKernelGGL(k_mech_keys, dim3(nb), dim3(256), 0, ctx->stream, cp.f[HC_F_POS_X], cp.f[HC_F_POS_Y],
+                           cp.f[HC_F_POS_Z], cp.u[HC_F_MECH_VOXEL - HC_NUM_F64_FIELDS], h.counts, h.slot, n,
+                           mgrid->x_min, mgrid->y_min, mgrid->z_min, mech->mechanics_voxel_size,
+                           mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz);
+        if ((r = check_launch(ctx, "k_mech_keys"))) return r;
+    }
+    if ((r = scan_exclusive(ctx, h.counts, h.offsets, n_mvox, h.partial))) return r;
+    if (n) {
+        const uint32_t* key = cp.u[HC_F_MECH_VOXEL - HC_NUM_F64_FIELDS];
+        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, ctx->stream, key, h.slot, h.offsets, h.tmp, n);
+        hipLaunchKernelGGL(k_stable_gather, dim3(nb), dim3(256), 0, ctx->stream, cp, key, h.offsets, h.tmp,
+                           h.sorted, h.rec, n);
+        if ((r = check_launch(ctx, "hash scatter"))) return r;
+    }
+    return 0;
+}
+
+int hc_forces(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech, uint32_t n, uint32_t max_cells,
+              uint32_t n_mvox) {
+    int r = check_cells(ctx, cells, n, max_cells, "hc_forces");
+    if (r || (r = check_hash(ctx, hash, n_mvox, max_cells, "hc_forces"))) return r;
+    if (!mech) return fail(ctx, HC_ERR_ARG, "hc_forces: null parameters");
+    if (!n) return 0;
+    HashPtrs h = hash_ptrs(hash, n_mvox, max_cells);
+    CellPtrs cp = cell_ptrs(cells->dptr, max_cells);
+    KernelTimer tm(ctx, HC_K_FORCES);
+    hipLaunchKernelGGL(k_forces, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, h.rec, h.offsets,
+                       cp.f[HC_F_VEL_X], cp.f[HC_F_VEL_Y], cp.f[HC_F_VEL_Z], cp.f[HC_F_PRESSURE], n,
+                       mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz,
+                       mech->max_interaction_distance);
+    return check_launch(ctx, "k_forces");
+}
+
+int hc_forces_only_pipeline(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech, const hc_grid* mgrid,
+                            uint32_t n, uint32_t max_cells, uint32_t n_mvox) {
+    int r = hc_build_hash(ctx, cells, hash, mech, mgrid, n, max_cells, n_mvox);
+    if (r) return r;
+    return hc_forces(ctx, cells, hash, mech, n, max_cells, n_mvox);
+}
+
+int hc_integrate(hc_ctx* ctx, hc_buf* cells, const hc_grid* mgrid, double dt, uint32_t n, uint32_t max_cells) {
+    int r = check_cells(ctx, cells, n, max_cells, "hc_integrate");
+    if (r) return r;
+    if (!mgrid) return fail(ctx, HC_ERR_ARG, "hc_integrate: null grid");
+    if (!n) return 0;
+    CellPtrs cp = cell_ptrs(cells->dptr, max_cells);
+    // domain bounds [min, min + n*d] of the mech grid (shaders/integrate.metal:63-69)
+    const double x_max = mgrid->x_min + (double)mgrid->nx * mgrid->dx;
+    const double y_max = mgrid->y_min + (double)mgrid->ny * mgrid->dy;
+    const double z_max = mgrid->z_min + (double)mgrid->nz * mgrid->dz;
+    KernelTimer tm(ctx, HC_K_INTEGRATE);
+    hipLaunchKernelGGL(k_integrate, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, cp, n, dt, mgrid->x_min,
+                       mgrid->y_min, mgrid->z_min, x_max, y_max, z_max);
+    return check_launch(ctx, "k_integrate");
+}
+
+int hc_mechanics_pipeline(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech, const hc_grid* mgrid,
+                          double dt, uint32_t n, uint32_t max_cells, uint32_t n_mvox) {
+    int r = hc_forces_only_pipeline(ctx, cells, hash, mech, mgrid, n, max_cells, n_mvox);
+    if (r) return r;
+    return hc_integrate(ctx, cells, mgrid, dt, n, max_cells);
+}
+
+int hc_hash_download(hc_ctx* ctx, hc_buf* hash, uint32_t n_mvox, uint32_t max_cells, uint32_t n,
+                     uint32_t* offsets_out, uint32_t* cells_out) {
+    if (!ctx || !hash || !offsets_out || (!cells_out && n)) return fail(ctx, HC_ERR_ARG, "hc_hash_download: null argument");
+    int r = check_hash(ctx, hash, n_mvox, max_cells, "hc_hash_download");
+    if (r) return r;
+    if (n > max_cells) return fail(ctx, HC_ERR_ARG, "hc_hash_download: n > max_cells");
+    HashPtrs h = hash_ptrs(hash, n_mvox, max_cells);
+    hipError_t e = hipMemcpyAsync(offsets_out, h.offsets, ((size_t)n_mvox + 1) * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && n) e = hipMemcpyAsync(cells_out, h.sorted, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? 0 : hip_fail(ctx, e, "hc_hash_download");
+}
+
+int hc_hash_max_bin(hc_ctx* ctx, hc_buf* hash, uint32_t n_mvox, uint32_t* max_out) {
+    if (!ctx || !hash || !max_out) return fail(ctx, HC_ERR_ARG, "hc_hash_max_bin: null argument");
+    const HashLayout L = hash_layout(n_mvox, 0);
+    const uint32_t* offsets = (const uint32_t*)((uint8_t*)hash->dptr + L.offsets);
+    int r = fill_u32(ctx, ctx->d_scalar, 1, 0);
+    if (r) return r;
+    hipLaunchKernelGGL(k_max_bin, dim3(blocks_for(n_mvox, 256)), dim3(256), 0, ctx->stream, offsets, n_mvox,
+                       ctx->d_scalar);
+    if ((r = check_launch(ctx, "k_max_bin"))) return r;
+    hipError_t e = hipMemcpyAsync(max_out, ctx->d_scalar, 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? 0 : hip_fail(ctx, e, "hc_hash_max_bin");
+}
+
+int hc_neighbor_lists(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech, uint32_t n,
+                      uint32_t max_cells, uint32_t n_mvox, uint32_t cap, hc_buf* counts, hc_buf* idx) {
+    int r = check_cells(ctx, cells, n, max_cells, "hc_neighbor_lists");
+    if (r || (r = check_hash(ctx, hash, n_mvox, max_cells, "hc_neighbor_lists"))) return r;
+    if (!mech || !counts || !idx) return fail(ctx, HC_ERR_ARG, "hc_neighbor_lists: null argument");
+    if (hc_buf_size(counts) < (size_t)n * 4 || hc_buf_size(idx) < (size_t)n * cap * 4)
+        return fail(ctx, HC_ERR_ARG, "hc_neighbor_lists: output buffers too small");
+    if (!n) return 0;
+    HashPtrs h = hash_ptrs(hash, n_mvox, max_cells);
+    hipLaunchKernelGGL(k_neighbors, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, h.rec, h.offsets,
+                       (uint32_t*)counts->dptr, (uint32_t*)idx->dptr, n, mech->mech_grid_nx, mech->mech_grid_ny,
+                       mech->mech_grid_nz, mech->max_interaction_distance, cap);
+    return check_launch(ctx, "k_neighbors");
+}
+
+}  // extern "C"

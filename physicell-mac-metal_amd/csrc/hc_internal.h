@@ -1,0 +1,99 @@
+// hc_internal.h -- private structures shared by the hip_context translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "physicell_hip.h"
+
+struct hc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool kernels_loaded = false;
+    std::string last_error;
+    // per-kernel event timing
+    uint32_t timing_mask = 0;
+    struct Pending { int kernel; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    double total_ms[HC_K_COUNT] = {};
+    int64_t launches[HC_K_COUNT] = {};
+    // small scratch for reductions read back by the host (max bin etc.)
+    uint32_t* d_scalar = nullptr;
+};
+
+struct hc_buf {
+    hc_ctx* ctx = nullptr;
+    void* dptr = nullptr;
+    void* host = nullptr;
+    size_t bytes = 0;
+    // capacity (max_cells) a secretion plan or hash workspace was last built for
+    uint64_t aux = 0;
+};
+
+namespace hc {
+
+int fail(hc_ctx* ctx, int code, const std::string& msg);
+int hip_fail(hc_ctx* ctx, hipError_t e, const char* what);
+
+// Event pair around one launch of `kernel` when its bit is set in timing_mask.
+struct KernelTimer {
+    hc_ctx* ctx;
+    int kernel;
+    hipEvent_t a = nullptr, b = nullptr;
+    KernelTimer(hc_ctx* c, int k);
+    ~KernelTimer();
+};
+
+// After a launch: report a launch error (returns 0 when fine).
+int check_launch(hc_ctx* ctx, const char* name);
+
+inline unsigned blocks_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// ---- device-side parameter blocks (passed by value) ----
+struct SubConst {          // per substrate, per axis coefficient c = D*dt/d^2
+    double c[3];
+    double h;              // decay share per sweep
+    double T;              // Dirichlet value
+    uint32_t dir;          // Dirichlet flag
+    uint32_t _pad;
+};
+
+// Thomas table layout: [(s*3 + axis)*2 + {0: c', 1: denominator}] * nmax + i
+__host__ __device__ inline size_t table_index(uint32_t s, int axis, int which, uint32_t nmax) {
+    return ((size_t)(s * 3 + axis) * 2 + which) * nmax;
+}
+
+// Layout of the table buffer: SubConst[ns] (padded to 256 B) then the tables.
+__host__ __device__ inline size_t subconst_bytes(uint32_t ns) { return ((sizeof(SubConst) * ns + 255) / 256) * 256; }
+inline uint32_t grid_nmax(const hc_grid* g) {
+    uint32_t m = g->nx > g->ny ? g->nx : g->ny;
+    return m > g->nz ? m : g->nz;
+}
+
+// ---- launchers implemented in diffusion.hip / cells.hip ----
+int launch_sweep(hc_ctx* ctx, int axis, double* rho, const hc_grid* g, const uint8_t* tables,
+                 int fuse_dirichlet);
+int launch_dirichlet(hc_ctx* ctx, double* rho, const hc_grid* g, const uint8_t* tables);
+
+// Hash workspace layout (byte offsets inside the hash buffer).
+struct HashLayout {
+    size_t counts, offsets, slot, tmp, sorted, rec, partial, total;
+};
+HashLayout hash_layout(uint32_t n_mvox, uint32_t max_cells);
+
+struct PlanLayout {
+    size_t header, counts, offsets, slot, tmp, sorted, gflag, gvox, gbegin, coef, partial, total;
+};
+PlanLayout plan_layout(const hc_grid* g, uint32_t max_cells);
+
+// exclusive scan of n uint32 values in place into out (out may equal in),
+// out[n] = total; partial is scratch of scan_partial_bytes(n).
+size_t scan_partial_bytes(uint64_t n);
+int scan_exclusive(hc_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* partial);
+
+}  // namespace hc

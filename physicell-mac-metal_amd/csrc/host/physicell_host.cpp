@@ -1,0 +1,589 @@
+// physicell_host.cpp -- host classes (Microenvironment, CellMechanics,
+// CellData, Simulation) over the hc_* C-ABI, and the pc_* C entry points.
+#include "physicell_host.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+// the context's stream (internal export of context.hip; not part of the C-ABI header)
+extern "C" hipStream_t hc_stream_of(hc_ctx* ctx);
+
+namespace physicell {
+
+void check(hc_ctx* ctx, int rc, const char* what) {
+    if (rc != 0) {
+        throw std::runtime_error(std::string(what) + " failed (" + std::to_string(rc) +
+                                 "): " + (ctx ? hc_last_error(ctx) : ""));
+    }
+}
+
+// ---------------------------------------------------------------- DeviceBuffer
+DeviceBuffer::DeviceBuffer(hc_ctx* ctx, size_t bytes) { allocate(ctx, bytes); }
+DeviceBuffer::~DeviceBuffer() { hc_free(buf_); }
+DeviceBuffer::DeviceBuffer(DeviceBuffer&& o) noexcept : ctx_(o.ctx_), buf_(o.buf_) { o.buf_ = nullptr; }
+DeviceBuffer& DeviceBuffer::operator=(DeviceBuffer&& o) noexcept {
+    if (this != &o) {
+        hc_free(buf_);
+        ctx_ = o.ctx_;
+        buf_ = o.buf_;
+        o.buf_ = nullptr;
+    }
+    return *this;
+}
+void DeviceBuffer::allocate(hc_ctx* ctx, size_t bytes) {
+    hc_free(buf_);
+    buf_ = nullptr;
+    ctx_ = ctx;
+    check(ctx, hc_alloc(ctx, bytes, &buf_), "allocateBuffer");
+}
+void* DeviceBuffer::host() {
+    void* h = hc_host_mirror(buf_);
+    if (!h && size()) throw std::runtime_error("host mirror allocation failed");
+    return h;
+}
+void DeviceBuffer::upload(size_t off, size_t n) { check(ctx_, hc_upload(ctx_, buf_, off, n), "upload"); }
+void DeviceBuffer::download(size_t off, size_t n) { check(ctx_, hc_download(ctx_, buf_, off, n), "download"); }
+
+// -------------------------------------------------------------------- CellData
+CellData::CellData(uint32_t max_cells_) : max_cells(max_cells_ ? max_cells_ : 1) {}
+
+void CellData::attach(hc_ctx* ctx, uint32_t ns) {
+    ctx_ = ctx;
+    n_substrates = ns;
+    soa_.allocate(ctx, hc_cells_bytes(max_cells));
+    rates_.allocate(ctx, std::max<size_t>(8, hc_rates_bytes(max_cells, ns)));
+    bind_views();
+}
+
+double* CellData::field(uint32_t f) {
+    return (double*)((char*)soa_.host() + hc_cells_field_offset(f, max_cells));
+}
+uint32_t* CellData::ufield(uint32_t f) {
+    return (uint32_t*)((char*)soa_.host() + hc_cells_field_offset(f, max_cells));
+}
+
+void CellData::bind_views() {
+    position_x = field(HC_F_POS_X); position_y = field(HC_F_POS_Y); position_z = field(HC_F_POS_Z);
+    velocity_x = field(HC_F_VEL_X); velocity_y = field(HC_F_VEL_Y); velocity_z = field(HC_F_VEL_Z);
+    prev_velocity_x = field(HC_F_PREV_VX); prev_velocity_y = field(HC_F_PREV_VY); prev_velocity_z = field(HC_F_PREV_VZ);
+    radius = field(HC_F_RADIUS); nuclear_radius = field(HC_F_NUCLEAR_RAD);
+    total_volume = field(HC_F_TOTAL_VOL); nuclear_volume = field(HC_F_NUCLEAR_VOL);
+    cell_cell_repulsion = field(HC_F_REPULSION); cell_cell_adhesion = field(HC_F_ADHESION);
+    relative_max_adhesion_distance = field(HC_F_REL_MAX_ADH);
+    simple_pressure = field(HC_F_PRESSURE);
+    cell_type = ufield(HC_F_CELL_TYPE); current_phase = ufield(HC_F_CUR_PHASE);
+    is_alive = ufield(HC_F_IS_ALIVE); voxel_index = ufield(HC_F_VOXEL_IDX);
+    mech_voxel_index = ufield(HC_F_MECH_VOXEL);
+    double* r = (double*)rates_.host();
+    const size_t stride = (size_t)max_cells * n_substrates;
+    secretion_rate = r;
+    uptake_rate = r + stride;
+    saturation_density = r + 2 * stride;
+    net_export_rate = r + 3 * stride;
+}
+
+uint32_t CellData::addCell() {
+    if (num_cells >= max_cells) return UINT32_MAX;   // src/cell_data.cpp:398-401
+    const uint32_t i = num_cells++;
+    for (uint32_t f = 0; f < HC_NUM_F64_FIELDS; f++) field(f)[i] = 0.0;
+    for (uint32_t f = HC_NUM_F64_FIELDS; f < HC_NUM_CELL_FIELDS; f++) ufield(f)[i] = 0;
+    is_alive[i] = 1;
+    for (uint32_t s = 0; s < n_substrates; s++) {
+        const size_t o = (size_t)i * n_substrates + s;
+        secretion_rate[o] = uptake_rate[o] = saturation_density[o] = net_export_rate[o] = 0.0;
+    }
+    return i;
+}
+
+void CellData::removeCell(uint32_t index) {
+    if (index >= num_cells) return;
+    const uint32_t last = num_cells - 1;
+    if (index != last) {
+        for (uint32_t f = 0; f < HC_NUM_F64_FIELDS; f++) field(f)[index] = field(f)[last];
+        for (uint32_t f = HC_NUM_F64_FIELDS; f < HC_NUM_CELL_FIELDS; f++) ufield(f)[index] = ufield(f)[last];
+        for (uint32_t s = 0; s < n_substrates; s++) {
+            const size_t oi = (size_t)index * n_substrates + s, ol = (size_t)last * n_substrates + s;
+            secretion_rate[oi] = secretion_rate[ol];
+            uptake_rate[oi] = uptake_rate[ol];
+            saturation_density[oi] = saturation_density[ol];
+            net_export_rate[oi] = net_export_rate[ol];
+        }
+    }
+    num_cells = last;
+}
+
+void CellData::syncToDevice() {
+    soa_.upload_all();
+    rates_.upload_all();
+}
+void CellData::syncToHost() { soa_.download_all(); check(ctx_, hc_wait(ctx_), "wait"); }
+void CellData::syncFieldToHost(uint32_t f) {
+    const size_t w = f < HC_NUM_F64_FIELDS ? sizeof(double) : sizeof(uint32_t);
+    soa_.download(hc_cells_field_offset(f, max_cells), (size_t)num_cells * w);
+    check(ctx_, hc_wait(ctx_), "wait");
+}
+
+// ------------------------------------------------------------ Microenvironment
+void Microenvironment::initialize(const SimConfig& config, hc_ctx* ctx) {
+    ctx_ = ctx;
+    // grid sizing by truncation (src/microenvironment.mm:33-35)
+    grid_.nx = (uint32_t)((config.x_max - config.x_min) / config.dx);
+    grid_.ny = (uint32_t)((config.y_max - config.y_min) / config.dy);
+    grid_.nz = config.use_2D ? 1u : (uint32_t)((config.z_max - config.z_min) / config.dz);
+    grid_.dx = config.dx; grid_.dy = config.dy; grid_.dz = config.dz;
+    grid_.x_min = config.x_min; grid_.y_min = config.y_min; grid_.z_min = config.z_min;
+    grid_.dt = config.dt_diffusion;
+    grid_.n_substrates = (uint32_t)config.substrates.size();
+    if (!grid_.nx || !grid_.ny || !grid_.nz) throw std::runtime_error("Microenvironment: empty grid");
+    subs_.resize(grid_.n_substrates);
+    for (uint32_t s = 0; s < grid_.n_substrates; s++) {
+        const auto& c = config.substrates[s];
+        subs_[s] = hc_substrate{c.diffusion_coeff, c.decay_rate, 0.0, c.dirichlet_value,
+                                c.dirichlet_enabled ? 1u : 0u, 0u};
+    }
+    const size_t V = numVoxels();
+    density_.allocate(ctx, std::max<size_t>(8, V * grid_.n_substrates * sizeof(double)));
+    double* d = (double*)density_.host();
+    for (uint32_t s = 0; s < grid_.n_substrates; s++)
+        std::fill(d + s * V, d + (s + 1) * V, config.substrates[s].initial_value);
+    density_.upload_all();
+    tables_.allocate(ctx, std::max<size_t>(8, hc_thomas_table_bytes(&grid_)));
+    check(ctx, hc_build_thomas_tables(ctx, &grid_, subs_.data(), tables_.get()), "build Thomas tables");
+}
+
+void Microenvironment::simulateDiffusionDecay() {
+    check(ctx_, hc_diffusion_substep(ctx_, density_.get(), &grid_, subs_.data(), tables_.get()),
+          "simulateDiffusionDecay");
+}
+
+void Microenvironment::applyDirichletBC() {
+    check(ctx_, hc_dirichlet(ctx_, density_.get(), &grid_, subs_.data()), "applyDirichletBC");
+}
+
+void Microenvironment::planSecretion(CellData& cells, double dt) {
+    if (plan_capacity_ != cells.max_cells) {
+        plan_.allocate(ctx_, hc_secretion_plan_bytes(&grid_, cells.max_cells));
+        plan_capacity_ = cells.max_cells;
+    }
+    check(ctx_, hc_secretion_plan(ctx_, cells.gpu_buffer(), cells.rates_buffer(), &grid_, dt, cells.num_cells,
+                                  cells.max_cells, plan_.get()),
+          "secretion plan");
+}
+
+void Microenvironment::applySecretion() {
+    if (!plan_.get()) return;
+    check(ctx_, hc_secretion_apply(ctx_, density_.get(), plan_.get(), &grid_), "secretion");
+}
+
+void Microenvironment::refreshVoxelIndices(CellData& cells) {
+    check(ctx_, hc_voxel_keys(ctx_, cells.gpu_buffer(), &grid_, cells.num_cells, cells.max_cells), "voxel keys");
+}
+
+double* Microenvironment::densityBuffer() {
+    syncToHost();
+    return (double*)density_.host();
+}
+void Microenvironment::syncToHost() {
+    density_.download_all();
+    check(ctx_, hc_wait(ctx_), "wait");
+}
+void Microenvironment::syncToDevice() { density_.upload_all(); }
+
+double Microenvironment::getDensity(uint32_t v, uint32_t s) const {
+    const double* d = (const double*)hc_host_mirror(density_.get());
+    return d[(size_t)s * numVoxels() + v];
+}
+void Microenvironment::setDensity(uint32_t v, uint32_t s, double value) {
+    double* d = (double*)hc_host_mirror(density_.get());
+    d[(size_t)s * numVoxels() + v] = value;
+}
+
+// src/microenvironment.mm:176-184
+uint32_t Microenvironment::voxelIndex(double x, double y, double z) const {
+    int i = (int)((x - grid_.x_min) / grid_.dx);
+    int j = (int)((y - grid_.y_min) / grid_.dy);
+    int k = (int)((z - grid_.z_min) / grid_.dz);
+    i = (i < 0) ? 0 : (i >= (int)grid_.nx ? (int)grid_.nx - 1 : i);
+    j = (j < 0) ? 0 : (j >= (int)grid_.ny ? (int)grid_.ny - 1 : j);
+    k = (k < 0) ? 0 : (k >= (int)grid_.nz ? (int)grid_.nz - 1 : k);
+    return (uint32_t)k * grid_.nx * grid_.ny + (uint32_t)j * grid_.nx + (uint32_t)i;
+}
+
+// --------------------------------------------------------------- CellMechanics
+void CellMechanics::initialize(const SimConfig& config, hc_ctx* ctx, uint32_t max_cells) {
+    ctx_ = ctx;
+    max_cells_ = max_cells;
+    // src/cell_mechanics.mm:27-39: 30 um voxels, max(1, trunc(L/30)), nz = 1 in 2-D
+    const double v = 30.0;
+    params_.dt = config.dt_mechanics;
+    params_.cell_cell_repulsion = 10.0;
+    params_.cell_cell_adhesion = 0.4;
+    params_.max_interaction_distance = v;
+    params_.mechanics_voxel_size = v;
+    params_.mech_grid_nx = std::max(1u, (uint32_t)((config.x_max - config.x_min) / v));
+    params_.mech_grid_ny = std::max(1u, (uint32_t)((config.y_max - config.y_min) / v));
+    params_.mech_grid_nz = config.use_2D ? 1u : std::max(1u, (uint32_t)((config.z_max - config.z_min) / v));
+    total_mech_voxels_ = params_.mech_grid_nx * params_.mech_grid_ny * params_.mech_grid_nz;
+    // mech GridParams (src/cell_mechanics.mm:46-59): origin, mech dims, 30 um spacing
+    mgrid_ = hc_grid{};
+    mgrid_.nx = params_.mech_grid_nx; mgrid_.ny = params_.mech_grid_ny; mgrid_.nz = params_.mech_grid_nz;
+    mgrid_.dx = mgrid_.dy = mgrid_.dz = v;
+    mgrid_.x_min = config.x_min; mgrid_.y_min = config.y_min; mgrid_.z_min = config.z_min;
+    hash_.allocate(ctx, hc_hash_bytes(total_mech_voxels_, max_cells));
+}
+
+void CellMechanics::computeForces(CellData& cells, double) {
+    if (cells.num_cells == 0) return;
+    check(ctx_, hc_forces_only_pipeline(ctx_, cells.gpu_buffer(), hash_.get(), &params_, &mgrid_, cells.num_cells,
+                                        cells.max_cells, total_mech_voxels_),
+          "computeForces");
+}
+
+void CellMechanics::integratePositions(CellData& cells, double dt) {
+    if (cells.num_cells == 0) return;
+    check(ctx_, hc_integrate(ctx_, cells.gpu_buffer(), &mgrid_, dt, cells.num_cells, cells.max_cells),
+          "integratePositions");
+}
+
+void CellMechanics::update(CellData& cells, double dt) {
+    if (cells.num_cells == 0) return;
+    check(ctx_, hc_mechanics_pipeline(ctx_, cells.gpu_buffer(), hash_.get(), &params_, &mgrid_, dt, cells.num_cells,
+                                      cells.max_cells, total_mech_voxels_),
+          "mechanics pipeline");
+}
+
+uint32_t CellMechanics::voxelIndexFromPosition(double px, double py, double pz) const {
+    const double v = params_.mechanics_voxel_size;
+    int vi = (int)((px - mgrid_.x_min) / v);
+    int vj = (int)((py - mgrid_.y_min) / v);
+    int vk = (int)((pz - mgrid_.z_min) / v);
+    vi = std::max(0, std::min(vi, (int)params_.mech_grid_nx - 1));
+    vj = std::max(0, std::min(vj, (int)params_.mech_grid_ny - 1));
+    vk = std::max(0, std::min(vk, (int)params_.mech_grid_nz - 1));
+    return (uint32_t)vk * params_.mech_grid_ny * params_.mech_grid_nx + (uint32_t)vj * params_.mech_grid_nx +
+           (uint32_t)vi;
+}
+
+void CellMechanics::syncHashPointers(uint32_t num_cells) {
+    host_offsets_.assign((size_t)total_mech_voxels_ + 1, 0);
+    host_cells_.assign(std::max<uint32_t>(1, num_cells), 0);
+    check(ctx_, hc_hash_download(ctx_, hash_.get(), total_mech_voxels_, max_cells_, num_cells, host_offsets_.data(),
+                                 host_cells_.data()),
+          "syncHashPointers");
+}
+
+uint32_t CellMechanics::voxelCellCount(uint32_t v) const {
+    if (host_offsets_.empty() || v >= total_mech_voxels_) return 0;
+    return host_offsets_[v + 1] - host_offsets_[v];
+}
+
+const uint32_t* CellMechanics::voxelCells(uint32_t v) const {
+    if (host_offsets_.empty() || v >= total_mech_voxels_) return nullptr;
+    return host_cells_.data() + host_offsets_[v];
+}
+
+uint32_t CellMechanics::maxVoxelOccupancy() {
+    uint32_t m = 0;
+    check(ctx_, hc_hash_max_bin(ctx_, hash_.get(), total_mech_voxels_, &m), "max bin");
+    return m;
+}
+
+// ------------------------------------------------------------------ Simulation
+static hc_ctx* open_context(int device) {
+    hc_ctx* ctx = nullptr;
+    int rc = hc_create(&ctx, device);
+    if (rc) throw std::runtime_error("hip_context: no usable HIP device (" + std::to_string(rc) + ")");
+    rc = hc_load_kernels(ctx, nullptr);
+    if (rc) {
+        std::string m = hc_last_error(ctx);
+        hc_destroy(ctx);
+        throw std::runtime_error("hip_context: " + m);
+    }
+    return ctx;
+}
+
+Simulation::Simulation(const SimConfig& cfg, int device)
+    : ctx(open_context(device)), config(cfg), cells(cfg.max_cells) {
+    try {
+        microenv.initialize(config, ctx);
+        cells.attach(ctx, (uint32_t)config.substrates.size());
+        mechanics.initialize(config, ctx, cells.max_cells);
+    } catch (...) {
+        hc_destroy(ctx);
+        throw;
+    }
+}
+
+Simulation::~Simulation() {
+    if (graph_exec_) hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
+    hc_wait(ctx);
+    // buffers are released by their owners before the context goes
+    cells = CellData(1);
+    microenv = Microenvironment();
+    mechanics = CellMechanics();
+    hc_destroy(ctx);
+}
+
+void Simulation::setCells(uint32_t n, const double* x, const double* y, const double* z, const double* radius,
+                          const double* volume, const double* rep, const double* adh, const double* rel_adh,
+                          const double* sec, const double* upt, const double* sat, const double* net,
+                          const uint32_t* alive) {
+    if (n > cells.max_cells) throw std::runtime_error("setCells: more cells than max_cells");
+    cells.num_cells = 0;
+    const uint32_t ns = cells.n_substrates;
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t c = cells.addCell();
+        cells.position_x[c] = x[i];
+        cells.position_y[c] = y[i];
+        cells.position_z[c] = z[i];
+        cells.radius[c] = radius[i];
+        cells.total_volume[c] = volume[i];
+        cells.cell_cell_repulsion[c] = rep[i];
+        cells.cell_cell_adhesion[c] = adh[i];
+        cells.relative_max_adhesion_distance[c] = rel_adh[i];
+        cells.is_alive[c] = alive ? alive[i] : 1u;
+        // placement sets the voxel with the 3-D formula (src/main.mm:95)
+        cells.voxel_index[c] = microenv.voxelIndex(x[i], y[i], z[i]);
+        for (uint32_t s = 0; s < ns; s++) {
+            const size_t o = (size_t)i * ns + s;
+            cells.secretion_rate[o] = sec ? sec[o] : 0.0;
+            cells.uptake_rate[o] = upt ? upt[o] : 0.0;
+            cells.saturation_density[o] = sat ? sat[o] : 0.0;
+            cells.net_export_rate[o] = net ? net[o] : 0.0;
+        }
+    }
+    cells.syncToDevice();
+    microenv.planSecretion(cells, config.dt_diffusion);
+    if (graph_exec_) {
+        hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
+        graph_exec_ = nullptr;
+    }
+}
+
+void Simulation::diffusionSubstep() {
+    microenv.simulateDiffusionDecay();
+    microenv.applySecretion();
+}
+
+void Simulation::mechanicsStep() {
+    mechanics.computeForces(cells, config.dt_mechanics);
+    // (the reference adds CPU motility to velocity here; no motility on the path)
+    mechanics.integratePositions(cells, config.dt_mechanics);
+}
+
+void Simulation::phenotypeStep() {
+    microenv.refreshVoxelIndices(cells);
+    microenv.planSecretion(cells, config.dt_diffusion);
+}
+
+// src/main.mm:275-415 (Appendix B): loop condition, accumulators and order.
+void Simulation::run(double max_time, int64_t counts[3]) {
+    double current_time = 0.0, mech_acc = 0.0, pheno_acc = 0.0;
+    const double dtd = config.dt_diffusion;
+    int64_t ns = 0, nm = 0, np = 0;
+    while (current_time < max_time + 0.5 * dtd) {
+        diffusionSubstep();
+        ns++;
+        mech_acc += dtd;
+        if (mech_acc >= config.dt_mechanics - 0.5 * dtd) {
+            mechanicsStep();
+            nm++;
+            mech_acc = 0.0;
+        }
+        pheno_acc += dtd;
+        if (pheno_acc >= config.dt_phenotype - 0.5 * dtd) {
+            phenotypeStep();
+            np++;
+            pheno_acc = 0.0;
+        }
+        current_time += dtd;
+    }
+    wait();
+    if (counts) { counts[0] = ns; counts[1] = nm; counts[2] = np; }
+}
+
+int Simulation::substepsPerMechanicsStep() const {
+    double acc = 0.0;
+    const double dtd = config.dt_diffusion;
+    for (int k = 1; k < 1000000; k++) {
+        acc += dtd;
+        if (acc >= config.dt_mechanics - 0.5 * dtd) return k;
+    }
+    return 1;
+}
+
+void Simulation::runPeriods(int periods, bool use_graph) {
+    const int k = substepsPerMechanicsStep();
+    if (!use_graph) {
+        for (int p = 0; p < periods; p++) {
+            for (int s = 0; s < k; s++) diffusionSubstep();
+            mechanicsStep();
+        }
+        return;
+    }
+    hipStream_t stream = hc_stream_of(ctx);
+    if (!graph_exec_) {
+        check(ctx, hc_wait(ctx), "wait");
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) throw std::runtime_error(std::string("graph capture: ") + hipGetErrorString(e));
+        try {
+            for (int s = 0; s < k; s++) diffusionSubstep();
+            mechanicsStep();
+        } catch (...) {
+            hipStreamEndCapture(stream, &graph);
+            if (graph) hipGraphDestroy(graph);
+            throw;
+        }
+        e = hipStreamEndCapture(stream, &graph);
+        if (e != hipSuccess) throw std::runtime_error(std::string("graph end capture: ") + hipGetErrorString(e));
+        hipGraphExec_t exec = nullptr;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e != hipSuccess) throw std::runtime_error(std::string("graph instantiate: ") + hipGetErrorString(e));
+        graph_exec_ = (void*)exec;
+    }
+    for (int p = 0; p < periods; p++) {
+        hipError_t e = hipGraphLaunch((hipGraphExec_t)graph_exec_, stream);
+        if (e != hipSuccess) throw std::runtime_error(std::string("graph launch: ") + hipGetErrorString(e));
+    }
+}
+
+void Simulation::wait() { check(ctx, hc_wait(ctx), "wait"); }
+
+}  // namespace physicell
+
+// =================================================================== pc_* C-ABI
+using namespace physicell;
+
+struct pc_sim {
+    std::unique_ptr<Simulation> sim;
+    std::string err;
+};
+
+template <class F>
+static int guarded(pc_sim* s, F&& f) {
+    if (!s || !s->sim) return HC_ERR_ARG;
+    try {
+        f(*s->sim);
+        return 0;
+    } catch (const std::exception& e) {
+        s->err = e.what();
+        return HC_ERR_STATE;
+    }
+}
+
+extern "C" {
+
+int pc_sim_create(pc_sim** out, const pc_config* cfg, int device) {
+    if (!out || !cfg || (cfg->n_substrates && !cfg->substrates)) return HC_ERR_ARG;
+    *out = nullptr;
+    SimConfig c;
+    c.x_min = cfg->x_min; c.x_max = cfg->x_max; c.y_min = cfg->y_min; c.y_max = cfg->y_max;
+    c.z_min = cfg->z_min; c.z_max = cfg->z_max;
+    c.dx = cfg->dx; c.dy = cfg->dy; c.dz = cfg->dz;
+    c.use_2D = cfg->use_2D != 0;
+    c.dt_diffusion = cfg->dt_diffusion; c.dt_mechanics = cfg->dt_mechanics;
+    c.dt_phenotype = cfg->dt_phenotype; c.max_time = cfg->max_time;
+    c.max_cells = cfg->max_cells ? cfg->max_cells : 1;
+    for (uint32_t s = 0; s < cfg->n_substrates; s++) {
+        SubstrateConfig sc;
+        sc.diffusion_coeff = cfg->substrates[s].diffusion_coeff;
+        sc.decay_rate = cfg->substrates[s].decay_rate;
+        sc.initial_value = cfg->substrates[s].initial_value;
+        sc.dirichlet_value = cfg->substrates[s].dirichlet_value;
+        sc.dirichlet_enabled = cfg->substrates[s].dirichlet_enabled != 0;
+        c.substrates.push_back(sc);
+    }
+    auto* s = new pc_sim();
+    try {
+        s->sim.reset(new Simulation(c, device));
+    } catch (const std::exception& e) {
+        fprintf(stderr, "[pc_sim_create] %s\n", e.what());
+        delete s;
+        return HC_ERR_HIP;
+    }
+    *out = s;
+    return 0;
+}
+
+void pc_sim_destroy(pc_sim* s) { delete s; }
+const char* pc_last_error(const pc_sim* s) { return s ? s->err.c_str() : "null simulation"; }
+
+int pc_sim_set_cells(pc_sim* s, uint32_t n, const double* x, const double* y, const double* z,
+                     const double* radius, const double* volume, const double* rep, const double* adh,
+                     const double* rel, const double* sec, const double* upt, const double* sat,
+                     const double* net, const uint32_t* alive) {
+    if (n && (!x || !y || !z || !radius || !volume || !rep || !adh || !rel)) return HC_ERR_ARG;
+    return guarded(s, [&](Simulation& m) { m.setCells(n, x, y, z, radius, volume, rep, adh, rel, sec, upt, sat, net, alive); });
+}
+
+int pc_sim_diffusion_substep(pc_sim* s) { return guarded(s, [](Simulation& m) { m.diffusionSubstep(); }); }
+int pc_sim_mechanics_step(pc_sim* s) { return guarded(s, [](Simulation& m) { m.mechanicsStep(); }); }
+int pc_sim_phenotype_step(pc_sim* s) { return guarded(s, [](Simulation& m) { m.phenotypeStep(); }); }
+int pc_sim_run(pc_sim* s, double max_time, int64_t* counts) {
+    return guarded(s, [&](Simulation& m) { m.run(max_time, counts); });
+}
+int pc_sim_run_periods(pc_sim* s, int periods, int use_graph) {
+    if (periods < 0) return HC_ERR_ARG;
+    return guarded(s, [&](Simulation& m) { m.runPeriods(periods, use_graph != 0); });
+}
+int pc_sim_wait(pc_sim* s) { return guarded(s, [](Simulation& m) { m.wait(); }); }
+
+int pc_sim_get_density(pc_sim* s, double* out) {
+    if (!out) return HC_ERR_ARG;
+    return guarded(s, [&](Simulation& m) {
+        const double* d = m.microenv.densityBuffer();
+        const hc_grid g = m.microenv.getGridParams();
+        std::memcpy(out, d, (size_t)g.nx * g.ny * g.nz * g.n_substrates * sizeof(double));
+    });
+}
+
+int pc_sim_set_density(pc_sim* s, const double* in) {
+    if (!in) return HC_ERR_ARG;
+    return guarded(s, [&](Simulation& m) {
+        m.wait();
+        const hc_grid g = m.microenv.getGridParams();
+        std::memcpy(hc_host_mirror(m.microenv.density()), in, (size_t)g.nx * g.ny * g.nz * g.n_substrates * sizeof(double));
+        m.microenv.syncToDevice();
+        m.wait();
+    });
+}
+
+int pc_sim_get_field(pc_sim* s, uint32_t field, void* out) {
+    if (!out || field >= HC_NUM_CELL_FIELDS) return HC_ERR_ARG;
+    return guarded(s, [&](Simulation& m) {
+        m.cells.syncFieldToHost(field);
+        if (field < HC_NUM_F64_FIELDS)
+            std::memcpy(out, m.cells.field(field), (size_t)m.cells.num_cells * sizeof(double));
+        else
+            std::memcpy(out, m.cells.ufield(field), (size_t)m.cells.num_cells * sizeof(uint32_t));
+    });
+}
+
+int pc_sim_get_hash(pc_sim* s, uint32_t* offsets, uint32_t* cells_out) {
+    if (!offsets) return HC_ERR_ARG;
+    return guarded(s, [&](Simulation& m) {
+        m.mechanics.syncHashPointers(m.cells.num_cells);
+        const auto& off = m.mechanics.hostOffsets();
+        std::memcpy(offsets, off.data(), off.size() * sizeof(uint32_t));
+        if (cells_out && m.cells.num_cells)
+            std::memcpy(cells_out, m.mechanics.voxelCells(0), (size_t)m.cells.num_cells * sizeof(uint32_t));
+    });
+}
+
+int pc_sim_info(const pc_sim* s, hc_grid* grid, hc_mech* mech, uint32_t* n_cells) {
+    if (!s || !s->sim) return HC_ERR_ARG;
+    if (grid) *grid = s->sim->microenv.getGridParams();
+    if (mech) *mech = s->sim->mechanics.params();
+    if (n_cells) *n_cells = s->sim->cells.num_cells;
+    return 0;
+}
+
+hc_ctx* pc_sim_context(pc_sim* s) { return (s && s->sim) ? s->sim->ctx : nullptr; }
+
+}  // extern "C"

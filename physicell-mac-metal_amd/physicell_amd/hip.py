@@ -1,0 +1,388 @@
+"""ctypes binding of libphysicell_hip.so (the C-ABI of include/physicell_hip.h).
+
+Two levels, as in the header:
+  Context / Buffer  -> hc_* (hip_context: device, stream, buffers, dispatches)
+  Simulation        -> pc_* (Microenvironment + CellMechanics + CellData + loop)
+
+The product path is the HIP library; there is no CPU fallback.  Loading
+fails loudly when the library is missing, and every call checks its return
+code and raises HipError with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libphysicell_hip.so")
+HEADER = os.path.join(os.path.dirname(os.path.dirname(HERE)), "include", "physicell_hip.h")
+
+c_u32, c_i32, c_f64, c_sz, c_vp = ctypes.c_uint32, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_void_p
+
+# field numbers (include/physicell_hip.h; the reference's CELL_FIELD_*)
+F_POS_X, F_POS_Y, F_POS_Z = 0, 1, 2
+F_VEL_X, F_VEL_Y, F_VEL_Z = 3, 4, 5
+F_PREV_VX, F_PREV_VY, F_PREV_VZ = 6, 7, 8
+F_RADIUS, F_TOTAL_VOL = 9, 11
+F_REPULSION, F_ADHESION, F_REL_MAX_ADH = 13, 14, 15
+F_PRESSURE = 22
+F_CELL_TYPE, F_CUR_PHASE, F_IS_ALIVE, F_VOXEL_IDX, F_MECH_VOXEL = 23, 24, 25, 26, 27
+NUM_F64_FIELDS = 23
+
+K_SWEEP_X, K_SWEEP_Y, K_SWEEP_Z, K_SWEEP_SMALL, K_DIRICHLET, K_VOXEL_KEYS = 0, 1, 2, 3, 4, 5
+K_SEC_PLAN, K_SEC_APPLY, K_HASH, K_FORCES, K_INTEGRATE = 6, 7, 8, 9, 10
+KERNEL_NAMES = ["sweep_x", "sweep_y", "sweep_z", "sweep_small", "dirichlet", "voxel_keys",
+                "secretion_plan", "secretion_apply", "hash", "forces", "integrate"]
+
+
+class HipError(RuntimeError):
+    pass
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("nx", c_u32), ("ny", c_u32), ("nz", c_u32), ("n_substrates", c_u32),
+                ("dx", c_f64), ("dy", c_f64), ("dz", c_f64), ("dt", c_f64),
+                ("x_min", c_f64), ("y_min", c_f64), ("z_min", c_f64)]
+
+
+class Substrate(ctypes.Structure):
+    _fields_ = [("diffusion_coefficient", c_f64), ("decay_rate", c_f64), ("supply_rate", c_f64),
+                ("target_value", c_f64), ("dirichlet_enabled", c_u32), ("_pad", c_u32)]
+
+
+class Mech(ctypes.Structure):
+    _fields_ = [("dt", c_f64), ("cell_cell_repulsion", c_f64), ("cell_cell_adhesion", c_f64),
+                ("max_interaction_distance", c_f64), ("mechanics_voxel_size", c_f64),
+                ("mech_grid_nx", c_u32), ("mech_grid_ny", c_u32), ("mech_grid_nz", c_u32),
+                ("_pad", c_u32)]
+
+
+class SubstrateCfg(ctypes.Structure):
+    _fields_ = [("diffusion_coeff", c_f64), ("decay_rate", c_f64), ("initial_value", c_f64),
+                ("dirichlet_value", c_f64), ("dirichlet_enabled", c_u32), ("_pad", c_u32)]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("x_min", c_f64), ("x_max", c_f64), ("y_min", c_f64), ("y_max", c_f64),
+                ("z_min", c_f64), ("z_max", c_f64), ("dx", c_f64), ("dy", c_f64), ("dz", c_f64),
+                ("use_2D", c_u32), ("n_substrates", c_u32), ("dt_diffusion", c_f64),
+                ("dt_mechanics", c_f64), ("dt_phenotype", c_f64), ("max_time", c_f64),
+                ("max_cells", c_u32), ("_pad", c_u32), ("substrates", ctypes.POINTER(SubstrateCfg))]
+
+
+_lib = None
+
+
+def declared_functions(header=HEADER):
+    """Every function name declared in include/physicell_hip.h."""
+    txt = open(header).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s*(hc_[a-z0-9_]+|pc_[a-z0-9_]+)\s*\(",
+                       txt, flags=re.M)
+    return sorted(set(names))
+
+
+def load(path=LIB_PATH):
+    """Load the library (no GPU work happens at load time)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise HipError(f"HIP library missing: {path} -- run `python physicell-mac-metal_amd/build.py`")
+    lib = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    sig = {
+        "hc_create": (c_i32, [P(c_vp), c_i32]), "hc_destroy": (None, [c_vp]),
+        "hc_load_kernels": (c_i32, [c_vp, ctypes.c_char_p]),
+        "hc_alloc": (c_i32, [c_vp, c_sz, P(c_vp)]), "hc_free": (None, [c_vp]),
+        "hc_host_mirror": (c_vp, [c_vp]), "hc_device_ptr": (c_vp, [c_vp]), "hc_buf_size": (c_sz, [c_vp]),
+        "hc_upload": (c_i32, [c_vp, c_vp, c_sz, c_sz]), "hc_download": (c_i32, [c_vp, c_vp, c_sz, c_sz]),
+        "hc_wait": (c_i32, [c_vp]), "hc_last_error": (ctypes.c_char_p, [c_vp]),
+        "hc_cells_bytes": (c_sz, [c_u32]), "hc_cells_field_offset": (c_sz, [c_u32, c_u32]),
+        "hc_rates_bytes": (c_sz, [c_u32, c_u32]),
+        "hc_thomas_table_bytes": (c_sz, [P(Grid)]),
+        "hc_build_thomas_tables": (c_i32, [c_vp, P(Grid), P(Substrate), c_vp]),
+        "hc_diffusion_sweep": (c_i32, [c_vp, c_i32, c_vp, P(Grid), P(Substrate), c_vp, c_i32]),
+        "hc_dirichlet": (c_i32, [c_vp, c_vp, P(Grid), P(Substrate)]),
+        "hc_diffusion_substep": (c_i32, [c_vp, c_vp, P(Grid), P(Substrate), c_vp]),
+        "hc_voxel_keys": (c_i32, [c_vp, c_vp, P(Grid), c_u32, c_u32]),
+        "hc_secretion_plan_bytes": (c_sz, [P(Grid), c_u32]),
+        "hc_secretion_plan": (c_i32, [c_vp, c_vp, c_vp, P(Grid), c_f64, c_u32, c_u32, c_vp]),
+        "hc_secretion_apply": (c_i32, [c_vp, c_vp, c_vp, P(Grid)]),
+        "hc_secretion": (c_i32, [c_vp, c_vp, c_vp, c_vp, P(Grid), c_f64, c_u32, c_u32, c_vp]),
+        "hc_hash_bytes": (c_sz, [c_u32, c_u32]),
+        "hc_build_hash": (c_i32, [c_vp, c_vp, c_vp, P(Mech), P(Grid), c_u32, c_u32, c_u32]),
+        "hc_forces": (c_i32, [c_vp, c_vp, c_vp, P(Mech), c_u32, c_u32, c_u32]),
+        "hc_forces_only_pipeline": (c_i32, [c_vp, c_vp, c_vp, P(Mech), P(Grid), c_u32, c_u32, c_u32]),
+        "hc_integrate": (c_i32, [c_vp, c_vp, P(Grid), c_f64, c_u32, c_u32]),
+        "hc_mechanics_pipeline": (c_i32, [c_vp, c_vp, c_vp, P(Mech), P(Grid), c_f64, c_u32, c_u32, c_u32]),
+        "hc_hash_download": (c_i32, [c_vp, c_vp, c_u32, c_u32, c_u32, c_vp, c_vp]),
+        "hc_hash_max_bin": (c_i32, [c_vp, c_vp, c_u32, P(c_u32)]),
+        "hc_neighbor_lists": (c_i32, [c_vp, c_vp, c_vp, P(Mech), c_u32, c_u32, c_u32, c_u32, c_vp, c_vp]),
+        "hc_timing_enable": (c_i32, [c_vp, c_u32]),
+        "hc_timing_get": (c_i32, [c_vp, c_i32, P(c_f64), P(ctypes.c_int64)]),
+        "hc_timing_reset": (c_i32, [c_vp]),
+        "pc_sim_create": (c_i32, [P(c_vp), P(Config), c_i32]), "pc_sim_destroy": (None, [c_vp]),
+        "pc_last_error": (ctypes.c_char_p, [c_vp]),
+        "pc_sim_set_cells": (c_i32, [c_vp, c_u32] + [c_vp] * 13),
+        "pc_sim_diffusion_substep": (c_i32, [c_vp]), "pc_sim_mechanics_step": (c_i32, [c_vp]),
+        "pc_sim_phenotype_step": (c_i32, [c_vp]),
+        "pc_sim_run": (c_i32, [c_vp, c_f64, c_vp]), "pc_sim_run_periods": (c_i32, [c_vp, c_i32, c_i32]),
+        "pc_sim_wait": (c_i32, [c_vp]), "pc_sim_get_density": (c_i32, [c_vp, c_vp]),
+        "pc_sim_set_density": (c_i32, [c_vp, c_vp]), "pc_sim_get_field": (c_i32, [c_vp, c_u32, c_vp]),
+        "pc_sim_get_hash": (c_i32, [c_vp, c_vp, c_vp]),
+        "pc_sim_info": (c_i32, [c_vp, P(Grid), P(Mech), P(c_u32)]),
+        "pc_sim_context": (c_vp, [c_vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def grid_of(spec):
+    return Grid(spec.nx, spec.ny, spec.nz, len(spec.substrates), spec.dx, spec.dy, spec.dz,
+                spec.dt_diff, spec.x_min, spec.y_min, spec.z_min)
+
+
+def mech_grid_of(spec):
+    v = spec.mech_voxel
+    return Grid(spec.mnx, spec.mny, spec.mnz, 0, v, v, v, 0.0, spec.x_min, spec.y_min, spec.z_min)
+
+
+def mech_of(spec):
+    v = spec.mech_voxel
+    return Mech(spec.dt_mech, 10.0, 0.4, v, v, spec.mnx, spec.mny, spec.mnz, 0)
+
+
+def substrates_of(spec):
+    arr = (Substrate * max(1, len(spec.substrates)))()
+    for i, s in enumerate(spec.substrates):
+        arr[i] = Substrate(s.D, s.decay, 0.0, s.dirichlet_value, 1 if s.dirichlet else 0, 0)
+    return arr
+
+
+class Buffer:
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        h = c_vp()
+        ctx._check(load().hc_alloc(ctx.h, self.nbytes, ctypes.byref(h)), "hc_alloc")
+        self.h = h
+
+    def upload(self, arr, offset=0):
+        arr = np.ascontiguousarray(arr)
+        lib = load()
+        host = lib.hc_host_mirror(self.h)
+        if not host:
+            raise HipError("no host mirror")
+        ctypes.memmove(host + offset, arr.ctypes.data, arr.nbytes)
+        self.ctx._check(lib.hc_upload(self.ctx.h, self.h, offset, arr.nbytes), "hc_upload")
+        self.ctx.wait()
+
+    def download(self, dtype, count, offset=0):
+        lib = load()
+        nb = np.dtype(dtype).itemsize * count
+        self.ctx._check(lib.hc_download(self.ctx.h, self.h, offset, nb), "hc_download")
+        self.ctx.wait()
+        host = lib.hc_host_mirror(self.h)
+        out = np.empty(count, dtype)
+        ctypes.memmove(out.ctypes.data, host + offset, nb)
+        return out
+
+    def free(self):
+        if self.h:
+            load().hc_free(self.h)
+            self.h = None
+
+
+class Context:
+    """hip_context on one device (hc_create + hc_load_kernels)."""
+
+    def __init__(self, device=0):
+        lib = load()
+        h = c_vp()
+        rc = lib.hc_create(ctypes.byref(h), device)
+        if rc:
+            raise HipError(f"hc_create failed ({rc}): no usable HIP device")
+        self.h = h
+        self._check(lib.hc_load_kernels(self.h, None), "hc_load_kernels")
+
+    def _check(self, rc, what):
+        if rc:
+            raise HipError(f"{what} failed ({rc}): {load().hc_last_error(self.h).decode()}")
+
+    def alloc(self, nbytes):
+        return Buffer(self, nbytes)
+
+    def wait(self):
+        self._check(load().hc_wait(self.h), "hc_wait")
+
+    def close(self):
+        if self.h:
+            load().hc_destroy(self.h)
+            self.h = None
+
+    # timing
+    def timing(self, mask):
+        self._check(load().hc_timing_enable(self.h, mask), "hc_timing_enable")
+
+    def timing_get(self, k):
+        ms, n = c_f64(), ctypes.c_int64()
+        self._check(load().hc_timing_get(self.h, k, ctypes.byref(ms), ctypes.byref(n)), "hc_timing_get")
+        return ms.value, n.value
+
+    def timing_reset(self):
+        self._check(load().hc_timing_reset(self.h), "hc_timing_reset")
+
+
+class CellBuffer:
+    """Device SoA (hc_cells_bytes) plus rate buffer, filled from a Cells population."""
+
+    def __init__(self, ctx, cells, ns, max_cells=None):
+        lib = load()
+        self.ctx = ctx
+        self.n = cells.n
+        self.max_cells = max_cells or max(1, cells.n)
+        self.ns = ns
+        self.soa = ctx.alloc(lib.hc_cells_bytes(self.max_cells))
+        self.rates = ctx.alloc(max(8, lib.hc_rates_bytes(self.max_cells, ns)))
+        n, M = self.n, self.max_cells
+        f64 = np.zeros((NUM_F64_FIELDS, M))
+        f64[F_POS_X, :n], f64[F_POS_Y, :n], f64[F_POS_Z, :n] = cells.x, cells.y, cells.z
+        f64[F_RADIUS, :n] = cells.radius
+        f64[F_TOTAL_VOL, :n] = cells.volume
+        f64[F_REPULSION, :n], f64[F_ADHESION, :n], f64[F_REL_MAX_ADH, :n] = cells.rep, cells.adh, cells.rel_adh
+        u32 = np.zeros((5, M), np.uint32)
+        u32[F_IS_ALIVE - NUM_F64_FIELDS, :n] = cells.alive
+        self.soa.upload(f64.ravel())
+        self.soa.upload(u32.ravel(), offset=lib.hc_cells_field_offset(23, M))
+        if ns:
+            r = np.zeros((4, M * ns))
+            for k, a in enumerate((cells.sec, cells.upt, cells.sat, cells.net)):
+                r[k, :n * ns] = np.asarray(a, np.float64).reshape(-1)
+            self.rates.upload(r.ravel())
+
+    def field(self, f, count=None):
+        lib = load()
+        count = self.n if count is None else count
+        dt = np.float64 if f < NUM_F64_FIELDS else np.uint32
+        return self.soa.download(dt, count, lib.hc_cells_field_offset(f, self.max_cells))
+
+    def set_field(self, f, values):
+        lib = load()
+        dt = np.float64 if f < NUM_F64_FIELDS else np.uint32
+        self.soa.upload(np.asarray(values, dt), lib.hc_cells_field_offset(f, self.max_cells))
+
+
+class Simulation:
+    """pc_sim: the C++ step loop on one device."""
+
+    def __init__(self, spec, cells, device=0, max_cells=None):
+        lib = load()
+        self.spec = spec
+        ns = len(spec.substrates)
+        self._subs = (SubstrateCfg * max(1, ns))()
+        for i, s in enumerate(spec.substrates):
+            self._subs[i] = SubstrateCfg(s.D, s.decay, s.initial, s.dirichlet_value, 1 if s.dirichlet else 0, 0)
+        cfg = Config(spec.x_min, spec.x_max, spec.y_min, spec.y_max, spec.z_min, spec.z_max,
+                     spec.dx, spec.dy, spec.dz, 1 if spec.use_2d else 0, ns, spec.dt_diff,
+                     spec.dt_mech, spec.dt_pheno, spec.max_time,
+                     max_cells or max(1, cells.n), 0, self._subs)
+        h = c_vp()
+        rc = lib.pc_sim_create(ctypes.byref(h), ctypes.byref(cfg), device)
+        if rc:
+            raise HipError(f"pc_sim_create failed ({rc})")
+        self.h = h
+        self.n = cells.n
+        self.ns = ns
+        self.set_cells(cells)
+
+    def _check(self, rc, what):
+        if rc:
+            raise HipError(f"{what} failed ({rc}): {load().pc_last_error(self.h).decode()}")
+
+    def set_cells(self, c):
+        n = c.n
+        self.n = n
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (c.x, c.y, c.z, c.radius, c.volume, c.rep, c.adh, c.rel_adh)]
+        rates = [np.ascontiguousarray(np.asarray(a, np.float64).reshape(-1)) for a in (c.sec, c.upt, c.sat, c.net)]
+        alive = np.ascontiguousarray(c.alive, np.uint32)
+        self._keep = arrs + rates + [alive]
+        self._check(load().pc_sim_set_cells(self.h, n, *[_ptr(a) for a in arrs], *[_ptr(a) for a in rates],
+                                            _ptr(alive)), "pc_sim_set_cells")
+
+    def substep(self):
+        self._check(load().pc_sim_diffusion_substep(self.h), "pc_sim_diffusion_substep")
+
+    def mech_step(self):
+        self._check(load().pc_sim_mechanics_step(self.h), "pc_sim_mechanics_step")
+
+    def pheno_step(self):
+        self._check(load().pc_sim_phenotype_step(self.h), "pc_sim_phenotype_step")
+
+    def run(self, max_time):
+        counts = np.zeros(3, np.int64)
+        self._check(load().pc_sim_run(self.h, max_time, _ptr(counts)), "pc_sim_run")
+        return tuple(int(v) for v in counts)
+
+    def run_periods(self, periods, use_graph=False):
+        self._check(load().pc_sim_run_periods(self.h, periods, 1 if use_graph else 0), "pc_sim_run_periods")
+
+    def wait(self):
+        self._check(load().pc_sim_wait(self.h), "pc_sim_wait")
+
+    def density(self):
+        out = np.empty(self.ns * self.spec.n_voxels)
+        self._check(load().pc_sim_get_density(self.h, _ptr(out)), "pc_sim_get_density")
+        return out
+
+    def set_density(self, rho):
+        rho = np.ascontiguousarray(rho, np.float64)
+        self._check(load().pc_sim_set_density(self.h, _ptr(rho)), "pc_sim_set_density")
+
+    def field(self, f):
+        out = np.empty(self.n, np.float64 if f < NUM_F64_FIELDS else np.uint32)
+        self._check(load().pc_sim_get_field(self.h, f, _ptr(out)), "pc_sim_get_field")
+        return out
+
+    def hash(self):
+        off = np.empty(self.spec.n_mech_voxels + 1, np.uint32)
+        cells = np.empty(max(1, self.n), np.uint32)
+        self._check(load().pc_sim_get_hash(self.h, _ptr(off), _ptr(cells)), "pc_sim_get_hash")
+        return off, cells[:self.n]
+
+    def context(self):
+        return load().pc_sim_context(self.h)
+
+    def timing(self, mask):
+        rc = load().hc_timing_enable(self.context(), mask)
+        if rc:
+            raise HipError("hc_timing_enable failed")
+
+    def timing_get(self, k):
+        ms, n = c_f64(), ctypes.c_int64()
+        rc = load().hc_timing_get(self.context(), k, ctypes.byref(ms), ctypes.byref(n))
+        if rc:
+            raise HipError("hc_timing_get failed")
+        return ms.value, n.value
+
+    def close(self):
+        if self.h:
+            load().pc_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
