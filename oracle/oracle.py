@@ -77,6 +77,11 @@ def ref_lib():
 
 
 def P(a, t=f64p):
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("oracle arrays must be C-contiguous")
+    want = np.uint32 if t is u32p else (np.float32 if t is f32p else np.float64)
+    if a.dtype != want:
+        raise TypeError(f"oracle array dtype {a.dtype}, expected {np.dtype(want)}")
     return a.ctypes.data_as(t)
 
 

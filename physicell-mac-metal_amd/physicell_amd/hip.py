@@ -146,7 +146,11 @@ def load(path=LIB_PATH):
 
 
 def _ptr(a):
-    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("arrays passed to the C-ABI must be C-contiguous")
+    return ctypes.c_void_p(a.ctypes.data)
 
 
 def grid_of(spec):

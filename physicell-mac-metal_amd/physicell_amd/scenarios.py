@@ -120,8 +120,8 @@ def uniform_cells(x, y, z, ns, radius=DEFAULT_RADIUS, volume=DEFAULT_VOLUME,
                   rep=10.0, adh=0.4, rel_adh=1.25):
     n = len(x)
     f = lambda v: np.full(n, v, np.float64)
-    return Cells(x=np.asarray(x, np.float64), y=np.asarray(y, np.float64),
-                 z=np.asarray(z, np.float64), radius=f(radius), volume=f(volume),
+    c64 = lambda a: np.ascontiguousarray(a, np.float64)
+    return Cells(x=c64(x), y=c64(y), z=c64(z), radius=f(radius), volume=f(volume),
                  rep=f(rep), adh=f(adh), rel_adh=f(rel_adh),
                  sec=np.zeros((n, ns)), upt=np.zeros((n, ns)), sat=np.zeros((n, ns)),
                  net=np.zeros((n, ns)), alive=np.ones(n, np.uint32),
