@@ -1,0 +1,185 @@
+"""Benchmark of the step hot path on one MI355X.
+
+Workload (default): 200^3 voxels x 4 substrates, 1,000,000 agents, plain FP64
+(the large single-simulation configuration, on which the HBM-roofline target
+is stated).  One "step" = one mechanics period: 10 diffusion substeps (x, y, z
+Thomas sweeps with the Dirichlet re-apply fused into the z sweep, then
+secretion/uptake) followed by one mechanics step (hash, forces, AB2).  The
+period is captured as a hipGraph and replayed.  Inputs are resident in HBM
+before the timed region.
+
+Prints ONE JSON line (rank 0) with agent-steps/s as `value`, voxel-substeps/s,
+p50/p99 step latency, the roofline of the dominant kernel (HIP events on the
+context stream around that kernel's launches, in the timed region) and the
+CPU baseline (the FP64 oracle on this host's cores, bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "physicell-mac-metal_amd"))
+
+from physicell_amd import hip as H  # noqa: E402
+from physicell_amd import scenarios as S  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "agent-steps/s + voxel-substeps/s at 1 GPU, % HBM roofline; p99 step latency"
+SWEEP_KERNELS = (H.K_SWEEP_X, H.K_SWEEP_Y, H.K_SWEEP_Z)
+
+
+def build_workload(name, n_cells):
+    if name == "config2":
+        return S.config2(n_cells=n_cells)
+    if name == "config1":
+        return S.config1()
+    if name == "config0":
+        return S.config0()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def algorithmic_bytes(kernel, spec, n_cells):
+    """Compulsory HBM bytes of one launch (FP64 plain form).
+    Sweeps: 1 read + 1 write of every field value = 16 B per voxel per
+    substrate.  Forces: 64-byte sorted record read + bin offsets (amortised
+    4 B) + velocity/pressure write 32 B per agent.  Integrate: read pos, v,
+    v_prev (72 B) + write pos, v_prev (48 B) per agent."""
+    V, ns = spec.n_voxels, len(spec.substrates)
+    if kernel in SWEEP_KERNELS:
+        return 16.0 * V * ns
+    if kernel == H.K_FORCES:
+        return (64.0 + 4.0 + 32.0) * n_cells
+    if kernel == H.K_INTEGRATE:
+        return 120.0 * n_cells
+    return None
+
+
+def cpu_baseline(spec, cells, periods, substeps):
+    """FP64 oracle (the reference's algorithm restated in C++, OpenMP) on this
+    host: `periods` x (substeps diffusion substeps + 1 mechanics step)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = max(1, min(16, os.cpu_count() or 1))
+    sim = O.OracleSim(spec, cells)
+    t0 = time.perf_counter()
+    for _ in range(periods):
+        for _ in range(substeps):
+            sim.substep(nthreads=threads)
+        sim.mech_step(nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": cells.n * periods / dt, "unit": "agent-steps/s", "cores": threads, "kind": "port",
+            "voxel_substeps_per_s": spec.n_voxels * len(spec.substrates) * substeps * periods / dt,
+            "sample": f"{periods} period(s) of {substeps} diffusion substeps + 1 mechanics step of the "
+                      f"same workload, FP64 oracle, OpenMP {threads} threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="config2")
+    ap.add_argument("--cells", type=int, default=1_000_000)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--cpu-periods", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_config2.json"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); every box here has one GPU "
+              f"and the multi-GPU decomposition is design-only.", file=sys.stderr)
+        sys.exit(2)
+
+    t_setup = time.perf_counter()
+    spec, cells = build_workload(args.workload, args.cells)
+    sim = H.Simulation(spec, cells)
+    k_sub = 10 if abs(spec.dt_mech / spec.dt_diff - 10.0) < 1e-9 else None
+    use_graph = not args.no_graph
+    print(f"[bench] {spec.name}: grid {spec.nx}x{spec.ny}x{spec.nz} x {len(spec.substrates)} substrates, "
+          f"{cells.n} agents; setup {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
+
+    # warmup (also builds / instantiates the graph)
+    sim.run_periods(args.warmup, use_graph=use_graph)
+    sim.wait()
+
+    # timed region: K periods, graph replay
+    sim.wait()
+    t0 = time.perf_counter()
+    sim.run_periods(args.steps, use_graph=use_graph)
+    sim.wait()
+    T = time.perf_counter() - t0
+    ms_per_step = 1e3 * T / args.steps
+    print(f"[bench] timed {args.steps} steps: {T:.3f} s", file=sys.stderr, flush=True)
+
+    # per-step latency: each period launched and waited for on its own
+    lat = []
+    for _ in range(max(10, min(args.steps, 100))):
+        t1 = time.perf_counter()
+        sim.run_periods(1, use_graph=use_graph)
+        sim.wait()
+        lat.append(1e3 * (time.perf_counter() - t1))
+    lat = np.array(lat)
+
+    # kernel timing: HIP events on the context stream around each launch
+    # (eager launches: events are not recorded inside graph replays)
+    sim.timing(0xFFFFFFFF)
+    kp = max(2, min(args.steps, 10))
+    sim.run_periods(kp, use_graph=False)
+    sim.wait()
+    kern = {}
+    for k, name in enumerate(H.KERNEL_NAMES):
+        ms, n = sim.timing_get(k)
+        if n:
+            kern[name] = {"total_ms": ms, "launches": n, "avg_us": 1e3 * ms / n}
+    sim.timing(0)
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    dom_id = H.KERNEL_NAMES.index(dom)
+    ab = algorithmic_bytes(dom_id, spec, cells.n)
+    avg_s = kern[dom]["avg_us"] * 1e-6
+    traffic = None
+    if os.path.exists(args.traffic):
+        try:
+            tr = json.load(open(args.traffic))
+            traffic = tr.get(dom, {}).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    roof = {"bound": "hbm", "kernel": dom, "achieved": ab / avg_s / 1e9 if ab else None,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": (ab / avg_s / 1e9) / HBM_PEAK_GBS if ab else None, "traffic": traffic,
+            "algorithmic_bytes_per_launch": ab, "avg_launch_us": kern[dom]["avg_us"]}
+    # the three sweeps together (the diffusion substep's roofline)
+    sw = [kern[n] for n in ("sweep_x", "sweep_y", "sweep_z") if n in kern]
+    if sw:
+        sweep_bytes = 16.0 * spec.n_voxels * len(spec.substrates) * len(sw)
+        sweep_s = sum(x["avg_us"] for x in sw) * 1e-6
+        roof["sweeps_combined_frac"] = sweep_bytes / sweep_s / 1e9 / HBM_PEAK_GBS
+
+    n_sub = k_sub or 10
+    value = cells.n * args.steps / T
+    vss = spec.n_voxels * len(spec.substrates) * n_sub * args.steps / T
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(spec, cells, args.cpu_periods, n_sub)
+    out = {
+        "metric": METRIC, "value": value, "unit": "agent-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{spec.name}: {spec.nx}x{spec.ny}x{spec.nz} voxels x {len(spec.substrates)} "
+                               f"substrates, {cells.n} agents, plain FP64; step = {n_sub} diffusion substeps "
+                               f"+ 1 mechanics step", "graph": use_graph},
+        "voxel_substeps_per_s": vss,
+        "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
+        "roofline": roof, "kernels": kern, "cpu_baseline": cpu,
+    }
+    sim.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
