@@ -88,7 +88,19 @@ def main():
     ap.add_argument("--cpu-periods", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_config2.json"))
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain FP64 device copy of 2^27 elements (1 GiB read + 1 GiB "
+                         "written) five times, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 8, "copy_bytes_written": y.numel() * 8}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); every box here has one GPU "
               f"and the multi-GPU decomposition is design-only.", file=sys.stderr)
