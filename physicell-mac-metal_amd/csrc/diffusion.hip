@@ -283,7 +283,9 @@ __global__ __launch_bounds__(256) void k_dirichlet(double* __restrict__ rho,
     if (face) rho[(uint64_t)s * V + v] = sc[s].T;
 }
 
+int probe_kernels_part(hc_ctx* ctx);
 int probe_kernels_diffusion(hc_ctx* ctx) {
+    if (int r = probe_kernels_part(ctx)) return r;
     hipFuncAttributes attr;
     const void* fns[] = {(const void*)k_sweep_lane<0, 8>, (const void*)k_sweep_lane<1, 8>,
                          (const void*)k_sweep_lane<2, 8>, (const void*)k_sweep_x_lds,
@@ -299,6 +301,10 @@ int launch_sweep(hc_ctx* ctx, int axis, double* rho, const hc_grid* g, const uin
                  int fuse_dirichlet) {
     const uint32_t nx = g->nx, ny = g->ny, nz = g->nz, ns = g->n_substrates;
     if (ns == 0 || nx == 0 || ny == 0 || nz == 0) return 0;
+    if (ctx->sweep_mode == 0) {
+        const int r = launch_sweep_part(ctx, axis, rho, g, tables, fuse_dirichlet);
+        if (r != 1) return r;
+    }
     const uint32_t nmax = grid_nmax(g);
     const uint64_t V = (uint64_t)nx * ny * nz;
     if (axis == 0 && nx >= 4 && !fuse_dirichlet) {
@@ -339,7 +345,8 @@ int launch_dirichlet(hc_ctx* ctx, double* rho, const hc_grid* g, const uint8_t* 
 // (shaders/diffusion_2d.metal:89-142) evaluated once in FP64.
 static void build_tables_host(const hc_grid* g, const hc_substrate* subs, std::vector<uint8_t>& out) {
     const uint32_t ns = g->n_substrates, nmax = grid_nmax(g);
-    out.assign(subconst_bytes(ns) + (size_t)ns * 3 * 2 * nmax * sizeof(double), 0);
+    out.assign(part_offset(ns, nmax) + (size_t)ns * 3 * part_block(nmax) * sizeof(double), 0);
+    build_part_tables_host(g, subs, reinterpret_cast<double*>(out.data() + part_offset(ns, nmax)));
     SubConst* sc = reinterpret_cast<SubConst*>(out.data());
     double* tab = reinterpret_cast<double*>(out.data() + subconst_bytes(ns));
     const double sp[3] = {g->dx, g->dy, g->dz};
@@ -384,7 +391,14 @@ extern "C" {
 
 size_t hc_thomas_table_bytes(const hc_grid* g) {
     if (!g) return 0;
-    return subconst_bytes(g->n_substrates) + (size_t)g->n_substrates * 3 * 2 * grid_nmax(g) * sizeof(double);
+    const uint32_t nmax = grid_nmax(g);
+    return part_offset(g->n_substrates, nmax) + (size_t)g->n_substrates * 3 * part_block(nmax) * sizeof(double);
+}
+
+int hc_set_sweep_mode(hc_ctx* ctx, int mode) {
+    if (!ctx || mode < 0 || mode > 1) return fail(ctx, HC_ERR_ARG, "hc_set_sweep_mode: mode must be 0 or 1");
+    ctx->sweep_mode = mode;
+    return 0;
 }
 
 int hc_build_thomas_tables(hc_ctx* ctx, const hc_grid* g, const hc_substrate* subs, hc_buf* tables) {

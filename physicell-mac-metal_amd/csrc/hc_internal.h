@@ -22,6 +22,9 @@ struct hc_ctx {
     std::vector<hipEvent_t> event_pool;
     double total_ms[HC_K_COUNT] = {};
     int64_t launches[HC_K_COUNT] = {};
+    // 0: partitioned sweeps where the line length suits them, 1: serial
+    // elimination (the reference's operation order, bit-identical to it in FP64)
+    int sweep_mode = 0;
     // small scratch for reductions read back by the host (max bin etc.)
     uint32_t* d_scalar = nullptr;
 };
@@ -75,7 +78,28 @@ inline uint32_t grid_nmax(const hc_grid* g) {
     return m > g->nz ? m : g->nz;
 }
 
-// ---- launchers implemented in diffusion.hip / cells.hip ----
+// ---- partitioned sweep tables (see sweep_part.hip) ----
+// A line's N = n-2 unknowns are split into PART_P contiguous segments, one per
+// lane.  Per (substrate, axis) block: for each segment k the arrays
+// 1/den, c', alpha, beta of length part_mt(nmax), then the two rows of the
+// inverse reduced interface matrix (left and right neighbour values) of
+// length 2*PART_P.
+static constexpr int PART_P = 4;
+__host__ __device__ inline uint32_t part_mt(uint32_t nmax) {
+    return nmax > 2 ? (nmax - 2 + PART_P - 1) / PART_P : 1u;
+}
+__host__ __device__ inline size_t part_block(uint32_t nmax) {
+    return (size_t)PART_P * 4 * part_mt(nmax) + (size_t)PART_P * 2 * 2 * PART_P;
+}
+__host__ __device__ inline size_t serial_table_bytes(uint32_t ns, uint32_t nmax) {
+    return ((subconst_bytes(ns) + (size_t)ns * 3 * 2 * nmax * sizeof(double)) + 255) / 256 * 256;
+}
+__host__ __device__ inline size_t part_offset(uint32_t ns, uint32_t nmax) { return serial_table_bytes(ns, nmax); }
+
+// ---- launchers implemented in diffusion.hip / cells.hip / sweep_part.hip ----
+int launch_sweep_part(hc_ctx* ctx, int axis, double* rho, const hc_grid* g, const uint8_t* tables,
+                      int fuse_dirichlet);   // returns 1 when the grid does not suit it (caller falls back)
+void build_part_tables_host(const hc_grid* g, const hc_substrate* subs, double* out);
 int launch_sweep(hc_ctx* ctx, int axis, double* rho, const hc_grid* g, const uint8_t* tables,
                  int fuse_dirichlet);
 int launch_dirichlet(hc_ctx* ctx, double* rho, const hc_grid* g, const uint8_t* tables);

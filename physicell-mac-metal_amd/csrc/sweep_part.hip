@@ -1,0 +1,346 @@
+// sweep_part.hip -- partitioned LOD Thomas sweep (FP64, gfx950).
+//
+// Solves the same per-line system as the serial sweep (edge rows with the
+// boundary diagonal 1+c+h and +c*T with Dirichlet; shaders/diffusion_2d.metal
+// :89-155) without the forward pass's d' ever leaving the chip:
+//
+//   * the N = n-2 unknowns of a line are split into 4 contiguous segments,
+//     one per lane; a lane keeps its segment in registers (<= MM values);
+//   * each lane solves its segment with the neighbouring unknowns set to zero
+//     (y = A_k^-1 d_k, elimination factors from a line-independent table);
+//   * the true solution is u = y + uL*alpha_k + uR*beta_k, with
+//     alpha_k = c A_k^-1 e_first, beta_k = c A_k^-1 e_last (tables), where uL
+//     and uR are the last unknown of the segment to the left and the first of
+//     the one to the right;
+//   * uL and uR follow from the 8 segment end values y of the line (gathered
+//     with cross-lane shuffles) through two rows of the inverse of the 8x8
+//     interface system, also a table.
+//
+// Every value is read once and written once (16 bytes per voxel per sweep in
+// FP64).  16 lines x 4 segments fill a wave; for y and z lanes of one segment
+// walk 16 adjacent x positions, so each load is a full 128-byte row segment.
+// x lines are contiguous: a wave stages its 16 lines through LDS.
+#include "hc_internal.h"
+
+#include <cmath>
+
+namespace hc {
+
+static constexpr int LPW = 64 / PART_P;      // lines per wave
+
+template <int AXIS, int MM>
+__global__ __launch_bounds__(AXIS == 0 ? 64 : 256) void k_sweep_part(
+    double* __restrict__ rho, const uint8_t* __restrict__ tables, uint32_t nx, uint32_t ny, uint32_t nz,
+    uint32_t nmax, int fuse_dirichlet) {
+    constexpr int P = PART_P;
+    constexpr int WPB = AXIS == 0 ? 1 : 4;
+    extern __shared__ double smem[];
+    const uint32_t s = blockIdx.y, ns = gridDim.y;
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t k = lane / LPW, l = lane % LPW;
+    const uint32_t n = AXIS == 0 ? nx : (AXIS == 1 ? ny : nz);
+    const uint32_t N = n - 2;
+    const uint32_t MT = part_mt(nmax);
+    const uint32_t blk = (uint32_t)part_block(nmax);
+    const uint64_t V = (uint64_t)nx * ny * nz;
+
+    // this (substrate, axis) table block -> LDS
+    const double* gt = reinterpret_cast<const double*>(tables + part_offset(ns, nmax)) + (size_t)(s * 3 + AXIS) * blk;
+    double* tab = smem;
+    for (uint32_t q = threadIdx.x; q < blk; q += blockDim.x) tab[q] = gt[q];
+
+    const uint64_t Lw0 = ((uint64_t)blockIdx.x * WPB + w) * LPW;     // first line of this wave
+    const uint64_t L = Lw0 + l;
+    uint64_t stride, base, lines;
+    bool face = false;
+    if (AXIS == 0) {
+        stride = 1; lines = (uint64_t)ny * nz; base = L * nx;
+    } else if (AXIS == 1) {
+        stride = nx; lines = (uint64_t)nz * nx;
+        const uint64_t kk = L / nx, i = L % nx;
+        base = kk * nx * ny + i;
+        face = (i == 0 || i == nx - 1);
+    } else {
+        stride = (uint64_t)nx * ny; lines = (uint64_t)ny * nx;
+        const uint64_t j = L / nx, i = L % nx;
+        base = L;
+        face = (i == 0 || i == nx - 1 || j == 0 || j == ny - 1);
+    }
+    const bool valid = L < lines;
+    const SubConst* sc = reinterpret_cast<const SubConst*>(tables);
+    const double c = sc[s].c[AXIS];
+    const double T = sc[s].T;
+    const bool dir = sc[s].dir != 0;
+    const bool fill = fuse_dirichlet && dir && face;
+    const uint32_t sk = (k * N) / P;
+    const uint32_t mk = ((k + 1) * N) / P - sk;
+    double* p = rho + (uint64_t)s * V + base;
+
+    double* tile = smem + ((blk + 31) / 32) * 32 + (size_t)w * LPW * (n + 1);
+    __syncthreads();                                    // tables visible
+
+    double v[MM];
+    if (AXIS == 0) {
+        // the wave's LPW consecutive x lines are LPW*n contiguous values
+        const double* src = rho + (uint64_t)s * V + Lw0 * nx;
+        const uint32_t total = LPW * n;
+        uint32_t ll = lane / n, col = lane % n;
+        const uint32_t step_l = 64 / n, step_c = 64 % n;
+        for (uint32_t e = lane; e < total; e += 64) {
+            if (Lw0 + ll < lines) tile[ll * (n + 1) + col] = src[e];
+            ll += step_l;
+            col += step_c;
+            if (col >= n) { col -= n; ll += 1; }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < MM; t++) v[t] = (t < (int)mk) ? tile[l * (n + 1) + sk + 1 + t] : 0.0;
+    } else {
+#pragma unroll
+        for (int t = 0; t < MM; t++) v[t] = (t < (int)mk && valid) ? p[(uint64_t)(sk + 1 + t) * stride] : 0.0;
+    }
+
+    const double* idn = tab + (k * 4 + 0) * MT;
+    const double* cpl = tab + (k * 4 + 1) * MT;
+    const double* al = tab + (k * 4 + 2) * MT;
+    const double* be = tab + (k * 4 + 3) * MT;
+    const double* gl = tab + (size_t)P * 4 * MT + (k * 2 + 0) * 2 * P;
+    const double* gr = tab + (size_t)P * 4 * MT + (k * 2 + 1) * 2 * P;
+
+    if (dir) {
+        const double ct = c * T;
+        if (k == 0) v[0] += ct;
+#pragma unroll
+        for (int t = 0; t < MM; t++)
+            if (k == P - 1 && t == (int)mk - 1) v[t] += ct;
+    }
+    // segment solve with zero coupling
+    double dp = 0.0;
+#pragma unroll
+    for (int t = 0; t < MM; t++)
+        if (t < (int)mk) {
+            dp = (v[t] + c * dp) * idn[t];
+            v[t] = dp;
+        }
+    double y = 0.0;
+#pragma unroll
+    for (int t = MM - 1; t >= 0; t--)
+        if (t < (int)mk) {
+            y = v[t] - cpl[t] * y;
+            v[t] = y;
+        }
+    const double yS = v[0];
+    double yE = 0.0;
+#pragma unroll
+    for (int t = 0; t < MM; t++)
+        if (t == (int)mk - 1) yE = v[t];
+    // interface values of the line (its 4 lanes are l, l+16, l+32, l+48)
+    double uL = 0.0, uR = 0.0;
+#pragma unroll
+    for (int j = 0; j < P; j++) {
+        const double a = __shfl(yS, j * LPW + l, 64);
+        const double b = __shfl(yE, j * LPW + l, 64);
+        uL += gl[2 * j] * a + gl[2 * j + 1] * b;
+        uR += gr[2 * j] * a + gr[2 * j + 1] * b;
+    }
+#pragma unroll
+    for (int t = 0; t < MM; t++)
+        if (t < (int)mk) v[t] = v[t] + al[t] * uL + be[t] * uR;
+
+    if (AXIS == 0) {
+#pragma unroll
+        for (int t = 0; t < MM; t++)
+            if (t < (int)mk) tile[l * (n + 1) + sk + 1 + t] = v[t];
+        if (dir) {
+            if (k == 0) tile[l * (n + 1)] = T;
+            if (k == P - 1) tile[l * (n + 1) + n - 1] = T;
+        }
+        __syncthreads();
+        double* dst = rho + (uint64_t)s * V + Lw0 * nx;
+        const uint32_t total = LPW * n;
+        uint32_t ll = lane / n, col = lane % n;
+        const uint32_t step_l = 64 / n, step_c = 64 % n;
+        for (uint32_t e = lane; e < total; e += 64) {
+            if (Lw0 + ll < lines) dst[e] = tile[ll * (n + 1) + col];
+            ll += step_l;
+            col += step_c;
+            if (col >= n) { col -= n; ll += 1; }
+        }
+    } else {
+        if (valid) {
+#pragma unroll
+            for (int t = 0; t < MM; t++)
+                if (t < (int)mk) p[(uint64_t)(sk + 1 + t) * stride] = fill ? T : v[t];
+            if (dir) {
+                if (k == 0) p[0] = T;
+                if (k == P - 1) p[(uint64_t)(n - 1) * stride] = T;
+            }
+        }
+    }
+}
+
+int probe_kernels_part(hc_ctx* ctx) {
+    hipFuncAttributes attr;
+    const void* fns[] = {(const void*)k_sweep_part<0, 16>, (const void*)k_sweep_part<0, 32>,
+                         (const void*)k_sweep_part<0, 64>, (const void*)k_sweep_part<1, 16>,
+                         (const void*)k_sweep_part<1, 32>, (const void*)k_sweep_part<1, 64>,
+                         (const void*)k_sweep_part<2, 16>, (const void*)k_sweep_part<2, 32>,
+                         (const void*)k_sweep_part<2, 64>};
+    for (const void* f : fns) {
+        hipError_t e = hipFuncGetAttributes(&attr, f);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: partitioned sweep");
+    }
+    return 0;
+}
+
+template <int AXIS>
+static void launch_part_axis(hc_ctx* ctx, int mm, dim3 grid, dim3 block, size_t shm, double* rho,
+                             const uint8_t* tables, const hc_grid* g, uint32_t nmax, int fuse) {
+    if (mm == 16)
+        hipLaunchKernelGGL((k_sweep_part<AXIS, 16>), grid, block, shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
+    else if (mm == 32)
+        hipLaunchKernelGGL((k_sweep_part<AXIS, 32>), grid, block, shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
+    else
+        hipLaunchKernelGGL((k_sweep_part<AXIS, 64>), grid, block, shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
+}
+
+int launch_sweep_part(hc_ctx* ctx, int axis, double* rho, const hc_grid* g, const uint8_t* tables,
+                      int fuse_dirichlet) {
+    const uint32_t n = axis == 0 ? g->nx : (axis == 1 ? g->ny : g->nz);
+    if (n < 2 + 2 * PART_P) return 1;                       // too short to partition
+    const uint32_t m = (n - 2 + PART_P - 1) / PART_P;        // longest segment
+    const int mm = m <= 16 ? 16 : (m <= 32 ? 32 : (m <= 64 ? 64 : 0));
+    if (!mm) return 1;
+    const uint32_t nmax = grid_nmax(g);
+    const size_t blk = part_block(nmax);
+    const size_t tab_doubles = (blk + 31) / 32 * 32;
+    size_t shm = tab_doubles * sizeof(double);
+    if (axis == 0) shm += (size_t)LPW * (n + 1) * sizeof(double);
+    if (shm > 64 * 1024) return 1;
+    const uint64_t lines = axis == 0 ? (uint64_t)g->ny * g->nz
+                                     : (axis == 1 ? (uint64_t)g->nz * g->nx : (uint64_t)g->ny * g->nx);
+    const int wpb = axis == 0 ? 1 : 4;
+    dim3 grid(blocks_for(lines, LPW * wpb), g->n_substrates);
+    dim3 block(64 * wpb);
+    KernelTimer tm(ctx, HC_K_SWEEP_X + axis);
+    if (axis == 0) launch_part_axis<0>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
+    else if (axis == 1) launch_part_axis<1>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
+    else launch_part_axis<2>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
+    return check_launch(ctx, "k_sweep_part");
+}
+
+// Host construction of the partitioned tables (FP64).
+static void solve_local(const std::vector<double>& den, const std::vector<double>& cpl, double c,
+                        const std::vector<double>& d, std::vector<double>& out) {
+    const size_t m = den.size();
+    out.assign(m, 0.0);
+    double dp = 0.0;
+    for (size_t t = 0; t < m; t++) {
+        dp = (d[t] + c * dp) / den[t];
+        out[t] = dp;
+    }
+    double y = 0.0;
+    for (size_t t = m; t-- > 0;) {
+        y = out[t] - cpl[t] * y;
+        out[t] = y;
+    }
+}
+
+static bool invert(std::vector<double>& A, int n, std::vector<double>& inv) {
+    inv.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) inv[(size_t)i * n + i] = 1.0;
+    for (int col = 0; col < n; col++) {
+        int piv = col;
+        for (int r = col + 1; r < n; r++)
+            if (std::fabs(A[(size_t)r * n + col]) > std::fabs(A[(size_t)piv * n + col])) piv = r;
+        if (A[(size_t)piv * n + col] == 0.0) return false;
+        if (piv != col)
+            for (int j = 0; j < n; j++) {
+                std::swap(A[(size_t)piv * n + j], A[(size_t)col * n + j]);
+                std::swap(inv[(size_t)piv * n + j], inv[(size_t)col * n + j]);
+            }
+        const double d = A[(size_t)col * n + col];
+        for (int j = 0; j < n; j++) { A[(size_t)col * n + j] /= d; inv[(size_t)col * n + j] /= d; }
+        for (int r = 0; r < n; r++) {
+            if (r == col) continue;
+            const double f = A[(size_t)r * n + col];
+            if (f == 0.0) continue;
+            for (int j = 0; j < n; j++) {
+                A[(size_t)r * n + j] -= f * A[(size_t)col * n + j];
+                inv[(size_t)r * n + j] -= f * inv[(size_t)col * n + j];
+            }
+        }
+    }
+    return true;
+}
+
+void build_part_tables_host(const hc_grid* g, const hc_substrate* subs, double* out) {
+    constexpr int P = PART_P;
+    const uint32_t ns = g->n_substrates, nmax = grid_nmax(g);
+    const uint32_t MT = part_mt(nmax);
+    const size_t blk = part_block(nmax);
+    const double sp[3] = {g->dx, g->dy, g->dz};
+    const uint32_t nn[3] = {g->nx, g->ny, g->nz};
+    for (uint32_t s = 0; s < ns; s++) {
+        const double D = subs[s].diffusion_coefficient, lam = subs[s].decay_rate;
+        const double h = g->nz > 1 ? lam * g->dt / 3.0 : 0.5 * lam * g->dt;
+        for (int ax = 0; ax < 3; ax++) {
+            double* B = out + (size_t)(s * 3 + ax) * blk;
+            std::fill(B, B + blk, 0.0);
+            const uint32_t n = nn[ax];
+            if (n < 2 + 2 * P) continue;
+            const uint32_t N = n - 2;
+            const double c = D * g->dt / (sp[ax] * sp[ax]);
+            const double bI = 1.0 + 2.0 * c + h, bB = 1.0 + c + h;
+            std::vector<std::vector<double>> al(P), be(P);
+            for (int k = 0; k < P; k++) {
+                const uint32_t sk = (k * N) / P, mk = ((k + 1) * N) / P - sk;
+                std::vector<double> den(mk), cp(mk), cpl(mk);
+                for (uint32_t t = 0; t < mk; t++) {
+                    const uint32_t r = sk + t;
+                    const double b = (r == 0 || r == N - 1) ? bB : bI;
+                    den[t] = t == 0 ? b : b - (-c) * cp[t - 1];
+                    cp[t] = -c / den[t];
+                }
+                cpl = cp;
+                cpl[mk - 1] = 0.0;
+                std::vector<double> e0(mk, 0.0), eL(mk, 0.0);
+                e0[0] = c;
+                eL[mk - 1] = c;
+                solve_local(den, cpl, c, e0, al[k]);
+                solve_local(den, cpl, c, eL, be[k]);
+                for (uint32_t t = 0; t < mk; t++) {
+                    B[(k * 4 + 0) * MT + t] = 1.0 / den[t];
+                    B[(k * 4 + 1) * MT + t] = cpl[t];
+                    B[(k * 4 + 2) * MT + t] = al[k][t];
+                    B[(k * 4 + 3) * MT + t] = be[k][t];
+                }
+            }
+            // interface system over z = (L0, R0, L1, R1, ...): L_k = u(first row of
+            // segment k), R_k = u(last row); u = y + R_{k-1} alpha_k + L_{k+1} beta_k
+            const int n2 = 2 * P;
+            std::vector<double> M((size_t)n2 * n2, 0.0), Mi;
+            for (int k = 0; k < P; k++) {
+                M[(size_t)(2 * k) * n2 + 2 * k] = 1.0;
+                M[(size_t)(2 * k + 1) * n2 + 2 * k + 1] = 1.0;
+                if (k > 0) {
+                    M[(size_t)(2 * k) * n2 + 2 * k - 1] -= al[k].front();
+                    M[(size_t)(2 * k + 1) * n2 + 2 * k - 1] -= al[k].back();
+                }
+                if (k < P - 1) {
+                    M[(size_t)(2 * k) * n2 + 2 * k + 2] -= be[k].front();
+                    M[(size_t)(2 * k + 1) * n2 + 2 * k + 2] -= be[k].back();
+                }
+            }
+            if (!invert(M, n2, Mi)) continue;   // never singular: |alpha|,|beta| < 1
+            double* G = B + (size_t)P * 4 * MT;
+            for (int k = 0; k < P; k++)
+                for (int j = 0; j < n2; j++) {
+                    G[(k * 2 + 0) * n2 + j] = k > 0 ? Mi[(size_t)(2 * k - 1) * n2 + j] : 0.0;
+                    G[(k * 2 + 1) * n2 + j] = k < P - 1 ? Mi[(size_t)(2 * k + 2) * n2 + j] : 0.0;
+                }
+        }
+    }
+}
+
+}  // namespace hc

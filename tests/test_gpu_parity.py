@@ -59,12 +59,16 @@ GRIDS = [
     (2, 7, 3, (False, True)),      # n == 2 along x, n == 3 along z
     (67, 33, 5, (False, False)),
     (17, 4, 19, (True, False)),
+    (220, 16, 10, (True, False)),   # x: 4 segments of 55 (64-wide register window)
+    (34, 66, 67, (False, True)),    # segment lengths 8, 16 and 17 (window 16 / 32)
+    (259, 12, 9, (True, True)),     # x too long for the partitioned window -> serial
 ]
 
 
+@pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("nx,ny,nz,dirs", GRIDS)
 @pytest.mark.parametrize("axis", [0, 1, 2])
-def test_single_sweep(ctx, nx, ny, nz, dirs, axis):
+def test_single_sweep(ctx, nx, ny, nz, dirs, axis, mode):
     if nz == 1 and axis == 2:
         pytest.skip("no z sweep in 2-D")
     spec, _ = S.small3d(nx, ny, nz, n_cells=1, dirichlet=dirs)
@@ -76,17 +80,22 @@ def test_single_sweep(ctx, nx, ny, nz, dirs, axis):
     rho0 = make_density(spec, 11 + axis)
     d = ctx.alloc(rho0.nbytes)
     d.upload(rho0)
-    print(f"sweep axis={axis} grid={nx}x{ny}x{nz}", flush=True)
+    print(f"sweep axis={axis} grid={nx}x{ny}x{nz} mode={mode}", flush=True)
+    ctx._check(lib.hc_set_sweep_mode(ctx.h, mode), "mode")
     ctx._check(lib.hc_diffusion_sweep(ctx.h, axis, d.h, g, subs, tb.h, 0), "sweep")
     got = d.download(np.float64, rho0.size)
     ref = rho0.copy()
     O.sweep(axis, ref, O.grid(spec), O.subs(spec))
-    assert_close(got, ref, f"sweep{axis} {nx}x{ny}x{nz}")
+    assert_close(got, ref, f"sweep{axis} {nx}x{ny}x{nz} mode {mode}")
+    if mode == 1:
+        assert np.array_equal(got, ref), "serial mode is bit-identical to the oracle"
+    ctx._check(lib.hc_set_sweep_mode(ctx.h, 0), "mode")
     d.free(); tb.free()
 
 
+@pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("nx,ny,nz,dirs", GRIDS)
-def test_substep_with_dirichlet(ctx, nx, ny, nz, dirs):
+def test_substep_with_dirichlet(ctx, nx, ny, nz, dirs, mode):
     spec, _ = S.small3d(nx, ny, nz, n_cells=1, dirichlet=dirs)
     if nz == 1:
         spec.use_2d = True
@@ -96,13 +105,15 @@ def test_substep_with_dirichlet(ctx, nx, ny, nz, dirs):
     rho0 = make_density(spec, 3)
     d = ctx.alloc(rho0.nbytes)
     d.upload(rho0)
-    print(f"substep grid={nx}x{ny}x{nz}", flush=True)
+    print(f"substep grid={nx}x{ny}x{nz} mode={mode}", flush=True)
+    ctx._check(lib.hc_set_sweep_mode(ctx.h, mode), "mode")
     ref = rho0.copy()
     for _ in range(5):
         ctx._check(lib.hc_diffusion_substep(ctx.h, d.h, g, subs, tb.h), "substep")
         O.diffusion_substep(ref, O.grid(spec), O.subs(spec))
     got = d.download(np.float64, rho0.size)
-    assert_close(got, ref, f"5 substeps {nx}x{ny}x{nz}")
+    assert_close(got, ref, f"5 substeps {nx}x{ny}x{nz} mode {mode}")
+    ctx._check(lib.hc_set_sweep_mode(ctx.h, 0), "mode")
     # standalone Dirichlet is idempotent after the fused substep
     ctx._check(lib.hc_dirichlet(ctx.h, d.h, g, subs), "dirichlet")
     got2 = d.download(np.float64, rho0.size)
