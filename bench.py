@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--workload", default="config2")
     ap.add_argument("--cells", type=int, default=1_000_000)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--sweep-mode", type=int, default=0,
+                    help="0 partitioned sweeps (default), 1 serial elimination, 2 partitioned with LDS-staged x")
     ap.add_argument("--cpu-periods", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_config2.json"))
@@ -109,6 +111,10 @@ def main():
     t_setup = time.perf_counter()
     spec, cells = build_workload(args.workload, args.cells)
     sim = H.Simulation(spec, cells)
+    if args.sweep_mode:
+        rc = H.load().hc_set_sweep_mode(sim.context(), args.sweep_mode)
+        if rc:
+            raise SystemExit("hc_set_sweep_mode failed")
     k_sub = 10 if abs(spec.dt_mech / spec.dt_diff - 10.0) < 1e-9 else None
     use_graph = not args.no_graph
     print(f"[bench] {spec.name}: grid {spec.nx}x{spec.ny}x{spec.nz} x {len(spec.substrates)} substrates, "
@@ -184,7 +190,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{spec.name}: {spec.nx}x{spec.ny}x{spec.nz} voxels x {len(spec.substrates)} "
                                f"substrates, {cells.n} agents, plain FP64; step = {n_sub} diffusion substeps "
-                               f"+ 1 mechanics step", "graph": use_graph},
+                               f"+ 1 mechanics step", "graph": use_graph, "sweep_mode": args.sweep_mode},
         "voxel_substeps_per_s": vss,
         "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
         "roofline": roof, "kernels": kern, "cpu_baseline": cpu,

@@ -150,7 +150,9 @@ int hc_dirichlet(hc_ctx* ctx, hc_buf* density, const hc_grid* grid, const hc_sub
  * other lengths use the serial elimination.  1: always the serial
  * elimination, which evaluates the reference's forward/backward recurrence
  * in its order (shaders/diffusion_2d.metal:111-148) and is bit-identical to
- * it in FP64.  The two agree to rounding (well below 1e-12 relative). */
+ * it in FP64.  2: as 0, with x lines staged through LDS (16 lines per wave)
+ * instead of loaded directly.  All agree to rounding (well below 1e-12
+ * relative). */
 int hc_set_sweep_mode(hc_ctx* ctx, int mode);
 /* One substep: x, y, [z] sweeps with Dirichlet fused into the last sweep
  * (src/microenvironment.mm:106-115 + src/main.mm:343-347). */

@@ -301,7 +301,7 @@ int launch_sweep(hc_ctx* ctx, int axis, double* rho, const hc_grid* g, const uin
                  int fuse_dirichlet) {
     const uint32_t nx = g->nx, ny = g->ny, nz = g->nz, ns = g->n_substrates;
     if (ns == 0 || nx == 0 || ny == 0 || nz == 0) return 0;
-    if (ctx->sweep_mode == 0) {
+    if (ctx->sweep_mode != 1) {
         const int r = launch_sweep_part(ctx, axis, rho, g, tables, fuse_dirichlet);
         if (r != 1) return r;
     }
@@ -396,7 +396,7 @@ size_t hc_thomas_table_bytes(const hc_grid* g) {
 }
 
 int hc_set_sweep_mode(hc_ctx* ctx, int mode) {
-    if (!ctx || mode < 0 || mode > 1) return fail(ctx, HC_ERR_ARG, "hc_set_sweep_mode: mode must be 0 or 1");
+    if (!ctx || mode < 0 || mode > 2) return fail(ctx, HC_ERR_ARG, "hc_set_sweep_mode: mode must be 0, 1 or 2");
     ctx->sweep_mode = mode;
     return 0;
 }

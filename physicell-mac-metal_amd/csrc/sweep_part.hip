@@ -28,12 +28,12 @@ namespace hc {
 
 static constexpr int LPW = 64 / PART_P;      // lines per wave
 
-template <int AXIS, int MM>
-__global__ __launch_bounds__(AXIS == 0 ? 64 : 256) void k_sweep_part(
+template <int AXIS, int MM, bool XLDS>
+__global__ __launch_bounds__(XLDS ? 64 : 256) void k_sweep_part(
     double* __restrict__ rho, const uint8_t* __restrict__ tables, uint32_t nx, uint32_t ny, uint32_t nz,
     uint32_t nmax, int fuse_dirichlet) {
     constexpr int P = PART_P;
-    constexpr int WPB = AXIS == 0 ? 1 : 4;
+    constexpr int WPB = XLDS ? 1 : 4;
     extern __shared__ double smem[];
     const uint32_t s = blockIdx.y, ns = gridDim.y;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(AXIS == 0 ? 64 : 256) void k_sweep_part(
     __syncthreads();                                    // tables visible
 
     double v[MM];
-    if (AXIS == 0) {
+    if (XLDS) {
         // the wave's LPW consecutive x lines are LPW*n contiguous values
         const double* src = rho + (uint64_t)s * V + Lw0 * nx;
         const uint32_t total = LPW * n;
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(AXIS == 0 ? 64 : 256) void k_sweep_part(
     for (int t = 0; t < MM; t++)
         if (t < (int)mk) v[t] = v[t] + al[t] * uL + be[t] * uR;
 
-    if (AXIS == 0) {
+    if (XLDS) {
 #pragma unroll
         for (int t = 0; t < MM; t++)
             if (t < (int)mk) tile[l * (n + 1) + sk + 1 + t] = v[t];
@@ -181,11 +181,12 @@ __global__ __launch_bounds__(AXIS == 0 ? 64 : 256) void k_sweep_part(
 
 int probe_kernels_part(hc_ctx* ctx) {
     hipFuncAttributes attr;
-    const void* fns[] = {(const void*)k_sweep_part<0, 16>, (const void*)k_sweep_part<0, 32>,
-                         (const void*)k_sweep_part<0, 64>, (const void*)k_sweep_part<1, 16>,
-                         (const void*)k_sweep_part<1, 32>, (const void*)k_sweep_part<1, 64>,
-                         (const void*)k_sweep_part<2, 16>, (const void*)k_sweep_part<2, 32>,
-                         (const void*)k_sweep_part<2, 64>};
+    const void* fns[] = {(const void*)k_sweep_part<0, 16, false>, (const void*)k_sweep_part<0, 32, false>,
+                         (const void*)k_sweep_part<0, 64, false>, (const void*)k_sweep_part<0, 16, true>,
+                         (const void*)k_sweep_part<0, 32, true>, (const void*)k_sweep_part<0, 64, true>,
+                         (const void*)k_sweep_part<1, 16, false>, (const void*)k_sweep_part<1, 32, false>,
+                         (const void*)k_sweep_part<1, 64, false>, (const void*)k_sweep_part<2, 16, false>,
+                         (const void*)k_sweep_part<2, 32, false>, (const void*)k_sweep_part<2, 64, false>};
     for (const void* f : fns) {
         hipError_t e = hipFuncGetAttributes(&attr, f);
         if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: partitioned sweep");
@@ -193,15 +194,15 @@ int probe_kernels_part(hc_ctx* ctx) {
     return 0;
 }
 
-template <int AXIS>
+template <int AXIS, bool XLDS>
 static void launch_part_axis(hc_ctx* ctx, int mm, dim3 grid, dim3 block, size_t shm, double* rho,
                              const uint8_t* tables, const hc_grid* g, uint32_t nmax, int fuse) {
     if (mm == 16)
-        hipLaunchKernelGGL((k_sweep_part<AXIS, 16>), grid, block, shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
+        hipLaunchKernelGGL((k_sweep_part<AXIS, 16, XLDS>), grid, block, shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
     else if (mm == 32)
-        hipLaunchKernelGGL((k_sweep_part<AXIS, 32>), grid, block, shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
+        hipLaunchKernelGGL((k_sweep_part<AXIS, 32, XLDS>), grid, block, shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
     else
-        hipLaunchKernelGGL((k_sweep_part<AXIS, 64>), grid, block, shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
+        hipLaunchKernelGGL((k_sweep_part<AXIS, 64, XLDS>), grid, block, shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
 }
 
 int launch_sweep_part(hc_ctx* ctx, int axis, double* rho, const hc_grid* g, const uint8_t* tables,
@@ -214,18 +215,20 @@ int launch_sweep_part(hc_ctx* ctx, int axis, double* rho, const hc_grid* g, cons
     const uint32_t nmax = grid_nmax(g);
     const size_t blk = part_block(nmax);
     const size_t tab_doubles = (blk + 31) / 32 * 32;
+    const bool xlds = axis == 0 && ctx->sweep_mode == 2;
     size_t shm = tab_doubles * sizeof(double);
-    if (axis == 0) shm += (size_t)LPW * (n + 1) * sizeof(double);
+    if (xlds) shm += (size_t)LPW * (n + 1) * sizeof(double);
     if (shm > 64 * 1024) return 1;
     const uint64_t lines = axis == 0 ? (uint64_t)g->ny * g->nz
                                      : (axis == 1 ? (uint64_t)g->nz * g->nx : (uint64_t)g->ny * g->nx);
-    const int wpb = axis == 0 ? 1 : 4;
+    const int wpb = xlds ? 1 : 4;
     dim3 grid(blocks_for(lines, LPW * wpb), g->n_substrates);
     dim3 block(64 * wpb);
     KernelTimer tm(ctx, HC_K_SWEEP_X + axis);
-    if (axis == 0) launch_part_axis<0>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
-    else if (axis == 1) launch_part_axis<1>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
-    else launch_part_axis<2>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
+    if (xlds) launch_part_axis<0, true>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
+    else if (axis == 0) launch_part_axis<0, false>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
+    else if (axis == 1) launch_part_axis<1, false>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
+    else launch_part_axis<2, false>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
     return check_launch(ctx, "k_sweep_part");
 }
 

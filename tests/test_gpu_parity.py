@@ -65,7 +65,7 @@ GRIDS = [
 ]
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 @pytest.mark.parametrize("nx,ny,nz,dirs", GRIDS)
 @pytest.mark.parametrize("axis", [0, 1, 2])
 def test_single_sweep(ctx, nx, ny, nz, dirs, axis, mode):
