@@ -252,12 +252,22 @@ __global__ __launch_bounds__(256) void k_stable_gather(CellPtrs cp, const uint32
 // ---------------------------------------------------------------------------
 // forces (shaders/mechanics.metal:91-216)
 // ---------------------------------------------------------------------------
+// Workgroups are handed to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// Renumber them so that each XCD works through one contiguous range of the
+// hash-sorted agents: neighbour bins then stay in that XCD's L2 instead of
+// every L2 fetching every plane.
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    constexpr uint32_t NX = 8;
+    const uint32_t x = b % NX, idx = b / NX, per = nb / NX, rem = nb % NX;
+    return x * per + (x < rem ? x : rem) + idx;
+}
+
 __global__ __launch_bounds__(256) void k_forces(const CellRec* __restrict__ rec, const uint32_t* __restrict__ offsets,
                                                 double* __restrict__ vx, double* __restrict__ vy,
                                                 double* __restrict__ vz, double* __restrict__ pressure,
                                                 uint32_t n, uint32_t mnx, uint32_t mny, uint32_t mnz,
                                                 double max_dist) {
-    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (t >= n) return;
     const CellRec me = rec[t];
     const double px = me.x, py = me.y, pz = me.z;

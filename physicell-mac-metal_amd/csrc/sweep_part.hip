@@ -28,6 +28,56 @@ namespace hc {
 
 static constexpr int LPW = 64 / PART_P;      // lines per wave
 
+// Contiguous copy of nval doubles between global memory and an LDS tile by
+// one wave.  The source may be only 8-byte aligned: a leading odd element is
+// copied alone so the rest moves as 16-byte pairs.  The loads of a lane (at
+// most MM/2 + 1 pairs, held in the caller's register window) are all issued
+// before the LDS writes.
+template <int MM>
+__device__ inline void stage_in(const double* __restrict__ src, double* tile, uint32_t nval, uint32_t lane,
+                                double* reg) {
+    const uint32_t head = ((uintptr_t)src & 15u) ? 1u : 0u;
+    if (head && lane == 0 && nval) tile[0] = src[0];
+    const uint32_t npair = (nval - (nval ? head : 0)) / 2;
+    const double2* s2 = reinterpret_cast<const double2*>(src + head);
+    constexpr int R = MM / 2 + 1;
+    double2 buf[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t e = lane + 64u * r;
+        if (e < npair) buf[r] = s2[e];
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t e = lane + 64u * r;
+        if (e < npair) {
+            tile[head + 2 * e] = buf[r].x;
+            tile[head + 2 * e + 1] = buf[r].y;
+        }
+    }
+    const uint32_t done = head + 2 * npair;
+    if (done < nval && lane == 0) tile[done] = src[done];
+    (void)reg;
+}
+
+template <int MM>
+__device__ inline void stage_out(double* __restrict__ dst, const double* tile, uint32_t nval, uint32_t lane,
+                                 double* reg) {
+    const uint32_t head = ((uintptr_t)dst & 15u) ? 1u : 0u;
+    if (head && lane == 0 && nval) dst[0] = tile[0];
+    const uint32_t npair = (nval - (nval ? head : 0)) / 2;
+    double2* d2 = reinterpret_cast<double2*>(dst + head);
+    constexpr int R = MM / 2 + 1;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t e = lane + 64u * r;
+        if (e < npair) d2[e] = make_double2(tile[head + 2 * e], tile[head + 2 * e + 1]);
+    }
+    const uint32_t done = head + 2 * npair;
+    if (done < nval && lane == 0) dst[done] = tile[done];
+    (void)reg;
+}
+
 template <int AXIS, int MM, bool XLDS>
 __global__ __launch_bounds__(XLDS ? 64 : 256) void k_sweep_part(
     double* __restrict__ rho, const uint8_t* __restrict__ tables, uint32_t nx, uint32_t ny, uint32_t nz,
@@ -76,25 +126,19 @@ __global__ __launch_bounds__(XLDS ? 64 : 256) void k_sweep_part(
     const uint32_t mk = ((k + 1) * N) / P - sk;
     double* p = rho + (uint64_t)s * V + base;
 
-    double* tile = smem + ((blk + 31) / 32) * 32 + (size_t)w * LPW * (n + 1);
+    double* tile = smem + ((blk + 31) / 32) * 32 + (size_t)w * LPW * n;
     __syncthreads();                                    // tables visible
 
     double v[MM];
     if (XLDS) {
-        // the wave's LPW consecutive x lines are LPW*n contiguous values
-        const double* src = rho + (uint64_t)s * V + Lw0 * nx;
-        const uint32_t total = LPW * n;
-        uint32_t ll = lane / n, col = lane % n;
-        const uint32_t step_l = 64 / n, step_c = 64 % n;
-        for (uint32_t e = lane; e < total; e += 64) {
-            if (Lw0 + ll < lines) tile[ll * (n + 1) + col] = src[e];
-            ll += step_l;
-            col += step_c;
-            if (col >= n) { col -= n; ll += 1; }
-        }
+        // The wave's LPW consecutive x lines are LPW*n contiguous values: copy
+        // them with 16-byte loads, all issued before the first LDS write (the
+        // register window v[] is free at this point), then read the segments.
+        const uint32_t nval = (uint32_t)((lines - Lw0 < (uint64_t)LPW ? lines - Lw0 : (uint64_t)LPW) * n);
+        stage_in<MM>(rho + (uint64_t)s * V + Lw0 * nx, tile, nval, lane, v);
         __syncthreads();
 #pragma unroll
-        for (int t = 0; t < MM; t++) v[t] = (t < (int)mk) ? tile[l * (n + 1) + sk + 1 + t] : 0.0;
+        for (int t = 0; t < MM; t++) v[t] = (t < (int)mk) ? tile[l * n + sk + 1 + t] : 0.0;
     } else {
 #pragma unroll
         for (int t = 0; t < MM; t++) v[t] = (t < (int)mk && valid) ? p[(uint64_t)(sk + 1 + t) * stride] : 0.0;
@@ -150,22 +194,14 @@ __global__ __launch_bounds__(XLDS ? 64 : 256) void k_sweep_part(
     if (XLDS) {
 #pragma unroll
         for (int t = 0; t < MM; t++)
-            if (t < (int)mk) tile[l * (n + 1) + sk + 1 + t] = v[t];
+            if (t < (int)mk) tile[l * n + sk + 1 + t] = v[t];
         if (dir) {
-            if (k == 0) tile[l * (n + 1)] = T;
-            if (k == P - 1) tile[l * (n + 1) + n - 1] = T;
+            if (k == 0) tile[l * n] = T;
+            if (k == P - 1) tile[l * n + n - 1] = T;
         }
         __syncthreads();
-        double* dst = rho + (uint64_t)s * V + Lw0 * nx;
-        const uint32_t total = LPW * n;
-        uint32_t ll = lane / n, col = lane % n;
-        const uint32_t step_l = 64 / n, step_c = 64 % n;
-        for (uint32_t e = lane; e < total; e += 64) {
-            if (Lw0 + ll < lines) dst[e] = tile[ll * (n + 1) + col];
-            ll += step_l;
-            col += step_c;
-            if (col >= n) { col -= n; ll += 1; }
-        }
+        const uint32_t nval = (uint32_t)((lines - Lw0 < (uint64_t)LPW ? lines - Lw0 : (uint64_t)LPW) * n);
+        stage_out<MM>(rho + (uint64_t)s * V + Lw0 * nx, tile, nval, lane, v);
     } else {
         if (valid) {
 #pragma unroll
@@ -217,7 +253,7 @@ int launch_sweep_part(hc_ctx* ctx, int axis, double* rho, const hc_grid* g, cons
     const size_t tab_doubles = (blk + 31) / 32 * 32;
     const bool xlds = axis == 0 && ctx->sweep_mode == 2;
     size_t shm = tab_doubles * sizeof(double);
-    if (xlds) shm += (size_t)LPW * (n + 1) * sizeof(double);
+    if (xlds) shm += (size_t)LPW * n * sizeof(double);
     if (shm > 64 * 1024) return 1;
     const uint64_t lines = axis == 0 ? (uint64_t)g->ny * g->nz
                                      : (axis == 1 ? (uint64_t)g->nz * g->nx : (uint64_t)g->ny * g->nx);
