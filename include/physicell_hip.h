@@ -144,15 +144,15 @@ int hc_diffusion_sweep(hc_ctx* ctx, int axis, hc_buf* density, const hc_grid* gr
 /* Microenvironment::applyDirichletBC (src/microenvironment.mm:127-152), all
  * faces (z faces included when nz > 1). */
 int hc_dirichlet(hc_ctx* ctx, hc_buf* density, const hc_grid* grid, const hc_substrate* subs);
-/* Sweep algorithm.  0 (default): lines of 10..258 nodes are solved by the
- * partitioned elimination (4 segments per line held in registers, joined by
- * a precomputed 8x8 interface solve; every value read and written once);
- * other lengths use the serial elimination.  1: always the serial
- * elimination, which evaluates the reference's forward/backward recurrence
- * in its order (shaders/diffusion_2d.metal:111-148) and is bit-identical to
- * it in FP64.  2: as 0, with x lines staged through LDS (16 lines per wave)
- * instead of loaded directly.  All agree to rounding (well below 1e-12
- * relative). */
+/* Sweep algorithm.  0 (default): lines long enough are solved by the
+ * partitioned elimination (segments held in registers -- 16 per x line,
+ * 4 per y/z line -- joined by a precomputed interface solve; every value is
+ * read and written once; x lines are copied through LDS); other lengths use
+ * the serial elimination.  1: always the serial elimination, which evaluates
+ * the reference's forward/backward recurrence in its order
+ * (shaders/diffusion_2d.metal:111-148) and is bit-identical to it in FP64.
+ * 2: as 0 with x segments loaded directly instead of through LDS.  All agree
+ * to rounding (well below 1e-12 relative). */
 int hc_set_sweep_mode(hc_ctx* ctx, int mode);
 /* One substep: x, y, [z] sweeps with Dirichlet fused into the last sweep
  * (src/microenvironment.mm:106-115 + src/main.mm:343-347). */

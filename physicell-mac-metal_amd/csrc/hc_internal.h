@@ -79,17 +79,21 @@ inline uint32_t grid_nmax(const hc_grid* g) {
 }
 
 // ---- partitioned sweep tables (see sweep_part.hip) ----
-// A line's N = n-2 unknowns are split into PART_P contiguous segments, one per
-// lane.  Per (substrate, axis) block: for each segment k the arrays
-// 1/den, c', alpha, beta of length part_mt(nmax), then the two rows of the
-// inverse reduced interface matrix (left and right neighbour values) of
-// length 2*PART_P.
-static constexpr int PART_P = 4;
-__host__ __device__ inline uint32_t part_mt(uint32_t nmax) {
-    return nmax > 2 ? (nmax - 2 + PART_P - 1) / PART_P : 1u;
+// A line's N = n-2 unknowns are split into P contiguous segments, one per
+// lane (P = 16 along x, 4 along y and z).  Per (substrate, axis) block: for
+// each segment k the arrays 1/den, c', alpha, beta of length
+// part_mt(nmax, P), then the two rows (left / right neighbour value) of the
+// inverse interface matrix, each of length 2P.  Blocks have a common stride.
+__host__ __device__ inline int part_p(int axis) { return axis == 0 ? 16 : 4; }
+__host__ __device__ inline uint32_t part_mt(uint32_t nmax, int P) {
+    return nmax > 2 ? (nmax - 2 + P - 1) / P : 1u;
+}
+__host__ __device__ inline size_t part_block_p(uint32_t nmax, int P) {
+    return (size_t)P * 4 * part_mt(nmax, P) + (size_t)P * 2 * 2 * P;
 }
 __host__ __device__ inline size_t part_block(uint32_t nmax) {
-    return (size_t)PART_P * 4 * part_mt(nmax) + (size_t)PART_P * 2 * 2 * PART_P;
+    const size_t a = part_block_p(nmax, 4), b = part_block_p(nmax, 16);
+    return ((a > b ? a : b) + 31) / 32 * 32;
 }
 __host__ __device__ inline size_t serial_table_bytes(uint32_t ns, uint32_t nmax) {
     return ((subconst_bytes(ns) + (size_t)ns * 3 * 2 * nmax * sizeof(double)) + 255) / 256 * 256;

@@ -17,16 +17,15 @@
 //     interface system, also a table.
 //
 // Every value is read once and written once (16 bytes per voxel per sweep in
-// FP64).  16 lines x 4 segments fill a wave; for y and z lanes of one segment
-// walk 16 adjacent x positions, so each load is a full 128-byte row segment.
-// x lines are contiguous: a wave stages its 16 lines through LDS.
+// FP64).  For y and z, 16 lines x 4 segments fill a wave and the lanes of one
+// segment walk 16 adjacent x positions, so each load is a full 128-byte row
+// segment.  x lines are contiguous: 4 lines x 16 segments fill a wave, which
+// copies its 4 lines (4*n contiguous values) through LDS with 16-byte loads.
 #include "hc_internal.h"
 
 #include <cmath>
 
 namespace hc {
-
-static constexpr int LPW = 64 / PART_P;      // lines per wave
 
 // Contiguous copy of nval doubles between global memory and an LDS tile by
 // one wave.  The source may be only 8-byte aligned: a leading odd element is
@@ -35,34 +34,35 @@ static constexpr int LPW = 64 / PART_P;      // lines per wave
 // before the LDS writes.
 template <int MM>
 __device__ inline void stage_in(const double* __restrict__ src, double* tile, uint32_t nval, uint32_t lane,
-                                double* reg) {
+                                double (&reg)[MM + 2]) {
     const uint32_t head = ((uintptr_t)src & 15u) ? 1u : 0u;
     if (head && lane == 0 && nval) tile[0] = src[0];
     const uint32_t npair = (nval - (nval ? head : 0)) / 2;
     const double2* s2 = reinterpret_cast<const double2*>(src + head);
     constexpr int R = MM / 2 + 1;
-    double2 buf[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const uint32_t e = lane + 64u * r;
-        if (e < npair) buf[r] = s2[e];
+        if (e < npair) {
+            const double2 q = s2[e];
+            reg[2 * r] = q.x;
+            reg[2 * r + 1] = q.y;
+        }
     }
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const uint32_t e = lane + 64u * r;
         if (e < npair) {
-            tile[head + 2 * e] = buf[r].x;
-            tile[head + 2 * e + 1] = buf[r].y;
+            tile[head + 2 * e] = reg[2 * r];
+            tile[head + 2 * e + 1] = reg[2 * r + 1];
         }
     }
     const uint32_t done = head + 2 * npair;
     if (done < nval && lane == 0) tile[done] = src[done];
-    (void)reg;
 }
 
 template <int MM>
-__device__ inline void stage_out(double* __restrict__ dst, const double* tile, uint32_t nval, uint32_t lane,
-                                 double* reg) {
+__device__ inline void stage_out(double* __restrict__ dst, const double* tile, uint32_t nval, uint32_t lane) {
     const uint32_t head = ((uintptr_t)dst & 15u) ? 1u : 0u;
     if (head && lane == 0 && nval) dst[0] = tile[0];
     const uint32_t npair = (nval - (nval ? head : 0)) / 2;
@@ -75,29 +75,29 @@ __device__ inline void stage_out(double* __restrict__ dst, const double* tile, u
     }
     const uint32_t done = head + 2 * npair;
     if (done < nval && lane == 0) dst[done] = tile[done];
-    (void)reg;
 }
 
-template <int AXIS, int MM, bool XLDS>
-__global__ __launch_bounds__(XLDS ? 64 : 256) void k_sweep_part(
+template <int AXIS, int P, int MM, bool XLDS>
+__global__ __launch_bounds__(256) void k_sweep_part(
     double* __restrict__ rho, const uint8_t* __restrict__ tables, uint32_t nx, uint32_t ny, uint32_t nz,
     uint32_t nmax, int fuse_dirichlet) {
-    constexpr int P = PART_P;
-    constexpr int WPB = XLDS ? 1 : 4;
+    constexpr int LPW = 64 / P;                          // lines per wave
+    constexpr int WPB = 4;
     extern __shared__ double smem[];
     const uint32_t s = blockIdx.y, ns = gridDim.y;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t k = lane / LPW, l = lane % LPW;
     const uint32_t n = AXIS == 0 ? nx : (AXIS == 1 ? ny : nz);
     const uint32_t N = n - 2;
-    const uint32_t MT = part_mt(nmax);
+    const uint32_t MT = part_mt(nmax, P);
     const uint32_t blk = (uint32_t)part_block(nmax);
     const uint64_t V = (uint64_t)nx * ny * nz;
 
     // this (substrate, axis) table block -> LDS
     const double* gt = reinterpret_cast<const double*>(tables + part_offset(ns, nmax)) + (size_t)(s * 3 + AXIS) * blk;
-    double* tab = smem;
-    for (uint32_t q = threadIdx.x; q < blk; q += blockDim.x) tab[q] = gt[q];
+    const uint32_t used = (uint32_t)part_block_p(nmax, P);
+    for (uint32_t q = threadIdx.x; q < used; q += blockDim.x) smem[q] = gt[q];
+    const double* tab = smem;
 
     const uint64_t Lw0 = ((uint64_t)blockIdx.x * WPB + w) * LPW;     // first line of this wave
     const uint64_t L = Lw0 + l;
@@ -126,15 +126,16 @@ __global__ __launch_bounds__(XLDS ? 64 : 256) void k_sweep_part(
     const uint32_t mk = ((k + 1) * N) / P - sk;
     double* p = rho + (uint64_t)s * V + base;
 
-    double* tile = smem + ((blk + 31) / 32) * 32 + (size_t)w * LPW * n;
+    double* tile = smem + blk + (size_t)w * LPW * n;
     __syncthreads();                                    // tables visible
 
-    double v[MM];
+    double v[MM + 2];
     if (XLDS) {
         // The wave's LPW consecutive x lines are LPW*n contiguous values: copy
         // them with 16-byte loads, all issued before the first LDS write (the
         // register window v[] is free at this point), then read the segments.
-        const uint32_t nval = (uint32_t)((lines - Lw0 < (uint64_t)LPW ? lines - Lw0 : (uint64_t)LPW) * n);
+        const uint64_t rem = lines > Lw0 ? lines - Lw0 : 0;
+        const uint32_t nval = (uint32_t)((rem < (uint64_t)LPW ? rem : (uint64_t)LPW) * n);
         stage_in<MM>(rho + (uint64_t)s * V + Lw0 * nx, tile, nval, lane, v);
         __syncthreads();
 #pragma unroll
@@ -178,7 +179,7 @@ __global__ __launch_bounds__(XLDS ? 64 : 256) void k_sweep_part(
 #pragma unroll
     for (int t = 0; t < MM; t++)
         if (t == (int)mk - 1) yE = v[t];
-    // interface values of the line (its 4 lanes are l, l+16, l+32, l+48)
+    // interface values of the line (its P lanes are l, l+LPW, l+2*LPW, ...)
     double uL = 0.0, uR = 0.0;
 #pragma unroll
     for (int j = 0; j < P; j++) {
@@ -200,8 +201,9 @@ __global__ __launch_bounds__(XLDS ? 64 : 256) void k_sweep_part(
             if (k == P - 1) tile[l * n + n - 1] = T;
         }
         __syncthreads();
-        const uint32_t nval = (uint32_t)((lines - Lw0 < (uint64_t)LPW ? lines - Lw0 : (uint64_t)LPW) * n);
-        stage_out<MM>(rho + (uint64_t)s * V + Lw0 * nx, tile, nval, lane, v);
+        const uint64_t rem = lines > Lw0 ? lines - Lw0 : 0;
+        const uint32_t nval = (uint32_t)((rem < (uint64_t)LPW ? rem : (uint64_t)LPW) * n);
+        stage_out<MM>(rho + (uint64_t)s * V + Lw0 * nx, tile, nval, lane);
     } else {
         if (valid) {
 #pragma unroll
@@ -217,12 +219,11 @@ __global__ __launch_bounds__(XLDS ? 64 : 256) void k_sweep_part(
 
 int probe_kernels_part(hc_ctx* ctx) {
     hipFuncAttributes attr;
-    const void* fns[] = {(const void*)k_sweep_part<0, 16, false>, (const void*)k_sweep_part<0, 32, false>,
-                         (const void*)k_sweep_part<0, 64, false>, (const void*)k_sweep_part<0, 16, true>,
-                         (const void*)k_sweep_part<0, 32, true>, (const void*)k_sweep_part<0, 64, true>,
-                         (const void*)k_sweep_part<1, 16, false>, (const void*)k_sweep_part<1, 32, false>,
-                         (const void*)k_sweep_part<1, 64, false>, (const void*)k_sweep_part<2, 16, false>,
-                         (const void*)k_sweep_part<2, 32, false>, (const void*)k_sweep_part<2, 64, false>};
+    const void* fns[] = {(const void*)k_sweep_part<0, 16, 16, true>, (const void*)k_sweep_part<0, 16, 32, true>,
+                         (const void*)k_sweep_part<0, 16, 16, false>, (const void*)k_sweep_part<0, 16, 32, false>,
+                         (const void*)k_sweep_part<1, 4, 16, false>, (const void*)k_sweep_part<1, 4, 32, false>,
+                         (const void*)k_sweep_part<1, 4, 64, false>, (const void*)k_sweep_part<2, 4, 16, false>,
+                         (const void*)k_sweep_part<2, 4, 32, false>, (const void*)k_sweep_part<2, 4, 64, false>};
     for (const void* f : fns) {
         hipError_t e = hipFuncGetAttributes(&attr, f);
         if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: partitioned sweep");
@@ -230,41 +231,43 @@ int probe_kernels_part(hc_ctx* ctx) {
     return 0;
 }
 
-template <int AXIS, bool XLDS>
+template <int AXIS, int P, bool XLDS>
 static void launch_part_axis(hc_ctx* ctx, int mm, dim3 grid, dim3 block, size_t shm, double* rho,
                              const uint8_t* tables, const hc_grid* g, uint32_t nmax, int fuse) {
     if (mm == 16)
-        hipLaunchKernelGGL((k_sweep_part<AXIS, 16, XLDS>), grid, block, shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
+        hipLaunchKernelGGL((k_sweep_part<AXIS, P, 16, XLDS>), grid, block, shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
     else if (mm == 32)
-        hipLaunchKernelGGL((k_sweep_part<AXIS, 32, XLDS>), grid, block, shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
-    else
-        hipLaunchKernelGGL((k_sweep_part<AXIS, 64, XLDS>), grid, block, shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
+        hipLaunchKernelGGL((k_sweep_part<AXIS, P, 32, XLDS>), grid, block, shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
+    else if (P == 4)
+        hipLaunchKernelGGL((k_sweep_part<AXIS, P, 64, XLDS>), grid, block, shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
 }
 
 int launch_sweep_part(hc_ctx* ctx, int axis, double* rho, const hc_grid* g, const uint8_t* tables,
                       int fuse_dirichlet) {
+    const int P = part_p(axis);
     const uint32_t n = axis == 0 ? g->nx : (axis == 1 ? g->ny : g->nz);
-    if (n < 2 + 2 * PART_P) return 1;                       // too short to partition
-    const uint32_t m = (n - 2 + PART_P - 1) / PART_P;        // longest segment
-    const int mm = m <= 16 ? 16 : (m <= 32 ? 32 : (m <= 64 ? 64 : 0));
+    if (n < 2 + 2 * (uint32_t)P) return 1;                  // too short to partition
+    const uint32_t m = (n - 2 + P - 1) / P;                  // longest segment
+    const int mm = m <= 16 ? 16 : (m <= 32 ? 32 : (m <= 64 && P == 4 ? 64 : 0));
     if (!mm) return 1;
     const uint32_t nmax = grid_nmax(g);
-    const size_t blk = part_block(nmax);
-    const size_t tab_doubles = (blk + 31) / 32 * 32;
-    const bool xlds = axis == 0 && ctx->sweep_mode == 2;
-    size_t shm = tab_doubles * sizeof(double);
-    if (xlds) shm += (size_t)LPW * n * sizeof(double);
+    const int LPW = 64 / P;
+    const bool xlds = axis == 0 && ctx->sweep_mode != 2;
+    const size_t shm = (part_block(nmax) + (xlds ? (size_t)4 * LPW * n : 0)) * sizeof(double);
     if (shm > 64 * 1024) return 1;
     const uint64_t lines = axis == 0 ? (uint64_t)g->ny * g->nz
                                      : (axis == 1 ? (uint64_t)g->nz * g->nx : (uint64_t)g->ny * g->nx);
-    const int wpb = xlds ? 1 : 4;
-    dim3 grid(blocks_for(lines, LPW * wpb), g->n_substrates);
-    dim3 block(64 * wpb);
+    dim3 grid(blocks_for(lines, LPW * 4), g->n_substrates);
+    dim3 block(256);
     KernelTimer tm(ctx, HC_K_SWEEP_X + axis);
-    if (xlds) launch_part_axis<0, true>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
-    else if (axis == 0) launch_part_axis<0, false>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
-    else if (axis == 1) launch_part_axis<1, false>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
-    else launch_part_axis<2, false>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
+    if (axis == 0) {
+        if (xlds) launch_part_axis<0, 16, true>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
+        else launch_part_axis<0, 16, false>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
+    } else if (axis == 1) {
+        launch_part_axis<1, 4, false>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
+    } else {
+        launch_part_axis<2, 4, false>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
+    }
     return check_launch(ctx, "k_sweep_part");
 }
 
@@ -314,9 +317,7 @@ static bool invert(std::vector<double>& A, int n, std::vector<double>& inv) {
 }
 
 void build_part_tables_host(const hc_grid* g, const hc_substrate* subs, double* out) {
-    constexpr int P = PART_P;
     const uint32_t ns = g->n_substrates, nmax = grid_nmax(g);
-    const uint32_t MT = part_mt(nmax);
     const size_t blk = part_block(nmax);
     const double sp[3] = {g->dx, g->dy, g->dz};
     const uint32_t nn[3] = {g->nx, g->ny, g->nz};
@@ -324,6 +325,8 @@ void build_part_tables_host(const hc_grid* g, const hc_substrate* subs, double* 
         const double D = subs[s].diffusion_coefficient, lam = subs[s].decay_rate;
         const double h = g->nz > 1 ? lam * g->dt / 3.0 : 0.5 * lam * g->dt;
         for (int ax = 0; ax < 3; ax++) {
+            const int P = part_p(ax);
+            const uint32_t MT = part_mt(nmax, P);
             double* B = out + (size_t)(s * 3 + ax) * blk;
             std::fill(B, B + blk, 0.0);
             const uint32_t n = nn[ax];

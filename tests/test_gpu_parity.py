@@ -61,7 +61,8 @@ GRIDS = [
     (17, 4, 19, (True, False)),
     (220, 16, 10, (True, False)),   # x: 4 segments of 55 (64-wide register window)
     (34, 66, 67, (False, True)),    # segment lengths 8, 16 and 17 (window 16 / 32)
-    (259, 12, 9, (True, True)),     # x too long for the partitioned window -> serial
+    (259, 12, 9, (True, True)),     # x: 16 segments of 17 (window 32), y/z short
+    (530, 3, 12, (True, False)),    # x too long for the partitioned window -> serial
 ]
 
 
