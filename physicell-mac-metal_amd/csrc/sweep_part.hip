@@ -23,6 +23,7 @@
 // copies its 4 lines (4*n contiguous values) through LDS with 16-byte loads.
 #include "hc_internal.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace hc {
@@ -99,7 +100,12 @@ __global__ __launch_bounds__(256) void k_sweep_part(
     for (uint32_t q = threadIdx.x; q < used; q += blockDim.x) smem[q] = gt[q];
     const double* tab = smem;
 
-    const uint64_t Lw0 = ((uint64_t)blockIdx.x * WPB + w) * LPW;     // first line of this wave
+    const uint64_t lines_total = AXIS == 0 ? (uint64_t)ny * nz : (AXIS == 1 ? (uint64_t)nz * nx : (uint64_t)ny * nx);
+    const uint64_t ngroups = (lines_total + (uint64_t)WPB * LPW - 1) / ((uint64_t)WPB * LPW);
+    __syncthreads();                                    // tables visible
+    // the block walks line groups so its table copy is reused
+    for (uint64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const uint64_t Lw0 = (grp * WPB + w) * LPW;     // first line of this wave
     const uint64_t L = Lw0 + l;
     uint64_t stride, base, lines;
     bool face = false;
@@ -127,7 +133,6 @@ __global__ __launch_bounds__(256) void k_sweep_part(
     double* p = rho + (uint64_t)s * V + base;
 
     double* tile = smem + blk + (size_t)w * LPW * n;
-    __syncthreads();                                    // tables visible
 
     double v[MM + 2];
     if (XLDS) {
@@ -215,6 +220,8 @@ __global__ __launch_bounds__(256) void k_sweep_part(
             }
         }
     }
+    if (XLDS) __syncthreads();                          // tile reused by the next group
+    }
 }
 
 int probe_kernels_part(hc_ctx* ctx) {
@@ -257,7 +264,10 @@ int launch_sweep_part(hc_ctx* ctx, int axis, double* rho, const hc_grid* g, cons
     if (shm > 64 * 1024) return 1;
     const uint64_t lines = axis == 0 ? (uint64_t)g->ny * g->nz
                                      : (axis == 1 ? (uint64_t)g->nz * g->nx : (uint64_t)g->ny * g->nx);
-    dim3 grid(blocks_for(lines, LPW * 4), g->n_substrates);
+    // enough blocks to fill the chip a few times over; each loops over line groups
+    const unsigned groups = blocks_for(lines, LPW * 4);
+    const unsigned per_sub = (unsigned)std::max<uint64_t>(1, 4096 / std::max<uint32_t>(1, g->n_substrates));
+    dim3 grid(std::min(groups, per_sub), g->n_substrates);
     dim3 block(256);
     KernelTimer tm(ctx, HC_K_SWEEP_X + axis);
     if (axis == 0) {
