@@ -103,8 +103,9 @@ __global__ __launch_bounds__(256) void k_sweep_part(
     const uint64_t lines_total = AXIS == 0 ? (uint64_t)ny * nz : (AXIS == 1 ? (uint64_t)nz * nx : (uint64_t)ny * nx);
     const uint64_t ngroups = (lines_total + (uint64_t)WPB * LPW - 1) / ((uint64_t)WPB * LPW);
     __syncthreads();                                    // tables visible
-    // the block walks line groups so its table copy is reused
-    for (uint64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    // x blocks walk several line groups so their table copy is reused; y/z
+    // blocks take one group each
+    auto group = [&](uint64_t grp, const double* tabg) {
     const uint64_t Lw0 = (grp * WPB + w) * LPW;     // first line of this wave
     const uint64_t L = Lw0 + l;
     uint64_t stride, base, lines;
@@ -150,12 +151,12 @@ __global__ __launch_bounds__(256) void k_sweep_part(
         for (int t = 0; t < MM; t++) v[t] = (t < (int)mk && valid) ? p[(uint64_t)(sk + 1 + t) * stride] : 0.0;
     }
 
-    const double* idn = tab + (k * 4 + 0) * MT;
-    const double* cpl = tab + (k * 4 + 1) * MT;
-    const double* al = tab + (k * 4 + 2) * MT;
-    const double* be = tab + (k * 4 + 3) * MT;
-    const double* gl = tab + (size_t)P * 4 * MT + (k * 2 + 0) * 2 * P;
-    const double* gr = tab + (size_t)P * 4 * MT + (k * 2 + 1) * 2 * P;
+    const double* idn = tabg + (k * 4 + 0) * MT;
+    const double* cpl = tabg + (k * 4 + 1) * MT;
+    const double* al = tabg + (k * 4 + 2) * MT;
+    const double* be = tabg + (k * 4 + 3) * MT;
+    const double* gl = tabg + (size_t)P * 4 * MT + (k * 2 + 0) * 2 * P;
+    const double* gr = tabg + (size_t)P * 4 * MT + (k * 2 + 1) * 2 * P;
 
     if (dir) {
         const double ct = c * T;
@@ -221,6 +222,17 @@ __global__ __launch_bounds__(256) void k_sweep_part(
         }
     }
     if (XLDS) __syncthreads();                          // tile reused by the next group
+    };
+    if (XLDS) {
+        for (uint64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+            // keep the table reads inside the loop (hoisting them all into
+            // registers would cost more registers than the window has)
+            uint32_t toff = 0;
+            asm volatile("" : "+v"(toff));
+            group(grp, tab + toff);
+        }
+    } else {
+        group(blockIdx.x, tab);
     }
 }
 
@@ -264,10 +276,11 @@ int launch_sweep_part(hc_ctx* ctx, int axis, double* rho, const hc_grid* g, cons
     if (shm > 64 * 1024) return 1;
     const uint64_t lines = axis == 0 ? (uint64_t)g->ny * g->nz
                                      : (axis == 1 ? (uint64_t)g->nz * g->nx : (uint64_t)g->ny * g->nx);
-    // enough blocks to fill the chip a few times over; each loops over line groups
+    // x: enough blocks to fill the chip a few times over, each looping over
+    // line groups; y/z: one block per group
     const unsigned groups = blocks_for(lines, LPW * 4);
     const unsigned per_sub = (unsigned)std::max<uint64_t>(1, 4096 / std::max<uint32_t>(1, g->n_substrates));
-    dim3 grid(std::min(groups, per_sub), g->n_substrates);
+    dim3 grid(xlds ? std::min(groups, per_sub) : groups, g->n_substrates);
     dim3 block(256);
     KernelTimer tm(ctx, HC_K_SWEEP_X + axis);
     if (axis == 0) {
