@@ -78,6 +78,14 @@ __device__ inline void stage_out(double* __restrict__ dst, const double* tile, u
     if (done < nval && lane == 0) dst[done] = tile[done];
 }
 
+// Each wave owns its LDS tile: lanes of one wave exchange data through it, so
+// ordering the wave's own LDS accesses is enough (no workgroup barrier).
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int AXIS, int P, int MM, bool XLDS>
 __global__ __launch_bounds__(256) void k_sweep_part(
     double* __restrict__ rho, const uint8_t* __restrict__ tables, uint32_t nx, uint32_t ny, uint32_t nz,
@@ -143,7 +151,7 @@ __global__ __launch_bounds__(256) void k_sweep_part(
         const uint64_t rem = lines > Lw0 ? lines - Lw0 : 0;
         const uint32_t nval = (uint32_t)((rem < (uint64_t)LPW ? rem : (uint64_t)LPW) * n);
         stage_in<MM>(rho + (uint64_t)s * V + Lw0 * nx, tile, nval, lane, v);
-        __syncthreads();
+        wave_sync();
 #pragma unroll
         for (int t = 0; t < MM; t++) v[t] = (t < (int)mk) ? tile[l * n + sk + 1 + t] : 0.0;
     } else {
@@ -206,7 +214,7 @@ __global__ __launch_bounds__(256) void k_sweep_part(
             if (k == 0) tile[l * n] = T;
             if (k == P - 1) tile[l * n + n - 1] = T;
         }
-        __syncthreads();
+        wave_sync();
         const uint64_t rem = lines > Lw0 ? lines - Lw0 : 0;
         const uint32_t nval = (uint32_t)((rem < (uint64_t)LPW ? rem : (uint64_t)LPW) * n);
         stage_out<MM>(rho + (uint64_t)s * V + Lw0 * nx, tile, nval, lane);
@@ -221,7 +229,7 @@ __global__ __launch_bounds__(256) void k_sweep_part(
             }
         }
     }
-    if (XLDS) __syncthreads();                          // tile reused by the next group
+    if (XLDS) wave_sync();                          // tile reused by the next group
     };
     if (XLDS) {
         for (uint64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
