@@ -32,9 +32,10 @@ def short(name):
 def trace(db):
     c = sqlite3.connect(db)
     rows = list(c.execute("select name, total_calls, total_duration, average, percentage from top_kernels"))
-    out = [f"{'kernel':<60s} {'calls':>7s} {'total_us':>12s} {'avg_us':>10s} {'pct':>6s}"]
+    # the top_kernels view reports durations in microseconds
+    out = [f"{'kernel':<60s} {'calls':>7s} {'total_ms':>10s} {'avg_us':>10s} {'pct':>6s}"]
     for name, calls, tot, avg, pct in rows:
-        out.append(f"{re.sub(r'[(].*', '', name)[:60]:<60s} {calls:>7d} {tot / 1e3:>12.1f} {avg / 1e3:>10.2f} {pct:>6.2f}")
+        out.append(f"{re.sub(r'[(].*', '', name)[:60]:<60s} {calls:>7d} {tot / 1e3:>10.2f} {avg:>10.1f} {pct:>6.2f}")
     return "\n".join(out)
 
 
