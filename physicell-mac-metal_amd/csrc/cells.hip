@@ -291,34 +291,55 @@ __global__ __launch_bounds__(256) void k_forces(const CellRec* __restrict__ rec,
                 if (ni < 0 || ni >= (int)mnx) continue;
                 const uint32_t nb = (uint32_t)nk * mny * mnx + (uint32_t)nj * mnx + (uint32_t)ni;
                 const uint32_t beg = offsets[nb], end = offsets[nb + 1];
-                for (uint32_t u = beg; u < end; u++) {
-                    if (u == t) continue;
-                    const double ddx = px - rec[u].x;
-                    const double ddy = py - rec[u].y;
-                    const double ddz = pz - rec[u].z;
-                    const double dist_sq = ddx * ddx + ddy * ddy + ddz * ddz;
-                    if (dist_sq >= cut2) continue;
-                    if (dist_sq < 1e-12) continue;
-                    const double dist = sqrt(dist_sq);
-                    const double inv_dist = 1.0 / dist;
-                    const double nxv = ddx * inv_dist;
-                    const double nyv = ddy * inv_dist;
-                    const double nzv = ddz * inv_dist;
-                    const double R_sum = radius_i + rec[u].r;
-                    double force_mag = 0.0;
-                    if (dist < R_sum) {
-                        const double overlap = 1.0 - dist / R_sum;
-                        force_mag += overlap * overlap * sqrt(rep_i * rec[u].rep);
-                        local_pressure += overlap * overlap / 0.027288820670331;
+                // Two passes per chunk of up to 32 bin members: a cheap distance
+                // test marks the candidates inside the cutoff, then the force
+                // law runs only over the marked ones, in the same ascending
+                // order.  Lanes of a wave then execute the expensive part about
+                // as often as they have neighbours, not once per candidate.
+                for (uint32_t u0 = beg; u0 < end; u0 += 32) {
+                    const uint32_t cnt = end - u0 < 32u ? end - u0 : 32u;
+                    uint32_t mask = 0;
+                    for (uint32_t j = 0; j < cnt; j++) {
+                        const uint32_t u = u0 + j;
+                        const double2 xy = *reinterpret_cast<const double2*>(&rec[u].x);
+                        const double ddx = px - xy.x;
+                        const double ddy = py - xy.y;
+                        const double ddz = pz - rec[u].z;
+                        const double dist_sq = ddx * ddx + ddy * ddy + ddz * ddz;
+                        const bool inside = u != t && !(dist_sq >= cut2) && !(dist_sq < 1e-12);
+                        mask |= inside ? (1u << j) : 0u;
                     }
-                    const double adhesion_distance = rel_max_adh * R_sum;
-                    if (dist < adhesion_distance) {
-                        const double t2 = 1.0 - dist / adhesion_distance;
-                        force_mag -= t2 * t2 * sqrt(adh_i * rec[u].adh);
+                    while (mask) {
+                        const uint32_t j = __builtin_ctz(mask);
+                        mask &= mask - 1u;
+                        const uint32_t u = u0 + j;
+                        const double2 xy = *reinterpret_cast<const double2*>(&rec[u].x);
+                        const double2 zr = *reinterpret_cast<const double2*>(&rec[u].z);
+                        const double ddx = px - xy.x;
+                        const double ddy = py - xy.y;
+                        const double ddz = pz - zr.x;
+                        const double dist_sq = ddx * ddx + ddy * ddy + ddz * ddz;
+                        const double dist = sqrt(dist_sq);
+                        const double inv_dist = 1.0 / dist;
+                        const double nxv = ddx * inv_dist;
+                        const double nyv = ddy * inv_dist;
+                        const double nzv = ddz * inv_dist;
+                        const double R_sum = radius_i + zr.y;
+                        double force_mag = 0.0;
+                        if (dist < R_sum) {
+                            const double overlap = 1.0 - dist / R_sum;
+                            force_mag += overlap * overlap * sqrt(rep_i * rec[u].rep);
+                            local_pressure += overlap * overlap / 0.027288820670331;
+                        }
+                        const double adhesion_distance = rel_max_adh * R_sum;
+                        if (dist < adhesion_distance) {
+                            const double t2 = 1.0 - dist / adhesion_distance;
+                            force_mag -= t2 * t2 * sqrt(adh_i * rec[u].adh);
+                        }
+                        fx += force_mag * nxv;
+                        fy += force_mag * nyv;
+                        fz += force_mag * nzv;
                     }
-                    fx += force_mag * nxv;
-                    fy += force_mag * nyv;
-                    fz += force_mag * nzv;
                 }
             }
         }
