@@ -42,7 +42,7 @@ def build_workload(name, n_cells):
     raise SystemExit(f"unknown workload {name}")
 
 
-def algorithmic_bytes(kernel, spec, n_cells):
+def algorithmic_bytes(kernel, spec, n_cells, mixed=False):
     """Compulsory HBM bytes of one launch (FP64 plain form).
     Sweeps: 1 read + 1 write of every field value = 16 B per voxel per
     substrate.  Forces: 64-byte sorted record read + bin offsets (amortised
@@ -50,9 +50,9 @@ def algorithmic_bytes(kernel, spec, n_cells):
     v_prev (72 B) + write pos, v_prev (48 B) per agent."""
     V, ns = spec.n_voxels, len(spec.substrates)
     if kernel in SWEEP_KERNELS:
-        return 16.0 * V * ns
+        return (8.0 if mixed else 16.0) * V * ns
     if kernel == H.K_FORCES:
-        return (64.0 + 4.0 + 32.0) * n_cells
+        return ((32.0 + 8.0) if mixed else 64.0) * n_cells + (4.0 + 32.0) * n_cells
     if kernel == H.K_INTEGRATE:
         return 120.0 * n_cells
     return None
@@ -85,6 +85,7 @@ def main():
     ap.add_argument("--workload", default="config2")
     ap.add_argument("--cells", type=int, default=1_000_000)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--mixed", action="store_true", help="FP32-mixed form instead of plain FP64")
     ap.add_argument("--sweep-mode", type=int, default=0,
                     help="0 partitioned sweeps (default), 1 serial elimination, 2 partitioned with LDS-staged x")
     ap.add_argument("--cpu-periods", type=int, default=3)
@@ -110,7 +111,7 @@ def main():
 
     t_setup = time.perf_counter()
     spec, cells = build_workload(args.workload, args.cells)
-    sim = H.Simulation(spec, cells)
+    sim = H.Simulation(spec, cells, mixed=args.mixed)
     if args.sweep_mode:
         rc = H.load().hc_set_sweep_mode(sim.context(), args.sweep_mode)
         if rc:
@@ -156,7 +157,7 @@ def main():
     sim.timing(0)
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
     dom_id = H.KERNEL_NAMES.index(dom)
-    ab = algorithmic_bytes(dom_id, spec, cells.n)
+    ab = algorithmic_bytes(dom_id, spec, cells.n, args.mixed)
     avg_s = kern[dom]["avg_us"] * 1e-6
     traffic = None
     if os.path.exists(args.traffic):
@@ -172,7 +173,7 @@ def main():
     # the three sweeps together (the diffusion substep's roofline)
     sw = [kern[n] for n in ("sweep_x", "sweep_y", "sweep_z") if n in kern]
     if sw:
-        sweep_bytes = 16.0 * spec.n_voxels * len(spec.substrates) * len(sw)
+        sweep_bytes = (8.0 if args.mixed else 16.0) * spec.n_voxels * len(spec.substrates) * len(sw)
         sweep_s = sum(x["avg_us"] for x in sw) * 1e-6
         roof["sweeps_combined_frac"] = sweep_bytes / sweep_s / 1e9 / HBM_PEAK_GBS
 
@@ -186,10 +187,10 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "agent-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if args.mixed else "f64",
         "data": "synthetic",
         "config": {"workload": f"{spec.name}: {spec.nx}x{spec.ny}x{spec.nz} voxels x {len(spec.substrates)} "
-                               f"substrates, {cells.n} agents, plain FP64; step = {n_sub} diffusion substeps "
+                               f"substrates, {cells.n} agents, {'FP32-mixed' if args.mixed else 'plain FP64'}; step = {n_sub} diffusion substeps "
                                f"+ 1 mechanics step", "graph": use_graph, "sweep_mode": args.sweep_mode},
         "voxel_substeps_per_s": vss,
         "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),

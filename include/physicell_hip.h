@@ -48,9 +48,13 @@ typedef enum hc_status {
 typedef struct hc_ctx hc_ctx;   /* opaque: device, stream, timing state      */
 typedef struct hc_buf hc_buf;   /* opaque: device allocation + host mirror   */
 
-/* Diffusion grid; FP64 form of GridParams (shaders/types.h:23-38). */
+/* Diffusion grid; FP64 form of GridParams (shaders/types.h:23-38).
+ * field_fp32 selects the storage of the density field: 0 = FP64 (the plain
+ * form), 1 = FP32 (the mixed form: FP32 values and segment recurrences,
+ * FP64 elimination tables, Dirichlet terms and interface solves). */
 typedef struct hc_grid {
     uint32_t nx, ny, nz, n_substrates;
+    uint32_t field_fp32, _pad;
     double dx, dy, dz, dt;
     double x_min, y_min, z_min;
 } hc_grid;
@@ -72,7 +76,11 @@ typedef struct hc_mech {
     double cell_cell_adhesion;        /* idem                                                   */
     double max_interaction_distance;  /* 30 um cutoff                                           */
     double mechanics_voxel_size;      /* 30 um                                                  */
-    uint32_t mech_grid_nx, mech_grid_ny, mech_grid_nz, _pad;
+    uint32_t mech_grid_nx, mech_grid_ny, mech_grid_nz;
+    /* 0: FP64 forces (plain form, 64-byte records); 1: FP32 force arithmetic
+     * over 32-byte records holding positions relative to their bin's corner
+     * (mixed form).  Keys, bins and AB2 positions are FP64 in both. */
+    uint32_t forces_fp32;
 } hc_mech;
 
 /* ---- cell SoA layout (device buffer of hc_cells_bytes(max_cells)) ----------
@@ -222,6 +230,10 @@ int hc_neighbor_lists(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* m
                       uint32_t n, uint32_t max_cells, uint32_t n_mvox, uint32_t cap,
                       hc_buf* counts, hc_buf* idx);
 
+/* Per-substrate totals sum_v rho[s][v] with compensated (Neumaier) FP64
+ * summation in a fixed order (deterministic); out[n_substrates]. */
+int hc_substrate_totals(hc_ctx* ctx, hc_buf* density, const hc_grid* grid, double* out);
+
 /* ---- timing ------------------------------------------------------------- */
 /* HIP events on the context stream around every launch of the selected
  * kernels (bit mask of hc_kernel_id); accumulated per kernel. */
@@ -251,7 +263,11 @@ typedef struct pc_config {
     double dx, dy, dz;
     uint32_t use_2D, n_substrates;
     double dt_diffusion, dt_mechanics, dt_phenotype, max_time;
-    uint32_t max_cells, _pad;
+    uint32_t max_cells;
+    /* 0: plain FP64 form; 1: FP32-mixed form (FP32 density and sweeps, FP32
+     * force arithmetic; FP64 tables, Dirichlet terms, interface solves,
+     * positions/AB2 and substrate totals). */
+    uint32_t mixed_precision;
     const pc_substrate_cfg* substrates;
 } pc_config;
 
@@ -279,12 +295,15 @@ int pc_sim_run(pc_sim* sim, double max_time, int64_t* counts);
  * use_graph != 0 captures one period as a hipGraph and replays it. */
 int pc_sim_run_periods(pc_sim* sim, int periods, int use_graph);
 int pc_sim_wait(pc_sim* sim);
-/* Copies out (waits first).  density: ns*nx*ny*nz FP64. */
+/* Copies out (waits first).  density: ns*nx*ny*nz FP64 (converted from FP32
+ * in the mixed form). */
 int pc_sim_get_density(pc_sim* sim, double* out);
 int pc_sim_set_density(pc_sim* sim, const double* in);
 /* field: an HC_F_* number; out has n entries (double or uint32 by field). */
 int pc_sim_get_field(pc_sim* sim, uint32_t field, void* out);
 int pc_sim_get_hash(pc_sim* sim, uint32_t* offsets, uint32_t* cells);
+/* Compensated per-substrate totals of the density field (hc_substrate_totals). */
+int pc_sim_substrate_totals(pc_sim* sim, double* out);
 int pc_sim_info(const pc_sim* sim, hc_grid* grid, hc_mech* mech, uint32_t* n_cells);
 hc_ctx* pc_sim_context(pc_sim* sim);
 

@@ -500,7 +500,8 @@ __global__ __launch_bounds__(256) void k_sec_groups(CellPtrs cp, const double* _
     if (any_net) atomicOr(&hdr->has_net, 1u);
 }
 
-__global__ __launch_bounds__(256) void k_sec_apply(double* __restrict__ rho, const uint32_t* __restrict__ gvox,
+template <typename F>
+__global__ __launch_bounds__(256) void k_sec_apply(F* __restrict__ rho, const uint32_t* __restrict__ gvox,
                                                    const uint32_t* __restrict__ gbegin,
                                                    const double* __restrict__ coef,
                                                    const PlanHeader* __restrict__ hdr, uint32_t max_cells,
@@ -514,13 +515,13 @@ __global__ __launch_bounds__(256) void k_sec_apply(double* __restrict__ rho, con
         const double* ca = coef + ((size_t)s * 3 + 0) * max_cells;
         const double* c1 = coef + ((size_t)s * 3 + 1) * max_cells;
         const double* c2 = coef + ((size_t)s * 3 + 2) * max_cells;
-        double r = rho[(uint64_t)s * V + v];
+        double r = (double)rho[(uint64_t)s * V + v];
         for (uint32_t t = b; t < e; t++) {
             if (has_net) r += ca[t];
             r = (r + c1[t]) / c2[t];
             if (r < 0.0) r = 0.0;
         }
-        rho[(uint64_t)s * V + v] = r;
+        rho[(uint64_t)s * V + v] = (F)r;
     }
 }
 
@@ -531,7 +532,7 @@ int probe_kernels_cells(hc_ctx* ctx) {
                          (const void*)k_scatter, (const void*)k_stable_gather, (const void*)k_forces,
                          (const void*)k_neighbors, (const void*)k_max_bin, (const void*)k_integrate,
                          (const void*)k_sec_keys, (const void*)k_sec_flags, (const void*)k_sec_groups,
-                         (const void*)k_sec_apply};
+                         (const void*)k_sec_apply<double>, (const void*)k_sec_apply<float>};
     for (const void* f : fns) {
         hipError_t e = hipFuncGetAttributes(&attr, f);
         if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: cell kernel");
@@ -624,11 +625,171 @@ __global__ __launch_bounds__(256) void k_stable_sort(const uint32_t* __restrict_
     sorted[beg + stable_rank(tmp, beg, end, c)] = c;
 }
 
+// ---------------------------------------------------------------------------
+// mixed form: 32-byte records, FP32 force arithmetic
+// ---------------------------------------------------------------------------
+// Positions are stored relative to the lower corner of the agent's own bin,
+// so a neighbour difference is (own offset - other offset) - bin delta * 30:
+// values below ~60 um in FP32 keep ~4e-6 um of resolution wherever the
+// domain sits.  Own relative adhesion distance is read from the SoA.
+struct __attribute__((aligned(16))) CellRec32 {
+    float x, y, z, r, rep, adh;
+    uint32_t idx, key;
+};
+static_assert(sizeof(CellRec32) == 32, "mixed record must be 32 bytes");
+
+__global__ __launch_bounds__(256) void k_stable_gather32(CellPtrs cp, const uint32_t* __restrict__ key,
+                                                         const uint32_t* __restrict__ offsets,
+                                                         const uint32_t* __restrict__ tmp,
+                                                         uint32_t* __restrict__ sorted, CellRec32* __restrict__ rec,
+                                                         uint32_t n, double x_min, double y_min, double z_min,
+                                                         double voxel, uint32_t mnx, uint32_t mny) {
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t b = key[c];
+    const uint32_t beg = offsets[b], end = offsets[b + 1];
+    const uint32_t pos = beg + stable_rank(tmp, beg, end, c);
+    sorted[pos] = c;
+    const uint32_t bi = b % mnx, bj = (b / mnx) % mny, bk = b / (mnx * mny);
+    CellRec32 r;
+    r.x = (float)(cp.f[HC_F_POS_X][c] - (x_min + bi * voxel));
+    r.y = (float)(cp.f[HC_F_POS_Y][c] - (y_min + bj * voxel));
+    r.z = (float)(cp.f[HC_F_POS_Z][c] - (z_min + bk * voxel));
+    r.r = (float)cp.f[HC_F_RADIUS][c];
+    r.rep = (float)cp.f[HC_F_REPULSION][c];
+    r.adh = (float)cp.f[HC_F_ADHESION][c];
+    r.idx = c;
+    r.key = b;
+    rec[pos] = r;
+}
+
+__global__ __launch_bounds__(256) void k_forces32(const CellRec32* __restrict__ rec, const uint32_t* __restrict__ offsets,
+                                                  const double* __restrict__ rel_adh, double* __restrict__ vx,
+                                                  double* __restrict__ vy, double* __restrict__ vz,
+                                                  double* __restrict__ pressure, uint32_t n, uint32_t mnx, uint32_t mny,
+                                                  uint32_t mnz, float max_dist, float voxel) {
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (t >= n) return;
+    const CellRec32 me = rec[t];
+    const float radius_i = me.r, rep_i = me.rep, adh_i = me.adh;
+    const float rel_max_adh = (float)rel_adh[me.idx];
+    float fx = 0.f, fy = 0.f, fz = 0.f, local_pressure = 0.f;
+    const uint32_t vi = me.key % mnx, vj = (me.key / mnx) % mny, vk = me.key / (mnx * mny);
+    const int dk_min = (mnz > 1) ? -1 : 0, dk_max = (mnz > 1) ? 1 : 0;
+    const float cut2 = max_dist * max_dist;
+    for (int dk = dk_min; dk <= dk_max; dk++) {
+        const int nk = (int)vk + dk;
+        if (nk < 0 || nk >= (int)mnz) continue;
+        for (int dj = -1; dj <= 1; dj++) {
+            const int nj = (int)vj + dj;
+            if (nj < 0 || nj >= (int)mny) continue;
+            for (int di = -1; di <= 1; di++) {
+                const int ni = (int)vi + di;
+                if (ni < 0 || ni >= (int)mnx) continue;
+                const float ox = (float)di * voxel, oy = (float)dj * voxel, oz = (float)dk * voxel;
+                const uint32_t nb = (uint32_t)nk * mny * mnx + (uint32_t)nj * mnx + (uint32_t)ni;
+                const uint32_t beg = offsets[nb], end = offsets[nb + 1];
+                for (uint32_t u0 = beg; u0 < end; u0 += 32) {
+                    const uint32_t cnt = end - u0 < 32u ? end - u0 : 32u;
+                    uint32_t mask = 0;
+                    for (uint32_t j = 0; j < cnt; j++) {
+                        const uint32_t u = u0 + j;
+                        const float4 o = *reinterpret_cast<const float4*>(&rec[u].x);
+                        const float ddx = me.x - o.x - ox, ddy = me.y - o.y - oy, ddz = me.z - o.z - oz;
+                        const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                        const bool inside = u != t && !(d2 >= cut2) && !(d2 < 1e-12f);
+                        mask |= inside ? (1u << j) : 0u;
+                    }
+                    while (mask) {
+                        const uint32_t j = __builtin_ctz(mask);
+                        mask &= mask - 1u;
+                        const uint32_t u = u0 + j;
+                        const float4 o = *reinterpret_cast<const float4*>(&rec[u].x);
+                        const float ddx = me.x - o.x - ox, ddy = me.y - o.y - oy, ddz = me.z - o.z - oz;
+                        const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                        const float dist = sqrtf(d2);
+                        const float inv_dist = 1.0f / dist;
+                        const float R_sum = radius_i + o.w;
+                        float force_mag = 0.f;
+                        if (dist < R_sum) {
+                            const float overlap = 1.0f - dist / R_sum;
+                            force_mag += overlap * overlap * sqrtf(rep_i * rec[u].rep);
+                            local_pressure += overlap * overlap / 0.027288820670331f;
+                        }
+                        const float adhesion_distance = rel_max_adh * R_sum;
+                        if (dist < adhesion_distance) {
+                            const float t2 = 1.0f - dist / adhesion_distance;
+                            force_mag -= t2 * t2 * sqrtf(adh_i * rec[u].adh);
+                        }
+                        fx += force_mag * (ddx * inv_dist);
+                        fy += force_mag * (ddy * inv_dist);
+                        fz += force_mag * (ddz * inv_dist);
+                    }
+                }
+            }
+        }
+    }
+    const uint32_t c = me.idx;
+    vx[c] = fx;
+    vy[c] = fy;
+    vz[c] = fz;
+    pressure[c] = local_pressure;
+}
+
+// ---------------------------------------------------------------------------
+// compensated substrate totals (Neumaier), fixed combination order
+// ---------------------------------------------------------------------------
+struct Neu {
+    double s, c;
+};
+__device__ inline void neu_add(Neu& a, double x) {
+    const double t = a.s + x;
+    if (fabs(a.s) >= fabs(x)) a.c += (a.s - t) + x;
+    else a.c += (x - t) + a.s;
+    a.s = t;
+}
+__device__ inline void neu_merge(Neu& a, const Neu& b) {
+    neu_add(a, b.s);
+    a.c += b.c;
+}
+
+static constexpr int TOT_BLOCKS = 256;
+
+template <typename F>
+__global__ __launch_bounds__(256) void k_totals(const F* __restrict__ rho, uint64_t V, Neu* __restrict__ partial) {
+    const uint32_t s = blockIdx.y;
+    const F* r = rho + (uint64_t)s * V;
+    Neu a{0.0, 0.0};
+    for (uint64_t v = (uint64_t)blockIdx.x * 256 + threadIdx.x; v < V; v += (uint64_t)gridDim.x * 256)
+        neu_add(a, (double)r[v]);
+    __shared__ Neu red[256];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) neu_merge(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(size_t)s * gridDim.x + blockIdx.x] = red[0];
+}
+
+__global__ void k_totals_final(const Neu* __restrict__ partial, uint32_t nb, uint32_t ns, double* __restrict__ out) {
+    const uint32_t s = threadIdx.x;
+    if (s >= ns) return;
+    Neu a{0.0, 0.0};
+    for (uint32_t b = 0; b < nb; b++) neu_merge(a, partial[(size_t)s * nb + b]);
+    out[s] = a.s + a.c;
+}
+
 int probe_kernels_cells2(hc_ctx* ctx);
 int probe_kernels_cells2(hc_ctx* ctx) {
     hipFuncAttributes attr;
-    hipError_t e = hipFuncGetAttributes(&attr, (const void*)k_stable_sort);
-    return e == hipSuccess ? 0 : hip_fail(ctx, e, "hc_load_kernels: k_stable_sort");
+    const void* fns[] = {(const void*)k_stable_sort, (const void*)k_stable_gather32, (const void*)k_forces32,
+                         (const void*)k_totals<double>, (const void*)k_totals<float>, (const void*)k_totals_final};
+    for (const void* f : fns) {
+        hipError_t e = hipFuncGetAttributes(&attr, f);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: cell kernel (mixed form)");
+    }
+    return 0;
 }
 
 struct HashPtrs {
@@ -715,7 +876,7 @@ int hc_secretion_plan(hc_ctx* ctx, hc_buf* cells, hc_buf* rates, const hc_grid* 
 int hc_secretion_apply(hc_ctx* ctx, hc_buf* density, hc_buf* plan, const hc_grid* g) {
     if (!ctx || !density || !plan || !g) return fail(ctx, HC_ERR_ARG, "hc_secretion_apply: null argument");
     const uint64_t V = (uint64_t)g->nx * g->ny * g->nz;
-    if (hc_buf_size(density) < V * g->n_substrates * sizeof(double))
+    if (hc_buf_size(density) < V * g->n_substrates * field_elem(g))
         return fail(ctx, HC_ERR_ARG, "hc_secretion_apply: density buffer too small");
     const uint8_t* P = (const uint8_t*)plan->dptr;
     const uint32_t max_cells = (uint32_t)plan->aux;
@@ -725,10 +886,16 @@ int hc_secretion_apply(hc_ctx* ctx, hc_buf* density, hc_buf* plan, const hc_grid
     const PlanLayout L = plan_layout(g, max_cells);
     if (!g->n_substrates || !max_cells) return 0;
     KernelTimer tm(ctx, HC_K_SEC_APPLY);
-    hipLaunchKernelGGL(k_sec_apply, dim3(blocks_for(max_cells, 256)), dim3(256), 0, ctx->stream,
-                       (double*)density->dptr, (const uint32_t*)(P + L.gvox), (const uint32_t*)(P + L.gbegin),
-                       (const double*)(P + L.coef), (const PlanHeader*)(P + L.header), max_cells,
-                       g->n_substrates, V);
+    if (g->field_fp32)
+        hipLaunchKernelGGL(k_sec_apply<float>, dim3(blocks_for(max_cells, 256)), dim3(256), 0, ctx->stream,
+                           (float*)density->dptr, (const uint32_t*)(P + L.gvox), (const uint32_t*)(P + L.gbegin),
+                           (const double*)(P + L.coef), (const PlanHeader*)(P + L.header), max_cells,
+                           g->n_substrates, V);
+    else
+        hipLaunchKernelGGL(k_sec_apply<double>, dim3(blocks_for(max_cells, 256)), dim3(256), 0, ctx->stream,
+                           (double*)density->dptr, (const uint32_t*)(P + L.gvox), (const uint32_t*)(P + L.gbegin),
+                           (const double*)(P + L.coef), (const PlanHeader*)(P + L.header), max_cells,
+                           g->n_substrates, V);
     return check_launch(ctx, "k_sec_apply");
 }
 
@@ -771,8 +938,13 @@ int hc_build_hash(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech,
     if (n) {
         const uint32_t* key = cp.u[HC_F_MECH_VOXEL - HC_NUM_F64_FIELDS];
         hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, ctx->stream, key, h.slot, h.offsets, h.tmp, n);
-        hipLaunchKernelGGL(k_stable_gather, dim3(nb), dim3(256), 0, ctx->stream, cp, key, h.offsets, h.tmp,
-                           h.sorted, h.rec, n);
+        if (mech->forces_fp32)
+            hipLaunchKernelGGL(k_stable_gather32, dim3(nb), dim3(256), 0, ctx->stream, cp, key, h.offsets, h.tmp,
+                               h.sorted, reinterpret_cast<CellRec32*>(h.rec), n, mgrid->x_min, mgrid->y_min,
+                               mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny);
+        else
+            hipLaunchKernelGGL(k_stable_gather, dim3(nb), dim3(256), 0, ctx->stream, cp, key, h.offsets, h.tmp,
+                               h.sorted, h.rec, n);
         if ((r = check_launch(ctx, "hash scatter"))) return r;
     }
     return 0;
@@ -787,10 +959,17 @@ int hc_forces(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech, uin
     HashPtrs h = hash_ptrs(hash, n_mvox, max_cells);
     CellPtrs cp = cell_ptrs(cells->dptr, max_cells);
     KernelTimer tm(ctx, HC_K_FORCES);
-    hipLaunchKernelGGL(k_forces, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, h.rec, h.offsets,
-                       cp.f[HC_F_VEL_X], cp.f[HC_F_VEL_Y], cp.f[HC_F_VEL_Z], cp.f[HC_F_PRESSURE], n,
-                       mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz,
-                       mech->max_interaction_distance);
+    if (mech->forces_fp32)
+        hipLaunchKernelGGL(k_forces32, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream,
+                           reinterpret_cast<const CellRec32*>(h.rec), h.offsets, cp.f[HC_F_REL_MAX_ADH],
+                           cp.f[HC_F_VEL_X], cp.f[HC_F_VEL_Y], cp.f[HC_F_VEL_Z], cp.f[HC_F_PRESSURE], n,
+                           mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz,
+                           (float)mech->max_interaction_distance, (float)mech->mechanics_voxel_size);
+    else
+        hipLaunchKernelGGL(k_forces, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, h.rec, h.offsets,
+                           cp.f[HC_F_VEL_X], cp.f[HC_F_VEL_Y], cp.f[HC_F_VEL_Z], cp.f[HC_F_PRESSURE], n,
+                           mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz,
+                           mech->max_interaction_distance);
     return check_launch(ctx, "k_forces");
 }
 
@@ -856,6 +1035,8 @@ int hc_neighbor_lists(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* m
     int r = check_cells(ctx, cells, n, max_cells, "hc_neighbor_lists");
     if (r || (r = check_hash(ctx, hash, n_mvox, max_cells, "hc_neighbor_lists"))) return r;
     if (!mech || !counts || !idx) return fail(ctx, HC_ERR_ARG, "hc_neighbor_lists: null argument");
+    if (mech->forces_fp32)
+        return fail(ctx, HC_ERR_STATE, "hc_neighbor_lists: needs the FP64 records (forces_fp32 = 0)");
     if (hc_buf_size(counts) < (size_t)n * 4 || hc_buf_size(idx) < (size_t)n * cap * 4)
         return fail(ctx, HC_ERR_ARG, "hc_neighbor_lists: output buffers too small");
     if (!n) return 0;
@@ -864,6 +1045,35 @@ int hc_neighbor_lists(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* m
                        (uint32_t*)counts->dptr, (uint32_t*)idx->dptr, n, mech->mech_grid_nx, mech->mech_grid_ny,
                        mech->mech_grid_nz, mech->max_interaction_distance, cap);
     return check_launch(ctx, "k_neighbors");
+}
+
+int hc_substrate_totals(hc_ctx* ctx, hc_buf* density, const hc_grid* g, double* out) {
+    if (!ctx || !density || !g || !out) return fail(ctx, HC_ERR_ARG, "hc_substrate_totals: null argument");
+    const uint64_t V = (uint64_t)g->nx * g->ny * g->nz;
+    const uint32_t ns = g->n_substrates;
+    if (!ns) return 0;
+    if (ns > 1024) return fail(ctx, HC_ERR_ARG, "hc_substrate_totals: more than 1024 substrates");
+    if (hc_buf_size(density) < V * ns * field_elem(g)) return fail(ctx, HC_ERR_ARG, "hc_substrate_totals: buffer too small");
+    void* scratch = nullptr;
+    const size_t pbytes = (size_t)ns * TOT_BLOCKS * sizeof(Neu);
+    hipError_t e = hipMalloc(&scratch, pbytes + ns * sizeof(double));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hc_substrate_totals: scratch");
+    Neu* part = (Neu*)scratch;
+    double* dout = (double*)((char*)scratch + pbytes);
+    if (g->field_fp32)
+        hipLaunchKernelGGL(k_totals<float>, dim3(TOT_BLOCKS, ns), dim3(256), 0, ctx->stream, (const float*)density->dptr, V, part);
+    else
+        hipLaunchKernelGGL(k_totals<double>, dim3(TOT_BLOCKS, ns), dim3(256), 0, ctx->stream, (const double*)density->dptr, V, part);
+    hipLaunchKernelGGL(k_totals_final, dim3(1), dim3(1024), 0, ctx->stream, part, (uint32_t)TOT_BLOCKS, ns, dout);
+    int r = check_launch(ctx, "k_totals");
+    if (!r) {
+        e = hipMemcpyAsync(out, dout, ns * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) r = hip_fail(ctx, e, "hc_substrate_totals");
+    }
+    hipStreamSynchronize(ctx->stream);
+    hipFree(scratch);
+    return r;
 }
 
 }  // extern "C"

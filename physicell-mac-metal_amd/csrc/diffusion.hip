@@ -36,9 +36,9 @@ static constexpr int XWAVES = 4;        // waves per block in the x kernel
 // ---------------------------------------------------------------------------
 // one line per lane
 // ---------------------------------------------------------------------------
-template <int AXIS, int PF>
+template <int AXIS, int PF, typename F>
 __global__ __launch_bounds__(LANE_BLOCK) void k_sweep_lane(
-    double* __restrict__ rho, const uint8_t* __restrict__ tables, uint32_t nx, uint32_t ny,
+    F* __restrict__ rho, const uint8_t* __restrict__ tables, uint32_t nx, uint32_t ny,
     uint32_t nz, uint32_t nmax, int fuse_dirichlet) {
     const uint32_t s = blockIdx.y;
     const uint64_t L = (uint64_t)blockIdx.x * LANE_BLOCK + threadIdx.x;
@@ -66,32 +66,32 @@ __global__ __launch_bounds__(LANE_BLOCK) void k_sweep_lane(
     const double c = sc[s].c[AXIS];
     const double T = sc[s].T;
     const bool dir = sc[s].dir != 0;
-    double* p = rho + (uint64_t)s * V + base;
+    F* p = rho + (uint64_t)s * V + base;
 
     if (fuse_dirichlet && dir && face) {
         // Every voxel of this line is on a face of another axis: the
         // Dirichlet re-apply that follows the substep sets all of it to T.
-        for (uint32_t m = 0; m < n; m++) p[(uint64_t)m * stride] = T;
+        for (uint32_t m = 0; m < n; m++) p[(uint64_t)m * stride] = (F)T;
         return;
     }
-    if (dir) { p[0] = T; p[(uint64_t)(n - 1) * stride] = T; }
+    if (dir) { p[0] = (F)T; p[(uint64_t)(n - 1) * stride] = (F)T; }
     if (n <= 2) return;
     const double* cpt = tab + table_index(s, AXIS, 0, nmax);
     const double* dnt = tab + table_index(s, AXIS, 1, nmax);
     if (n == 3) {
-        double d = p[stride];
+        double d = (double)p[stride];
         if (dir) { d += c * T; d += c * T; }
-        p[stride] = d / dnt[1];
-        if (dir) { p[0] = T; p[2 * stride] = T; }
+        p[stride] = (F)(d / dnt[1]);
+        if (dir) { p[0] = (F)T; p[2 * stride] = (F)T; }
         return;
     }
     const double a = -c;
     double dprev;
     {
-        double d = p[stride];
+        double d = (double)p[stride];
         if (dir) d += c * T;
         dprev = d / dnt[1];
-        p[stride] = dprev;
+        p[stride] = (F)dprev;
     }
     // interior rows 2..n-3, PF-element chunks, next chunk loaded ahead
     const uint32_t iend = n - 2;
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(LANE_BLOCK) void k_sweep_lane(
 #pragma unroll
             for (int k = 0; k < PF; k++) {
                 dprev = (cur[k] - a * dprev) / dnt[i + k];
-                p[(uint64_t)(i + k) * stride] = dprev;
+                p[(uint64_t)(i + k) * stride] = (F)dprev;
             }
             i += PF;
             if (!more) break;
@@ -119,15 +119,15 @@ __global__ __launch_bounds__(LANE_BLOCK) void k_sweep_lane(
         }
     }
     for (; i < iend; i++) {
-        const double d = p[(uint64_t)i * stride];
+        const double d = (double)p[(uint64_t)i * stride];
         dprev = (d - a * dprev) / dnt[i];
-        p[(uint64_t)i * stride] = dprev;
+        p[(uint64_t)i * stride] = (F)dprev;
     }
     {
-        double d = p[(uint64_t)(n - 2) * stride];
+        double d = (double)p[(uint64_t)(n - 2) * stride];
         if (dir) d += c * T;
         dprev = (d - a * dprev) / dnt[n - 2];
-        p[(uint64_t)(n - 2) * stride] = dprev;
+        p[(uint64_t)(n - 2) * stride] = (F)dprev;
     }
     // back substitution rows n-3..1
     double u = dprev;
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(LANE_BLOCK) void k_sweep_lane(
 #pragma unroll
             for (int k = 0; k < PF; k++) {
                 u = cur[k] - cpt[j - k] * u;
-                p[(uint64_t)(j - k) * stride] = u;
+                p[(uint64_t)(j - k) * stride] = (F)u;
             }
             j -= PF;
             if (!more) break;
@@ -155,17 +155,18 @@ __global__ __launch_bounds__(LANE_BLOCK) void k_sweep_lane(
         }
     }
     for (; j >= 1; j--) {
-        u = p[(uint64_t)j * stride] - cpt[j] * u;
-        p[(uint64_t)j * stride] = u;
+        u = (double)p[(uint64_t)j * stride] - cpt[j] * u;
+        p[(uint64_t)j * stride] = (F)u;
     }
-    if (dir) { p[0] = T; p[(uint64_t)(n - 1) * stride] = T; }
+    if (dir) { p[0] = (F)T; p[(uint64_t)(n - 1) * stride] = (F)T; }
 }
 
 // ---------------------------------------------------------------------------
 // x lines through LDS (n >= 4)
 // ---------------------------------------------------------------------------
+template <typename F>
 __global__ __launch_bounds__(64 * XWAVES) void k_sweep_x_lds(
-    double* __restrict__ rho, const uint8_t* __restrict__ tables, uint32_t nx,
+    F* __restrict__ rho, const uint8_t* __restrict__ tables, uint32_t nx,
     uint64_t lines, uint64_t V, uint32_t nmax) {
     __shared__ double tile[XWAVES][XCH][65];
     const uint32_t w = threadIdx.x >> 6;
@@ -182,7 +183,7 @@ __global__ __launch_bounds__(64 * XWAVES) void k_sweep_x_lds(
     const double a = -c;
     const double* cpt = tab + table_index(s, 0, 0, nmax);
     const double* dnt = tab + table_index(s, 0, 1, nmax);
-    double* base = rho + (uint64_t)s * V;
+    F* base = rho + (uint64_t)s * V;
     const uint32_t n = nx;
     double (*t)[65] = tile[w];
 
@@ -195,7 +196,7 @@ __global__ __launch_bounds__(64 * XWAVES) void k_sweep_x_lds(
             const uint32_t l = idx / XCH, e = idx % XCH;
             const uint64_t Lg = line0 + l;
             const uint32_t col = c0 + e;
-            t[e][l] = (Lg < lines && col < n) ? base[Lg * nx + col] : 0.0;
+            t[e][l] = (Lg < lines && col < n) ? (double)base[Lg * nx + col] : 0.0;
         }
         __syncthreads();
         if (valid) {
@@ -224,7 +225,7 @@ __global__ __launch_bounds__(64 * XWAVES) void k_sweep_x_lds(
             const uint32_t l = idx / XCH, e = idx % XCH;
             const uint64_t Lg = line0 + l;
             const uint32_t col = c0 + e;
-            if (Lg < lines && col < n) base[Lg * nx + col] = t[e][l];
+            if (Lg < lines && col < n) base[Lg * nx + col] = (F)t[e][l];
         }
         __syncthreads();
     }
@@ -240,7 +241,7 @@ __global__ __launch_bounds__(64 * XWAVES) void k_sweep_x_lds(
             const uint32_t l = idx / XCH, e = idx % XCH;
             const uint64_t Lg = line0 + l;
             const uint32_t col = (uint32_t)c0 + e;
-            t[e][l] = (Lg < lines && col < n) ? base[Lg * nx + col] : 0.0;
+            t[e][l] = (Lg < lines && col < n) ? (double)base[Lg * nx + col] : 0.0;
         }
         __syncthreads();
         if (valid) {
@@ -258,7 +259,7 @@ __global__ __launch_bounds__(64 * XWAVES) void k_sweep_x_lds(
             const uint32_t l = idx / XCH, e = idx % XCH;
             const uint64_t Lg = line0 + l;
             const uint32_t col = (uint32_t)c0 + e;
-            if (Lg < lines && col < n) base[Lg * nx + col] = t[e][l];
+            if (Lg < lines && col < n) base[Lg * nx + col] = (F)t[e][l];
         }
         __syncthreads();
     }
@@ -267,7 +268,8 @@ __global__ __launch_bounds__(64 * XWAVES) void k_sweep_x_lds(
 // ---------------------------------------------------------------------------
 // standalone Dirichlet re-apply (all faces)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_dirichlet(double* __restrict__ rho,
+template <typename F>
+__global__ __launch_bounds__(256) void k_dirichlet(F* __restrict__ rho,
                                                    const uint8_t* __restrict__ tables,
                                                    uint32_t nx, uint32_t ny, uint32_t nz) {
     const uint32_t s = blockIdx.y;
@@ -280,16 +282,18 @@ __global__ __launch_bounds__(256) void k_dirichlet(double* __restrict__ rho,
     const uint32_t j = (uint32_t)((v / nx) % ny);
     const uint32_t k = (uint32_t)(v / ((uint64_t)nx * ny));
     const bool face = i == 0 || i == nx - 1 || j == 0 || j == ny - 1 || (nz > 1 && (k == 0 || k == nz - 1));
-    if (face) rho[(uint64_t)s * V + v] = sc[s].T;
+    if (face) rho[(uint64_t)s * V + v] = (F)sc[s].T;
 }
 
 int probe_kernels_part(hc_ctx* ctx);
 int probe_kernels_diffusion(hc_ctx* ctx) {
     if (int r = probe_kernels_part(ctx)) return r;
     hipFuncAttributes attr;
-    const void* fns[] = {(const void*)k_sweep_lane<0, 8>, (const void*)k_sweep_lane<1, 8>,
-                         (const void*)k_sweep_lane<2, 8>, (const void*)k_sweep_x_lds,
-                         (const void*)k_dirichlet};
+    const void* fns[] = {(const void*)k_sweep_lane<0, 8, double>, (const void*)k_sweep_lane<1, 8, double>,
+                         (const void*)k_sweep_lane<2, 8, double>, (const void*)k_sweep_x_lds<double>,
+                         (const void*)k_dirichlet<double>, (const void*)k_sweep_lane<0, 8, float>,
+                         (const void*)k_sweep_lane<1, 8, float>, (const void*)k_sweep_lane<2, 8, float>,
+                         (const void*)k_sweep_x_lds<float>, (const void*)k_dirichlet<float>};
     for (const void* f : fns) {
         hipError_t e = hipFuncGetAttributes(&attr, f);
         if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: diffusion kernel");
@@ -297,22 +301,17 @@ int probe_kernels_diffusion(hc_ctx* ctx) {
     return 0;
 }
 
-int launch_sweep(hc_ctx* ctx, int axis, double* rho, const hc_grid* g, const uint8_t* tables,
-                 int fuse_dirichlet) {
+template <typename F>
+static int launch_sweep_serial(hc_ctx* ctx, int axis, F* rho, const hc_grid* g, const uint8_t* tables,
+                               int fuse_dirichlet) {
     const uint32_t nx = g->nx, ny = g->ny, nz = g->nz, ns = g->n_substrates;
-    if (ns == 0 || nx == 0 || ny == 0 || nz == 0) return 0;
-    if (ctx->sweep_mode != 1) {
-        const int r = launch_sweep_part(ctx, axis, rho, g, tables, fuse_dirichlet);
-        if (r != 1) return r;
-    }
     const uint32_t nmax = grid_nmax(g);
     const uint64_t V = (uint64_t)nx * ny * nz;
     if (axis == 0 && nx >= 4 && !fuse_dirichlet) {
         KernelTimer tm(ctx, HC_K_SWEEP_X);
         const uint64_t lines = (uint64_t)ny * nz;
         dim3 grid(blocks_for(lines, 64 * XWAVES), ns);
-        hipLaunchKernelGGL(k_sweep_x_lds, grid, dim3(64 * XWAVES), 0, ctx->stream, rho, tables,
-                           nx, lines, V, nmax);
+        hipLaunchKernelGGL(k_sweep_x_lds<F>, grid, dim3(64 * XWAVES), 0, ctx->stream, rho, tables, nx, lines, V, nmax);
         return check_launch(ctx, "k_sweep_x_lds");
     }
     const uint32_t n = axis == 0 ? nx : (axis == 1 ? ny : nz);
@@ -320,33 +319,47 @@ int launch_sweep(hc_ctx* ctx, int axis, double* rho, const hc_grid* g, const uin
     dim3 grid(blocks_for(lines, LANE_BLOCK), ns);
     KernelTimer tm(ctx, n <= 3 ? HC_K_SWEEP_SMALL : HC_K_SWEEP_X + axis);
     if (axis == 0)
-        hipLaunchKernelGGL((k_sweep_lane<0, 8>), grid, dim3(LANE_BLOCK), 0, ctx->stream, rho, tables,
-                           nx, ny, nz, nmax, fuse_dirichlet);
+        hipLaunchKernelGGL((k_sweep_lane<0, 8, F>), grid, dim3(LANE_BLOCK), 0, ctx->stream, rho, tables, nx, ny, nz, nmax,
+                           fuse_dirichlet);
     else if (axis == 1)
-        hipLaunchKernelGGL((k_sweep_lane<1, 8>), grid, dim3(LANE_BLOCK), 0, ctx->stream, rho, tables,
-                           nx, ny, nz, nmax, fuse_dirichlet);
+        hipLaunchKernelGGL((k_sweep_lane<1, 8, F>), grid, dim3(LANE_BLOCK), 0, ctx->stream, rho, tables, nx, ny, nz, nmax,
+                           fuse_dirichlet);
     else
-        hipLaunchKernelGGL((k_sweep_lane<2, 8>), grid, dim3(LANE_BLOCK), 0, ctx->stream, rho, tables,
-                           nx, ny, nz, nmax, fuse_dirichlet);
+        hipLaunchKernelGGL((k_sweep_lane<2, 8, F>), grid, dim3(LANE_BLOCK), 0, ctx->stream, rho, tables, nx, ny, nz, nmax,
+                           fuse_dirichlet);
     return check_launch(ctx, "k_sweep_lane");
 }
 
-int launch_dirichlet(hc_ctx* ctx, double* rho, const hc_grid* g, const uint8_t* tables) {
+int launch_sweep(hc_ctx* ctx, int axis, void* rho, const hc_grid* g, const uint8_t* tables, int fuse_dirichlet) {
+    if (g->n_substrates == 0 || g->nx == 0 || g->ny == 0 || g->nz == 0) return 0;
+    if (ctx->sweep_mode != 1) {
+        const int r = launch_sweep_part(ctx, axis, rho, g, tables, fuse_dirichlet);
+        if (r != 1) return r;
+    }
+    if (g->field_fp32) return launch_sweep_serial<float>(ctx, axis, (float*)rho, g, tables, fuse_dirichlet);
+    return launch_sweep_serial<double>(ctx, axis, (double*)rho, g, tables, fuse_dirichlet);
+}
+
+int launch_dirichlet(hc_ctx* ctx, void* rho, const hc_grid* g, const uint8_t* tables) {
     const uint64_t V = (uint64_t)g->nx * g->ny * g->nz;
     if (!V || !g->n_substrates) return 0;
     KernelTimer tm(ctx, HC_K_DIRICHLET);
     dim3 grid(blocks_for(V, 256), g->n_substrates);
-    hipLaunchKernelGGL(k_dirichlet, grid, dim3(256), 0, ctx->stream, rho, tables, g->nx, g->ny, g->nz);
+    if (g->field_fp32)
+        hipLaunchKernelGGL(k_dirichlet<float>, grid, dim3(256), 0, ctx->stream, (float*)rho, tables, g->nx, g->ny, g->nz);
+    else
+        hipLaunchKernelGGL(k_dirichlet<double>, grid, dim3(256), 0, ctx->stream, (double*)rho, tables, g->nx, g->ny, g->nz);
     return check_launch(ctx, "k_dirichlet");
 }
 
 // Host construction of the per-substrate constants and the elimination
 // tables.  The operation sequence is the reference's in-kernel recurrence
 // (shaders/diffusion_2d.metal:89-142) evaluated once in FP64.
-static void build_tables_host(const hc_grid* g, const hc_substrate* subs, std::vector<uint8_t>& out) {
+void build_tables_host(const hc_grid* g, const hc_substrate* subs, std::vector<uint8_t>& out) {
     const uint32_t ns = g->n_substrates, nmax = grid_nmax(g);
-    out.assign(part_offset(ns, nmax) + (size_t)ns * 3 * part_block(nmax) * sizeof(double), 0);
-    build_part_tables_host(g, subs, reinterpret_cast<double*>(out.data() + part_offset(ns, nmax)));
+    out.assign(table_total_bytes(ns, nmax), 0);
+    build_part_tables_host(g, subs, reinterpret_cast<double*>(out.data() + part_offset(ns, nmax)),
+                           reinterpret_cast<float*>(out.data() + part_f_offset(ns, nmax)));
     SubConst* sc = reinterpret_cast<SubConst*>(out.data());
     double* tab = reinterpret_cast<double*>(out.data() + subconst_bytes(ns));
     const double sp[3] = {g->dx, g->dy, g->dz};
@@ -391,8 +404,7 @@ extern "C" {
 
 size_t hc_thomas_table_bytes(const hc_grid* g) {
     if (!g) return 0;
-    const uint32_t nmax = grid_nmax(g);
-    return part_offset(g->n_substrates, nmax) + (size_t)g->n_substrates * 3 * part_block(nmax) * sizeof(double);
+    return table_total_bytes(g->n_substrates, grid_nmax(g));
 }
 
 int hc_set_sweep_mode(hc_ctx* ctx, int mode) {
@@ -414,7 +426,7 @@ int hc_build_thomas_tables(hc_ctx* ctx, const hc_grid* g, const hc_substrate* su
 
 static int check_grid_args(hc_ctx* ctx, hc_buf* density, const hc_grid* g, hc_buf* tables, const char* who) {
     if (!ctx || !density || !g || !tables) return fail(ctx, HC_ERR_ARG, std::string(who) + ": null argument");
-    const uint64_t need = (uint64_t)g->nx * g->ny * g->nz * g->n_substrates * sizeof(double);
+    const uint64_t need = (uint64_t)g->nx * g->ny * g->nz * g->n_substrates * field_elem(g);
     if (hc_buf_size(density) < need) return fail(ctx, HC_ERR_ARG, std::string(who) + ": density buffer too small");
     if (hc_buf_size(tables) < hc_thomas_table_bytes(g))
         return fail(ctx, HC_ERR_ARG, std::string(who) + ": tables too small (build them first)");
@@ -427,12 +439,12 @@ int hc_diffusion_sweep(hc_ctx* ctx, int axis, hc_buf* density, const hc_grid* g,
     int r = check_grid_args(ctx, density, g, tables, "hc_diffusion_sweep");
     if (r) return r;
     if (axis < 0 || axis > 2) return fail(ctx, HC_ERR_ARG, "hc_diffusion_sweep: axis must be 0, 1 or 2");
-    return launch_sweep(ctx, axis, (double*)density->dptr, g, (const uint8_t*)tables->dptr, fuse_dirichlet);
+    return launch_sweep(ctx, axis, density->dptr, g, (const uint8_t*)tables->dptr, fuse_dirichlet);
 }
 
 int hc_dirichlet(hc_ctx* ctx, hc_buf* density, const hc_grid* g, const hc_substrate* subs) {
     if (!ctx || !density || !g || (!subs && g->n_substrates)) return fail(ctx, HC_ERR_ARG, "hc_dirichlet: null argument");
-    const uint64_t need = (uint64_t)g->nx * g->ny * g->nz * g->n_substrates * sizeof(double);
+    const uint64_t need = (uint64_t)g->nx * g->ny * g->nz * g->n_substrates * field_elem(g);
     if (hc_buf_size(density) < need) return fail(ctx, HC_ERR_ARG, "hc_dirichlet: density buffer too small");
     // the kernel reads the Dirichlet flags/values from a SubConst block
     std::vector<uint8_t> host(subconst_bytes(g->n_substrates), 0);
@@ -445,7 +457,7 @@ int hc_dirichlet(hc_ctx* ctx, hc_buf* density, const hc_grid* g, const hc_substr
     hipError_t e = hipMalloc(&d, host.size());
     if (e != hipSuccess) return hip_fail(ctx, e, "hc_dirichlet: constants");
     e = hipMemcpyAsync(d, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream);
-    int rr = e == hipSuccess ? launch_dirichlet(ctx, (double*)density->dptr, g, (const uint8_t*)d)
+    int rr = e == hipSuccess ? launch_dirichlet(ctx, density->dptr, g, (const uint8_t*)d)
                              : hip_fail(ctx, e, "hc_dirichlet: upload");
     hipStreamSynchronize(ctx->stream);
     hipFree(d);
@@ -457,7 +469,7 @@ int hc_diffusion_substep(hc_ctx* ctx, hc_buf* density, const hc_grid* g, const h
     (void)subs;
     int r = check_grid_args(ctx, density, g, tables, "hc_diffusion_substep");
     if (r) return r;
-    double* rho = (double*)density->dptr;
+    void* rho = density->dptr;
     const uint8_t* tb = (const uint8_t*)tables->dptr;
     const bool three_d = g->nz > 1;
     if ((r = launch_sweep(ctx, 0, rho, g, tb, 0))) return r;

@@ -99,14 +99,41 @@ __host__ __device__ inline size_t serial_table_bytes(uint32_t ns, uint32_t nmax)
     return ((subconst_bytes(ns) + (size_t)ns * 3 * 2 * nmax * sizeof(double)) + 255) / 256 * 256;
 }
 __host__ __device__ inline size_t part_offset(uint32_t ns, uint32_t nmax) { return serial_table_bytes(ns, nmax); }
+// FP32 copies of the per-segment arrays (for FP32 fields), after the FP64 blocks
+__host__ __device__ inline size_t part_fblock(uint32_t nmax) {
+    const size_t a = (size_t)4 * 4 * part_mt(nmax, 4), b = (size_t)16 * 4 * part_mt(nmax, 16);
+    return ((a > b ? a : b) + 63) / 64 * 64;
+}
+__host__ __device__ inline size_t part_f_offset(uint32_t ns, uint32_t nmax) {
+    return part_offset(ns, nmax) + (size_t)ns * 3 * part_block(nmax) * sizeof(double);
+}
+__host__ __device__ inline size_t table_total_bytes(uint32_t ns, uint32_t nmax) {
+    return part_f_offset(ns, nmax) + (size_t)ns * 3 * part_fblock(nmax) * sizeof(float);
+}
+
+// ---- packed ensemble descriptors (batch.hip) ----
+struct BatchSim {            // one simulation of a packed ensemble
+    uint64_t rho_off;        // element offset of its density in the packed field buffer
+    uint64_t tab_off;        // byte offset of its table block in the packed table buffer
+    uint32_t nx, ny, nz, ns, nmax, _pad;
+};
+struct BatchWork {           // one line group of one substrate of one simulation
+    uint32_t sim, s;
+    uint64_t grp;
+};
 
 // ---- launchers implemented in diffusion.hip / cells.hip / sweep_part.hip ----
-int launch_sweep_part(hc_ctx* ctx, int axis, double* rho, const hc_grid* g, const uint8_t* tables,
+int launch_sweep_part(hc_ctx* ctx, int axis, void* rho, const hc_grid* g, const uint8_t* tables,
                       int fuse_dirichlet);   // returns 1 when the grid does not suit it (caller falls back)
-void build_part_tables_host(const hc_grid* g, const hc_substrate* subs, double* out);
-int launch_sweep(hc_ctx* ctx, int axis, double* rho, const hc_grid* g, const uint8_t* tables,
+int launch_sweep_batch(hc_ctx* ctx, int axis, void* rho_all, bool fp32, const uint8_t* tab_all, const BatchSim* sims,
+                       const BatchWork* work, uint32_t nwork, uint32_t nmax_all, uint32_t max_line, int fuse,
+                       unsigned grid_blocks);
+void build_part_tables_host(const hc_grid* g, const hc_substrate* subs, double* out, float* fout);
+void build_tables_host(const hc_grid* g, const hc_substrate* subs, std::vector<uint8_t>& out);
+inline size_t field_elem(const hc_grid* g) { return g->field_fp32 ? sizeof(float) : sizeof(double); }
+int launch_sweep(hc_ctx* ctx, int axis, void* rho, const hc_grid* g, const uint8_t* tables,
                  int fuse_dirichlet);
-int launch_dirichlet(hc_ctx* ctx, double* rho, const hc_grid* g, const uint8_t* tables);
+int launch_dirichlet(hc_ctx* ctx, void* rho, const hc_grid* g, const uint8_t* tables);
 
 // Hash workspace layout (byte offsets inside the hash buffer).
 struct HashLayout {

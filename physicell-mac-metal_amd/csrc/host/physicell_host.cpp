@@ -138,6 +138,7 @@ void Microenvironment::initialize(const SimConfig& config, hc_ctx* ctx) {
     grid_.x_min = config.x_min; grid_.y_min = config.y_min; grid_.z_min = config.z_min;
     grid_.dt = config.dt_diffusion;
     grid_.n_substrates = (uint32_t)config.substrates.size();
+    grid_.field_fp32 = config.mixed_precision ? 1u : 0u;
     if (!grid_.nx || !grid_.ny || !grid_.nz) throw std::runtime_error("Microenvironment: empty grid");
     subs_.resize(grid_.n_substrates);
     for (uint32_t s = 0; s < grid_.n_substrates; s++) {
@@ -146,11 +147,12 @@ void Microenvironment::initialize(const SimConfig& config, hc_ctx* ctx) {
                                 c.dirichlet_enabled ? 1u : 0u, 0u};
     }
     const size_t V = numVoxels();
-    density_.allocate(ctx, std::max<size_t>(8, V * grid_.n_substrates * sizeof(double)));
-    double* d = (double*)density_.host();
+    const size_t elem = grid_.field_fp32 ? sizeof(float) : sizeof(double);
+    density_.allocate(ctx, std::max<size_t>(8, V * grid_.n_substrates * elem));
+    host_density_.assign(V * grid_.n_substrates, 0.0);
     for (uint32_t s = 0; s < grid_.n_substrates; s++)
-        std::fill(d + s * V, d + (s + 1) * V, config.substrates[s].initial_value);
-    density_.upload_all();
+        std::fill(host_density_.begin() + s * V, host_density_.begin() + (s + 1) * V, config.substrates[s].initial_value);
+    syncToDevice();
     tables_.allocate(ctx, std::max<size_t>(8, hc_thomas_table_bytes(&grid_)));
     check(ctx, hc_build_thomas_tables(ctx, &grid_, subs_.data(), tables_.get()), "build Thomas tables");
 }
@@ -185,21 +187,42 @@ void Microenvironment::refreshVoxelIndices(CellData& cells) {
 
 double* Microenvironment::densityBuffer() {
     syncToHost();
-    return (double*)density_.host();
+    return host_density_.data();
 }
+// device field -> FP64 host copy
 void Microenvironment::syncToHost() {
     density_.download_all();
     check(ctx_, hc_wait(ctx_), "wait");
+    const size_t cnt = host_density_.size();
+    if (grid_.field_fp32) {
+        const float* f = (const float*)density_.host();
+        for (size_t i = 0; i < cnt; i++) host_density_[i] = (double)f[i];
+    } else {
+        std::memcpy(host_density_.data(), density_.host(), cnt * sizeof(double));
+    }
 }
-void Microenvironment::syncToDevice() { density_.upload_all(); }
+// FP64 host copy -> device field (rounded to FP32 in the mixed form)
+void Microenvironment::syncToDevice() {
+    const size_t cnt = host_density_.size();
+    if (grid_.field_fp32) {
+        float* f = (float*)density_.host();
+        for (size_t i = 0; i < cnt; i++) f[i] = (float)host_density_[i];
+    } else {
+        std::memcpy(density_.host(), host_density_.data(), cnt * sizeof(double));
+    }
+    density_.upload_all();
+    check(ctx_, hc_wait(ctx_), "wait");
+}
+
+void Microenvironment::totals(double* out) {
+    check(ctx_, hc_substrate_totals(ctx_, density_.get(), &grid_, out), "substrate totals");
+}
 
 double Microenvironment::getDensity(uint32_t v, uint32_t s) const {
-    const double* d = (const double*)hc_host_mirror(density_.get());
-    return d[(size_t)s * numVoxels() + v];
+    return host_density_[(size_t)s * numVoxels() + v];
 }
 void Microenvironment::setDensity(uint32_t v, uint32_t s, double value) {
-    double* d = (double*)hc_host_mirror(density_.get());
-    d[(size_t)s * numVoxels() + v] = value;
+    host_density_[(size_t)s * numVoxels() + v] = value;
 }
 
 // src/microenvironment.mm:176-184
@@ -227,6 +250,7 @@ void CellMechanics::initialize(const SimConfig& config, hc_ctx* ctx, uint32_t ma
     params_.mech_grid_nx = std::max(1u, (uint32_t)((config.x_max - config.x_min) / v));
     params_.mech_grid_ny = std::max(1u, (uint32_t)((config.y_max - config.y_min) / v));
     params_.mech_grid_nz = config.use_2D ? 1u : std::max(1u, (uint32_t)((config.z_max - config.z_min) / v));
+    params_.forces_fp32 = config.mixed_precision ? 1u : 0u;
     total_mech_voxels_ = params_.mech_grid_nx * params_.mech_grid_ny * params_.mech_grid_nz;
     // mech GridParams (src/cell_mechanics.mm:46-59): origin, mech dims, 30 um spacing
     mgrid_ = hc_grid{};
@@ -490,6 +514,7 @@ int pc_sim_create(pc_sim** out, const pc_config* cfg, int device) {
     c.dt_diffusion = cfg->dt_diffusion; c.dt_mechanics = cfg->dt_mechanics;
     c.dt_phenotype = cfg->dt_phenotype; c.max_time = cfg->max_time;
     c.max_cells = cfg->max_cells ? cfg->max_cells : 1;
+    c.mixed_precision = cfg->mixed_precision != 0;
     for (uint32_t s = 0; s < cfg->n_substrates; s++) {
         SubstrateConfig sc;
         sc.diffusion_coeff = cfg->substrates[s].diffusion_coeff;
@@ -548,10 +573,16 @@ int pc_sim_set_density(pc_sim* s, const double* in) {
     return guarded(s, [&](Simulation& m) {
         m.wait();
         const hc_grid g = m.microenv.getGridParams();
-        std::memcpy(hc_host_mirror(m.microenv.density()), in, (size_t)g.nx * g.ny * g.nz * g.n_substrates * sizeof(double));
+        const size_t V = (size_t)g.nx * g.ny * g.nz;
+        for (uint32_t sb = 0; sb < g.n_substrates; sb++)
+            for (size_t v = 0; v < V; v++) m.microenv.setDensity((uint32_t)v, sb, in[sb * V + v]);
         m.microenv.syncToDevice();
-        m.wait();
     });
+}
+
+int pc_sim_substrate_totals(pc_sim* s, double* out) {
+    if (!out) return HC_ERR_ARG;
+    return guarded(s, [&](Simulation& m) { m.microenv.totals(out); });
 }
 
 int pc_sim_get_field(pc_sim* s, uint32_t field, void* out) {

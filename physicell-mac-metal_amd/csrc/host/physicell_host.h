@@ -31,6 +31,7 @@ struct SimConfig {
     bool use_2D = true;
     double dt_diffusion = 0.01, dt_mechanics = 0.1, dt_phenotype = 6.0, max_time = 60.0;
     uint32_t max_cells = 1;
+    bool mixed_precision = false;      // FP32-mixed form
     std::vector<SubstrateConfig> substrates;
 };
 
@@ -127,7 +128,8 @@ public:
     hc_grid getGridParams() const { return grid_; }
     const std::vector<hc_substrate>& substrateParams() const { return subs_; }
 
-    double* densityBuffer();      // host mirror, refreshed from the device
+    double* densityBuffer();      // FP64 host copy, refreshed from the device
+    void totals(double* out);     // compensated per-substrate totals
     void syncToHost();
     void syncToDevice();
     hc_buf* density() const { return density_.get(); }
@@ -139,6 +141,7 @@ private:
     hc_grid grid_{};
     std::vector<hc_substrate> subs_;
     DeviceBuffer density_, tables_, plan_;
+    std::vector<double> host_density_;   // FP64 view (the device field may be FP32)
     uint32_t plan_capacity_ = 0;
 };
 

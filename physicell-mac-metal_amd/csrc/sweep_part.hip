@@ -1,10 +1,10 @@
-// sweep_part.hip -- partitioned LOD Thomas sweep (FP64, gfx950).
+// sweep_part.hip -- partitioned LOD Thomas sweep (gfx950), FP64 or FP32 fields.
 //
 // Solves the same per-line system as the serial sweep (edge rows with the
 // boundary diagonal 1+c+h and +c*T with Dirichlet; shaders/diffusion_2d.metal
 // :89-155) without the forward pass's d' ever leaving the chip:
 //
-//   * the N = n-2 unknowns of a line are split into 4 contiguous segments,
+//   * the N = n-2 unknowns of a line are split into P contiguous segments,
 //     one per lane; a lane keeps its segment in registers (<= MM values);
 //   * each lane solves its segment with the neighbouring unknowns set to zero
 //     (y = A_k^-1 d_k, elimination factors from a line-independent table);
@@ -12,15 +12,24 @@
 //     alpha_k = c A_k^-1 e_first, beta_k = c A_k^-1 e_last (tables), where uL
 //     and uR are the last unknown of the segment to the left and the first of
 //     the one to the right;
-//   * uL and uR follow from the 8 segment end values y of the line (gathered
-//     with cross-lane shuffles) through two rows of the inverse of the 8x8
+//   * uL and uR follow from the 2P segment end values y of the line (gathered
+//     with cross-lane shuffles) through two rows of the inverse of the 2P x 2P
 //     interface system, also a table.
 //
 // Every value is read once and written once (16 bytes per voxel per sweep in
-// FP64).  For y and z, 16 lines x 4 segments fill a wave and the lanes of one
-// segment walk 16 adjacent x positions, so each load is a full 128-byte row
-// segment.  x lines are contiguous: 4 lines x 16 segments fill a wave, which
-// copies its 4 lines (4*n contiguous values) through LDS with 16-byte loads.
+// FP64, 8 in FP32).  For y and z, 16 lines x 4 segments fill a wave and the
+// lanes of one segment walk 16 adjacent x positions, so each load is a full
+// row segment.  x lines are contiguous: 4 lines x 16 segments fill a wave,
+// which copies its 4 lines (4*n contiguous values) through LDS with 16-byte
+// loads.
+//
+// FP32 fields (the mixed form): values and the segment recurrence are FP32;
+// the elimination tables are computed in FP64 and rounded once; the Dirichlet
+// terms c*T and the interface solve (uL, uR) are evaluated in FP64.
+//
+// The same group solver serves one simulation (k_sweep_part: one workgroup
+// per line group) and a packed ensemble (k_sweep_part_batch: a persistent
+// grid walking a work list of (simulation, substrate, line group)).
 #include "hc_internal.h"
 
 #include <algorithm>
@@ -28,54 +37,61 @@
 
 namespace hc {
 
-// Contiguous copy of nval doubles between global memory and an LDS tile by
-// one wave.  The source may be only 8-byte aligned: a leading odd element is
-// copied alone so the rest moves as 16-byte pairs.  The loads of a lane (at
-// most MM/2 + 1 pairs, held in the caller's register window) are all issued
-// before the LDS writes.
-template <int MM>
-__device__ inline void stage_in(const double* __restrict__ src, double* tile, uint32_t nval, uint32_t lane,
-                                double (&reg)[MM + 2]) {
-    const uint32_t head = ((uintptr_t)src & 15u) ? 1u : 0u;
-    if (head && lane == 0 && nval) tile[0] = src[0];
-    const uint32_t npair = (nval - (nval ? head : 0)) / 2;
-    const double2* s2 = reinterpret_cast<const double2*>(src + head);
-    constexpr int R = MM / 2 + 1;
+template <typename T>
+struct alignas(16) Vec16 {
+    T v[16 / sizeof(T)];
+};
+
+// Contiguous copy of nval values between global memory and an LDS tile by
+// one wave.  A misaligned head and the tail are copied as scalars so the bulk
+// moves in 16-byte vectors; all of a lane's loads (at most RV vectors) are
+// issued before its first LDS write.
+template <typename T, int RV>
+__device__ inline void stage_in(const T* __restrict__ src, T* tile, uint32_t nval, uint32_t lane) {
+    constexpr uint32_t E = 16 / sizeof(T);
+    uint32_t head = (uint32_t)(((16u - ((uintptr_t)src & 15u)) & 15u) / sizeof(T));
+    if (head > nval) head = nval;
+    if (lane < head) tile[lane] = src[lane];
+    const uint32_t nvec = (nval - head) / E;
+    const Vec16<T>* s2 = reinterpret_cast<const Vec16<T>*>(src + head);
+    Vec16<T> buf[RV];
 #pragma unroll
-    for (int r = 0; r < R; r++) {
+    for (int r = 0; r < RV; r++) {
         const uint32_t e = lane + 64u * r;
-        if (e < npair) {
-            const double2 q = s2[e];
-            reg[2 * r] = q.x;
-            reg[2 * r + 1] = q.y;
-        }
+        if (e < nvec) buf[r] = s2[e];
     }
 #pragma unroll
-    for (int r = 0; r < R; r++) {
+    for (int r = 0; r < RV; r++) {
         const uint32_t e = lane + 64u * r;
-        if (e < npair) {
-            tile[head + 2 * e] = reg[2 * r];
-            tile[head + 2 * e + 1] = reg[2 * r + 1];
+        if (e < nvec) {
+#pragma unroll
+            for (uint32_t q = 0; q < E; q++) tile[head + E * e + q] = buf[r].v[q];
         }
     }
-    const uint32_t done = head + 2 * npair;
-    if (done < nval && lane == 0) tile[done] = src[done];
+    const uint32_t done = head + E * nvec;
+    if (lane < nval - done) tile[done + lane] = src[done + lane];
 }
 
-template <int MM>
-__device__ inline void stage_out(double* __restrict__ dst, const double* tile, uint32_t nval, uint32_t lane) {
-    const uint32_t head = ((uintptr_t)dst & 15u) ? 1u : 0u;
-    if (head && lane == 0 && nval) dst[0] = tile[0];
-    const uint32_t npair = (nval - (nval ? head : 0)) / 2;
-    double2* d2 = reinterpret_cast<double2*>(dst + head);
-    constexpr int R = MM / 2 + 1;
+template <typename T, int RV>
+__device__ inline void stage_out(T* __restrict__ dst, const T* tile, uint32_t nval, uint32_t lane) {
+    constexpr uint32_t E = 16 / sizeof(T);
+    uint32_t head = (uint32_t)(((16u - ((uintptr_t)dst & 15u)) & 15u) / sizeof(T));
+    if (head > nval) head = nval;
+    if (lane < head) dst[lane] = tile[lane];
+    const uint32_t nvec = (nval - head) / E;
+    Vec16<T>* d2 = reinterpret_cast<Vec16<T>*>(dst + head);
 #pragma unroll
-    for (int r = 0; r < R; r++) {
+    for (int r = 0; r < RV; r++) {
         const uint32_t e = lane + 64u * r;
-        if (e < npair) d2[e] = make_double2(tile[head + 2 * e], tile[head + 2 * e + 1]);
+        if (e < nvec) {
+            Vec16<T> q;
+#pragma unroll
+            for (uint32_t j = 0; j < E; j++) q.v[j] = tile[head + E * e + j];
+            d2[e] = q;
+        }
     }
-    const uint32_t done = head + 2 * npair;
-    if (done < nval && lane == 0) dst[done] = tile[done];
+    const uint32_t done = head + E * nvec;
+    if (lane < nval - done) dst[done + lane] = tile[done + lane];
 }
 
 // Each wave owns its LDS tile: lanes of one wave exchange data through it, so
@@ -86,35 +102,46 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int AXIS, int P, int MM, bool XLDS>
-__global__ __launch_bounds__(256) void k_sweep_part(
-    double* __restrict__ rho, const uint8_t* __restrict__ tables, uint32_t nx, uint32_t ny, uint32_t nz,
-    uint32_t nmax, int fuse_dirichlet) {
-    constexpr int LPW = 64 / P;                          // lines per wave
+// LDS image of one (substrate, axis) table block: interface rows (FP64) then
+// the per-segment arrays 1/den, c', alpha, beta (field type).
+template <int P, typename T>
+struct PartLds {
+    static __host__ __device__ size_t g_count() { return (size_t)P * 2 * 2 * P; }
+    static __host__ __device__ size_t bytes(uint32_t nmax) {
+        const size_t b = g_count() * sizeof(double) + (size_t)P * 4 * part_mt(nmax, P) * sizeof(T);
+        return (b + 15) / 16 * 16;
+    }
+};
+
+template <int P, typename T>
+__device__ inline void load_part_tables(const uint8_t* __restrict__ tables, uint32_t ns, uint32_t nmax, uint32_t s,
+                                        int axis, double* tG, T* tarr) {
+    const uint32_t MT = part_mt(nmax, P);
+    const size_t na = (size_t)P * 4 * MT;
+    const double* dblk = reinterpret_cast<const double*>(tables + part_offset(ns, nmax)) + (size_t)(s * 3 + axis) * part_block(nmax);
+    for (uint32_t q = threadIdx.x; q < PartLds<P, T>::g_count(); q += blockDim.x) tG[q] = dblk[na + q];
+    if (sizeof(T) == sizeof(double)) {
+        for (uint32_t q = threadIdx.x; q < na; q += blockDim.x) tarr[q] = (T)dblk[q];
+    } else {
+        const float* fblk = reinterpret_cast<const float*>(tables + part_f_offset(ns, nmax)) + (size_t)(s * 3 + axis) * part_fblock(nmax);
+        for (uint32_t q = threadIdx.x; q < na; q += blockDim.x) tarr[q] = (T)fblk[q];
+    }
+}
+
+// One group of 4*LPW lines (a workgroup of 4 waves) of one substrate of one
+// simulation.  rho points at that substrate's field.
+template <int AXIS, int P, int MM, bool XLDS, typename T>
+__device__ inline void sweep_group(T* __restrict__ rho, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t MT,
+                                   const T* tarr, const double* tG, double c, double T0, bool dir, int fuse_dirichlet,
+                                   uint64_t grp, T* tile_base) {
+    constexpr int LPW = 64 / P;
     constexpr int WPB = 4;
-    extern __shared__ double smem[];
-    const uint32_t s = blockIdx.y, ns = gridDim.y;
+    constexpr int RV = MM / (16 / (int)sizeof(T)) + 2;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t k = lane / LPW, l = lane % LPW;
     const uint32_t n = AXIS == 0 ? nx : (AXIS == 1 ? ny : nz);
     const uint32_t N = n - 2;
-    const uint32_t MT = part_mt(nmax, P);
-    const uint32_t blk = (uint32_t)part_block(nmax);
-    const uint64_t V = (uint64_t)nx * ny * nz;
-
-    // this (substrate, axis) table block -> LDS
-    const double* gt = reinterpret_cast<const double*>(tables + part_offset(ns, nmax)) + (size_t)(s * 3 + AXIS) * blk;
-    const uint32_t used = (uint32_t)part_block_p(nmax, P);
-    for (uint32_t q = threadIdx.x; q < used; q += blockDim.x) smem[q] = gt[q];
-    const double* tab = smem;
-
-    const uint64_t lines_total = AXIS == 0 ? (uint64_t)ny * nz : (AXIS == 1 ? (uint64_t)nz * nx : (uint64_t)ny * nx);
-    const uint64_t ngroups = (lines_total + (uint64_t)WPB * LPW - 1) / ((uint64_t)WPB * LPW);
-    __syncthreads();                                    // tables visible
-    // x blocks walk several line groups so their table copy is reused; y/z
-    // blocks take one group each
-    auto group = [&](uint64_t grp, const double* tabg) {
-    const uint64_t Lw0 = (grp * WPB + w) * LPW;     // first line of this wave
+    const uint64_t Lw0 = (grp * WPB + w) * LPW;          // first line of this wave
     const uint64_t L = Lw0 + l;
     uint64_t stride, base, lines;
     bool face = false;
@@ -132,68 +159,62 @@ __global__ __launch_bounds__(256) void k_sweep_part(
         face = (i == 0 || i == nx - 1 || j == 0 || j == ny - 1);
     }
     const bool valid = L < lines;
-    const SubConst* sc = reinterpret_cast<const SubConst*>(tables);
-    const double c = sc[s].c[AXIS];
-    const double T = sc[s].T;
-    const bool dir = sc[s].dir != 0;
     const bool fill = fuse_dirichlet && dir && face;
     const uint32_t sk = (k * N) / P;
     const uint32_t mk = ((k + 1) * N) / P - sk;
-    double* p = rho + (uint64_t)s * V + base;
+    T* p = rho + base;
+    T* tile = tile_base + (size_t)w * LPW * n;
+    const uint64_t rem = lines > Lw0 ? lines - Lw0 : 0;
+    const uint32_t nval = (uint32_t)((rem < (uint64_t)LPW ? rem : (uint64_t)LPW) * n);
 
-    double* tile = smem + blk + (size_t)w * LPW * n;
-
-    double v[MM + 2];
+    T v[MM];
     if (XLDS) {
-        // The wave's LPW consecutive x lines are LPW*n contiguous values: copy
-        // them with 16-byte loads, all issued before the first LDS write (the
-        // register window v[] is free at this point), then read the segments.
-        const uint64_t rem = lines > Lw0 ? lines - Lw0 : 0;
-        const uint32_t nval = (uint32_t)((rem < (uint64_t)LPW ? rem : (uint64_t)LPW) * n);
-        stage_in<MM>(rho + (uint64_t)s * V + Lw0 * nx, tile, nval, lane, v);
+        // the wave's LPW consecutive x lines are LPW*n contiguous values
+        stage_in<T, RV>(rho + Lw0 * nx, tile, nval, lane);
         wave_sync();
 #pragma unroll
-        for (int t = 0; t < MM; t++) v[t] = (t < (int)mk) ? tile[l * n + sk + 1 + t] : 0.0;
+        for (int t = 0; t < MM; t++) v[t] = (t < (int)mk) ? tile[l * n + sk + 1 + t] : (T)0;
     } else {
 #pragma unroll
-        for (int t = 0; t < MM; t++) v[t] = (t < (int)mk && valid) ? p[(uint64_t)(sk + 1 + t) * stride] : 0.0;
+        for (int t = 0; t < MM; t++) v[t] = (t < (int)mk && valid) ? p[(uint64_t)(sk + 1 + t) * stride] : (T)0;
     }
 
-    const double* idn = tabg + (k * 4 + 0) * MT;
-    const double* cpl = tabg + (k * 4 + 1) * MT;
-    const double* al = tabg + (k * 4 + 2) * MT;
-    const double* be = tabg + (k * 4 + 3) * MT;
-    const double* gl = tabg + (size_t)P * 4 * MT + (k * 2 + 0) * 2 * P;
-    const double* gr = tabg + (size_t)P * 4 * MT + (k * 2 + 1) * 2 * P;
+    const T* idn = tarr + (k * 4 + 0) * MT;
+    const T* cpl = tarr + (k * 4 + 1) * MT;
+    const T* al = tarr + (k * 4 + 2) * MT;
+    const T* be = tarr + (k * 4 + 3) * MT;
+    const double* gl = tG + (k * 2 + 0) * 2 * P;
+    const double* gr = tG + (k * 2 + 1) * 2 * P;
 
     if (dir) {
-        const double ct = c * T;
-        if (k == 0) v[0] += ct;
+        const double ct = c * T0;                           // Dirichlet term in FP64
+        if (k == 0) v[0] = (T)((double)v[0] + ct);
 #pragma unroll
         for (int t = 0; t < MM; t++)
-            if (k == P - 1 && t == (int)mk - 1) v[t] += ct;
+            if (k == P - 1 && t == (int)mk - 1) v[t] = (T)((double)v[t] + ct);
     }
     // segment solve with zero coupling
-    double dp = 0.0;
+    const T cc = (T)c;
+    T dp = 0;
 #pragma unroll
     for (int t = 0; t < MM; t++)
         if (t < (int)mk) {
-            dp = (v[t] + c * dp) * idn[t];
+            dp = (v[t] + cc * dp) * idn[t];
             v[t] = dp;
         }
-    double y = 0.0;
+    T y = 0;
 #pragma unroll
     for (int t = MM - 1; t >= 0; t--)
         if (t < (int)mk) {
             y = v[t] - cpl[t] * y;
             v[t] = y;
         }
-    const double yS = v[0];
+    const double yS = (double)v[0];
     double yE = 0.0;
 #pragma unroll
     for (int t = 0; t < MM; t++)
-        if (t == (int)mk - 1) yE = v[t];
-    // interface values of the line (its P lanes are l, l+LPW, l+2*LPW, ...)
+        if (t == (int)mk - 1) yE = (double)v[t];
+    // interface values of the line (its P lanes are l, l+LPW, l+2*LPW, ...), FP64
     double uL = 0.0, uR = 0.0;
 #pragma unroll
     for (int j = 0; j < P; j++) {
@@ -202,55 +223,126 @@ __global__ __launch_bounds__(256) void k_sweep_part(
         uL += gl[2 * j] * a + gl[2 * j + 1] * b;
         uR += gr[2 * j] * a + gr[2 * j + 1] * b;
     }
+    const T tuL = (T)uL, tuR = (T)uR;
 #pragma unroll
     for (int t = 0; t < MM; t++)
-        if (t < (int)mk) v[t] = v[t] + al[t] * uL + be[t] * uR;
+        if (t < (int)mk) v[t] = v[t] + al[t] * tuL + be[t] * tuR;
 
+    const T tT = (T)T0;
     if (XLDS) {
 #pragma unroll
         for (int t = 0; t < MM; t++)
             if (t < (int)mk) tile[l * n + sk + 1 + t] = v[t];
         if (dir) {
-            if (k == 0) tile[l * n] = T;
-            if (k == P - 1) tile[l * n + n - 1] = T;
+            if (k == 0) tile[l * n] = tT;
+            if (k == P - 1) tile[l * n + n - 1] = tT;
         }
         wave_sync();
-        const uint64_t rem = lines > Lw0 ? lines - Lw0 : 0;
-        const uint32_t nval = (uint32_t)((rem < (uint64_t)LPW ? rem : (uint64_t)LPW) * n);
-        stage_out<MM>(rho + (uint64_t)s * V + Lw0 * nx, tile, nval, lane);
-    } else {
-        if (valid) {
+        stage_out<T, RV>(rho + Lw0 * nx, tile, nval, lane);
+        wave_sync();                                      // tile reused by the next group
+    } else if (valid) {
 #pragma unroll
-            for (int t = 0; t < MM; t++)
-                if (t < (int)mk) p[(uint64_t)(sk + 1 + t) * stride] = fill ? T : v[t];
-            if (dir) {
-                if (k == 0) p[0] = T;
-                if (k == P - 1) p[(uint64_t)(n - 1) * stride] = T;
-            }
+        for (int t = 0; t < MM; t++)
+            if (t < (int)mk) p[(uint64_t)(sk + 1 + t) * stride] = fill ? tT : v[t];
+        if (dir) {
+            if (k == 0) p[0] = tT;
+            if (k == P - 1) p[(uint64_t)(n - 1) * stride] = tT;
         }
     }
-    if (XLDS) wave_sync();                          // tile reused by the next group
-    };
+}
+
+__host__ __device__ inline uint64_t sweep_lines(int axis, uint32_t nx, uint32_t ny, uint32_t nz) {
+    return axis == 0 ? (uint64_t)ny * nz : (axis == 1 ? (uint64_t)nz * nx : (uint64_t)ny * nx);
+}
+
+// ---------------------------------------------------------------------------
+// one simulation: workgroup (group, substrate); x workgroups loop over groups
+// ---------------------------------------------------------------------------
+template <int AXIS, int P, int MM, bool XLDS, typename T>
+__global__ __launch_bounds__(256) void k_sweep_part(T* __restrict__ rho, const uint8_t* __restrict__ tables,
+                                                    uint32_t nx, uint32_t ny, uint32_t nz, uint32_t nmax,
+                                                    int fuse_dirichlet) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const uint32_t s = blockIdx.y, ns = gridDim.y;
+    double* tG = reinterpret_cast<double*>(smem_raw);
+    T* tarr = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::g_count() * sizeof(double));
+    T* tile = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::bytes(nmax));
+    load_part_tables<P, T>(tables, ns, nmax, s, AXIS, tG, tarr);
+    const SubConst* sc = reinterpret_cast<const SubConst*>(tables);
+    const double c = sc[s].c[AXIS], T0 = sc[s].T;
+    const bool dir = sc[s].dir != 0;
+    const uint32_t MT = part_mt(nmax, P);
+    T* rs = rho + (uint64_t)s * nx * ny * nz;
+    __syncthreads();                                      // tables visible
     if (XLDS) {
+        const uint64_t ngroups = (sweep_lines(AXIS, nx, ny, nz) + 4 * (64 / P) - 1) / (4 * (64 / P));
         for (uint64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
             // keep the table reads inside the loop (hoisting them all into
             // registers would cost more registers than the window has)
             uint32_t toff = 0;
             asm volatile("" : "+v"(toff));
-            group(grp, tab + toff);
+            sweep_group<AXIS, P, MM, XLDS, T>(rs, nx, ny, nz, MT, tarr + toff, tG, c, T0, dir, fuse_dirichlet, grp,
+                                              tile);
         }
     } else {
-        group(blockIdx.x, tab);
+        sweep_group<AXIS, P, MM, XLDS, T>(rs, nx, ny, nz, MT, tarr, tG, c, T0, dir, fuse_dirichlet, blockIdx.x,
+                                          tile);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// packed ensemble: a persistent grid walks a work list; each workgroup takes a
+// contiguous run of items, so consecutive items mostly share their tables
+// ---------------------------------------------------------------------------
+template <int AXIS, int P, int MM, bool XLDS, typename T>
+__global__ __launch_bounds__(256) void k_sweep_part_batch(T* __restrict__ rho_all,
+                                                          const uint8_t* __restrict__ tab_all,
+                                                          const BatchSim* __restrict__ sims,
+                                                          const BatchWork* __restrict__ work, uint32_t nwork,
+                                                          uint32_t nmax_all, int fuse_dirichlet) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* tG = reinterpret_cast<double*>(smem_raw);
+    T* tarr = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::g_count() * sizeof(double));
+    T* tile = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::bytes(nmax_all));
+    const uint32_t per = (nwork + gridDim.x - 1) / gridDim.x;
+    const uint32_t w0 = blockIdx.x * per;
+    const uint32_t w1 = w0 + per < nwork ? w0 + per : nwork;
+    uint32_t cur_sim = 0xFFFFFFFFu, cur_s = 0xFFFFFFFFu;
+    for (uint32_t wi = w0; wi < w1; wi++) {
+        const BatchWork it = work[wi];
+        const BatchSim sm = sims[it.sim];
+        if (it.sim != cur_sim || it.s != cur_s) {
+            __syncthreads();                              // previous tables no longer read
+            load_part_tables<P, T>(tab_all + sm.tab_off, sm.ns, sm.nmax, it.s, AXIS, tG, tarr);
+            __syncthreads();
+            cur_sim = it.sim;
+            cur_s = it.s;
+        }
+        const SubConst* sc = reinterpret_cast<const SubConst*>(tab_all + sm.tab_off);
+        uint32_t toff = 0;
+        asm volatile("" : "+v"(toff));
+        T* rs = rho_all + sm.rho_off + (uint64_t)it.s * sm.nx * sm.ny * sm.nz;
+        sweep_group<AXIS, P, MM, XLDS, T>(rs, sm.nx, sm.ny, sm.nz, part_mt(sm.nmax, P), tarr + toff, tG,
+                                          sc[it.s].c[AXIS], sc[it.s].T, sc[it.s].dir != 0, fuse_dirichlet,
+                                          it.grp, tile);
     }
 }
 
 int probe_kernels_part(hc_ctx* ctx) {
     hipFuncAttributes attr;
-    const void* fns[] = {(const void*)k_sweep_part<0, 16, 16, true>, (const void*)k_sweep_part<0, 16, 32, true>,
-                         (const void*)k_sweep_part<0, 16, 16, false>, (const void*)k_sweep_part<0, 16, 32, false>,
-                         (const void*)k_sweep_part<1, 4, 16, false>, (const void*)k_sweep_part<1, 4, 32, false>,
-                         (const void*)k_sweep_part<1, 4, 64, false>, (const void*)k_sweep_part<2, 4, 16, false>,
-                         (const void*)k_sweep_part<2, 4, 32, false>, (const void*)k_sweep_part<2, 4, 64, false>};
+    const void* fns[] = {
+        (const void*)k_sweep_part<0, 16, 16, true, double>, (const void*)k_sweep_part<0, 16, 32, true, double>,
+        (const void*)k_sweep_part<0, 16, 16, false, double>, (const void*)k_sweep_part<0, 16, 32, false, double>,
+        (const void*)k_sweep_part<1, 4, 16, false, double>, (const void*)k_sweep_part<1, 4, 32, false, double>,
+        (const void*)k_sweep_part<1, 4, 64, false, double>, (const void*)k_sweep_part<2, 4, 16, false, double>,
+        (const void*)k_sweep_part<2, 4, 32, false, double>, (const void*)k_sweep_part<2, 4, 64, false, double>,
+        (const void*)k_sweep_part<0, 16, 16, true, float>, (const void*)k_sweep_part<0, 16, 32, true, float>,
+        (const void*)k_sweep_part<1, 4, 16, false, float>, (const void*)k_sweep_part<1, 4, 32, false, float>,
+        (const void*)k_sweep_part<1, 4, 64, false, float>, (const void*)k_sweep_part<2, 4, 16, false, float>,
+        (const void*)k_sweep_part<2, 4, 32, false, float>, (const void*)k_sweep_part<2, 4, 64, false, float>,
+        (const void*)k_sweep_part_batch<0, 16, 16, true, double>, (const void*)k_sweep_part_batch<1, 4, 32, false, double>,
+        (const void*)k_sweep_part_batch<2, 4, 32, false, double>, (const void*)k_sweep_part_batch<0, 16, 16, true, float>,
+        (const void*)k_sweep_part_batch<1, 4, 32, false, float>, (const void*)k_sweep_part_batch<2, 4, 32, false, float>};
     for (const void* f : fns) {
         hipError_t e = hipFuncGetAttributes(&attr, f);
         if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: partitioned sweep");
@@ -258,51 +350,102 @@ int probe_kernels_part(hc_ctx* ctx) {
     return 0;
 }
 
-template <int AXIS, int P, bool XLDS>
-static void launch_part_axis(hc_ctx* ctx, int mm, dim3 grid, dim3 block, size_t shm, double* rho,
-                             const uint8_t* tables, const hc_grid* g, uint32_t nmax, int fuse) {
-    if (mm == 16)
-        hipLaunchKernelGGL((k_sweep_part<AXIS, P, 16, XLDS>), grid, block, shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
-    else if (mm == 32)
-        hipLaunchKernelGGL((k_sweep_part<AXIS, P, 32, XLDS>), grid, block, shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
-    else if (P == 4)
-        hipLaunchKernelGGL((k_sweep_part<AXIS, P, 64, XLDS>), grid, block, shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
+// window size for a line of n nodes split into P segments (0 = does not fit)
+static int part_window(uint32_t n, int P) {
+    if (n < 2 + 2 * (uint32_t)P) return 0;
+    const uint32_t m = (n - 2 + P - 1) / P;
+    return m <= 16 ? 16 : (m <= 32 ? 32 : (m <= 64 && P == 4 ? 64 : 0));
 }
 
-int launch_sweep_part(hc_ctx* ctx, int axis, double* rho, const hc_grid* g, const uint8_t* tables,
-                      int fuse_dirichlet) {
+template <int AXIS, int P, bool XLDS, typename T>
+static void launch_part_axis(hc_ctx* ctx, int mm, dim3 grid, size_t shm, T* rho, const uint8_t* tables,
+                             const hc_grid* g, uint32_t nmax, int fuse) {
+    if (mm == 16)
+        hipLaunchKernelGGL((k_sweep_part<AXIS, P, 16, XLDS, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
+    else if (mm == 32)
+        hipLaunchKernelGGL((k_sweep_part<AXIS, P, 32, XLDS, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
+    else if (P == 4)
+        hipLaunchKernelGGL((k_sweep_part<AXIS, 4, 64, false, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
+}
+
+template <typename T>
+static int launch_sweep_part_t(hc_ctx* ctx, int axis, T* rho, const hc_grid* g, const uint8_t* tables,
+                               int fuse_dirichlet) {
     const int P = part_p(axis);
     const uint32_t n = axis == 0 ? g->nx : (axis == 1 ? g->ny : g->nz);
-    if (n < 2 + 2 * (uint32_t)P) return 1;                  // too short to partition
-    const uint32_t m = (n - 2 + P - 1) / P;                  // longest segment
-    const int mm = m <= 16 ? 16 : (m <= 32 ? 32 : (m <= 64 && P == 4 ? 64 : 0));
+    const int mm = part_window(n, P);
     if (!mm) return 1;
     const uint32_t nmax = grid_nmax(g);
     const int LPW = 64 / P;
     const bool xlds = axis == 0 && ctx->sweep_mode != 2;
-    const size_t shm = (part_block(nmax) + (xlds ? (size_t)4 * LPW * n : 0)) * sizeof(double);
+    const size_t tab_bytes = P == 16 ? PartLds<16, T>::bytes(nmax) : PartLds<4, T>::bytes(nmax);
+    const size_t shm = tab_bytes + (xlds ? (size_t)4 * LPW * n * sizeof(T) : 0);
     if (shm > 64 * 1024) return 1;
-    const uint64_t lines = axis == 0 ? (uint64_t)g->ny * g->nz
-                                     : (axis == 1 ? (uint64_t)g->nz * g->nx : (uint64_t)g->ny * g->nx);
-    // x: enough blocks to fill the chip a few times over, each looping over
-    // line groups; y/z: one block per group
-    const unsigned groups = blocks_for(lines, LPW * 4);
+    // x: enough workgroups to fill the chip a few times over, each looping over
+    // line groups; y/z: one workgroup per group
+    const unsigned groups = blocks_for(sweep_lines(axis, g->nx, g->ny, g->nz), LPW * 4);
     const unsigned per_sub = (unsigned)std::max<uint64_t>(1, 4096 / std::max<uint32_t>(1, g->n_substrates));
     dim3 grid(xlds ? std::min(groups, per_sub) : groups, g->n_substrates);
-    dim3 block(256);
     KernelTimer tm(ctx, HC_K_SWEEP_X + axis);
     if (axis == 0) {
-        if (xlds) launch_part_axis<0, 16, true>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
-        else launch_part_axis<0, 16, false>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
+        if (xlds) launch_part_axis<0, 16, true, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet);
+        else launch_part_axis<0, 16, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet);
     } else if (axis == 1) {
-        launch_part_axis<1, 4, false>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
+        launch_part_axis<1, 4, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet);
     } else {
-        launch_part_axis<2, 4, false>(ctx, mm, grid, block, shm, rho, tables, g, nmax, fuse_dirichlet);
+        launch_part_axis<2, 4, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet);
     }
     return check_launch(ctx, "k_sweep_part");
 }
 
-// Host construction of the partitioned tables (FP64).
+int launch_sweep_part(hc_ctx* ctx, int axis, void* rho, const hc_grid* g, const uint8_t* tables, int fuse) {
+    if (g->field_fp32) return launch_sweep_part_t<float>(ctx, axis, (float*)rho, g, tables, fuse);
+    return launch_sweep_part_t<double>(ctx, axis, (double*)rho, g, tables, fuse);
+}
+
+template <typename T>
+static int launch_batch_t(hc_ctx* ctx, int axis, T* rho_all, const uint8_t* tab_all, const BatchSim* sims,
+                          const BatchWork* work, uint32_t nwork, uint32_t nmax_all, uint32_t max_line, int fuse,
+                          unsigned grid_blocks) {
+    const int P = part_p(axis);
+    const int mm = part_window(max_line, P);
+    if (nwork == 0) return 0;
+    if (!mm) return fail(ctx, HC_ERR_CAPACITY, "batched sweep: line length outside the partitioned window");
+    const int LPW = 64 / P;
+    const size_t tab_bytes = P == 16 ? PartLds<16, T>::bytes(nmax_all) : PartLds<4, T>::bytes(nmax_all);
+    const size_t shm = tab_bytes + (axis == 0 ? (size_t)4 * LPW * max_line * sizeof(T) : 0);
+    if (shm > 64 * 1024) return fail(ctx, HC_ERR_CAPACITY, "batched sweep: LDS");
+    dim3 grid(std::min<unsigned>(grid_blocks, nwork));
+    KernelTimer tm(ctx, HC_K_SWEEP_X + axis);
+#define HC_BATCH(AX, PP, M, X)                                                                                   \
+    hipLaunchKernelGGL((k_sweep_part_batch<AX, PP, M, X, T>), grid, dim3(256), shm, ctx->stream, rho_all, tab_all, \
+                       sims, work, nwork, nmax_all, fuse)
+    if (axis == 0) {
+        if (mm != 16) return fail(ctx, HC_ERR_CAPACITY, "batched x sweep: lines longer than 258");
+        HC_BATCH(0, 16, 16, true);
+    } else if (axis == 1) {
+        if (mm > 32) return fail(ctx, HC_ERR_CAPACITY, "batched y sweep: lines longer than 130");
+        HC_BATCH(1, 4, 32, false);
+    } else {
+        if (mm > 32) return fail(ctx, HC_ERR_CAPACITY, "batched z sweep: lines longer than 130");
+        HC_BATCH(2, 4, 32, false);
+    }
+#undef HC_BATCH
+    return check_launch(ctx, "k_sweep_part_batch");
+}
+
+int launch_sweep_batch(hc_ctx* ctx, int axis, void* rho_all, bool fp32, const uint8_t* tab_all, const BatchSim* sims,
+                       const BatchWork* work, uint32_t nwork, uint32_t nmax_all, uint32_t max_line, int fuse,
+                       unsigned grid_blocks) {
+    if (fp32)
+        return launch_batch_t<float>(ctx, axis, (float*)rho_all, tab_all, sims, work, nwork, nmax_all, max_line, fuse,
+                                     grid_blocks);
+    return launch_batch_t<double>(ctx, axis, (double*)rho_all, tab_all, sims, work, nwork, nmax_all, max_line, fuse,
+                                  grid_blocks);
+}
+
+// Host construction of the partitioned tables (FP64, plus an FP32 rounding
+// of the per-segment arrays for FP32 fields).
 static void solve_local(const std::vector<double>& den, const std::vector<double>& cpl, double c,
                         const std::vector<double>& d, std::vector<double>& out) {
     const size_t m = den.size();
@@ -347,9 +490,9 @@ static bool invert(std::vector<double>& A, int n, std::vector<double>& inv) {
     return true;
 }
 
-void build_part_tables_host(const hc_grid* g, const hc_substrate* subs, double* out) {
+void build_part_tables_host(const hc_grid* g, const hc_substrate* subs, double* out, float* fout) {
     const uint32_t ns = g->n_substrates, nmax = grid_nmax(g);
-    const size_t blk = part_block(nmax);
+    const size_t blk = part_block(nmax), fblk = part_fblock(nmax);
     const double sp[3] = {g->dx, g->dy, g->dz};
     const uint32_t nn[3] = {g->nx, g->ny, g->nz};
     for (uint32_t s = 0; s < ns; s++) {
@@ -359,9 +502,11 @@ void build_part_tables_host(const hc_grid* g, const hc_substrate* subs, double* 
             const int P = part_p(ax);
             const uint32_t MT = part_mt(nmax, P);
             double* B = out + (size_t)(s * 3 + ax) * blk;
+            float* F = fout + (size_t)(s * 3 + ax) * fblk;
             std::fill(B, B + blk, 0.0);
+            std::fill(F, F + fblk, 0.0f);
             const uint32_t n = nn[ax];
-            if (n < 2 + 2 * P) continue;
+            if (n < 2 + 2 * (uint32_t)P) continue;
             const uint32_t N = n - 2;
             const double c = D * g->dt / (sp[ax] * sp[ax]);
             const double bI = 1.0 + 2.0 * c + h, bB = 1.0 + c + h;
@@ -383,10 +528,11 @@ void build_part_tables_host(const hc_grid* g, const hc_substrate* subs, double* 
                 solve_local(den, cpl, c, e0, al[k]);
                 solve_local(den, cpl, c, eL, be[k]);
                 for (uint32_t t = 0; t < mk; t++) {
-                    B[(k * 4 + 0) * MT + t] = 1.0 / den[t];
-                    B[(k * 4 + 1) * MT + t] = cpl[t];
-                    B[(k * 4 + 2) * MT + t] = al[k][t];
-                    B[(k * 4 + 3) * MT + t] = be[k][t];
+                    const double vals[4] = {1.0 / den[t], cpl[t], al[k][t], be[k][t]};
+                    for (int q = 0; q < 4; q++) {
+                        B[(k * 4 + q) * MT + t] = vals[q];
+                        F[(k * 4 + q) * MT + t] = (float)vals[q];
+                    }
                 }
             }
             // interface system over z = (L0, R0, L1, R1, ...): L_k = u(first row of

@@ -44,6 +44,7 @@ class HipError(RuntimeError):
 
 class Grid(ctypes.Structure):
     _fields_ = [("nx", c_u32), ("ny", c_u32), ("nz", c_u32), ("n_substrates", c_u32),
+                ("field_fp32", c_u32), ("_pad", c_u32),
                 ("dx", c_f64), ("dy", c_f64), ("dz", c_f64), ("dt", c_f64),
                 ("x_min", c_f64), ("y_min", c_f64), ("z_min", c_f64)]
 
@@ -57,7 +58,7 @@ class Mech(ctypes.Structure):
     _fields_ = [("dt", c_f64), ("cell_cell_repulsion", c_f64), ("cell_cell_adhesion", c_f64),
                 ("max_interaction_distance", c_f64), ("mechanics_voxel_size", c_f64),
                 ("mech_grid_nx", c_u32), ("mech_grid_ny", c_u32), ("mech_grid_nz", c_u32),
-                ("_pad", c_u32)]
+                ("forces_fp32", c_u32)]
 
 
 class SubstrateCfg(ctypes.Structure):
@@ -70,7 +71,7 @@ class Config(ctypes.Structure):
                 ("z_min", c_f64), ("z_max", c_f64), ("dx", c_f64), ("dy", c_f64), ("dz", c_f64),
                 ("use_2D", c_u32), ("n_substrates", c_u32), ("dt_diffusion", c_f64),
                 ("dt_mechanics", c_f64), ("dt_phenotype", c_f64), ("max_time", c_f64),
-                ("max_cells", c_u32), ("_pad", c_u32), ("substrates", ctypes.POINTER(SubstrateCfg))]
+                ("max_cells", c_u32), ("mixed_precision", c_u32), ("substrates", ctypes.POINTER(SubstrateCfg))]
 
 
 _lib = None
@@ -123,6 +124,7 @@ def load(path=LIB_PATH):
         "hc_hash_max_bin": (c_i32, [c_vp, c_vp, c_u32, P(c_u32)]),
         "hc_neighbor_lists": (c_i32, [c_vp, c_vp, c_vp, P(Mech), c_u32, c_u32, c_u32, c_u32, c_vp, c_vp]),
         "hc_set_sweep_mode": (c_i32, [c_vp, c_i32]),
+        "hc_substrate_totals": (c_i32, [c_vp, c_vp, P(Grid), c_vp]),
         "hc_timing_enable": (c_i32, [c_vp, c_u32]),
         "hc_timing_get": (c_i32, [c_vp, c_i32, P(c_f64), P(ctypes.c_int64)]),
         "hc_timing_reset": (c_i32, [c_vp]),
@@ -135,6 +137,7 @@ def load(path=LIB_PATH):
         "pc_sim_wait": (c_i32, [c_vp]), "pc_sim_get_density": (c_i32, [c_vp, c_vp]),
         "pc_sim_set_density": (c_i32, [c_vp, c_vp]), "pc_sim_get_field": (c_i32, [c_vp, c_u32, c_vp]),
         "pc_sim_get_hash": (c_i32, [c_vp, c_vp, c_vp]),
+        "pc_sim_substrate_totals": (c_i32, [c_vp, c_vp]),
         "pc_sim_info": (c_i32, [c_vp, P(Grid), P(Mech), P(c_u32)]),
         "pc_sim_context": (c_vp, [c_vp]),
     }
@@ -154,19 +157,19 @@ def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
 
 
-def grid_of(spec):
-    return Grid(spec.nx, spec.ny, spec.nz, len(spec.substrates), spec.dx, spec.dy, spec.dz,
+def grid_of(spec, fp32=False):
+    return Grid(spec.nx, spec.ny, spec.nz, len(spec.substrates), 1 if fp32 else 0, 0, spec.dx, spec.dy, spec.dz,
                 spec.dt_diff, spec.x_min, spec.y_min, spec.z_min)
 
 
 def mech_grid_of(spec):
     v = spec.mech_voxel
-    return Grid(spec.mnx, spec.mny, spec.mnz, 0, v, v, v, 0.0, spec.x_min, spec.y_min, spec.z_min)
+    return Grid(spec.mnx, spec.mny, spec.mnz, 0, 0, 0, v, v, v, 0.0, spec.x_min, spec.y_min, spec.z_min)
 
 
-def mech_of(spec):
+def mech_of(spec, fp32=False):
     v = spec.mech_voxel
-    return Mech(spec.dt_mech, 10.0, 0.4, v, v, spec.mnx, spec.mny, spec.mnz, 0)
+    return Mech(spec.dt_mech, 10.0, 0.4, v, v, spec.mnx, spec.mny, spec.mnz, 1 if fp32 else 0)
 
 
 def substrates_of(spec):
@@ -292,7 +295,7 @@ class CellBuffer:
 class Simulation:
     """pc_sim: the C++ step loop on one device."""
 
-    def __init__(self, spec, cells, device=0, max_cells=None):
+    def __init__(self, spec, cells, device=0, max_cells=None, mixed=False):
         lib = load()
         self.spec = spec
         ns = len(spec.substrates)
@@ -302,7 +305,7 @@ class Simulation:
         cfg = Config(spec.x_min, spec.x_max, spec.y_min, spec.y_max, spec.z_min, spec.z_max,
                      spec.dx, spec.dy, spec.dz, 1 if spec.use_2d else 0, ns, spec.dt_diff,
                      spec.dt_mech, spec.dt_pheno, spec.max_time,
-                     max_cells or max(1, cells.n), 0, self._subs)
+                     max_cells or max(1, cells.n), 1 if mixed else 0, self._subs)
         h = c_vp()
         rc = lib.pc_sim_create(ctypes.byref(h), ctypes.byref(cfg), device)
         if rc:
@@ -365,6 +368,11 @@ class Simulation:
         cells = np.empty(max(1, self.n), np.uint32)
         self._check(load().pc_sim_get_hash(self.h, _ptr(off), _ptr(cells)), "pc_sim_get_hash")
         return off, cells[:self.n]
+
+    def totals(self):
+        out = np.empty(max(1, self.ns))
+        self._check(load().pc_sim_substrate_totals(self.h, _ptr(out)), "pc_sim_substrate_totals")
+        return out[:self.ns]
 
     def context(self):
         return load().pc_sim_context(self.h)

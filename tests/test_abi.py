@@ -27,6 +27,6 @@ def test_sizes_are_consistent():
     assert lib.hc_cells_field_offset(23, M) == 23 * 8 * M
     assert lib.hc_cells_field_offset(27, M) == 23 * 8 * M + 4 * 4 * M
     assert lib.hc_rates_bytes(M, 3) == 4 * M * 3 * 8
-    g = H.Grid(10, 20, 30, 2, 1, 1, 1, 0.01, 0, 0, 0)
+    g = H.Grid(10, 20, 30, 2, 0, 0, 1, 1, 1, 0.01, 0, 0, 0)
     assert lib.hc_thomas_table_bytes(ctypes.byref(g)) >= 2 * 3 * 2 * 30 * 8
     assert lib.hc_hash_bytes(100, M) > 64 * M
