@@ -131,7 +131,7 @@ __device__ inline void load_part_tables(const uint8_t* __restrict__ tables, uint
 // One group of 4*LPW lines (a workgroup of 4 waves) of one substrate of one
 // simulation.  rho points at that substrate's field.
 template <int AXIS, int P, int MM, bool XLDS, typename T>
-__device__ inline void sweep_group(T* __restrict__ rho, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t MT,
+__device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict__ rho, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t MT,
                                    const T* tarr, const double* tG, double c, double T0, bool dir, int fuse_dirichlet,
                                    uint64_t grp, T* tile_base) {
     constexpr int LPW = 64 / P;
@@ -281,8 +281,8 @@ __global__ __launch_bounds__(256) void k_sweep_part(T* __restrict__ rho, const u
             // registers would cost more registers than the window has)
             uint32_t toff = 0;
             asm volatile("" : "+v"(toff));
-            sweep_group<AXIS, P, MM, XLDS, T>(rs, nx, ny, nz, MT, tarr + toff, tG, c, T0, dir, fuse_dirichlet, grp,
-                                              tile);
+            sweep_group<AXIS, P, MM, XLDS, T>(rs, nx, ny, nz, MT, tarr + toff, tG + toff, c, T0, dir, fuse_dirichlet,
+                                              grp, tile);
         }
     } else {
         sweep_group<AXIS, P, MM, XLDS, T>(rs, nx, ny, nz, MT, tarr, tG, c, T0, dir, fuse_dirichlet, blockIdx.x,
@@ -322,7 +322,7 @@ __global__ __launch_bounds__(256) void k_sweep_part_batch(T* __restrict__ rho_al
         uint32_t toff = 0;
         asm volatile("" : "+v"(toff));
         T* rs = rho_all + sm.rho_off + (uint64_t)it.s * sm.nx * sm.ny * sm.nz;
-        sweep_group<AXIS, P, MM, XLDS, T>(rs, sm.nx, sm.ny, sm.nz, part_mt(sm.nmax, P), tarr + toff, tG,
+        sweep_group<AXIS, P, MM, XLDS, T>(rs, sm.nx, sm.ny, sm.nz, part_mt(sm.nmax, P), tarr + toff, tG + toff,
                                           sc[it.s].c[AXIS], sc[it.s].T, sc[it.s].dir != 0, fuse_dirichlet,
                                           it.grp, tile);
     }
