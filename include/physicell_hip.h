@@ -307,6 +307,51 @@ int pc_sim_substrate_totals(pc_sim* sim, double* out);
 int pc_sim_info(const pc_sim* sim, hc_grid* grid, hc_mech* mech, uint32_t* n_cells);
 hc_ctx* pc_sim_context(pc_sim* sim);
 
+/* ==== pe_*: packed ensembles of independent simulations ====================
+ * No counterpart in the reference (which runs one simulation per process):
+ * many simulations of uneven grid and population size are stepped together.
+ * Each member keeps its own fields, tables and cells on the device; the step
+ * walks packed work lists (sweep line groups, agent tiles, one hash and one
+ * secretion plan over the concatenated bin/voxel spaces).  A member's results
+ * do not depend on the other members: they equal the pc_sim results of the
+ * same inputs.  Members share dt_diffusion and dt_mechanics; a period is
+ * (substeps per mechanics step) diffusion substeps + one mechanics step; each
+ * member's phenotype refresh (voxel keys + secretion plan) runs at the end of
+ * the periods where its own dt_phenotype clock fires.  Grid limits: x edge
+ * 18..258, y and z edges 6..130 (or nz = 1) voxels. */
+typedef struct pe_ensemble pe_ensemble;
+
+int  pe_create(pe_ensemble** out, int device_ordinal, uint32_t mixed_precision);
+void pe_destroy(pe_ensemble* e);
+const char* pe_last_error(const pe_ensemble* e);
+/* Adds a member (cells as pc_sim_set_cells; max_cells = n); returns its id. */
+int pe_add(pe_ensemble* e, const pc_config* cfg, uint32_t n, const double* x, const double* y, const double* z,
+           const double* radius, const double* volume, const double* repulsion, const double* adhesion,
+           const double* rel_max_adhesion, const double* secretion, const double* uptake,
+           const double* saturation, const double* net_export, const uint32_t* alive, uint32_t* member_id);
+/* Removes a member and frees its device state (waits for queued work). */
+int pe_remove(pe_ensemble* e, uint32_t member_id);
+/* Runs `periods` periods of every member; use_graph != 0 captures one period
+ * as a hipGraph (re-captured after membership changes) and replays it. */
+int pe_run_periods(pe_ensemble* e, int periods, int use_graph);
+int pe_wait(pe_ensemble* e);
+/* Member results (wait first): density as pc_sim_get_density, field as
+ * pc_sim_get_field. */
+int pe_get_density(pe_ensemble* e, uint32_t member_id, double* out);
+int pe_get_field(pe_ensemble* e, uint32_t member_id, uint32_t field, void* out);
+/* bytes: the member's device bytes; counts (int64[3]): substeps, mechanics
+ * steps, phenotype refreshes done. */
+int pe_member_info(pe_ensemble* e, uint32_t member_id, uint64_t* bytes, int64_t* counts);
+int pe_info(pe_ensemble* e, uint32_t* n_members, uint64_t* total_cells, uint64_t* total_voxel_values,
+            uint64_t* workspace_bytes);
+/* Persistent-grid size (workgroups) of the batched sweep along axis 0..2. */
+int pe_set_grid_blocks(pe_ensemble* e, int axis, uint32_t blocks);
+/* Autotuner: times each axis's batched sweep at candidate grid sizes on the
+ * current members (fields restored afterwards), keeps the fastest; writes the
+ * chosen sizes and their mean launch times (us). */
+int pe_autotune(pe_ensemble* e, int reps, uint32_t* blocks_out, double* us_out);
+hc_ctx* pe_context(pe_ensemble* e);
+
 #ifdef __cplusplus
 }
 #endif

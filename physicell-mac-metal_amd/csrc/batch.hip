@@ -1,0 +1,1065 @@
+// batch.hip -- packed ensembles: many simulations of uneven grid and
+// population size stepped together (gfx950).
+//
+// Layout.  Each member keeps its own density field, tables, cell SoA and rate
+// arrays (so members can join and leave between steps); everything the step
+// iterates over is packed:
+//   * sweep work lists: one entry per (member, substrate, line group), walked
+//     by one persistent grid per axis (k_sweep_part_batch, sweep_part.hip);
+//   * agent tiles: one entry per (member, 256 consecutive agents);
+//   * the hash: one counting sort over the concatenated bin space of all
+//     members (member m's mech voxel v is global bin bin_off[m] + v), so keys,
+//     counts, the ascending list and the hash-sorted records are packed arrays
+//     over all agents;
+//   * the secretion plan: one counting sort over the concatenated voxel space.
+// A member's arithmetic does not depend on the other members: its results are
+// identical alone and in any mix (and identical to the single-simulation path).
+#include "cells_common.h"
+#include "host/physicell_host.h"
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+namespace hc {
+
+struct EnsSim {
+    double* soa;            // member cell SoA (hc_cells_bytes(max_cells))
+    const double* rates;    // member rate arrays
+    void* rho;              // member density field
+    const uint8_t* tables;  // member Thomas tables
+    uint32_t max_cells, n, ns, nmax;
+    uint32_t nx, ny, nz, V;
+    double x_min, y_min, z_min, dx, dy, dz, dt_diff;
+    double mvoxel, max_dist, dt_mech;
+    uint32_t mnx, mny, mnz, _pad;
+    uint32_t cell_off, bin_off, vox_off, _pad2;
+};
+
+struct AgentTile {
+    uint32_t sim, c0;
+};
+
+__device__ inline double* sim_field(const EnsSim& m, int f) { return m.soa + (size_t)f * m.max_cells; }
+__device__ inline uint32_t* sim_ufield(const EnsSim& m, int f) {
+    return reinterpret_cast<uint32_t*>(m.soa + (size_t)HC_NUM_F64_FIELDS * m.max_cells) +
+           (size_t)(f - HC_NUM_F64_FIELDS) * m.max_cells;
+}
+
+// mech keys (shaders/mechanics.metal:61-72) into the global bin space
+__global__ __launch_bounds__(256) void k_b_mech_keys(const EnsSim* __restrict__ sims,
+                                                     const AgentTile* __restrict__ tiles,
+                                                     uint32_t* __restrict__ counts, uint32_t* __restrict__ kglob,
+                                                     uint32_t* __restrict__ slot) {
+    const AgentTile tl = tiles[blockIdx.x];
+    const EnsSim m = sims[tl.sim];
+    const uint32_t c = tl.c0 + threadIdx.x;
+    const bool valid = c < m.n;
+    uint32_t key = 0;
+    if (valid) {
+        const double* x = sim_field(m, HC_F_POS_X);
+        const double* y = sim_field(m, HC_F_POS_Y);
+        const double* z = sim_field(m, HC_F_POS_Z);
+        int vi = (int)((x[c] - m.x_min) / m.mvoxel);
+        int vj = (int)((y[c] - m.y_min) / m.mvoxel);
+        int vk = (int)((z[c] - m.z_min) / m.mvoxel);
+        vi = vi < 0 ? 0 : (vi > (int)m.mnx - 1 ? (int)m.mnx - 1 : vi);
+        vj = vj < 0 ? 0 : (vj > (int)m.mny - 1 ? (int)m.mny - 1 : vj);
+        vk = vk < 0 ? 0 : (vk > (int)m.mnz - 1 ? (int)m.mnz - 1 : vk);
+        const uint32_t local = (uint32_t)vk * m.mny * m.mnx + (uint32_t)vj * m.mnx + (uint32_t)vi;
+        sim_ufield(m, HC_F_MECH_VOXEL)[c] = local;
+        key = m.bin_off + local;
+        kglob[m.cell_off + c] = key;
+    }
+    const uint32_t s = ballot_bin(valid, key, counts);
+    if (valid) slot[m.cell_off + c] = s;
+}
+
+// stable placement + hash-sorted records (FP64 or bin-relative FP32)
+template <bool FP32>
+__global__ __launch_bounds__(256) void k_b_gather(const EnsSim* __restrict__ sims, const uint32_t* __restrict__ agent_sim,
+                                                  const uint32_t* __restrict__ kglob, const uint32_t* __restrict__ offsets,
+                                                  const uint32_t* __restrict__ tmp, uint32_t* __restrict__ sorted,
+                                                  void* __restrict__ recs, uint32_t ntot) {
+    const uint32_t g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= ntot) return;
+    const EnsSim m = sims[agent_sim[g]];
+    const uint32_t c = g - m.cell_off;
+    const uint32_t b = kglob[g];
+    const uint32_t beg = offsets[b], end = offsets[b + 1];
+    const uint32_t pos = beg + stable_rank(tmp, beg, end, g);
+    sorted[pos] = g;
+    const double px = sim_field(m, HC_F_POS_X)[c], py = sim_field(m, HC_F_POS_Y)[c], pz = sim_field(m, HC_F_POS_Z)[c];
+    if (FP32) {
+        const uint32_t lb = b - m.bin_off;
+        const uint32_t bi = lb % m.mnx, bj = (lb / m.mnx) % m.mny, bk = lb / (m.mnx * m.mny);
+        CellRec32 r;
+        r.x = (float)(px - (m.x_min + bi * m.mvoxel));
+        r.y = (float)(py - (m.y_min + bj * m.mvoxel));
+        r.z = (float)(pz - (m.z_min + bk * m.mvoxel));
+        r.r = (float)sim_field(m, HC_F_RADIUS)[c];
+        r.rep = (float)sim_field(m, HC_F_REPULSION)[c];
+        r.adh = (float)sim_field(m, HC_F_ADHESION)[c];
+        r.idx = g;
+        r.key = b;
+        reinterpret_cast<CellRec32*>(recs)[pos] = r;
+    } else {
+        CellRec r;
+        r.x = px;
+        r.y = py;
+        r.z = pz;
+        r.r = sim_field(m, HC_F_RADIUS)[c];
+        r.rep = sim_field(m, HC_F_REPULSION)[c];
+        r.adh = sim_field(m, HC_F_ADHESION)[c];
+        r.rel = sim_field(m, HC_F_REL_MAX_ADH)[c];
+        r.idx = g;
+        r.key = b;
+        reinterpret_cast<CellRec*>(recs)[pos] = r;
+    }
+}
+
+template <bool FP32>
+__global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sims, const uint32_t* __restrict__ agent_sim,
+                                                  const void* __restrict__ recs, const uint32_t* __restrict__ offsets,
+                                                  uint32_t ntot) {
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (t >= ntot) return;
+    uint32_t idx, key;
+    if (FP32) {
+        const CellRec32 me = reinterpret_cast<const CellRec32*>(recs)[t];
+        idx = me.idx;
+        key = me.key;
+    } else {
+        const CellRec me = reinterpret_cast<const CellRec*>(recs)[t];
+        idx = me.idx;
+        key = me.key;
+    }
+    const EnsSim m = sims[agent_sim[idx]];
+    const uint32_t c = idx - m.cell_off;
+    const uint32_t lk = key - m.bin_off;
+    const uint32_t vi = lk % m.mnx, vj = (lk / m.mnx) % m.mny, vk = lk / (m.mnx * m.mny);
+    const int dk_min = (m.mnz > 1) ? -1 : 0, dk_max = (m.mnz > 1) ? 1 : 0;
+    double ofx = 0.0, ofy = 0.0, ofz = 0.0, opr = 0.0;
+    if (FP32) {
+        const CellRec32 me = reinterpret_cast<const CellRec32*>(recs)[t];
+        const CellRec32* rec = reinterpret_cast<const CellRec32*>(recs);
+        const float rel = (float)sim_field(m, HC_F_REL_MAX_ADH)[c];
+        const float cut = (float)m.max_dist, vox = (float)m.mvoxel;
+        float fx = 0.f, fy = 0.f, fz = 0.f, pr = 0.f;
+        for (int dk = dk_min; dk <= dk_max; dk++) {
+            const int nk = (int)vk + dk;
+            if (nk < 0 || nk >= (int)m.mnz) continue;
+            for (int dj = -1; dj <= 1; dj++) {
+                const int nj = (int)vj + dj;
+                if (nj < 0 || nj >= (int)m.mny) continue;
+                for (int di = -1; di <= 1; di++) {
+                    const int ni = (int)vi + di;
+                    if (ni < 0 || ni >= (int)m.mnx) continue;
+                    const uint32_t nb = m.bin_off + (uint32_t)nk * m.mny * m.mnx + (uint32_t)nj * m.mnx + (uint32_t)ni;
+                    force_bin32(rec, offsets[nb], offsets[nb + 1], t, me.x, me.y, me.z, (float)di * vox,
+                                (float)dj * vox, (float)dk * vox, me.r, me.rep, me.adh, rel, cut * cut, fx, fy, fz, pr);
+                }
+            }
+        }
+        ofx = fx; ofy = fy; ofz = fz; opr = pr;
+    } else {
+        const CellRec me = reinterpret_cast<const CellRec*>(recs)[t];
+        const CellRec* rec = reinterpret_cast<const CellRec*>(recs);
+        const double cut2 = m.max_dist * m.max_dist;
+        double fx = 0.0, fy = 0.0, fz = 0.0, pr = 0.0;
+        for (int dk = dk_min; dk <= dk_max; dk++) {
+            const int nk = (int)vk + dk;
+            if (nk < 0 || nk >= (int)m.mnz) continue;
+            for (int dj = -1; dj <= 1; dj++) {
+                const int nj = (int)vj + dj;
+                if (nj < 0 || nj >= (int)m.mny) continue;
+                for (int di = -1; di <= 1; di++) {
+                    const int ni = (int)vi + di;
+                    if (ni < 0 || ni >= (int)m.mnx) continue;
+                    const uint32_t nb = m.bin_off + (uint32_t)nk * m.mny * m.mnx + (uint32_t)nj * m.mnx + (uint32_t)ni;
+                    force_bin64(rec, offsets[nb], offsets[nb + 1], t, me.x, me.y, me.z, me.r, me.rep, me.adh, me.rel,
+                                cut2, fx, fy, fz, pr);
+                }
+            }
+        }
+        ofx = fx; ofy = fy; ofz = fz; opr = pr;
+    }
+    sim_field(m, HC_F_VEL_X)[c] = ofx;
+    sim_field(m, HC_F_VEL_Y)[c] = ofy;
+    sim_field(m, HC_F_VEL_Z)[c] = ofz;
+    sim_field(m, HC_F_PRESSURE)[c] = opr;
+}
+
+// AB2 (shaders/integrate.metal:56-79), clamp to the member's mech-grid extent
+__global__ __launch_bounds__(256) void k_b_integrate(const EnsSim* __restrict__ sims,
+                                                     const AgentTile* __restrict__ tiles) {
+    const AgentTile tl = tiles[blockIdx.x];
+    const EnsSim m = sims[tl.sim];
+    const uint32_t i = tl.c0 + threadIdx.x;
+    if (i >= m.n) return;
+    const double x_max = m.x_min + (double)m.mnx * m.mvoxel;
+    const double y_max = m.y_min + (double)m.mny * m.mvoxel;
+    const double z_max = m.z_min + (double)m.mnz * m.mvoxel;
+    const double dt = m.dt_mech;
+    double* X = sim_field(m, HC_F_POS_X);
+    double* Y = sim_field(m, HC_F_POS_Y);
+    double* Z = sim_field(m, HC_F_POS_Z);
+    double* PX = sim_field(m, HC_F_PREV_VX);
+    double* PY = sim_field(m, HC_F_PREV_VY);
+    double* PZ = sim_field(m, HC_F_PREV_VZ);
+    const double vx = sim_field(m, HC_F_VEL_X)[i], vy = sim_field(m, HC_F_VEL_Y)[i], vz = sim_field(m, HC_F_VEL_Z)[i];
+    double px = X[i], py = Y[i], pz = Z[i];
+    px += dt * (1.5 * vx - 0.5 * PX[i]);
+    py += dt * (1.5 * vy - 0.5 * PY[i]);
+    pz += dt * (1.5 * vz - 0.5 * PZ[i]);
+    px = px < m.x_min ? m.x_min : (px > x_max ? x_max : px);
+    py = py < m.y_min ? m.y_min : (py > y_max ? y_max : py);
+    pz = pz < m.z_min ? m.z_min : (pz > z_max ? z_max : pz);
+    X[i] = px; Y[i] = py; Z[i] = pz;
+    PX[i] = vx; PY[i] = vy; PZ[i] = vz;
+}
+
+// secretion plan (src/cell_phenotype.cpp:526-564) over the global voxel space
+__global__ __launch_bounds__(256) void k_b_sec_keys(const EnsSim* __restrict__ sims, const AgentTile* __restrict__ tiles,
+                                                    uint32_t* __restrict__ counts, uint32_t* __restrict__ kv,
+                                                    uint32_t* __restrict__ slot, uint32_t vtot) {
+    const AgentTile tl = tiles[blockIdx.x];
+    const EnsSim m = sims[tl.sim];
+    const uint32_t c = tl.c0 + threadIdx.x;
+    const bool valid = c < m.n;
+    uint32_t key = vtot;
+    if (valid) {
+        const uint32_t v = sim_ufield(m, HC_F_VOXEL_IDX)[c];
+        if (sim_ufield(m, HC_F_IS_ALIVE)[c] && v < m.V) key = m.vox_off + v;
+        kv[m.cell_off + c] = key;
+    }
+    const uint32_t s = ballot_bin(valid, key, counts);
+    if (valid) slot[m.cell_off + c] = s;
+}
+
+__global__ __launch_bounds__(256) void k_b_sec_groups(const EnsSim* __restrict__ sims,
+                                                      const uint32_t* __restrict__ agent_sim,
+                                                      const uint32_t* __restrict__ kv, const uint32_t* __restrict__ sorted,
+                                                      const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ gid,
+                                                      uint32_t* __restrict__ gsim, uint32_t* __restrict__ gvox,
+                                                      uint32_t* __restrict__ gbegin, double* __restrict__ coef,
+                                                      PlanHeader* __restrict__ hdr, uint32_t ntot, uint32_t vtot,
+                                                      uint32_t ns_max) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t n_elig = offsets[vtot];
+    if (t == 0) {
+        hdr->n_elig = n_elig;
+        hdr->ngroups = gid[ntot];
+        hdr->n = ntot;
+        gbegin[gid[ntot]] = n_elig;
+    }
+    if (t >= n_elig) return;
+    const uint32_t g = sorted[t];
+    const uint32_t simi = agent_sim[g];
+    const EnsSim m = sims[simi];
+    const uint32_t c = g - m.cell_off;
+    const uint32_t v = kv[g];
+    if (t == 0 || kv[sorted[t - 1]] != v) {
+        const uint32_t q = gid[t];
+        gsim[q] = simi;
+        gvox[q] = v - m.vox_off;
+        gbegin[q] = t;
+    }
+    const size_t stride = (size_t)m.max_cells * m.ns;
+    const double* sec = m.rates;
+    const double* upt = m.rates + stride;
+    const double* sat = m.rates + 2 * stride;
+    const double* net = m.rates + 3 * stride;
+    const double V_voxel = m.dx * m.dy * m.dz;
+    const double dt = m.dt_diff;
+    const double V_cell = sim_field(m, HC_F_TOTAL_VOL)[c];
+    const double internal_constant = dt * V_cell / V_voxel;
+    bool any_net = false;
+    for (uint32_t s = 0; s < m.ns; s++) {
+        const size_t off = (size_t)c * m.ns + s;
+        const double a = dt * net[off] / V_voxel;
+        const double temp1 = internal_constant * sec[off] * sat[off];
+        const double temp2 = 1.0 + internal_constant * (sec[off] + upt[off]);
+        coef[((size_t)s * 3 + 0) * ntot + t] = a;
+        coef[((size_t)s * 3 + 1) * ntot + t] = temp1;
+        coef[((size_t)s * 3 + 2) * ntot + t] = temp2;
+        any_net |= (a != 0.0);
+    }
+    (void)ns_max;
+    if (any_net) atomicOr(&hdr->has_net, 1u);
+}
+
+template <typename F>
+__global__ __launch_bounds__(256) void k_b_sec_apply(const EnsSim* __restrict__ sims, const uint32_t* __restrict__ gsim,
+                                                     const uint32_t* __restrict__ gvox, const uint32_t* __restrict__ gbegin,
+                                                     const double* __restrict__ coef, const PlanHeader* __restrict__ hdr,
+                                                     uint32_t ntot) {
+    const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= hdr->ngroups) return;
+    const bool has_net = hdr->has_net != 0;
+    const EnsSim m = sims[gsim[q]];
+    F* rho = reinterpret_cast<F*>(m.rho);
+    const uint32_t v = gvox[q];
+    const uint32_t b = gbegin[q], e = gbegin[q + 1];
+    for (uint32_t s = 0; s < m.ns; s++) {
+        const double* ca = coef + ((size_t)s * 3 + 0) * ntot;
+        const double* c1 = coef + ((size_t)s * 3 + 1) * ntot;
+        const double* c2 = coef + ((size_t)s * 3 + 2) * ntot;
+        double r = (double)rho[(uint64_t)s * m.V + v];
+        for (uint32_t t = b; t < e; t++) {
+            if (has_net) r += ca[t];
+            r = (r + c1[t]) / c2[t];
+            if (r < 0.0) r = 0.0;
+        }
+        rho[(uint64_t)s * m.V + v] = (F)r;
+    }
+}
+
+int probe_kernels_batch(hc_ctx* ctx) {
+    hipFuncAttributes attr;
+    const void* fns[] = {(const void*)k_b_mech_keys, (const void*)k_b_gather<false>,
+                         (const void*)k_b_gather<true>, (const void*)k_b_forces<false>, (const void*)k_b_forces<true>,
+                         (const void*)k_b_integrate, (const void*)k_b_sec_keys, (const void*)k_b_sec_groups,
+                         (const void*)k_b_sec_apply<double>, (const void*)k_b_sec_apply<float>};
+    for (const void* f : fns) {
+        hipError_t e = hipFuncGetAttributes(&attr, f);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: ensemble kernel");
+    }
+    return 0;
+}
+
+}  // namespace hc
+
+// ============================================================================
+// Host side: the ensemble (members, packing, step loop) and the pe_* C-ABI
+// ============================================================================
+extern "C" hipStream_t hc_stream_of(hc_ctx* ctx);
+
+namespace physicell {
+
+using hc::AgentTile;
+using hc::BatchSim;
+using hc::BatchWork;
+using hc::EnsSim;
+
+struct Member {
+    uint32_t id = 0;
+    SimConfig cfg;
+    hc_grid grid{};
+    std::vector<hc_substrate> subs;
+    hc_mech mech{};
+    hc_grid mgrid{};
+    uint32_t n = 0, max_cells = 0;
+    DeviceBuffer density, tables, soa, rates;
+    // the member's own schedule (the step loop of src/main.mm:275-415)
+    double mech_acc = 0.0, pheno_acc = 0.0;
+    int64_t substeps = 0, mech_steps = 0, pheno_steps = 0;
+    size_t bytes() const { return density.size() + tables.size() + soa.size() + rates.size(); }
+};
+
+class Ensemble {
+public:
+    Ensemble(int device, bool mixed) : mixed_(mixed) {
+        int rc = hc_create(&ctx_, device);
+        if (rc) throw std::runtime_error("ensemble: no usable HIP device");
+        rc = hc_load_kernels(ctx_, nullptr);
+        if (rc) {
+            std::string m = hc_last_error(ctx_);
+            hc_destroy(ctx_);
+            throw std::runtime_error("ensemble: " + m);
+        }
+    }
+    ~Ensemble() {
+        drop_graph();
+        hc_wait(ctx_);
+        members_.clear();
+        ws_ = Workspace();
+        hc_destroy(ctx_);
+    }
+
+    uint32_t add(const SimConfig& c_in, uint32_t n, const double* x, const double* y, const double* z,
+                 const double* radius, const double* volume, const double* rep, const double* adh, const double* rel,
+                 const double* sec, const double* upt, const double* sat, const double* net, const uint32_t* alive) {
+        auto m = std::make_unique<Member>();
+        m->id = next_id_++;
+        m->cfg = c_in;
+        m->cfg.mixed_precision = mixed_;
+        SimConfig& c = m->cfg;
+        hc_grid& g = m->grid;
+        g.nx = (uint32_t)((c.x_max - c.x_min) / c.dx);
+        g.ny = (uint32_t)((c.y_max - c.y_min) / c.dy);
+        g.nz = c.use_2D ? 1u : (uint32_t)((c.z_max - c.z_min) / c.dz);
+        g.dx = c.dx; g.dy = c.dy; g.dz = c.dz;
+        g.x_min = c.x_min; g.y_min = c.y_min; g.z_min = c.z_min;
+        g.dt = c.dt_diffusion;
+        g.n_substrates = (uint32_t)c.substrates.size();
+        g.field_fp32 = mixed_ ? 1u : 0u;
+        if (!g.nx || !g.ny || !g.nz) throw std::runtime_error("ensemble: empty grid");
+        if (!grid_fits(g))
+            throw std::runtime_error("ensemble: grid edges must be 18..258 (x) and 6..130 (y, z) voxels; "
+                                     "run other grids as single simulations");
+        check_schedule(c);
+        m->subs.resize(g.n_substrates);
+        for (uint32_t s = 0; s < g.n_substrates; s++)
+            m->subs[s] = hc_substrate{c.substrates[s].diffusion_coeff, c.substrates[s].decay_rate, 0.0,
+                                      c.substrates[s].dirichlet_value, c.substrates[s].dirichlet_enabled ? 1u : 0u, 0u};
+        const double v = 30.0;
+        m->mech.dt = c.dt_mechanics;
+        m->mech.cell_cell_repulsion = 10.0;
+        m->mech.cell_cell_adhesion = 0.4;
+        m->mech.max_interaction_distance = v;
+        m->mech.mechanics_voxel_size = v;
+        m->mech.mech_grid_nx = std::max(1u, (uint32_t)((c.x_max - c.x_min) / v));
+        m->mech.mech_grid_ny = std::max(1u, (uint32_t)((c.y_max - c.y_min) / v));
+        m->mech.mech_grid_nz = c.use_2D ? 1u : std::max(1u, (uint32_t)((c.z_max - c.z_min) / v));
+        m->mech.forces_fp32 = mixed_ ? 1u : 0u;
+        m->mgrid = hc_grid{};
+        m->mgrid.nx = m->mech.mech_grid_nx; m->mgrid.ny = m->mech.mech_grid_ny; m->mgrid.nz = m->mech.mech_grid_nz;
+        m->mgrid.dx = m->mgrid.dy = m->mgrid.dz = v;
+        m->mgrid.x_min = c.x_min; m->mgrid.y_min = c.y_min; m->mgrid.z_min = c.z_min;
+        // device state
+        const size_t V = (size_t)g.nx * g.ny * g.nz;
+        const size_t elem = mixed_ ? sizeof(float) : sizeof(double);
+        m->density.allocate(ctx_, std::max<size_t>(8, V * g.n_substrates * elem));
+        if (mixed_) {
+            float* d = (float*)m->density.host();
+            for (uint32_t s = 0; s < g.n_substrates; s++)
+                std::fill(d + s * V, d + (s + 1) * V, (float)c.substrates[s].initial_value);
+        } else {
+            double* d = (double*)m->density.host();
+            for (uint32_t s = 0; s < g.n_substrates; s++)
+                std::fill(d + s * V, d + (s + 1) * V, c.substrates[s].initial_value);
+        }
+        m->density.upload_all();
+        m->tables.allocate(ctx_, hc_thomas_table_bytes(&g));
+        check(ctx_, hc_build_thomas_tables(ctx_, &g, m->subs.data(), m->tables.get()), "ensemble tables");
+        m->n = n;
+        m->max_cells = std::max(1u, n);
+        m->soa.allocate(ctx_, hc_cells_bytes(m->max_cells));
+        m->rates.allocate(ctx_, std::max<size_t>(8, hc_rates_bytes(m->max_cells, g.n_substrates)));
+        char* h = (char*)m->soa.host();
+        auto F = [&](uint32_t f) { return (double*)(h + hc_cells_field_offset(f, m->max_cells)); };
+        auto U = [&](uint32_t f) { return (uint32_t*)(h + hc_cells_field_offset(f, m->max_cells)); };
+        std::memset(h, 0, m->soa.size());
+        for (uint32_t i = 0; i < n; i++) {
+            F(HC_F_POS_X)[i] = x[i]; F(HC_F_POS_Y)[i] = y[i]; F(HC_F_POS_Z)[i] = z[i];
+            F(HC_F_RADIUS)[i] = radius[i]; F(HC_F_TOTAL_VOL)[i] = volume[i];
+            F(HC_F_REPULSION)[i] = rep[i]; F(HC_F_ADHESION)[i] = adh[i]; F(HC_F_REL_MAX_ADH)[i] = rel[i];
+            U(HC_F_IS_ALIVE)[i] = alive ? alive[i] : 1u;
+            // placement voxel, 3-D formula (src/microenvironment.mm:176-184)
+            int ii = (int)((x[i] - g.x_min) / g.dx), jj = (int)((y[i] - g.y_min) / g.dy), kk = (int)((z[i] - g.z_min) / g.dz);
+            ii = ii < 0 ? 0 : (ii >= (int)g.nx ? (int)g.nx - 1 : ii);
+            jj = jj < 0 ? 0 : (jj >= (int)g.ny ? (int)g.ny - 1 : jj);
+            kk = kk < 0 ? 0 : (kk >= (int)g.nz ? (int)g.nz - 1 : kk);
+            U(HC_F_VOXEL_IDX)[i] = (uint32_t)kk * g.nx * g.ny + (uint32_t)jj * g.nx + (uint32_t)ii;
+        }
+        double* r = (double*)m->rates.host();
+        const size_t stride = (size_t)m->max_cells * g.n_substrates;
+        std::memset(r, 0, m->rates.size());
+        for (size_t i = 0; i < (size_t)n * g.n_substrates; i++) {
+            r[i] = sec ? sec[i] : 0.0;
+            r[stride + i] = upt ? upt[i] : 0.0;
+            r[2 * stride + i] = sat ? sat[i] : 0.0;
+            r[3 * stride + i] = net ? net[i] : 0.0;
+        }
+        m->soa.upload_all();
+        m->rates.upload_all();
+        check(ctx_, hc_wait(ctx_), "wait");
+        const uint32_t id = m->id;
+        members_.push_back(std::move(m));
+        dirty_ = true;
+        plan_dirty_ = true;
+        return id;
+    }
+
+    void remove(uint32_t id) {
+        auto it = std::find_if(members_.begin(), members_.end(), [&](const std::unique_ptr<Member>& m) { return m->id == id; });
+        if (it == members_.end()) throw std::runtime_error("ensemble: unknown member");
+        check(ctx_, hc_wait(ctx_), "wait");
+        members_.erase(it);
+        dirty_ = true;
+        plan_dirty_ = true;
+    }
+
+    // one period: substeps_per_mech diffusion substeps, one mechanics step,
+    // then each member's own phenotype-cadence refresh when it fires
+    void run_periods(int periods, bool use_graph) {
+        rebuild();
+        if (members_.empty()) return;
+        for (int p = 0; p < periods; p++) {
+            if (use_graph) {
+                if (!graph_) capture();
+                hipError_t e = hipGraphLaunch((hipGraphExec_t)graph_, hc_stream_of(ctx_));
+                if (e != hipSuccess) throw std::runtime_error(std::string("ensemble graph launch: ") + hipGetErrorString(e));
+            } else {
+                period_kernels();
+            }
+            advance_clocks();
+        }
+    }
+
+    void wait() { check(ctx_, hc_wait(ctx_), "wait"); }
+
+    Member& member(uint32_t id) {
+        for (auto& m : members_)
+            if (m->id == id) return *m;
+        throw std::runtime_error("ensemble: unknown member");
+    }
+
+    void get_density(uint32_t id, double* out) {
+        Member& m = member(id);
+        wait();
+        m.density.download_all();
+        wait();
+        const size_t cnt = (size_t)m.grid.nx * m.grid.ny * m.grid.nz * m.grid.n_substrates;
+        if (mixed_) {
+            const float* f = (const float*)m.density.host();
+            for (size_t i = 0; i < cnt; i++) out[i] = f[i];
+        } else {
+            std::memcpy(out, m.density.host(), cnt * sizeof(double));
+        }
+    }
+
+    void get_field(uint32_t id, uint32_t field, void* out) {
+        Member& m = member(id);
+        wait();
+        const size_t w = field < HC_NUM_F64_FIELDS ? 8 : 4;
+        const size_t off = hc_cells_field_offset(field, m.max_cells);
+        m.soa.download(off, (size_t)m.n * w);
+        wait();
+        std::memcpy(out, (char*)m.soa.host() + off, (size_t)m.n * w);
+    }
+
+    size_t member_bytes(uint32_t id) { return member(id).bytes(); }
+    size_t workspace_bytes() const { return ws_.bytes; }
+    size_t n_members() const { return members_.size(); }
+    uint64_t total_cells() const { uint64_t t = 0; for (auto& m : members_) t += m->n; return t; }
+    uint64_t total_voxel_values() const {
+        uint64_t t = 0;
+        for (auto& m : members_) t += (uint64_t)m->grid.nx * m->grid.ny * m->grid.nz * m->grid.n_substrates;
+        return t;
+    }
+    int substeps_per_mech() const {
+        if (members_.empty()) return 1;
+        double acc = 0.0;
+        const double dtd = members_[0]->cfg.dt_diffusion, dtm = members_[0]->cfg.dt_mechanics;
+        for (int k = 1; k < 1000000; k++) {
+            acc += dtd;
+            if (acc >= dtm - 0.5 * dtd) return k;
+        }
+        return 1;
+    }
+    void set_grid_blocks(int axis, unsigned b) {
+        if (axis < 0 || axis > 2) throw std::runtime_error("ensemble: axis must be 0..2");
+        grid_blocks_[axis] = b ? b : 1024;
+        drop_graph();
+    }
+    unsigned grid_blocks(int axis) const { return grid_blocks_[axis]; }
+    hc_ctx* ctx() { return ctx_; }
+    const std::vector<std::unique_ptr<Member>>& members() const { return members_; }
+
+    // Times the batched sweep of each axis at candidate persistent-grid sizes
+    // (mean over `reps` launches after one warm-up) and keeps the fastest.
+    // The sweeps advance the fields, so they are saved first and put back.
+    void autotune(int reps, uint32_t blocks_out[3], double us_out[3]) {
+        rebuild();
+        drop_graph();
+        hipStream_t st = hc_stream_of(ctx_);
+        std::vector<DeviceBuffer> saved(members_.size());
+        for (size_t i = 0; i < members_.size(); i++) {
+            saved[i].allocate(ctx_, members_[i]->density.size());
+            hipError_t e = hipMemcpyAsync(hc_device_ptr(saved[i].get()), hc_device_ptr(members_[i]->density.get()),
+                                          members_[i]->density.size(), hipMemcpyDeviceToDevice, st);
+            if (e != hipSuccess) throw std::runtime_error(std::string("autotune save: ") + hipGetErrorString(e));
+        }
+        hipEvent_t a, b;
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        const unsigned cand[] = {256, 512, 1024, 2048, 4096, 8192};
+        Workspace& w = ws_;
+        const BatchSim* bs = (const BatchSim*)hc_device_ptr(w.bsims.get());
+        for (int ax = 0; ax < 3; ax++) {
+            blocks_out[ax] = grid_blocks_[ax];
+            us_out[ax] = 0.0;
+            if (!w.nwork[ax]) continue;
+            double best = 1e300;
+            for (unsigned c : cand) {
+                if (c > 2 * w.nwork[ax] && c != cand[0]) break;
+                auto launch = [&]() {
+                    check(ctx_, hc::launch_sweep_batch(ctx_, ax, mixed_, bs,
+                                                       (const BatchWork*)hc_device_ptr(w.work[ax].get()), w.nwork[ax],
+                                                       w.nmax_all, w.max_line[ax], c),
+                          "autotune sweep");
+                };
+                launch();
+                hipEventRecord(a, st);
+                for (int r = 0; r < reps; r++) launch();
+                hipEventRecord(b, st);
+                hipEventSynchronize(b);
+                float ms = 0.f;
+                hipEventElapsedTime(&ms, a, b);
+                const double us = 1e3 * ms / std::max(1, reps);
+                if (us < best) {
+                    best = us;
+                    blocks_out[ax] = c;
+                }
+            }
+            us_out[ax] = best;
+            grid_blocks_[ax] = blocks_out[ax];
+        }
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+        for (size_t i = 0; i < members_.size(); i++) {
+            hipError_t e = hipMemcpyAsync(hc_device_ptr(members_[i]->density.get()), hc_device_ptr(saved[i].get()),
+                                          members_[i]->density.size(), hipMemcpyDeviceToDevice, st);
+            if (e != hipSuccess) throw std::runtime_error(std::string("autotune restore: ") + hipGetErrorString(e));
+        }
+        check(ctx_, hc_wait(ctx_), "wait");
+    }
+    std::string last_error;
+
+private:
+    struct Workspace {
+        DeviceBuffer sims, bsims, tiles, work[3], agent_sim, kglob, slot, tmp, sorted, recs, counts, offsets, partial;
+        DeviceBuffer pcounts, poffsets, pkv, pslot, ptmp, psorted, pflag, pgsim, pgvox, pgbegin, pcoef, phdr, ppartial;
+        uint32_t ntot = 0, nbins = 0, vtot = 0, ntiles = 0, nwork[3] = {0, 0, 0}, ns_max = 0, nmax_all = 0;
+        uint32_t max_line[3] = {0, 0, 0};
+        size_t bytes = 0;
+    };
+
+    // the batched sweeps hold up to 16 values per segment along x (16
+    // segments) and 32 along y and z (4 segments); every segment needs one
+    static bool grid_fits(const hc_grid& g) {
+        return g.nx >= 18 && g.nx <= 258 && g.ny >= 6 && g.ny <= 130 && (g.nz == 1 || (g.nz >= 6 && g.nz <= 130));
+    }
+
+    static int substeps_until(double dtd, double dt) {
+        double acc = 0.0;
+        for (int k = 1; k < 1000000; k++) {
+            acc += dtd;
+            if (acc >= dt - 0.5 * dtd) return k;
+        }
+        return 1000000;
+    }
+
+    // Members step in lockstep periods of k diffusion substeps + one mechanics
+    // step, so they share dt_diffusion and dt_mechanics; each member's
+    // phenotype cadence (its voxel refresh + secretion re-plan) must fall on a
+    // period end, which is where the single-simulation loop runs it too.
+    void check_schedule(const SimConfig& c) const {
+        if (!members_.empty()) {
+            const SimConfig& f = members_.front()->cfg;
+            if (f.dt_diffusion != c.dt_diffusion || f.dt_mechanics != c.dt_mechanics)
+                throw std::runtime_error("ensemble: members must share dt_diffusion and dt_mechanics");
+        }
+        const int k = substeps_until(c.dt_diffusion, c.dt_mechanics);
+        const int kp = substeps_until(c.dt_diffusion, c.dt_phenotype);
+        if (kp % k) throw std::runtime_error("ensemble: dt_phenotype must be a whole number of mechanics periods");
+    }
+
+    void drop_graph() {
+        if (graph_) {
+            hipGraphExecDestroy((hipGraphExec_t)graph_);
+            graph_ = nullptr;
+        }
+    }
+
+    void rebuild() {
+        if (!dirty_) return;
+        drop_graph();
+        check(ctx_, hc_wait(ctx_), "wait");        // queued work may still read the old workspace
+        Workspace& w = ws_;
+        std::vector<EnsSim> sims;
+        std::vector<BatchSim> bsims;
+        std::vector<AgentTile> tiles;
+        std::vector<BatchWork> work[3];
+        std::vector<uint32_t> agent_sim;
+        uint32_t cell_off = 0, bin_off = 0, vox_off = 0, ns_max = 0, nmax_all = 0;
+        uint32_t max_line[3] = {0, 0, 0};
+        for (uint32_t si = 0; si < members_.size(); si++) {
+            Member& m = *members_[si];
+            const hc_grid& g = m.grid;
+            EnsSim e{};
+            e.soa = (double*)hc_device_ptr(m.soa.get());
+            e.rates = (const double*)hc_device_ptr(m.rates.get());
+            e.rho = hc_device_ptr(m.density.get());
+            e.tables = (const uint8_t*)hc_device_ptr(m.tables.get());
+            e.max_cells = m.max_cells; e.n = m.n; e.ns = g.n_substrates; e.nmax = std::max({g.nx, g.ny, g.nz});
+            e.nx = g.nx; e.ny = g.ny; e.nz = g.nz; e.V = g.nx * g.ny * g.nz;
+            e.x_min = g.x_min; e.y_min = g.y_min; e.z_min = g.z_min;
+            e.dx = g.dx; e.dy = g.dy; e.dz = g.dz; e.dt_diff = g.dt;
+            e.mvoxel = m.mech.mechanics_voxel_size; e.max_dist = m.mech.max_interaction_distance; e.dt_mech = m.mech.dt;
+            e.mnx = m.mech.mech_grid_nx; e.mny = m.mech.mech_grid_ny; e.mnz = m.mech.mech_grid_nz;
+            e.cell_off = cell_off; e.bin_off = bin_off; e.vox_off = vox_off;
+            sims.push_back(e);
+            BatchSim b{};
+            b.rho = e.rho;
+            b.tab = e.tables;
+            b.nx = g.nx; b.ny = g.ny; b.nz = g.nz; b.ns = g.n_substrates; b.nmax = e.nmax;
+            b.last_axis = g.nz > 1 ? 2u : 1u;
+            bsims.push_back(b);
+            for (uint32_t c0 = 0; c0 < m.n; c0 += 256) tiles.push_back(AgentTile{si, c0});
+            for (uint32_t c = 0; c < m.n; c++) agent_sim.push_back(si);
+            const uint32_t nn[3] = {g.nx, g.ny, g.nz};
+            for (int ax = 0; ax < 3; ax++) {
+                if (ax == 2 && g.nz == 1) continue;
+                const int P = hc::part_p(ax);
+                const uint64_t lines = hc::sweep_lines(ax, g.nx, g.ny, g.nz);
+                const uint64_t groups = (lines + 4 * (64 / P) - 1) / (4 * (64 / P));
+                for (uint32_t s = 0; s < g.n_substrates; s++)
+                    for (uint64_t q = 0; q < groups; q++) work[ax].push_back(BatchWork{si, s, q});
+                max_line[ax] = std::max(max_line[ax], nn[ax]);
+            }
+            cell_off += m.n;
+            bin_off += m.mech.mech_grid_nx * m.mech.mech_grid_ny * m.mech.mech_grid_nz;
+            vox_off += e.V;
+            ns_max = std::max(ns_max, g.n_substrates);
+            nmax_all = std::max(nmax_all, e.nmax);
+        }
+        w.ntot = cell_off; w.nbins = bin_off; w.vtot = vox_off; w.ns_max = ns_max; w.nmax_all = nmax_all;
+        w.ntiles = (uint32_t)tiles.size();
+        for (int ax = 0; ax < 3; ax++) { w.nwork[ax] = (uint32_t)work[ax].size(); w.max_line[ax] = max_line[ax]; }
+        size_t bytes = 0;
+        auto put = [&](DeviceBuffer& b, const void* src, size_t nb) {
+            b.allocate(ctx_, std::max<size_t>(16, nb));
+            if (nb) {
+                std::memcpy(b.host(), src, nb);
+                b.upload(0, nb);
+            }
+            bytes += b.size();
+        };
+        auto alloc = [&](DeviceBuffer& b, size_t nb) {
+            b.allocate(ctx_, std::max<size_t>(16, nb));
+            bytes += b.size();
+        };
+        put(w.sims, sims.data(), sims.size() * sizeof(EnsSim));
+        put(w.bsims, bsims.data(), bsims.size() * sizeof(BatchSim));
+        put(w.tiles, tiles.data(), tiles.size() * sizeof(AgentTile));
+        for (int ax = 0; ax < 3; ax++) put(w.work[ax], work[ax].data(), work[ax].size() * sizeof(BatchWork));
+        put(w.agent_sim, agent_sim.data(), agent_sim.size() * sizeof(uint32_t));
+        const size_t N = std::max<uint32_t>(1, w.ntot);
+        alloc(w.kglob, N * 4); alloc(w.slot, N * 4); alloc(w.tmp, N * 4); alloc(w.sorted, N * 4);
+        alloc(w.recs, N * (mixed_ ? sizeof(hc::CellRec32) : sizeof(hc::CellRec)));
+        alloc(w.counts, ((size_t)w.nbins + 1) * 4); alloc(w.offsets, ((size_t)w.nbins + 2) * 4);
+        alloc(w.partial, hc::scan_partial_bytes((uint64_t)w.nbins + 1));
+        alloc(w.pcounts, ((size_t)w.vtot + 2) * 4); alloc(w.poffsets, ((size_t)w.vtot + 3) * 4);
+        alloc(w.pkv, N * 4); alloc(w.pslot, N * 4); alloc(w.ptmp, N * 4); alloc(w.psorted, N * 4);
+        alloc(w.pflag, (N + 1) * 4); alloc(w.pgsim, (N + 1) * 4); alloc(w.pgvox, (N + 1) * 4);
+        alloc(w.pgbegin, (N + 2) * 4); alloc(w.pcoef, (size_t)3 * std::max(1u, w.ns_max) * N * 8);
+        alloc(w.phdr, sizeof(hc::PlanHeader));
+        alloc(w.ppartial, std::max(hc::scan_partial_bytes((uint64_t)w.vtot + 1), hc::scan_partial_bytes(N)));
+        w.bytes = bytes;
+        check(ctx_, hc_wait(ctx_), "wait");
+        dirty_ = false;
+        plan_dirty_ = true;
+    }
+
+    hc_ctx* c() { return ctx_; }
+
+    // x, y, [z] over every member; each member re-applies Dirichlet in its
+    // own last sweep (z in 3-D, y in 2-D)
+    void sweeps() {
+        Workspace& w = ws_;
+        const BatchSim* bs = (const BatchSim*)hc_device_ptr(w.bsims.get());
+        for (int ax = 0; ax < 3; ax++)
+            check(ctx_, hc::launch_sweep_batch(ctx_, ax, mixed_, bs, (const BatchWork*)hc_device_ptr(w.work[ax].get()),
+                                               w.nwork[ax], w.nmax_all, w.max_line[ax], grid_blocks_[ax]),
+                  "ensemble sweep");
+    }
+
+    void plan_secretion() {
+        Workspace& w = ws_;
+        plan_dirty_ = false;
+        if (!w.ntot) return;
+        const EnsSim* sims = (const EnsSim*)hc_device_ptr(w.sims.get());
+        const AgentTile* tiles = (const AgentTile*)hc_device_ptr(w.tiles.get());
+        uint32_t* counts = (uint32_t*)hc_device_ptr(w.pcounts.get());
+        uint32_t* offsets = (uint32_t*)hc_device_ptr(w.poffsets.get());
+        uint32_t* kv = (uint32_t*)hc_device_ptr(w.pkv.get());
+        uint32_t* slot = (uint32_t*)hc_device_ptr(w.pslot.get());
+        uint32_t* tmp = (uint32_t*)hc_device_ptr(w.ptmp.get());
+        uint32_t* sorted = (uint32_t*)hc_device_ptr(w.psorted.get());
+        uint32_t* flag = (uint32_t*)hc_device_ptr(w.pflag.get());
+        uint32_t* partial = (uint32_t*)hc_device_ptr(w.ppartial.get());
+        hc::PlanHeader* hdr = (hc::PlanHeader*)hc_device_ptr(w.phdr.get());
+        hipStream_t st = hc_stream_of(ctx_);
+        check(ctx_, hc::fill_u32(ctx_, counts, (uint64_t)w.vtot + 1, 0), "fill");
+        check(ctx_, hc::fill_u32(ctx_, (uint32_t*)hdr, sizeof(hc::PlanHeader) / 4, 0), "fill");
+        if (w.ntiles)
+            hipLaunchKernelGGL(hc::k_b_sec_keys, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, counts, kv, slot, w.vtot);
+        check(ctx_, hc::check_launch(ctx_, "k_b_sec_keys"), "sec keys");
+        check(ctx_, hc::scan_exclusive(ctx_, counts, offsets, (uint64_t)w.vtot + 1, partial), "scan");
+        check(ctx_, hc::launch_scatter(ctx_, kv, slot, offsets, tmp, w.ntot), "scatter");
+        check(ctx_, hc::launch_stable_sort(ctx_, kv, offsets, tmp, sorted, w.ntot), "stable");
+        check(ctx_, hc::launch_sec_flags(ctx_, kv, sorted, offsets, flag, w.ntot, w.vtot), "flags");
+        check(ctx_, hc::scan_exclusive(ctx_, flag, flag, w.ntot, partial), "scan flags");
+        hipLaunchKernelGGL(hc::k_b_sec_groups, dim3(hc::blocks_for(std::max(1u, w.ntot), 256)), dim3(256), 0, st, sims,
+                           (const uint32_t*)hc_device_ptr(w.agent_sim.get()), kv, sorted, offsets, flag,
+                           (uint32_t*)hc_device_ptr(w.pgsim.get()), (uint32_t*)hc_device_ptr(w.pgvox.get()),
+                           (uint32_t*)hc_device_ptr(w.pgbegin.get()), (double*)hc_device_ptr(w.pcoef.get()), hdr,
+                           w.ntot, w.vtot, w.ns_max);
+        check(ctx_, hc::check_launch(ctx_, "k_b_sec_groups"), "sec groups");
+        plan_dirty_ = false;
+    }
+
+    void secretion() {
+        Workspace& w = ws_;
+        if (!w.ntot) return;
+        hipStream_t st = hc_stream_of(ctx_);
+        const dim3 grid(hc::blocks_for(w.ntot, 256));
+        const EnsSim* sims = (const EnsSim*)hc_device_ptr(w.sims.get());
+        hc::KernelTimer tm(ctx_, HC_K_SEC_APPLY);
+        if (mixed_)
+            hipLaunchKernelGGL(hc::k_b_sec_apply<float>, grid, dim3(256), 0, st, sims,
+                               (const uint32_t*)hc_device_ptr(w.pgsim.get()), (const uint32_t*)hc_device_ptr(w.pgvox.get()),
+                               (const uint32_t*)hc_device_ptr(w.pgbegin.get()), (const double*)hc_device_ptr(w.pcoef.get()),
+                               (const hc::PlanHeader*)hc_device_ptr(w.phdr.get()), w.ntot);
+        else
+            hipLaunchKernelGGL(hc::k_b_sec_apply<double>, grid, dim3(256), 0, st, sims,
+                               (const uint32_t*)hc_device_ptr(w.pgsim.get()), (const uint32_t*)hc_device_ptr(w.pgvox.get()),
+                               (const uint32_t*)hc_device_ptr(w.pgbegin.get()), (const double*)hc_device_ptr(w.pcoef.get()),
+                               (const hc::PlanHeader*)hc_device_ptr(w.phdr.get()), w.ntot);
+        check(ctx_, hc::check_launch(ctx_, "k_b_sec_apply"), "secretion");
+    }
+
+    void mechanics() {
+        Workspace& w = ws_;
+        if (!w.ntot) return;
+        hipStream_t st = hc_stream_of(ctx_);
+        const EnsSim* sims = (const EnsSim*)hc_device_ptr(w.sims.get());
+        const AgentTile* tiles = (const AgentTile*)hc_device_ptr(w.tiles.get());
+        uint32_t* counts = (uint32_t*)hc_device_ptr(w.counts.get());
+        uint32_t* offsets = (uint32_t*)hc_device_ptr(w.offsets.get());
+        uint32_t* kglob = (uint32_t*)hc_device_ptr(w.kglob.get());
+        uint32_t* slot = (uint32_t*)hc_device_ptr(w.slot.get());
+        uint32_t* tmp = (uint32_t*)hc_device_ptr(w.tmp.get());
+        uint32_t* sorted = (uint32_t*)hc_device_ptr(w.sorted.get());
+        const uint32_t* asim = (const uint32_t*)hc_device_ptr(w.agent_sim.get());
+        void* recs = hc_device_ptr(w.recs.get());
+        {
+            hc::KernelTimer tm(ctx_, HC_K_HASH);
+            check(ctx_, hc::fill_u32(ctx_, counts, w.nbins, 0), "fill");
+            hipLaunchKernelGGL(hc::k_b_mech_keys, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, counts, kglob, slot);
+            check(ctx_, hc::check_launch(ctx_, "k_b_mech_keys"), "keys");
+            check(ctx_, hc::scan_exclusive(ctx_, counts, offsets, w.nbins, (uint32_t*)hc_device_ptr(w.partial.get())), "scan");
+            check(ctx_, hc::launch_scatter(ctx_, kglob, slot, offsets, tmp, w.ntot), "scatter");
+            const dim3 g(hc::blocks_for(w.ntot, 256));
+            if (mixed_)
+                hipLaunchKernelGGL(hc::k_b_gather<true>, g, dim3(256), 0, st, sims, asim, kglob, offsets, tmp, sorted, recs, w.ntot);
+            else
+                hipLaunchKernelGGL(hc::k_b_gather<false>, g, dim3(256), 0, st, sims, asim, kglob, offsets, tmp, sorted, recs, w.ntot);
+            check(ctx_, hc::check_launch(ctx_, "k_b_gather"), "gather");
+        }
+        {
+            hc::KernelTimer tm(ctx_, HC_K_FORCES);
+            const dim3 g(hc::blocks_for(w.ntot, 256));
+            if (mixed_)
+                hipLaunchKernelGGL(hc::k_b_forces<true>, g, dim3(256), 0, st, sims, asim, recs, offsets, w.ntot);
+            else
+                hipLaunchKernelGGL(hc::k_b_forces<false>, g, dim3(256), 0, st, sims, asim, recs, offsets, w.ntot);
+            check(ctx_, hc::check_launch(ctx_, "k_b_forces"), "forces");
+        }
+        {
+            hc::KernelTimer tm(ctx_, HC_K_INTEGRATE);
+            hipLaunchKernelGGL(hc::k_b_integrate, dim3(w.ntiles), dim3(256), 0, st, sims, tiles);
+            check(ctx_, hc::check_launch(ctx_, "k_b_integrate"), "integrate");
+        }
+    }
+
+    void period_kernels() {
+        if (plan_dirty_) plan_secretion();
+        const int k = substeps_per_mech();
+        for (int s = 0; s < k; s++) {
+            sweeps();
+            secretion();
+        }
+        mechanics();
+    }
+
+    void capture() {
+        if (plan_dirty_) plan_secretion();
+        hipStream_t st = hc_stream_of(ctx_);
+        check(ctx_, hc_wait(ctx_), "wait");
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) throw std::runtime_error(std::string("ensemble capture: ") + hipGetErrorString(e));
+        try {
+            period_kernels();
+        } catch (...) {
+            hipStreamEndCapture(st, &graph);
+            if (graph) hipGraphDestroy(graph);
+            throw;
+        }
+        e = hipStreamEndCapture(st, &graph);
+        if (e != hipSuccess) throw std::runtime_error(std::string("ensemble end capture: ") + hipGetErrorString(e));
+        hipGraphExec_t exec = nullptr;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e != hipSuccess) throw std::runtime_error(std::string("ensemble instantiate: ") + hipGetErrorString(e));
+        graph_ = (void*)exec;
+    }
+
+    // member clocks: the accumulators of src/main.mm:375-412, per member
+    void advance_clocks() {
+        const int k = substeps_per_mech();
+        bool refreshed = false;
+        for (auto& mp : members_) {
+            Member& m = *mp;
+            const double dtd = m.cfg.dt_diffusion;
+            for (int s = 0; s < k; s++) {
+                m.substeps++;
+                m.mech_acc += dtd;
+                if (m.mech_acc >= m.cfg.dt_mechanics - 0.5 * dtd) {
+                    m.mech_steps++;
+                    m.mech_acc = 0.0;
+                }
+                m.pheno_acc += dtd;
+                if (m.pheno_acc >= m.cfg.dt_phenotype - 0.5 * dtd) {
+                    m.pheno_steps++;
+                    m.pheno_acc = 0.0;
+                    check(ctx_, hc_voxel_keys(ctx_, m.soa.get(), &m.grid, m.n, m.max_cells), "voxel refresh");
+                    refreshed = true;
+                }
+            }
+        }
+        if (refreshed) {
+            // the plan is rebuilt outside any graph; the graph reads the plan
+            // buffers, which keep their addresses
+            plan_secretion();
+        }
+    }
+
+    hc_ctx* ctx_ = nullptr;
+    bool mixed_ = false, dirty_ = true, plan_dirty_ = true;
+    uint32_t next_id_ = 0;
+    unsigned grid_blocks_[3] = {1024, 1024, 1024};
+    void* graph_ = nullptr;
+    std::vector<std::unique_ptr<Member>> members_;
+    Workspace ws_;
+};
+
+}  // namespace physicell
+
+// ---------------------------------------------------------------- pe_* C-ABI
+using physicell::Ensemble;
+
+struct pe_ensemble {
+    std::unique_ptr<Ensemble> e;
+    std::string err;
+};
+
+template <class F>
+static int eguard(pe_ensemble* p, F&& f) {
+    if (!p || !p->e) return HC_ERR_ARG;
+    try {
+        f(*p->e);
+        return 0;
+    } catch (const std::exception& ex) {
+        p->err = ex.what();
+        return HC_ERR_STATE;
+    }
+}
+
+static physicell::SimConfig to_simconfig(const pc_config* cfg) {
+    physicell::SimConfig c;
+    c.x_min = cfg->x_min; c.x_max = cfg->x_max; c.y_min = cfg->y_min; c.y_max = cfg->y_max;
+    c.z_min = cfg->z_min; c.z_max = cfg->z_max;
+    c.dx = cfg->dx; c.dy = cfg->dy; c.dz = cfg->dz;
+    c.use_2D = cfg->use_2D != 0;
+    c.dt_diffusion = cfg->dt_diffusion; c.dt_mechanics = cfg->dt_mechanics;
+    c.dt_phenotype = cfg->dt_phenotype; c.max_time = cfg->max_time;
+    c.max_cells = cfg->max_cells;
+    for (uint32_t s = 0; s < cfg->n_substrates; s++) {
+        physicell::SubstrateConfig sc;
+        sc.diffusion_coeff = cfg->substrates[s].diffusion_coeff;
+        sc.decay_rate = cfg->substrates[s].decay_rate;
+        sc.initial_value = cfg->substrates[s].initial_value;
+        sc.dirichlet_value = cfg->substrates[s].dirichlet_value;
+        sc.dirichlet_enabled = cfg->substrates[s].dirichlet_enabled != 0;
+        c.substrates.push_back(sc);
+    }
+    return c;
+}
+
+extern "C" {
+
+int pe_create(pe_ensemble** out, int device, uint32_t mixed_precision) {
+    if (!out) return HC_ERR_ARG;
+    *out = nullptr;
+    auto* p = new pe_ensemble();
+    try {
+        p->e.reset(new Ensemble(device, mixed_precision != 0));
+    } catch (const std::exception& ex) {
+        fprintf(stderr, "[pe_create] %s\n", ex.what());
+        delete p;
+        return HC_ERR_HIP;
+    }
+    *out = p;
+    return 0;
+}
+
+void pe_destroy(pe_ensemble* p) { delete p; }
+const char* pe_last_error(const pe_ensemble* p) { return p ? p->err.c_str() : "null ensemble"; }
+
+int pe_add(pe_ensemble* p, const pc_config* cfg, uint32_t n, const double* x, const double* y, const double* z,
+           const double* radius, const double* volume, const double* rep, const double* adh, const double* rel,
+           const double* sec, const double* upt, const double* sat, const double* net, const uint32_t* alive,
+           uint32_t* member_id) {
+    if (!cfg || (cfg->n_substrates && !cfg->substrates) || !member_id) return HC_ERR_ARG;
+    if (n && (!x || !y || !z || !radius || !volume || !rep || !adh || !rel)) return HC_ERR_ARG;
+    return eguard(p, [&](Ensemble& e) {
+        *member_id = e.add(to_simconfig(cfg), n, x, y, z, radius, volume, rep, adh, rel, sec, upt, sat, net, alive);
+    });
+}
+
+int pe_remove(pe_ensemble* p, uint32_t member_id) {
+    return eguard(p, [&](Ensemble& e) { e.remove(member_id); });
+}
+
+int pe_run_periods(pe_ensemble* p, int periods, int use_graph) {
+    if (periods < 0) return HC_ERR_ARG;
+    return eguard(p, [&](Ensemble& e) { e.run_periods(periods, use_graph != 0); });
+}
+
+int pe_wait(pe_ensemble* p) { return eguard(p, [](Ensemble& e) { e.wait(); }); }
+
+int pe_get_density(pe_ensemble* p, uint32_t member_id, double* out) {
+    if (!out) return HC_ERR_ARG;
+    return eguard(p, [&](Ensemble& e) { e.get_density(member_id, out); });
+}
+
+int pe_get_field(pe_ensemble* p, uint32_t member_id, uint32_t field, void* out) {
+    if (!out || field >= HC_NUM_CELL_FIELDS) return HC_ERR_ARG;
+    return eguard(p, [&](Ensemble& e) { e.get_field(member_id, field, out); });
+}
+
+int pe_member_info(pe_ensemble* p, uint32_t member_id, uint64_t* bytes, int64_t* counts) {
+    return eguard(p, [&](Ensemble& e) {
+        physicell::Member& m = e.member(member_id);
+        if (bytes) *bytes = m.bytes();
+        if (counts) { counts[0] = m.substeps; counts[1] = m.mech_steps; counts[2] = m.pheno_steps; }
+    });
+}
+
+int pe_info(pe_ensemble* p, uint32_t* n_members, uint64_t* total_cells, uint64_t* total_voxel_values,
+            uint64_t* workspace_bytes) {
+    return eguard(p, [&](Ensemble& e) {
+        if (n_members) *n_members = (uint32_t)e.n_members();
+        if (total_cells) *total_cells = e.total_cells();
+        if (total_voxel_values) *total_voxel_values = e.total_voxel_values();
+        if (workspace_bytes) *workspace_bytes = e.workspace_bytes();
+    });
+}
+
+int pe_set_grid_blocks(pe_ensemble* p, int axis, uint32_t blocks) {
+    return eguard(p, [&](Ensemble& e) { e.set_grid_blocks(axis, blocks); });
+}
+
+int pe_autotune(pe_ensemble* p, int reps, uint32_t* blocks_out, double* us_out) {
+    if (!blocks_out || !us_out || reps < 1) return HC_ERR_ARG;
+    return eguard(p, [&](Ensemble& e) { e.autotune(reps, blocks_out, us_out); });
+}
+
+hc_ctx* pe_context(pe_ensemble* p) { return (p && p->e) ? p->e->ctx() : nullptr; }
+
+}  // extern "C"

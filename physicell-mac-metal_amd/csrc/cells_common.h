@@ -1,0 +1,199 @@
+// cells_common.h -- device helpers shared by the single-simulation and the
+// packed-ensemble agent kernels.
+#pragma once
+
+#include "hc_internal.h"
+
+namespace hc {
+
+// 64-byte hash-sorted record (FP64 forces)
+struct __attribute__((aligned(16))) CellRec {
+    double x, y, z, r, rep, adh, rel;
+    uint32_t idx, key;
+};
+static_assert(sizeof(CellRec) == 64, "record must be 64 bytes");
+
+struct CellPtrs {
+    double* f[HC_NUM_F64_FIELDS];
+    uint32_t* u[HC_NUM_U32_FIELDS];
+};
+
+inline CellPtrs cell_ptrs(void* base, uint32_t max_cells) {
+    CellPtrs p;
+    for (int f = 0; f < HC_NUM_F64_FIELDS; f++) p.f[f] = (double*)base + (size_t)f * max_cells;
+    uint32_t* ub = (uint32_t*)((double*)base + (size_t)HC_NUM_F64_FIELDS * max_cells);
+    for (int f = 0; f < HC_NUM_U32_FIELDS; f++) p.u[f] = ub + (size_t)f * max_cells;
+    return p;
+}
+
+struct __attribute__((aligned(16))) CellRec32 {
+    float x, y, z, r, rep, adh;
+    uint32_t idx, key;
+};
+static_assert(sizeof(CellRec32) == 32, "mixed record must be 32 bytes");
+
+// ---------------------------------------------------------------------------
+// wavefront primitives
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Ballot binning: returns the slot of this lane's key inside its bin.
+// One atomicAdd per distinct key in the wave.
+__device__ inline uint32_t ballot_bin(bool valid, uint32_t key, uint32_t* counts) {
+    const int lane = threadIdx.x & 63;
+    uint64_t pending = __ballot(valid);
+    uint32_t slot = 0;
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lk = __shfl(key, leader, 64);
+        const uint64_t same = __ballot(valid && key == lk) & pending;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&counts[lk], (uint32_t)__popcll(same));
+        base = __shfl(base, leader, 64);
+        if ((same >> lane) & 1ull) slot = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        pending &= ~same;
+    }
+    return slot;
+}
+
+// rank inside the bin = number of bin members with a smaller index
+__device__ inline uint32_t stable_rank(const uint32_t* __restrict__ tmp, uint32_t beg, uint32_t end, uint32_t c) {
+    uint32_t r = 0;
+    for (uint32_t t = beg; t < end; t++) r += tmp[t] < c ? 1u : 0u;
+    return r;
+}
+
+// Workgroups are handed to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// Renumber them so that each XCD works through one contiguous range of the
+// hash-sorted agents: neighbour bins then stay in that XCD's L2 instead of
+// every L2 fetching every plane.
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    constexpr uint32_t NX = 8;
+    const uint32_t x = b % NX, idx = b / NX, per = nb / NX, rem = nb % NX;
+    return x * per + (x < rem ? x : rem) + idx;
+}
+
+int fill_u32(hc_ctx* ctx, uint32_t* p, uint64_t n, uint32_t v);
+int launch_scatter(hc_ctx* ctx, const uint32_t* key, const uint32_t* slot, const uint32_t* offsets, uint32_t* tmp,
+                   uint32_t n);
+int launch_stable_sort(hc_ctx* ctx, const uint32_t* key, const uint32_t* offsets, const uint32_t* tmp,
+                       uint32_t* sorted, uint32_t n);
+int launch_sec_flags(hc_ctx* ctx, const uint32_t* key, const uint32_t* sorted, const uint32_t* offsets, uint32_t* flag,
+                     uint32_t n, uint32_t V);
+
+struct PlanHeader {
+    uint32_t n_elig, ngroups, has_net, n;
+};
+
+// Pair forces of one agent against the members [beg, end) of one neighbour
+// bin (shaders/mechanics.metal:151-203), members in ascending index.
+__device__ __attribute__((always_inline)) inline void force_bin64(
+    const CellRec* __restrict__ rec, uint32_t beg, uint32_t end, uint32_t t, double px, double py, double pz,
+    double radius_i, double rep_i, double adh_i, double rel_max_adh, double cut2, double& fx, double& fy,
+    double& fz, double& local_pressure) {
+    // Two passes per chunk of up to 32 bin members: a cheap distance
+    // test marks the candidates inside the cutoff, then the force
+    // law runs only over the marked ones, in the same ascending
+    // order.  Lanes of a wave then execute the expensive part about
+    // as often as they have neighbours, not once per candidate.
+    for (uint32_t u0 = beg; u0 < end; u0 += 32) {
+        const uint32_t cnt = end - u0 < 32u ? end - u0 : 32u;
+        uint32_t mask = 0;
+        for (uint32_t j = 0; j < cnt; j++) {
+            const uint32_t u = u0 + j;
+            const double2 xy = *reinterpret_cast<const double2*>(&rec[u].x);
+            const double ddx = px - xy.x;
+            const double ddy = py - xy.y;
+            const double ddz = pz - rec[u].z;
+            const double dist_sq = ddx * ddx + ddy * ddy + ddz * ddz;
+            const bool inside = u != t && !(dist_sq >= cut2) && !(dist_sq < 1e-12);
+            mask |= inside ? (1u << j) : 0u;
+        }
+        while (mask) {
+            const uint32_t j = __builtin_ctz(mask);
+            mask &= mask - 1u;
+            const uint32_t u = u0 + j;
+            const double2 xy = *reinterpret_cast<const double2*>(&rec[u].x);
+            const double2 zr = *reinterpret_cast<const double2*>(&rec[u].z);
+            const double ddx = px - xy.x;
+            const double ddy = py - xy.y;
+            const double ddz = pz - zr.x;
+            const double dist_sq = ddx * ddx + ddy * ddy + ddz * ddz;
+            const double dist = sqrt(dist_sq);
+            const double inv_dist = 1.0 / dist;
+            const double nxv = ddx * inv_dist;
+            const double nyv = ddy * inv_dist;
+            const double nzv = ddz * inv_dist;
+            const double R_sum = radius_i + zr.y;
+            double force_mag = 0.0;
+            if (dist < R_sum) {
+                const double overlap = 1.0 - dist / R_sum;
+                force_mag += overlap * overlap * sqrt(rep_i * rec[u].rep);
+                local_pressure += overlap * overlap / 0.027288820670331;
+            }
+            const double adhesion_distance = rel_max_adh * R_sum;
+            if (dist < adhesion_distance) {
+                const double t2 = 1.0 - dist / adhesion_distance;
+                force_mag -= t2 * t2 * sqrt(adh_i * rec[u].adh);
+            }
+            fx += force_mag * nxv;
+            fy += force_mag * nyv;
+            fz += force_mag * nzv;
+        }
+    }
+}
+
+// FP32 form: offsets are relative to each agent's own bin corner; (ox, oy,
+// oz) is the neighbour bin's corner relative to the agent's.
+__device__ __attribute__((always_inline)) inline void force_bin32(
+    const CellRec32* __restrict__ rec, uint32_t beg, uint32_t end, uint32_t t, float mx, float my, float mz,
+    float ox, float oy, float oz, float radius_i, float rep_i, float adh_i, float rel_max_adh, float cut2, float& fx,
+    float& fy, float& fz, float& local_pressure) {
+    for (uint32_t u0 = beg; u0 < end; u0 += 32) {
+        const uint32_t cnt = end - u0 < 32u ? end - u0 : 32u;
+        uint32_t mask = 0;
+        for (uint32_t j = 0; j < cnt; j++) {
+            const uint32_t u = u0 + j;
+            const float4 o = *reinterpret_cast<const float4*>(&rec[u].x);
+            const float ddx = mx - o.x - ox, ddy = my - o.y - oy, ddz = mz - o.z - oz;
+            const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            const bool inside = u != t && !(d2 >= cut2) && !(d2 < 1e-12f);
+            mask |= inside ? (1u << j) : 0u;
+        }
+        while (mask) {
+            const uint32_t j = __builtin_ctz(mask);
+            mask &= mask - 1u;
+            const uint32_t u = u0 + j;
+            const float4 o = *reinterpret_cast<const float4*>(&rec[u].x);
+            const float ddx = mx - o.x - ox, ddy = my - o.y - oy, ddz = mz - o.z - oz;
+            const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            const float dist = sqrtf(d2);
+            const float inv_dist = 1.0f / dist;
+            const float R_sum = radius_i + o.w;
+            float force_mag = 0.f;
+            if (dist < R_sum) {
+                const float overlap = 1.0f - dist / R_sum;
+                force_mag += overlap * overlap * sqrtf(rep_i * rec[u].rep);
+                local_pressure += overlap * overlap / 0.027288820670331f;
+            }
+            const float adhesion_distance = rel_max_adh * R_sum;
+            if (dist < adhesion_distance) {
+                const float t2 = 1.0f - dist / adhesion_distance;
+                force_mag -= t2 * t2 * sqrtf(adh_i * rec[u].adh);
+            }
+            fx += force_mag * (ddx * inv_dist);
+            fy += force_mag * (ddy * inv_dist);
+            fz += force_mag * (ddz * inv_dist);
+        }
+    }
+}
+
+}  // namespace hc

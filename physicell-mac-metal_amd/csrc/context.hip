@@ -50,6 +50,7 @@ KernelTimer::~KernelTimer() {
 int probe_kernels_diffusion(hc_ctx* ctx);
 int probe_kernels_cells(hc_ctx* ctx);
 int probe_kernels_cells2(hc_ctx* ctx);
+int probe_kernels_batch(hc_ctx* ctx);
 
 }  // namespace hc
 
@@ -115,6 +116,8 @@ int hc_load_kernels(hc_ctx* ctx, const char* code_object_or_null) {
     r = probe_kernels_cells(ctx);
     if (r) return r;
     r = probe_kernels_cells2(ctx);
+    if (r) return r;
+    r = probe_kernels_batch(ctx);
     if (r) return r;
     ctx->kernels_loaded = true;
     return 0;

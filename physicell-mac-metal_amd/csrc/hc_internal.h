@@ -113,10 +113,15 @@ __host__ __device__ inline size_t table_total_bytes(uint32_t ns, uint32_t nmax) 
 
 // ---- packed ensemble descriptors (batch.hip) ----
 struct BatchSim {            // one simulation of a packed ensemble
-    uint64_t rho_off;        // element offset of its density in the packed field buffer
-    uint64_t tab_off;        // byte offset of its table block in the packed table buffer
-    uint32_t nx, ny, nz, ns, nmax, _pad;
+    void* rho;               // its density field (FP64 or FP32, as the launch)
+    const uint8_t* tab;      // its Thomas tables
+    uint32_t nx, ny, nz, ns, nmax;
+    uint32_t last_axis;      // the sweep that re-applies Dirichlet (2 in 3-D, 1 in 2-D)
 };
+// number of lines a sweep along `axis` solves
+__host__ __device__ inline uint64_t sweep_lines(int axis, uint32_t nx, uint32_t ny, uint32_t nz) {
+    return axis == 0 ? (uint64_t)ny * nz : (axis == 1 ? (uint64_t)nz * nx : (uint64_t)ny * nx);
+}
 struct BatchWork {           // one line group of one substrate of one simulation
     uint32_t sim, s;
     uint64_t grp;
@@ -125,9 +130,8 @@ struct BatchWork {           // one line group of one substrate of one simulatio
 // ---- launchers implemented in diffusion.hip / cells.hip / sweep_part.hip ----
 int launch_sweep_part(hc_ctx* ctx, int axis, void* rho, const hc_grid* g, const uint8_t* tables,
                       int fuse_dirichlet);   // returns 1 when the grid does not suit it (caller falls back)
-int launch_sweep_batch(hc_ctx* ctx, int axis, void* rho_all, bool fp32, const uint8_t* tab_all, const BatchSim* sims,
-                       const BatchWork* work, uint32_t nwork, uint32_t nmax_all, uint32_t max_line, int fuse,
-                       unsigned grid_blocks);
+int launch_sweep_batch(hc_ctx* ctx, int axis, bool fp32, const BatchSim* sims, const BatchWork* work, uint32_t nwork,
+                       uint32_t nmax_all, uint32_t max_line, unsigned grid_blocks);
 void build_part_tables_host(const hc_grid* g, const hc_substrate* subs, double* out, float* fout);
 void build_tables_host(const hc_grid* g, const hc_substrate* subs, std::vector<uint8_t>& out);
 inline size_t field_elem(const hc_grid* g) { return g->field_fp32 ? sizeof(float) : sizeof(double); }

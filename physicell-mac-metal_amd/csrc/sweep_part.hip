@@ -251,10 +251,6 @@ __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict_
     }
 }
 
-__host__ __device__ inline uint64_t sweep_lines(int axis, uint32_t nx, uint32_t ny, uint32_t nz) {
-    return axis == 0 ? (uint64_t)ny * nz : (axis == 1 ? (uint64_t)nz * nx : (uint64_t)ny * nx);
-}
-
 // ---------------------------------------------------------------------------
 // one simulation: workgroup (group, substrate); x workgroups loop over groups
 // ---------------------------------------------------------------------------
@@ -295,11 +291,9 @@ __global__ __launch_bounds__(256) void k_sweep_part(T* __restrict__ rho, const u
 // contiguous run of items, so consecutive items mostly share their tables
 // ---------------------------------------------------------------------------
 template <int AXIS, int P, int MM, bool XLDS, typename T>
-__global__ __launch_bounds__(256) void k_sweep_part_batch(T* __restrict__ rho_all,
-                                                          const uint8_t* __restrict__ tab_all,
-                                                          const BatchSim* __restrict__ sims,
+__global__ __launch_bounds__(256) void k_sweep_part_batch(const BatchSim* __restrict__ sims,
                                                           const BatchWork* __restrict__ work, uint32_t nwork,
-                                                          uint32_t nmax_all, int fuse_dirichlet) {
+                                                          uint32_t nmax_all) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* tG = reinterpret_cast<double*>(smem_raw);
     T* tarr = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::g_count() * sizeof(double));
@@ -313,18 +307,18 @@ __global__ __launch_bounds__(256) void k_sweep_part_batch(T* __restrict__ rho_al
         const BatchSim sm = sims[it.sim];
         if (it.sim != cur_sim || it.s != cur_s) {
             __syncthreads();                              // previous tables no longer read
-            load_part_tables<P, T>(tab_all + sm.tab_off, sm.ns, sm.nmax, it.s, AXIS, tG, tarr);
+            load_part_tables<P, T>(sm.tab, sm.ns, sm.nmax, it.s, AXIS, tG, tarr);
             __syncthreads();
             cur_sim = it.sim;
             cur_s = it.s;
         }
-        const SubConst* sc = reinterpret_cast<const SubConst*>(tab_all + sm.tab_off);
+        const SubConst* sc = reinterpret_cast<const SubConst*>(sm.tab);
         uint32_t toff = 0;
         asm volatile("" : "+v"(toff));
-        T* rs = rho_all + sm.rho_off + (uint64_t)it.s * sm.nx * sm.ny * sm.nz;
+        T* rs = reinterpret_cast<T*>(sm.rho) + (uint64_t)it.s * sm.nx * sm.ny * sm.nz;
         sweep_group<AXIS, P, MM, XLDS, T>(rs, sm.nx, sm.ny, sm.nz, part_mt(sm.nmax, P), tarr + toff, tG + toff,
-                                          sc[it.s].c[AXIS], sc[it.s].T, sc[it.s].dir != 0, fuse_dirichlet,
-                                          it.grp, tile);
+                                          sc[it.s].c[AXIS], sc[it.s].T, sc[it.s].dir != 0,
+                                          sm.last_axis == (uint32_t)AXIS ? 1 : 0, it.grp, tile);
     }
 }
 
@@ -352,7 +346,7 @@ int probe_kernels_part(hc_ctx* ctx) {
 
 // window size for a line of n nodes split into P segments (0 = does not fit)
 static int part_window(uint32_t n, int P) {
-    if (n < 2 + 2 * (uint32_t)P) return 0;
+    if (n < 2 + (uint32_t)P) return 0;          // every segment holds at least one unknown
     const uint32_t m = (n - 2 + P - 1) / P;
     return m <= 16 ? 16 : (m <= 32 ? 32 : (m <= 64 && P == 4 ? 64 : 0));
 }
@@ -404,9 +398,8 @@ int launch_sweep_part(hc_ctx* ctx, int axis, void* rho, const hc_grid* g, const 
 }
 
 template <typename T>
-static int launch_batch_t(hc_ctx* ctx, int axis, T* rho_all, const uint8_t* tab_all, const BatchSim* sims,
-                          const BatchWork* work, uint32_t nwork, uint32_t nmax_all, uint32_t max_line, int fuse,
-                          unsigned grid_blocks) {
+static int launch_batch_t(hc_ctx* ctx, int axis, const BatchSim* sims, const BatchWork* work, uint32_t nwork,
+                          uint32_t nmax_all, uint32_t max_line, unsigned grid_blocks) {
     const int P = part_p(axis);
     const int mm = part_window(max_line, P);
     if (nwork == 0) return 0;
@@ -418,8 +411,8 @@ static int launch_batch_t(hc_ctx* ctx, int axis, T* rho_all, const uint8_t* tab_
     dim3 grid(std::min<unsigned>(grid_blocks, nwork));
     KernelTimer tm(ctx, HC_K_SWEEP_X + axis);
 #define HC_BATCH(AX, PP, M, X)                                                                                   \
-    hipLaunchKernelGGL((k_sweep_part_batch<AX, PP, M, X, T>), grid, dim3(256), shm, ctx->stream, rho_all, tab_all, \
-                       sims, work, nwork, nmax_all, fuse)
+    hipLaunchKernelGGL((k_sweep_part_batch<AX, PP, M, X, T>), grid, dim3(256), shm, ctx->stream, sims, work, nwork, \
+                       nmax_all)
     if (axis == 0) {
         if (mm != 16) return fail(ctx, HC_ERR_CAPACITY, "batched x sweep: lines longer than 258");
         HC_BATCH(0, 16, 16, true);
@@ -434,14 +427,10 @@ static int launch_batch_t(hc_ctx* ctx, int axis, T* rho_all, const uint8_t* tab_
     return check_launch(ctx, "k_sweep_part_batch");
 }
 
-int launch_sweep_batch(hc_ctx* ctx, int axis, void* rho_all, bool fp32, const uint8_t* tab_all, const BatchSim* sims,
-                       const BatchWork* work, uint32_t nwork, uint32_t nmax_all, uint32_t max_line, int fuse,
-                       unsigned grid_blocks) {
-    if (fp32)
-        return launch_batch_t<float>(ctx, axis, (float*)rho_all, tab_all, sims, work, nwork, nmax_all, max_line, fuse,
-                                     grid_blocks);
-    return launch_batch_t<double>(ctx, axis, (double*)rho_all, tab_all, sims, work, nwork, nmax_all, max_line, fuse,
-                                  grid_blocks);
+int launch_sweep_batch(hc_ctx* ctx, int axis, bool fp32, const BatchSim* sims, const BatchWork* work, uint32_t nwork,
+                       uint32_t nmax_all, uint32_t max_line, unsigned grid_blocks) {
+    if (fp32) return launch_batch_t<float>(ctx, axis, sims, work, nwork, nmax_all, max_line, grid_blocks);
+    return launch_batch_t<double>(ctx, axis, sims, work, nwork, nmax_all, max_line, grid_blocks);
 }
 
 // Host construction of the partitioned tables (FP64, plus an FP32 rounding
@@ -506,7 +495,7 @@ void build_part_tables_host(const hc_grid* g, const hc_substrate* subs, double* 
             std::fill(B, B + blk, 0.0);
             std::fill(F, F + fblk, 0.0f);
             const uint32_t n = nn[ax];
-            if (n < 2 + 2 * (uint32_t)P) continue;
+            if (n < 2 + (uint32_t)P) continue;
             const uint32_t N = n - 2;
             const double c = D * g->dt / (sp[ax] * sp[ax]);
             const double bI = 1.0 + 2.0 * c + h, bB = 1.0 + c + h;

@@ -81,7 +81,7 @@ def declared_functions(header=HEADER):
     """Every function name declared in include/physicell_hip.h."""
     txt = open(header).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s*(hc_[a-z0-9_]+|pc_[a-z0-9_]+)\s*\(",
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s*(hc_[a-z0-9_]+|pc_[a-z0-9_]+|pe_[a-z0-9_]+)\s*\(",
                        txt, flags=re.M)
     return sorted(set(names))
 
@@ -140,6 +140,17 @@ def load(path=LIB_PATH):
         "pc_sim_substrate_totals": (c_i32, [c_vp, c_vp]),
         "pc_sim_info": (c_i32, [c_vp, P(Grid), P(Mech), P(c_u32)]),
         "pc_sim_context": (c_vp, [c_vp]),
+        "pe_create": (c_i32, [P(c_vp), c_i32, c_u32]), "pe_destroy": (None, [c_vp]),
+        "pe_last_error": (ctypes.c_char_p, [c_vp]),
+        "pe_add": (c_i32, [c_vp, P(Config), c_u32] + [c_vp] * 13 + [P(c_u32)]),
+        "pe_remove": (c_i32, [c_vp, c_u32]), "pe_run_periods": (c_i32, [c_vp, c_i32, c_i32]),
+        "pe_wait": (c_i32, [c_vp]), "pe_get_density": (c_i32, [c_vp, c_u32, c_vp]),
+        "pe_get_field": (c_i32, [c_vp, c_u32, c_u32, c_vp]),
+        "pe_member_info": (c_i32, [c_vp, c_u32, P(ctypes.c_uint64), c_vp]),
+        "pe_info": (c_i32, [c_vp, P(c_u32), P(ctypes.c_uint64), P(ctypes.c_uint64), P(ctypes.c_uint64)]),
+        "pe_set_grid_blocks": (c_i32, [c_vp, c_i32, c_u32]),
+        "pe_autotune": (c_i32, [c_vp, c_i32, c_vp, c_vp]),
+        "pe_context": (c_vp, [c_vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -292,6 +303,25 @@ class CellBuffer:
         self.soa.upload(np.asarray(values, dt), lib.hc_cells_field_offset(f, self.max_cells))
 
 
+def config_of(spec, max_cells, mixed=False):
+    """pc_config of a scenario; returns (config, substrate array) -- keep both alive."""
+    ns = len(spec.substrates)
+    subs = (SubstrateCfg * max(1, ns))()
+    for i, s in enumerate(spec.substrates):
+        subs[i] = SubstrateCfg(s.D, s.decay, s.initial, s.dirichlet_value, 1 if s.dirichlet else 0, 0)
+    cfg = Config(spec.x_min, spec.x_max, spec.y_min, spec.y_max, spec.z_min, spec.z_max,
+                 spec.dx, spec.dy, spec.dz, 1 if spec.use_2d else 0, ns, spec.dt_diff,
+                 spec.dt_mech, spec.dt_pheno, spec.max_time, max_cells, 1 if mixed else 0, subs)
+    return cfg, subs
+
+
+def cell_arrays(c):
+    """The 13 per-cell arrays of pc_sim_set_cells / pe_add, contiguous."""
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (c.x, c.y, c.z, c.radius, c.volume, c.rep, c.adh, c.rel_adh)]
+    rates = [np.ascontiguousarray(np.asarray(a, np.float64).reshape(-1)) for a in (c.sec, c.upt, c.sat, c.net)]
+    return arrs + rates + [np.ascontiguousarray(c.alive, np.uint32)]
+
+
 class Simulation:
     """pc_sim: the C++ step loop on one device."""
 
@@ -299,13 +329,7 @@ class Simulation:
         lib = load()
         self.spec = spec
         ns = len(spec.substrates)
-        self._subs = (SubstrateCfg * max(1, ns))()
-        for i, s in enumerate(spec.substrates):
-            self._subs[i] = SubstrateCfg(s.D, s.decay, s.initial, s.dirichlet_value, 1 if s.dirichlet else 0, 0)
-        cfg = Config(spec.x_min, spec.x_max, spec.y_min, spec.y_max, spec.z_min, spec.z_max,
-                     spec.dx, spec.dy, spec.dz, 1 if spec.use_2d else 0, ns, spec.dt_diff,
-                     spec.dt_mech, spec.dt_pheno, spec.max_time,
-                     max_cells or max(1, cells.n), 1 if mixed else 0, self._subs)
+        cfg, self._subs = config_of(spec, max_cells or max(1, cells.n), mixed)
         h = c_vp()
         rc = lib.pc_sim_create(ctypes.byref(h), ctypes.byref(cfg), device)
         if rc:
@@ -392,6 +416,102 @@ class Simulation:
     def close(self):
         if self.h:
             load().pc_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ensemble:
+    """pe_ensemble: members stepped together in packed launches."""
+
+    def __init__(self, device=0, mixed=False):
+        lib = load()
+        h = c_vp()
+        rc = lib.pe_create(ctypes.byref(h), device, 1 if mixed else 0)
+        if rc:
+            raise HipError(f"pe_create failed ({rc})")
+        self.h = h
+        self.mixed = mixed
+        self.specs = {}
+        self.ncells = {}
+
+    def _check(self, rc, what):
+        if rc:
+            raise HipError(f"{what} failed ({rc}): {load().pe_last_error(self.h).decode()}")
+
+    def add(self, spec, cells):
+        cfg, subs = config_of(spec, max(1, cells.n), self.mixed)
+        a = cell_arrays(cells)
+        mid = c_u32()
+        self._check(load().pe_add(self.h, ctypes.byref(cfg), cells.n, *[_ptr(x) for x in a], ctypes.byref(mid)),
+                    "pe_add")
+        self.specs[mid.value] = spec
+        self.ncells[mid.value] = cells.n
+        return mid.value
+
+    def remove(self, mid):
+        self._check(load().pe_remove(self.h, mid), "pe_remove")
+        self.specs.pop(mid, None)
+        self.ncells.pop(mid, None)
+
+    def run_periods(self, periods, use_graph=True):
+        self._check(load().pe_run_periods(self.h, periods, 1 if use_graph else 0), "pe_run_periods")
+
+    def wait(self):
+        self._check(load().pe_wait(self.h), "pe_wait")
+
+    def density(self, mid):
+        spec = self.specs[mid]
+        out = np.empty(len(spec.substrates) * spec.n_voxels)
+        self._check(load().pe_get_density(self.h, mid, _ptr(out)), "pe_get_density")
+        return out
+
+    def field(self, mid, f):
+        out = np.empty(max(1, self.ncells[mid]), np.float64 if f < NUM_F64_FIELDS else np.uint32)
+        self._check(load().pe_get_field(self.h, mid, f, _ptr(out)), "pe_get_field")
+        return out[:self.ncells[mid]]
+
+    def member_info(self, mid):
+        b = ctypes.c_uint64()
+        counts = np.zeros(3, np.int64)
+        self._check(load().pe_member_info(self.h, mid, ctypes.byref(b), _ptr(counts)), "pe_member_info")
+        return b.value, tuple(int(v) for v in counts)
+
+    def info(self):
+        n, cells, vox, ws = c_u32(), ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(load().pe_info(self.h, ctypes.byref(n), ctypes.byref(cells), ctypes.byref(vox), ctypes.byref(ws)),
+                    "pe_info")
+        return {"members": n.value, "cells": cells.value, "voxel_values": vox.value, "workspace_bytes": ws.value}
+
+    def set_grid_blocks(self, axis, blocks):
+        self._check(load().pe_set_grid_blocks(self.h, axis, blocks), "pe_set_grid_blocks")
+
+    def autotune(self, reps=5):
+        blocks = np.zeros(3, np.uint32)
+        us = np.zeros(3, np.float64)
+        self._check(load().pe_autotune(self.h, reps, _ptr(blocks), _ptr(us)), "pe_autotune")
+        return [int(b) for b in blocks], [float(u) for u in us]
+
+    def context(self):
+        return load().pe_context(self.h)
+
+    def timing(self, mask):
+        if load().hc_timing_enable(self.context(), mask):
+            raise HipError("hc_timing_enable failed")
+
+    def timing_get(self, k):
+        ms, n = c_f64(), ctypes.c_int64()
+        if load().hc_timing_get(self.context(), k, ctypes.byref(ms), ctypes.byref(n)):
+            raise HipError("hc_timing_get failed")
+        return ms.value, n.value
+
+    def close(self):
+        if self.h:
+            load().pe_destroy(self.h)
             self.h = None
 
     def __del__(self):

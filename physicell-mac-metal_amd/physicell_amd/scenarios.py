@@ -232,6 +232,84 @@ def small3d(nx=12, ny=10, nz=8, n_cells=300, seed=5, ns=2, dirichlet=(True, Fals
     return sc, c
 
 
+HCP_DENSITY = math.sqrt(2.0)       # sites per spacing^3 of a close-packed lattice
+
+
+def hcp_sites_in_spheroid(n, semi_axes, spacing):
+    """The n close-packed lattice sites (lattice of ``hcp_sites_in_sphere``)
+    with the smallest normalised radius (x/a)^2 + (y/b)^2 + (z/c)^2 inside the
+    spheroid of semi-axes (a, b, c) centred at the origin; ties by lattice
+    enumeration order."""
+    a, b, c = (float(v) for v in semi_axes)
+    r = max(a, b, c)
+    x, y, z = hcp_sites_in_sphere(_sites_in_sphere_count(r, spacing), r, spacing)
+    q = (x / a) ** 2 + (y / b) ** 2 + (z / c) ** 2
+    inside = q <= 1.0
+    x, y, z, q = x[inside], y[inside], z[inside], q[inside]
+    if len(x) < n:
+        raise ValueError(f"spheroid {semi_axes} holds {len(x)} sites < {n}")
+    order = np.argsort(q, kind="stable")[:n]
+    return x[order], y[order], z[order]
+
+
+def _sites_in_sphere_count(r, spacing):
+    """Number of lattice sites inside the sphere of radius r (all of them)."""
+    a = float(spacing)
+    ry = a * math.sqrt(3.0) / 2.0
+    rz = a * math.sqrt(2.0 / 3.0)
+    ni = int(math.ceil(r / a)) + 2
+    nj = int(math.ceil(r / ry)) + 2
+    nk = int(math.ceil(r / rz)) + 2
+    k, j, i = np.meshgrid(np.arange(-nk, nk + 1), np.arange(-nj, nj + 1), np.arange(-ni, ni + 1), indexing="ij")
+    b = (k & 1).astype(np.float64)
+    x = a * (i + 0.5 * (j & 1)) + b * 0.5 * a
+    y = ry * j + b * a / (2.0 * math.sqrt(3.0))
+    zz = rz * k
+    return int(np.count_nonzero(x * x + y * y + zz * zz <= r * r))
+
+
+def ensemble_member(seed, edges=(32, 128), cells=(1000, 200_000), dx=20.0, fill=0.8):
+    """One simulation of the batched ensemble (config-1 type, 3-D).
+
+    Draw order with ``default_rng(seed)``:
+      1. edge[3]  = floor(exp(U(ln edges[0], ln edges[1])))  voxels along x, y, z;
+      2. n        = floor(exp(U(ln cells[0], ln cells[1])));
+      3. f[7]     = U(0.5, 1.5): factors on O2 D, O2 decay, factor D, factor
+                    decay, O2 uptake, repulsion, adhesion (config-1 values 1e5,
+                    0.1, 1000, 0.01, 10, 10, 0.4);
+      4. jitter   = U(-1, 1, (n', 3)) um.
+    Cells sit on the close-packed lattice (spacing 0.95 * 2 * radius) inside a
+    spheroid with semi-axes 0.4 * L per axis, centred in the domain
+    [-L/2, L/2]; n' = min(n, fill * lattice sites in that spheroid), since a
+    small grid cannot hold 200k cells.  The factor is secreted at 1/min to
+    saturation 1; O2 has Dirichlet 38 on all faces."""
+    rng = np.random.default_rng(seed)
+    e = np.floor(np.exp(rng.uniform(math.log(edges[0]), math.log(edges[1]), 3))).astype(int)
+    n_draw = int(math.floor(math.exp(rng.uniform(math.log(cells[0]), math.log(cells[1])))))
+    f = rng.uniform(0.5, 1.5, 7)
+    L = e * dx
+    subs = [Substrate("oxygen", 1e5 * f[0], 0.1 * f[1], 38.0, True, 38.0),
+            Substrate("factor", 1000.0 * f[2], 0.01 * f[3], 0.0, False, 0.0)]
+    sc = Scenario(f"member_{seed}", -L[0] / 2, L[0] / 2, -L[1] / 2, L[1] / 2, -L[2] / 2, L[2] / 2,
+                  dx, dx, dx, False, subs, max_time=100.0)
+    spacing = 0.95 * 2 * DEFAULT_RADIUS
+    semi = 0.4 * L
+    cap = int(fill * 4.0 / 3.0 * math.pi * semi[0] * semi[1] * semi[2] * HCP_DENSITY / spacing ** 3)
+    n = max(1, min(n_draw, cap))
+    x, y, z = hcp_sites_in_spheroid(n, semi, spacing)
+    j = rng.uniform(-1.0, 1.0, (n, 3))
+    c = uniform_cells(x + j[:, 0], y + j[:, 1], z + j[:, 2], 2, rep=10.0 * f[5], adh=0.4 * f[6])
+    c.upt[:, 0] = 10.0 * f[4]
+    c.sec[:, 1] = 1.0
+    c.sat[:, 1] = 1.0
+    return sc, c
+
+
+def config3(n_sims=64, seed0=100, **kw):
+    """The 64-simulation ensemble: members ensemble_member(100..163)."""
+    return [ensemble_member(seed0 + i, **kw) for i in range(n_sims)]
+
+
 BUILDERS = {"config0": config0, "config1": config1, "config2": config2, "small3d": small3d}
 
 

@@ -1,0 +1,179 @@
+"""Packed ensembles (pe_*): each member's results equal the single-simulation
+path's (pc_sim) on the same inputs, bit for bit, alone and in any mix, across
+membership changes, graph replay and autotuning."""
+import math
+
+import numpy as np
+import pytest
+
+from physicell_amd import hip, scenarios as S
+
+pytestmark = pytest.mark.gpu
+
+POS = (hip.F_POS_X, hip.F_POS_Y, hip.F_POS_Z)
+VEL = (hip.F_VEL_X, hip.F_VEL_Y, hip.F_VEL_Z)
+
+
+def small_member(seed, dt_pheno=0.5, **kw):
+    kw.setdefault("edges", (18, 40))
+    kw.setdefault("cells", (100, 3000))
+    sc, c = S.ensemble_member(seed, **kw)
+    sc.dt_pheno = dt_pheno
+    return sc, c
+
+
+def substeps_until(dtd, dt):
+    acc = 0.0
+    for k in range(1, 1000000):
+        acc += dtd
+        if acc >= dt - 0.5 * dtd:
+            return k
+    return 1000000
+
+
+def run_single(sc, c, periods, mixed, start=0):
+    """pc_sim driven through the same schedule: per period k substeps + one
+    mechanics step, the phenotype step where the member's own clock fires."""
+    sim = hip.Simulation(sc, c, mixed=mixed)
+    k = substeps_until(sc.dt_diff, sc.dt_mech)
+    kp = substeps_until(sc.dt_diff, sc.dt_pheno)
+    for p in range(periods):
+        for _ in range(k):
+            sim.substep()
+        sim.mech_step()
+        if ((p + 1) * k) % kp == 0:
+            sim.pheno_step()
+    sim.wait()
+    out = {"rho": sim.density()}
+    for f in POS + VEL:
+        out[f] = sim.field(f)
+    sim.close()
+    return out
+
+
+def member_state(ens, mid):
+    out = {"rho": ens.density(mid)}
+    for f in POS + VEL:
+        out[f] = ens.field(mid, f)
+    return out
+
+
+def assert_same(a, b, what):
+    for key in a:
+        if not np.array_equal(a[key], b[key]):
+            d = np.max(np.abs(a[key] - b[key]))
+            raise AssertionError(f"{what}: {key} differs (max abs {d:.3e})")
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_member_equals_single(mixed):
+    members = [small_member(200), small_member(201, dt_pheno=1.0), small_member(202)]
+    P = 12
+    ens = hip.Ensemble(mixed=mixed)
+    ids = [ens.add(sc, c) for sc, c in members]
+    ens.run_periods(P, use_graph=True)
+    ens.wait()
+    for mid, (sc, c) in zip(ids, members):
+        print(f"member {mid}: grid {sc.nx}x{sc.ny}x{sc.nz} cells {c.n} mixed={mixed}", flush=True)
+        assert_same(member_state(ens, mid), run_single(sc, c, P, mixed), f"member {mid}")
+        b, counts = ens.member_info(mid)
+        k = substeps_until(sc.dt_diff, sc.dt_mech)
+        kp = substeps_until(sc.dt_diff, sc.dt_pheno)
+        assert counts == (P * k, P, (P * k) // kp)
+        assert b > 0
+    ens.close()
+
+
+def test_alone_and_in_mix_mixed():
+    x = small_member(210)
+    others = [small_member(s) for s in (211, 212, 213)]
+    P = 10
+    alone = hip.Ensemble(mixed=True)
+    a = alone.add(*x)
+    alone.run_periods(P)
+    ra = member_state(alone, a)
+    alone.close()
+    mix = hip.Ensemble(mixed=True)
+    mix.add(*others[0])
+    b = mix.add(*x)
+    for o in others[1:]:
+        mix.add(*o)
+    mix.run_periods(P)
+    rb = member_state(mix, b)
+    mix.close()
+    assert_same(ra, rb, "alone vs mix")
+
+
+def test_membership_changes():
+    """Continuous batching: members join and leave between periods."""
+    A, B, C = small_member(220), small_member(221), small_member(222)
+    ens = hip.Ensemble(mixed=False)
+    ia = ens.add(*A)
+    ib = ens.add(*B)
+    ens.run_periods(3)
+    ic = ens.add(*C)
+    ens.run_periods(3, use_graph=False)
+    ens.remove(ia)
+    ens.run_periods(2)
+    ens.wait()
+    assert ens.info()["members"] == 2
+    assert_same(member_state(ens, ib), run_single(*B, 8, False), "B (8 periods)")
+    assert_same(member_state(ens, ic), run_single(*C, 5, False), "C (5 periods)")
+    with pytest.raises(hip.HipError, match="unknown member"):
+        ens.remove(ia)
+    ens.close()
+
+
+def test_autotune_keeps_results():
+    members = [small_member(s) for s in (230, 231)]
+    ref = hip.Ensemble(mixed=True)
+    ids = [ref.add(*m) for m in members]
+    ref.run_periods(4)
+    r0 = [member_state(ref, i) for i in ids]
+    ref.close()
+    ens = hip.Ensemble(mixed=True)
+    ids = [ens.add(*m) for m in members]
+    ens.run_periods(2)
+    blocks, us = ens.autotune(reps=2)
+    print("autotune", blocks, us, flush=True)
+    assert all(b >= 256 for b in blocks)
+    ens.run_periods(2)
+    for i, r in zip(ids, r0):
+        assert_same(member_state(ens, i), r, "after autotune")
+    ens.close()
+
+
+def test_schedule_and_grid_errors():
+    ens = hip.Ensemble(mixed=False)
+    sc, c = small_member(240)
+    ens.add(sc, c)
+    sc2, c2 = small_member(241)
+    sc2.dt_mech = 0.2
+    with pytest.raises(hip.HipError, match="share dt_diffusion"):
+        ens.add(sc2, c2)
+    sc3, c3 = S.small3d(nx=12, ny=10, nz=8)
+    with pytest.raises(hip.HipError, match="grid edges"):
+        ens.add(sc3, c3)
+    sc4, c4 = small_member(242)
+    sc4.dt_pheno = 0.15
+    with pytest.raises(hip.HipError, match="whole number"):
+        ens.add(sc4, c4)
+    ens.close()
+
+
+def test_mixed_ensemble_error_vs_fp64():
+    members = [small_member(s, edges=(24, 48), cells=(500, 4000)) for s in (250, 251)]
+    P = 20
+    r = {}
+    for mixed in (False, True):
+        ens = hip.Ensemble(mixed=mixed)
+        ids = [ens.add(*m) for m in members]
+        ens.run_periods(P)
+        r[mixed] = [member_state(ens, i) for i in ids]
+        ens.close()
+    for a, b in zip(r[False], r[True]):
+        rel = np.max(np.abs(a["rho"] - b["rho"]) / np.maximum(np.abs(a["rho"]), 1e-3))
+        pos = max(np.max(np.abs(a[f] - b[f])) for f in POS)
+        print(f"mixed vs FP64: density max rel {rel:.3e}, positions max abs {pos:.3e} um", flush=True)
+        assert rel <= 1e-4
+        assert pos <= 1e-3
