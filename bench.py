@@ -77,6 +77,123 @@ def cpu_baseline(spec, cells, periods, substeps):
                       f"same workload, FP64 oracle, OpenMP {threads} threads, {dt:.1f} s"}
 
 
+def bench_ensemble(args):
+    """64-simulation ensemble (members ensemble_member(100..163)): the batched
+    FP32-mixed form against the batched FP64 form and against the plain FP64
+    single-simulation path (one pc_sim per member, each replaying its own
+    period graph), all on the same members.  One step = one period of every
+    member (10 diffusion substeps + 1 mechanics step)."""
+    t_setup = time.perf_counter()
+    members = S.config3(n_sims=args.members)
+    n_cells = sum(c.n for _, c in members)
+    n_vals = sum(sc.n_voxels * len(sc.substrates) for sc, _ in members)
+    n_sub = 10
+    print(f"[bench] ensemble: {len(members)} members, {n_cells} agents, {n_vals} voxel values; "
+          f"setup {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
+
+    def time_ensemble(mixed):
+        ens = H.Ensemble(mixed=mixed)
+        for sc, c in members:
+            ens.add(sc, c)
+        blocks, us = ens.autotune(reps=3)
+        ens.run_periods(args.warmup)
+        ens.wait()
+        t0 = time.perf_counter()
+        ens.run_periods(args.steps)
+        ens.wait()
+        T = time.perf_counter() - t0
+        lat = []
+        for _ in range(max(10, min(args.steps, 50))):
+            t1 = time.perf_counter()
+            ens.run_periods(1)
+            ens.wait()
+            lat.append(1e3 * (time.perf_counter() - t1))
+        ens.timing(0xFFFFFFFF)
+        ens.run_periods(2, use_graph=False)
+        ens.wait()
+        kern = {}
+        for k, name in enumerate(H.KERNEL_NAMES):
+            ms, n = ens.timing_get(k)
+            if n:
+                kern[name] = {"total_ms": ms, "launches": n, "avg_us": 1e3 * ms / n}
+        ens.timing(0)
+        ens.close()
+        return {"ms_per_step": 1e3 * T / args.steps, "agent_steps_per_s": n_cells * args.steps / T,
+                "voxel_substeps_per_s": n_vals * n_sub * args.steps / T,
+                "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
+                "autotune": {"grid_blocks": blocks, "sweep_us": us}, "kernels": kern}
+
+    res = {"batched_mixed": time_ensemble(True)}
+    print(f"[bench] batched mixed: {res['batched_mixed']['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
+    res["batched_fp64"] = time_ensemble(False)
+    print(f"[bench] batched FP64: {res['batched_fp64']['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
+    sims = [H.Simulation(sc, c) for sc, c in members]
+    for s in sims:
+        s.run_periods(args.warmup, use_graph=True)
+    for s in sims:
+        s.wait()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for s in sims:
+            s.run_periods(1, use_graph=True)
+    for s in sims:
+        s.wait()
+    T = time.perf_counter() - t0
+    res["plain_fp64_per_sim"] = {"ms_per_step": 1e3 * T / args.steps, "agent_steps_per_s": n_cells * args.steps / T,
+                                 "voxel_substeps_per_s": n_vals * n_sub * args.steps / T}
+    for s in sims:
+        s.close()
+    print(f"[bench] plain FP64 per-sim: {res['plain_fp64_per_sim']['ms_per_step']:.2f} ms/step",
+          file=sys.stderr, flush=True)
+    bm, pl = res["batched_mixed"], res["plain_fp64_per_sim"]
+    kern = bm["kernels"]
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    ab = None
+    if dom in ("sweep_x", "sweep_y", "sweep_z"):
+        ab = 8.0 * n_vals
+    elif dom == "forces":
+        ab = (32.0 + 8.0 + 4.0 + 32.0) * n_cells
+    roof = {"bound": "hbm", "kernel": dom, "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
+            "avg_launch_us": kern[dom]["avg_us"], "algorithmic_bytes_per_launch": ab,
+            "achieved": ab / (kern[dom]["avg_us"] * 1e-6) / 1e9 if ab else None}
+    roof["frac"] = roof["achieved"] / HBM_PEAK_GBS if ab else None
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        threads = max(1, min(16, os.cpu_count() or 1))
+        sample = members[:args.cpu_members]
+        dt = 0.0
+        for sc, c in sample:
+            o = O.OracleSim(sc, c)
+            t1 = time.perf_counter()
+            for _ in range(args.cpu_periods):
+                for _ in range(n_sub):
+                    o.substep(nthreads=threads)
+                o.mech_step(nthreads=threads)
+            dt += time.perf_counter() - t1
+        P = args.cpu_periods
+        cpu = {"value": sum(c.n for _, c in sample) * P / dt, "unit": "agent-steps/s", "cores": threads,
+               "kind": "port",
+               "voxel_substeps_per_s": sum(sc.n_voxels * len(sc.substrates) for sc, _ in sample) * n_sub * P / dt,
+               "sample": f"{P} period(s) of members 0..{len(sample) - 1}, FP64 oracle, OpenMP {threads} threads, "
+                         f"{dt:.1f} s"}
+    out = {
+        "metric": METRIC, "value": bm["agent_steps_per_s"], "unit": "agent-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": bm["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"config3 ensemble: {len(members)} members (seeds 100..{99 + len(members)}), "
+                               f"{n_cells} agents, {n_vals} voxel values, batched FP32-mixed; step = one period "
+                               f"({n_sub} substeps + 1 mechanics step) of every member", "graph": True},
+        "voxel_substeps_per_s": bm["voxel_substeps_per_s"], "p50_step_ms": bm["p50_step_ms"],
+        "p99_step_ms": bm["p99_step_ms"], "roofline": roof, "cpu_baseline": cpu,
+        "speedup_vs_plain_fp64_path": bm["agent_steps_per_s"] / pl["agent_steps_per_s"],
+        "speedup_batched_fp64_vs_plain": res["batched_fp64"]["agent_steps_per_s"] / pl["agent_steps_per_s"],
+        "forms": res,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +208,8 @@ def main():
     ap.add_argument("--cpu-periods", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_config2.json"))
+    ap.add_argument("--members", type=int, default=64, help="ensemble size (workload config3)")
+    ap.add_argument("--cpu-members", type=int, default=64, help="members in the CPU sample (workload config3)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain FP64 device copy of 2^27 elements (1 GiB read + 1 GiB "
                          "written) five times, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -108,6 +227,9 @@ def main():
         print(f"bench.py measures one GPU only (asked for {args.gpus}); every box here has one GPU "
               f"and the multi-GPU decomposition is design-only.", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "config3":
+        return bench_ensemble(args)
 
     t_setup = time.perf_counter()
     spec, cells = build_workload(args.workload, args.cells)

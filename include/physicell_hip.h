@@ -318,7 +318,7 @@ hc_ctx* pc_sim_context(pc_sim* sim);
  * (substeps per mechanics step) diffusion substeps + one mechanics step; each
  * member's phenotype refresh (voxel keys + secretion plan) runs at the end of
  * the periods where its own dt_phenotype clock fires.  Grid limits: x edge
- * 18..258, y and z edges 6..130 (or nz = 1) voxels. */
+ * 6..258, y and z edges 6..130 (or nz = 1) voxels. */
 typedef struct pe_ensemble pe_ensemble;
 
 int  pe_create(pe_ensemble** out, int device_ordinal, uint32_t mixed_precision);

@@ -358,6 +358,11 @@ struct Member {
 };
 
 class Ensemble {
+    // sweep work lists: x with 4 segments per line, y, z, x with 16 segments
+    static constexpr int NLIST = 4;
+    static constexpr int LIST_AXIS[NLIST] = {0, 1, 2, 0};
+    static constexpr int LIST_P[NLIST] = {4, 4, 4, 16};
+
 public:
     Ensemble(int device, bool mixed) : mixed_(mixed) {
         int rc = hc_create(&ctx_, device);
@@ -396,7 +401,7 @@ public:
         g.field_fp32 = mixed_ ? 1u : 0u;
         if (!g.nx || !g.ny || !g.nz) throw std::runtime_error("ensemble: empty grid");
         if (!grid_fits(g))
-            throw std::runtime_error("ensemble: grid edges must be 18..258 (x) and 6..130 (y, z) voxels; "
+            throw std::runtime_error("ensemble: grid edges must be 6..258 (x) and 6..130 (y, z) voxels; "
                                      "run other grids as single simulations");
         check_schedule(c);
         m->subs.resize(g.n_substrates);
@@ -551,10 +556,10 @@ public:
     }
     void set_grid_blocks(int axis, unsigned b) {
         if (axis < 0 || axis > 2) throw std::runtime_error("ensemble: axis must be 0..2");
-        grid_blocks_[axis] = b ? b : 1024;
+        for (int li = 0; li < NLIST; li++)
+            if (LIST_AXIS[li] == axis) grid_blocks_[li] = b ? b : 1024;
         drop_graph();
     }
-    unsigned grid_blocks(int axis) const { return grid_blocks_[axis]; }
     hc_ctx* ctx() { return ctx_; }
     const std::vector<std::unique_ptr<Member>>& members() const { return members_; }
 
@@ -575,25 +580,21 @@ public:
         hipEvent_t a, b;
         hipEventCreate(&a);
         hipEventCreate(&b);
-        const unsigned cand[] = {256, 512, 1024, 2048, 4096, 8192};
+        const unsigned cand[] = {256, 512, 1024, 2048, 4096, 8192, 16384};
         Workspace& w = ws_;
-        const BatchSim* bs = (const BatchSim*)hc_device_ptr(w.bsims.get());
         for (int ax = 0; ax < 3; ax++) {
             blocks_out[ax] = grid_blocks_[ax];
             us_out[ax] = 0.0;
-            if (!w.nwork[ax]) continue;
+        }
+        for (int li = 0; li < NLIST; li++) {
+            if (!w.nwork[li]) continue;
             double best = 1e300;
+            unsigned pick = grid_blocks_[li];
             for (unsigned c : cand) {
-                if (c > 2 * w.nwork[ax] && c != cand[0]) break;
-                auto launch = [&]() {
-                    check(ctx_, hc::launch_sweep_batch(ctx_, ax, mixed_, bs,
-                                                       (const BatchWork*)hc_device_ptr(w.work[ax].get()), w.nwork[ax],
-                                                       w.nmax_all, w.max_line[ax], c),
-                          "autotune sweep");
-                };
-                launch();
+                if (c > 2 * w.nwork[li] && c != cand[0]) break;
+                sweep_list(li, c);
                 hipEventRecord(a, st);
-                for (int r = 0; r < reps; r++) launch();
+                for (int r = 0; r < reps; r++) sweep_list(li, c);
                 hipEventRecord(b, st);
                 hipEventSynchronize(b);
                 float ms = 0.f;
@@ -601,11 +602,18 @@ public:
                 const double us = 1e3 * ms / std::max(1, reps);
                 if (us < best) {
                     best = us;
-                    blocks_out[ax] = c;
+                    pick = c;
                 }
             }
-            us_out[ax] = best;
-            grid_blocks_[ax] = blocks_out[ax];
+            grid_blocks_[li] = pick;
+            const int ax = LIST_AXIS[li];
+            // per axis: the x entry reports the 4-segment list when it has work
+            if (li < 3 || !w.nwork[0]) {
+                blocks_out[ax] = pick;
+                us_out[ax] = best;
+            } else {
+                us_out[ax] += best;
+            }
         }
         hipEventDestroy(a);
         hipEventDestroy(b);
@@ -620,17 +628,18 @@ public:
 
 private:
     struct Workspace {
-        DeviceBuffer sims, bsims, tiles, work[3], agent_sim, kglob, slot, tmp, sorted, recs, counts, offsets, partial;
+        DeviceBuffer sims, bsims, tiles, work[NLIST], agent_sim, kglob, slot, tmp, sorted, recs, counts, offsets, partial;
         DeviceBuffer pcounts, poffsets, pkv, pslot, ptmp, psorted, pflag, pgsim, pgvox, pgbegin, pcoef, phdr, ppartial;
-        uint32_t ntot = 0, nbins = 0, vtot = 0, ntiles = 0, nwork[3] = {0, 0, 0}, ns_max = 0, nmax_all = 0;
-        uint32_t max_line[3] = {0, 0, 0};
+        uint32_t ntot = 0, nbins = 0, vtot = 0, ntiles = 0, nwork[NLIST] = {0, 0, 0, 0}, ns_max = 0, nmax_all = 0;
+        uint32_t max_line[NLIST] = {0, 0, 0, 0};
         size_t bytes = 0;
     };
 
-    // the batched sweeps hold up to 16 values per segment along x (16
-    // segments) and 32 along y and z (4 segments); every segment needs one
+    // the batched sweeps hold up to 32 values per segment with 4 segments
+    // (y, z, short x lines) and 16 with 16 segments (long x lines); every
+    // segment needs at least one unknown
     static bool grid_fits(const hc_grid& g) {
-        return g.nx >= 18 && g.nx <= 258 && g.ny >= 6 && g.ny <= 130 && (g.nz == 1 || (g.nz >= 6 && g.nz <= 130));
+        return g.nx >= 6 && g.nx <= 258 && g.ny >= 6 && g.ny <= 130 && (g.nz == 1 || (g.nz >= 6 && g.nz <= 130));
     }
 
     static int substeps_until(double dtd, double dt) {
@@ -672,10 +681,10 @@ private:
         std::vector<EnsSim> sims;
         std::vector<BatchSim> bsims;
         std::vector<AgentTile> tiles;
-        std::vector<BatchWork> work[3];
+        std::vector<BatchWork> work[NLIST];
         std::vector<uint32_t> agent_sim;
         uint32_t cell_off = 0, bin_off = 0, vox_off = 0, ns_max = 0, nmax_all = 0;
-        uint32_t max_line[3] = {0, 0, 0};
+        uint32_t max_line[NLIST] = {0, 0, 0, 0};
         for (uint32_t si = 0; si < members_.size(); si++) {
             Member& m = *members_[si];
             const hc_grid& g = m.grid;
@@ -703,12 +712,13 @@ private:
             const uint32_t nn[3] = {g.nx, g.ny, g.nz};
             for (int ax = 0; ax < 3; ax++) {
                 if (ax == 2 && g.nz == 1) continue;
-                const int P = hc::part_p(ax);
+                const int P = hc::part_p(ax, nn[ax], mixed_);
+                const int li = (ax == 0 && P == 16) ? 3 : ax;
                 const uint64_t lines = hc::sweep_lines(ax, g.nx, g.ny, g.nz);
                 const uint64_t groups = (lines + 4 * (64 / P) - 1) / (4 * (64 / P));
                 for (uint32_t s = 0; s < g.n_substrates; s++)
-                    for (uint64_t q = 0; q < groups; q++) work[ax].push_back(BatchWork{si, s, q});
-                max_line[ax] = std::max(max_line[ax], nn[ax]);
+                    for (uint64_t q = 0; q < groups; q++) work[li].push_back(BatchWork{si, s, q});
+                max_line[li] = std::max(max_line[li], nn[ax]);
             }
             cell_off += m.n;
             bin_off += m.mech.mech_grid_nx * m.mech.mech_grid_ny * m.mech.mech_grid_nz;
@@ -718,7 +728,7 @@ private:
         }
         w.ntot = cell_off; w.nbins = bin_off; w.vtot = vox_off; w.ns_max = ns_max; w.nmax_all = nmax_all;
         w.ntiles = (uint32_t)tiles.size();
-        for (int ax = 0; ax < 3; ax++) { w.nwork[ax] = (uint32_t)work[ax].size(); w.max_line[ax] = max_line[ax]; }
+        for (int li = 0; li < NLIST; li++) { w.nwork[li] = (uint32_t)work[li].size(); w.max_line[li] = max_line[li]; }
         size_t bytes = 0;
         auto put = [&](DeviceBuffer& b, const void* src, size_t nb) {
             b.allocate(ctx_, std::max<size_t>(16, nb));
@@ -735,7 +745,7 @@ private:
         put(w.sims, sims.data(), sims.size() * sizeof(EnsSim));
         put(w.bsims, bsims.data(), bsims.size() * sizeof(BatchSim));
         put(w.tiles, tiles.data(), tiles.size() * sizeof(AgentTile));
-        for (int ax = 0; ax < 3; ax++) put(w.work[ax], work[ax].data(), work[ax].size() * sizeof(BatchWork));
+        for (int li = 0; li < NLIST; li++) put(w.work[li], work[li].data(), work[li].size() * sizeof(BatchWork));
         put(w.agent_sim, agent_sim.data(), agent_sim.size() * sizeof(uint32_t));
         const size_t N = std::max<uint32_t>(1, w.ntot);
         alloc(w.kglob, N * 4); alloc(w.slot, N * 4); alloc(w.tmp, N * 4); alloc(w.sorted, N * 4);
@@ -758,13 +768,16 @@ private:
 
     // x, y, [z] over every member; each member re-applies Dirichlet in its
     // own last sweep (z in 3-D, y in 2-D)
-    void sweeps() {
+    void sweep_list(int li, unsigned blocks) {
         Workspace& w = ws_;
-        const BatchSim* bs = (const BatchSim*)hc_device_ptr(w.bsims.get());
-        for (int ax = 0; ax < 3; ax++)
-            check(ctx_, hc::launch_sweep_batch(ctx_, ax, mixed_, bs, (const BatchWork*)hc_device_ptr(w.work[ax].get()),
-                                               w.nwork[ax], w.nmax_all, w.max_line[ax], grid_blocks_[ax]),
-                  "ensemble sweep");
+        check(ctx_, hc::launch_sweep_batch(ctx_, LIST_AXIS[li], LIST_P[li], mixed_,
+                                           (const BatchSim*)hc_device_ptr(w.bsims.get()),
+                                           (const BatchWork*)hc_device_ptr(w.work[li].get()), w.nwork[li], w.nmax_all,
+                                           w.max_line[li], blocks),
+              "ensemble sweep");
+    }
+    void sweeps() {
+        for (int li : {0, 3, 1, 2}) sweep_list(li, grid_blocks_[li]);
     }
 
     void plan_secretion() {
@@ -932,7 +945,7 @@ private:
     hc_ctx* ctx_ = nullptr;
     bool mixed_ = false, dirty_ = true, plan_dirty_ = true;
     uint32_t next_id_ = 0;
-    unsigned grid_blocks_[3] = {1024, 1024, 1024};
+    unsigned grid_blocks_[NLIST] = {1024, 1024, 1024, 1024};
     void* graph_ = nullptr;
     std::vector<std::unique_ptr<Member>> members_;
     Workspace ws_;

@@ -80,11 +80,19 @@ inline uint32_t grid_nmax(const hc_grid* g) {
 
 // ---- partitioned sweep tables (see sweep_part.hip) ----
 // A line's N = n-2 unknowns are split into P contiguous segments, one per
-// lane (P = 16 along x, 4 along y and z).  Per (substrate, axis) block: for
+// lane (P = 4: 16 lines per wave, or P = 16: 4 lines per wave; see part_p).
+// Per (substrate, axis) block: for
 // each segment k the arrays 1/den, c', alpha, beta of length
 // part_mt(nmax, P), then the two rows (left / right neighbour value) of the
 // inverse interface matrix, each of length 2P.  Blocks have a common stride.
-__host__ __device__ inline int part_p(int axis) { return axis == 0 ? 16 : 4; }
+// Segments per line.  y and z: 4 (a wave solves 16 lines).  x lines are
+// staged through LDS, a wave's lines at a time: 4 segments (16 lines per
+// wave) while 64 lines of n values fit the staging tile (n <= 130 in FP32,
+// n <= 96 in FP64), else 16 segments (4 lines per wave).
+__host__ __device__ inline int part_p(int axis, uint32_t n, bool fp32) {
+    if (axis != 0) return 4;
+    return n <= (fp32 ? 130u : 96u) ? 4 : 16;
+}
 __host__ __device__ inline uint32_t part_mt(uint32_t nmax, int P) {
     return nmax > 2 ? (nmax - 2 + P - 1) / P : 1u;
 }
@@ -130,8 +138,9 @@ struct BatchWork {           // one line group of one substrate of one simulatio
 // ---- launchers implemented in diffusion.hip / cells.hip / sweep_part.hip ----
 int launch_sweep_part(hc_ctx* ctx, int axis, void* rho, const hc_grid* g, const uint8_t* tables,
                       int fuse_dirichlet);   // returns 1 when the grid does not suit it (caller falls back)
-int launch_sweep_batch(hc_ctx* ctx, int axis, bool fp32, const BatchSim* sims, const BatchWork* work, uint32_t nwork,
-                       uint32_t nmax_all, uint32_t max_line, unsigned grid_blocks);
+// P: the segments per line of every member in the work list (part_p)
+int launch_sweep_batch(hc_ctx* ctx, int axis, int P, bool fp32, const BatchSim* sims, const BatchWork* work,
+                       uint32_t nwork, uint32_t nmax_all, uint32_t max_line, unsigned grid_blocks);
 void build_part_tables_host(const hc_grid* g, const hc_substrate* subs, double* out, float* fout);
 void build_tables_host(const hc_grid* g, const hc_substrate* subs, std::vector<uint8_t>& out);
 inline size_t field_elem(const hc_grid* g) { return g->field_fp32 ? sizeof(float) : sizeof(double); }

@@ -19,9 +19,10 @@
 // Every value is read once and written once (16 bytes per voxel per sweep in
 // FP64, 8 in FP32).  For y and z, 16 lines x 4 segments fill a wave and the
 // lanes of one segment walk 16 adjacent x positions, so each load is a full
-// row segment.  x lines are contiguous: 4 lines x 16 segments fill a wave,
-// which copies its 4 lines (4*n contiguous values) through LDS with 16-byte
-// loads.
+// row segment.  x lines are contiguous: a wave copies its lines (LPW*n
+// contiguous values) through LDS with 16-byte loads; short lines use 16
+// lines x 4 segments per wave, long ones 4 lines x 16 segments, so that the
+// staging tile fits and the per-line interface work stays small.
 //
 // FP32 fields (the mixed form): values and the segment recurrence are FP32;
 // the elimination tables are computed in FP64 and rounded once; the Dirichlet
@@ -336,7 +337,12 @@ int probe_kernels_part(hc_ctx* ctx) {
         (const void*)k_sweep_part<2, 4, 32, false, float>, (const void*)k_sweep_part<2, 4, 64, false, float>,
         (const void*)k_sweep_part_batch<0, 16, 16, true, double>, (const void*)k_sweep_part_batch<1, 4, 32, false, double>,
         (const void*)k_sweep_part_batch<2, 4, 32, false, double>, (const void*)k_sweep_part_batch<0, 16, 16, true, float>,
-        (const void*)k_sweep_part_batch<1, 4, 32, false, float>, (const void*)k_sweep_part_batch<2, 4, 32, false, float>};
+        (const void*)k_sweep_part_batch<1, 4, 32, false, float>, (const void*)k_sweep_part_batch<2, 4, 32, false, float>,
+        (const void*)k_sweep_part<0, 4, 16, true, double>, (const void*)k_sweep_part<0, 4, 32, true, double>,
+        (const void*)k_sweep_part<0, 4, 16, false, double>, (const void*)k_sweep_part<0, 4, 32, false, double>,
+        (const void*)k_sweep_part<0, 4, 16, true, float>, (const void*)k_sweep_part<0, 4, 32, true, float>,
+        (const void*)k_sweep_part<0, 4, 16, false, float>, (const void*)k_sweep_part<0, 4, 32, false, float>,
+        (const void*)k_sweep_part_batch<0, 4, 32, true, double>, (const void*)k_sweep_part_batch<0, 4, 32, true, float>};
     for (const void* f : fns) {
         hipError_t e = hipFuncGetAttributes(&attr, f);
         if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: partitioned sweep");
@@ -358,15 +364,15 @@ static void launch_part_axis(hc_ctx* ctx, int mm, dim3 grid, size_t shm, T* rho,
         hipLaunchKernelGGL((k_sweep_part<AXIS, P, 16, XLDS, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
     else if (mm == 32)
         hipLaunchKernelGGL((k_sweep_part<AXIS, P, 32, XLDS, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
-    else if (P == 4)
+    else if constexpr (P == 4 && AXIS != 0)
         hipLaunchKernelGGL((k_sweep_part<AXIS, 4, 64, false, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
 }
 
 template <typename T>
 static int launch_sweep_part_t(hc_ctx* ctx, int axis, T* rho, const hc_grid* g, const uint8_t* tables,
                                int fuse_dirichlet) {
-    const int P = part_p(axis);
     const uint32_t n = axis == 0 ? g->nx : (axis == 1 ? g->ny : g->nz);
+    const int P = part_p(axis, n, g->field_fp32 != 0);
     const int mm = part_window(n, P);
     if (!mm) return 1;
     const uint32_t nmax = grid_nmax(g);
@@ -381,7 +387,10 @@ static int launch_sweep_part_t(hc_ctx* ctx, int axis, T* rho, const hc_grid* g, 
     const unsigned per_sub = (unsigned)std::max<uint64_t>(1, 4096 / std::max<uint32_t>(1, g->n_substrates));
     dim3 grid(xlds ? std::min(groups, per_sub) : groups, g->n_substrates);
     KernelTimer tm(ctx, HC_K_SWEEP_X + axis);
-    if (axis == 0) {
+    if (axis == 0 && P == 4) {
+        if (xlds) launch_part_axis<0, 4, true, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet);
+        else launch_part_axis<0, 4, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet);
+    } else if (axis == 0) {
         if (xlds) launch_part_axis<0, 16, true, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet);
         else launch_part_axis<0, 16, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet);
     } else if (axis == 1) {
@@ -398,9 +407,9 @@ int launch_sweep_part(hc_ctx* ctx, int axis, void* rho, const hc_grid* g, const 
 }
 
 template <typename T>
-static int launch_batch_t(hc_ctx* ctx, int axis, const BatchSim* sims, const BatchWork* work, uint32_t nwork,
+static int launch_batch_t(hc_ctx* ctx, int axis, int P, const BatchSim* sims, const BatchWork* work, uint32_t nwork,
                           uint32_t nmax_all, uint32_t max_line, unsigned grid_blocks) {
-    const int P = part_p(axis);
+    if (axis != 0 && P != 4) return fail(ctx, HC_ERR_ARG, "batched sweep: y and z use 4 segments per line");
     const int mm = part_window(max_line, P);
     if (nwork == 0) return 0;
     if (!mm) return fail(ctx, HC_ERR_CAPACITY, "batched sweep: line length outside the partitioned window");
@@ -413,7 +422,10 @@ static int launch_batch_t(hc_ctx* ctx, int axis, const BatchSim* sims, const Bat
 #define HC_BATCH(AX, PP, M, X)                                                                                   \
     hipLaunchKernelGGL((k_sweep_part_batch<AX, PP, M, X, T>), grid, dim3(256), shm, ctx->stream, sims, work, nwork, \
                        nmax_all)
-    if (axis == 0) {
+    if (axis == 0 && P == 4) {
+        if (mm > 32) return fail(ctx, HC_ERR_CAPACITY, "batched x sweep: lines longer than 130 with 4 segments");
+        HC_BATCH(0, 4, 32, true);
+    } else if (axis == 0) {
         if (mm != 16) return fail(ctx, HC_ERR_CAPACITY, "batched x sweep: lines longer than 258");
         HC_BATCH(0, 16, 16, true);
     } else if (axis == 1) {
@@ -427,10 +439,10 @@ static int launch_batch_t(hc_ctx* ctx, int axis, const BatchSim* sims, const Bat
     return check_launch(ctx, "k_sweep_part_batch");
 }
 
-int launch_sweep_batch(hc_ctx* ctx, int axis, bool fp32, const BatchSim* sims, const BatchWork* work, uint32_t nwork,
-                       uint32_t nmax_all, uint32_t max_line, unsigned grid_blocks) {
-    if (fp32) return launch_batch_t<float>(ctx, axis, sims, work, nwork, nmax_all, max_line, grid_blocks);
-    return launch_batch_t<double>(ctx, axis, sims, work, nwork, nmax_all, max_line, grid_blocks);
+int launch_sweep_batch(hc_ctx* ctx, int axis, int P, bool fp32, const BatchSim* sims, const BatchWork* work,
+                       uint32_t nwork, uint32_t nmax_all, uint32_t max_line, unsigned grid_blocks) {
+    if (fp32) return launch_batch_t<float>(ctx, axis, P, sims, work, nwork, nmax_all, max_line, grid_blocks);
+    return launch_batch_t<double>(ctx, axis, P, sims, work, nwork, nmax_all, max_line, grid_blocks);
 }
 
 // Host construction of the partitioned tables (FP64, plus an FP32 rounding
@@ -488,7 +500,7 @@ void build_part_tables_host(const hc_grid* g, const hc_substrate* subs, double* 
         const double D = subs[s].diffusion_coefficient, lam = subs[s].decay_rate;
         const double h = g->nz > 1 ? lam * g->dt / 3.0 : 0.5 * lam * g->dt;
         for (int ax = 0; ax < 3; ax++) {
-            const int P = part_p(ax);
+            const int P = part_p(ax, nn[ax], g->field_fp32 != 0);
             const uint32_t MT = part_mt(nmax, P);
             double* B = out + (size_t)(s * 3 + ax) * blk;
             float* F = fout + (size_t)(s * 3 + ax) * fblk;

@@ -151,7 +151,7 @@ def test_schedule_and_grid_errors():
     sc2.dt_mech = 0.2
     with pytest.raises(hip.HipError, match="share dt_diffusion"):
         ens.add(sc2, c2)
-    sc3, c3 = S.small3d(nx=12, ny=10, nz=8)
+    sc3, c3 = S.small3d(nx=12, ny=10, nz=4)
     with pytest.raises(hip.HipError, match="grid edges"):
         ens.add(sc3, c3)
     sc4, c4 = small_member(242)
