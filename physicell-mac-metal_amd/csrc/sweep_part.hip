@@ -104,12 +104,17 @@ __device__ inline void wave_sync() {
 }
 
 // LDS image of one (substrate, axis) table block: interface rows (FP64) then
-// the per-segment arrays 1/den, c', alpha, beta (field type).
+// the per-segment arrays 1/den, c', alpha, beta (field type).  Each segment's
+// part is padded by one element (odd stride), so that the segments of a wave,
+// which read the same row or array position at the same time, hit different
+// LDS banks.
 template <int P, typename T>
 struct PartLds {
-    static __host__ __device__ size_t g_count() { return (size_t)P * 2 * 2 * P; }
+    static constexpr uint32_t GS = 4 * P + 1;                 // per segment: left row, right row, pad
+    static __host__ __device__ uint32_t arr_stride(uint32_t MT) { return 4 * MT + 1; }
+    static __host__ __device__ size_t g_count() { return (size_t)P * GS; }
     static __host__ __device__ size_t bytes(uint32_t nmax) {
-        const size_t b = g_count() * sizeof(double) + (size_t)P * 4 * part_mt(nmax, P) * sizeof(T);
+        const size_t b = g_count() * sizeof(double) + (size_t)P * arr_stride(part_mt(nmax, P)) * sizeof(T);
         return (b + 15) / 16 * 16;
     }
 };
@@ -118,14 +123,15 @@ template <int P, typename T>
 __device__ inline void load_part_tables(const uint8_t* __restrict__ tables, uint32_t ns, uint32_t nmax, uint32_t s,
                                         int axis, double* tG, T* tarr) {
     const uint32_t MT = part_mt(nmax, P);
-    const size_t na = (size_t)P * 4 * MT;
+    const uint32_t na = (uint32_t)P * 4 * MT, AS = PartLds<P, T>::arr_stride(MT);
     const double* dblk = reinterpret_cast<const double*>(tables + part_offset(ns, nmax)) + (size_t)(s * 3 + axis) * part_block(nmax);
-    for (uint32_t q = threadIdx.x; q < PartLds<P, T>::g_count(); q += blockDim.x) tG[q] = dblk[na + q];
+    for (uint32_t q = threadIdx.x; q < (uint32_t)(4 * P * P); q += blockDim.x)
+        tG[(q / (4 * P)) * PartLds<P, T>::GS + q % (4 * P)] = dblk[na + q];
     if (sizeof(T) == sizeof(double)) {
-        for (uint32_t q = threadIdx.x; q < na; q += blockDim.x) tarr[q] = (T)dblk[q];
+        for (uint32_t q = threadIdx.x; q < na; q += blockDim.x) tarr[(q / (4 * MT)) * AS + q % (4 * MT)] = (T)dblk[q];
     } else {
         const float* fblk = reinterpret_cast<const float*>(tables + part_f_offset(ns, nmax)) + (size_t)(s * 3 + axis) * part_fblock(nmax);
-        for (uint32_t q = threadIdx.x; q < na; q += blockDim.x) tarr[q] = (T)fblk[q];
+        for (uint32_t q = threadIdx.x; q < na; q += blockDim.x) tarr[(q / (4 * MT)) * AS + q % (4 * MT)] = (T)fblk[q];
     }
 }
 
@@ -180,12 +186,12 @@ __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict_
         for (int t = 0; t < MM; t++) v[t] = (t < (int)mk && valid) ? p[(uint64_t)(sk + 1 + t) * stride] : (T)0;
     }
 
-    const T* idn = tarr + (k * 4 + 0) * MT;
-    const T* cpl = tarr + (k * 4 + 1) * MT;
-    const T* al = tarr + (k * 4 + 2) * MT;
-    const T* be = tarr + (k * 4 + 3) * MT;
-    const double* gl = tG + (k * 2 + 0) * 2 * P;
-    const double* gr = tG + (k * 2 + 1) * 2 * P;
+    const T* idn = tarr + k * PartLds<P, T>::arr_stride(MT);
+    const T* cpl = idn + MT;
+    const T* al = idn + 2 * MT;
+    const T* be = idn + 3 * MT;
+    const double* gl = tG + k * PartLds<P, T>::GS;
+    const double* gr = gl + 2 * P;
 
     if (dir) {
         const double ct = c * T0;                           // Dirichlet term in FP64
