@@ -706,6 +706,7 @@ private:
             b.tab = e.tables;
             b.nx = g.nx; b.ny = g.ny; b.nz = g.nz; b.ns = g.n_substrates; b.nmax = e.nmax;
             b.last_axis = g.nz > 1 ? 2u : 1u;
+            b.xmap = (uint32_t)hc::ctx_x_lane_order(ctx_, g.nx, hc::part_p(0, g.nx, mixed_), mixed_ ? 4 : 8);
             bsims.push_back(b);
             for (uint32_t c0 = 0; c0 < m.n; c0 += 256) tiles.push_back(AgentTile{si, c0});
             for (uint32_t c = 0; c < m.n; c++) agent_sim.push_back(si);

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -27,6 +28,8 @@ struct hc_ctx {
     int sweep_mode = 0;
     // small scratch for reductions read back by the host (max bin etc.)
     uint32_t* d_scalar = nullptr;
+    // x lane order per (line length, segments, element size)
+    std::map<uint64_t, int> xmap_cache;
 };
 
 struct hc_buf {
@@ -125,6 +128,8 @@ struct BatchSim {            // one simulation of a packed ensemble
     const uint8_t* tab;      // its Thomas tables
     uint32_t nx, ny, nz, ns, nmax;
     uint32_t last_axis;      // the sweep that re-applies Dirichlet (2 in 3-D, 1 in 2-D)
+    uint32_t xmap;           // x lane order (x_lane_order)
+    uint32_t _pad;
 };
 // number of lines a sweep along `axis` solves
 __host__ __device__ inline uint64_t sweep_lines(int axis, uint32_t nx, uint32_t ny, uint32_t nz) {
@@ -134,6 +139,49 @@ struct BatchWork {           // one line group of one substrate of one simulatio
     uint32_t sim, s;
     uint64_t grp;
 };
+
+// Lane order of an LDS-staged x sweep: lane -> (line l, segment k) is either
+// k = lane / LPW, l = lane % LPW (0) or k = lane % P, l = lane / P (1).  Both
+// give the same results; they differ in how the lanes' reads of the staging
+// tile (line l at l*n, segment k at floor(k*(n-2)/P)) fall on the 32 LDS
+// banks.  Returns the order with fewer bank cycles for lines of n values:
+// every read step is costed as the sum, over groups of lanes served together
+// (128 bytes per cycle), of the largest number of distinct words on one bank.
+inline int x_lane_order(uint32_t n, int P, size_t elem) {
+    const int LPW = 64 / P, wpl = (int)(elem / 4), per = 32 / wpl;
+    const uint32_t N = n - 2, MM = (N + P - 1) / P;
+    long cost[2] = {0, 0};
+    for (int tr = 0; tr < 2; tr++)
+        for (uint32_t t = 0; t < MM; t++)
+            for (int g0 = 0; g0 < 64; g0 += per) {
+                uint32_t words[32][32];
+                int cnt[32] = {0};
+                int worst = 0;
+                for (int lane = g0; lane < g0 + per; lane++) {
+                    const uint32_t k = tr ? lane % P : lane / LPW, l = tr ? lane / P : lane % LPW;
+                    const uint32_t sk = (k * N) / P, mk = ((k + 1) * N) / P - sk;
+                    const uint32_t idx = l * n + sk + 1 + (t < mk ? t : mk - 1);
+                    for (int q = 0; q < wpl; q++) {
+                        const uint32_t wd = idx * wpl + q, b = wd % 32;
+                        bool seen = false;
+                        for (int i = 0; i < cnt[b]; i++) seen |= words[b][i] == wd;
+                        if (!seen && cnt[b] < 32) words[b][cnt[b]++] = wd;
+                        worst = cnt[b] > worst ? cnt[b] : worst;
+                    }
+                }
+                cost[tr] += worst;
+            }
+    return cost[1] < cost[0] ? 1 : 0;
+}
+
+inline int ctx_x_lane_order(hc_ctx* ctx, uint32_t n, int P, size_t elem) {
+    const uint64_t key = ((uint64_t)n << 16) | ((uint64_t)P << 4) | (uint64_t)elem;
+    auto it = ctx->xmap_cache.find(key);
+    if (it != ctx->xmap_cache.end()) return it->second;
+    const int v = x_lane_order(n, P, elem);
+    ctx->xmap_cache[key] = v;
+    return v;
+}
 
 // ---- launchers implemented in diffusion.hip / cells.hip / sweep_part.hip ----
 int launch_sweep_part(hc_ctx* ctx, int axis, void* rho, const hc_grid* g, const uint8_t* tables,

@@ -38,35 +38,54 @@
 
 namespace hc {
 
+// 16-byte vector of T as a native vector value (one dwordx4 / b128 access)
 template <typename T>
-struct alignas(16) Vec16 {
-    T v[16 / sizeof(T)];
+struct Vec16T;
+template <>
+struct Vec16T<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
 };
+template <>
+struct Vec16T<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+};
+template <typename T>
+using Vec16 = typename Vec16T<T>::type;
+// T in the global address space: field accesses through such pointers are
+// global loads and stores even where the pointer was read from memory (the
+// packed ensemble's descriptors), not passed as a kernel argument
+template <typename T>
+using global_t = __attribute__((address_space(1))) T;
 
 // Contiguous copy of nval values between global memory and an LDS tile by
 // one wave.  A misaligned head and the tail are copied as scalars so the bulk
 // moves in 16-byte vectors; all of a lane's loads (at most RV vectors) are
-// issued before its first LDS write.
+// issued before its first LDS write.  The tile pointer carries the same
+// misalignment as the global pointer (it is shifted by lead elements from a
+// 16-byte boundary), so the vectors are 16-byte LDS accesses as well.
 template <typename T, int RV>
-__device__ inline void stage_in(const T* __restrict__ src, T* tile, uint32_t nval, uint32_t lane) {
+__device__ inline void stage_in(const global_t<T>* __restrict__ src, T* tile, uint32_t nval, uint32_t lane) {
     constexpr uint32_t E = 16 / sizeof(T);
     uint32_t head = (uint32_t)(((16u - ((uintptr_t)src & 15u)) & 15u) / sizeof(T));
     if (head > nval) head = nval;
     if (lane < head) tile[lane] = src[lane];
     const uint32_t nvec = (nval - head) / E;
-    const Vec16<T>* s2 = reinterpret_cast<const Vec16<T>*>(src + head);
-    Vec16<T> buf[RV];
+    const global_t<Vec16<T>>* s2 = reinterpret_cast<const global_t<Vec16<T>>*>(src + head);
+    Vec16<T>* t2 = reinterpret_cast<Vec16<T>*>(tile + head);
+    if (nvec) {                                           // wave-uniform
+        // unconditional loads (lanes past the end re-read the last vector),
+        // so that all of them are in flight before the first wait
+        Vec16<T> buf[RV];
 #pragma unroll
-    for (int r = 0; r < RV; r++) {
-        const uint32_t e = lane + 64u * r;
-        if (e < nvec) buf[r] = s2[e];
-    }
+        for (int r = 0; r < RV; r++) {
+            const uint32_t e = lane + 64u * r;
+            buf[r] = s2[e < nvec ? e : nvec - 1];
+        }
+        __builtin_amdgcn_sched_barrier(0);               // keep the loads ahead of the LDS writes
 #pragma unroll
-    for (int r = 0; r < RV; r++) {
-        const uint32_t e = lane + 64u * r;
-        if (e < nvec) {
-#pragma unroll
-            for (uint32_t q = 0; q < E; q++) tile[head + E * e + q] = buf[r].v[q];
+        for (int r = 0; r < RV; r++) {
+            const uint32_t e = lane + 64u * r;
+            if (e < nvec) t2[e] = buf[r];
         }
     }
     const uint32_t done = head + E * nvec;
@@ -74,25 +93,30 @@ __device__ inline void stage_in(const T* __restrict__ src, T* tile, uint32_t nva
 }
 
 template <typename T, int RV>
-__device__ inline void stage_out(T* __restrict__ dst, const T* tile, uint32_t nval, uint32_t lane) {
+__device__ inline void stage_out(global_t<T>* __restrict__ dst, const T* tile, uint32_t nval, uint32_t lane) {
     constexpr uint32_t E = 16 / sizeof(T);
     uint32_t head = (uint32_t)(((16u - ((uintptr_t)dst & 15u)) & 15u) / sizeof(T));
     if (head > nval) head = nval;
     if (lane < head) dst[lane] = tile[lane];
     const uint32_t nvec = (nval - head) / E;
-    Vec16<T>* d2 = reinterpret_cast<Vec16<T>*>(dst + head);
+    global_t<Vec16<T>>* d2 = reinterpret_cast<global_t<Vec16<T>>*>(dst + head);
+    const Vec16<T>* t2 = reinterpret_cast<const Vec16<T>*>(tile + head);
 #pragma unroll
     for (int r = 0; r < RV; r++) {
         const uint32_t e = lane + 64u * r;
-        if (e < nvec) {
-            Vec16<T> q;
-#pragma unroll
-            for (uint32_t j = 0; j < E; j++) q.v[j] = tile[head + E * e + j];
-            d2[e] = q;
-        }
+        if (e < nvec) d2[e] = t2[e];
     }
     const uint32_t done = head + E * nvec;
     if (lane < nval - done) dst[done + lane] = tile[done + lane];
+}
+
+// Elements of one wave's x staging tile: LPW lines of n values, one vector
+// of lead-in (the tile is shifted to share the global misalignment), rounded
+// to whole 16-byte vectors.
+template <typename T>
+__host__ __device__ inline uint32_t xtile_elems(uint32_t lpw, uint32_t n) {
+    constexpr uint32_t E = 16 / sizeof(T);
+    return (lpw * n + 2 * E - 1) / E * E;
 }
 
 // Each wave owns its LDS tile: lanes of one wave exchange data through it, so
@@ -120,32 +144,34 @@ struct PartLds {
 };
 
 template <int P, typename T>
-__device__ inline void load_part_tables(const uint8_t* __restrict__ tables, uint32_t ns, uint32_t nmax, uint32_t s,
-                                        int axis, double* tG, T* tarr) {
+__device__ inline void load_part_tables(const global_t<uint8_t>* __restrict__ tables, uint32_t ns, uint32_t nmax,
+                                        uint32_t s, int axis, double* tG, T* tarr) {
     const uint32_t MT = part_mt(nmax, P);
     const uint32_t na = (uint32_t)P * 4 * MT, AS = PartLds<P, T>::arr_stride(MT);
-    const double* dblk = reinterpret_cast<const double*>(tables + part_offset(ns, nmax)) + (size_t)(s * 3 + axis) * part_block(nmax);
+    const global_t<double>* dblk = reinterpret_cast<const global_t<double>*>(tables + part_offset(ns, nmax)) + (size_t)(s * 3 + axis) * part_block(nmax);
     for (uint32_t q = threadIdx.x; q < (uint32_t)(4 * P * P); q += blockDim.x)
         tG[(q / (4 * P)) * PartLds<P, T>::GS + q % (4 * P)] = dblk[na + q];
     if (sizeof(T) == sizeof(double)) {
         for (uint32_t q = threadIdx.x; q < na; q += blockDim.x) tarr[(q / (4 * MT)) * AS + q % (4 * MT)] = (T)dblk[q];
     } else {
-        const float* fblk = reinterpret_cast<const float*>(tables + part_f_offset(ns, nmax)) + (size_t)(s * 3 + axis) * part_fblock(nmax);
+        const global_t<float>* fblk = reinterpret_cast<const global_t<float>*>(tables + part_f_offset(ns, nmax)) + (size_t)(s * 3 + axis) * part_fblock(nmax);
         for (uint32_t q = threadIdx.x; q < na; q += blockDim.x) tarr[(q / (4 * MT)) * AS + q % (4 * MT)] = (T)fblk[q];
     }
 }
 
 // One group of 4*LPW lines (a workgroup of 4 waves) of one substrate of one
 // simulation.  rho points at that substrate's field.
+// xmap: lane order of the x staging tile (x_lane_order); y and z use order 0.
 template <int AXIS, int P, int MM, bool XLDS, typename T>
 __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict__ rho, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t MT,
                                    const T* tarr, const double* tG, double c, double T0, bool dir, int fuse_dirichlet,
-                                   uint64_t grp, T* tile_base) {
+                                   uint64_t grp, T* tile_base, uint32_t xmap) {
     constexpr int LPW = 64 / P;
     constexpr int WPB = 4;
     constexpr int RV = MM / (16 / (int)sizeof(T)) + 2;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t k = lane / LPW, l = lane % LPW;
+    const bool tr = XLDS && xmap;
+    const uint32_t k = tr ? lane % P : lane / LPW, l = tr ? lane / P : lane % LPW;
     const uint32_t n = AXIS == 0 ? nx : (AXIS == 1 ? ny : nz);
     const uint32_t N = n - 2;
     const uint64_t Lw0 = (grp * WPB + w) * LPW;          // first line of this wave
@@ -169,15 +195,16 @@ __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict_
     const bool fill = fuse_dirichlet && dir && face;
     const uint32_t sk = (k * N) / P;
     const uint32_t mk = ((k + 1) * N) / P - sk;
-    T* p = rho + base;
-    T* tile = tile_base + (size_t)w * LPW * n;
+    global_t<T>* grho = (global_t<T>*)rho;
+    global_t<T>* p = grho + base;
+    T* tile = tile_base + (size_t)w * xtile_elems<T>(LPW, n) + (((uintptr_t)(rho + Lw0 * nx) & 15u) / sizeof(T));
     const uint64_t rem = lines > Lw0 ? lines - Lw0 : 0;
     const uint32_t nval = (uint32_t)((rem < (uint64_t)LPW ? rem : (uint64_t)LPW) * n);
 
     T v[MM];
     if (XLDS) {
         // the wave's LPW consecutive x lines are LPW*n contiguous values
-        stage_in<T, RV>(rho + Lw0 * nx, tile, nval, lane);
+        stage_in<T, RV>(grho + Lw0 * nx, tile, nval, lane);
         wave_sync();
 #pragma unroll
         for (int t = 0; t < MM; t++) v[t] = (t < (int)mk) ? tile[l * n + sk + 1 + t] : (T)0;
@@ -225,8 +252,9 @@ __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict_
     double uL = 0.0, uR = 0.0;
 #pragma unroll
     for (int j = 0; j < P; j++) {
-        const double a = __shfl(yS, j * LPW + l, 64);
-        const double b = __shfl(yE, j * LPW + l, 64);
+        const int src = tr ? (int)l * P + j : j * LPW + (int)l;
+        const double a = __shfl(yS, src, 64);
+        const double b = __shfl(yE, src, 64);
         uL += gl[2 * j] * a + gl[2 * j + 1] * b;
         uR += gr[2 * j] * a + gr[2 * j + 1] * b;
     }
@@ -245,7 +273,7 @@ __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict_
             if (k == P - 1) tile[l * n + n - 1] = tT;
         }
         wave_sync();
-        stage_out<T, RV>(rho + Lw0 * nx, tile, nval, lane);
+        stage_out<T, RV>(grho + Lw0 * nx, tile, nval, lane);
         wave_sync();                                      // tile reused by the next group
     } else if (valid) {
 #pragma unroll
@@ -264,13 +292,13 @@ __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict_
 template <int AXIS, int P, int MM, bool XLDS, typename T>
 __global__ __launch_bounds__(256) void k_sweep_part(T* __restrict__ rho, const uint8_t* __restrict__ tables,
                                                     uint32_t nx, uint32_t ny, uint32_t nz, uint32_t nmax,
-                                                    int fuse_dirichlet) {
+                                                    int fuse_dirichlet, uint32_t xmap) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const uint32_t s = blockIdx.y, ns = gridDim.y;
     double* tG = reinterpret_cast<double*>(smem_raw);
     T* tarr = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::g_count() * sizeof(double));
     T* tile = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::bytes(nmax));
-    load_part_tables<P, T>(tables, ns, nmax, s, AXIS, tG, tarr);
+    load_part_tables<P, T>((const global_t<uint8_t>*)tables, ns, nmax, s, AXIS, tG, tarr);
     const SubConst* sc = reinterpret_cast<const SubConst*>(tables);
     const double c = sc[s].c[AXIS], T0 = sc[s].T;
     const bool dir = sc[s].dir != 0;
@@ -285,11 +313,11 @@ __global__ __launch_bounds__(256) void k_sweep_part(T* __restrict__ rho, const u
             uint32_t toff = 0;
             asm volatile("" : "+v"(toff));
             sweep_group<AXIS, P, MM, XLDS, T>(rs, nx, ny, nz, MT, tarr + toff, tG + toff, c, T0, dir, fuse_dirichlet,
-                                              grp, tile);
+                                              grp, tile, xmap);
         }
     } else {
         sweep_group<AXIS, P, MM, XLDS, T>(rs, nx, ny, nz, MT, tarr, tG, c, T0, dir, fuse_dirichlet, blockIdx.x,
-                                          tile);
+                                          tile, 0u);
     }
 }
 
@@ -314,18 +342,18 @@ __global__ __launch_bounds__(256) void k_sweep_part_batch(const BatchSim* __rest
         const BatchSim sm = sims[it.sim];
         if (it.sim != cur_sim || it.s != cur_s) {
             __syncthreads();                              // previous tables no longer read
-            load_part_tables<P, T>(sm.tab, sm.ns, sm.nmax, it.s, AXIS, tG, tarr);
+            load_part_tables<P, T>((const global_t<uint8_t>*)sm.tab, sm.ns, sm.nmax, it.s, AXIS, tG, tarr);
             __syncthreads();
             cur_sim = it.sim;
             cur_s = it.s;
         }
-        const SubConst* sc = reinterpret_cast<const SubConst*>(sm.tab);
+        const global_t<SubConst>* sc = (const global_t<SubConst>*)sm.tab;
         uint32_t toff = 0;
         asm volatile("" : "+v"(toff));
         T* rs = reinterpret_cast<T*>(sm.rho) + (uint64_t)it.s * sm.nx * sm.ny * sm.nz;
         sweep_group<AXIS, P, MM, XLDS, T>(rs, sm.nx, sm.ny, sm.nz, part_mt(sm.nmax, P), tarr + toff, tG + toff,
                                           sc[it.s].c[AXIS], sc[it.s].T, sc[it.s].dir != 0,
-                                          sm.last_axis == (uint32_t)AXIS ? 1 : 0, it.grp, tile);
+                                          sm.last_axis == (uint32_t)AXIS ? 1 : 0, it.grp, tile, sm.xmap);
     }
 }
 
@@ -365,13 +393,13 @@ static int part_window(uint32_t n, int P) {
 
 template <int AXIS, int P, bool XLDS, typename T>
 static void launch_part_axis(hc_ctx* ctx, int mm, dim3 grid, size_t shm, T* rho, const uint8_t* tables,
-                             const hc_grid* g, uint32_t nmax, int fuse) {
+                             const hc_grid* g, uint32_t nmax, int fuse, uint32_t xmap) {
     if (mm == 16)
-        hipLaunchKernelGGL((k_sweep_part<AXIS, P, 16, XLDS, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
+        hipLaunchKernelGGL((k_sweep_part<AXIS, P, 16, XLDS, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse, xmap);
     else if (mm == 32)
-        hipLaunchKernelGGL((k_sweep_part<AXIS, P, 32, XLDS, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
+        hipLaunchKernelGGL((k_sweep_part<AXIS, P, 32, XLDS, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse, xmap);
     else if constexpr (P == 4 && AXIS != 0)
-        hipLaunchKernelGGL((k_sweep_part<AXIS, 4, 64, false, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse);
+        hipLaunchKernelGGL((k_sweep_part<AXIS, 4, 64, false, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse, xmap);
 }
 
 template <typename T>
@@ -385,7 +413,8 @@ static int launch_sweep_part_t(hc_ctx* ctx, int axis, T* rho, const hc_grid* g, 
     const int LPW = 64 / P;
     const bool xlds = axis == 0 && ctx->sweep_mode != 2;
     const size_t tab_bytes = P == 16 ? PartLds<16, T>::bytes(nmax) : PartLds<4, T>::bytes(nmax);
-    const size_t shm = tab_bytes + (xlds ? (size_t)4 * LPW * n * sizeof(T) : 0);
+    const size_t shm = tab_bytes + (xlds ? (size_t)4 * xtile_elems<T>(LPW, n) * sizeof(T) : 0);
+    const uint32_t xmap = xlds ? (uint32_t)ctx_x_lane_order(ctx, n, P, sizeof(T)) : 0u;
     if (shm > 64 * 1024) return 1;
     // x: enough workgroups to fill the chip a few times over, each looping over
     // line groups; y/z: one workgroup per group
@@ -394,15 +423,15 @@ static int launch_sweep_part_t(hc_ctx* ctx, int axis, T* rho, const hc_grid* g, 
     dim3 grid(xlds ? std::min(groups, per_sub) : groups, g->n_substrates);
     KernelTimer tm(ctx, HC_K_SWEEP_X + axis);
     if (axis == 0 && P == 4) {
-        if (xlds) launch_part_axis<0, 4, true, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet);
-        else launch_part_axis<0, 4, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet);
+        if (xlds) launch_part_axis<0, 4, true, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap);
+        else launch_part_axis<0, 4, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap);
     } else if (axis == 0) {
-        if (xlds) launch_part_axis<0, 16, true, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet);
-        else launch_part_axis<0, 16, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet);
+        if (xlds) launch_part_axis<0, 16, true, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap);
+        else launch_part_axis<0, 16, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap);
     } else if (axis == 1) {
-        launch_part_axis<1, 4, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet);
+        launch_part_axis<1, 4, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap);
     } else {
-        launch_part_axis<2, 4, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet);
+        launch_part_axis<2, 4, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap);
     }
     return check_launch(ctx, "k_sweep_part");
 }
@@ -421,7 +450,7 @@ static int launch_batch_t(hc_ctx* ctx, int axis, int P, const BatchSim* sims, co
     if (!mm) return fail(ctx, HC_ERR_CAPACITY, "batched sweep: line length outside the partitioned window");
     const int LPW = 64 / P;
     const size_t tab_bytes = P == 16 ? PartLds<16, T>::bytes(nmax_all) : PartLds<4, T>::bytes(nmax_all);
-    const size_t shm = tab_bytes + (axis == 0 ? (size_t)4 * LPW * max_line * sizeof(T) : 0);
+    const size_t shm = tab_bytes + (axis == 0 ? (size_t)4 * xtile_elems<T>(LPW, max_line) * sizeof(T) : 0);
     if (shm > 64 * 1024) return fail(ctx, HC_ERR_CAPACITY, "batched sweep: LDS");
     dim3 grid(std::min<unsigned>(grid_blocks, nwork));
     KernelTimer tm(ctx, HC_K_SWEEP_X + axis);
