@@ -205,6 +205,8 @@ def main():
     ap.add_argument("--mixed", action="store_true", help="FP32-mixed form instead of plain FP64")
     ap.add_argument("--sweep-mode", type=int, default=0,
                     help="0 partitioned sweeps (default), 1 serial elimination, 2 partitioned with LDS-staged x")
+    ap.add_argument("--substep-order", type=int, default=0,
+                    help="0 default (axis-major), 1 axis-major, 2 substrate-major sweeps inside a substep")
     ap.add_argument("--cpu-periods", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_config2.json"))
@@ -238,6 +240,9 @@ def main():
         rc = H.load().hc_set_sweep_mode(sim.context(), args.sweep_mode)
         if rc:
             raise SystemExit("hc_set_sweep_mode failed")
+    if args.substep_order:
+        if H.load().hc_set_substep_order(sim.context(), args.substep_order):
+            raise SystemExit("hc_set_substep_order failed")
     k_sub = 10 if abs(spec.dt_mech / spec.dt_diff - 10.0) < 1e-9 else None
     use_graph = not args.no_graph
     print(f"[bench] {spec.name}: grid {spec.nx}x{spec.ny}x{spec.nz} x {len(spec.substrates)} substrates, "
@@ -281,6 +286,9 @@ def main():
     dom_id = H.KERNEL_NAMES.index(dom)
     ab = algorithmic_bytes(dom_id, spec, cells.n, args.mixed)
     avg_s = kern[dom]["avg_us"] * 1e-6
+    if ab and dom_id in SWEEP_KERNELS:
+        # a substep may run one launch per substrate (substrate-major order)
+        ab /= kern[dom]["launches"] / (kp * (k_sub or 10))
     traffic = None
     if os.path.exists(args.traffic):
         try:
@@ -296,7 +304,7 @@ def main():
     sw = [kern[n] for n in ("sweep_x", "sweep_y", "sweep_z") if n in kern]
     if sw:
         sweep_bytes = (8.0 if args.mixed else 16.0) * spec.n_voxels * len(spec.substrates) * len(sw)
-        sweep_s = sum(x["avg_us"] for x in sw) * 1e-6
+        sweep_s = sum(x["total_ms"] for x in sw) * 1e-3 / (kp * (k_sub or 10))
         roof["sweeps_combined_frac"] = sweep_bytes / sweep_s / 1e9 / HBM_PEAK_GBS
 
     n_sub = k_sub or 10
@@ -313,7 +321,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{spec.name}: {spec.nx}x{spec.ny}x{spec.nz} voxels x {len(spec.substrates)} "
                                f"substrates, {cells.n} agents, {'FP32-mixed' if args.mixed else 'plain FP64'}; step = {n_sub} diffusion substeps "
-                               f"+ 1 mechanics step", "graph": use_graph, "sweep_mode": args.sweep_mode},
+                               f"+ 1 mechanics step", "graph": use_graph, "sweep_mode": args.sweep_mode,
+                   "substep_order": args.substep_order},
         "voxel_substeps_per_s": vss,
         "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
         "roofline": roof, "kernels": kern, "cpu_baseline": cpu,

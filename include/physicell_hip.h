@@ -162,6 +162,10 @@ int hc_dirichlet(hc_ctx* ctx, hc_buf* density, const hc_grid* grid, const hc_sub
  * 2: as 0 with x segments loaded directly instead of through LDS.  All agree
  * to rounding (well below 1e-12 relative). */
 int hc_set_sweep_mode(hc_ctx* ctx, int mode);
+/* Order of the sweeps inside hc_diffusion_substep (results are identical):
+ * 0 default (= 1), 1 axis-major (x of every substrate, then y, then z),
+ * 2 substrate-major (x, y, z of substrate 0, then substrate 1, ...). */
+int hc_set_substep_order(hc_ctx* ctx, int order);
 /* One substep: x, y, [z] sweeps with Dirichlet fused into the last sweep
  * (src/microenvironment.mm:106-115 + src/main.mm:343-347). */
 int hc_diffusion_substep(hc_ctx* ctx, hc_buf* density, const hc_grid* grid,

@@ -407,6 +407,12 @@ size_t hc_thomas_table_bytes(const hc_grid* g) {
     return table_total_bytes(g->n_substrates, grid_nmax(g));
 }
 
+int hc_set_substep_order(hc_ctx* ctx, int order) {
+    if (!ctx || order < 0 || order > 2) return fail(ctx, HC_ERR_ARG, "hc_set_substep_order: order must be 0, 1 or 2");
+    ctx->substep_order = order;
+    return 0;
+}
+
 int hc_set_sweep_mode(hc_ctx* ctx, int mode) {
     if (!ctx || mode < 0 || mode > 2) return fail(ctx, HC_ERR_ARG, "hc_set_sweep_mode: mode must be 0, 1 or 2");
     ctx->sweep_mode = mode;
@@ -472,6 +478,20 @@ int hc_diffusion_substep(hc_ctx* ctx, hc_buf* density, const hc_grid* g, const h
     void* rho = density->dptr;
     const uint8_t* tb = (const uint8_t*)tables->dptr;
     const bool three_d = g->nz > 1;
+    // Substrate-major order (x, y, z of one substrate, then the next) gives
+    // the same results -- the sweeps of different substrates are independent
+    // -- and could keep one substrate's field in the last-level cache between
+    // its sweeps.  Measured on the 200^3 x 4 grid it is slower (four launches
+    // per axis of a quarter of the work each: 424 us vs 313 us per substep in
+    // FP64), so the default is axis-major.
+    if (ctx->substep_order == 2 && part_fits_all(g, ctx->sweep_mode)) {
+        for (uint32_t s = 0; s < g->n_substrates; s++) {
+            if ((r = launch_sweep_part(ctx, 0, rho, g, tb, 0, s, 1))) return r;
+            if ((r = launch_sweep_part(ctx, 1, rho, g, tb, three_d ? 0 : 1, s, 1))) return r;
+            if (three_d && (r = launch_sweep_part(ctx, 2, rho, g, tb, 1, s, 1))) return r;
+        }
+        return 0;
+    }
     if ((r = launch_sweep(ctx, 0, rho, g, tb, 0))) return r;
     if ((r = launch_sweep(ctx, 1, rho, g, tb, three_d ? 0 : 1))) return r;
     if (three_d && (r = launch_sweep(ctx, 2, rho, g, tb, 1))) return r;

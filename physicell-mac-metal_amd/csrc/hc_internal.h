@@ -26,6 +26,9 @@ struct hc_ctx {
     // 0: partitioned sweeps where the line length suits them, 1: serial
     // elimination (the reference's operation order, bit-identical to it in FP64)
     int sweep_mode = 0;
+    // substep order: 0 automatic, 1 axis-major (x of every substrate, then
+    // y, then z), 2 substrate-major (x, y, z of substrate 0, then 1, ...)
+    int substep_order = 0;
     // small scratch for reductions read back by the host (max bin etc.)
     uint32_t* d_scalar = nullptr;
     // x lane order per (line length, segments, element size)
@@ -184,8 +187,12 @@ inline int ctx_x_lane_order(hc_ctx* ctx, uint32_t n, int P, size_t elem) {
 }
 
 // ---- launchers implemented in diffusion.hip / cells.hip / sweep_part.hip ----
+// substrates s0 .. s0+scount-1 (scount 0: all from s0); returns 1 when the
+// grid does not suit the partitioned form (the caller falls back)
 int launch_sweep_part(hc_ctx* ctx, int axis, void* rho, const hc_grid* g, const uint8_t* tables,
-                      int fuse_dirichlet);   // returns 1 when the grid does not suit it (caller falls back)
+                      int fuse_dirichlet, uint32_t s0 = 0, uint32_t scount = 0);
+// 1 when every sweep of the grid takes the partitioned form
+bool part_fits_all(const hc_grid* g, int sweep_mode);
 // P: the segments per line of every member in the work list (part_p)
 int launch_sweep_batch(hc_ctx* ctx, int axis, int P, bool fp32, const BatchSim* sims, const BatchWork* work,
                        uint32_t nwork, uint32_t nmax_all, uint32_t max_line, unsigned grid_blocks);

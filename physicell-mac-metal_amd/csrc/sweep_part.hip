@@ -292,9 +292,9 @@ __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict_
 template <int AXIS, int P, int MM, bool XLDS, typename T>
 __global__ __launch_bounds__(256) void k_sweep_part(T* __restrict__ rho, const uint8_t* __restrict__ tables,
                                                     uint32_t nx, uint32_t ny, uint32_t nz, uint32_t nmax,
-                                                    int fuse_dirichlet, uint32_t xmap) {
+                                                    int fuse_dirichlet, uint32_t xmap, uint32_t s0, uint32_t ns) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const uint32_t s = blockIdx.y, ns = gridDim.y;
+    const uint32_t s = s0 + blockIdx.y;                  // substrates s0 .. s0+gridDim.y-1 of ns
     double* tG = reinterpret_cast<double*>(smem_raw);
     T* tarr = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::g_count() * sizeof(double));
     T* tile = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::bytes(nmax));
@@ -393,52 +393,73 @@ static int part_window(uint32_t n, int P) {
 
 template <int AXIS, int P, bool XLDS, typename T>
 static void launch_part_axis(hc_ctx* ctx, int mm, dim3 grid, size_t shm, T* rho, const uint8_t* tables,
-                             const hc_grid* g, uint32_t nmax, int fuse, uint32_t xmap) {
+                             const hc_grid* g, uint32_t nmax, int fuse, uint32_t xmap, uint32_t s0) {
     if (mm == 16)
-        hipLaunchKernelGGL((k_sweep_part<AXIS, P, 16, XLDS, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse, xmap);
+        hipLaunchKernelGGL((k_sweep_part<AXIS, P, 16, XLDS, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse, xmap, s0, g->n_substrates);
     else if (mm == 32)
-        hipLaunchKernelGGL((k_sweep_part<AXIS, P, 32, XLDS, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse, xmap);
+        hipLaunchKernelGGL((k_sweep_part<AXIS, P, 32, XLDS, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse, xmap, s0, g->n_substrates);
     else if constexpr (P == 4 && AXIS != 0)
-        hipLaunchKernelGGL((k_sweep_part<AXIS, 4, 64, false, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse, xmap);
+        hipLaunchKernelGGL((k_sweep_part<AXIS, 4, 64, false, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse, xmap, s0, g->n_substrates);
+}
+
+// LDS bytes of a partitioned sweep launch along `axis` (0: the grid does not
+// take the partitioned form)
+template <typename T>
+static size_t part_shm(int axis, const hc_grid* g, int sweep_mode) {
+    const uint32_t n = axis == 0 ? g->nx : (axis == 1 ? g->ny : g->nz);
+    const int P = part_p(axis, n, g->field_fp32 != 0);
+    if (!part_window(n, P)) return 0;
+    const uint32_t nmax = grid_nmax(g);
+    const bool xlds = axis == 0 && sweep_mode != 2;
+    const size_t tab_bytes = P == 16 ? PartLds<16, T>::bytes(nmax) : PartLds<4, T>::bytes(nmax);
+    const size_t shm = tab_bytes + (xlds ? (size_t)4 * xtile_elems<T>(64 / P, n) * sizeof(T) : 0);
+    return shm > 64 * 1024 ? 0 : shm;
+}
+
+bool part_fits_all(const hc_grid* g, int sweep_mode) {
+    if (sweep_mode == 1) return false;
+    for (int ax = 0; ax < (g->nz > 1 ? 3 : 2); ax++)
+        if (!(g->field_fp32 ? part_shm<float>(ax, g, sweep_mode) : part_shm<double>(ax, g, sweep_mode))) return false;
+    return true;
 }
 
 template <typename T>
 static int launch_sweep_part_t(hc_ctx* ctx, int axis, T* rho, const hc_grid* g, const uint8_t* tables,
-                               int fuse_dirichlet) {
+                               int fuse_dirichlet, uint32_t s0, uint32_t scount) {
     const uint32_t n = axis == 0 ? g->nx : (axis == 1 ? g->ny : g->nz);
     const int P = part_p(axis, n, g->field_fp32 != 0);
     const int mm = part_window(n, P);
-    if (!mm) return 1;
+    const size_t shm = part_shm<T>(axis, g, ctx->sweep_mode);
+    if (!mm || !shm) return 1;
     const uint32_t nmax = grid_nmax(g);
     const int LPW = 64 / P;
     const bool xlds = axis == 0 && ctx->sweep_mode != 2;
-    const size_t tab_bytes = P == 16 ? PartLds<16, T>::bytes(nmax) : PartLds<4, T>::bytes(nmax);
-    const size_t shm = tab_bytes + (xlds ? (size_t)4 * xtile_elems<T>(LPW, n) * sizeof(T) : 0);
     const uint32_t xmap = xlds ? (uint32_t)ctx_x_lane_order(ctx, n, P, sizeof(T)) : 0u;
-    if (shm > 64 * 1024) return 1;
     // x: enough workgroups to fill the chip a few times over, each looping over
     // line groups; y/z: one workgroup per group
     const unsigned groups = blocks_for(sweep_lines(axis, g->nx, g->ny, g->nz), LPW * 4);
-    const unsigned per_sub = (unsigned)std::max<uint64_t>(1, 4096 / std::max<uint32_t>(1, g->n_substrates));
-    dim3 grid(xlds ? std::min(groups, per_sub) : groups, g->n_substrates);
+    const unsigned per_sub = (unsigned)std::max<uint64_t>(1, 4096 / std::max<uint32_t>(1, scount));
+    dim3 grid(xlds ? std::min(groups, per_sub) : groups, scount);
     KernelTimer tm(ctx, HC_K_SWEEP_X + axis);
     if (axis == 0 && P == 4) {
-        if (xlds) launch_part_axis<0, 4, true, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap);
-        else launch_part_axis<0, 4, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap);
+        if (xlds) launch_part_axis<0, 4, true, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap, s0);
+        else launch_part_axis<0, 4, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap, s0);
     } else if (axis == 0) {
-        if (xlds) launch_part_axis<0, 16, true, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap);
-        else launch_part_axis<0, 16, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap);
+        if (xlds) launch_part_axis<0, 16, true, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap, s0);
+        else launch_part_axis<0, 16, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap, s0);
     } else if (axis == 1) {
-        launch_part_axis<1, 4, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap);
+        launch_part_axis<1, 4, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap, s0);
     } else {
-        launch_part_axis<2, 4, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap);
+        launch_part_axis<2, 4, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap, s0);
     }
     return check_launch(ctx, "k_sweep_part");
 }
 
-int launch_sweep_part(hc_ctx* ctx, int axis, void* rho, const hc_grid* g, const uint8_t* tables, int fuse) {
-    if (g->field_fp32) return launch_sweep_part_t<float>(ctx, axis, (float*)rho, g, tables, fuse);
-    return launch_sweep_part_t<double>(ctx, axis, (double*)rho, g, tables, fuse);
+int launch_sweep_part(hc_ctx* ctx, int axis, void* rho, const hc_grid* g, const uint8_t* tables, int fuse, uint32_t s0,
+                      uint32_t scount) {
+    if (scount == 0) scount = g->n_substrates - s0;
+    if (g->field_fp32) return launch_sweep_part_t<float>(ctx, axis, (float*)rho, g, tables, fuse, s0, scount);
+    return launch_sweep_part_t<double>(ctx, axis, (double*)rho, g, tables, fuse, s0, scount);
 }
 
 template <typename T>

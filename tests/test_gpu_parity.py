@@ -122,6 +122,32 @@ def test_substep_with_dirichlet(ctx, nx, ny, nz, dirs, mode):
     d.free(); tb.free()
 
 
+@pytest.mark.parametrize("fp32", [False, True])
+def test_substep_orders_agree(ctx, fp32):
+    """Axis-major and substrate-major sweep orders give the same field, bit
+    for bit (the substrates' sweeps are independent)."""
+    spec, _ = S.small3d(44, 30, 26, n_cells=1, ns=2, dirichlet=(True, False))
+    lib = H.load()
+    g = H.grid_of(spec, fp32)
+    subs = H.substrates_of(spec)
+    tb = ctx.alloc(max(8, lib.hc_thomas_table_bytes(g)))
+    ctx._check(lib.hc_build_thomas_tables(ctx.h, g, subs, tb.h), "tables")
+    rho0 = make_density(spec, 5).astype(np.float32 if fp32 else np.float64)
+    out = {}
+    for order in (1, 2):
+        d = ctx.alloc(rho0.nbytes)
+        d.upload(rho0)
+        ctx._check(lib.hc_set_substep_order(ctx.h, order), "order")
+        for _ in range(4):
+            ctx._check(lib.hc_diffusion_substep(ctx.h, d.h, g, subs, tb.h), "substep")
+        out[order] = d.download(rho0.dtype, rho0.size)
+        d.free()
+    ctx._check(lib.hc_set_substep_order(ctx.h, 0), "order")
+    print(f"substep orders fp32={fp32}: bit-identical={np.array_equal(out[1], out[2])}", flush=True)
+    assert np.array_equal(out[1], out[2])
+    tb.free()
+
+
 def cells_buffer(ctx, spec, cells):
     return H.CellBuffer(ctx, cells, len(spec.substrates))
 
