@@ -138,7 +138,8 @@ __global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sim
     const uint32_t c = idx - m.cell_off;
     const uint32_t lk = key - m.bin_off;
     const uint32_t vi = lk % m.mnx, vj = (lk / m.mnx) % m.mny, vk = lk / (m.mnx * m.mny);
-    const int dk_min = (m.mnz > 1) ? -1 : 0, dk_max = (m.mnz > 1) ? 1 : 0;
+    NeighborRows nr;
+    neighbor_rows(offsets, m.bin_off, vi, vj, vk, m.mnx, m.mny, m.mnz, nr);
     double ofx = 0.0, ofy = 0.0, ofz = 0.0, opr = 0.0;
     if (FP32) {
         const CellRec32 me = reinterpret_cast<const CellRec32*>(recs)[t];
@@ -146,42 +147,20 @@ __global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sim
         const float rel = (float)sim_field(m, HC_F_REL_MAX_ADH)[c];
         const float cut = (float)m.max_dist, vox = (float)m.mvoxel;
         float fx = 0.f, fy = 0.f, fz = 0.f, pr = 0.f;
-        for (int dk = dk_min; dk <= dk_max; dk++) {
-            const int nk = (int)vk + dk;
-            if (nk < 0 || nk >= (int)m.mnz) continue;
-            for (int dj = -1; dj <= 1; dj++) {
-                const int nj = (int)vj + dj;
-                if (nj < 0 || nj >= (int)m.mny) continue;
-                for (int di = -1; di <= 1; di++) {
-                    const int ni = (int)vi + di;
-                    if (ni < 0 || ni >= (int)m.mnx) continue;
-                    const uint32_t nb = m.bin_off + (uint32_t)nk * m.mny * m.mnx + (uint32_t)nj * m.mnx + (uint32_t)ni;
-                    force_bin32(rec, offsets[nb], offsets[nb + 1], t, me.x, me.y, me.z, (float)di * vox,
-                                (float)dj * vox, (float)dk * vox, me.r, me.rep, me.adh, rel, cut * cut, fx, fy, fz, pr);
-                }
-            }
-        }
+#pragma unroll
+        for (int q = 0; q < 9; q++)
+            force_range32(rec, nr.beg[q], nr.end[q], t, nr.mid0[q], nr.mid1[q], me.x, me.y, me.z, (float)nr.dj[q] * vox,
+                          (float)nr.dk[q] * vox, vox, me.r, me.rep, me.adh, rel, cut * cut, fx, fy, fz, pr);
         ofx = fx; ofy = fy; ofz = fz; opr = pr;
     } else {
         const CellRec me = reinterpret_cast<const CellRec*>(recs)[t];
         const CellRec* rec = reinterpret_cast<const CellRec*>(recs);
         const double cut2 = m.max_dist * m.max_dist;
         double fx = 0.0, fy = 0.0, fz = 0.0, pr = 0.0;
-        for (int dk = dk_min; dk <= dk_max; dk++) {
-            const int nk = (int)vk + dk;
-            if (nk < 0 || nk >= (int)m.mnz) continue;
-            for (int dj = -1; dj <= 1; dj++) {
-                const int nj = (int)vj + dj;
-                if (nj < 0 || nj >= (int)m.mny) continue;
-                for (int di = -1; di <= 1; di++) {
-                    const int ni = (int)vi + di;
-                    if (ni < 0 || ni >= (int)m.mnx) continue;
-                    const uint32_t nb = m.bin_off + (uint32_t)nk * m.mny * m.mnx + (uint32_t)nj * m.mnx + (uint32_t)ni;
-                    force_bin64(rec, offsets[nb], offsets[nb + 1], t, me.x, me.y, me.z, me.r, me.rep, me.adh, me.rel,
-                                cut2, fx, fy, fz, pr);
-                }
-            }
-        }
+#pragma unroll
+        for (int q = 0; q < 9; q++)
+            force_range64(rec, nr.beg[q], nr.end[q], t, me.x, me.y, me.z, me.r, me.rep, me.adh, me.rel, cut2, fx, fy,
+                          fz, pr);
         ofx = fx; ofy = fy; ofz = fz; opr = pr;
     }
     sim_field(m, HC_F_VEL_X)[c] = ofx;

@@ -208,30 +208,15 @@ __global__ __launch_bounds__(256) void k_forces(const CellRec* __restrict__ rec,
     const CellRec me = rec[t];
     const double px = me.x, py = me.y, pz = me.z;
     const double radius_i = me.r, rep_i = me.rep, adh_i = me.adh, rel_max_adh = me.rel;
-    const uint32_t my_voxel = me.key;
     double fx = 0.0, fy = 0.0, fz = 0.0, local_pressure = 0.0;
-    const uint32_t vi = my_voxel % mnx;
-    const uint32_t vj = (my_voxel / mnx) % mny;
-    const uint32_t vk = my_voxel / (mnx * mny);
-    const int dk_min = (mnz > 1) ? -1 : 0;
-    const int dk_max = (mnz > 1) ? 1 : 0;
+    const uint32_t vi = me.key % mnx, vj = (me.key / mnx) % mny, vk = me.key / (mnx * mny);
     const double cut2 = max_dist * max_dist;
-    for (int dk = dk_min; dk <= dk_max; dk++) {
-        const int nk = (int)vk + dk;
-        if (nk < 0 || nk >= (int)mnz) continue;
-        for (int dj = -1; dj <= 1; dj++) {
-            const int nj = (int)vj + dj;
-            if (nj < 0 || nj >= (int)mny) continue;
-            for (int di = -1; di <= 1; di++) {
-                const int ni = (int)vi + di;
-                if (ni < 0 || ni >= (int)mnx) continue;
-                const uint32_t nb = (uint32_t)nk * mny * mnx + (uint32_t)nj * mnx + (uint32_t)ni;
-                const uint32_t beg = offsets[nb], end = offsets[nb + 1];
-                force_bin64(rec, beg, end, t, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz,
-                            local_pressure);
-            }
-        }
-    }
+    NeighborRows nr;
+    neighbor_rows(offsets, 0u, vi, vj, vk, mnx, mny, mnz, nr);
+#pragma unroll
+    for (int q = 0; q < 9; q++)
+        force_range64(rec, nr.beg[q], nr.end[q], t, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz,
+                      local_pressure);
     const uint32_t c = me.idx;
     vx[c] = fx;
     vy[c] = fy;
@@ -555,25 +540,14 @@ __global__ __launch_bounds__(256) void k_forces32(const CellRec32* __restrict__ 
     const float rel_max_adh = (float)rel_adh[me.idx];
     float fx = 0.f, fy = 0.f, fz = 0.f, local_pressure = 0.f;
     const uint32_t vi = me.key % mnx, vj = (me.key / mnx) % mny, vk = me.key / (mnx * mny);
-    const int dk_min = (mnz > 1) ? -1 : 0, dk_max = (mnz > 1) ? 1 : 0;
     const float cut2 = max_dist * max_dist;
-    for (int dk = dk_min; dk <= dk_max; dk++) {
-        const int nk = (int)vk + dk;
-        if (nk < 0 || nk >= (int)mnz) continue;
-        for (int dj = -1; dj <= 1; dj++) {
-            const int nj = (int)vj + dj;
-            if (nj < 0 || nj >= (int)mny) continue;
-            for (int di = -1; di <= 1; di++) {
-                const int ni = (int)vi + di;
-                if (ni < 0 || ni >= (int)mnx) continue;
-                const float ox = (float)di * voxel, oy = (float)dj * voxel, oz = (float)dk * voxel;
-                const uint32_t nb = (uint32_t)nk * mny * mnx + (uint32_t)nj * mnx + (uint32_t)ni;
-                const uint32_t beg = offsets[nb], end = offsets[nb + 1];
-                force_bin32(rec, beg, end, t, me.x, me.y, me.z, ox, oy, oz, radius_i, rep_i, adh_i, rel_max_adh, cut2,
-                            fx, fy, fz, local_pressure);
-            }
-        }
-    }
+    NeighborRows nr;
+    neighbor_rows(offsets, 0u, vi, vj, vk, mnx, mny, mnz, nr);
+#pragma unroll
+    for (int q = 0; q < 9; q++)
+        force_range32(rec, nr.beg[q], nr.end[q], t, nr.mid0[q], nr.mid1[q], me.x, me.y, me.z, (float)nr.dj[q] * voxel,
+                      (float)nr.dk[q] * voxel, voxel, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz,
+                      local_pressure);
     const uint32_t c = me.idx;
     vx[c] = fx;
     vy[c] = fy;

@@ -93,29 +93,69 @@ struct PlanHeader {
     uint32_t n_elig, ngroups, has_net, n;
 };
 
-// Pair forces of one agent against the members [beg, end) of one neighbour
-// bin (shaders/mechanics.metal:151-203), members in ascending index.
-__device__ __attribute__((always_inline)) inline void force_bin64(
+// Neighbour rows of an agent in bin (vi, vj, vk): the 3 (2-D) or 9 (3-D)
+// rows (dk, dj) of up to 3 bins (di = -1, 0, 1).  The bins of one row are
+// consecutive keys, so their members form one contiguous range of the
+// hash-sorted order, in the reference's visit order (dk, dj, di) x ascending
+// index (shaders/mechanics.metal:151-203).  All bin offsets are loaded up
+// front (independent loads).  bin0 is the key of the simulation's bin 0.
+struct NeighborRows {
+    uint32_t beg[9], end[9];
+    uint32_t mid0[9], mid1[9];  // first member of the agent's column and of the next one
+    int dk[9], dj[9];
+};
+__device__ __attribute__((always_inline)) inline void neighbor_rows(const uint32_t* __restrict__ offsets, uint32_t bin0,
+                                                                    uint32_t vi, uint32_t vj, uint32_t vk, uint32_t mnx,
+                                                                    uint32_t mny, uint32_t mnz, NeighborRows& r) {
+    const uint32_t lo = vi > 0 ? vi - 1 : 0, hi = vi + 1 < mnx ? vi + 1 : mnx - 1;
+    const bool three_d = mnz > 1;
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        const int dk = three_d ? q / 3 - 1 : 0, dj = q % 3 - 1;
+        const int nk = (int)vk + dk, nj = (int)vj + dj;
+        const bool ok = (three_d || q < 3) && nk >= 0 && nk < (int)mnz && nj >= 0 && nj < (int)mny;
+        const uint32_t rb = bin0 + (ok ? (uint32_t)nk * mny * mnx + (uint32_t)nj * mnx : 0u);
+        r.beg[q] = ok ? offsets[rb + lo] : 0u;
+        r.end[q] = ok ? offsets[rb + hi + 1] : 0u;
+        r.mid0[q] = ok ? offsets[rb + vi] : 0u;
+        r.mid1[q] = ok ? offsets[rb + vi + 1] : 0u;
+        r.dk[q] = dk;
+        r.dj[q] = dj;
+    }
+}
+
+// Pair forces of one agent against the hash-sorted members [beg, end), in
+// ascending order (shaders/mechanics.metal:151-203).  Two passes per chunk
+// of up to 32 members: a distance test marks the candidates inside the
+// cutoff (their loads issued 4 at a time), then the force law runs over the
+// marked ones in the same order, so the lanes of a wave run the expensive
+// part about as often as they have neighbours, not once per candidate.
+__device__ __attribute__((always_inline)) inline void force_range64(
     const CellRec* __restrict__ rec, uint32_t beg, uint32_t end, uint32_t t, double px, double py, double pz,
     double radius_i, double rep_i, double adh_i, double rel_max_adh, double cut2, double& fx, double& fy,
     double& fz, double& local_pressure) {
-    // Two passes per chunk of up to 32 bin members: a cheap distance
-    // test marks the candidates inside the cutoff, then the force
-    // law runs only over the marked ones, in the same ascending
-    // order.  Lanes of a wave then execute the expensive part about
-    // as often as they have neighbours, not once per candidate.
     for (uint32_t u0 = beg; u0 < end; u0 += 32) {
         const uint32_t cnt = end - u0 < 32u ? end - u0 : 32u;
         uint32_t mask = 0;
-        for (uint32_t j = 0; j < cnt; j++) {
-            const uint32_t u = u0 + j;
-            const double2 xy = *reinterpret_cast<const double2*>(&rec[u].x);
-            const double ddx = px - xy.x;
-            const double ddy = py - xy.y;
-            const double ddz = pz - rec[u].z;
-            const double dist_sq = ddx * ddx + ddy * ddy + ddz * ddz;
-            const bool inside = u != t && !(dist_sq >= cut2) && !(dist_sq < 1e-12);
-            mask |= inside ? (1u << j) : 0u;
+        for (uint32_t j0 = 0; j0 < cnt; j0 += 4) {
+            double2 xy[4];
+            double zz[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t u = u0 + (j0 + q < cnt ? j0 + q : cnt - 1);
+                xy[q] = *reinterpret_cast<const double2*>(&rec[u].x);
+                zz[q] = rec[u].z;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t j = j0 + q;
+                const double ddx = px - xy[q].x;
+                const double ddy = py - xy[q].y;
+                const double ddz = pz - zz[q];
+                const double dist_sq = ddx * ddx + ddy * ddy + ddz * ddz;
+                const bool inside = j < cnt && u0 + j != t && !(dist_sq >= cut2) && !(dist_sq < 1e-12);
+                mask |= inside ? (1u << j) : 0u;
+            }
         }
         while (mask) {
             const uint32_t j = __builtin_ctz(mask);
@@ -151,28 +191,42 @@ __device__ __attribute__((always_inline)) inline void force_bin64(
     }
 }
 
-// FP32 form: offsets are relative to each agent's own bin corner; (ox, oy,
-// oz) is the neighbour bin's corner relative to the agent's.
-__device__ __attribute__((always_inline)) inline void force_bin32(
-    const CellRec32* __restrict__ rec, uint32_t beg, uint32_t end, uint32_t t, float mx, float my, float mz,
-    float ox, float oy, float oz, float radius_i, float rep_i, float adh_i, float rel_max_adh, float cut2, float& fx,
-    float& fy, float& fz, float& local_pressure) {
+// FP32 form: positions are relative to each agent's own bin corner.  A
+// member's bin corner relative to the agent's is (di, dj, dk) * voxel: dj, dk
+// are the row's (oy, oz), and di = -1, 0, +1 for members before mid0 (the
+// first member of the agent's own column), before mid1 (the first of the
+// next column), and after.
+__device__ __attribute__((always_inline)) inline void force_range32(
+    const CellRec32* __restrict__ rec, uint32_t beg, uint32_t end, uint32_t t, uint32_t mid0, uint32_t mid1, float mx,
+    float my, float mz, float oy, float oz, float voxel, float radius_i, float rep_i, float adh_i, float rel_max_adh,
+    float cut2, float& fx, float& fy, float& fz, float& local_pressure) {
+    const float oxm = -1.0f * voxel, ox0 = 0.0f * voxel, oxp = 1.0f * voxel;
     for (uint32_t u0 = beg; u0 < end; u0 += 32) {
         const uint32_t cnt = end - u0 < 32u ? end - u0 : 32u;
         uint32_t mask = 0;
-        for (uint32_t j = 0; j < cnt; j++) {
-            const uint32_t u = u0 + j;
-            const float4 o = *reinterpret_cast<const float4*>(&rec[u].x);
-            const float ddx = mx - o.x - ox, ddy = my - o.y - oy, ddz = mz - o.z - oz;
-            const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            const bool inside = u != t && !(d2 >= cut2) && !(d2 < 1e-12f);
-            mask |= inside ? (1u << j) : 0u;
+        for (uint32_t j0 = 0; j0 < cnt; j0 += 4) {
+            float4 o[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t u = u0 + (j0 + q < cnt ? j0 + q : cnt - 1);
+                o[q] = *reinterpret_cast<const float4*>(&rec[u].x);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t j = j0 + q, u = u0 + j;
+                const float ox = u < mid0 ? oxm : (u < mid1 ? ox0 : oxp);
+                const float ddx = mx - o[q].x - ox, ddy = my - o[q].y - oy, ddz = mz - o[q].z - oz;
+                const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                const bool inside = j < cnt && u != t && !(d2 >= cut2) && !(d2 < 1e-12f);
+                mask |= inside ? (1u << j) : 0u;
+            }
         }
         while (mask) {
             const uint32_t j = __builtin_ctz(mask);
             mask &= mask - 1u;
             const uint32_t u = u0 + j;
             const float4 o = *reinterpret_cast<const float4*>(&rec[u].x);
+            const float ox = u < mid0 ? oxm : (u < mid1 ? ox0 : oxp);
             const float ddx = mx - o.x - ox, ddy = my - o.y - oy, ddz = mz - o.z - oz;
             const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
             const float dist = sqrtf(d2);
