@@ -45,23 +45,31 @@ __device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
     return v;
 }
 
-// Ballot binning: returns the slot of this lane's key inside its bin.
-// One atomicAdd per distinct key in the wave.
+// Ballot binning: returns the slot of this lane's key inside its bin.  The
+// lanes are grouped by key with ballots (no memory traffic in the loop); then
+// each group's lowest lane adds the group's size to the bin counter with one
+// atomic, all groups' atomics in flight together, and the members take
+// consecutive slots from the returned base.
 __device__ inline uint32_t ballot_bin(bool valid, uint32_t key, uint32_t* counts) {
     const int lane = threadIdx.x & 63;
     uint64_t pending = __ballot(valid);
-    uint32_t slot = 0;
+    int leader_of = 0;
+    uint32_t rank = 0, cnt = 0;
     while (pending) {
         const int leader = __ffsll((unsigned long long)pending) - 1;
-        const uint32_t lk = __shfl(key, leader, 64);
+        const uint32_t lk = __builtin_amdgcn_readlane(key, leader);
         const uint64_t same = __ballot(valid && key == lk) & pending;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&counts[lk], (uint32_t)__popcll(same));
-        base = __shfl(base, leader, 64);
-        if ((same >> lane) & 1ull) slot = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        if ((same >> lane) & 1ull) {
+            leader_of = leader;
+            rank = (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        }
+        if (lane == leader) cnt = (uint32_t)__popcll(same);
         pending &= ~same;
     }
-    return slot;
+    uint32_t base = 0;
+    if (valid && leader_of == lane) base = atomicAdd(&counts[key], cnt);
+    base = __shfl(base, leader_of, 64);
+    return base + rank;
 }
 
 // rank inside the bin = number of bin members with a smaller index
