@@ -348,11 +348,14 @@ int pe_get_field(pe_ensemble* e, uint32_t member_id, uint32_t field, void* out);
 int pe_member_info(pe_ensemble* e, uint32_t member_id, uint64_t* bytes, int64_t* counts);
 int pe_info(pe_ensemble* e, uint32_t* n_members, uint64_t* total_cells, uint64_t* total_voxel_values,
             uint64_t* workspace_bytes);
-/* Persistent-grid size (workgroups) of the batched sweep along axis 0..2. */
+/* Grid of the batched sweeps along axis 0..2: 0 (default) launches one
+ * workgroup per work item; a positive value launches a persistent grid of
+ * that many workgroups, each walking a contiguous run of items. */
 int pe_set_grid_blocks(pe_ensemble* e, int axis, uint32_t blocks);
-/* Autotuner: times each axis's batched sweep at candidate grid sizes on the
- * current members (fields restored afterwards), keeps the fastest; writes the
- * chosen sizes and their mean launch times (us). */
+/* Autotuner: times each batched sweep list at candidate grids (0 and
+ * 256..16384 workgroups) on the current members (fields restored
+ * afterwards) and keeps the fastest; writes, per axis, the grid of its
+ * busiest list and the summed mean launch times (us). */
 int pe_autotune(pe_ensemble* e, int reps, uint32_t* blocks_out, double* us_out);
 hc_ctx* pe_context(pe_ensemble* e);
 

@@ -337,10 +337,17 @@ struct Member {
 };
 
 class Ensemble {
-    // sweep work lists: x with 4 segments per line, y, z, x with 16 segments
-    static constexpr int NLIST = 4;
-    static constexpr int LIST_AXIS[NLIST] = {0, 1, 2, 0};
-    static constexpr int LIST_P[NLIST] = {4, 4, 4, 16};
+    // sweep work lists, one per (axis, segments per line, register window),
+    // in launch order: every x list, then y, then z
+    static constexpr int NLIST = 7;
+    static constexpr int LIST_AXIS[NLIST] = {0, 0, 0, 1, 1, 2, 2};
+    static constexpr int LIST_P[NLIST] = {4, 4, 16, 4, 4, 4, 4};
+    static constexpr int LIST_MM[NLIST] = {16, 32, 16, 16, 32, 16, 32};
+    static int list_of(int axis, int P, int mm) {
+        for (int li = 0; li < NLIST; li++)
+            if (LIST_AXIS[li] == axis && LIST_P[li] == P && LIST_MM[li] == mm) return li;
+        return -1;
+    }
 
 public:
     Ensemble(int device, bool mixed) : mixed_(mixed) {
@@ -536,7 +543,7 @@ public:
     void set_grid_blocks(int axis, unsigned b) {
         if (axis < 0 || axis > 2) throw std::runtime_error("ensemble: axis must be 0..2");
         for (int li = 0; li < NLIST; li++)
-            if (LIST_AXIS[li] == axis) grid_blocks_[li] = b ? b : 1024;
+            if (LIST_AXIS[li] == axis) grid_blocks_[li] = b;
         drop_graph();
     }
     hc_ctx* ctx() { return ctx_; }
@@ -559,10 +566,12 @@ public:
         hipEvent_t a, b;
         hipEventCreate(&a);
         hipEventCreate(&b);
-        const unsigned cand[] = {256, 512, 1024, 2048, 4096, 8192, 16384};
+        const unsigned cand[] = {0, 256, 512, 1024, 2048, 4096, 8192, 16384};
         Workspace& w = ws_;
+        bool counted[3] = {false, false, false};
+        uint32_t busiest[3] = {0, 0, 0};
         for (int ax = 0; ax < 3; ax++) {
-            blocks_out[ax] = grid_blocks_[ax];
+            blocks_out[ax] = 0;
             us_out[ax] = 0.0;
         }
         for (int li = 0; li < NLIST; li++) {
@@ -570,7 +579,7 @@ public:
             double best = 1e300;
             unsigned pick = grid_blocks_[li];
             for (unsigned c : cand) {
-                if (c > 2 * w.nwork[li] && c != cand[0]) break;
+                if (c > 2 * w.nwork[li]) break;
                 sweep_list(li, c);
                 hipEventRecord(a, st);
                 for (int r = 0; r < reps; r++) sweep_list(li, c);
@@ -586,13 +595,13 @@ public:
             }
             grid_blocks_[li] = pick;
             const int ax = LIST_AXIS[li];
-            // per axis: the x entry reports the 4-segment list when it has work
-            if (li < 3 || !w.nwork[0]) {
+            // per axis: the grid of the axis's busiest list, the summed time
+            if (!counted[ax] || w.nwork[li] > busiest[ax]) {
                 blocks_out[ax] = pick;
-                us_out[ax] = best;
-            } else {
-                us_out[ax] += best;
+                busiest[ax] = w.nwork[li];
             }
+            us_out[ax] += best;
+            counted[ax] = true;
         }
         hipEventDestroy(a);
         hipEventDestroy(b);
@@ -609,8 +618,8 @@ private:
     struct Workspace {
         DeviceBuffer sims, bsims, tiles, work[NLIST], agent_sim, kglob, slot, tmp, sorted, recs, counts, offsets, partial;
         DeviceBuffer pcounts, poffsets, pkv, pslot, ptmp, psorted, pflag, pgsim, pgvox, pgbegin, pcoef, phdr, ppartial;
-        uint32_t ntot = 0, nbins = 0, vtot = 0, ntiles = 0, nwork[NLIST] = {0, 0, 0, 0}, ns_max = 0, nmax_all = 0;
-        uint32_t max_line[NLIST] = {0, 0, 0, 0};
+        uint32_t ntot = 0, nbins = 0, vtot = 0, ntiles = 0, nwork[NLIST] = {}, ns_max = 0, nmax_all = 0;
+        uint32_t max_line[NLIST] = {};
         size_t bytes = 0;
     };
 
@@ -663,7 +672,7 @@ private:
         std::vector<BatchWork> work[NLIST];
         std::vector<uint32_t> agent_sim;
         uint32_t cell_off = 0, bin_off = 0, vox_off = 0, ns_max = 0, nmax_all = 0;
-        uint32_t max_line[NLIST] = {0, 0, 0, 0};
+        uint32_t max_line[NLIST] = {};
         for (uint32_t si = 0; si < members_.size(); si++) {
             Member& m = *members_[si];
             const hc_grid& g = m.grid;
@@ -693,7 +702,7 @@ private:
             for (int ax = 0; ax < 3; ax++) {
                 if (ax == 2 && g.nz == 1) continue;
                 const int P = hc::part_p(ax, nn[ax], mixed_);
-                const int li = (ax == 0 && P == 16) ? 3 : ax;
+                const int li = list_of(ax, P, hc::part_window(nn[ax], P));
                 const uint64_t lines = hc::sweep_lines(ax, g.nx, g.ny, g.nz);
                 const uint64_t groups = (lines + 4 * (64 / P) - 1) / (4 * (64 / P));
                 for (uint32_t s = 0; s < g.n_substrates; s++)
@@ -750,14 +759,14 @@ private:
     // own last sweep (z in 3-D, y in 2-D)
     void sweep_list(int li, unsigned blocks) {
         Workspace& w = ws_;
-        check(ctx_, hc::launch_sweep_batch(ctx_, LIST_AXIS[li], LIST_P[li], mixed_,
+        check(ctx_, hc::launch_sweep_batch(ctx_, LIST_AXIS[li], LIST_P[li], LIST_MM[li], mixed_,
                                            (const BatchSim*)hc_device_ptr(w.bsims.get()),
                                            (const BatchWork*)hc_device_ptr(w.work[li].get()), w.nwork[li], w.nmax_all,
                                            w.max_line[li], blocks),
               "ensemble sweep");
     }
     void sweeps() {
-        for (int li : {0, 3, 1, 2}) sweep_list(li, grid_blocks_[li]);
+        for (int li = 0; li < NLIST; li++) sweep_list(li, grid_blocks_[li]);
     }
 
     void plan_secretion() {
@@ -925,7 +934,7 @@ private:
     hc_ctx* ctx_ = nullptr;
     bool mixed_ = false, dirty_ = true, plan_dirty_ = true;
     uint32_t next_id_ = 0;
-    unsigned grid_blocks_[NLIST] = {1024, 1024, 1024, 1024};
+    unsigned grid_blocks_[NLIST] = {};                 // 0: one workgroup per work item
     void* graph_ = nullptr;
     std::vector<std::unique_ptr<Member>> members_;
     Workspace ws_;

@@ -193,9 +193,13 @@ int launch_sweep_part(hc_ctx* ctx, int axis, void* rho, const hc_grid* g, const 
                       int fuse_dirichlet, uint32_t s0 = 0, uint32_t scount = 0);
 // 1 when every sweep of the grid takes the partitioned form
 bool part_fits_all(const hc_grid* g, int sweep_mode);
-// P: the segments per line of every member in the work list (part_p)
-int launch_sweep_batch(hc_ctx* ctx, int axis, int P, bool fp32, const BatchSim* sims, const BatchWork* work,
+// P: the segments per line of every member in the work list (part_p); mm:
+// their register window (16 or 32); grid_blocks 0: one workgroup per item,
+// else a persistent grid of that many workgroups
+int launch_sweep_batch(hc_ctx* ctx, int axis, int P, int mm, bool fp32, const BatchSim* sims, const BatchWork* work,
                        uint32_t nwork, uint32_t nmax_all, uint32_t max_line, unsigned grid_blocks);
+// register window of a line of n values in P segments (0: does not fit)
+int part_window(uint32_t n, int P);
 void build_part_tables_host(const hc_grid* g, const hc_substrate* subs, double* out, float* fout);
 void build_tables_host(const hc_grid* g, const hc_substrate* subs, std::vector<uint8_t>& out);
 inline size_t field_elem(const hc_grid* g) { return g->field_fp32 ? sizeof(float) : sizeof(double); }

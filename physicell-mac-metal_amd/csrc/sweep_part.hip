@@ -357,6 +357,25 @@ __global__ __launch_bounds__(256) void k_sweep_part_batch(const BatchSim* __rest
     }
 }
 
+// packed ensemble, one workgroup per work item (no loop: fewer registers)
+template <int AXIS, int P, int MM, bool XLDS, typename T>
+__global__ __launch_bounds__(256) void k_sweep_part_item(const BatchSim* __restrict__ sims,
+                                                         const BatchWork* __restrict__ work, uint32_t nmax_all) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* tG = reinterpret_cast<double*>(smem_raw);
+    T* tarr = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::g_count() * sizeof(double));
+    T* tile = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::bytes(nmax_all));
+    const BatchWork it = work[blockIdx.x];
+    const BatchSim sm = sims[it.sim];
+    load_part_tables<P, T>((const global_t<uint8_t>*)sm.tab, sm.ns, sm.nmax, it.s, AXIS, tG, tarr);
+    const global_t<SubConst>* sc = (const global_t<SubConst>*)sm.tab;
+    T* rs = reinterpret_cast<T*>(sm.rho) + (uint64_t)it.s * sm.nx * sm.ny * sm.nz;
+    __syncthreads();
+    sweep_group<AXIS, P, MM, XLDS, T>(rs, sm.nx, sm.ny, sm.nz, part_mt(sm.nmax, P), tarr, tG, sc[it.s].c[AXIS],
+                                      sc[it.s].T, sc[it.s].dir != 0, sm.last_axis == (uint32_t)AXIS ? 1 : 0, it.grp,
+                                      tile, sm.xmap);
+}
+
 int probe_kernels_part(hc_ctx* ctx) {
     hipFuncAttributes attr;
     const void* fns[] = {
@@ -369,14 +388,18 @@ int probe_kernels_part(hc_ctx* ctx) {
         (const void*)k_sweep_part<1, 4, 16, false, float>, (const void*)k_sweep_part<1, 4, 32, false, float>,
         (const void*)k_sweep_part<1, 4, 64, false, float>, (const void*)k_sweep_part<2, 4, 16, false, float>,
         (const void*)k_sweep_part<2, 4, 32, false, float>, (const void*)k_sweep_part<2, 4, 64, false, float>,
-        (const void*)k_sweep_part_batch<0, 16, 16, true, double>, (const void*)k_sweep_part_batch<1, 4, 32, false, double>,
-        (const void*)k_sweep_part_batch<2, 4, 32, false, double>, (const void*)k_sweep_part_batch<0, 16, 16, true, float>,
-        (const void*)k_sweep_part_batch<1, 4, 32, false, float>, (const void*)k_sweep_part_batch<2, 4, 32, false, float>,
         (const void*)k_sweep_part<0, 4, 16, true, double>, (const void*)k_sweep_part<0, 4, 32, true, double>,
         (const void*)k_sweep_part<0, 4, 16, false, double>, (const void*)k_sweep_part<0, 4, 32, false, double>,
         (const void*)k_sweep_part<0, 4, 16, true, float>, (const void*)k_sweep_part<0, 4, 32, true, float>,
         (const void*)k_sweep_part<0, 4, 16, false, float>, (const void*)k_sweep_part<0, 4, 32, false, float>,
-        (const void*)k_sweep_part_batch<0, 4, 32, true, double>, (const void*)k_sweep_part_batch<0, 4, 32, true, float>};
+        (const void*)k_sweep_part_batch<0, 16, 16, true, double>, (const void*)k_sweep_part_item<0, 16, 16, true, double>,
+        (const void*)k_sweep_part_batch<0, 4, 32, true, double>, (const void*)k_sweep_part_item<0, 4, 32, true, double>,
+        (const void*)k_sweep_part_batch<1, 4, 32, false, double>, (const void*)k_sweep_part_item<1, 4, 16, false, double>,
+        (const void*)k_sweep_part_batch<2, 4, 32, false, double>, (const void*)k_sweep_part_item<2, 4, 16, false, double>,
+        (const void*)k_sweep_part_batch<0, 16, 16, true, float>, (const void*)k_sweep_part_item<0, 16, 16, true, float>,
+        (const void*)k_sweep_part_batch<0, 4, 32, true, float>, (const void*)k_sweep_part_item<0, 4, 16, true, float>,
+        (const void*)k_sweep_part_batch<1, 4, 32, false, float>, (const void*)k_sweep_part_item<1, 4, 32, false, float>,
+        (const void*)k_sweep_part_batch<2, 4, 32, false, float>, (const void*)k_sweep_part_item<2, 4, 32, false, float>};
     for (const void* f : fns) {
         hipError_t e = hipFuncGetAttributes(&attr, f);
         if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: partitioned sweep");
@@ -385,7 +408,7 @@ int probe_kernels_part(hc_ctx* ctx) {
 }
 
 // window size for a line of n nodes split into P segments (0 = does not fit)
-static int part_window(uint32_t n, int P) {
+int part_window(uint32_t n, int P) {
     if (n < 2 + (uint32_t)P) return 0;          // every segment holds at least one unknown
     const uint32_t m = (n - 2 + P - 1) / P;
     return m <= 16 ? 16 : (m <= 32 ? 32 : (m <= 64 && P == 4 ? 64 : 0));
@@ -462,43 +485,43 @@ int launch_sweep_part(hc_ctx* ctx, int axis, void* rho, const hc_grid* g, const 
     return launch_sweep_part_t<double>(ctx, axis, (double*)rho, g, tables, fuse, s0, scount);
 }
 
+template <int AX, int PP, int M, bool X, typename T>
+static void launch_batch_kernel(hc_ctx* ctx, unsigned grid_blocks, size_t shm, const BatchSim* sims,
+                                const BatchWork* work, uint32_t nwork, uint32_t nmax_all) {
+    if (grid_blocks == 0)
+        hipLaunchKernelGGL((k_sweep_part_item<AX, PP, M, X, T>), dim3(nwork), dim3(256), shm, ctx->stream, sims, work,
+                           nmax_all);
+    else
+        hipLaunchKernelGGL((k_sweep_part_batch<AX, PP, M, X, T>), dim3(std::min<unsigned>(grid_blocks, nwork)),
+                           dim3(256), shm, ctx->stream, sims, work, nwork, nmax_all);
+}
+
 template <typename T>
-static int launch_batch_t(hc_ctx* ctx, int axis, int P, const BatchSim* sims, const BatchWork* work, uint32_t nwork,
-                          uint32_t nmax_all, uint32_t max_line, unsigned grid_blocks) {
-    if (axis != 0 && P != 4) return fail(ctx, HC_ERR_ARG, "batched sweep: y and z use 4 segments per line");
-    const int mm = part_window(max_line, P);
+static int launch_batch_t(hc_ctx* ctx, int axis, int P, int mm, const BatchSim* sims, const BatchWork* work,
+                          uint32_t nwork, uint32_t nmax_all, uint32_t max_line, unsigned grid_blocks) {
     if (nwork == 0) return 0;
-    if (!mm) return fail(ctx, HC_ERR_CAPACITY, "batched sweep: line length outside the partitioned window");
+    if (axis != 0 && P != 4) return fail(ctx, HC_ERR_ARG, "batched sweep: y and z use 4 segments per line");
+    if (part_window(max_line, P) > mm || (mm != 16 && mm != 32) || (P == 16 && mm != 16))
+        return fail(ctx, HC_ERR_CAPACITY, "batched sweep: line length outside the list's register window");
     const int LPW = 64 / P;
     const size_t tab_bytes = P == 16 ? PartLds<16, T>::bytes(nmax_all) : PartLds<4, T>::bytes(nmax_all);
     const size_t shm = tab_bytes + (axis == 0 ? (size_t)4 * xtile_elems<T>(LPW, max_line) * sizeof(T) : 0);
     if (shm > 64 * 1024) return fail(ctx, HC_ERR_CAPACITY, "batched sweep: LDS");
-    dim3 grid(std::min<unsigned>(grid_blocks, nwork));
     KernelTimer tm(ctx, HC_K_SWEEP_X + axis);
-#define HC_BATCH(AX, PP, M, X)                                                                                   \
-    hipLaunchKernelGGL((k_sweep_part_batch<AX, PP, M, X, T>), grid, dim3(256), shm, ctx->stream, sims, work, nwork, \
-                       nmax_all)
-    if (axis == 0 && P == 4) {
-        if (mm > 32) return fail(ctx, HC_ERR_CAPACITY, "batched x sweep: lines longer than 130 with 4 segments");
-        HC_BATCH(0, 4, 32, true);
-    } else if (axis == 0) {
-        if (mm != 16) return fail(ctx, HC_ERR_CAPACITY, "batched x sweep: lines longer than 258");
-        HC_BATCH(0, 16, 16, true);
-    } else if (axis == 1) {
-        if (mm > 32) return fail(ctx, HC_ERR_CAPACITY, "batched y sweep: lines longer than 130");
-        HC_BATCH(1, 4, 32, false);
-    } else {
-        if (mm > 32) return fail(ctx, HC_ERR_CAPACITY, "batched z sweep: lines longer than 130");
-        HC_BATCH(2, 4, 32, false);
-    }
-#undef HC_BATCH
-    return check_launch(ctx, "k_sweep_part_batch");
+    if (axis == 0 && P == 16) launch_batch_kernel<0, 16, 16, true, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
+    else if (axis == 0 && mm == 16) launch_batch_kernel<0, 4, 16, true, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
+    else if (axis == 0) launch_batch_kernel<0, 4, 32, true, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
+    else if (axis == 1 && mm == 16) launch_batch_kernel<1, 4, 16, false, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
+    else if (axis == 1) launch_batch_kernel<1, 4, 32, false, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
+    else if (mm == 16) launch_batch_kernel<2, 4, 16, false, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
+    else launch_batch_kernel<2, 4, 32, false, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
+    return check_launch(ctx, "batched sweep");
 }
 
-int launch_sweep_batch(hc_ctx* ctx, int axis, int P, bool fp32, const BatchSim* sims, const BatchWork* work,
+int launch_sweep_batch(hc_ctx* ctx, int axis, int P, int mm, bool fp32, const BatchSim* sims, const BatchWork* work,
                        uint32_t nwork, uint32_t nmax_all, uint32_t max_line, unsigned grid_blocks) {
-    if (fp32) return launch_batch_t<float>(ctx, axis, P, sims, work, nwork, nmax_all, max_line, grid_blocks);
-    return launch_batch_t<double>(ctx, axis, P, sims, work, nwork, nmax_all, max_line, grid_blocks);
+    if (fp32) return launch_batch_t<float>(ctx, axis, P, mm, sims, work, nwork, nmax_all, max_line, grid_blocks);
+    return launch_batch_t<double>(ctx, axis, P, mm, sims, work, nwork, nmax_all, max_line, grid_blocks);
 }
 
 // Host construction of the partitioned tables (FP64, plus an FP32 rounding

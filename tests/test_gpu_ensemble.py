@@ -136,7 +136,7 @@ def test_autotune_keeps_results():
     ens.run_periods(2)
     blocks, us = ens.autotune(reps=2)
     print("autotune", blocks, us, flush=True)
-    assert all(b >= 256 for b in blocks)
+    assert all(b == 0 or b >= 256 for b in blocks)    # 0: one workgroup per work item
     ens.run_periods(2)
     for i, r in zip(ids, r0):
         assert_same(member_state(ens, i), r, "after autotune")
