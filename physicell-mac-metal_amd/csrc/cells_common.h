@@ -237,19 +237,21 @@ __device__ __attribute__((always_inline)) inline void force_range32(
             const float ox = u < mid0 ? oxm : (u < mid1 ? ox0 : oxp);
             const float ddx = mx - o.x - ox, ddy = my - o.y - oy, ddz = mz - o.z - oz;
             const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            const float dist = sqrtf(d2);
-            const float inv_dist = 1.0f / dist;
+            // FP32 form: the hardware square root and reciprocal (about 1 ulp)
+            // instead of the correctly rounded sequences
+            const float dist = __builtin_amdgcn_sqrtf(d2);
+            const float inv_dist = __builtin_amdgcn_rcpf(dist);
             const float R_sum = radius_i + o.w;
             float force_mag = 0.f;
             if (dist < R_sum) {
-                const float overlap = 1.0f - dist / R_sum;
-                force_mag += overlap * overlap * sqrtf(rep_i * rec[u].rep);
-                local_pressure += overlap * overlap / 0.027288820670331f;
+                const float overlap = 1.0f - dist * __builtin_amdgcn_rcpf(R_sum);
+                force_mag += overlap * overlap * __builtin_amdgcn_sqrtf(rep_i * rec[u].rep);
+                local_pressure += overlap * overlap * (1.0f / 0.027288820670331f);
             }
             const float adhesion_distance = rel_max_adh * R_sum;
             if (dist < adhesion_distance) {
-                const float t2 = 1.0f - dist / adhesion_distance;
-                force_mag -= t2 * t2 * sqrtf(adh_i * rec[u].adh);
+                const float t2 = 1.0f - dist * __builtin_amdgcn_rcpf(adhesion_distance);
+                force_mag -= t2 * t2 * __builtin_amdgcn_sqrtf(adh_i * rec[u].adh);
             }
             fx += force_mag * (ddx * inv_dist);
             fy += force_mag * (ddy * inv_dist);
