@@ -46,16 +46,17 @@ __device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
 }
 
 // Ballot binning: returns the slot of this lane's key inside its bin.  The
-// lanes are grouped by key with ballots (no memory traffic in the loop); then
-// each group's lowest lane adds the group's size to the bin counter with one
-// atomic, all groups' atomics in flight together, and the members take
-// consecutive slots from the returned base.
+// lanes are grouped by key with ballots (no memory traffic in the loop), at
+// most 8 groups; each group's lowest lane adds the group's size to the bin
+// counter with one atomic, and lanes left after 8 groups (a wave whose keys
+// are mostly distinct) add 1 on their own.  All atomics of the wave are in
+// flight together; members take consecutive slots from the returned base.
 __device__ inline uint32_t ballot_bin(bool valid, uint32_t key, uint32_t* counts) {
     const int lane = threadIdx.x & 63;
     uint64_t pending = __ballot(valid);
-    int leader_of = 0;
-    uint32_t rank = 0, cnt = 0;
-    while (pending) {
+    int leader_of = lane;
+    uint32_t rank = 0, cnt = 1;
+    for (int g = 0; g < 8 && pending; g++) {
         const int leader = __ffsll((unsigned long long)pending) - 1;
         const uint32_t lk = __builtin_amdgcn_readlane(key, leader);
         const uint64_t same = __ballot(valid && key == lk) & pending;
