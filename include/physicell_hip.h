@@ -311,6 +311,38 @@ int pc_sim_substrate_totals(pc_sim* sim, double* out);
 int pc_sim_info(const pc_sim* sim, hc_grid* grid, hc_mech* mech, uint32_t* n_cells);
 hc_ctx* pc_sim_context(pc_sim* sim);
 
+/* ==== MATLAB Level-4 snapshots in the reference's output format =========== */
+
+/* One matrix file as src/matlab_writer.cpp:5-49 writes it: uint32 header
+ * {type 0 (little-endian double, full), rows, cols, 0, name length + 1}, the
+ * name with its NUL, then rows*cols doubles, column-major
+ * (data[c*rows + r] = M[r][c]).  Host only. */
+int pc_write_matlab4(const char* path, const char* name, uint32_t rows, uint32_t cols, const double* data);
+
+/* Host view of one snapshot of the path's state. */
+typedef struct pc_snapshot {
+    uint32_t n_cells, n_substrates;
+    const double *x, *y, *z, *total_volume, *nuclear_volume, *radius, *nuclear_radius;
+    const double *vx, *vy, *vz, *pressure, *adhesion, *repulsion, *rel_max_adhesion; /* may be NULL */
+    const uint32_t *cell_type, *current_phase, *is_alive;                          /* may be NULL */
+    const double *secretion, *uptake, *saturation, *net_export;     /* [cell*ns + s], may be NULL */
+    uint32_t nx, ny, nz, _pad;
+    double dx, dy, dz, x_min, y_min, z_min;
+    const double* density;      /* ns*nx*ny*nz substrate-major; NULL: no microenvironment file */
+} pc_snapshot;
+
+/* writeMatlabSnapshots (src/output_writer.cpp:295-453): writes
+ * <folder>/output%08d_cells.mat (one column per cell, the reference's row
+ * schema with one cell type and no custom variables; rows outside the path
+ * hold a placed reference cell's values) and
+ * <folder>/output%08d_microenvironment0.mat (one column per voxel: centre and
+ * volume computed in FP32 as the reference does, then the densities).  Host
+ * only. */
+int pc_write_snapshot(const char* folder, int frame, const pc_snapshot* snapshot);
+/* The simulation's current state through pc_write_snapshot (waits, copies the
+ * SoA and density to the host). */
+int pc_sim_write_snapshot(pc_sim* sim, const char* folder, int frame);
+
 /* ==== pe_*: packed ensembles of independent simulations ====================
  * No counterpart in the reference (which runs one simulation per process):
  * many simulations of uneven grid and population size are stepped together.

@@ -1,7 +1,8 @@
 #!/bin/sh
 # Builds oracle/_ref/libref_path.so from the reference's own C++ sources where
 # they lie under /root/reference (nothing is copied into this repository):
-# the minimal set that links processSecretion with no undefined symbols.
+# the minimal set that links processSecretion and the .mat snapshot writer
+# (output_writer.cpp, matlab_writer.cpp) with no undefined symbols.
 # Test infrastructure only.  Exits 0 without building when the reference is
 # absent (the GPU box has no /root/reference).
 set -e
@@ -18,6 +19,7 @@ g++ -std=c++17 -O2 -fPIC -shared -Wl,--no-undefined -w \
   "$HERE/ref_driver.cpp" \
   "$REF/src/cell_data.cpp" "$REF/src/cell_phenotype.cpp" \
   "$REF/src/cell_definitions.cpp" "$REF/src/config_parser.cpp" \
+  "$REF/src/output_writer.cpp" "$REF/src/matlab_writer.cpp" \
   "$REF/lib/pugixml/pugixml.cpp" \
   -o "$OUT/libref_path.so.tmp"
 mv "$OUT/libref_path.so.tmp" "$OUT/libref_path.so"

@@ -91,10 +91,27 @@ uint32_t CellData::addCell() {
     const uint32_t i = num_cells++;
     for (uint32_t f = 0; f < HC_NUM_F64_FIELDS; f++) field(f)[i] = 0.0;
     for (uint32_t f = HC_NUM_F64_FIELDS; f < HC_NUM_CELL_FIELDS; f++) ufield(f)[i] = 0;
+    // the reference's fresh-cell values (src/cell_data.cpp:544-680): MCF-7
+    // volumes 2494 / 540 um^3 and their equivalent-sphere radii, mechanics,
+    // motility (speed 1, bias 0.5) and per-substrate rates; computed in FP64
+    const double pi = 3.14159265358979323846;
+    field(HC_F_TOTAL_VOL)[i] = 2494.0;
+    field(HC_F_NUCLEAR_VOL)[i] = 540.0;
+    field(HC_F_RADIUS)[i] = std::cbrt(3.0 * 2494.0 / (4.0 * pi));
+    field(HC_F_NUCLEAR_RAD)[i] = std::cbrt(3.0 * 540.0 / (4.0 * pi));
+    field(HC_F_REPULSION)[i] = 10.0;
+    field(HC_F_ADHESION)[i] = 0.4;
+    field(HC_F_REL_MAX_ADH)[i] = 1.25;
+    field(HC_F_MOT_SPEED)[i] = 1.0;
+    field(HC_F_MOT_BIAS)[i] = 0.5;
+    field(HC_F_ONCOPROTEIN)[i] = 1.0;
     is_alive[i] = 1;
     for (uint32_t s = 0; s < n_substrates; s++) {
         const size_t o = (size_t)i * n_substrates + s;
-        secretion_rate[o] = uptake_rate[o] = saturation_density[o] = net_export_rate[o] = 0.0;
+        secretion_rate[o] = 0.0;
+        uptake_rate[o] = 10.0;
+        saturation_density[o] = 1.0;
+        net_export_rate[o] = 0.0;
     }
     return i;
 }
@@ -578,6 +595,35 @@ int pc_sim_set_density(pc_sim* s, const double* in) {
             for (size_t v = 0; v < V; v++) m.microenv.setDensity((uint32_t)v, sb, in[sb * V + v]);
         m.microenv.syncToDevice();
     });
+}
+
+int pc_sim_write_snapshot(pc_sim* s, const char* folder, int frame) {
+    if (!folder) return HC_ERR_ARG;
+    int rc = 0;
+    const int g = guarded(s, [&](Simulation& m) {
+        m.cells.syncToHost();
+        CellData& c = m.cells;
+        const hc_grid gr = m.microenv.getGridParams();
+        pc_snapshot snap{};
+        snap.n_cells = c.num_cells;
+        snap.n_substrates = c.n_substrates;
+        snap.x = c.position_x; snap.y = c.position_y; snap.z = c.position_z;
+        snap.total_volume = c.total_volume; snap.nuclear_volume = c.nuclear_volume;
+        snap.radius = c.radius; snap.nuclear_radius = c.nuclear_radius;
+        snap.vx = c.velocity_x; snap.vy = c.velocity_y; snap.vz = c.velocity_z;
+        snap.pressure = c.simple_pressure; snap.adhesion = c.cell_cell_adhesion;
+        snap.repulsion = c.cell_cell_repulsion; snap.rel_max_adhesion = c.relative_max_adhesion_distance;
+        snap.cell_type = c.cell_type; snap.current_phase = c.current_phase; snap.is_alive = c.is_alive;
+        snap.secretion = c.secretion_rate; snap.uptake = c.uptake_rate;
+        snap.saturation = c.saturation_density; snap.net_export = c.net_export_rate;
+        snap.nx = gr.nx; snap.ny = gr.ny; snap.nz = gr.nz;
+        snap.dx = gr.dx; snap.dy = gr.dy; snap.dz = gr.dz;
+        snap.x_min = gr.x_min; snap.y_min = gr.y_min; snap.z_min = gr.z_min;
+        snap.density = m.microenv.densityBuffer();
+        rc = pc_write_snapshot(folder, frame, &snap);
+        if (rc) throw std::runtime_error(std::string("pc_sim_write_snapshot: cannot write under ") + folder);
+    });
+    return g ? g : rc;
 }
 
 int pc_sim_substrate_totals(pc_sim* s, double* out) {

@@ -89,8 +89,10 @@ public:
     double* field(uint32_t f);
     uint32_t* ufield(uint32_t f);
 
-    // CellData::addCell / removeCell (src/cell_data.cpp:398-542): append, and
-    // swap-with-last removal.  Host side; push with syncToDevice.
+    // CellData::addCell / removeCell (src/cell_data.cpp:398-542): append a
+    // cell with the reference's fresh-cell values (setDefaults: volumes,
+    // radii, mechanics, motility, rates), and swap-with-last removal.  Host
+    // side; push with syncToDevice.
     uint32_t addCell();
     void removeCell(uint32_t index);
 

@@ -77,6 +77,16 @@ class Config(ctypes.Structure):
 _lib = None
 
 
+class Snapshot(ctypes.Structure):
+    _fields_ = [("n_cells", c_u32), ("n_substrates", c_u32)] + \
+               [(f, c_vp) for f in ("x", "y", "z", "total_volume", "nuclear_volume", "radius", "nuclear_radius",
+                                    "vx", "vy", "vz", "pressure", "adhesion", "repulsion", "rel_max_adhesion",
+                                    "cell_type", "current_phase", "is_alive", "secretion", "uptake", "saturation",
+                                    "net_export")] + \
+               [("nx", c_u32), ("ny", c_u32), ("nz", c_u32), ("_pad", c_u32)] + \
+               [(f, c_f64) for f in ("dx", "dy", "dz", "x_min", "y_min", "z_min")] + [("density", c_vp)]
+
+
 def declared_functions(header=HEADER):
     """Every function name declared in include/physicell_hip.h."""
     txt = open(header).read()
@@ -140,6 +150,9 @@ def load(path=LIB_PATH):
         "pc_sim_substrate_totals": (c_i32, [c_vp, c_vp]),
         "pc_sim_info": (c_i32, [c_vp, P(Grid), P(Mech), P(c_u32)]),
         "pc_sim_context": (c_vp, [c_vp]),
+        "pc_write_matlab4": (c_i32, [ctypes.c_char_p, ctypes.c_char_p, c_u32, c_u32, c_vp]),
+        "pc_write_snapshot": (c_i32, [ctypes.c_char_p, c_i32, P(Snapshot)]),
+        "pc_sim_write_snapshot": (c_i32, [c_vp, ctypes.c_char_p, c_i32]),
         "pe_create": (c_i32, [P(c_vp), c_i32, c_u32]), "pe_destroy": (None, [c_vp]),
         "pe_last_error": (ctypes.c_char_p, [c_vp]),
         "pe_add": (c_i32, [c_vp, P(Config), c_u32] + [c_vp] * 13 + [P(c_u32)]),
@@ -400,6 +413,9 @@ class Simulation:
 
     def context(self):
         return load().pc_sim_context(self.h)
+
+    def write_snapshot(self, folder, frame):
+        self._check(load().pc_sim_write_snapshot(self.h, str(folder).encode(), frame), "pc_sim_write_snapshot")
 
     def timing(self, mask):
         rc = load().hc_timing_enable(self.context(), mask)
