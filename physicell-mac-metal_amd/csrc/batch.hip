@@ -77,14 +77,15 @@ __global__ __launch_bounds__(256) void k_b_mech_keys(const EnsSim* __restrict__ 
 
 // stable placement + hash-sorted records (FP64 or bin-relative FP32)
 template <bool FP32>
-__global__ __launch_bounds__(256) void k_b_gather(const EnsSim* __restrict__ sims, const uint32_t* __restrict__ agent_sim,
+__global__ __launch_bounds__(256) void k_b_gather(const EnsSim* __restrict__ sims, const AgentTile* __restrict__ tiles,
                                                   const uint32_t* __restrict__ kglob, const uint32_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ tmp, uint32_t* __restrict__ sorted,
-                                                  void* __restrict__ recs, uint32_t ntot) {
-    const uint32_t g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= ntot) return;
-    const EnsSim m = sims[agent_sim[g]];
-    const uint32_t c = g - m.cell_off;
+                                                  void* __restrict__ recs) {
+    const AgentTile tl = tiles[blockIdx.x];
+    const EnsSim m = sims[tl.sim];                       // one member per workgroup
+    const uint32_t c = tl.c0 + threadIdx.x;
+    if (c >= m.n) return;
+    const uint32_t g = m.cell_off + c;
     const uint32_t b = kglob[g];
     const uint32_t beg = offsets[b], end = offsets[b + 1];
     const uint32_t pos = beg + stable_rank(tmp, beg, end, g);
@@ -845,11 +846,12 @@ private:
             check(ctx_, hc::check_launch(ctx_, "k_b_mech_keys"), "keys");
             check(ctx_, hc::scan_exclusive(ctx_, counts, offsets, w.nbins, (uint32_t*)hc_device_ptr(w.partial.get())), "scan");
             check(ctx_, hc::launch_scatter(ctx_, kglob, slot, offsets, tmp, w.ntot), "scatter");
-            const dim3 g(hc::blocks_for(w.ntot, 256));
             if (mixed_)
-                hipLaunchKernelGGL(hc::k_b_gather<true>, g, dim3(256), 0, st, sims, asim, kglob, offsets, tmp, sorted, recs, w.ntot);
+                hipLaunchKernelGGL(hc::k_b_gather<true>, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, kglob, offsets, tmp,
+                                   sorted, recs);
             else
-                hipLaunchKernelGGL(hc::k_b_gather<false>, g, dim3(256), 0, st, sims, asim, kglob, offsets, tmp, sorted, recs, w.ntot);
+                hipLaunchKernelGGL(hc::k_b_gather<false>, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, kglob, offsets,
+                                   tmp, sorted, recs);
             check(ctx_, hc::check_launch(ctx_, "k_b_gather"), "gather");
         }
         {
