@@ -95,6 +95,7 @@ def bench_ensemble(args):
         ens = H.Ensemble(mixed=mixed)
         for sc, c in members:
             ens.add(sc, c)
+        ens.set_overlap(not args.no_overlap)
         blocks, us = ens.autotune(reps=3)
         ens.run_periods(args.warmup)
         ens.wait()
@@ -108,6 +109,7 @@ def bench_ensemble(args):
             ens.run_periods(1)
             ens.wait()
             lat.append(1e3 * (time.perf_counter() - t1))
+        ens.set_overlap(False)
         ens.timing(0xFFFFFFFF)
         ens.run_periods(2, use_graph=False)
         ens.wait()
@@ -127,8 +129,12 @@ def bench_ensemble(args):
     print(f"[bench] batched mixed: {res['batched_mixed']['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
     res["batched_fp64"] = time_ensemble(False)
     print(f"[bench] batched FP64: {res['batched_fp64']['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
+    # the per-member path at its faster setting: with 64 members on 64
+    # streams the device is already busy, and a second stream per member
+    # for the mechanics step (overlap) measured slower (15.0 vs 11.3 ms/step)
     sims = [H.Simulation(sc, c) for sc, c in members]
     for s in sims:
+        s.set_overlap(False)
         s.run_periods(args.warmup, use_graph=True)
     for s in sims:
         s.wait()
@@ -140,7 +146,8 @@ def bench_ensemble(args):
         s.wait()
     T = time.perf_counter() - t0
     res["plain_fp64_per_sim"] = {"ms_per_step": 1e3 * T / args.steps, "agent_steps_per_s": n_cells * args.steps / T,
-                                 "voxel_substeps_per_s": n_vals * n_sub * args.steps / T}
+                                 "voxel_substeps_per_s": n_vals * n_sub * args.steps / T,
+                                 "setting": "one pc_sim per member, own stream, period graph, no overlap"}
     for s in sims:
         s.close()
     print(f"[bench] plain FP64 per-sim: {res['plain_fp64_per_sim']['ms_per_step']:.2f} ms/step",
@@ -202,6 +209,8 @@ def main():
     ap.add_argument("--workload", default="config2")
     ap.add_argument("--cells", type=int, default=1_000_000)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="run each period's mechanics step after its substeps instead of concurrently")
     ap.add_argument("--mixed", action="store_true", help="FP32-mixed form instead of plain FP64")
     ap.add_argument("--sweep-mode", type=int, default=0,
                     help="0 partitioned sweeps (default), 1 serial elimination, 2 partitioned with LDS-staged x")
@@ -236,6 +245,7 @@ def main():
     t_setup = time.perf_counter()
     spec, cells = build_workload(args.workload, args.cells)
     sim = H.Simulation(spec, cells, mixed=args.mixed)
+    sim.set_overlap(not args.no_overlap)
     if args.sweep_mode:
         rc = H.load().hc_set_sweep_mode(sim.context(), args.sweep_mode)
         if rc:
@@ -271,7 +281,9 @@ def main():
     lat = np.array(lat)
 
     # kernel timing: HIP events on the context stream around each launch
-    # (eager launches: events are not recorded inside graph replays)
+    # (eager launches: events are not recorded inside graph replays; the
+    # mechanics step is not overlapped here, so each kernel is timed alone)
+    sim.set_overlap(False)
     sim.timing(0xFFFFFFFF)
     kp = max(2, min(args.steps, 10))
     sim.run_periods(kp, use_graph=False)
@@ -321,7 +333,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{spec.name}: {spec.nx}x{spec.ny}x{spec.nz} voxels x {len(spec.substrates)} "
                                f"substrates, {cells.n} agents, {'FP32-mixed' if args.mixed else 'plain FP64'}; step = {n_sub} diffusion substeps "
-                               f"+ 1 mechanics step", "graph": use_graph, "sweep_mode": args.sweep_mode,
+                               f"+ 1 mechanics step", "graph": use_graph, "overlap": not args.no_overlap,
+                   "sweep_mode": args.sweep_mode,
                    "substep_order": args.substep_order},
         "voxel_substeps_per_s": vss,
         "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),

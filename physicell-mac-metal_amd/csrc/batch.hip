@@ -314,6 +314,7 @@ int probe_kernels_batch(hc_ctx* ctx) {
 // Host side: the ensemble (members, packing, step loop) and the pe_* C-ABI
 // ============================================================================
 extern "C" hipStream_t hc_stream_of(hc_ctx* ctx);
+extern "C" hipStream_t hc_stream_swap(hc_ctx* ctx, hipStream_t s);
 
 namespace physicell {
 
@@ -355,8 +356,13 @@ public:
         int rc = hc_create(&ctx_, device);
         if (rc) throw std::runtime_error("ensemble: no usable HIP device");
         rc = hc_load_kernels(ctx_, nullptr);
+        if (rc == 0 && (hipStreamCreateWithFlags(&side_, hipStreamNonBlocking) != hipSuccess ||
+                        hipEventCreateWithFlags(&fork_, hipEventDisableTiming) != hipSuccess ||
+                        hipEventCreateWithFlags(&join_, hipEventDisableTiming) != hipSuccess))
+            rc = HC_ERR_HIP;
         if (rc) {
-            std::string m = hc_last_error(ctx_);
+            std::string m = rc == HC_ERR_HIP ? "stream/event creation failed" : hc_last_error(ctx_);
+            release_streams();
             hc_destroy(ctx_);
             throw std::runtime_error("ensemble: " + m);
         }
@@ -366,7 +372,13 @@ public:
         hc_wait(ctx_);
         members_.clear();
         ws_ = Workspace();
+        release_streams();
         hc_destroy(ctx_);
+    }
+    // mechanics on a second stream, concurrent with the period's diffusion
+    void set_overlap(bool on) {
+        overlap_ = on;
+        drop_graph();
     }
 
     uint32_t add(const SimConfig& c_in, uint32_t n, const double* x, const double* y, const double* z,
@@ -870,14 +882,52 @@ private:
         }
     }
 
+    void release_streams() {
+        if (join_) hipEventDestroy(join_);
+        if (fork_) hipEventDestroy(fork_);
+        if (side_) hipStreamDestroy(side_);
+        join_ = fork_ = nullptr;
+        side_ = nullptr;
+    }
+
+    static void hip_ok(hipError_t e, const char* what) {
+        if (e != hipSuccess) throw std::runtime_error(std::string("ensemble ") + what + ": " + hipGetErrorString(e));
+    }
+
+    // One period.  Within a period the diffusion substeps read and write the
+    // fields and read the secretion plan, and the mechanics step reads and
+    // writes positions, velocities, pressure and the hash: disjoint data, so
+    // with overlap on the mechanics step runs on a second stream while the
+    // substeps run, and the period ends when both are done.  The results are
+    // the same either way.
     void period_kernels() {
         if (plan_dirty_) plan_secretion();
         const int k = substeps_per_mech();
+        if (!overlap_) {
+            for (int s = 0; s < k; s++) {
+                sweeps();
+                secretion();
+            }
+            mechanics();
+            return;
+        }
+        hipStream_t main = hc_stream_of(ctx_);
+        hip_ok(hipEventRecord(fork_, main), "fork");
+        hip_ok(hipStreamWaitEvent(side_, fork_, 0), "fork wait");
+        hc_stream_swap(ctx_, side_);
+        try {
+            mechanics();
+        } catch (...) {
+            hc_stream_swap(ctx_, main);
+            throw;
+        }
+        hc_stream_swap(ctx_, main);
+        hip_ok(hipEventRecord(join_, side_), "join");
         for (int s = 0; s < k; s++) {
             sweeps();
             secretion();
         }
-        mechanics();
+        hip_ok(hipStreamWaitEvent(main, join_, 0), "join wait");
     }
 
     void capture() {
@@ -934,6 +984,9 @@ private:
     }
 
     hc_ctx* ctx_ = nullptr;
+    hipStream_t side_ = nullptr;
+    hipEvent_t fork_ = nullptr, join_ = nullptr;
+    bool overlap_ = true;
     bool mixed_ = false, dirty_ = true, plan_dirty_ = true;
     uint32_t next_id_ = 0;
     unsigned grid_blocks_[NLIST] = {};                 // 0: one workgroup per work item
@@ -1057,6 +1110,10 @@ int pe_info(pe_ensemble* p, uint32_t* n_members, uint64_t* total_cells, uint64_t
 
 int pe_set_grid_blocks(pe_ensemble* p, int axis, uint32_t blocks) {
     return eguard(p, [&](Ensemble& e) { e.set_grid_blocks(axis, blocks); });
+}
+
+int pe_set_overlap(pe_ensemble* p, int on) {
+    return eguard(p, [&](Ensemble& e) { e.set_overlap(on != 0); });
 }
 
 int pe_autotune(pe_ensemble* p, int reps, uint32_t* blocks_out, double* us_out) {

@@ -204,6 +204,14 @@ const char* hc_last_error(const hc_ctx* ctx) { return ctx ? ctx->last_error.c_st
 
 // internal: the stream the context dispatches on (used by the host step loop for graph capture)
 hipStream_t hc_stream_of(hc_ctx* ctx) { return ctx ? ctx->stream : nullptr; }
+// internal: dispatch on another stream from now on; returns the previous one
+// (the host step loop runs the mechanics step on a second stream)
+hipStream_t hc_stream_swap(hc_ctx* ctx, hipStream_t s) {
+    if (!ctx) return nullptr;
+    hipStream_t old = ctx->stream;
+    ctx->stream = s;
+    return old;
+}
 
 int hc_timing_enable(hc_ctx* ctx, uint32_t mask) {
     if (!ctx) return HC_ERR_ARG;

@@ -11,6 +11,7 @@
 
 // the context's stream (internal export of context.hip; not part of the C-ABI header)
 extern "C" hipStream_t hc_stream_of(hc_ctx* ctx);
+extern "C" hipStream_t hc_stream_swap(hc_ctx* ctx, hipStream_t s);
 
 namespace physicell {
 
@@ -362,6 +363,9 @@ Simulation::Simulation(const SimConfig& cfg, int device)
 Simulation::~Simulation() {
     if (graph_exec_) hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
     hc_wait(ctx);
+    if (join_) hipEventDestroy((hipEvent_t)join_);
+    if (fork_) hipEventDestroy((hipEvent_t)fork_);
+    if (side_) hipStreamDestroy((hipStream_t)side_);
     // buffers are released by their owners before the context goes
     cells = CellData(1);
     microenv = Microenvironment();
@@ -457,13 +461,58 @@ int Simulation::substepsPerMechanicsStep() const {
     return 1;
 }
 
-void Simulation::runPeriods(int periods, bool use_graph) {
+void Simulation::setOverlap(bool on) {
+    overlap_ = on;
+    if (graph_exec_) {
+        hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
+        graph_exec_ = nullptr;
+    }
+}
+
+static void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// One period.  The substeps read and write the fields and read the secretion
+// plan; the mechanics step reads and writes positions, velocities, pressure
+// and the hash.  With overlap on, the mechanics step runs on a second stream
+// while the substeps run; the period ends when both are done.
+void Simulation::periodBody() {
     const int k = substepsPerMechanicsStep();
+    if (!overlap_) {
+        for (int s = 0; s < k; s++) diffusionSubstep();
+        mechanicsStep();
+        return;
+    }
+    if (!side_) {
+        hipStream_t st = nullptr;
+        hipEvent_t a = nullptr, b = nullptr;
+        hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "side stream");
+        side_ = st;
+        hip_check(hipEventCreateWithFlags(&a, hipEventDisableTiming), "fork event");
+        fork_ = a;
+        hip_check(hipEventCreateWithFlags(&b, hipEventDisableTiming), "join event");
+        join_ = b;
+    }
+    hipStream_t main = hc_stream_of(ctx), side = (hipStream_t)side_;
+    hip_check(hipEventRecord((hipEvent_t)fork_, main), "fork");
+    hip_check(hipStreamWaitEvent(side, (hipEvent_t)fork_, 0), "fork wait");
+    hc_stream_swap(ctx, side);
+    try {
+        mechanicsStep();
+    } catch (...) {
+        hc_stream_swap(ctx, main);
+        throw;
+    }
+    hc_stream_swap(ctx, main);
+    hip_check(hipEventRecord((hipEvent_t)join_, side), "join");
+    for (int s = 0; s < k; s++) diffusionSubstep();
+    hip_check(hipStreamWaitEvent(main, (hipEvent_t)join_, 0), "join wait");
+}
+
+void Simulation::runPeriods(int periods, bool use_graph) {
     if (!use_graph) {
-        for (int p = 0; p < periods; p++) {
-            for (int s = 0; s < k; s++) diffusionSubstep();
-            mechanicsStep();
-        }
+        for (int p = 0; p < periods; p++) periodBody();
         return;
     }
     hipStream_t stream = hc_stream_of(ctx);
@@ -473,8 +522,7 @@ void Simulation::runPeriods(int periods, bool use_graph) {
         hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) throw std::runtime_error(std::string("graph capture: ") + hipGetErrorString(e));
         try {
-            for (int s = 0; s < k; s++) diffusionSubstep();
-            mechanicsStep();
+            periodBody();
         } catch (...) {
             hipStreamEndCapture(stream, &graph);
             if (graph) hipGraphDestroy(graph);
@@ -624,6 +672,10 @@ int pc_sim_write_snapshot(pc_sim* s, const char* folder, int frame) {
         if (rc) throw std::runtime_error(std::string("pc_sim_write_snapshot: cannot write under ") + folder);
     });
     return g ? g : rc;
+}
+
+int pc_sim_set_overlap(pc_sim* s, int on) {
+    return guarded(s, [&](Simulation& m) { m.setOverlap(on != 0); });
 }
 
 int pc_sim_substrate_totals(pc_sim* s, double* out) {

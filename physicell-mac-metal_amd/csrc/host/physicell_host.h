@@ -194,7 +194,11 @@ public:
     void mechanicsStep();
     void phenotypeStep();
     void run(double max_time, int64_t counts[3]);
+    // periods x (substeps + one mechanics step); with overlap on (default),
+    // the mechanics step of a period runs on a second stream concurrently
+    // with its substeps (they touch disjoint data; same results)
     void runPeriods(int periods, bool use_graph);
+    void setOverlap(bool on);
     void wait();
     int substepsPerMechanicsStep() const;
 
@@ -206,7 +210,12 @@ public:
     std::string last_error;
 
 private:
+    void periodBody();
     void* graph_exec_ = nullptr;   // hipGraphExec_t of one period
+    void* side_ = nullptr;         // hipStream_t for the overlapped mechanics step
+    void* fork_ = nullptr;         // hipEvent_t
+    void* join_ = nullptr;         // hipEvent_t
+    bool overlap_ = true;
 };
 
 }  // namespace physicell

@@ -145,6 +145,7 @@ def load(path=LIB_PATH):
         "pc_sim_phenotype_step": (c_i32, [c_vp]),
         "pc_sim_run": (c_i32, [c_vp, c_f64, c_vp]), "pc_sim_run_periods": (c_i32, [c_vp, c_i32, c_i32]),
         "pc_sim_wait": (c_i32, [c_vp]), "pc_sim_get_density": (c_i32, [c_vp, c_vp]),
+        "pc_sim_set_overlap": (c_i32, [c_vp, c_i32]), "pe_set_overlap": (c_i32, [c_vp, c_i32]),
         "pc_sim_set_density": (c_i32, [c_vp, c_vp]), "pc_sim_get_field": (c_i32, [c_vp, c_u32, c_vp]),
         "pc_sim_get_hash": (c_i32, [c_vp, c_vp, c_vp]),
         "pc_sim_substrate_totals": (c_i32, [c_vp, c_vp]),
@@ -386,6 +387,9 @@ class Simulation:
     def wait(self):
         self._check(load().pc_sim_wait(self.h), "pc_sim_wait")
 
+    def set_overlap(self, on):
+        self._check(load().pc_sim_set_overlap(self.h, 1 if on else 0), "pc_sim_set_overlap")
+
     def density(self):
         out = np.empty(self.ns * self.spec.n_voxels)
         self._check(load().pc_sim_get_density(self.h, _ptr(out)), "pc_sim_get_density")
@@ -502,6 +506,9 @@ class Ensemble:
         self._check(load().pe_info(self.h, ctypes.byref(n), ctypes.byref(cells), ctypes.byref(vox), ctypes.byref(ws)),
                     "pe_info")
         return {"members": n.value, "cells": cells.value, "voxel_values": vox.value, "workspace_bytes": ws.value}
+
+    def set_overlap(self, on):
+        self._check(load().pe_set_overlap(self.h, 1 if on else 0), "pe_set_overlap")
 
     def set_grid_blocks(self, axis, blocks):
         self._check(load().pe_set_grid_blocks(self.h, axis, blocks), "pe_set_grid_blocks")
