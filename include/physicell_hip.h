@@ -303,6 +303,14 @@ int pc_sim_run_periods(pc_sim* sim, int periods, int use_graph);
  * touch disjoint data; the results are the same).  pc_sim_run keeps the
  * reference's serial order. */
 int pc_sim_set_overlap(pc_sim* sim, int on);
+/* Enables motility in the mechanics step (forces -> motility -> AB2, as
+ * src/main.mm:376-387).  Per-cell arrays as MotilityData (src/motility.h);
+ * sensitivity [cell*ns + s] may be NULL (then chem_index / chem_dir bias).
+ * While motility is on, periods run in the serial order (motility reads the
+ * density the substeps write). */
+int pc_sim_set_motility(pc_sim* sim, uint64_t seed, const double* persistence, const double* bias,
+                        const double* speed, const uint32_t* is_motile, const uint32_t* restrict_2d,
+                        const int32_t* chem_index, const int32_t* chem_dir, const double* sensitivity);
 int pc_sim_wait(pc_sim* sim);
 /* Copies out (waits first).  density: ns*nx*ny*nz FP64 (converted from FP32
  * in the mixed form). */
@@ -315,6 +323,33 @@ int pc_sim_get_hash(pc_sim* sim, uint32_t* offsets, uint32_t* cells);
 int pc_sim_substrate_totals(pc_sim* sim, double* out);
 int pc_sim_info(const pc_sim* sim, hc_grid* grid, hc_mech* mech, uint32_t* n_cells);
 hc_ctx* pc_sim_context(pc_sim* sim);
+
+/* ==== motility on the device (src/motility.cpp:179-293) ==================== */
+
+/* Per-cell motility state (MotilityData, src/motility.h), one buffer:
+ * a 256-byte header (uint64 step counter, uint64 seed), then FP64
+ * persistence time, elapsed time, bias, motility vector x/y/z; uint32
+ * is-motile and restrict-to-2D flags; int32 chemotaxis substrate index and
+ * direction (each max_cells long); then FP64 chemotactic sensitivities
+ * [cell*ns + s]. */
+enum {
+    HC_M_PERSISTENCE = 0, HC_M_ELAPSED = 1, HC_M_BIAS = 2, HC_M_VEC_X = 3, HC_M_VEC_Y = 4, HC_M_VEC_Z = 5,
+    HC_M_MOTILE = 6, HC_M_RESTRICT_2D = 7, HC_M_CHEM_INDEX = 8, HC_M_CHEM_DIR = 9, HC_M_SENSITIVITY = 10
+};
+size_t hc_motility_bytes(uint32_t max_cells, uint32_t n_substrates);
+size_t hc_motility_field_offset(uint32_t field, uint32_t max_cells, uint32_t n_substrates);
+/* updateMotility: for each live motile cell, elapsed += dt; resample the
+ * direction when persistence < dt or uniform < dt/persistence (random unit
+ * vector on the circle if restricted to 2-D, else the sphere, blended with
+ * the normalised chemotactic bias: the sensitivity-weighted gradient sum if
+ * use_sensitivity, else direction * gradient of the indexed substrate),
+ * normalise, scale by the cell's speed (field HC_F_MOT_SPEED); add the vector
+ * to the velocity.  Non-motile live cells get a zero vector.  Uniform draws
+ * come from a counter-based stream keyed by (seed, step, cell, draw); the step
+ * counter in the header advances by one per call.  density may be NULL (no
+ * bias). */
+int hc_motility(hc_ctx* ctx, hc_buf* cells, hc_buf* motility, hc_buf* density, const hc_grid* grid, double dt,
+                uint32_t n, uint32_t max_cells, int use_sensitivity);
 
 /* ==== MATLAB Level-4 snapshots in the reference's output format =========== */
 

@@ -458,3 +458,99 @@ void or_run(or_state* st, const or_grid* g, const or_sub* subs, const or_mech* m
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// motility (src/motility.cpp:179-293); the random stream is counter-based
+// (SplitMix64 finaliser, Steele, Lea & Flood 2014) so that it is reproducible
+// and the same on the device
+// ---------------------------------------------------------------------------
+static uint64_t or_mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+double or_uniform(uint64_t seed, uint64_t step, uint32_t cell, uint32_t draw) {
+    const uint64_t h = or_mix64(seed ^ or_mix64(step ^ or_mix64(((uint64_t)cell << 2) | draw)));
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// central differences of substrate s at the cell's voxel, indices clamped
+// (src/motility.cpp:45-86)
+static void or_gradient(const double* rho, const or_grid* g, double px, double py, double pz, uint32_t s,
+                        double& gx, double& gy, double& gz) {
+    const int nx = (int)g->nx, ny = (int)g->ny, nz = (int)g->nz;
+    int ix = (int)((px - g->x_min) / g->dx);
+    int iy = (int)((py - g->y_min) / g->dy);
+    int iz = (int)((pz - g->z_min) / g->dz);
+    ix = ix < 0 ? 0 : (ix > nx - 1 ? nx - 1 : ix);
+    iy = iy < 0 ? 0 : (iy > ny - 1 ? ny - 1 : iy);
+    iz = iz < 0 ? 0 : (iz > nz - 1 ? nz - 1 : iz);
+    const size_t V = (size_t)g->nx * g->ny * g->nz;
+    auto D = [&](int i, int j, int k) {
+        i = i < 0 ? 0 : (i > nx - 1 ? nx - 1 : i);
+        j = j < 0 ? 0 : (j > ny - 1 ? ny - 1 : j);
+        k = k < 0 ? 0 : (k > nz - 1 ? nz - 1 : k);
+        return rho[(size_t)s * V + (size_t)k * ny * nx + (size_t)j * nx + (size_t)i];
+    };
+    gx = (D(ix + 1, iy, iz) - D(ix - 1, iy, iz)) / (2.0 * g->dx);
+    gy = (D(ix, iy + 1, iz) - D(ix, iy - 1, iz)) / (2.0 * g->dy);
+    gz = 0.0;
+    if (nz > 1) gz = (D(ix, iy, iz + 1) - D(ix, iy, iz - 1)) / (2.0 * g->dz);
+}
+
+void or_motility(const double* x, const double* y, const double* z, double* vx, double* vy, double* vz,
+                 const uint32_t* alive, const double* speed, uint32_t n, const or_grid* g, const double* rho,
+                 double dt, uint64_t seed, uint64_t step, double* persistence, double* elapsed, const double* bias,
+                 double* mvx, double* mvy, double* mvz, const uint32_t* motile, const uint32_t* restrict_2d,
+                 const int32_t* chem_index, const int32_t* chem_dir, const double* sensitivity) {
+    const double pi = 3.14159265358979323846;
+    for (uint32_t i = 0; i < n; i++) {
+        if (!alive[i]) continue;
+        if (!motile[i]) {
+            mvx[i] = mvy[i] = mvz[i] = 0.0;
+            continue;
+        }
+        elapsed[i] += dt;
+        const double pt = persistence[i];
+        const bool resample = (pt < dt) || (or_uniform(seed, step, i, 0) < dt / pt);
+        if (resample) {
+            elapsed[i] = 0.0;
+            double rx, ry, rz;
+            const double theta = 2.0 * pi * or_uniform(seed, step, i, 1);
+            if (restrict_2d[i]) {
+                rx = cos(theta); ry = sin(theta); rz = 0.0;
+            } else {
+                const double phi = pi * or_uniform(seed, step, i, 2);
+                const double sp = sin(phi);
+                rx = sp * cos(theta); ry = sp * sin(theta); rz = cos(phi);
+            }
+            double bx = 0.0, by = 0.0, bz = 0.0;
+            const double b = bias[i];
+            if (b > 0.0 && rho) {
+                if (sensitivity) {
+                    for (uint32_t s = 0; s < g->ns; s++) {
+                        const double sens = sensitivity[(size_t)i * g->ns + s];
+                        if (fabs(sens) < 1e-16) continue;
+                        double gx, gy, gz;
+                        or_gradient(rho, g, x[i], y[i], z[i], s, gx, gy, gz);
+                        bx += sens * gx; by += sens * gy; bz += sens * gz;
+                    }
+                } else {
+                    double gx, gy, gz;
+                    or_gradient(rho, g, x[i], y[i], z[i], (uint32_t)chem_index[i], gx, gy, gz);
+                    bx = chem_dir[i] * gx; by = chem_dir[i] * gy; bz = chem_dir[i] * gz;
+                }
+                const double bmag = sqrt(bx * bx + by * by + bz * bz);
+                if (bmag > 1e-16) { bx /= bmag; by /= bmag; bz /= bmag; }
+            }
+            const double omb = 1.0 - b;
+            double mx = omb * rx + b * bx, my = omb * ry + b * by, mz = omb * rz + b * bz;
+            const double mag = sqrt(mx * mx + my * my + mz * mz);
+            if (mag > 1e-16) { mx /= mag; my /= mag; mz /= mag; }
+            mvx[i] = mx * speed[i]; mvy[i] = my * speed[i]; mvz[i] = mz * speed[i];
+        }
+        vx[i] += mvx[i]; vy[i] += mvy[i]; vz[i] += mvz[i];
+    }
+}

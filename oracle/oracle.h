@@ -140,6 +140,27 @@ void or_run(or_state* st, const or_grid* g, const or_sub* subs, const or_mech* m
 void or_mech_step(or_state* st, const or_mech* m, double dt_mech,
                   uint32_t* hash_offsets, uint32_t* hash_sorted, int nthreads);
 
+/* Motility (restates updateMotility, src/motility.cpp:179-293): for every
+ * live motile cell, the elapsed time grows by dt; the direction is resampled
+ * when the persistence time is below dt or a uniform draw is below
+ * dt/persistence; a resampled direction blends a random unit vector (circle
+ * in 2-D, sphere otherwise) with the normalised chemotactic bias, then is
+ * normalised and scaled by the cell's speed; the motility vector is added to
+ * the velocity.  Non-motile live cells get a zero motility vector.
+ * The bias is the sensitivity-weighted sum of central-difference gradients
+ * over substrates (src/motility.cpp:220-236) when `sensitivity` is given
+ * ([cell*ns + s]), else direction * gradient of substrate `chem_index`
+ * (:237-250).  Gradients read the substrate-major field (the reference reads
+ * density[voxel*ns + s], which equals it for one substrate).
+ * Uniform draws are a counter-based stream: or_uniform(seed, step, cell, k)
+ * for draw k = 0 (resample), 1 (theta), 2 (phi). */
+double or_uniform(uint64_t seed, uint64_t step, uint32_t cell, uint32_t draw);
+void or_motility(const double* x, const double* y, const double* z, double* vx, double* vy, double* vz,
+                 const uint32_t* alive, const double* speed, uint32_t n, const or_grid* g, const double* rho,
+                 double dt, uint64_t seed, uint64_t step, double* persistence, double* elapsed, const double* bias,
+                 double* mvx, double* mvy, double* mvz, const uint32_t* motile, const uint32_t* restrict_2d,
+                 const int32_t* chem_index, const int32_t* chem_dir, const double* sensitivity);
+
 #ifdef __cplusplus
 }
 #endif

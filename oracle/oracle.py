@@ -17,6 +17,7 @@ REF_LIB_PATH = os.path.join(HERE, "_ref", "libref_path.so")
 u32p = ctypes.POINTER(ctypes.c_uint32)
 f64p = ctypes.POINTER(ctypes.c_double)
 f32p = ctypes.POINTER(ctypes.c_float)
+i32p = ctypes.POINTER(ctypes.c_int32)
 
 
 class OrGrid(ctypes.Structure):
@@ -79,7 +80,7 @@ def ref_lib():
 def P(a, t=f64p):
     if not a.flags["C_CONTIGUOUS"]:
         raise ValueError("oracle arrays must be C-contiguous")
-    want = np.uint32 if t is u32p else (np.float32 if t is f32p else np.float64)
+    want = {u32p: np.uint32, f32p: np.float32, i32p: np.int32}.get(t, np.float64)
     if a.dtype != want:
         raise TypeError(f"oracle array dtype {a.dtype}, expected {np.dtype(want)}")
     return a.ctypes.data_as(t)
@@ -238,3 +239,22 @@ def ref_secretion(n, ns, nx, ny, nz, dx, dy, dz, dt, voxel, volume_f32, sec, upt
                     ctypes.c_double(dt), P(voxel, u32p), P(volume_f32, f32p), P(sec), P(upt),
                     P(sat), P(net), P(density_f32, f32p))
     return density_f32
+
+
+def uniform(seed, step, cell, draw):
+    """or_uniform: the counter-based uniform stream of the motility step."""
+    f = lib().or_uniform
+    f.restype = ctypes.c_double
+    return f(ctypes.c_uint64(seed), ctypes.c_uint64(step), ctypes.c_uint32(cell), ctypes.c_uint32(draw))
+
+
+def motility(x, y, z, vx, vy, vz, alive, speed, g, rho, dt, seed, step, persistence, elapsed, bias,
+             mvx, mvy, mvz, motile, restrict_2d, chem_index, chem_dir, sensitivity=None):
+    """or_motility (src/motility.cpp:179-293); updates vx, vy, vz, elapsed and
+    the motility vector in place.  g: OrGrid; rho: substrate-major density or
+    None."""
+    lib().or_motility(P(x), P(y), P(z), P(vx), P(vy), P(vz), P(alive, u32p), P(speed), ctypes.c_uint32(len(x)),
+                      ctypes.byref(g), None if rho is None else P(rho), ctypes.c_double(dt),
+                      ctypes.c_uint64(seed), ctypes.c_uint64(step), P(persistence), P(elapsed), P(bias),
+                      P(mvx), P(mvy), P(mvz), P(motile, u32p), P(restrict_2d, u32p), P(chem_index, i32p),
+                      P(chem_dir, i32p), None if sensitivity is None else P(sensitivity))

@@ -1,5 +1,6 @@
 // ref_driver.cpp -- thin C entry points into the reference's own compiled
-// processSecretion (src/cell_phenotype.cpp:526-564) and .mat writers
+// processSecretion (src/cell_phenotype.cpp:526-564), updateMotility
+// (src/motility.cpp:179-293) and .mat writers
 // (src/matlab_writer.cpp, src/output_writer.cpp:295-453), used only by tests
 // to pin the FP64 oracle's secretion restatement and the library's snapshot
 // writer against the reference code.
@@ -106,5 +107,59 @@ extern "C" int ref_write_snapshot(const char* folder, int frame, uint32_t n, con
     gp.dx = dx; gp.dy = dy; gp.dz = dz;
     gp.x_min = x_min; gp.y_min = y_min; gp.z_min = z_min;
     writeMatlabSnapshots(cells, density, gp, frame, folder);
+    return 0;
+}
+
+// The reference's updateMotility (src/motility.cpp:179-293) on n cells.
+// Positions, velocities and speeds are the reference's FP32 cell fields;
+// density is read the way the reference reads it (density[voxel*ns + s]).
+// legacy != 0 takes the single-substrate chemotaxis branch (:237-250) by
+// leaving the sensitivity table unallocated.  Its random draws come from its
+// own generator, so only cases that do not depend on them (forced resample
+// with bias 1, non-motile cells) are comparable with another stream.
+#include "motility.h"
+
+extern "C" int ref_motility(uint32_t n, const float* x, const float* y, const float* z, float* vx, float* vy,
+                            float* vz, const uint32_t* alive, const float* speed, uint32_t nx, uint32_t ny,
+                            uint32_t nz, uint32_t ns, float dx, float dy, float dz, float x_min, float y_min,
+                            float z_min, const float* density, double dt, double* persistence, double* elapsed,
+                            const double* bias, double* mvx, double* mvy, double* mvz, const uint32_t* motile,
+                            const uint32_t* restrict_2d, const int32_t* chem_index, const int32_t* chem_dir,
+                            const double* sensitivity, int legacy) {
+    CellData cells(n > 0 ? n : 1);
+    std::vector<float> gpu(cells.gpuBufferSize() / sizeof(float), 0.0f);
+    cells.setGPUBufferBase(gpu.data());
+    cells.num_cells = n;
+    MotilityData mot;
+    mot.allocate(n > 0 ? n : 1);
+    if (!legacy) mot.allocateSubstrate(ns);
+    for (uint32_t i = 0; i < n; i++) {
+        cells.position_x[i] = x[i]; cells.position_y[i] = y[i]; cells.position_z[i] = z[i];
+        cells.velocity_x[i] = vx[i]; cells.velocity_y[i] = vy[i]; cells.velocity_z[i] = vz[i];
+        cells.is_alive[i] = alive[i];
+        cells.motility_speed[i] = speed[i];
+        mot.persistence_time[i] = persistence[i];
+        mot.motility_elapsed[i] = elapsed[i];
+        mot.motility_bias[i] = bias[i];
+        mot.motility_vec_x[i] = mvx[i]; mot.motility_vec_y[i] = mvy[i]; mot.motility_vec_z[i] = mvz[i];
+        mot.is_motile[i] = motile[i];
+        mot.restrict_to_2D[i] = restrict_2d[i];
+        mot.chemotaxis_index[i] = chem_index[i];
+        mot.chemotaxis_direction[i] = chem_dir[i];
+        if (!legacy)
+            for (uint32_t s = 0; s < ns; s++)
+                mot.chemotactic_sensitivity[(size_t)i * ns + s] = sensitivity[(size_t)i * ns + s];
+    }
+    GridParams gp{};
+    gp.nx = nx; gp.ny = ny; gp.nz = nz; gp.n_substrates = ns;
+    gp.dx = dx; gp.dy = dy; gp.dz = dz;
+    gp.x_min = x_min; gp.y_min = y_min; gp.z_min = z_min;
+    updateMotility(cells, mot, density, gp, dt);
+    for (uint32_t i = 0; i < n; i++) {
+        vx[i] = cells.velocity_x[i]; vy[i] = cells.velocity_y[i]; vz[i] = cells.velocity_z[i];
+        elapsed[i] = mot.motility_elapsed[i];
+        mvx[i] = mot.motility_vec_x[i]; mvy[i] = mot.motility_vec_y[i]; mvz[i] = mot.motility_vec_z[i];
+    }
+    mot.free_all();
     return 0;
 }

@@ -51,6 +51,7 @@ int probe_kernels_diffusion(hc_ctx* ctx);
 int probe_kernels_cells(hc_ctx* ctx);
 int probe_kernels_cells2(hc_ctx* ctx);
 int probe_kernels_batch(hc_ctx* ctx);
+int probe_kernels_motility(hc_ctx* ctx);
 
 }  // namespace hc
 
@@ -118,6 +119,8 @@ int hc_load_kernels(hc_ctx* ctx, const char* code_object_or_null) {
     r = probe_kernels_cells2(ctx);
     if (r) return r;
     r = probe_kernels_batch(ctx);
+    if (r) return r;
+    r = probe_kernels_motility(ctx);
     if (r) return r;
     ctx->kernels_loaded = true;
     return 0;

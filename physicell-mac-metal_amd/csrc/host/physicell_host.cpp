@@ -370,6 +370,7 @@ Simulation::~Simulation() {
     cells = CellData(1);
     microenv = Microenvironment();
     mechanics = CellMechanics();
+    motility_ = DeviceBuffer();
     hc_destroy(ctx);
 }
 
@@ -416,8 +417,50 @@ void Simulation::diffusionSubstep() {
 
 void Simulation::mechanicsStep() {
     mechanics.computeForces(cells, config.dt_mechanics);
-    // (the reference adds CPU motility to velocity here; no motility on the path)
+    if (motility_on_) {
+        // updateMotility between forces and AB2 (src/main.mm:382-384)
+        const hc_grid g = microenv.getGridParams();
+        check(ctx, hc_motility(ctx, cells.gpu_buffer(), motility_.get(), microenv.density(), &g,
+                               config.dt_mechanics, cells.num_cells, cells.max_cells, motility_use_sensitivity_),
+              "motility");
+    }
     mechanics.integratePositions(cells, config.dt_mechanics);
+}
+
+void Simulation::setMotility(uint64_t seed, const double* persistence, const double* bias, const double* speed,
+                             const uint32_t* is_motile, const uint32_t* restrict_2d, const int32_t* chem_index,
+                             const int32_t* chem_dir, const double* sensitivity) {
+    const uint32_t ns = cells.n_substrates, mc = cells.max_cells, n = cells.num_cells;
+    motility_.allocate(ctx, hc_motility_bytes(mc, ns));
+    uint8_t* h = (uint8_t*)motility_.host();
+    std::memset(h, 0, motility_.size());
+    uint64_t* hdr = (uint64_t*)h;
+    hdr[0] = 0;          // step counter
+    hdr[1] = seed;
+    auto F = [&](uint32_t f) { return (double*)(h + hc_motility_field_offset(f, mc, ns)); };
+    auto U = [&](uint32_t f) { return (uint32_t*)(h + hc_motility_field_offset(f, mc, ns)); };
+    auto I = [&](uint32_t f) { return (int32_t*)(h + hc_motility_field_offset(f, mc, ns)); };
+    for (uint32_t i = 0; i < n; i++) {
+        // MotilityData::setDefaults (src/motility.cpp:137-157) where no value is given
+        F(HC_M_PERSISTENCE)[i] = persistence ? persistence[i] : 15.0;
+        F(HC_M_BIAS)[i] = bias ? bias[i] : 0.5;
+        U(HC_M_MOTILE)[i] = is_motile ? is_motile[i] : 0u;
+        U(HC_M_RESTRICT_2D)[i] = restrict_2d ? restrict_2d[i] : (config.use_2D ? 1u : 0u);
+        I(HC_M_CHEM_INDEX)[i] = chem_index ? chem_index[i] : 0;
+        I(HC_M_CHEM_DIR)[i] = chem_dir ? chem_dir[i] : 1;
+        for (uint32_t q = 0; q < ns; q++)
+            F(HC_M_SENSITIVITY)[(size_t)i * ns + q] = sensitivity ? sensitivity[(size_t)i * ns + q] : 0.0;
+        if (speed) cells.field(HC_F_MOT_SPEED)[i] = speed[i];
+    }
+    motility_.upload_all();
+    if (speed) cells.syncToDevice();
+    check(ctx, hc_wait(ctx), "wait");
+    motility_use_sensitivity_ = sensitivity ? 1 : 0;
+    motility_on_ = true;
+    if (graph_exec_) {
+        hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
+        graph_exec_ = nullptr;
+    }
 }
 
 void Simulation::phenotypeStep() {
@@ -479,7 +522,7 @@ static void hip_check(hipError_t e, const char* what) {
 // while the substeps run; the period ends when both are done.
 void Simulation::periodBody() {
     const int k = substepsPerMechanicsStep();
-    if (!overlap_) {
+    if (!overlap_ || motility_on_) {             // motility reads the substeps' density
         for (int s = 0; s < k; s++) diffusionSubstep();
         mechanicsStep();
         return;
@@ -672,6 +715,14 @@ int pc_sim_write_snapshot(pc_sim* s, const char* folder, int frame) {
         if (rc) throw std::runtime_error(std::string("pc_sim_write_snapshot: cannot write under ") + folder);
     });
     return g ? g : rc;
+}
+
+int pc_sim_set_motility(pc_sim* s, uint64_t seed, const double* persistence, const double* bias,
+                        const double* speed, const uint32_t* is_motile, const uint32_t* restrict_2d,
+                        const int32_t* chem_index, const int32_t* chem_dir, const double* sensitivity) {
+    return guarded(s, [&](Simulation& m) {
+        m.setMotility(seed, persistence, bias, speed, is_motile, restrict_2d, chem_index, chem_dir, sensitivity);
+    });
 }
 
 int pc_sim_set_overlap(pc_sim* s, int on) {

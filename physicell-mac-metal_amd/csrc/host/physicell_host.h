@@ -199,6 +199,11 @@ public:
     // with its substeps (they touch disjoint data; same results)
     void runPeriods(int periods, bool use_graph);
     void setOverlap(bool on);
+    // motility in the mechanics step (forces -> motility -> AB2); per-cell
+    // arrays as MotilityData (src/motility.h); sensitivity may be null
+    void setMotility(uint64_t seed, const double* persistence, const double* bias, const double* speed,
+                     const uint32_t* is_motile, const uint32_t* restrict_2d, const int32_t* chem_index,
+                     const int32_t* chem_dir, const double* sensitivity);
     void wait();
     int substepsPerMechanicsStep() const;
 
@@ -216,6 +221,9 @@ private:
     void* fork_ = nullptr;         // hipEvent_t
     void* join_ = nullptr;         // hipEvent_t
     bool overlap_ = true;
+    DeviceBuffer motility_;
+    bool motility_on_ = false;
+    int motility_use_sensitivity_ = 0;
 };
 
 }  // namespace physicell

@@ -1,0 +1,201 @@
+// motility.hip -- motility on the device (src/motility.cpp:179-293).
+//
+// Between the force step and AB2, every live motile cell adds its motility
+// vector to its velocity; the vector is resampled (persistent random walk
+// biased along substrate gradients) when the persistence time is below dt or
+// a uniform draw is below dt/persistence.  The random stream is
+// counter-based: uniform(seed, step, cell, draw) from the SplitMix64
+// finaliser (Steele, Lea & Flood 2014), so runs are reproducible, graph
+// replays draw fresh numbers (the step counter lives in device memory and is
+// advanced by a kernel), and the FP64 oracle draws the same numbers.
+//
+// Gradients are central differences of the substrate-major field at the
+// cell's voxel, indices clamped (src/motility.cpp:45-86; the reference reads
+// density[voxel*ns + s], the same thing for one substrate).
+#include "cells_common.h"
+
+namespace hc {
+
+__device__ inline uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ inline double uniform01(uint64_t seed, uint64_t step, uint32_t cell, uint32_t draw) {
+    const uint64_t h = mix64(seed ^ mix64(step ^ mix64(((uint64_t)cell << 2) | draw)));
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+struct MotPtrs {
+    double *persistence, *elapsed, *bias, *vec[3];
+    uint32_t *motile, *restrict_2d;
+    int32_t *chem_index, *chem_dir;
+    double* sensitivity;
+    uint64_t* hdr;       // [0] step, [1] seed
+};
+
+inline MotPtrs mot_ptrs(void* base, uint32_t max_cells, uint32_t ns) {
+    uint8_t* b = (uint8_t*)base;
+    MotPtrs m;
+    m.hdr = (uint64_t*)b;
+    auto off = [&](uint32_t f) { return b + hc_motility_field_offset(f, max_cells, ns); };
+    m.persistence = (double*)off(HC_M_PERSISTENCE);
+    m.elapsed = (double*)off(HC_M_ELAPSED);
+    m.bias = (double*)off(HC_M_BIAS);
+    m.vec[0] = (double*)off(HC_M_VEC_X);
+    m.vec[1] = (double*)off(HC_M_VEC_Y);
+    m.vec[2] = (double*)off(HC_M_VEC_Z);
+    m.motile = (uint32_t*)off(HC_M_MOTILE);
+    m.restrict_2d = (uint32_t*)off(HC_M_RESTRICT_2D);
+    m.chem_index = (int32_t*)off(HC_M_CHEM_INDEX);
+    m.chem_dir = (int32_t*)off(HC_M_CHEM_DIR);
+    m.sensitivity = (double*)off(HC_M_SENSITIVITY);
+    return m;
+}
+
+template <typename F>
+__device__ inline void gradient(const F* __restrict__ rho, const hc_grid& g, double px, double py, double pz,
+                                uint32_t s, double& gx, double& gy, double& gz) {
+    const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz;
+    int ix = (int)((px - g.x_min) / g.dx);
+    int iy = (int)((py - g.y_min) / g.dy);
+    int iz = (int)((pz - g.z_min) / g.dz);
+    ix = ix < 0 ? 0 : (ix > nx - 1 ? nx - 1 : ix);
+    iy = iy < 0 ? 0 : (iy > ny - 1 ? ny - 1 : iy);
+    iz = iz < 0 ? 0 : (iz > nz - 1 ? nz - 1 : iz);
+    const uint64_t V = (uint64_t)g.nx * g.ny * g.nz;
+    auto D = [&](int i, int j, int k) {
+        i = i < 0 ? 0 : (i > nx - 1 ? nx - 1 : i);
+        j = j < 0 ? 0 : (j > ny - 1 ? ny - 1 : j);
+        k = k < 0 ? 0 : (k > nz - 1 ? nz - 1 : k);
+        return (double)rho[(uint64_t)s * V + (uint64_t)k * ny * nx + (uint64_t)j * nx + (uint64_t)i];
+    };
+    gx = (D(ix + 1, iy, iz) - D(ix - 1, iy, iz)) / (2.0 * g.dx);
+    gy = (D(ix, iy + 1, iz) - D(ix, iy - 1, iz)) / (2.0 * g.dy);
+    gz = 0.0;
+    if (nz > 1) gz = (D(ix, iy, iz + 1) - D(ix, iy, iz - 1)) / (2.0 * g.dz);
+}
+
+template <typename F>
+__global__ __launch_bounds__(256) void k_motility(CellPtrs cp, MotPtrs mp, const F* __restrict__ rho, hc_grid g,
+                                                  double dt, uint32_t n, int use_sensitivity) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    if (!cp.u[HC_F_IS_ALIVE - HC_NUM_F64_FIELDS][i]) return;
+    if (!mp.motile[i]) {
+        mp.vec[0][i] = mp.vec[1][i] = mp.vec[2][i] = 0.0;
+        return;
+    }
+    const uint64_t step = mp.hdr[0], seed = mp.hdr[1];
+    mp.elapsed[i] += dt;
+    const double pt = mp.persistence[i];
+    const bool resample = (pt < dt) || (uniform01(seed, step, i, 0) < dt / pt);
+    if (resample) {
+        const double pi = 3.14159265358979323846;
+        mp.elapsed[i] = 0.0;
+        double rx, ry, rz;
+        const double theta = 2.0 * pi * uniform01(seed, step, i, 1);
+        if (mp.restrict_2d[i]) {
+            rx = cos(theta); ry = sin(theta); rz = 0.0;
+        } else {
+            const double phi = pi * uniform01(seed, step, i, 2);
+            const double sp = sin(phi);
+            rx = sp * cos(theta); ry = sp * sin(theta); rz = cos(phi);
+        }
+        double bx = 0.0, by = 0.0, bz = 0.0;
+        const double b = mp.bias[i];
+        if (b > 0.0 && rho) {
+            const double px = cp.f[HC_F_POS_X][i], py = cp.f[HC_F_POS_Y][i], pz = cp.f[HC_F_POS_Z][i];
+            if (use_sensitivity) {
+                for (uint32_t s = 0; s < g.n_substrates; s++) {
+                    const double sens = mp.sensitivity[(uint64_t)i * g.n_substrates + s];
+                    if (fabs(sens) < 1e-16) continue;
+                    double gx, gy, gz;
+                    gradient(rho, g, px, py, pz, s, gx, gy, gz);
+                    bx += sens * gx; by += sens * gy; bz += sens * gz;
+                }
+            } else {
+                double gx, gy, gz;
+                gradient(rho, g, px, py, pz, (uint32_t)mp.chem_index[i], gx, gy, gz);
+                const double d = (double)mp.chem_dir[i];
+                bx = d * gx; by = d * gy; bz = d * gz;
+            }
+            const double bmag = sqrt(bx * bx + by * by + bz * bz);
+            if (bmag > 1e-16) { bx /= bmag; by /= bmag; bz /= bmag; }
+        }
+        const double omb = 1.0 - b;
+        double mx = omb * rx + b * bx, my = omb * ry + b * by, mz = omb * rz + b * bz;
+        const double mag = sqrt(mx * mx + my * my + mz * mz);
+        if (mag > 1e-16) { mx /= mag; my /= mag; mz /= mag; }
+        const double speed = cp.f[HC_F_MOT_SPEED][i];
+        mp.vec[0][i] = mx * speed; mp.vec[1][i] = my * speed; mp.vec[2][i] = mz * speed;
+    }
+    cp.f[HC_F_VEL_X][i] += mp.vec[0][i];
+    cp.f[HC_F_VEL_Y][i] += mp.vec[1][i];
+    cp.f[HC_F_VEL_Z][i] += mp.vec[2][i];
+}
+
+// the step counter advances after every call (one thread; graph-safe)
+__global__ void k_motility_advance(uint64_t* hdr) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) hdr[0] += 1;
+}
+
+int probe_kernels_motility(hc_ctx* ctx) {
+    hipFuncAttributes attr;
+    const void* fns[] = {(const void*)k_motility<double>, (const void*)k_motility<float>,
+                         (const void*)k_motility_advance};
+    for (const void* f : fns) {
+        hipError_t e = hipFuncGetAttributes(&attr, f);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: motility kernel");
+    }
+    return 0;
+}
+
+}  // namespace hc
+
+using namespace hc;
+
+extern "C" {
+
+size_t hc_motility_field_offset(uint32_t field, uint32_t max_cells, uint32_t n_substrates) {
+    const size_t m = max_cells, head = 256;
+    const size_t d = ((head + 6 * m * 8 + 4 * m * 4) + 7) / 8 * 8;   // start of the sensitivities
+    if (field <= HC_M_VEC_Z) return head + field * m * 8;
+    if (field <= HC_M_CHEM_DIR) return head + 6 * m * 8 + (field - HC_M_MOTILE) * m * 4;
+    (void)n_substrates;
+    return d;
+}
+
+size_t hc_motility_bytes(uint32_t max_cells, uint32_t n_substrates) {
+    return hc_motility_field_offset(HC_M_SENSITIVITY, max_cells, n_substrates) +
+           (size_t)max_cells * n_substrates * 8;
+}
+
+int hc_motility(hc_ctx* ctx, hc_buf* cells, hc_buf* motility, hc_buf* density, const hc_grid* g, double dt,
+                uint32_t n, uint32_t max_cells, int use_sensitivity) {
+    if (!ctx || !cells || !motility || !g) return fail(ctx, HC_ERR_ARG, "hc_motility: null argument");
+    if (n > max_cells) return fail(ctx, HC_ERR_ARG, "hc_motility: n > max_cells");
+    if (hc_buf_size(cells) < hc_cells_bytes(max_cells)) return fail(ctx, HC_ERR_ARG, "hc_motility: cells buffer too small");
+    if (hc_buf_size(motility) < hc_motility_bytes(max_cells, g->n_substrates))
+        return fail(ctx, HC_ERR_ARG, "hc_motility: motility buffer too small");
+    const uint64_t V = (uint64_t)g->nx * g->ny * g->nz;
+    if (density && hc_buf_size(density) < V * g->n_substrates * field_elem(g))
+        return fail(ctx, HC_ERR_ARG, "hc_motility: density buffer too small");
+    const CellPtrs cp = cell_ptrs(cells->dptr, max_cells);
+    const MotPtrs mp = mot_ptrs(motility->dptr, max_cells, g->n_substrates);
+    if (n) {
+        const dim3 grid(blocks_for(n, 256));
+        if (g->field_fp32)
+            hipLaunchKernelGGL(k_motility<float>, grid, dim3(256), 0, ctx->stream, cp, mp,
+                               density ? (const float*)density->dptr : nullptr, *g, dt, n, use_sensitivity);
+        else
+            hipLaunchKernelGGL(k_motility<double>, grid, dim3(256), 0, ctx->stream, cp, mp,
+                               density ? (const double*)density->dptr : nullptr, *g, dt, n, use_sensitivity);
+    }
+    hipLaunchKernelGGL(k_motility_advance, dim3(1), dim3(64), 0, ctx->stream, mp.hdr);
+    return check_launch(ctx, "k_motility");
+}
+
+}  // extern "C"

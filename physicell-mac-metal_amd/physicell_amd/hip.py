@@ -30,6 +30,10 @@ F_RADIUS, F_TOTAL_VOL = 9, 11
 F_REPULSION, F_ADHESION, F_REL_MAX_ADH = 13, 14, 15
 F_PRESSURE = 22
 F_CELL_TYPE, F_CUR_PHASE, F_IS_ALIVE, F_VOXEL_IDX, F_MECH_VOXEL = 23, 24, 25, 26, 27
+F_NUCLEAR_RAD, F_NUCLEAR_VOL, F_MOT_SPEED = 10, 12, 16
+# motility buffer fields (include/physicell_hip.h, HC_M_*)
+M_PERSISTENCE, M_ELAPSED, M_BIAS, M_VEC_X, M_VEC_Y, M_VEC_Z = 0, 1, 2, 3, 4, 5
+M_MOTILE, M_RESTRICT_2D, M_CHEM_INDEX, M_CHEM_DIR, M_SENSITIVITY = 6, 7, 8, 9, 10
 NUM_F64_FIELDS = 23
 
 K_SWEEP_X, K_SWEEP_Y, K_SWEEP_Z, K_SWEEP_SMALL, K_DIRICHLET, K_VOXEL_KEYS = 0, 1, 2, 3, 4, 5
@@ -151,6 +155,9 @@ def load(path=LIB_PATH):
         "pc_sim_substrate_totals": (c_i32, [c_vp, c_vp]),
         "pc_sim_info": (c_i32, [c_vp, P(Grid), P(Mech), P(c_u32)]),
         "pc_sim_context": (c_vp, [c_vp]),
+        "hc_motility_bytes": (c_sz, [c_u32, c_u32]), "hc_motility_field_offset": (c_sz, [c_u32, c_u32, c_u32]),
+        "hc_motility": (c_i32, [c_vp, c_vp, c_vp, c_vp, P(Grid), c_f64, c_u32, c_u32, c_i32]),
+        "pc_sim_set_motility": (c_i32, [c_vp, ctypes.c_uint64] + [c_vp] * 8),
         "pc_write_matlab4": (c_i32, [ctypes.c_char_p, ctypes.c_char_p, c_u32, c_u32, c_vp]),
         "pc_write_snapshot": (c_i32, [ctypes.c_char_p, c_i32, P(Snapshot)]),
         "pc_sim_write_snapshot": (c_i32, [c_vp, ctypes.c_char_p, c_i32]),
@@ -389,6 +396,14 @@ class Simulation:
 
     def set_overlap(self, on):
         self._check(load().pc_sim_set_overlap(self.h, 1 if on else 0), "pc_sim_set_overlap")
+
+    def set_motility(self, seed, persistence=None, bias=None, speed=None, is_motile=None, restrict_2d=None,
+                     chem_index=None, chem_dir=None, sensitivity=None):
+        conv = [(persistence, np.float64), (bias, np.float64), (speed, np.float64), (is_motile, np.uint32),
+                (restrict_2d, np.uint32), (chem_index, np.int32), (chem_dir, np.int32), (sensitivity, np.float64)]
+        arrs = [None if a is None else np.ascontiguousarray(np.asarray(a, t).reshape(-1)) for a, t in conv]
+        self._keep_mot = arrs
+        self._check(load().pc_sim_set_motility(self.h, seed, *[_ptr(a) for a in arrs]), "pc_sim_set_motility")
 
     def density(self):
         out = np.empty(self.ns * self.spec.n_voxels)
