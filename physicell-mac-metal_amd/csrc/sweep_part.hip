@@ -159,13 +159,72 @@ __device__ inline void load_part_tables(const global_t<uint8_t>* __restrict__ ta
     }
 }
 
+// The part of one (substrate, axis) table block that a sweep of lines of n
+// nodes reads (segment positions t < part_mt(n, P) <= MM), held in registers
+// between its global loads and its LDS stores.  A workgroup that loads its
+// tables issues them first, then its field loads, and stores the tables to
+// LDS only when the field loads are in flight: one memory latency before the
+// solve instead of two.
+template <int P, int MM, typename T>
+struct TableRegs {
+    static constexpr int NA = (P * 4 * MM + 255) / 256;
+    static constexpr int NG = (4 * P * P + 255) / 256;
+    T a[NA];
+    double g[NG];
+
+    __device__ inline void load(const global_t<uint8_t>* __restrict__ tables, uint32_t ns, uint32_t nmax, uint32_t s,
+                                int axis, uint32_t n) {
+        const uint32_t MT = part_mt(nmax, P), mt = part_mt(n, P), na = (uint32_t)P * 4 * MT;
+        const global_t<double>* dblk = reinterpret_cast<const global_t<double>*>(tables + part_offset(ns, nmax)) +
+                                       (size_t)(s * 3 + axis) * part_block(nmax);
+        const global_t<T>* ablk;
+        if constexpr (sizeof(T) == sizeof(double))
+            ablk = (const global_t<T>*)dblk;
+        else
+            ablk = reinterpret_cast<const global_t<T>*>(tables + part_f_offset(ns, nmax)) +
+                   (size_t)(s * 3 + axis) * part_fblock(nmax);
+#pragma unroll
+        for (int r = 0; r < NG; r++) {
+            const uint32_t q = threadIdx.x + 256u * r;
+            g[r] = q < (uint32_t)(4 * P * P) ? dblk[na + q] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < NA; r++) {
+            const uint32_t q = threadIdx.x + 256u * r;
+            const uint32_t k = q / (4 * MM), j = (q / MM) % 4, t = q % MM;
+            a[r] = (q < (uint32_t)(P * 4 * MM) && t < mt) ? ablk[(k * 4 + j) * MT + t] : (T)0;
+        }
+    }
+    __device__ inline void store(uint32_t nmax, uint32_t n, double* tG, T* tarr) const {
+        const uint32_t MT = part_mt(nmax, P), mt = part_mt(n, P), AS = PartLds<P, T>::arr_stride(MT);
+#pragma unroll
+        for (int r = 0; r < NG; r++) {
+            const uint32_t q = threadIdx.x + 256u * r;
+            if (q < (uint32_t)(4 * P * P)) tG[(q / (4 * P)) * PartLds<P, T>::GS + q % (4 * P)] = g[r];
+        }
+#pragma unroll
+        for (int r = 0; r < NA; r++) {
+            const uint32_t q = threadIdx.x + 256u * r;
+            const uint32_t k = q / (4 * MM), j = (q / MM) % 4, t = q % MM;
+            if (q < (uint32_t)(P * 4 * MM) && t < mt) tarr[k * AS + j * MT + t] = a[r];
+        }
+    }
+};
+
+struct NoTables {
+    __device__ inline void operator()() const {}
+};
+
 // One group of 4*LPW lines (a workgroup of 4 waves) of one substrate of one
 // simulation.  rho points at that substrate's field.
 // xmap: lane order of the x staging tile (x_lane_order); y and z use order 0.
-template <int AXIS, int P, int MM, bool XLDS, typename T>
+// mid() runs once the group's field loads are issued and before the first
+// table read (a workgroup that stages its tables stores them and syncs there;
+// every thread of the workgroup calls it).
+template <int AXIS, int P, int MM, bool XLDS, typename T, typename Mid = NoTables>
 __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict__ rho, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t MT,
                                    const T* tarr, const double* tG, double c, double T0, bool dir, int fuse_dirichlet,
-                                   uint64_t grp, T* tile_base, uint32_t xmap) {
+                                   uint64_t grp, T* tile_base, uint32_t xmap, const Mid& mid = Mid()) {
     constexpr int LPW = 64 / P;
     constexpr int WPB = 4;
     constexpr int RV = MM / (16 / (int)sizeof(T)) + 2;
@@ -205,12 +264,14 @@ __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict_
     if (XLDS) {
         // the wave's LPW consecutive x lines are LPW*n contiguous values
         stage_in<T, RV>(grho + Lw0 * nx, tile, nval, lane);
+        mid();
         wave_sync();
 #pragma unroll
         for (int t = 0; t < MM; t++) v[t] = (t < (int)mk) ? tile[l * n + sk + 1 + t] : (T)0;
     } else {
 #pragma unroll
         for (int t = 0; t < MM; t++) v[t] = (t < (int)mk && valid) ? p[(uint64_t)(sk + 1 + t) * stride] : (T)0;
+        mid();
     }
 
     const T* idn = tarr + k * PartLds<P, T>::arr_stride(MT);
@@ -298,14 +359,14 @@ __global__ __launch_bounds__(256) void k_sweep_part(T* __restrict__ rho, const u
     double* tG = reinterpret_cast<double*>(smem_raw);
     T* tarr = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::g_count() * sizeof(double));
     T* tile = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::bytes(nmax));
-    load_part_tables<P, T>((const global_t<uint8_t>*)tables, ns, nmax, s, AXIS, tG, tarr);
     const SubConst* sc = reinterpret_cast<const SubConst*>(tables);
     const double c = sc[s].c[AXIS], T0 = sc[s].T;
     const bool dir = sc[s].dir != 0;
     const uint32_t MT = part_mt(nmax, P);
     T* rs = rho + (uint64_t)s * nx * ny * nz;
-    __syncthreads();                                      // tables visible
     if (XLDS) {
+        load_part_tables<P, T>((const global_t<uint8_t>*)tables, ns, nmax, s, AXIS, tG, tarr);
+        __syncthreads();                                  // tables visible
         const uint64_t ngroups = (sweep_lines(AXIS, nx, ny, nz) + 4 * (64 / P) - 1) / (4 * (64 / P));
         for (uint64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
             // keep the table reads inside the loop (hoisting them all into
@@ -316,8 +377,17 @@ __global__ __launch_bounds__(256) void k_sweep_part(T* __restrict__ rho, const u
                                               grp, tile, xmap);
         }
     } else {
+        // one group per workgroup: table loads first, then the field loads,
+        // then the tables go to LDS
+        const uint32_t n = AXIS == 0 ? nx : (AXIS == 1 ? ny : nz);
+        TableRegs<P, MM, T> treg;
+        treg.load((const global_t<uint8_t>*)tables, ns, nmax, s, AXIS, n);
+        __builtin_amdgcn_sched_barrier(0);
         sweep_group<AXIS, P, MM, XLDS, T>(rs, nx, ny, nz, MT, tarr, tG, c, T0, dir, fuse_dirichlet, blockIdx.x,
-                                          tile, 0u);
+                                          tile, 0u, [&]() {
+                                              treg.store(nmax, n, tG, tarr);
+                                              __syncthreads();
+                                          });
     }
 }
 
@@ -367,13 +437,18 @@ __global__ __launch_bounds__(256) void k_sweep_part_item(const BatchSim* __restr
     T* tile = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::bytes(nmax_all));
     const BatchWork it = work[blockIdx.x];
     const BatchSim sm = sims[it.sim];
-    load_part_tables<P, T>((const global_t<uint8_t>*)sm.tab, sm.ns, sm.nmax, it.s, AXIS, tG, tarr);
+    const uint32_t n = AXIS == 0 ? sm.nx : (AXIS == 1 ? sm.ny : sm.nz);
+    TableRegs<P, MM, T> treg;
+    treg.load((const global_t<uint8_t>*)sm.tab, sm.ns, sm.nmax, it.s, AXIS, n);
+    __builtin_amdgcn_sched_barrier(0);
     const global_t<SubConst>* sc = (const global_t<SubConst>*)sm.tab;
     T* rs = reinterpret_cast<T*>(sm.rho) + (uint64_t)it.s * sm.nx * sm.ny * sm.nz;
-    __syncthreads();
     sweep_group<AXIS, P, MM, XLDS, T>(rs, sm.nx, sm.ny, sm.nz, part_mt(sm.nmax, P), tarr, tG, sc[it.s].c[AXIS],
                                       sc[it.s].T, sc[it.s].dir != 0, sm.last_axis == (uint32_t)AXIS ? 1 : 0, it.grp,
-                                      tile, sm.xmap);
+                                      tile, sm.xmap, [&]() {
+                                          treg.store(sm.nmax, n, tG, tarr);
+                                          __syncthreads();
+                                      });
 }
 
 int probe_kernels_part(hc_ctx* ctx) {
