@@ -339,15 +339,18 @@ struct Member {
 };
 
 class Ensemble {
-    // sweep work lists, one per (axis, segments per line, register window),
-    // in launch order: every x list, then y, then z
-    static constexpr int NLIST = 7;
-    static constexpr int LIST_AXIS[NLIST] = {0, 0, 0, 1, 1, 2, 2};
-    static constexpr int LIST_P[NLIST] = {4, 4, 16, 4, 4, 4, 4};
-    static constexpr int LIST_MM[NLIST] = {16, 32, 16, 16, 32, 16, 32};
-    static int list_of(int axis, int P, int mm) {
+    // sweep work lists, one per (axis, segments per line), in launch order:
+    // x (4 segments), x (16 segments), y, z.  The 4-segment lists hold members
+    // of both register windows (16 or 32 values per lane, each member's own:
+    // BatchSim::wide), those with the longer lines first; the 16-segment x
+    // list holds lines of at most 16 values per lane.
+    static constexpr int NLIST = 4;
+    static constexpr int LIST_AXIS[NLIST] = {0, 0, 1, 2};
+    static constexpr int LIST_P[NLIST] = {4, 16, 4, 4};
+    static constexpr int LIST_MM[NLIST] = {0, 16, 0, 0};
+    static int list_of(int axis, int P) {
         for (int li = 0; li < NLIST; li++)
-            if (LIST_AXIS[li] == axis && LIST_P[li] == P && LIST_MM[li] == mm) return li;
+            if (LIST_AXIS[li] == axis && LIST_P[li] == P) return li;
         return -1;
     }
 
@@ -682,7 +685,7 @@ private:
         std::vector<EnsSim> sims;
         std::vector<BatchSim> bsims;
         std::vector<AgentTile> tiles;
-        std::vector<BatchWork> work[NLIST];
+        std::vector<BatchWork> work[NLIST], narrow[NLIST];
         std::vector<uint32_t> agent_sim;
         uint32_t cell_off = 0, bin_off = 0, vox_off = 0, ns_max = 0, nmax_all = 0;
         uint32_t max_line[NLIST] = {};
@@ -708,26 +711,31 @@ private:
             b.nx = g.nx; b.ny = g.ny; b.nz = g.nz; b.ns = g.n_substrates; b.nmax = e.nmax;
             b.last_axis = g.nz > 1 ? 2u : 1u;
             b.xmap = (uint32_t)hc::ctx_x_lane_order(ctx_, g.nx, hc::part_p(0, g.nx, mixed_), mixed_ ? 4 : 8);
-            bsims.push_back(b);
+            b.wide = 0;
             for (uint32_t c0 = 0; c0 < m.n; c0 += 256) tiles.push_back(AgentTile{si, c0});
             for (uint32_t c = 0; c < m.n; c++) agent_sim.push_back(si);
             const uint32_t nn[3] = {g.nx, g.ny, g.nz};
             for (int ax = 0; ax < 3; ax++) {
                 if (ax == 2 && g.nz == 1) continue;
                 const int P = hc::part_p(ax, nn[ax], mixed_);
-                const int li = list_of(ax, P, hc::part_window(nn[ax], P));
+                const int li = list_of(ax, P);
+                const int mm = hc::part_window(nn[ax], P);
+                if (mm == 32) b.wide |= 1u << ax;
                 const uint64_t lines = hc::sweep_lines(ax, g.nx, g.ny, g.nz);
                 const uint64_t groups = (lines + 4 * (64 / P) - 1) / (4 * (64 / P));
+                std::vector<BatchWork>& dst = mm == 32 ? work[li] : narrow[li];
                 for (uint32_t s = 0; s < g.n_substrates; s++)
-                    for (uint64_t q = 0; q < groups; q++) work[li].push_back(BatchWork{si, s, q});
+                    for (uint64_t q = 0; q < groups; q++) dst.push_back(BatchWork{si, s, q});
                 max_line[li] = std::max(max_line[li], nn[ax]);
             }
+            bsims.push_back(b);
             cell_off += m.n;
             bin_off += m.mech.mech_grid_nx * m.mech.mech_grid_ny * m.mech.mech_grid_nz;
             vox_off += e.V;
             ns_max = std::max(ns_max, g.n_substrates);
             nmax_all = std::max(nmax_all, e.nmax);
         }
+        for (int li = 0; li < NLIST; li++) work[li].insert(work[li].end(), narrow[li].begin(), narrow[li].end());
         w.ntot = cell_off; w.nbins = bin_off; w.vtot = vox_off; w.ns_max = ns_max; w.nmax_all = nmax_all;
         w.ntiles = (uint32_t)tiles.size();
         for (int li = 0; li < NLIST; li++) { w.nwork[li] = (uint32_t)work[li].size(); w.max_line[li] = max_line[li]; }

@@ -132,7 +132,7 @@ struct BatchSim {            // one simulation of a packed ensemble
     uint32_t nx, ny, nz, ns, nmax;
     uint32_t last_axis;      // the sweep that re-applies Dirichlet (2 in 3-D, 1 in 2-D)
     uint32_t xmap;           // x lane order (x_lane_order)
-    uint32_t _pad;
+    uint32_t wide;           // bit a: lines along axis a need the 32-value register window (else 16)
 };
 // number of lines a sweep along `axis` solves
 __host__ __device__ inline uint64_t sweep_lines(int axis, uint32_t nx, uint32_t ny, uint32_t nz) {
@@ -194,8 +194,9 @@ int launch_sweep_part(hc_ctx* ctx, int axis, void* rho, const hc_grid* g, const 
 // 1 when every sweep of the grid takes the partitioned form
 bool part_fits_all(const hc_grid* g, int sweep_mode);
 // P: the segments per line of every member in the work list (part_p); mm:
-// their register window (16 or 32); grid_blocks 0: one workgroup per item,
-// else a persistent grid of that many workgroups
+// their register window, 16, or 0 for each member's own (16 or 32 values,
+// BatchSim::wide; P = 4); grid_blocks 0: one workgroup per item, else a
+// persistent grid of that many workgroups
 int launch_sweep_batch(hc_ctx* ctx, int axis, int P, int mm, bool fp32, const BatchSim* sims, const BatchWork* work,
                        uint32_t nwork, uint32_t nmax_all, uint32_t max_line, unsigned grid_blocks);
 // register window of a line of n values in P segments (0: does not fit)

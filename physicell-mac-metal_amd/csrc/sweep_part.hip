@@ -392,9 +392,25 @@ __global__ __launch_bounds__(256) void k_sweep_part(T* __restrict__ rho, const u
 }
 
 // ---------------------------------------------------------------------------
-// packed ensemble: a persistent grid walks a work list; each workgroup takes a
-// contiguous run of items, so consecutive items mostly share their tables
+// packed ensemble.  MM = 0: the list holds members of both register windows
+// (16 and 32 values per lane, BatchSim::wide) and each item takes its
+// member's window (a uniform branch); one launch per axis then covers them
+// all, with no tail between a list of long lines and a list of short ones.
 // ---------------------------------------------------------------------------
+template <int AXIS, int P, int MM, bool XLDS, typename T>
+__device__ __attribute__((always_inline)) inline void batch_group(const BatchSim& sm, uint32_t s, uint64_t grp,
+                                                                  const T* tarr, const double* tG, T* tile) {
+    const global_t<SubConst>* sc = (const global_t<SubConst>*)sm.tab;
+    uint32_t toff = 0;
+    asm volatile("" : "+v"(toff));
+    T* rs = reinterpret_cast<T*>(sm.rho) + (uint64_t)s * sm.nx * sm.ny * sm.nz;
+    sweep_group<AXIS, P, MM, XLDS, T>(rs, sm.nx, sm.ny, sm.nz, part_mt(sm.nmax, P), tarr + toff, tG + toff,
+                                      sc[s].c[AXIS], sc[s].T, sc[s].dir != 0, sm.last_axis == (uint32_t)AXIS ? 1 : 0,
+                                      grp, tile, sm.xmap);
+}
+
+// a persistent grid walks the work list; each workgroup takes a contiguous
+// run of items, so consecutive items mostly share their tables
 template <int AXIS, int P, int MM, bool XLDS, typename T>
 __global__ __launch_bounds__(256) void k_sweep_part_batch(const BatchSim* __restrict__ sims,
                                                           const BatchWork* __restrict__ work, uint32_t nwork,
@@ -417,26 +433,20 @@ __global__ __launch_bounds__(256) void k_sweep_part_batch(const BatchSim* __rest
             cur_sim = it.sim;
             cur_s = it.s;
         }
-        const global_t<SubConst>* sc = (const global_t<SubConst>*)sm.tab;
-        uint32_t toff = 0;
-        asm volatile("" : "+v"(toff));
-        T* rs = reinterpret_cast<T*>(sm.rho) + (uint64_t)it.s * sm.nx * sm.ny * sm.nz;
-        sweep_group<AXIS, P, MM, XLDS, T>(rs, sm.nx, sm.ny, sm.nz, part_mt(sm.nmax, P), tarr + toff, tG + toff,
-                                          sc[it.s].c[AXIS], sc[it.s].T, sc[it.s].dir != 0,
-                                          sm.last_axis == (uint32_t)AXIS ? 1 : 0, it.grp, tile, sm.xmap);
+        if constexpr (MM != 0)
+            batch_group<AXIS, P, MM, XLDS, T>(sm, it.s, it.grp, tarr, tG, tile);
+        else if ((sm.wide >> AXIS) & 1u)
+            batch_group<AXIS, P, 32, XLDS, T>(sm, it.s, it.grp, tarr, tG, tile);
+        else
+            batch_group<AXIS, P, 16, XLDS, T>(sm, it.s, it.grp, tarr, tG, tile);
     }
 }
 
-// packed ensemble, one workgroup per work item (no loop: fewer registers)
+// one workgroup per work item: table loads first, then the field loads, then
+// the tables go to LDS
 template <int AXIS, int P, int MM, bool XLDS, typename T>
-__global__ __launch_bounds__(256) void k_sweep_part_item(const BatchSim* __restrict__ sims,
-                                                         const BatchWork* __restrict__ work, uint32_t nmax_all) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    double* tG = reinterpret_cast<double*>(smem_raw);
-    T* tarr = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::g_count() * sizeof(double));
-    T* tile = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::bytes(nmax_all));
-    const BatchWork it = work[blockIdx.x];
-    const BatchSim sm = sims[it.sim];
+__device__ __attribute__((always_inline)) inline void item_group(const BatchSim& sm, const BatchWork& it, double* tG,
+                                                                 T* tarr, T* tile) {
     const uint32_t n = AXIS == 0 ? sm.nx : (AXIS == 1 ? sm.ny : sm.nz);
     TableRegs<P, MM, T> treg;
     treg.load((const global_t<uint8_t>*)sm.tab, sm.ns, sm.nmax, it.s, AXIS, n);
@@ -449,6 +459,23 @@ __global__ __launch_bounds__(256) void k_sweep_part_item(const BatchSim* __restr
                                           treg.store(sm.nmax, n, tG, tarr);
                                           __syncthreads();
                                       });
+}
+
+template <int AXIS, int P, int MM, bool XLDS, typename T>
+__global__ __launch_bounds__(256) void k_sweep_part_item(const BatchSim* __restrict__ sims,
+                                                         const BatchWork* __restrict__ work, uint32_t nmax_all) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* tG = reinterpret_cast<double*>(smem_raw);
+    T* tarr = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::g_count() * sizeof(double));
+    T* tile = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::bytes(nmax_all));
+    const BatchWork it = work[blockIdx.x];
+    const BatchSim sm = sims[it.sim];
+    if constexpr (MM != 0)
+        item_group<AXIS, P, MM, XLDS, T>(sm, it, tG, tarr, tile);
+    else if ((sm.wide >> AXIS) & 1u)
+        item_group<AXIS, P, 32, XLDS, T>(sm, it, tG, tarr, tile);
+    else
+        item_group<AXIS, P, 16, XLDS, T>(sm, it, tG, tarr, tile);
 }
 
 int probe_kernels_part(hc_ctx* ctx) {
@@ -468,13 +495,13 @@ int probe_kernels_part(hc_ctx* ctx) {
         (const void*)k_sweep_part<0, 4, 16, true, float>, (const void*)k_sweep_part<0, 4, 32, true, float>,
         (const void*)k_sweep_part<0, 4, 16, false, float>, (const void*)k_sweep_part<0, 4, 32, false, float>,
         (const void*)k_sweep_part_batch<0, 16, 16, true, double>, (const void*)k_sweep_part_item<0, 16, 16, true, double>,
-        (const void*)k_sweep_part_batch<0, 4, 32, true, double>, (const void*)k_sweep_part_item<0, 4, 32, true, double>,
-        (const void*)k_sweep_part_batch<1, 4, 32, false, double>, (const void*)k_sweep_part_item<1, 4, 16, false, double>,
-        (const void*)k_sweep_part_batch<2, 4, 32, false, double>, (const void*)k_sweep_part_item<2, 4, 16, false, double>,
+        (const void*)k_sweep_part_batch<0, 4, 0, true, double>, (const void*)k_sweep_part_item<0, 4, 0, true, double>,
+        (const void*)k_sweep_part_batch<1, 4, 0, false, double>, (const void*)k_sweep_part_item<1, 4, 0, false, double>,
+        (const void*)k_sweep_part_batch<2, 4, 0, false, double>, (const void*)k_sweep_part_item<2, 4, 0, false, double>,
         (const void*)k_sweep_part_batch<0, 16, 16, true, float>, (const void*)k_sweep_part_item<0, 16, 16, true, float>,
-        (const void*)k_sweep_part_batch<0, 4, 32, true, float>, (const void*)k_sweep_part_item<0, 4, 16, true, float>,
-        (const void*)k_sweep_part_batch<1, 4, 32, false, float>, (const void*)k_sweep_part_item<1, 4, 32, false, float>,
-        (const void*)k_sweep_part_batch<2, 4, 32, false, float>, (const void*)k_sweep_part_item<2, 4, 32, false, float>};
+        (const void*)k_sweep_part_batch<0, 4, 0, true, float>, (const void*)k_sweep_part_item<0, 4, 0, true, float>,
+        (const void*)k_sweep_part_batch<1, 4, 0, false, float>, (const void*)k_sweep_part_item<1, 4, 0, false, float>,
+        (const void*)k_sweep_part_batch<2, 4, 0, false, float>, (const void*)k_sweep_part_item<2, 4, 0, false, float>};
     for (const void* f : fns) {
         hipError_t e = hipFuncGetAttributes(&attr, f);
         if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: partitioned sweep");
@@ -576,7 +603,9 @@ static int launch_batch_t(hc_ctx* ctx, int axis, int P, int mm, const BatchSim* 
                           uint32_t nwork, uint32_t nmax_all, uint32_t max_line, unsigned grid_blocks) {
     if (nwork == 0) return 0;
     if (axis != 0 && P != 4) return fail(ctx, HC_ERR_ARG, "batched sweep: y and z use 4 segments per line");
-    if (part_window(max_line, P) > mm || (mm != 16 && mm != 32) || (P == 16 && mm != 16))
+    // mm: 16 (every member's lines fit 16 values per lane), or 0 (each
+    // member's own window, 16 or 32; P = 4 only)
+    if ((mm != 0 && mm != 16) || (P == 16 && mm != 16) || part_window(max_line, P) > (mm ? mm : 32))
         return fail(ctx, HC_ERR_CAPACITY, "batched sweep: line length outside the list's register window");
     const int LPW = 64 / P;
     const size_t tab_bytes = P == 16 ? PartLds<16, T>::bytes(nmax_all) : PartLds<4, T>::bytes(nmax_all);
@@ -584,12 +613,9 @@ static int launch_batch_t(hc_ctx* ctx, int axis, int P, int mm, const BatchSim* 
     if (shm > 64 * 1024) return fail(ctx, HC_ERR_CAPACITY, "batched sweep: LDS");
     KernelTimer tm(ctx, HC_K_SWEEP_X + axis);
     if (axis == 0 && P == 16) launch_batch_kernel<0, 16, 16, true, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
-    else if (axis == 0 && mm == 16) launch_batch_kernel<0, 4, 16, true, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
-    else if (axis == 0) launch_batch_kernel<0, 4, 32, true, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
-    else if (axis == 1 && mm == 16) launch_batch_kernel<1, 4, 16, false, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
-    else if (axis == 1) launch_batch_kernel<1, 4, 32, false, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
-    else if (mm == 16) launch_batch_kernel<2, 4, 16, false, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
-    else launch_batch_kernel<2, 4, 32, false, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
+    else if (axis == 0) launch_batch_kernel<0, 4, 0, true, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
+    else if (axis == 1) launch_batch_kernel<1, 4, 0, false, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
+    else launch_batch_kernel<2, 4, 0, false, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
     return check_launch(ctx, "batched sweep");
 }
 
