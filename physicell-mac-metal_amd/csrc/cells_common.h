@@ -135,34 +135,46 @@ __device__ __attribute__((always_inline)) inline void neighbor_rows(const uint32
 
 // Pair forces of one agent against the hash-sorted members [beg, end), in
 // ascending order (shaders/mechanics.metal:151-203).  Two passes per chunk
-// of up to 32 members: a distance test marks the candidates inside the
-// cutoff (their loads issued 4 at a time), then the force law runs over the
+// of up to 32 members: a distance test marks the candidates that can exert a
+// force (their loads issued 4 at a time), then the force law runs over the
 // marked ones in the same order, so the lanes of a wave run the expensive
-// part about as often as they have neighbours, not once per candidate.
+// part about as often as they have interacting neighbours, not once per
+// candidate.
+//
+// A pair inside the cutoff exerts a force only when dist < R_sum or dist <
+// rel_max_adh * R_sum; otherwise the force law adds +0 to the sums, which
+// leaves them unchanged (they start at +0, so they are never -0).  The mark
+// pass therefore also requires d^2 < L^2 (1 + m), L = R_sum * max(1,
+// rel_max_adh), with a margin m far above the rounding of L^2 and of the
+// square root (2^-40 in FP64, 2^-16 in FP32): every pair it drops has a
+// rounded dist >= L, so dropping it changes no bit of the result.
 __device__ __attribute__((always_inline)) inline void force_range64(
     const CellRec* __restrict__ rec, uint32_t beg, uint32_t end, uint32_t t, double px, double py, double pz,
     double radius_i, double rep_i, double adh_i, double rel_max_adh, double cut2, double& fx, double& fy,
     double& fz, double& local_pressure) {
+    const double relf = rel_max_adh > 1.0 ? rel_max_adh : 1.0;
+    constexpr double margin = 1.0 + 0x1p-40;
     for (uint32_t u0 = beg; u0 < end; u0 += 32) {
         const uint32_t cnt = end - u0 < 32u ? end - u0 : 32u;
         uint32_t mask = 0;
         for (uint32_t j0 = 0; j0 < cnt; j0 += 4) {
-            double2 xy[4];
-            double zz[4];
+            double2 xy[4], zr[4];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const uint32_t u = u0 + (j0 + q < cnt ? j0 + q : cnt - 1);
                 xy[q] = *reinterpret_cast<const double2*>(&rec[u].x);
-                zz[q] = rec[u].z;
+                zr[q] = *reinterpret_cast<const double2*>(&rec[u].z);
             }
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const uint32_t j = j0 + q;
                 const double ddx = px - xy[q].x;
                 const double ddy = py - xy[q].y;
-                const double ddz = pz - zz[q];
+                const double ddz = pz - zr[q].x;
                 const double dist_sq = ddx * ddx + ddy * ddy + ddz * ddz;
-                const bool inside = j < cnt && u0 + j != t && !(dist_sq >= cut2) && !(dist_sq < 1e-12);
+                const double L = (radius_i + zr[q].y) * relf;
+                const bool inside = j < cnt && u0 + j != t && !(dist_sq >= cut2) && !(dist_sq < 1e-12) &&
+                                    dist_sq < L * L * margin;
                 mask |= inside ? (1u << j) : 0u;
             }
         }
@@ -210,6 +222,8 @@ __device__ __attribute__((always_inline)) inline void force_range32(
     float my, float mz, float oy, float oz, float voxel, float radius_i, float rep_i, float adh_i, float rel_max_adh,
     float cut2, float& fx, float& fy, float& fz, float& local_pressure) {
     const float oxm = -1.0f * voxel, ox0 = 0.0f * voxel, oxp = 1.0f * voxel;
+    const float relf = rel_max_adh > 1.0f ? rel_max_adh : 1.0f;
+    constexpr float margin = 1.0f + 0x1p-16f;
     for (uint32_t u0 = beg; u0 < end; u0 += 32) {
         const uint32_t cnt = end - u0 < 32u ? end - u0 : 32u;
         uint32_t mask = 0;
@@ -226,7 +240,8 @@ __device__ __attribute__((always_inline)) inline void force_range32(
                 const float ox = u < mid0 ? oxm : (u < mid1 ? ox0 : oxp);
                 const float ddx = mx - o[q].x - ox, ddy = my - o[q].y - oy, ddz = mz - o[q].z - oz;
                 const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                const bool inside = j < cnt && u != t && !(d2 >= cut2) && !(d2 < 1e-12f);
+                const float L = (radius_i + o[q].w) * relf;
+                const bool inside = j < cnt && u != t && !(d2 >= cut2) && !(d2 < 1e-12f) && d2 < L * L * margin;
                 mask |= inside ? (1u << j) : 0u;
             }
         }
