@@ -80,15 +80,6 @@ __device__ inline uint32_t stable_rank(const uint32_t* __restrict__ tmp, uint32_
     return r;
 }
 
-// Workgroups are handed to the 8 XCDs round-robin (block b runs on XCD b % 8).
-// Renumber them so that each XCD works through one contiguous range of the
-// hash-sorted agents: neighbour bins then stay in that XCD's L2 instead of
-// every L2 fetching every plane.
-__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb) {
-    constexpr uint32_t NX = 8;
-    const uint32_t x = b % NX, idx = b / NX, per = nb / NX, rem = nb % NX;
-    return x * per + (x < rem ? x : rem) + idx;
-}
 
 int fill_u32(hc_ctx* ctx, uint32_t* p, uint64_t n, uint32_t v);
 int launch_scatter(hc_ctx* ctx, const uint32_t* key, const uint32_t* slot, const uint32_t* offsets, uint32_t* tmp,
@@ -224,18 +215,19 @@ __device__ __attribute__((always_inline)) inline void force_range32(
     const float oxm = -1.0f * voxel, ox0 = 0.0f * voxel, oxp = 1.0f * voxel;
     const float relf = rel_max_adh > 1.0f ? rel_max_adh : 1.0f;
     constexpr float margin = 1.0f + 0x1p-16f;
+    constexpr uint32_t FB = 4;                     // candidate loads in flight per lane (8 measured slower)
     for (uint32_t u0 = beg; u0 < end; u0 += 32) {
         const uint32_t cnt = end - u0 < 32u ? end - u0 : 32u;
         uint32_t mask = 0;
-        for (uint32_t j0 = 0; j0 < cnt; j0 += 4) {
-            float4 o[4];
+        for (uint32_t j0 = 0; j0 < cnt; j0 += FB) {
+            float4 o[FB];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
+            for (int q = 0; q < FB; q++) {
                 const uint32_t u = u0 + (j0 + q < cnt ? j0 + q : cnt - 1);
                 o[q] = *reinterpret_cast<const float4*>(&rec[u].x);
             }
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
+            for (int q = 0; q < FB; q++) {
                 const uint32_t j = j0 + q, u = u0 + j;
                 const float ox = u < mid0 ? oxm : (u < mid1 ? ox0 : oxp);
                 const float ddx = mx - o[q].x - ox, ddy = my - o[q].y - oy, ddz = mz - o[q].z - oz;

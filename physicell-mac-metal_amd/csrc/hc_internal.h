@@ -85,6 +85,28 @@ inline uint32_t grid_nmax(const hc_grid* g) {
 }
 
 // ---- partitioned sweep tables (see sweep_part.hip) ----
+// Workgroups are handed to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// xcd_block renumbers them so that each XCD works through one contiguous
+// range of the work: neighbour bins of hash-sorted agents, and the cache lines
+// that adjacent line groups of a sweep share, then stay in one XCD's L2
+// instead of being fetched by several.
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    constexpr uint32_t NX = 8;
+    const uint32_t x = b % NX, idx = b / NX, per = nb / NX, rem = nb % NX;
+    return x * per + (x < rem ? x : rem) + idx;
+}
+// The same in chunks of C consecutive work items dealt to the XCDs in turn:
+// neighbouring items still share one L2 (all but one boundary in C), and
+// work lists whose cost varies along the list (longer lines first) stay
+// balanced across the XCDs.  A tail shorter than 8*C keeps its order.
+template <uint32_t C>
+__device__ inline uint32_t xcd_chunk(uint32_t b, uint32_t nb) {
+    constexpr uint32_t NX = 8;
+    if (b >= nb / (NX * C) * (NX * C)) return b;
+    const uint32_t x = b % NX, idx = b / NX;
+    return (idx / C) * (NX * C) + x * C + idx % C;
+}
+
 // A line's N = n-2 unknowns are split into P contiguous segments, one per
 // lane (P = 4: 16 lines per wave, or P = 16: 4 lines per wave; see part_p).
 // Per (substrate, axis) block: for

@@ -355,16 +355,17 @@ __global__ __launch_bounds__(256) void k_sweep_part(T* __restrict__ rho, const u
                                                     uint32_t nx, uint32_t ny, uint32_t nz, uint32_t nmax,
                                                     int fuse_dirichlet, uint32_t xmap, uint32_t s0, uint32_t ns) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const uint32_t s = s0 + blockIdx.y;                  // substrates s0 .. s0+gridDim.y-1 of ns
     double* tG = reinterpret_cast<double*>(smem_raw);
     T* tarr = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::g_count() * sizeof(double));
     T* tile = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::bytes(nmax));
     const SubConst* sc = reinterpret_cast<const SubConst*>(tables);
-    const double c = sc[s].c[AXIS], T0 = sc[s].T;
-    const bool dir = sc[s].dir != 0;
     const uint32_t MT = part_mt(nmax, P);
-    T* rs = rho + (uint64_t)s * nx * ny * nz;
+    // substrates s0 .. s0+gridDim.y-1 of ns
     if (XLDS) {
+        const uint32_t s = s0 + blockIdx.y;
+        const double c = sc[s].c[AXIS], T0 = sc[s].T;
+        const bool dir = sc[s].dir != 0;
+        T* rs = rho + (uint64_t)s * nx * ny * nz;
         load_part_tables<P, T>((const global_t<uint8_t>*)tables, ns, nmax, s, AXIS, tG, tarr);
         __syncthreads();                                  // tables visible
         const uint64_t ngroups = (sweep_lines(AXIS, nx, ny, nz) + 4 * (64 / P) - 1) / (4 * (64 / P));
@@ -377,13 +378,19 @@ __global__ __launch_bounds__(256) void k_sweep_part(T* __restrict__ rho, const u
                                               grp, tile, xmap);
         }
     } else {
-        // one group per workgroup: table loads first, then the field loads,
-        // then the tables go to LDS
+        // one group per workgroup, in XCD chunks of consecutive groups
+        // (xcd_chunk); table loads first, then the field loads, then the
+        // tables go to LDS
+        const uint32_t lin = xcd_chunk<16>(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+        const uint32_t grp = lin % gridDim.x, s = s0 + lin / gridDim.x;
+        const double c = sc[s].c[AXIS], T0 = sc[s].T;
+        const bool dir = sc[s].dir != 0;
+        T* rs = rho + (uint64_t)s * nx * ny * nz;
         const uint32_t n = AXIS == 0 ? nx : (AXIS == 1 ? ny : nz);
         TableRegs<P, MM, T> treg;
         treg.load((const global_t<uint8_t>*)tables, ns, nmax, s, AXIS, n);
         __builtin_amdgcn_sched_barrier(0);
-        sweep_group<AXIS, P, MM, XLDS, T>(rs, nx, ny, nz, MT, tarr, tG, c, T0, dir, fuse_dirichlet, blockIdx.x,
+        sweep_group<AXIS, P, MM, XLDS, T>(rs, nx, ny, nz, MT, tarr, tG, c, T0, dir, fuse_dirichlet, grp,
                                           tile, 0u, [&]() {
                                               treg.store(nmax, n, tG, tarr);
                                               __syncthreads();
@@ -468,7 +475,7 @@ __global__ __launch_bounds__(256) void k_sweep_part_item(const BatchSim* __restr
     double* tG = reinterpret_cast<double*>(smem_raw);
     T* tarr = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::g_count() * sizeof(double));
     T* tile = reinterpret_cast<T*>(smem_raw + PartLds<P, T>::bytes(nmax_all));
-    const BatchWork it = work[blockIdx.x];
+    const BatchWork it = work[xcd_chunk<16>(blockIdx.x, gridDim.x)];
     const BatchSim sm = sims[it.sim];
     if constexpr (MM != 0)
         item_group<AXIS, P, MM, XLDS, T>(sm, it, tG, tarr, tile);
