@@ -244,7 +244,8 @@ int hc_substrate_totals(hc_ctx* ctx, hc_buf* density, const hc_grid* grid, doubl
 typedef enum hc_kernel_id {
     HC_K_SWEEP_X = 0, HC_K_SWEEP_Y = 1, HC_K_SWEEP_Z = 2, HC_K_SWEEP_SMALL = 3,
     HC_K_DIRICHLET = 4, HC_K_VOXEL_KEYS = 5, HC_K_SEC_PLAN = 6, HC_K_SEC_APPLY = 7,
-    HC_K_HASH = 8, HC_K_FORCES = 9, HC_K_INTEGRATE = 10, HC_K_COUNT = 11
+    HC_K_HASH = 8, HC_K_FORCES = 9, HC_K_INTEGRATE = 10, HC_K_MOTILITY = 11, HC_K_VOLUME = 12,
+    HC_K_DIVIDE = 13, HC_K_REMOVE = 14, HC_K_COUNT = 15
 } hc_kernel_id;
 int hc_timing_enable(hc_ctx* ctx, uint32_t kernel_mask);
 /* Waits, then returns the summed event time (ms) and launch count of `kernel`. */
@@ -311,6 +312,18 @@ int pc_sim_set_overlap(pc_sim* sim, int on);
 int pc_sim_set_motility(pc_sim* sim, uint64_t seed, const double* persistence, const double* bias,
                         const double* speed, const uint32_t* is_motile, const uint32_t* restrict_2d,
                         const int32_t* chem_index, const int32_t* chem_dir, const double* sensitivity);
+/* Volume state for the device phenotype step: fields[HC_V_*] points at
+ * n_cells values or is NULL (CellData's fresh value, src/cell_data.cpp:
+ * 597-624; fields itself may be NULL); seed keys division's draws. */
+int pc_sim_set_volume(pc_sim* sim, uint64_t seed, const double* const* fields);
+/* updatePhenotype's SoA side on the device (src/cell_phenotype.cpp:792-845),
+ * with the decisions from the host: hc_volume_update on the cells with
+ * mask[i] != 0 (mask: n_cells values, or NULL for all), hc_divide of
+ * parents, hc_remove of removed (indices below the count after the
+ * divisions), then the voxel refresh and secretion re-plan of
+ * pc_sim_phenotype_step.  *n_out (may be NULL) = the new cell count. */
+int pc_sim_update_phenotype(pc_sim* sim, double dt, int use_ratio, const uint32_t* mask, const uint32_t* parents,
+                            uint32_t n_parents, const uint32_t* removed, uint32_t n_removed, uint32_t* n_out);
 int pc_sim_wait(pc_sim* sim);
 /* Copies out (waits first).  density: ns*nx*ny*nz FP64 (converted from FP32
  * in the mixed form). */
@@ -350,6 +363,68 @@ size_t hc_motility_field_offset(uint32_t field, uint32_t max_cells, uint32_t n_s
  * bias). */
 int hc_motility(hc_ctx* ctx, hc_buf* cells, hc_buf* motility, hc_buf* density, const hc_grid* grid, double dt,
                 uint32_t n, uint32_t max_cells, int use_sensitivity);
+
+/* ==== phenotype-side SoA mutation on the device ============================
+ * The volume model, division and removal of updatePhenotype
+ * (src/cell_phenotype.cpp:52-113, 571-720, 833-844; src/cell_data.cpp:398-542).
+ * Which cells divide or are removed is decided on the host (cycle and death
+ * models) and passed as index lists. */
+
+/* Per-cell volume state (the Volume fields of CellData, src/cell_data.h), one
+ * buffer: a 256-byte header (uint64 division counter, uint64 seed), then one
+ * FP64 array of max_cells values per field. */
+enum {
+    HC_V_FLUID = 0, HC_V_SOLID_NUC = 1, HC_V_SOLID_CYTO = 2, HC_V_FLUID_FRAC = 3, HC_V_CALC_FRAC = 4,
+    HC_V_TARGET_SOLID_CYTO = 5, HC_V_TARGET_SOLID_NUC = 6, HC_V_TARGET_FLUID_FRAC = 7, HC_V_TARGET_CN_RATIO = 8,
+    HC_V_FLUID_RATE = 9, HC_V_NUC_RATE = 10, HC_V_CYTO_RATE = 11, HC_V_CALC_RATE = 12, HC_V_RUPTURE_VOL = 13,
+    HC_V_REL_RUPTURE_VOL = 14, HC_NUM_V_FIELDS = 15
+};
+size_t hc_volume_bytes(uint32_t max_cells);
+size_t hc_volume_field_offset(uint32_t field, uint32_t max_cells);
+/* updateVolume (src/cell_phenotype.cpp:52-113) for every cell i < n whose
+ * mask[i] != 0 (mask: a device buffer of n uint32, or NULL for every cell):
+ * fluid, nuclear and cytoplasmic solid relax toward their targets, the
+ * calcified fraction grows, total and nuclear volume (HC_F_TOTAL_VOL,
+ * HC_F_NUCLEAR_VOL) are rebuilt from the compartments and the radii
+ * (HC_F_RADIUS, HC_F_NUCLEAR_RAD) follow as cbrt(V / (4/3 pi)).  use_ratio
+ * != 0 recomputes the cytoplasmic solid target as ratio * nuclear solid
+ * target first (:80-87; the reference does so whenever the ratio array
+ * exists). */
+int hc_volume_update(hc_ctx* ctx, hc_buf* cells, hc_buf* volume, hc_buf* mask, uint32_t n, uint32_t max_cells,
+                     double dt, int use_ratio);
+/* divideCell (src/cell_phenotype.cpp:571-720) for parents[0..count) (host
+ * array, distinct indices < n), in list order: parent k's daughter is cell
+ * n + k.  Both halve their volumes and volume compartments and recompute
+ * their radii; the daughter sits radius/2 from the parent at a random angle
+ * in the x-y plane, inherits type, mechanics, motility speed and bias,
+ * volume rates, rates [cell*ns + s] and voxels, and takes CellData's fresh
+ * values (src/cell_data.cpp:544-640) elsewhere; the oncoprotein level gets
+ * +noise in the daughter and -noise in the parent, noise ~ N(0, 0.02 level),
+ * both clamped at 0.  The angle and the noise come from the counter-based
+ * stream keyed by (seed, division counter, k, draw); the counter advances by
+ * one per call.  As in the reference, divisions beyond max_cells are skipped.
+ * motility may be NULL; else the daughter takes the parent's motility state
+ * with zero elapsed time (the reference leaves it unset).  *n_out = the new
+ * cell count. */
+int hc_divide(hc_ctx* ctx, hc_buf* cells, hc_buf* rates, hc_buf* volume, hc_buf* motility, uint32_t ns,
+              const uint32_t* parents, uint32_t count, uint32_t n, uint32_t max_cells, uint32_t* n_out);
+/* The reference's removal loop (src/cell_phenotype.cpp:833-844): the
+ * indices (host array, distinct, < n, any order) are removed in descending
+ * order, each by CellData::removeCell (src/cell_data.cpp:408-542: the last
+ * cell is copied into the hole).  The resulting arrangement is computed on
+ * the host from the index list alone, then every moved cell's fields (cell
+ * SoA, rates, volume state, and, when motility is not NULL, the motility
+ * state as MotilityData::swapRemove, src/motility.cpp:154-175) are copied in
+ * one parallel pass.  *n_out = n - count. */
+int hc_remove(hc_ctx* ctx, hc_buf* cells, hc_buf* rates, hc_buf* volume, hc_buf* motility, uint32_t ns,
+              const uint32_t* indices, uint32_t count, uint32_t n, uint32_t max_cells, uint32_t* n_out);
+/* The moves hc_remove makes, host only: after removing `indices` from n
+ * cells, cell dst[k] holds what was cell src[k] (dst < n - count <= src);
+ * every other cell below n - count keeps its index.  dst and src hold count
+ * entries (or are NULL); *n_pairs = the number of moves.  A host that keeps
+ * per-cell data of its own (cycle and death state) applies the same moves. */
+int hc_remove_plan(const uint32_t* indices, uint32_t count, uint32_t n, uint32_t* dst, uint32_t* src,
+                   uint32_t* n_pairs);
 
 /* ==== MATLAB Level-4 snapshots in the reference's output format =========== */
 

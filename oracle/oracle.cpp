@@ -8,7 +8,9 @@
 // sequence the reference's kernels spell out.
 #include "oracle.h"
 
+#include <algorithm>
 #include <cmath>
+#include <functional>
 #include <cstring>
 #include <vector>
 #include <omp.h>
@@ -553,4 +555,150 @@ void or_motility(const double* x, const double* y, const double* z, double* vx, 
         }
         vx[i] += mvx[i]; vy[i] += mvy[i]; vz[i] += mvz[i];
     }
+}
+
+// ---------------------------------------------------------------------------
+// phenotype-side SoA mutation (src/cell_phenotype.cpp:52-113, 571-720,
+// 833-844; src/cell_data.cpp:398-542)
+// ---------------------------------------------------------------------------
+namespace {
+enum { F_PX = 0, F_PY = 1, F_PZ = 2, F_RAD = 9, F_NRAD = 10, F_TVOL = 11, F_NVOL = 12, F_REP = 13, F_ADH = 14,
+       F_REL = 15, F_SPEED = 16, F_BIAS = 17, F_ONCO = 21 };
+enum { U_TYPE = 0, U_PHASE = 1, U_ALIVE = 2, U_VOX = 3, U_MVOX = 4 };
+enum { V_FLUID = 0, V_SNUC = 1, V_SCYTO = 2, V_FFRAC = 3, V_CFRAC = 4, V_TSCYTO = 5, V_TSNUC = 6, V_TFFRAC = 7,
+       V_RATIO = 8, V_FRATE = 9, V_NRATE = 10, V_CRATE = 11, V_CALCRATE = 12, V_RUPT = 13, V_RELRUPT = 14 };
+const double OR_FOUR_THIRDS_PI = 4.188790204786391;    // src/cell_phenotype.cpp:45
+}
+
+void or_volume_update(or_cells* c, const uint32_t* mask, uint32_t n, double dt, int use_ratio) {
+    for (uint32_t i = 0; i < n; i++) {
+        if (mask && !mask[i]) continue;
+        double** v = c->vol;
+        double total = c->f64[F_TVOL][i];
+        double V_fluid = v[V_FLUID][i];
+        double nuclear_solid = v[V_SNUC][i];
+        double cyto_solid = v[V_SCYTO][i];
+        V_fluid += dt * v[V_FRATE][i] * (v[V_TFFRAC][i] * total - V_fluid);
+        if (V_fluid < 0.0) V_fluid = 0.0;
+        double nuclear = c->f64[F_NVOL][i];
+        const double nuclear_fluid = (nuclear / (total + 1e-16)) * V_fluid;
+        const double cyto_fluid = V_fluid - nuclear_fluid;
+        nuclear_solid += dt * v[V_NRATE][i] * (v[V_TSNUC][i] - nuclear_solid);
+        if (nuclear_solid < 0.0) nuclear_solid = 0.0;
+        double target_sc = v[V_TSCYTO][i];
+        if (use_ratio) {
+            target_sc = v[V_RATIO][i] * v[V_TSNUC][i];
+            v[V_TSCYTO][i] = target_sc;
+        }
+        cyto_solid += dt * v[V_CRATE][i] * (target_sc - cyto_solid);
+        if (cyto_solid < 0.0) cyto_solid = 0.0;
+        double calc_frac = v[V_CFRAC][i];
+        calc_frac += dt * v[V_CALCRATE][i] * (1.0 - calc_frac);
+        nuclear = nuclear_solid + nuclear_fluid;
+        const double cytoplasmic = cyto_solid + cyto_fluid;
+        total = cytoplasmic + nuclear;
+        const double fluid_fraction = V_fluid / (total + 1e-16);
+        c->f64[F_TVOL][i] = total;
+        c->f64[F_NVOL][i] = nuclear;
+        v[V_FLUID][i] = V_fluid;
+        v[V_SNUC][i] = nuclear_solid;
+        v[V_SCYTO][i] = cyto_solid;
+        v[V_FFRAC][i] = fluid_fraction;
+        v[V_CFRAC][i] = calc_frac;
+        c->f64[F_RAD][i] = std::cbrt(total / OR_FOUR_THIRDS_PI);
+        c->f64[F_NRAD][i] = std::cbrt(nuclear / OR_FOUR_THIRDS_PI);
+    }
+}
+
+uint32_t or_divide(or_cells* c, const uint32_t* parents, uint32_t count, uint32_t n, uint32_t max_cells,
+                   uint64_t seed, uint64_t counter) {
+    const double pi = 3.14159265358979323846;
+    uint32_t cur = n;
+    for (uint32_t k = 0; k < count; k++) {
+        if (cur >= max_cells) break;                     // CellData::addCell: no room
+        const uint32_t p = parents[k], d = cur++;
+        // fresh values (src/cell_data.cpp:544-640) for what divideCell leaves
+        for (int f = 0; f < 23; f++) c->f64[f][d] = 0.0;
+        c->f64[F_ONCO][d] = 1.0;
+        for (int f : {F_TVOL, F_NVOL}) {
+            const double h = c->f64[f][p] * 0.5;
+            c->f64[f][p] = c->f64[f][d] = h;
+        }
+        for (int f : {V_FLUID, V_SCYTO, V_SNUC, V_RUPT, V_TSNUC, V_TSCYTO}) {
+            const double h = c->vol[f][p] * 0.5;
+            c->vol[f][p] = c->vol[f][d] = h;
+        }
+        for (int f : {V_FFRAC, V_CRATE, V_NRATE, V_FRATE, V_CALCRATE, V_CFRAC, V_TFFRAC, V_RELRUPT})
+            c->vol[f][d] = c->vol[f][p];
+        c->vol[V_RATIO][d] = 3.62;
+        const double r = std::cbrt(c->f64[F_TVOL][p] / OR_FOUR_THIRDS_PI);
+        const double nr = std::cbrt(c->f64[F_NVOL][p] / OR_FOUR_THIRDS_PI);
+        c->f64[F_RAD][p] = c->f64[F_RAD][d] = r;
+        c->f64[F_NRAD][p] = c->f64[F_NRAD][d] = nr;
+        const double theta = or_uniform(seed, counter, k, 0) * 2.0 * pi;
+        const double offset = r * 0.5;
+        c->f64[F_PX][d] = c->f64[F_PX][p] + offset * std::cos(theta);
+        c->f64[F_PY][d] = c->f64[F_PY][p] + offset * std::sin(theta);
+        c->f64[F_PZ][d] = c->f64[F_PZ][p];
+        for (int f : {F_REP, F_ADH, F_REL, F_SPEED, F_BIAS}) c->f64[f][d] = c->f64[f][p];
+        const double onco = c->f64[F_ONCO][p];
+        const double u1 = 1.0 - or_uniform(seed, counter, k, 1), u2 = or_uniform(seed, counter, k, 2);
+        const double noise = 0.02 * onco * (std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * pi * u2));
+        c->f64[F_ONCO][d] = std::max(0.0, onco + noise);
+        c->f64[F_ONCO][p] = std::max(0.0, onco - noise);
+        c->u32[U_TYPE][d] = c->u32[U_TYPE][p];
+        c->u32[U_PHASE][p] = c->u32[U_PHASE][d] = 0u;
+        c->u32[U_ALIVE][d] = 1u;
+        c->u32[U_VOX][d] = c->u32[U_VOX][p];
+        c->u32[U_MVOX][d] = c->u32[U_MVOX][p];
+        for (int a = 0; a < 4; a++)
+            for (uint32_t s = 0; s < c->ns; s++)
+                c->rates[a][(size_t)d * c->ns + s] = c->rates[a][(size_t)p * c->ns + s];
+        if (c->persistence) {
+            c->persistence[d] = c->persistence[p];
+            c->elapsed[d] = 0.0;
+            c->bias[d] = c->bias[p];
+            for (int q = 0; q < 3; q++) c->vec[q][d] = c->vec[q][p];
+            c->motile[d] = c->motile[p];
+            c->restrict_2d[d] = c->restrict_2d[p];
+            c->chem_index[d] = c->chem_index[p];
+            c->chem_dir[d] = c->chem_dir[p];
+            for (uint32_t s = 0; s < c->ns; s++)
+                c->sensitivity[(size_t)d * c->ns + s] = c->sensitivity[(size_t)p * c->ns + s];
+        }
+    }
+    return cur;
+}
+
+uint32_t or_remove(or_cells* c, const uint32_t* indices, uint32_t count, uint32_t n) {
+    std::vector<uint32_t> idx(indices, indices + count);
+    std::sort(idx.begin(), idx.end(), std::greater<uint32_t>());
+    uint32_t cur = n;
+    for (uint32_t index : idx) {
+        if (index >= cur) continue;                      // src/cell_phenotype.cpp:840
+        const uint32_t last = cur - 1;
+        if (index != last) {
+            for (int f = 0; f < 23; f++) c->f64[f][index] = c->f64[f][last];
+            for (int f = 0; f < 5; f++) c->u32[f][index] = c->u32[f][last];
+            for (int a = 0; a < 4; a++)
+                for (uint32_t s = 0; s < c->ns; s++)
+                    c->rates[a][(size_t)index * c->ns + s] = c->rates[a][(size_t)last * c->ns + s];
+            for (int f = 0; f < 15; f++)
+                if (c->vol[f]) c->vol[f][index] = c->vol[f][last];
+            if (c->persistence) {
+                c->persistence[index] = c->persistence[last];
+                c->elapsed[index] = c->elapsed[last];
+                c->bias[index] = c->bias[last];
+                for (int q = 0; q < 3; q++) c->vec[q][index] = c->vec[q][last];
+                c->motile[index] = c->motile[last];
+                c->restrict_2d[index] = c->restrict_2d[last];
+                c->chem_index[index] = c->chem_index[last];
+                c->chem_dir[index] = c->chem_dir[last];
+                for (uint32_t s = 0; s < c->ns; s++)
+                    c->sensitivity[(size_t)index * c->ns + s] = c->sensitivity[(size_t)last * c->ns + s];
+            }
+        }
+        cur--;
+    }
+    return cur;
 }

@@ -161,6 +161,36 @@ void or_motility(const double* x, const double* y, const double* z, double* vx, 
                  double* mvx, double* mvy, double* mvz, const uint32_t* motile, const uint32_t* restrict_2d,
                  const int32_t* chem_index, const int32_t* chem_dir, const double* sensitivity);
 
+/* Phenotype-side SoA mutation (src/cell_phenotype.cpp:52-113, 571-720,
+ * 833-844; src/cell_data.cpp:398-542, 544-640).  The cells are the library's
+ * field arrays: f64[23] and u32[5] in HC_F_* order, rates[4] (secretion,
+ * uptake, saturation, net export; [cell*ns + s]), vol[15] in HC_V_* order,
+ * and optionally the motility state (persistence == NULL: none). */
+typedef struct or_cells {
+    double* f64[23];
+    uint32_t* u32[5];
+    double* rates[4];
+    double* vol[15];
+    double *persistence, *elapsed, *bias, *vec[3];
+    uint32_t *motile, *restrict_2d;
+    int32_t *chem_index, *chem_dir;
+    double* sensitivity;
+    uint32_t ns, _pad;
+} or_cells;
+/* updateVolume (src/cell_phenotype.cpp:52-113) on every i < n with mask[i]
+ * != 0 (mask NULL: all); use_ratio as the reference's ratio-array test. */
+void or_volume_update(or_cells* c, const uint32_t* mask, uint32_t n, double dt, int use_ratio);
+/* divideCell (src/cell_phenotype.cpp:571-720) for parents[k] in order,
+ * daughter n + k; the angle is 2 pi or_uniform(seed, counter, k, 0) and the
+ * oncoprotein noise 0.02 level sqrt(-2 ln(1 - u1)) cos(2 pi u2), u1, u2 the
+ * draws 1 and 2.  Divisions beyond max_cells are skipped.  Returns the new n. */
+uint32_t or_divide(or_cells* c, const uint32_t* parents, uint32_t count, uint32_t n, uint32_t max_cells,
+                   uint64_t seed, uint64_t counter);
+/* The removal loop (src/cell_phenotype.cpp:833-844): indices in descending
+ * order, each CellData::removeCell (src/cell_data.cpp:408-542; motility as
+ * MotilityData::swapRemove, src/motility.cpp:154-175).  Returns n - count. */
+uint32_t or_remove(or_cells* c, const uint32_t* indices, uint32_t count, uint32_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -258,3 +258,69 @@ def motility(x, y, z, vx, vy, vz, alive, speed, g, rho, dt, seed, step, persiste
                       ctypes.c_uint64(seed), ctypes.c_uint64(step), P(persistence), P(elapsed), P(bias),
                       P(mvx), P(mvy), P(mvz), P(motile, u32p), P(restrict_2d, u32p), P(chem_index, i32p),
                       P(chem_dir, i32p), None if sensitivity is None else P(sensitivity))
+
+
+class OrCells(ctypes.Structure):
+    _fields_ = [("f64", f64p * 23), ("u32", u32p * 5), ("rates", f64p * 4), ("vol", f64p * 15),
+                ("persistence", f64p), ("elapsed", f64p), ("bias", f64p), ("vec", f64p * 3),
+                ("motile", u32p), ("restrict_2d", u32p), ("chem_index", i32p), ("chem_dir", i32p),
+                ("sensitivity", f64p), ("ns", ctypes.c_uint32), ("_pad", ctypes.c_uint32)]
+
+
+class PhenoCells:
+    """Host copy of the library's per-cell buffers for the phenotype oracle:
+    f64 (23, M), u32 (5, M), rates (4, M*ns), vol (15, M) and, when
+    with_motility, the motility fields (numpy arrays, modified in place)."""
+
+    def __init__(self, f64, u32, rates, vol, ns, mot=None):
+        self.f64, self.u32, self.rates, self.vol, self.ns, self.mot = f64, u32, rates, vol, ns, mot
+        for a in (f64, rates, vol):
+            assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+        assert u32.dtype == np.uint32 and u32.flags["C_CONTIGUOUS"]
+
+    def struct(self):
+        c = OrCells()
+        for f in range(23):
+            c.f64[f] = self.f64[f].ctypes.data_as(f64p)
+        for f in range(5):
+            c.u32[f] = self.u32[f].ctypes.data_as(u32p)
+        for a in range(4):
+            c.rates[a] = self.rates[a].ctypes.data_as(f64p)
+        for f in range(15):
+            c.vol[f] = self.vol[f].ctypes.data_as(f64p)
+        if self.mot is not None:
+            m = self.mot
+            c.persistence, c.elapsed, c.bias = P(m["persistence"]), P(m["elapsed"]), P(m["bias"])
+            for q in range(3):
+                c.vec[q] = P(m["vec"][q])
+            c.motile, c.restrict_2d = P(m["motile"], u32p), P(m["r2d"], u32p)
+            c.chem_index, c.chem_dir = P(m["chem_index"], i32p), P(m["chem_dir"], i32p)
+            c.sensitivity = P(m["sens"])
+        c.ns = self.ns
+        return c
+
+
+def volume_update(cells, mask, n, dt, use_ratio):
+    """or_volume_update (src/cell_phenotype.cpp:52-113)."""
+    c = cells.struct()
+    lib().or_volume_update(ctypes.byref(c), None if mask is None else P(mask, u32p), ctypes.c_uint32(n),
+                           ctypes.c_double(dt), ctypes.c_int(use_ratio))
+
+
+def divide(cells, parents, n, max_cells, seed, counter):
+    """or_divide (src/cell_phenotype.cpp:571-720); returns the new n."""
+    c = cells.struct()
+    parents = np.ascontiguousarray(parents, np.uint32)
+    f = lib().or_divide
+    f.restype = ctypes.c_uint32
+    return f(ctypes.byref(c), P(parents, u32p), ctypes.c_uint32(len(parents)), ctypes.c_uint32(n),
+             ctypes.c_uint32(max_cells), ctypes.c_uint64(seed), ctypes.c_uint64(counter))
+
+
+def remove(cells, indices, n):
+    """or_remove (src/cell_phenotype.cpp:833-844, src/cell_data.cpp:408-542); returns n - count."""
+    c = cells.struct()
+    indices = np.ascontiguousarray(indices, np.uint32)
+    f = lib().or_remove
+    f.restype = ctypes.c_uint32
+    return f(ctypes.byref(c), P(indices, u32p), ctypes.c_uint32(len(indices)), ctypes.c_uint32(n))

@@ -1,6 +1,7 @@
 // ref_driver.cpp -- thin C entry points into the reference's own compiled
 // processSecretion (src/cell_phenotype.cpp:526-564), updateMotility
-// (src/motility.cpp:179-293) and .mat writers
+// (src/motility.cpp:179-293), updateVolume / divideCell / removeCell
+// (src/cell_phenotype.cpp, src/cell_data.cpp) and .mat writers
 // (src/matlab_writer.cpp, src/output_writer.cpp:295-453), used only by tests
 // to pin the FP64 oracle's secretion restatement and the library's snapshot
 // writer against the reference code.
@@ -161,5 +162,72 @@ extern "C" int ref_motility(uint32_t n, const float* x, const float* y, const fl
         mvx[i] = mot.motility_vec_x[i]; mvy[i] = mot.motility_vec_y[i]; mvz[i] = mot.motility_vec_z[i];
     }
     mot.free_all();
+    return 0;
+}
+
+// The reference's phenotype-side SoA mutation on a cell store filled from the
+// library's field layout (f64: 23 arrays of max_cells in HC_F_* order, stored
+// in the reference's FP32 fields; u32: 5 arrays; vol: 15 FP64 arrays in HC_V_*
+// order; rates: 4 arrays [cell*ns + s]).  op 0: updateVolume
+// (src/cell_phenotype.cpp:52-113) on every cell with list[i] != 0 (list NULL:
+// all); use_ratio 0 runs it without the cyto/nuclear ratio array.  op 1:
+// divideCell (:571-720) for each parent in list.  op 2: the removal loop
+// (:833-844), removeCell in descending order.  Every field is written back
+// (FP32 ones widened).  *n_out = the resulting cell count.
+#include "cell_phenotype.h"
+#include <algorithm>
+#include <functional>
+
+extern "C" int ref_phenotype_op(int op, uint32_t n, uint32_t max_cells, uint32_t ns, double* f64, uint32_t* u32,
+                                double* vol, double* rates, const uint32_t* list, uint32_t count, double dt,
+                                int use_ratio, uint32_t* n_out) {
+    CellData cells(max_cells);
+    std::vector<float> gpu(cells.gpuBufferSize() / sizeof(float), 0.0f);
+    cells.setGPUBufferBase(gpu.data());
+    cells.allocateCPUBuffers();
+    cells.allocateSubstrateBuffers(ns);
+    float* F[23] = {cells.position_x, cells.position_y, cells.position_z, cells.velocity_x, cells.velocity_y,
+                    cells.velocity_z, cells.prev_velocity_x, cells.prev_velocity_y, cells.prev_velocity_z,
+                    cells.radius, cells.nuclear_radius, cells.total_volume, cells.nuclear_volume,
+                    cells.cell_cell_repulsion, cells.cell_cell_adhesion, cells.relative_max_adhesion_distance,
+                    cells.motility_speed, cells.motility_bias, cells.motility_dir_x, cells.motility_dir_y,
+                    cells.motility_dir_z, cells.oncoprotein, cells.simple_pressure};
+    uint32_t* U[5] = {cells.cell_type, cells.current_phase, cells.is_alive, cells.voxel_index, cells.mech_voxel_index};
+    double* V[15] = {cells.fluid, cells.solid_nuclear, cells.solid_cytoplasmic, cells.fluid_fraction,
+                     cells.calcified_fraction, cells.target_solid_cytoplasmic, cells.target_solid_nuclear,
+                     cells.target_fluid_fraction, cells.target_cyto_to_nuclear_ratio, cells.fluid_change_rate,
+                     cells.nuclear_biomass_change_rate, cells.cytoplasmic_biomass_change_rate,
+                     cells.calcification_rate, cells.rupture_volume, cells.relative_rupture_volume};
+    double* R[4] = {cells.secretion_rate, cells.uptake_rate, cells.saturation_density, cells.net_export_rate};
+    for (uint32_t i = 0; i < n; i++) cells.addCell();
+    for (uint32_t i = 0; i < max_cells; i++) {
+        for (int f = 0; f < 23; f++) F[f][i] = (float)f64[(size_t)f * max_cells + i];
+        for (int f = 0; f < 5; f++) U[f][i] = u32[(size_t)f * max_cells + i];
+        for (int f = 0; f < 15; f++) V[f][i] = vol[(size_t)f * max_cells + i];
+        for (int a = 0; a < 4; a++)
+            for (uint32_t s = 0; s < ns; s++) R[a][(size_t)i * ns + s] = rates[(size_t)a * max_cells * ns + (size_t)i * ns + s];
+    }
+    if (op == 0) {
+        double* keep = cells.target_cyto_to_nuclear_ratio;
+        if (!use_ratio) cells.target_cyto_to_nuclear_ratio = nullptr;
+        for (uint32_t i = 0; i < n; i++)
+            if (!list || list[i]) updateVolume(cells, i, dt);
+        cells.target_cyto_to_nuclear_ratio = keep;
+    } else if (op == 1) {
+        for (uint32_t k = 0; k < count; k++) divideCell(cells, list[k]);
+    } else {
+        std::vector<uint32_t> idx(list, list + count);
+        std::sort(idx.begin(), idx.end(), std::greater<uint32_t>());
+        for (uint32_t i : idx)
+            if (i < cells.num_cells) cells.removeCell(i);
+    }
+    for (uint32_t i = 0; i < max_cells; i++) {
+        for (int f = 0; f < 23; f++) f64[(size_t)f * max_cells + i] = (double)F[f][i];
+        for (int f = 0; f < 5; f++) u32[(size_t)f * max_cells + i] = U[f][i];
+        for (int f = 0; f < 15; f++) vol[(size_t)f * max_cells + i] = V[f][i];
+        for (int a = 0; a < 4; a++)
+            for (uint32_t s = 0; s < ns; s++) rates[(size_t)a * max_cells * ns + (size_t)i * ns + s] = R[a][(size_t)i * ns + s];
+    }
+    *n_out = cells.num_cells;
     return 0;
 }

@@ -81,6 +81,49 @@ __device__ inline uint32_t stable_rank(const uint32_t* __restrict__ tmp, uint32_
 }
 
 
+// Counter-based uniform stream on [0, 1): the SplitMix64 finaliser (Steele,
+// Lea & Flood 2014) applied to (seed, step, cell, draw).  Stateless, so graph
+// replays and parallel lanes need no generator state.
+__device__ inline uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ inline double uniform01(uint64_t seed, uint64_t step, uint32_t cell, uint32_t draw) {
+    const uint64_t h = mix64(seed ^ mix64(step ^ mix64(((uint64_t)cell << 2) | draw)));
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// the motility buffer (hc_motility_field_offset)
+struct MotPtrs {
+    double *persistence, *elapsed, *bias, *vec[3];
+    uint32_t *motile, *restrict_2d;
+    int32_t *chem_index, *chem_dir;
+    double* sensitivity;
+    uint64_t* hdr;       // [0] step, [1] seed
+};
+
+inline MotPtrs mot_ptrs(void* base, uint32_t max_cells, uint32_t ns) {
+    uint8_t* b = (uint8_t*)base;
+    MotPtrs m;
+    m.hdr = (uint64_t*)b;
+    auto off = [&](uint32_t f) { return b + hc_motility_field_offset(f, max_cells, ns); };
+    m.persistence = (double*)off(HC_M_PERSISTENCE);
+    m.elapsed = (double*)off(HC_M_ELAPSED);
+    m.bias = (double*)off(HC_M_BIAS);
+    m.vec[0] = (double*)off(HC_M_VEC_X);
+    m.vec[1] = (double*)off(HC_M_VEC_Y);
+    m.vec[2] = (double*)off(HC_M_VEC_Z);
+    m.motile = (uint32_t*)off(HC_M_MOTILE);
+    m.restrict_2d = (uint32_t*)off(HC_M_RESTRICT_2D);
+    m.chem_index = (int32_t*)off(HC_M_CHEM_INDEX);
+    m.chem_dir = (int32_t*)off(HC_M_CHEM_DIR);
+    m.sensitivity = (double*)off(HC_M_SENSITIVITY);
+    return m;
+}
+
 int fill_u32(hc_ctx* ctx, uint32_t* p, uint64_t n, uint32_t v);
 int launch_scatter(hc_ctx* ctx, const uint32_t* key, const uint32_t* slot, const uint32_t* offsets, uint32_t* tmp,
                    uint32_t n);

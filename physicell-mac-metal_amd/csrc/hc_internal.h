@@ -33,6 +33,9 @@ struct hc_ctx {
     uint32_t* d_scalar = nullptr;
     // x lane order per (line length, segments, element size)
     std::map<uint64_t, int> xmap_cache;
+    // device copy of host index lists (division parents, removal moves)
+    void* d_list = nullptr;
+    size_t d_list_bytes = 0;
 };
 
 struct hc_buf {

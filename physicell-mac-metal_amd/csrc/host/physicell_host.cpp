@@ -371,6 +371,8 @@ Simulation::~Simulation() {
     microenv = Microenvironment();
     mechanics = CellMechanics();
     motility_ = DeviceBuffer();
+    volume_ = DeviceBuffer();
+    mask_ = DeviceBuffer();
     hc_destroy(ctx);
 }
 
@@ -461,6 +463,60 @@ void Simulation::setMotility(uint64_t seed, const double* persistence, const dou
         hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
         graph_exec_ = nullptr;
     }
+}
+
+void Simulation::setVolume(uint64_t seed, const double* const* fields) {
+    const uint32_t mc = cells.max_cells, n = cells.num_cells;
+    volume_.allocate(ctx, hc_volume_bytes(mc));
+    uint8_t* h = (uint8_t*)volume_.host();
+    std::memset(h, 0, volume_.size());
+    uint64_t* hdr = (uint64_t*)h;
+    hdr[0] = 0;          // division counter
+    hdr[1] = seed;
+    // CellData::setDefaults (src/cell_data.cpp:555-624)
+    const double V = 2494.0, Vn = 540.0, ff = 0.75, Vc = V - Vn;
+    const double Vns = Vn - ff * Vn, Vcs = Vc - ff * Vc;
+    const double fresh[HC_NUM_V_FIELDS] = {ff * V, Vns, Vcs, ff, 0.0, Vcs, Vns, ff, 3.62,
+                                           3.0 / 60.0, 0.33 / 60.0, 0.27 / 60.0, 0.0, 2.0 * V, 2.0};
+    for (uint32_t f = 0; f < HC_NUM_V_FIELDS; f++) {
+        double* a = (double*)(h + hc_volume_field_offset(f, mc));
+        for (uint32_t i = 0; i < mc; i++) a[i] = fresh[f];
+        if (fields && fields[f]) std::memcpy(a, fields[f], (size_t)n * sizeof(double));
+    }
+    volume_.upload_all();
+    check(ctx, hc_wait(ctx), "wait");
+}
+
+uint32_t Simulation::updatePhenotypeSoA(double dt, int use_ratio, const uint32_t* mask, const uint32_t* parents,
+                                        uint32_t n_parents, const uint32_t* removed, uint32_t n_removed) {
+    if (!volume_.get()) throw std::runtime_error("updatePhenotypeSoA: call setVolume first");
+    const uint32_t mc = cells.max_cells, ns = cells.n_substrates;
+    uint32_t n = cells.num_cells;
+    hc_buf* mbuf = nullptr;
+    if (mask && n) {
+        if (!mask_.get() || mask_.size() < (size_t)mc * 4) mask_.allocate(ctx, (size_t)mc * 4);
+        std::memcpy(mask_.host(), mask, (size_t)n * 4);
+        mask_.upload(0, (size_t)n * 4);
+        mbuf = mask_.get();
+    }
+    check(ctx, hc_volume_update(ctx, cells.gpu_buffer(), volume_.get(), mbuf, n, mc, dt, use_ratio), "volume");
+    hc_buf* mot = motility_on_ ? motility_.get() : nullptr;
+    uint32_t n1 = n;
+    check(ctx, hc_divide(ctx, cells.gpu_buffer(), cells.rates_buffer(), volume_.get(), mot, ns, parents, n_parents,
+                         n, mc, &n1), "divide");
+    uint32_t n2 = n1;
+    check(ctx, hc_remove(ctx, cells.gpu_buffer(), cells.rates_buffer(), volume_.get(), mot, ns, removed, n_removed,
+                         n1, mc, &n2), "remove");
+    cells.num_cells = n2;
+    // the rates moved on the device: refresh the host copy
+    if (ns) check(ctx, hc_download(ctx, cells.rates_buffer(), 0, hc_rates_bytes(mc, ns)), "download rates");
+    check(ctx, hc_wait(ctx), "wait");
+    phenotypeStep();
+    if (graph_exec_) {
+        hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
+        graph_exec_ = nullptr;
+    }
+    return n2;
 }
 
 void Simulation::phenotypeStep() {
@@ -715,6 +771,19 @@ int pc_sim_write_snapshot(pc_sim* s, const char* folder, int frame) {
         if (rc) throw std::runtime_error(std::string("pc_sim_write_snapshot: cannot write under ") + folder);
     });
     return g ? g : rc;
+}
+
+int pc_sim_set_volume(pc_sim* s, uint64_t seed, const double* const* fields) {
+    return guarded(s, [&](Simulation& m) { m.setVolume(seed, fields); });
+}
+
+int pc_sim_update_phenotype(pc_sim* s, double dt, int use_ratio, const uint32_t* mask, const uint32_t* parents,
+                            uint32_t n_parents, const uint32_t* removed, uint32_t n_removed, uint32_t* n_out) {
+    if ((n_parents && !parents) || (n_removed && !removed)) return HC_ERR_ARG;
+    return guarded(s, [&](Simulation& m) {
+        const uint32_t n = m.updatePhenotypeSoA(dt, use_ratio, mask, parents, n_parents, removed, n_removed);
+        if (n_out) *n_out = n;
+    });
 }
 
 int pc_sim_set_motility(pc_sim* s, uint64_t seed, const double* persistence, const double* bias,

@@ -204,6 +204,18 @@ public:
     void setMotility(uint64_t seed, const double* persistence, const double* bias, const double* speed,
                      const uint32_t* is_motile, const uint32_t* restrict_2d, const int32_t* chem_index,
                      const int32_t* chem_dir, const double* sensitivity);
+    // Volume state (HC_V_* fields: one array of num_cells values each, or
+    // null for CellData's fresh value, src/cell_data.cpp:597-624) and the seed
+    // of division's draws.  Required before updatePhenotypeSoA.
+    void setVolume(uint64_t seed, const double* const* fields);
+    // updatePhenotype's SoA side on the device (src/cell_phenotype.cpp:
+    // 792-845): the volume model for cells with mask[i] != 0 (mask: num_cells
+    // values, or null for all), then the divisions of `parents` (daughters
+    // appended), then the removal of `removed` (indices below the count after
+    // the divisions); voxel keys and the secretion plan follow.  Returns the
+    // new cell count.
+    uint32_t updatePhenotypeSoA(double dt, int use_ratio, const uint32_t* mask, const uint32_t* parents,
+                                uint32_t n_parents, const uint32_t* removed, uint32_t n_removed);
     void wait();
     int substepsPerMechanicsStep() const;
 
@@ -222,6 +234,7 @@ private:
     void* join_ = nullptr;         // hipEvent_t
     bool overlap_ = true;
     DeviceBuffer motility_;
+    DeviceBuffer volume_, mask_;
     bool motility_on_ = false;
     int motility_use_sensitivity_ = 0;
 };

@@ -16,45 +16,6 @@
 
 namespace hc {
 
-__device__ inline uint64_t mix64(uint64_t z) {
-    z += 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-
-__device__ inline double uniform01(uint64_t seed, uint64_t step, uint32_t cell, uint32_t draw) {
-    const uint64_t h = mix64(seed ^ mix64(step ^ mix64(((uint64_t)cell << 2) | draw)));
-    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
-}
-
-struct MotPtrs {
-    double *persistence, *elapsed, *bias, *vec[3];
-    uint32_t *motile, *restrict_2d;
-    int32_t *chem_index, *chem_dir;
-    double* sensitivity;
-    uint64_t* hdr;       // [0] step, [1] seed
-};
-
-inline MotPtrs mot_ptrs(void* base, uint32_t max_cells, uint32_t ns) {
-    uint8_t* b = (uint8_t*)base;
-    MotPtrs m;
-    m.hdr = (uint64_t*)b;
-    auto off = [&](uint32_t f) { return b + hc_motility_field_offset(f, max_cells, ns); };
-    m.persistence = (double*)off(HC_M_PERSISTENCE);
-    m.elapsed = (double*)off(HC_M_ELAPSED);
-    m.bias = (double*)off(HC_M_BIAS);
-    m.vec[0] = (double*)off(HC_M_VEC_X);
-    m.vec[1] = (double*)off(HC_M_VEC_Y);
-    m.vec[2] = (double*)off(HC_M_VEC_Z);
-    m.motile = (uint32_t*)off(HC_M_MOTILE);
-    m.restrict_2d = (uint32_t*)off(HC_M_RESTRICT_2D);
-    m.chem_index = (int32_t*)off(HC_M_CHEM_INDEX);
-    m.chem_dir = (int32_t*)off(HC_M_CHEM_DIR);
-    m.sensitivity = (double*)off(HC_M_SENSITIVITY);
-    return m;
-}
-
 template <typename F>
 __device__ inline void gradient(const F* __restrict__ rho, const hc_grid& g, double px, double py, double pz,
                                 uint32_t s, double& gx, double& gy, double& gz) {
@@ -187,6 +148,7 @@ int hc_motility(hc_ctx* ctx, hc_buf* cells, hc_buf* motility, hc_buf* density, c
     const MotPtrs mp = mot_ptrs(motility->dptr, max_cells, g->n_substrates);
     if (n) {
         const dim3 grid(blocks_for(n, 256));
+        KernelTimer tm(ctx, HC_K_MOTILITY);
         if (g->field_fp32)
             hipLaunchKernelGGL(k_motility<float>, grid, dim3(256), 0, ctx->stream, cp, mp,
                                density ? (const float*)density->dptr : nullptr, *g, dt, n, use_sensitivity);

@@ -30,7 +30,7 @@ F_RADIUS, F_TOTAL_VOL = 9, 11
 F_REPULSION, F_ADHESION, F_REL_MAX_ADH = 13, 14, 15
 F_PRESSURE = 22
 F_CELL_TYPE, F_CUR_PHASE, F_IS_ALIVE, F_VOXEL_IDX, F_MECH_VOXEL = 23, 24, 25, 26, 27
-F_NUCLEAR_RAD, F_NUCLEAR_VOL, F_MOT_SPEED = 10, 12, 16
+F_NUCLEAR_RAD, F_NUCLEAR_VOL, F_MOT_SPEED, F_MOT_BIAS, F_ONCOPROTEIN = 10, 12, 16, 17, 21
 # motility buffer fields (include/physicell_hip.h, HC_M_*)
 M_PERSISTENCE, M_ELAPSED, M_BIAS, M_VEC_X, M_VEC_Y, M_VEC_Z = 0, 1, 2, 3, 4, 5
 M_MOTILE, M_RESTRICT_2D, M_CHEM_INDEX, M_CHEM_DIR, M_SENSITIVITY = 6, 7, 8, 9, 10
@@ -38,8 +38,15 @@ NUM_F64_FIELDS = 23
 
 K_SWEEP_X, K_SWEEP_Y, K_SWEEP_Z, K_SWEEP_SMALL, K_DIRICHLET, K_VOXEL_KEYS = 0, 1, 2, 3, 4, 5
 K_SEC_PLAN, K_SEC_APPLY, K_HASH, K_FORCES, K_INTEGRATE = 6, 7, 8, 9, 10
+K_MOTILITY, K_VOLUME, K_DIVIDE, K_REMOVE = 11, 12, 13, 14
 KERNEL_NAMES = ["sweep_x", "sweep_y", "sweep_z", "sweep_small", "dirichlet", "voxel_keys",
-                "secretion_plan", "secretion_apply", "hash", "forces", "integrate"]
+                "secretion_plan", "secretion_apply", "hash", "forces", "integrate", "motility",
+                "volume", "divide", "remove"]
+# volume state fields (include/physicell_hip.h, HC_V_*)
+(V_FLUID, V_SOLID_NUC, V_SOLID_CYTO, V_FLUID_FRAC, V_CALC_FRAC, V_TARGET_SOLID_CYTO, V_TARGET_SOLID_NUC,
+ V_TARGET_FLUID_FRAC, V_TARGET_CN_RATIO, V_FLUID_RATE, V_NUC_RATE, V_CYTO_RATE, V_CALC_RATE, V_RUPTURE_VOL,
+ V_REL_RUPTURE_VOL) = range(15)
+NUM_V_FIELDS = 15
 
 
 class HipError(RuntimeError):
@@ -156,8 +163,15 @@ def load(path=LIB_PATH):
         "pc_sim_info": (c_i32, [c_vp, P(Grid), P(Mech), P(c_u32)]),
         "pc_sim_context": (c_vp, [c_vp]),
         "hc_motility_bytes": (c_sz, [c_u32, c_u32]), "hc_motility_field_offset": (c_sz, [c_u32, c_u32, c_u32]),
+        "hc_volume_bytes": (c_sz, [c_u32]), "hc_volume_field_offset": (c_sz, [c_u32, c_u32]),
+        "hc_volume_update": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_u32, c_u32, c_f64, c_i32]),
+        "hc_divide": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u32, c_vp, c_u32, c_u32, c_u32, c_vp]),
+        "hc_remove": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u32, c_vp, c_u32, c_u32, c_u32, c_vp]),
+        "hc_remove_plan": (c_i32, [c_vp, c_u32, c_u32, c_vp, c_vp, c_vp]),
         "hc_motility": (c_i32, [c_vp, c_vp, c_vp, c_vp, P(Grid), c_f64, c_u32, c_u32, c_i32]),
         "pc_sim_set_motility": (c_i32, [c_vp, ctypes.c_uint64] + [c_vp] * 8),
+        "pc_sim_set_volume": (c_i32, [c_vp, ctypes.c_uint64, c_vp]),
+        "pc_sim_update_phenotype": (c_i32, [c_vp, c_f64, c_i32, c_vp, c_vp, c_u32, c_vp, c_u32, c_vp]),
         "pc_write_matlab4": (c_i32, [ctypes.c_char_p, ctypes.c_char_p, c_u32, c_u32, c_vp]),
         "pc_write_snapshot": (c_i32, [ctypes.c_char_p, c_i32, P(Snapshot)]),
         "pc_sim_write_snapshot": (c_i32, [c_vp, ctypes.c_char_p, c_i32]),
@@ -404,6 +418,26 @@ class Simulation:
         arrs = [None if a is None else np.ascontiguousarray(np.asarray(a, t).reshape(-1)) for a, t in conv]
         self._keep_mot = arrs
         self._check(load().pc_sim_set_motility(self.h, seed, *[_ptr(a) for a in arrs]), "pc_sim_set_motility")
+
+    def set_volume(self, seed, fields=None):
+        """fields: {HC_V_* number: array of n values}; missing ones take
+        CellData's fresh values."""
+        fields = fields or {}
+        arrs = [None if f not in fields else np.ascontiguousarray(fields[f], np.float64) for f in range(NUM_V_FIELDS)]
+        table = (c_vp * NUM_V_FIELDS)(*[None if a is None else a.ctypes.data for a in arrs])
+        self._check(load().pc_sim_set_volume(self.h, seed, table), "pc_sim_set_volume")
+
+    def update_phenotype(self, dt, mask=None, parents=(), removed=(), use_ratio=True):
+        """Volume model (cells with mask != 0), divisions, removals on the
+        device; returns the new cell count."""
+        m = None if mask is None else np.ascontiguousarray(mask, np.uint32)
+        pa = np.ascontiguousarray(parents, np.uint32)
+        rm = np.ascontiguousarray(removed, np.uint32)
+        out = ctypes.c_uint32()
+        self._check(load().pc_sim_update_phenotype(self.h, dt, 1 if use_ratio else 0, _ptr(m), _ptr(pa), len(pa),
+                                                   _ptr(rm), len(rm), ctypes.byref(out)), "pc_sim_update_phenotype")
+        self.n = out.value
+        return self.n
 
     def density(self):
         out = np.empty(self.ns * self.spec.n_voxels)
