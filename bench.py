@@ -14,6 +14,7 @@ context stream around that kernel's launches, in the timed region) and the
 CPU baseline (the FP64 oracle on this host's cores, bounded sample).
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -201,6 +202,120 @@ def bench_ensemble(args):
     print(json.dumps(out))
 
 
+def bench_rows(args):
+    """The rows beside the step, on the config-2 population (1M agents, 200^3
+    x 4 substrates, plain FP64): motility in every mechanics step
+    (hc_motility, every agent motile), and the device phenotype step
+    (hc_volume_update on every agent, hc_divide of 1%, hc_remove of 1%).
+    Kernel times are HIP events on the context stream; algorithmic bytes per
+    launch are stated in the output; the oracle runs the same operations on
+    this host (one thread) beside them."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    spec, cells = build_workload("config2", args.cells)
+    n = cells.n
+    M = n + n // 50
+    sim = H.Simulation(spec, cells, max_cells=M)
+    r = np.random.default_rng(2024)
+    pers = r.uniform(1.0, 10.0, n)
+    sim.set_motility(seed=11, persistence=pers, bias=np.full(n, 0.5), speed=np.ones(n),
+                     is_motile=np.ones(n, np.uint32), restrict_2d=np.zeros(n, np.uint32),
+                     chem_index=np.zeros(n, np.int32), chem_dir=np.ones(n, np.int32))
+    sim.set_volume(seed=12)
+    ctx = sim.context()
+    lib = H.load()
+    print(f"[bench] rows: {n} agents, motility on, warmup {args.warmup} periods", file=sys.stderr, flush=True)
+    sim.run_periods(max(1, args.warmup), use_graph=True)
+    sim.wait()
+    t0 = time.perf_counter()
+    sim.run_periods(args.steps, use_graph=True)
+    sim.wait()
+    ms_step = 1e3 * (time.perf_counter() - t0) / args.steps
+    # kernel times from eager periods (graph replays record no events)
+    lib.hc_timing_reset(ctx)
+    lib.hc_timing_enable(ctx, 1 << H.K_MOTILITY)
+    sim.run_periods(args.steps, use_graph=False)
+    sim.wait()
+    out = {}
+
+    def kern(k):
+        ms, cnt = ctypes.c_double(), ctypes.c_int64()
+        lib.hc_timing_get(ctx, k, ctypes.byref(ms), ctypes.byref(cnt))
+        return 1e3 * ms.value / max(1, cnt.value), cnt.value
+
+    us, cnt = kern(H.K_MOTILITY)
+    resample = float(np.minimum(1.0, spec.dt_mech / pers).sum())
+    mot_bytes = 104.0 * n + 100.0 * resample
+    out["motility"] = {"avg_us": us, "launches": cnt, "bytes_per_launch": mot_bytes,
+                       "achieved_gbs": mot_bytes / us * 1e-3, "frac": mot_bytes / us * 1e-3 / HBM_PEAK_GBS,
+                       "bytes_basis": "104 B per motile agent (alive, motile, elapsed RMW, persistence, velocity "
+                                      "RMW, vector) + 100 B per resample (bias, flags, speed, position, gradient)"}
+    # device phenotype step, 1% divisions and 1% removals per call
+    lib.hc_timing_reset(ctx)
+    lib.hc_timing_enable(ctx, (1 << H.K_VOLUME) | (1 << H.K_DIVIDE) | (1 << H.K_REMOVE))
+    calls, walls = 3, []
+    cur = n
+    for _ in range(calls):
+        parents = r.choice(cur, n // 100, replace=False).astype(np.uint32)
+        removed = r.choice(cur + len(parents), n // 100, replace=False).astype(np.uint32)
+        t1 = time.perf_counter()
+        cur = sim.update_phenotype(6.0, parents=parents, removed=removed)
+        walls.append(1e3 * (time.perf_counter() - t1))
+    ns = len(spec.substrates)
+    per = {H.K_VOLUME: ("volume", 192.0 * n, "192 B per agent (2 volumes + 12 state reads; 4 geometry + 6 state writes)"),
+           H.K_DIVIDE: ("divide", (2 * 8 * 23 + 2 * 4 * 5 + 2 * 8 * 15 + 4 * 8 * ns * 2) * (n // 100),
+                        "per division: parent read + parent and daughter writes of the cell SoA, volume state, rates"),
+           H.K_REMOVE: ("remove", 2 * (8 * 23 + 4 * 5 + 8 * 15 + 4 * 8 * ns) * (n // 100),
+                        "per removal at most one move: read + write of the cell SoA, volume state, rates")}
+    for k, (name, b, basis) in per.items():
+        us, cnt = kern(k)
+        out[name] = {"avg_us": us, "launches": cnt, "bytes_per_launch": b, "achieved_gbs": b / us * 1e-3,
+                     "frac": b / us * 1e-3 / HBM_PEAK_GBS, "bytes_basis": basis}
+    # the oracle beside it (one host thread), same operations and sizes
+    cpu = {}
+    x, y, z = (np.ascontiguousarray(np.asarray(a, np.float64)) for a in (cells.x, cells.y, cells.z))
+    vel = [np.zeros(n) for _ in range(3)]
+    mv = [np.zeros(n) for _ in range(3)]
+    rho = np.full(spec.n_voxels * ns, 1.0)
+    t1 = time.perf_counter()
+    O.motility(x, y, z, *vel, np.ones(n, np.uint32), np.ones(n), O.grid(spec), rho, spec.dt_mech, 11, 0, pers.copy(),
+               np.zeros(n), np.full(n, 0.5), *mv, np.ones(n, np.uint32), np.zeros(n, np.uint32),
+               np.zeros(n, np.int32), np.ones(n, np.int32))
+    cpu["motility_ms"] = 1e3 * (time.perf_counter() - t1)
+    f64 = np.zeros((23, M))
+    f64[H.F_TOTAL_VOL, :n], f64[H.F_NUCLEAR_VOL, :n], f64[H.F_ONCOPROTEIN, :n] = 2494.0, 540.0, 1.0
+    vol = np.zeros((15, M))
+    vol[:, :] = np.array([0.75 * 2494, 135.0, 488.5, 0.75, 0, 488.5, 135.0, 0.75, 3.62, 0.05, 0.0055, 0.0045, 0,
+                          4988.0, 2.0])[:, None]
+    pc = O.PhenoCells(f64, np.zeros((5, M), np.uint32), np.zeros((4, M * ns)), vol, ns)
+    t1 = time.perf_counter()
+    O.volume_update(pc, None, n, 6.0, 1)
+    cpu["volume_ms"] = 1e3 * (time.perf_counter() - t1)
+    parents = r.choice(n, n // 100, replace=False).astype(np.uint32)
+    t1 = time.perf_counter()
+    n1 = O.divide(pc, parents, n, M, 12, 0)
+    cpu["divide_ms"] = 1e3 * (time.perf_counter() - t1)
+    t1 = time.perf_counter()
+    O.remove(pc, r.choice(n1, n // 100, replace=False).astype(np.uint32), n1)
+    cpu["remove_ms"] = 1e3 * (time.perf_counter() - t1)
+    line = {
+        "metric": "kernel time and HBM roofline fraction of the rows beside the step (motility every mechanics "
+                  "step; device phenotype step: volume model, 1% divisions, 1% removals)",
+        "value": n / (out["motility"]["avg_us"] * 1e-6), "unit": "motile agent-steps/s (motility kernel)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"config2 population ({n} agents, 200^3 x {ns} substrates), every agent motile "
+                               f"(persistence U[1,10] min, bias 0.5, speed 1); phenotype calls: {n // 100} "
+                               f"divisions + {n // 100} removals each", "graph": True},
+        "kernels": out,
+        "phenotype_call_ms": walls,
+        "cpu_baseline": {"kind": "port", "cores": 1, **cpu,
+                         "sample": f"the FP64 oracle on this host, one thread: motility over {n} agents, volume "
+                                   f"model over {n}, {n // 100} divisions, {n // 100} removals"},
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,6 +356,8 @@ def main():
 
     if args.workload == "config3":
         return bench_ensemble(args)
+    if args.workload == "rows":
+        return bench_rows(args)
 
     t_setup = time.perf_counter()
     spec, cells = build_workload(args.workload, args.cells)
