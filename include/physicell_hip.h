@@ -234,6 +234,29 @@ int hc_neighbor_lists(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* m
                       uint32_t n, uint32_t max_cells, uint32_t n_mvox, uint32_t cap,
                       hc_buf* counts, hc_buf* idx);
 
+/* Neighbour queries on the hash for the phenotype side, as CSR lists per cell:
+ * offsets[n + 1] (device) and idx[*total] (device); cell i's neighbours are
+ * idx[offsets[i] .. offsets[i+1]).  The hash must have been built from the
+ * current positions (hc_build_hash, FP64 or FP32 records).  Positions and
+ * radii come from the cell SoA in FP64.
+ *   HC_QUERY_CONTACT: writeCellNeighborGraph (src/output_writer.cpp:457-490,
+ *     an O(N^2) loop there): for live cells with volume > 1e-15, the cells
+ *     j > i (live, volume > 1e-15) with |d|^2 <= (max(rel_i, rel_j) (r_i +
+ *     r_j))^2, ascending j.  Fails (HC_ERR_CAPACITY) when the largest such
+ *     distance, max rel x 2 max r, exceeds the mechanics voxel (the bins
+ *     would not hold every pair).
+ *   HC_QUERY_INTERACT: the candidate scan of standardCellCellInteractions
+ *     (src/cell_phenotype.cpp:893-967): the first 512 cells j != i of the
+ *     3x3x3 bins in visiting order (dk, dj, di, then ascending index), kept
+ *     when volume_j >= 1e-15, 2-D distance^2 <= (2.5 r_i + r_j)^2 and
+ *     distance <= 1.25 (r_i + r_j); visiting order kept.  Which cells query
+ *     (death model, interaction rates) is the caller's.
+ * idx may be NULL: then only offsets and *total are produced (size the idx
+ * buffer and call again); an idx buffer smaller than *total fails. */
+enum { HC_QUERY_CONTACT = 0, HC_QUERY_INTERACT = 1 };
+int hc_neighbor_query(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech, uint32_t n, uint32_t max_cells,
+                      uint32_t n_mvox, int kind, hc_buf* offsets, hc_buf* idx, uint64_t* total);
+
 /* Per-substrate totals sum_v rho[s][v] with compensated (Neumaier) FP64
  * summation in a fixed order (deterministic); out[n_substrates]. */
 int hc_substrate_totals(hc_ctx* ctx, hc_buf* density, const hc_grid* grid, double* out);
@@ -324,6 +347,11 @@ int pc_sim_set_volume(pc_sim* sim, uint64_t seed, const double* const* fields);
  * pc_sim_phenotype_step.  *n_out (may be NULL) = the new cell count. */
 int pc_sim_update_phenotype(pc_sim* sim, double dt, int use_ratio, const uint32_t* mask, const uint32_t* parents,
                             uint32_t n_parents, const uint32_t* removed, uint32_t n_removed, uint32_t* n_out);
+/* Builds the hash from the current positions and runs hc_neighbor_query;
+ * copies out offsets[n_cells + 1] and, when idx is not NULL and capacity >=
+ * *total, idx[*total] (host arrays).  *total is always set. */
+int pc_sim_neighbor_query(pc_sim* sim, int kind, uint32_t* offsets, uint32_t* idx, uint64_t capacity,
+                          uint64_t* total);
 int pc_sim_wait(pc_sim* sim);
 /* Copies out (waits first).  density: ns*nx*ny*nz FP64 (converted from FP32
  * in the mixed form). */
@@ -433,6 +461,10 @@ int hc_remove_plan(const uint32_t* indices, uint32_t count, uint32_t n, uint32_t
  * name with its NUL, then rows*cols doubles, column-major
  * (data[c*rows + r] = M[r][c]).  Host only. */
 int pc_write_matlab4(const char* path, const char* name, uint32_t rows, uint32_t cols, const double* data);
+/* final_cell_neighbor_graph.txt as writeCellNeighborGraph writes it
+ * (src/output_writer.cpp:457-490): one "i j" line per pair, i ascending, then
+ * j ascending; from a contact query's CSR lists (host arrays). */
+int pc_write_neighbor_graph(const char* path, uint32_t n, const uint32_t* offsets, const uint32_t* idx);
 
 /* Host view of one snapshot of the path's state. */
 typedef struct pc_snapshot {

@@ -702,3 +702,70 @@ uint32_t or_remove(or_cells* c, const uint32_t* indices, uint32_t count, uint32_
     }
     return cur;
 }
+
+// ---------------------------------------------------------------------------
+// neighbour queries (src/output_writer.cpp:457-490, src/cell_phenotype.cpp:
+// 893-967)
+// ---------------------------------------------------------------------------
+uint64_t or_contact_graph(const double* x, const double* y, const double* z, const double* radius,
+                          const double* volume, const double* rel, const uint32_t* alive, uint32_t n,
+                          uint32_t* offsets, uint32_t* idx) {
+    uint64_t tot = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        offsets[i] = (uint32_t)tot;
+        if (!alive[i] || !(volume[i] > 1e-15)) continue;
+        for (uint32_t j = i + 1; j < n; j++) {
+            if (!alive[j] || !(volume[j] > 1e-15)) continue;
+            const double dx = x[i] - x[j], dy = y[i] - y[j], dz = z[i] - z[j];
+            const double md = std::max(rel[i], rel[j]) * (radius[i] + radius[j]);
+            if (dx * dx + dy * dy + dz * dz <= md * md) {
+                if (idx) idx[tot] = j;
+                tot++;
+            }
+        }
+    }
+    offsets[n] = (uint32_t)tot;
+    return tot;
+}
+
+uint64_t or_interaction_candidates(const double* x, const double* y, const double* radius, const double* volume,
+                                   uint32_t n, const uint32_t* keys, const uint32_t* off, const uint32_t* srt,
+                                   const or_mech* m, uint32_t* offsets, uint32_t* idx) {
+    uint64_t tot = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        offsets[i] = (uint32_t)tot;
+        const uint32_t key = keys[i];
+        const int vx = (int)(key % m->mnx), vy = (int)((key / m->mnx) % m->mny), vz = (int)(key / (m->mnx * m->mny));
+        std::vector<uint32_t> cand;
+        for (int dk = -1; dk <= 1; dk++) {
+            const int nz = vz + dk;
+            if (nz < 0 || nz >= (int)m->mnz) continue;
+            for (int dj = -1; dj <= 1; dj++) {
+                const int ny = vy + dj;
+                if (ny < 0 || ny >= (int)m->mny) continue;
+                for (int di = -1; di <= 1; di++) {
+                    const int nx = vx + di;
+                    if (nx < 0 || nx >= (int)m->mnx) continue;
+                    const uint32_t b = ((uint32_t)nz * m->mny + (uint32_t)ny) * m->mnx + (uint32_t)nx;
+                    for (uint32_t u = off[b]; u < off[b + 1] && cand.size() < 512; u++) {
+                        const uint32_t j = srt[u];
+                        if (j != i && j < n) cand.push_back(j);
+                    }
+                }
+            }
+        }
+        const double ri = radius[i];
+        for (uint32_t j : cand) {
+            if (volume[j] < 1e-15) continue;
+            const double dx = x[j] - x[i], dy = y[j] - y[i];
+            const double d2 = dx * dx + dy * dy, rj = radius[j];
+            const double max_r = 2.5 * ri + rj;
+            if (d2 > max_r * max_r) continue;
+            if (std::sqrt(d2) > (ri + rj) * 1.25) continue;
+            if (idx) idx[tot] = j;
+            tot++;
+        }
+    }
+    offsets[n] = (uint32_t)tot;
+    return tot;
+}

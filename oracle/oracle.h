@@ -191,6 +191,22 @@ uint32_t or_divide(or_cells* c, const uint32_t* parents, uint32_t count, uint32_
  * MotilityData::swapRemove, src/motility.cpp:154-175).  Returns n - count. */
 uint32_t or_remove(or_cells* c, const uint32_t* indices, uint32_t count, uint32_t n);
 
+/* writeCellNeighborGraph (src/output_writer.cpp:457-490), the O(N^2) pair
+ * loop: for i < j, both alive with volume > 1e-15, |d|^2 <= (max(rel_i,
+ * rel_j) (r_i + r_j))^2.  CSR by i (offsets[n+1]), j ascending; returns the
+ * number of pairs (idx may be NULL to count). */
+uint64_t or_contact_graph(const double* x, const double* y, const double* z, const double* radius,
+                          const double* volume, const double* rel, const uint32_t* alive, uint32_t n,
+                          uint32_t* offsets, uint32_t* idx);
+/* The candidate scan of standardCellCellInteractions (src/cell_phenotype.cpp:
+ * 893-967) over the hash (keys, off, srt of or_mech_keys / or_hash): the
+ * first 512 cells j != i of the 3x3x3 bins around key[i] in visiting order,
+ * kept when volume_j >= 1e-15, 2-D d^2 <= (2.5 r_i + r_j)^2 and d <= 1.25
+ * (r_i + r_j).  CSR by i; returns the number of entries. */
+uint64_t or_interaction_candidates(const double* x, const double* y, const double* radius, const double* volume,
+                                   uint32_t n, const uint32_t* keys, const uint32_t* off, const uint32_t* srt,
+                                   const or_mech* m, uint32_t* offsets, uint32_t* idx);
+
 #ifdef __cplusplus
 }
 #endif

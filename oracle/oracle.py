@@ -324,3 +324,30 @@ def remove(cells, indices, n):
     f = lib().or_remove
     f.restype = ctypes.c_uint32
     return f(ctypes.byref(c), P(indices, u32p), ctypes.c_uint32(len(indices)), ctypes.c_uint32(n))
+
+
+def contact_graph(x, y, z, radius, volume, rel, alive):
+    """or_contact_graph (src/output_writer.cpp:457-490): (offsets, idx)."""
+    n = len(x)
+    off = np.zeros(n + 1, np.uint32)
+    f = lib().or_contact_graph
+    f.restype = ctypes.c_uint64
+    tot = f(P(x), P(y), P(z), P(radius), P(volume), P(rel), P(alive, u32p), ctypes.c_uint32(n), P(off, u32p), None)
+    idx = np.zeros(max(1, tot), np.uint32)
+    f(P(x), P(y), P(z), P(radius), P(volume), P(rel), P(alive, u32p), ctypes.c_uint32(n), P(off, u32p),
+      P(idx, u32p))
+    return off, idx[:tot]
+
+
+def interaction_candidates(x, y, radius, volume, keys, off, srt, m):
+    """or_interaction_candidates (src/cell_phenotype.cpp:893-967): (offsets, idx)."""
+    n = len(x)
+    offs = np.zeros(n + 1, np.uint32)
+    f = lib().or_interaction_candidates
+    f.restype = ctypes.c_uint64
+    args = (P(x), P(y), P(radius), P(volume), ctypes.c_uint32(n), P(keys, u32p), P(off, u32p), P(srt, u32p),
+            ctypes.byref(m), P(offs, u32p))
+    tot = f(*args, None)
+    idx = np.zeros(max(1, tot), np.uint32)
+    f(*args, P(idx, u32p))
+    return offs, idx[:tot]

@@ -231,3 +231,24 @@ extern "C" int ref_phenotype_op(int op, uint32_t n, uint32_t max_cells, uint32_t
     *n_out = cells.num_cells;
     return 0;
 }
+
+// The reference's writeCellNeighborGraph (src/output_writer.cpp:457-490) on n
+// cells (FP32 positions, radii, volumes and adhesion scales, as its store
+// holds them); writes <folder>/final_cell_neighbor_graph.txt.
+extern "C" int ref_write_neighbor_graph(const char* folder, uint32_t n, const float* x, const float* y, const float* z,
+                                        const float* radius, const float* volume, const float* rel,
+                                        const uint32_t* alive) {
+    CellData cells(n > 0 ? n : 1);
+    std::vector<float> gpu(cells.gpuBufferSize() / sizeof(float), 0.0f);
+    cells.setGPUBufferBase(gpu.data());
+    for (uint32_t i = 0; i < n; i++) cells.addCell();
+    for (uint32_t i = 0; i < n; i++) {
+        cells.position_x[i] = x[i]; cells.position_y[i] = y[i]; cells.position_z[i] = z[i];
+        cells.radius[i] = radius[i];
+        cells.total_volume[i] = volume[i];
+        cells.relative_max_adhesion_distance[i] = rel[i];
+        cells.is_alive[i] = alive[i];
+    }
+    writeCellNeighborGraph(cells, folder);
+    return 0;
+}

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "physicell_amd", "libphysicell_hip.so")
 ARCH = os.environ.get("PHYSICELL_HIP_ARCH", "gfx950")
 
-SOURCES = ["context.hip", "diffusion.hip", "sweep_part.hip", "cells.hip", "batch.hip", "motility.hip", "phenotype.hip", "host/physicell_host.cpp", "host/snapshot.cpp"]
+SOURCES = ["context.hip", "diffusion.hip", "sweep_part.hip", "cells.hip", "batch.hip", "motility.hip", "phenotype.hip", "neighbors.hip", "host/physicell_host.cpp", "host/snapshot.cpp"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          f"--offload-arch={ARCH}", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,

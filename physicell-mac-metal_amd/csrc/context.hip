@@ -53,6 +53,7 @@ int probe_kernels_cells2(hc_ctx* ctx);
 int probe_kernels_batch(hc_ctx* ctx);
 int probe_kernels_motility(hc_ctx* ctx);
 int probe_kernels_phenotype(hc_ctx* ctx);
+int probe_kernels_neighbors(hc_ctx* ctx);
 
 }  // namespace hc
 
@@ -105,6 +106,7 @@ void hc_destroy(hc_ctx* ctx) {
     for (auto e : ctx->event_pool) hipEventDestroy(e);
     if (ctx->d_scalar) hipFree(ctx->d_scalar);
     if (ctx->d_list) hipFree(ctx->d_list);
+    if (ctx->d_query) hipFree(ctx->d_query);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -125,6 +127,8 @@ int hc_load_kernels(hc_ctx* ctx, const char* code_object_or_null) {
     r = probe_kernels_motility(ctx);
     if (r) return r;
     r = probe_kernels_phenotype(ctx);
+    if (r) return r;
+    r = probe_kernels_neighbors(ctx);
     if (r) return r;
     ctx->kernels_loaded = true;
     return 0;

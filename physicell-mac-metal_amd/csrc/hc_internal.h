@@ -36,6 +36,9 @@ struct hc_ctx {
     // device copy of host index lists (division parents, removal moves)
     void* d_list = nullptr;
     size_t d_list_bytes = 0;
+    // neighbour queries: two maxima and the scan's partials
+    void* d_query = nullptr;
+    size_t d_query_bytes = 0;
 };
 
 struct hc_buf {

@@ -334,6 +334,36 @@ uint32_t CellMechanics::maxVoxelOccupancy() {
     return m;
 }
 
+uint64_t CellMechanics::neighborQuery(CellData& cells, int kind, uint32_t* offsets, uint32_t* idx,
+                                      uint64_t capacity) {
+    const uint32_t n = cells.num_cells;
+    check(ctx_, hc_build_hash(ctx_, cells.gpu_buffer(), hash_.get(), &params_, &mgrid_, n, cells.max_cells,
+                              total_mech_voxels_),
+          "neighbour query: hash");
+    if (!q_off_.get() || q_off_.size() < ((size_t)cells.max_cells + 1) * 4)
+        q_off_.allocate(ctx_, ((size_t)cells.max_cells + 1) * 4);
+    uint64_t total = 0;
+    check(ctx_, hc_neighbor_query(ctx_, cells.gpu_buffer(), hash_.get(), &params_, n, cells.max_cells,
+                                  total_mech_voxels_, kind, q_off_.get(), nullptr, &total),
+          "neighbour query: count");
+    if (!q_idx_.get() || q_idx_.size() < std::max<uint64_t>(16, total * 4))
+        q_idx_.allocate(ctx_, std::max<uint64_t>(16, total * 4));
+    check(ctx_, hc_neighbor_query(ctx_, cells.gpu_buffer(), hash_.get(), &params_, n, cells.max_cells,
+                                  total_mech_voxels_, kind, q_off_.get(), q_idx_.get(), &total),
+          "neighbour query");
+    if (offsets) {
+        q_off_.download(0, ((size_t)n + 1) * 4);
+        check(ctx_, hc_wait(ctx_), "wait");
+        std::memcpy(offsets, q_off_.host(), ((size_t)n + 1) * 4);
+    }
+    if (idx && capacity >= total && total) {
+        q_idx_.download(0, total * 4);
+        check(ctx_, hc_wait(ctx_), "wait");
+        std::memcpy(idx, q_idx_.host(), total * 4);
+    }
+    return total;
+}
+
 // ------------------------------------------------------------------ Simulation
 static hc_ctx* open_context(int device) {
     hc_ctx* ctx = nullptr;
@@ -771,6 +801,14 @@ int pc_sim_write_snapshot(pc_sim* s, const char* folder, int frame) {
         if (rc) throw std::runtime_error(std::string("pc_sim_write_snapshot: cannot write under ") + folder);
     });
     return g ? g : rc;
+}
+
+int pc_sim_neighbor_query(pc_sim* s, int kind, uint32_t* offsets, uint32_t* idx, uint64_t capacity,
+                          uint64_t* total) {
+    if (!total) return HC_ERR_ARG;
+    return guarded(s, [&](Simulation& m) {
+        *total = m.mechanics.neighborQuery(m.cells, kind, offsets, idx, capacity);
+    });
 }
 
 int pc_sim_set_volume(pc_sim* s, uint64_t seed, const double* const* fields) {

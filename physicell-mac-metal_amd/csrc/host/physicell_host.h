@@ -168,13 +168,17 @@ public:
     uint32_t maxVoxelOccupancy();
     const std::vector<uint32_t>& hostOffsets() const { return host_offsets_; }
     hc_buf* hash() const { return hash_.get(); }
+    // hash from the current positions, then hc_neighbor_query (kind
+    // HC_QUERY_*); copies out offsets[num_cells + 1] and, when idx is not null
+    // and capacity suffices, the lists.  Returns the number of entries.
+    uint64_t neighborQuery(CellData& cells, int kind, uint32_t* offsets, uint32_t* idx, uint64_t capacity);
 
 private:
     hc_ctx* ctx_ = nullptr;
     hc_mech params_{};
     hc_grid mgrid_{};
     uint32_t total_mech_voxels_ = 0, max_cells_ = 0;
-    DeviceBuffer hash_;
+    DeviceBuffer hash_, q_off_, q_idx_;
     std::vector<uint32_t> host_offsets_, host_cells_;
 };
 

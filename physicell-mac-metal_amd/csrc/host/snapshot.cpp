@@ -152,4 +152,25 @@ int pc_write_snapshot(const char* folder, int frame, const pc_snapshot* s) {
     return 0;
 }
 
+int pc_write_neighbor_graph(const char* path, uint32_t n, const uint32_t* offsets, const uint32_t* idx) {
+    if (!path || !offsets || (offsets[n] && !idx)) return HC_ERR_ARG;
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) return HC_ERR_ARG;
+    // the reference streams "i j\n" lines with std::ofstream (decimal, no padding)
+    std::vector<char> buf;
+    buf.reserve(1 << 20);
+    char line[32];
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t k = offsets[i]; k < offsets[i + 1]; k++) {
+            const int len = std::snprintf(line, sizeof(line), "%u %u\n", i, idx[k]);
+            buf.insert(buf.end(), line, line + len);
+            if (buf.size() >= (1u << 20)) {
+                std::fwrite(buf.data(), 1, buf.size(), fp);
+                buf.clear();
+            }
+        }
+    const bool ok = std::fwrite(buf.data(), 1, buf.size(), fp) == buf.size();
+    return std::fclose(fp) == 0 && ok ? 0 : HC_ERR_ARG;
+}
+
 }  // extern "C"

@@ -47,6 +47,7 @@ KERNEL_NAMES = ["sweep_x", "sweep_y", "sweep_z", "sweep_small", "dirichlet", "vo
  V_TARGET_FLUID_FRAC, V_TARGET_CN_RATIO, V_FLUID_RATE, V_NUC_RATE, V_CYTO_RATE, V_CALC_RATE, V_RUPTURE_VOL,
  V_REL_RUPTURE_VOL) = range(15)
 NUM_V_FIELDS = 15
+QUERY_CONTACT, QUERY_INTERACT = 0, 1
 
 
 class HipError(RuntimeError):
@@ -171,6 +172,9 @@ def load(path=LIB_PATH):
         "hc_motility": (c_i32, [c_vp, c_vp, c_vp, c_vp, P(Grid), c_f64, c_u32, c_u32, c_i32]),
         "pc_sim_set_motility": (c_i32, [c_vp, ctypes.c_uint64] + [c_vp] * 8),
         "pc_sim_set_volume": (c_i32, [c_vp, ctypes.c_uint64, c_vp]),
+        "pc_sim_neighbor_query": (c_i32, [c_vp, c_i32, c_vp, c_vp, ctypes.c_uint64, c_vp]),
+        "hc_neighbor_query": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_u32, c_u32, c_u32, c_i32, c_vp, c_vp, c_vp]),
+        "pc_write_neighbor_graph": (c_i32, [ctypes.c_char_p, c_u32, c_vp, c_vp]),
         "pc_sim_update_phenotype": (c_i32, [c_vp, c_f64, c_i32, c_vp, c_vp, c_u32, c_vp, c_u32, c_vp]),
         "pc_write_matlab4": (c_i32, [ctypes.c_char_p, ctypes.c_char_p, c_u32, c_u32, c_vp]),
         "pc_write_snapshot": (c_i32, [ctypes.c_char_p, c_i32, P(Snapshot)]),
@@ -438,6 +442,18 @@ class Simulation:
                                                    _ptr(rm), len(rm), ctypes.byref(out)), "pc_sim_update_phenotype")
         self.n = out.value
         return self.n
+
+    def neighbor_query(self, kind):
+        """pc_sim_neighbor_query: (offsets[n + 1], idx) of QUERY_CONTACT or
+        QUERY_INTERACT lists over the current positions."""
+        off = np.zeros(self.n + 1, np.uint32)
+        total = ctypes.c_uint64()
+        self._check(load().pc_sim_neighbor_query(self.h, kind, _ptr(off), None, 0, ctypes.byref(total)),
+                    "pc_sim_neighbor_query")
+        idx = np.zeros(max(1, total.value), np.uint32)
+        self._check(load().pc_sim_neighbor_query(self.h, kind, _ptr(off), _ptr(idx), total.value,
+                                                 ctypes.byref(total)), "pc_sim_neighbor_query")
+        return off, idx[:total.value]
 
     def density(self):
         out = np.empty(self.ns * self.spec.n_voxels)

@@ -1,0 +1,59 @@
+"""Neighbour queries on the device hash (pc_sim_neighbor_query /
+hc_neighbor_query) against the oracle on the same positions: the contact
+graph (or_contact_graph, the reference's O(N^2) loop restated) and the
+interaction candidates (or_interaction_candidates), both bit for bit as CSR
+lists, FP64 and mixed hashes, and the reach check."""
+import numpy as np
+import pytest
+
+import oracle as O
+from physicell_amd import hip as H, scenarios as S
+
+pytestmark = pytest.mark.gpu
+
+
+def sim_arrays(sim):
+    g = {k: sim.field(f) for k, f in (("x", H.F_POS_X), ("y", H.F_POS_Y), ("z", H.F_POS_Z), ("radius", H.F_RADIUS),
+                                       ("volume", H.F_TOTAL_VOL), ("rel", H.F_REL_MAX_ADH))}
+    g["alive"] = sim.field(H.F_IS_ALIVE)
+    return g
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("case", ["small3d", "config1"])
+def test_queries_match_oracle(case, mixed):
+    if case == "small3d":
+        spec, cells = S.small3d(30, 24, 20, n_cells=3000, seed=71)
+    else:
+        spec, cells = S.config1()
+    r = np.random.default_rng(72)
+    cells.alive = (r.uniform(size=cells.n) < 0.95).astype(np.uint32)
+    sim = H.Simulation(spec, cells, mixed=mixed)
+    sim.run_periods(2)
+    sim.wait()
+    print(f"neighbour queries: {case} n={cells.n} mixed={mixed}", flush=True)
+    off, idx = sim.neighbor_query(H.QUERY_CONTACT)
+    a = sim_arrays(sim)
+    woff, widx = O.contact_graph(a["x"], a["y"], a["z"], a["radius"], a["volume"], a["rel"], a["alive"])
+    assert np.array_equal(off, woff) and np.array_equal(idx, widx)
+    assert len(idx) > cells.n // 4
+    off, idx = sim.neighbor_query(H.QUERY_INTERACT)
+    keys = sim.field(H.F_MECH_VOXEL)
+    v = spec.mech_voxel
+    m = O.OrMech(spec.mnx, spec.mny, spec.mnz, 0, v, v, spec.x_min, spec.y_min, spec.z_min)
+    assert np.array_equal(keys, O.mech_keys(a["x"], a["y"], a["z"], m))
+    boff, bsrt = O.hash_sort(keys, spec.mnx * spec.mny * spec.mnz)
+    woff, widx = O.interaction_candidates(a["x"], a["y"], a["radius"], a["volume"], keys, boff, bsrt, m)
+    assert np.array_equal(off, woff) and np.array_equal(idx, widx)
+    sim.close()
+
+
+def test_contact_reach_check():
+    spec, cells = S.small3d(12, 10, 8, n_cells=200, seed=73)
+    cells.radius = np.full(cells.n, 14.0)           # 1.25 x 28 um > one 30 um voxel
+    sim = H.Simulation(spec, cells)
+    with pytest.raises(H.HipError, match="exceeds the mechanics voxel"):
+        sim.neighbor_query(H.QUERY_CONTACT)
+    off, idx = sim.neighbor_query(H.QUERY_INTERACT)    # the reference's own scan has no such limit
+    assert off[-1] == len(idx)
+    sim.close()
