@@ -271,8 +271,24 @@ def bench_rows(args):
         us, cnt = kern(k)
         out[name] = {"avg_us": us, "launches": cnt, "bytes_per_launch": b, "achieved_gbs": b / us * 1e-3,
                      "frac": b / us * 1e-3 / HBM_PEAK_GBS, "bytes_basis": basis}
+    # neighbour queries over all agents (hash from the current positions,
+    # count, scan, fill, copy-out); the second call of each is timed
+    queries = {}
+    for kind, name in ((H.QUERY_CONTACT, "contact_graph"), (H.QUERY_INTERACT, "interaction_candidates")):
+        for _ in range(2):
+            t1 = time.perf_counter()
+            qoff, qidx = sim.neighbor_query(kind)
+            ms = 1e3 * (time.perf_counter() - t1)
+        queries[name] = {"ms": ms, "entries": int(len(qidx)), "cells": int(len(qoff) - 1)}
     # the oracle beside it (one host thread), same operations and sizes
     cpu = {}
+    sample = 20000
+    a = {k: np.ascontiguousarray(sim.field(f)[:sample]) for k, f in
+         (("x", H.F_POS_X), ("y", H.F_POS_Y), ("z", H.F_POS_Z), ("r", H.F_RADIUS), ("v", H.F_TOTAL_VOL),
+          ("rel", H.F_REL_MAX_ADH))}
+    t1 = time.perf_counter()
+    O.contact_graph(a["x"], a["y"], a["z"], a["r"], a["v"], a["rel"], np.ascontiguousarray(sim.field(H.F_IS_ALIVE)[:sample]))
+    cpu["contact_graph_ms_20k_cells"] = 1e3 * (time.perf_counter() - t1)
     x, y, z = (np.ascontiguousarray(np.asarray(a, np.float64)) for a in (cells.x, cells.y, cells.z))
     vel = [np.zeros(n) for _ in range(3)]
     mv = [np.zeros(n) for _ in range(3)]
@@ -309,9 +325,11 @@ def bench_rows(args):
                                f"divisions + {n // 100} removals each", "graph": True},
         "kernels": out,
         "phenotype_call_ms": walls,
+        "neighbor_queries": queries,
         "cpu_baseline": {"kind": "port", "cores": 1, **cpu,
                          "sample": f"the FP64 oracle on this host, one thread: motility over {n} agents, volume "
-                                   f"model over {n}, {n // 100} divisions, {n // 100} removals"},
+                                   f"model over {n}, {n // 100} divisions, {n // 100} removals; the contact "
+                                   f"graph's O(N^2) loop (the reference's) over the first {sample} agents only"},
     }
     print(json.dumps(line))
 
