@@ -139,8 +139,6 @@ __global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sim
     const uint32_t c = idx - m.cell_off;
     const uint32_t lk = key - m.bin_off;
     const uint32_t vi = lk % m.mnx, vj = (lk / m.mnx) % m.mny, vk = lk / (m.mnx * m.mny);
-    NeighborRows nr;
-    neighbor_rows(offsets, m.bin_off, vi, vj, vk, m.mnx, m.mny, m.mnz, nr);
     double ofx = 0.0, ofy = 0.0, ofz = 0.0, opr = 0.0;
     if (FP32) {
         const CellRec32 me = reinterpret_cast<const CellRec32*>(recs)[t];
@@ -148,20 +146,24 @@ __global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sim
         const float rel = (float)sim_field(m, HC_F_REL_MAX_ADH)[c];
         const float cut = (float)m.max_dist, vox = (float)m.mvoxel;
         float fx = 0.f, fy = 0.f, fz = 0.f, pr = 0.f;
-#pragma unroll
-        for (int q = 0; q < 9; q++)
-            force_range32(rec, nr.beg[q], nr.end[q], t, nr.mid0[q], nr.mid1[q], me.x, me.y, me.z, (float)nr.dj[q] * vox,
-                          (float)nr.dk[q] * vox, vox, me.r, me.rep, me.adh, rel, cut * cut, fx, fy, fz, pr);
+#pragma unroll 1
+        for (int q = 0; q < 9; q++) {
+            const RowRange rr = neighbor_row(offsets, m.bin_off, vi, vj, vk, m.mnx, m.mny, m.mnz, q);
+            force_range32(rec, rr.beg, rr.end, t, rr.mid0, rr.mid1, me.x, me.y, me.z, (float)rr.dj * vox,
+                          (float)rr.dk * vox, vox, me.r, me.rep, me.adh, rel, cut * cut, fx, fy, fz, pr);
+        }
         ofx = fx; ofy = fy; ofz = fz; opr = pr;
     } else {
         const CellRec me = reinterpret_cast<const CellRec*>(recs)[t];
         const CellRec* rec = reinterpret_cast<const CellRec*>(recs);
         const double cut2 = m.max_dist * m.max_dist;
         double fx = 0.0, fy = 0.0, fz = 0.0, pr = 0.0;
-#pragma unroll
-        for (int q = 0; q < 9; q++)
-            force_range64(rec, nr.beg[q], nr.end[q], t, me.x, me.y, me.z, me.r, me.rep, me.adh, me.rel, cut2, fx, fy,
+#pragma unroll 1
+        for (int q = 0; q < 9; q++) {
+            const RowRange rr = neighbor_row(offsets, m.bin_off, vi, vj, vk, m.mnx, m.mny, m.mnz, q);
+            force_range64(rec, rr.beg, rr.end, t, me.x, me.y, me.z, me.r, me.rep, me.adh, me.rel, cut2, fx, fy,
                           fz, pr);
+        }
         ofx = fx; ofy = fy; ofz = fz; opr = pr;
     }
     sim_field(m, HC_F_VEL_X)[c] = ofx;

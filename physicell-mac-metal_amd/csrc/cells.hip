@@ -211,12 +211,12 @@ __global__ __launch_bounds__(256) void k_forces(const CellRec* __restrict__ rec,
     double fx = 0.0, fy = 0.0, fz = 0.0, local_pressure = 0.0;
     const uint32_t vi = me.key % mnx, vj = (me.key / mnx) % mny, vk = me.key / (mnx * mny);
     const double cut2 = max_dist * max_dist;
-    NeighborRows nr;
-    neighbor_rows(offsets, 0u, vi, vj, vk, mnx, mny, mnz, nr);
-#pragma unroll
-    for (int q = 0; q < 9; q++)
-        force_range64(rec, nr.beg[q], nr.end[q], t, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz,
+#pragma unroll 1
+    for (int q = 0; q < 9; q++) {
+        const RowRange rr = neighbor_row(offsets, 0u, vi, vj, vk, mnx, mny, mnz, q);
+        force_range64(rec, rr.beg, rr.end, t, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz,
                       local_pressure);
+    }
     const uint32_t c = me.idx;
     vx[c] = fx;
     vy[c] = fy;
@@ -541,13 +541,13 @@ __global__ __launch_bounds__(256) void k_forces32(const CellRec32* __restrict__ 
     float fx = 0.f, fy = 0.f, fz = 0.f, local_pressure = 0.f;
     const uint32_t vi = me.key % mnx, vj = (me.key / mnx) % mny, vk = me.key / (mnx * mny);
     const float cut2 = max_dist * max_dist;
-    NeighborRows nr;
-    neighbor_rows(offsets, 0u, vi, vj, vk, mnx, mny, mnz, nr);
-#pragma unroll
-    for (int q = 0; q < 9; q++)
-        force_range32(rec, nr.beg[q], nr.end[q], t, nr.mid0[q], nr.mid1[q], me.x, me.y, me.z, (float)nr.dj[q] * voxel,
-                      (float)nr.dk[q] * voxel, voxel, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz,
+#pragma unroll 1
+    for (int q = 0; q < 9; q++) {
+        const RowRange rr = neighbor_row(offsets, 0u, vi, vj, vk, mnx, mny, mnz, q);
+        force_range32(rec, rr.beg, rr.end, t, rr.mid0, rr.mid1, me.x, me.y, me.z, (float)rr.dj * voxel,
+                      (float)rr.dk * voxel, voxel, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz,
                       local_pressure);
+    }
     const uint32_t c = me.idx;
     vx[c] = fx;
     vy[c] = fy;

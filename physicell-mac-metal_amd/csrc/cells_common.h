@@ -142,29 +142,30 @@ struct PlanHeader {
 // hash-sorted order, in the reference's visit order (dk, dj, di) x ascending
 // index (shaders/mechanics.metal:151-203).  All bin offsets are loaded up
 // front (independent loads).  bin0 is the key of the simulation's bin 0.
-struct NeighborRows {
-    uint32_t beg[9], end[9];
-    uint32_t mid0[9], mid1[9];  // first member of the agent's column and of the next one
-    int dk[9], dj[9];
+// One of the 9 neighbour rows (q = (dk+1)*3 + (dj+1); 3 in 2-D): the members
+// of bins (vi-1 .. vi+1, vj+dj, vk+dk) are one contiguous range of the
+// hash-sorted order; mid0 / mid1 start the agent's column and the next.
+struct RowRange {
+    uint32_t beg, end, mid0, mid1;
+    int dk, dj;
 };
-__device__ __attribute__((always_inline)) inline void neighbor_rows(const uint32_t* __restrict__ offsets, uint32_t bin0,
-                                                                    uint32_t vi, uint32_t vj, uint32_t vk, uint32_t mnx,
-                                                                    uint32_t mny, uint32_t mnz, NeighborRows& r) {
+__device__ __attribute__((always_inline)) inline RowRange neighbor_row(const uint32_t* __restrict__ offsets,
+                                                                       uint32_t bin0, uint32_t vi, uint32_t vj,
+                                                                       uint32_t vk, uint32_t mnx, uint32_t mny,
+                                                                       uint32_t mnz, int q) {
     const uint32_t lo = vi > 0 ? vi - 1 : 0, hi = vi + 1 < mnx ? vi + 1 : mnx - 1;
     const bool three_d = mnz > 1;
-#pragma unroll
-    for (int q = 0; q < 9; q++) {
-        const int dk = three_d ? q / 3 - 1 : 0, dj = q % 3 - 1;
-        const int nk = (int)vk + dk, nj = (int)vj + dj;
-        const bool ok = (three_d || q < 3) && nk >= 0 && nk < (int)mnz && nj >= 0 && nj < (int)mny;
-        const uint32_t rb = bin0 + (ok ? (uint32_t)nk * mny * mnx + (uint32_t)nj * mnx : 0u);
-        r.beg[q] = ok ? offsets[rb + lo] : 0u;
-        r.end[q] = ok ? offsets[rb + hi + 1] : 0u;
-        r.mid0[q] = ok ? offsets[rb + vi] : 0u;
-        r.mid1[q] = ok ? offsets[rb + vi + 1] : 0u;
-        r.dk[q] = dk;
-        r.dj[q] = dj;
-    }
+    RowRange r;
+    r.dk = three_d ? q / 3 - 1 : 0;
+    r.dj = q % 3 - 1;
+    const int nk = (int)vk + r.dk, nj = (int)vj + r.dj;
+    const bool ok = (three_d || q < 3) && nk >= 0 && nk < (int)mnz && nj >= 0 && nj < (int)mny;
+    const uint32_t rb = bin0 + (ok ? (uint32_t)nk * mny * mnx + (uint32_t)nj * mnx : 0u);
+    r.beg = ok ? offsets[rb + lo] : 0u;
+    r.end = ok ? offsets[rb + hi + 1] : 0u;
+    r.mid0 = ok ? offsets[rb + vi] : 0u;
+    r.mid1 = ok ? offsets[rb + vi + 1] : 0u;
+    return r;
 }
 
 // Pair forces of one agent against the hash-sorted members [beg, end), in
