@@ -20,10 +20,12 @@ def sim_arrays(sim):
 
 
 @pytest.mark.parametrize("mixed", [False, True])
-@pytest.mark.parametrize("case", ["small3d", "config1"])
+@pytest.mark.parametrize("case", ["small3d", "config1", "config0"])
 def test_queries_match_oracle(case, mixed):
     if case == "small3d":
         spec, cells = S.small3d(30, 24, 20, n_cells=3000, seed=71)
+    elif case == "config0":
+        spec, cells = S.config0()            # 2-D: 9 bins around each cell
     else:
         spec, cells = S.config1()
     r = np.random.default_rng(72)
@@ -36,7 +38,7 @@ def test_queries_match_oracle(case, mixed):
     a = sim_arrays(sim)
     woff, widx = O.contact_graph(a["x"], a["y"], a["z"], a["radius"], a["volume"], a["rel"], a["alive"])
     assert np.array_equal(off, woff) and np.array_equal(idx, widx)
-    assert len(idx) > cells.n // 4
+    assert len(idx) > cells.n // 4 or case == "config0"
     off, idx = sim.neighbor_query(H.QUERY_INTERACT)
     keys = sim.field(H.F_MECH_VOXEL)
     v = spec.mech_voxel
@@ -56,4 +58,18 @@ def test_contact_reach_check():
         sim.neighbor_query(H.QUERY_CONTACT)
     off, idx = sim.neighbor_query(H.QUERY_INTERACT)    # the reference's own scan has no such limit
     assert off[-1] == len(idx)
+    sim.close()
+
+
+def test_empty_population():
+    """No cells: empty lists, and the phenotype step and queries accept it."""
+    spec, cells = S.small3d(12, 10, 8, n_cells=0, seed=74)
+    sim = H.Simulation(spec, cells, max_cells=16)
+    for kind in (H.QUERY_CONTACT, H.QUERY_INTERACT):
+        off, idx = sim.neighbor_query(kind)
+        assert off.tolist() == [0] and len(idx) == 0
+    sim.set_volume(seed=1)
+    assert sim.update_phenotype(6.0) == 0
+    sim.run_periods(1)
+    sim.wait()
     sim.close()
