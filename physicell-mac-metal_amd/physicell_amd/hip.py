@@ -498,6 +498,10 @@ class Simulation:
             raise HipError("hc_timing_get failed")
         return ms.value, n.value
 
+    def timing_reset(self):
+        if load().hc_timing_reset(self.context()):
+            raise HipError("hc_timing_reset failed")
+
     def close(self):
         if self.h:
             load().pc_sim_destroy(self.h)
