@@ -358,7 +358,7 @@ def main():
                     help="0 partitioned sweeps (default), 1 serial elimination, 2 partitioned with LDS-staged x")
     ap.add_argument("--substep-order", type=int, default=0,
                     help="0 default (axis-major), 1 axis-major, 2 substrate-major sweeps inside a substep")
-    ap.add_argument("--cpu-periods", type=int, default=3)
+    ap.add_argument("--cpu-periods", type=int, default=12, help="periods in the CPU sample (about 9 s on 16 host threads)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_config2.json"))
     ap.add_argument("--members", type=int, default=64, help="ensemble size (workload config3)")
