@@ -581,9 +581,11 @@ public:
                                           members_[i]->density.size(), hipMemcpyDeviceToDevice, st);
             if (e != hipSuccess) throw std::runtime_error(std::string("autotune save: ") + hipGetErrorString(e));
         }
-        hipEvent_t a, b;
-        hipEventCreate(&a);
-        hipEventCreate(&b);
+        hipEvent_t a = nullptr, b = nullptr;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+            if (a) (void)hipEventDestroy(a);
+            throw std::runtime_error("autotune: hipEventCreate failed");
+        }
         const unsigned cand[] = {0, 256, 512, 1024, 2048, 4096, 8192, 16384};
         Workspace& w = ws_;
         bool counted[3] = {false, false, false};
@@ -599,12 +601,17 @@ public:
             for (unsigned c : cand) {
                 if (c > 2 * w.nwork[li]) break;
                 sweep_list(li, c);
-                hipEventRecord(a, st);
+                hipError_t e = hipEventRecord(a, st);
                 for (int r = 0; r < reps; r++) sweep_list(li, c);
-                hipEventRecord(b, st);
-                hipEventSynchronize(b);
                 float ms = 0.f;
-                hipEventElapsedTime(&ms, a, b);
+                if (e == hipSuccess) e = hipEventRecord(b, st);
+                if (e == hipSuccess) e = hipEventSynchronize(b);
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+                if (e != hipSuccess) {
+                    (void)hipEventDestroy(a);
+                    (void)hipEventDestroy(b);
+                    throw std::runtime_error(std::string("autotune timing: ") + hipGetErrorString(e));
+                }
                 const double us = 1e3 * ms / std::max(1, reps);
                 if (us < best) {
                     best = us;
@@ -621,8 +628,8 @@ public:
             us_out[ax] += best;
             counted[ax] = true;
         }
-        hipEventDestroy(a);
-        hipEventDestroy(b);
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
         for (size_t i = 0; i < members_.size(); i++) {
             hipError_t e = hipMemcpyAsync(hc_device_ptr(members_[i]->density.get()), hc_device_ptr(saved[i].get()),
                                           members_[i]->density.size(), hipMemcpyDeviceToDevice, st);
@@ -674,7 +681,7 @@ private:
 
     void drop_graph() {
         if (graph_) {
-            hipGraphExecDestroy((hipGraphExec_t)graph_);
+            (void)hipGraphExecDestroy((hipGraphExec_t)graph_);
             graph_ = nullptr;
         }
     }
@@ -893,9 +900,9 @@ private:
     }
 
     void release_streams() {
-        if (join_) hipEventDestroy(join_);
-        if (fork_) hipEventDestroy(fork_);
-        if (side_) hipStreamDestroy(side_);
+        if (join_) (void)hipEventDestroy(join_);
+        if (fork_) (void)hipEventDestroy(fork_);
+        if (side_) (void)hipStreamDestroy(side_);
         join_ = fork_ = nullptr;
         side_ = nullptr;
     }
@@ -950,15 +957,15 @@ private:
         try {
             period_kernels();
         } catch (...) {
-            hipStreamEndCapture(st, &graph);
-            if (graph) hipGraphDestroy(graph);
+            (void)hipStreamEndCapture(st, &graph);    // already failing: end the capture, rethrow
+            if (graph) (void)hipGraphDestroy(graph);
             throw;
         }
         e = hipStreamEndCapture(st, &graph);
         if (e != hipSuccess) throw std::runtime_error(std::string("ensemble end capture: ") + hipGetErrorString(e));
         hipGraphExec_t exec = nullptr;
         e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         if (e != hipSuccess) throw std::runtime_error(std::string("ensemble instantiate: ") + hipGetErrorString(e));
         graph_ = (void*)exec;
     }

@@ -911,8 +911,8 @@ int hc_substrate_totals(hc_ctx* ctx, hc_buf* density, const hc_grid* g, double* 
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) r = hip_fail(ctx, e, "hc_substrate_totals");
     }
-    hipStreamSynchronize(ctx->stream);
-    hipFree(scratch);
+    (void)hipStreamSynchronize(ctx->stream);        // cleanup: r holds the first failure
+    (void)hipFree(scratch);
     return r;
 }
 

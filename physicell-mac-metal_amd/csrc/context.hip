@@ -36,14 +36,23 @@ KernelTimer::KernelTimer(hc_ctx* c, int k) : ctx(c), kernel(k) {
     if (ctx && (ctx->timing_mask >> k) & 1u) {
         a = take_event(ctx);
         b = take_event(ctx);
-        if (a && b) hipEventRecord(a, ctx->stream);
+        // a launch that cannot be timed is simply not timed
+        if (!a || !b || hipEventRecord(a, ctx->stream) != hipSuccess) {
+            if (a) ctx->event_pool.push_back(a);
+            if (b) ctx->event_pool.push_back(b);
+            a = b = nullptr;
+        }
     }
 }
 
 KernelTimer::~KernelTimer() {
     if (a && b) {
-        hipEventRecord(b, ctx->stream);
-        ctx->pending.push_back({kernel, a, b});
+        if (hipEventRecord(b, ctx->stream) == hipSuccess) {
+            ctx->pending.push_back({kernel, a, b});
+        } else {
+            ctx->event_pool.push_back(a);
+            ctx->event_pool.push_back(b);
+        }
     }
 }
 
@@ -100,14 +109,15 @@ int hc_create(hc_ctx** out, int device_ordinal) {
 
 void hc_destroy(hc_ctx* ctx) {
     if (!ctx) return;
-    hipSetDevice(ctx->device);
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    for (auto& p : ctx->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
-    for (auto e : ctx->event_pool) hipEventDestroy(e);
-    if (ctx->d_scalar) hipFree(ctx->d_scalar);
-    if (ctx->d_list) hipFree(ctx->d_list);
-    if (ctx->d_query) hipFree(ctx->d_query);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    // teardown: nothing to report to, so failures are ignored
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& p : ctx->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->d_scalar) (void)hipFree(ctx->d_scalar);
+    if (ctx->d_list) (void)hipFree(ctx->d_list);
+    if (ctx->d_query) (void)hipFree(ctx->d_query);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
@@ -115,7 +125,8 @@ int hc_load_kernels(hc_ctx* ctx, const char* code_object_or_null) {
     if (!ctx) return HC_ERR_ARG;
     if (code_object_or_null)
         return fail(ctx, HC_ERR_ARG, "hc_load_kernels: external code objects are not supported; pass NULL");
-    hipSetDevice(ctx->device);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: hipSetDevice");
     int r = probe_kernels_diffusion(ctx);
     if (r) return r;
     r = probe_kernels_cells(ctx);
@@ -137,7 +148,8 @@ int hc_load_kernels(hc_ctx* ctx, const char* code_object_or_null) {
 int hc_alloc(hc_ctx* ctx, size_t bytes, hc_buf** out) {
     if (!ctx || !out) return HC_ERR_ARG;
     *out = nullptr;
-    hipSetDevice(ctx->device);
+    hipError_t e0 = hipSetDevice(ctx->device);
+    if (e0 != hipSuccess) return hip_fail(ctx, e0, "hc_alloc: hipSetDevice");
     hc_buf* b = new hc_buf();
     b->ctx = ctx;
     b->bytes = bytes;
@@ -151,7 +163,7 @@ int hc_alloc(hc_ctx* ctx, size_t bytes, hc_buf** out) {
         e = hipMemsetAsync(b->dptr, 0, bytes, ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) {
-            hipFree(b->dptr);
+            (void)hipFree(b->dptr);
             delete b;
             return hip_fail(ctx, e, "hc_alloc: zero fill");
         }
@@ -163,11 +175,12 @@ int hc_alloc(hc_ctx* ctx, size_t bytes, hc_buf** out) {
 void hc_free(hc_buf* b) {
     if (!b) return;
     if (b->ctx) {
-        hipSetDevice(b->ctx->device);
-        hipStreamSynchronize(b->ctx->stream);
+        // release: queued work may still use the buffer; failures have no caller to go to
+        (void)hipSetDevice(b->ctx->device);
+        (void)hipStreamSynchronize(b->ctx->stream);
     }
-    if (b->dptr) hipFree(b->dptr);
-    if (b->host) hipHostFree(b->host);
+    if (b->dptr) (void)hipFree(b->dptr);
+    if (b->host) (void)hipHostFree(b->host);
     delete b;
 }
 

@@ -465,8 +465,8 @@ int hc_dirichlet(hc_ctx* ctx, hc_buf* density, const hc_grid* g, const hc_substr
     e = hipMemcpyAsync(d, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream);
     int rr = e == hipSuccess ? launch_dirichlet(ctx, density->dptr, g, (const uint8_t*)d)
                              : hip_fail(ctx, e, "hc_dirichlet: upload");
-    hipStreamSynchronize(ctx->stream);
-    hipFree(d);
+    (void)hipStreamSynchronize(ctx->stream);        // cleanup: rr holds the first failure
+    (void)hipFree(d);
     return rr;
 }
 

@@ -391,11 +391,11 @@ Simulation::Simulation(const SimConfig& cfg, int device)
 }
 
 Simulation::~Simulation() {
-    if (graph_exec_) hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
+    if (graph_exec_) (void)hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
     hc_wait(ctx);
-    if (join_) hipEventDestroy((hipEvent_t)join_);
-    if (fork_) hipEventDestroy((hipEvent_t)fork_);
-    if (side_) hipStreamDestroy((hipStream_t)side_);
+    if (join_) (void)hipEventDestroy((hipEvent_t)join_);
+    if (fork_) (void)hipEventDestroy((hipEvent_t)fork_);
+    if (side_) (void)hipStreamDestroy((hipStream_t)side_);
     // buffers are released by their owners before the context goes
     cells = CellData(1);
     microenv = Microenvironment();
@@ -437,7 +437,7 @@ void Simulation::setCells(uint32_t n, const double* x, const double* y, const do
     cells.syncToDevice();
     microenv.planSecretion(cells, config.dt_diffusion);
     if (graph_exec_) {
-        hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
+        (void)hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
         graph_exec_ = nullptr;
     }
 }
@@ -490,7 +490,7 @@ void Simulation::setMotility(uint64_t seed, const double* persistence, const dou
     motility_use_sensitivity_ = sensitivity ? 1 : 0;
     motility_on_ = true;
     if (graph_exec_) {
-        hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
+        (void)hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
         graph_exec_ = nullptr;
     }
 }
@@ -543,7 +543,7 @@ uint32_t Simulation::updatePhenotypeSoA(double dt, int use_ratio, const uint32_t
     check(ctx, hc_wait(ctx), "wait");
     phenotypeStep();
     if (graph_exec_) {
-        hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
+        (void)hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
         graph_exec_ = nullptr;
     }
     return n2;
@@ -593,7 +593,7 @@ int Simulation::substepsPerMechanicsStep() const {
 void Simulation::setOverlap(bool on) {
     overlap_ = on;
     if (graph_exec_) {
-        hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
+        (void)hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
         graph_exec_ = nullptr;
     }
 }
@@ -653,15 +653,15 @@ void Simulation::runPeriods(int periods, bool use_graph) {
         try {
             periodBody();
         } catch (...) {
-            hipStreamEndCapture(stream, &graph);
-            if (graph) hipGraphDestroy(graph);
+            (void)hipStreamEndCapture(stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
             throw;
         }
         e = hipStreamEndCapture(stream, &graph);
         if (e != hipSuccess) throw std::runtime_error(std::string("graph end capture: ") + hipGetErrorString(e));
         hipGraphExec_t exec = nullptr;
         e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         if (e != hipSuccess) throw std::runtime_error(std::string("graph instantiate: ") + hipGetErrorString(e));
         graph_exec_ = (void*)exec;
     }

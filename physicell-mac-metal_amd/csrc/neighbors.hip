@@ -168,7 +168,7 @@ int hc_neighbor_query(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* m
     if (ctx->d_query_bytes < need) {
         hipError_t e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) return hip_fail(ctx, e, "hc_neighbor_query: wait");
-        if (ctx->d_query) hipFree(ctx->d_query);
+        if (ctx->d_query) (void)hipFree(ctx->d_query);
         ctx->d_query = nullptr;
         ctx->d_query_bytes = 0;
         if ((e = hipMalloc(&ctx->d_query, need)) != hipSuccess) return hip_fail(ctx, e, "hc_neighbor_query: scratch");

@@ -202,7 +202,7 @@ static int upload_list(hc_ctx* ctx, const uint32_t* host, size_t count, uint32_t
         if (ctx->d_list) {
             hipError_t e = hipStreamSynchronize(ctx->stream);
             if (e != hipSuccess) return hip_fail(ctx, e, "list scratch: wait");
-            hipFree(ctx->d_list);
+            (void)hipFree(ctx->d_list);
             ctx->d_list = nullptr;
             ctx->d_list_bytes = 0;
         }
