@@ -80,7 +80,7 @@ template <bool FP32>
 __global__ __launch_bounds__(256) void k_b_gather(const EnsSim* __restrict__ sims, const AgentTile* __restrict__ tiles,
                                                   const uint32_t* __restrict__ kglob, const uint32_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ tmp, uint32_t* __restrict__ sorted,
-                                                  void* __restrict__ recs) {
+                                                  void* __restrict__ recs, float4* __restrict__ pos32) {
     const AgentTile tl = tiles[blockIdx.x];
     const EnsSim m = sims[tl.sim];                       // one member per workgroup
     const uint32_t c = tl.c0 + threadIdx.x;
@@ -105,6 +105,12 @@ __global__ __launch_bounds__(256) void k_b_gather(const EnsSim* __restrict__ sim
         r.key = b;
         reinterpret_cast<CellRec32*>(recs)[pos] = r;
     } else {
+        if (pos32) {
+            const uint32_t lb = b - m.bin_off;
+            const uint32_t bi = lb % m.mnx, bj = (lb / m.mnx) % m.mny, bk = lb / (m.mnx * m.mny);
+            pos32[pos] = make_float4((float)(px - (m.x_min + bi * m.mvoxel)), (float)(py - (m.y_min + bj * m.mvoxel)),
+                                     (float)(pz - (m.z_min + bk * m.mvoxel)), (float)sim_field(m, HC_F_RADIUS)[c]);
+        }
         CellRec r;
         r.x = px;
         r.y = py;
@@ -121,8 +127,8 @@ __global__ __launch_bounds__(256) void k_b_gather(const EnsSim* __restrict__ sim
 
 template <bool FP32>
 __global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sims, const uint32_t* __restrict__ agent_sim,
-                                                  const void* __restrict__ recs, const uint32_t* __restrict__ offsets,
-                                                  uint32_t ntot) {
+                                                  const void* __restrict__ recs, const float4* __restrict__ pos32,
+                                                  const uint32_t* __restrict__ offsets, uint32_t ntot) {
     const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (t >= ntot) return;
     uint32_t idx, key;
@@ -158,11 +164,21 @@ __global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sim
         const CellRec* rec = reinterpret_cast<const CellRec*>(recs);
         const double cut2 = m.max_dist * m.max_dist;
         double fx = 0.0, fy = 0.0, fz = 0.0, pr = 0.0;
+        // FP32 mark pass where its error bound holds (force_range64m)
+        const float vox = (float)m.mvoxel, near = 2.0f * vox;
+        const float4 m32 = pos32 ? pos32[t] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const bool mark32 = pos32 && m.mvoxel > 0.0 && m.mvoxel <= 32.0 && m.max_dist <= 32.0 &&
+                            fabsf(m32.x) < near && fabsf(m32.y) < near && fabsf(m32.z) < near;
 #pragma unroll 1
         for (int q = 0; q < 9; q++) {
             const RowRange rr = neighbor_row(offsets, m.bin_off, vi, vj, vk, m.mnx, m.mny, m.mnz, q);
-            force_range64(rec, rr.beg, rr.end, t, me.x, me.y, me.z, me.r, me.rep, me.adh, me.rel, cut2, fx, fy,
-                          fz, pr);
+            if (mark32)
+                force_range64m(rec, pos32, rr.beg, rr.end, t, rr.mid0, rr.mid1, m32.x, m32.y, m32.z,
+                               (float)rr.dj * vox, (float)rr.dk * vox, vox, me.x, me.y, me.z, me.r, me.rep, me.adh,
+                               me.rel, cut2, fx, fy, fz, pr);
+            else
+                force_range64(rec, rr.beg, rr.end, t, me.x, me.y, me.z, me.r, me.rep, me.adh, me.rel, cut2, fx,
+                              fy, fz, pr);
         }
         ofx = fx; ofy = fy; ofz = fz; opr = pr;
     }
@@ -641,7 +657,8 @@ public:
 
 private:
     struct Workspace {
-        DeviceBuffer sims, bsims, tiles, work[NLIST], agent_sim, kglob, slot, tmp, sorted, recs, counts, offsets, partial;
+        DeviceBuffer sims, bsims, tiles, work[NLIST], agent_sim, kglob, slot, tmp, sorted, recs, pos32, counts, offsets,
+            partial;
         DeviceBuffer pcounts, poffsets, pkv, pslot, ptmp, psorted, pflag, pgsim, pgvox, pgbegin, pcoef, phdr, ppartial;
         uint32_t ntot = 0, nbins = 0, vtot = 0, ntiles = 0, nwork[NLIST] = {}, ns_max = 0, nmax_all = 0;
         uint32_t max_line[NLIST] = {};
@@ -769,6 +786,7 @@ private:
         const size_t N = std::max<uint32_t>(1, w.ntot);
         alloc(w.kglob, N * 4); alloc(w.slot, N * 4); alloc(w.tmp, N * 4); alloc(w.sorted, N * 4);
         alloc(w.recs, N * (mixed_ ? sizeof(hc::CellRec32) : sizeof(hc::CellRec)));
+        if (!mixed_) alloc(w.pos32, N * sizeof(float4));
         alloc(w.counts, ((size_t)w.nbins + 1) * 4); alloc(w.offsets, ((size_t)w.nbins + 2) * 4);
         alloc(w.partial, hc::scan_partial_bytes((uint64_t)w.nbins + 1));
         alloc(w.pcounts, ((size_t)w.vtot + 2) * 4); alloc(w.poffsets, ((size_t)w.vtot + 3) * 4);
@@ -868,6 +886,7 @@ private:
         uint32_t* sorted = (uint32_t*)hc_device_ptr(w.sorted.get());
         const uint32_t* asim = (const uint32_t*)hc_device_ptr(w.agent_sim.get());
         void* recs = hc_device_ptr(w.recs.get());
+        float4* pos32 = mixed_ ? nullptr : (float4*)hc_device_ptr(w.pos32.get());
         {
             hc::KernelTimer tm(ctx_, HC_K_HASH);
             check(ctx_, hc::fill_u32(ctx_, counts, w.nbins, 0), "fill");
@@ -877,19 +896,21 @@ private:
             check(ctx_, hc::launch_scatter(ctx_, kglob, slot, offsets, tmp, w.ntot), "scatter");
             if (mixed_)
                 hipLaunchKernelGGL(hc::k_b_gather<true>, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, kglob, offsets, tmp,
-                                   sorted, recs);
+                                   sorted, recs, nullptr);
             else
                 hipLaunchKernelGGL(hc::k_b_gather<false>, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, kglob, offsets,
-                                   tmp, sorted, recs);
+                                   tmp, sorted, recs, pos32);
             check(ctx_, hc::check_launch(ctx_, "k_b_gather"), "gather");
         }
         {
             hc::KernelTimer tm(ctx_, HC_K_FORCES);
             const dim3 g(hc::blocks_for(w.ntot, 256));
             if (mixed_)
-                hipLaunchKernelGGL(hc::k_b_forces<true>, g, dim3(256), 0, st, sims, asim, recs, offsets, w.ntot);
+                hipLaunchKernelGGL(hc::k_b_forces<true>, g, dim3(256), 0, st, sims, asim, recs, nullptr, offsets,
+                                   w.ntot);
             else
-                hipLaunchKernelGGL(hc::k_b_forces<false>, g, dim3(256), 0, st, sims, asim, recs, offsets, w.ntot);
+                hipLaunchKernelGGL(hc::k_b_forces<false>, g, dim3(256), 0, st, sims, asim, recs, pos32, offsets,
+                                   w.ntot);
             check(ctx_, hc::check_launch(ctx_, "k_b_forces"), "forces");
         }
         {

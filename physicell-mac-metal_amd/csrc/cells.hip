@@ -174,13 +174,21 @@ __global__ __launch_bounds__(256) void k_stable_gather(CellPtrs cp, const uint32
                                                        const uint32_t* __restrict__ offsets,
                                                        const uint32_t* __restrict__ tmp,
                                                        uint32_t* __restrict__ sorted, CellRec* __restrict__ rec,
-                                                       uint32_t n) {
+                                                       float4* __restrict__ pos32, uint32_t n, double x_min,
+                                                       double y_min, double z_min, double voxel, uint32_t mnx,
+                                                       uint32_t mny) {
     const uint32_t c = blockIdx.x * 256 + threadIdx.x;
     if (c >= n) return;
     const uint32_t b = key[c];
     const uint32_t beg = offsets[b], end = offsets[b + 1];
     const uint32_t pos = beg + stable_rank(tmp, beg, end, c);
     sorted[pos] = c;
+    if (pos32) {
+        const uint32_t bi = b % mnx, bj = (b / mnx) % mny, bk = b / (mnx * mny);
+        pos32[pos] = make_float4((float)(cp.f[HC_F_POS_X][c] - (x_min + bi * voxel)),
+                                 (float)(cp.f[HC_F_POS_Y][c] - (y_min + bj * voxel)),
+                                 (float)(cp.f[HC_F_POS_Z][c] - (z_min + bk * voxel)), (float)cp.f[HC_F_RADIUS][c]);
+    }
     CellRec r;
     r.x = cp.f[HC_F_POS_X][c];
     r.y = cp.f[HC_F_POS_Y][c];
@@ -198,11 +206,15 @@ __global__ __launch_bounds__(256) void k_stable_gather(CellPtrs cp, const uint32
 // forces (shaders/mechanics.metal:91-216)
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_forces(const CellRec* __restrict__ rec, const uint32_t* __restrict__ offsets,
+// pos32 is null unless the bin size and the cutoff allow the FP32 mark pass
+// (force_range64m); agents more than two bins from their bin corner take the
+// FP64 mark pass either way.
+__global__ __launch_bounds__(256) void k_forces(const CellRec* __restrict__ rec, const float4* __restrict__ pos32,
+                                                const uint32_t* __restrict__ offsets,
                                                 double* __restrict__ vx, double* __restrict__ vy,
                                                 double* __restrict__ vz, double* __restrict__ pressure,
                                                 uint32_t n, uint32_t mnx, uint32_t mny, uint32_t mnz,
-                                                double max_dist) {
+                                                double max_dist, float voxel) {
     const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (t >= n) return;
     const CellRec me = rec[t];
@@ -211,11 +223,19 @@ __global__ __launch_bounds__(256) void k_forces(const CellRec* __restrict__ rec,
     double fx = 0.0, fy = 0.0, fz = 0.0, local_pressure = 0.0;
     const uint32_t vi = me.key % mnx, vj = (me.key / mnx) % mny, vk = me.key / (mnx * mny);
     const double cut2 = max_dist * max_dist;
+    const float4 m = pos32 ? pos32[t] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float near = 2.0f * voxel;
+    const bool mark32 = pos32 && fabsf(m.x) < near && fabsf(m.y) < near && fabsf(m.z) < near;
 #pragma unroll 1
     for (int q = 0; q < 9; q++) {
         const RowRange rr = neighbor_row(offsets, 0u, vi, vj, vk, mnx, mny, mnz, q);
-        force_range64(rec, rr.beg, rr.end, t, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz,
-                      local_pressure);
+        if (mark32)
+            force_range64m(rec, pos32, rr.beg, rr.end, t, rr.mid0, rr.mid1, m.x, m.y, m.z, (float)rr.dj * voxel,
+                           (float)rr.dk * voxel, voxel, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx,
+                           fy, fz, local_pressure);
+        else
+            force_range64(rec, rr.beg, rr.end, t, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy,
+                          fz, local_pressure);
     }
     const uint32_t c = me.idx;
     vx[c] = fx;
@@ -421,6 +441,7 @@ HashLayout hash_layout(uint32_t n_mvox, uint32_t max_cells) {
     L.tmp = o;     o = align256(o + (size_t)max_cells * 4);
     L.sorted = o;  o = align256(o + (size_t)max_cells * 4);
     L.rec = o;     o = align256(o + (size_t)max_cells * sizeof(CellRec));
+    L.pos32 = o;   o = align256(o + (size_t)max_cells * sizeof(float4));
     L.partial = o; o = align256(o + scan_partial_bytes((uint64_t)n_mvox + 1));
     L.total = o;
     return L;
@@ -635,6 +656,7 @@ int launch_sec_flags(hc_ctx* ctx, const uint32_t* key, const uint32_t* sorted, c
 struct HashPtrs {
     uint32_t *counts, *offsets, *slot, *tmp, *sorted, *partial;
     CellRec* rec;
+    float4* pos32;
 };
 
 static HashPtrs hash_ptrs(hc_buf* hash, uint32_t n_mvox, uint32_t max_cells) {
@@ -648,6 +670,7 @@ static HashPtrs hash_ptrs(hc_buf* hash, uint32_t n_mvox, uint32_t max_cells) {
     h.sorted = (uint32_t*)(P + L.sorted);
     h.partial = (uint32_t*)(P + L.partial);
     h.rec = (CellRec*)(P + L.rec);
+    h.pos32 = (float4*)(P + L.pos32);
     return h;
 }
 
@@ -748,6 +771,13 @@ int hc_secretion(hc_ctx* ctx, hc_buf* density, hc_buf* cells, hc_buf* rates, con
 
 size_t hc_hash_bytes(uint32_t n_mvox, uint32_t max_cells) { return hash_layout(n_mvox, max_cells).total; }
 
+// The FP32 mark pass of the FP64 force kernel is exact for bin sizes and
+// cutoffs up to 32 um (the error bound in force_range64m).
+static bool mark32_ok(const hc_mech* mech) {
+    return mech->mechanics_voxel_size > 0.0 && mech->mechanics_voxel_size <= 32.0 &&
+           mech->max_interaction_distance <= 32.0;
+}
+
 static int check_hash(hc_ctx* ctx, hc_buf* hash, uint32_t n_mvox, uint32_t max_cells, const char* who) {
     if (!hash || n_mvox == 0) return fail(ctx, HC_ERR_ARG, std::string(who) + ": bad hash argument");
     if (hc_buf_size(hash) < hc_hash_bytes(n_mvox, max_cells))
@@ -784,7 +814,8 @@ int hc_build_hash(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech,
                                mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny);
         else
             hipLaunchKernelGGL(k_stable_gather, dim3(nb), dim3(256), 0, ctx->stream, cp, key, h.offsets, h.tmp,
-                               h.sorted, h.rec, n);
+                               h.sorted, h.rec, mark32_ok(mech) ? h.pos32 : nullptr, n, mgrid->x_min, mgrid->y_min,
+                               mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny);
         if ((r = check_launch(ctx, "hash scatter"))) return r;
     }
     return 0;
@@ -806,10 +837,10 @@ int hc_forces(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech, uin
                            mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz,
                            (float)mech->max_interaction_distance, (float)mech->mechanics_voxel_size);
     else
-        hipLaunchKernelGGL(k_forces, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, h.rec, h.offsets,
-                           cp.f[HC_F_VEL_X], cp.f[HC_F_VEL_Y], cp.f[HC_F_VEL_Z], cp.f[HC_F_PRESSURE], n,
-                           mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz,
-                           mech->max_interaction_distance);
+        hipLaunchKernelGGL(k_forces, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, h.rec,
+                           mark32_ok(mech) ? h.pos32 : nullptr, h.offsets, cp.f[HC_F_VEL_X], cp.f[HC_F_VEL_Y],
+                           cp.f[HC_F_VEL_Z], cp.f[HC_F_PRESSURE], n, mech->mech_grid_nx, mech->mech_grid_ny,
+                           mech->mech_grid_nz, mech->max_interaction_distance, (float)mech->mechanics_voxel_size);
     return check_launch(ctx, "k_forces");
 }
 

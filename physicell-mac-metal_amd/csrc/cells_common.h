@@ -73,10 +73,18 @@ __device__ inline uint32_t ballot_bin(bool valid, uint32_t key, uint32_t* counts
     return base + rank;
 }
 
-// rank inside the bin = number of bin members with a smaller index
+// rank inside the bin = number of bin members with a smaller index.  The
+// members are read 8 at a time (indices clamped into the bin) so their loads
+// are in flight together instead of one latency per member.
 __device__ inline uint32_t stable_rank(const uint32_t* __restrict__ tmp, uint32_t beg, uint32_t end, uint32_t c) {
     uint32_t r = 0;
-    for (uint32_t t = beg; t < end; t++) r += tmp[t] < c ? 1u : 0u;
+    for (uint32_t t0 = beg; t0 < end; t0 += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = tmp[t0 + q < end ? t0 + q : end - 1];
+#pragma unroll
+        for (int q = 0; q < 8; q++) r += (t0 + q < end && v[q] < c) ? 1u : 0u;
+    }
     return r;
 }
 
@@ -223,6 +231,85 @@ __device__ __attribute__((always_inline)) inline void force_range64(
             const double ddy = py - xy.y;
             const double ddz = pz - zr.x;
             const double dist_sq = ddx * ddx + ddy * ddy + ddz * ddz;
+            const double dist = sqrt(dist_sq);
+            const double inv_dist = 1.0 / dist;
+            const double nxv = ddx * inv_dist;
+            const double nyv = ddy * inv_dist;
+            const double nzv = ddz * inv_dist;
+            const double R_sum = radius_i + zr.y;
+            double force_mag = 0.0;
+            if (dist < R_sum) {
+                const double overlap = 1.0 - dist / R_sum;
+                force_mag += overlap * overlap * sqrt(rep_i * rec[u].rep);
+                local_pressure += overlap * overlap / 0.027288820670331;
+            }
+            const double adhesion_distance = rel_max_adh * R_sum;
+            if (dist < adhesion_distance) {
+                const double t2 = 1.0 - dist / adhesion_distance;
+                force_mag -= t2 * t2 * sqrt(adh_i * rec[u].adh);
+            }
+            fx += force_mag * nxv;
+            fy += force_mag * nyv;
+            fz += force_mag * nzv;
+        }
+    }
+}
+
+// FP64 forces with an FP32 mark pass.  pos32 holds each hash-sorted agent's
+// position relative to the lower corner of its own bin, and its radius, in
+// FP32: 16 bytes per candidate instead of the 32 the FP64 mark pass reads,
+// and FP32 arithmetic.  The FP32 test keeps every pair the exact test keeps
+// when the bin size and the cutoff are at most 32 um and the agent lies
+// within two bins of its bin corner (always, for agents inside the grid):
+// every coordinate of a candidate truly within the cutoff is then below
+// 128 um, the FP32 differences are within 3e-5 um per axis of the exact
+// ones, d^2 within 4e-3 um^2, and the FP32 cutoff and interaction radius
+// within 2e-3 um^2; the threshold carries 0.01 um^2 of slack.  The marked
+// candidates are then tested exactly in FP64 with the expressions of
+// force_range64 before its force law, so the pairs that contribute, their
+// order and every bit of the result are those of force_range64.
+__device__ __attribute__((always_inline)) inline void force_range64m(
+    const CellRec* __restrict__ rec, const float4* __restrict__ pos32, uint32_t beg, uint32_t end, uint32_t t,
+    uint32_t mid0, uint32_t mid1, float mx, float my, float mz, float oy, float oz, float voxel, double px, double py,
+    double pz, double radius_i, double rep_i, double adh_i, double rel_max_adh, double cut2, double& fx, double& fy,
+    double& fz, double& local_pressure) {
+    const double relf = rel_max_adh > 1.0 ? rel_max_adh : 1.0;
+    constexpr double margin = 1.0 + 0x1p-40;
+    const float relf32 = (float)relf, ri32 = (float)radius_i, cut2f = (float)cut2;
+    const float oxm = -1.0f * voxel, ox0 = 0.0f * voxel, oxp = 1.0f * voxel;
+    constexpr uint32_t FB = 4;
+    for (uint32_t u0 = beg; u0 < end; u0 += 32) {
+        const uint32_t cnt = end - u0 < 32u ? end - u0 : 32u;
+        uint32_t mask = 0;
+        for (uint32_t j0 = 0; j0 < cnt; j0 += FB) {
+            float4 o[FB];
+#pragma unroll
+            for (int q = 0; q < (int)FB; q++) o[q] = pos32[u0 + (j0 + q < cnt ? j0 + q : cnt - 1)];
+#pragma unroll
+            for (int q = 0; q < (int)FB; q++) {
+                const uint32_t j = j0 + q, u = u0 + j;
+                const float ox = u < mid0 ? oxm : (u < mid1 ? ox0 : oxp);
+                const float ddx = mx - o[q].x - ox, ddy = my - o[q].y - oy, ddz = mz - o[q].z - oz;
+                const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                const float L = (ri32 + o[q].w) * relf32;
+                const float L2 = L * L * (1.0f + 0x1p-16f);
+                const float lim = (L2 < cut2f ? L2 : cut2f) + 0.01f;
+                const bool inside = j < cnt && u != t && d2 < lim;
+                mask |= inside ? (1u << j) : 0u;
+            }
+        }
+        while (mask) {
+            const uint32_t j = __builtin_ctz(mask);
+            mask &= mask - 1u;
+            const uint32_t u = u0 + j;
+            const double2 xy = *reinterpret_cast<const double2*>(&rec[u].x);
+            const double2 zr = *reinterpret_cast<const double2*>(&rec[u].z);
+            const double ddx = px - xy.x;
+            const double ddy = py - xy.y;
+            const double ddz = pz - zr.x;
+            const double dist_sq = ddx * ddx + ddy * ddy + ddz * ddz;
+            const double Lx = (radius_i + zr.y) * relf;
+            if (dist_sq >= cut2 || dist_sq < 1e-12 || !(dist_sq < Lx * Lx * margin)) continue;
             const double dist = sqrt(dist_sq);
             const double inv_dist = 1.0 / dist;
             const double nxv = ddx * inv_dist;
