@@ -746,6 +746,9 @@ private:
                 const int P = hc::part_p(ax, nn[ax], mixed_);
                 const int li = list_of(ax, P);
                 const int mm = hc::part_window(nn[ax], P);
+                if (li < 0 || mm == 0 || mm > 32)
+                    throw std::runtime_error("ensemble: a member's line length is outside the batched sweep's "
+                                             "register windows (FP64: at most 130 along y and z)");
                 if (mm == 32) b.wide |= 1u << ax;
                 const uint64_t lines = hc::sweep_lines(ax, g.nx, g.ny, g.nz);
                 const uint64_t groups = (lines + 4 * (64 / P) - 1) / (4 * (64 / P));

@@ -119,12 +119,15 @@ __device__ inline uint32_t xcd_chunk(uint32_t b, uint32_t nb) {
 // each segment k the arrays 1/den, c', alpha, beta of length
 // part_mt(nmax, P), then the two rows (left / right neighbour value) of the
 // inverse interface matrix, each of length 2P.  Blocks have a common stride.
-// Segments per line.  y and z: 4 (a wave solves 16 lines).  x lines are
-// staged through LDS, a wave's lines at a time: 4 segments (16 lines per
-// wave) while 64 lines of n values fit the staging tile (n <= 130 in FP32,
-// n <= 96 in FP64), else 16 segments (4 lines per wave).
+// Segments per line.  y and z: 4 (a wave solves 16 lines), or 8 (8 lines)
+// for FP64 lines longer than 130, whose 4-segment windows of up to 64 values
+// take 161 VGPRs (3 waves per SIMD) against about 100 (5 waves) for 8
+// segments of up to 32.  x lines are staged through LDS, a wave's lines at a
+// time: 4 segments (16 lines per wave) while 64 lines of n values fit the
+// staging tile (n <= 130 in FP32, n <= 96 in FP64), else 16 segments (4
+// lines per wave).
 __host__ __device__ inline int part_p(int axis, uint32_t n, bool fp32) {
-    if (axis != 0) return 4;
+    if (axis != 0) return (!fp32 && n > 130u) ? 8 : 4;
     return n <= (fp32 ? 130u : 96u) ? 4 : 16;
 }
 __host__ __device__ inline uint32_t part_mt(uint32_t nmax, int P) {
@@ -134,8 +137,9 @@ __host__ __device__ inline size_t part_block_p(uint32_t nmax, int P) {
     return (size_t)P * 4 * part_mt(nmax, P) + (size_t)P * 2 * 2 * P;
 }
 __host__ __device__ inline size_t part_block(uint32_t nmax) {
-    const size_t a = part_block_p(nmax, 4), b = part_block_p(nmax, 16);
-    return ((a > b ? a : b) + 31) / 32 * 32;
+    const size_t a = part_block_p(nmax, 4), b = part_block_p(nmax, 16), c = part_block_p(nmax, 8);
+    const size_t m = a > b ? a : b;
+    return ((m > c ? m : c) + 31) / 32 * 32;
 }
 __host__ __device__ inline size_t serial_table_bytes(uint32_t ns, uint32_t nmax) {
     return ((subconst_bytes(ns) + (size_t)ns * 3 * 2 * nmax * sizeof(double)) + 255) / 256 * 256;
@@ -144,7 +148,8 @@ __host__ __device__ inline size_t part_offset(uint32_t ns, uint32_t nmax) { retu
 // FP32 copies of the per-segment arrays (for FP32 fields), after the FP64 blocks
 __host__ __device__ inline size_t part_fblock(uint32_t nmax) {
     const size_t a = (size_t)4 * 4 * part_mt(nmax, 4), b = (size_t)16 * 4 * part_mt(nmax, 16);
-    return ((a > b ? a : b) + 63) / 64 * 64;
+    const size_t c = (size_t)8 * 4 * part_mt(nmax, 8), m = a > b ? a : b;
+    return ((m > c ? m : c) + 63) / 64 * 64;
 }
 __host__ __device__ inline size_t part_f_offset(uint32_t ns, uint32_t nmax) {
     return part_offset(ns, nmax) + (size_t)ns * 3 * part_block(nmax) * sizeof(double);

@@ -170,10 +170,10 @@ __device__ inline void load_part_tables(const global_t<uint8_t>* __restrict__ ta
 // tables issues them first, then its field loads, and stores the tables to
 // LDS only when the field loads are in flight: one memory latency before the
 // solve instead of two.
-template <int P, int MM, typename T>
+template <int P, int MM, typename T, int BS = 256>
 struct TableRegs {
-    static constexpr int NA = (P * 4 * MM + 255) / 256;
-    static constexpr int NG = (4 * P * P + 255) / 256;
+    static constexpr int NA = (P * 4 * MM + BS - 1) / BS;
+    static constexpr int NG = (4 * P * P + BS - 1) / BS;
     T a[NA];
     double g[NG];
 
@@ -190,12 +190,12 @@ struct TableRegs {
                    (size_t)(s * 3 + axis) * part_fblock(nmax);
 #pragma unroll
         for (int r = 0; r < NG; r++) {
-            const uint32_t q = threadIdx.x + 256u * r;
+            const uint32_t q = threadIdx.x + (uint32_t)BS * r;
             g[r] = q < (uint32_t)(4 * P * P) ? dblk[na + q] : 0.0;
         }
 #pragma unroll
         for (int r = 0; r < NA; r++) {
-            const uint32_t q = threadIdx.x + 256u * r;
+            const uint32_t q = threadIdx.x + (uint32_t)BS * r;
             const uint32_t k = q / (4 * MM), j = (q / MM) % 4, t = q % MM;
             a[r] = (q < (uint32_t)(P * 4 * MM) && t < mt) ? ablk[(k * 4 + j) * MT + t] : (T)0;
         }
@@ -204,12 +204,12 @@ struct TableRegs {
         const uint32_t MT = part_mt(nmax, P), mt = part_mt(n, P), AS = PartLds<P, T>::arr_stride(MT);
 #pragma unroll
         for (int r = 0; r < NG; r++) {
-            const uint32_t q = threadIdx.x + 256u * r;
+            const uint32_t q = threadIdx.x + (uint32_t)BS * r;
             if (q < (uint32_t)(4 * P * P)) tG[(q / (4 * P)) * PartLds<P, T>::GS + q % (4 * P)] = g[r];
         }
 #pragma unroll
         for (int r = 0; r < NA; r++) {
-            const uint32_t q = threadIdx.x + 256u * r;
+            const uint32_t q = threadIdx.x + (uint32_t)BS * r;
             const uint32_t k = q / (4 * MM), j = (q / MM) % 4, t = q % MM;
             if (q < (uint32_t)(P * 4 * MM) && t < mt) tarr[k * AS + j * MT + t] = a[r];
         }
@@ -220,18 +220,17 @@ struct NoTables {
     __device__ inline void operator()() const {}
 };
 
-// One group of 4*LPW lines (a workgroup of 4 waves) of one substrate of one
-// simulation.  rho points at that substrate's field.
+// One group of WPB*LPW lines (a workgroup of WPB waves) of one substrate of
+// one simulation.  rho points at that substrate's field.
 // xmap: lane order of the x staging tile (x_lane_order); y and z use order 0.
 // mid() runs once the group's field loads are issued and before the first
 // table read (a workgroup that stages its tables stores them and syncs there;
 // every thread of the workgroup calls it).
-template <int AXIS, int P, int MM, bool XLDS, typename T, typename Mid = NoTables>
+template <int AXIS, int P, int MM, bool XLDS, typename T, typename Mid = NoTables, int WPB = 4>
 __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict__ rho, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t MT,
                                    const T* tarr, const double* tG, double c, double T0, bool dir, int fuse_dirichlet,
                                    uint64_t grp, T* tile_base, uint32_t xmap, const Mid& mid = Mid()) {
     constexpr int LPW = 64 / P;
-    constexpr int WPB = 4;
     constexpr int RV = MM / (16 / (int)sizeof(T)) + 2;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const bool tr = XLDS && xmap;
@@ -379,11 +378,12 @@ __global__ __launch_bounds__(256) void k_sweep_part(T* __restrict__ rho, const u
     TableRegs<P, MM, T> treg;
     treg.load((const global_t<uint8_t>*)tables, ns, nmax, s, AXIS, n);
     __builtin_amdgcn_sched_barrier(0);
-    sweep_group<AXIS, P, MM, XLDS, T>(rs, nx, ny, nz, MT, tarr, tG, c, T0, dir, fuse_dirichlet, grp,
-                                      tile, XLDS ? xmap : 0u, [&]() {
-                                          treg.store(nmax, n, tG, tarr);
-                                          __syncthreads();
-                                      });
+    auto mid = [&]() {
+        treg.store(nmax, n, tG, tarr);
+        __syncthreads();
+    };
+    sweep_group<AXIS, P, MM, XLDS, T, decltype(mid)>(rs, nx, ny, nz, MT, tarr, tG, c, T0, dir, fuse_dirichlet, grp,
+                                                     tile, XLDS ? xmap : 0u, mid);
 }
 
 // ---------------------------------------------------------------------------
@@ -481,6 +481,9 @@ int probe_kernels_part(hc_ctx* ctx) {
         (const void*)k_sweep_part<1, 4, 16, false, double>, (const void*)k_sweep_part<1, 4, 32, false, double>,
         (const void*)k_sweep_part<1, 4, 64, false, double>, (const void*)k_sweep_part<2, 4, 16, false, double>,
         (const void*)k_sweep_part<2, 4, 32, false, double>, (const void*)k_sweep_part<2, 4, 64, false, double>,
+        (const void*)k_sweep_part<1, 8, 16, false, double>, (const void*)k_sweep_part<1, 8, 32, false, double>,
+        (const void*)k_sweep_part<1, 8, 64, false, double>, (const void*)k_sweep_part<2, 8, 16, false, double>,
+        (const void*)k_sweep_part<2, 8, 32, false, double>, (const void*)k_sweep_part<2, 8, 64, false, double>,
         (const void*)k_sweep_part<0, 16, 16, true, float>, (const void*)k_sweep_part<0, 16, 32, true, float>,
         (const void*)k_sweep_part<1, 4, 16, false, float>, (const void*)k_sweep_part<1, 4, 32, false, float>,
         (const void*)k_sweep_part<1, 4, 64, false, float>, (const void*)k_sweep_part<2, 4, 16, false, float>,
@@ -508,7 +511,7 @@ int probe_kernels_part(hc_ctx* ctx) {
 int part_window(uint32_t n, int P) {
     if (n < 2 + (uint32_t)P) return 0;          // every segment holds at least one unknown
     const uint32_t m = (n - 2 + P - 1) / P;
-    return m <= 16 ? 16 : (m <= 32 ? 32 : (m <= 64 && P == 4 ? 64 : 0));
+    return m <= 16 ? 16 : (m <= 32 ? 32 : (m <= 64 && (P == 4 || P == 8) ? 64 : 0));
 }
 
 template <int AXIS, int P, bool XLDS, typename T>
@@ -518,8 +521,8 @@ static void launch_part_axis(hc_ctx* ctx, int mm, dim3 grid, size_t shm, T* rho,
         hipLaunchKernelGGL((k_sweep_part<AXIS, P, 16, XLDS, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse, xmap, s0, g->n_substrates);
     else if (mm == 32)
         hipLaunchKernelGGL((k_sweep_part<AXIS, P, 32, XLDS, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse, xmap, s0, g->n_substrates);
-    else if constexpr (P == 4 && AXIS != 0)
-        hipLaunchKernelGGL((k_sweep_part<AXIS, 4, 64, false, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse, xmap, s0, g->n_substrates);
+    else if constexpr ((P == 4 || P == 8) && AXIS != 0)
+        hipLaunchKernelGGL((k_sweep_part<AXIS, P, 64, false, T>), grid, dim3(256), shm, ctx->stream, rho, tables, g->nx, g->ny, g->nz, nmax, fuse, xmap, s0, g->n_substrates);
 }
 
 // LDS bytes of a partitioned sweep launch along `axis` (0: the grid does not
@@ -531,7 +534,7 @@ static size_t part_shm(int axis, const hc_grid* g, int sweep_mode) {
     if (!part_window(n, P)) return 0;
     const uint32_t nmax = grid_nmax(g);
     const bool xlds = axis == 0 && sweep_mode != 2;
-    const size_t tab_bytes = P == 16 ? PartLds<16, T>::bytes(nmax) : PartLds<4, T>::bytes(nmax);
+    const size_t tab_bytes = P == 16 ? PartLds<16, T>::bytes(nmax) : (P == 8 ? PartLds<8, T>::bytes(nmax) : PartLds<4, T>::bytes(nmax));
     const size_t shm = tab_bytes + (xlds ? (size_t)4 * xtile_elems<T>(64 / P, n) * sizeof(T) : 0);
     return shm > 64 * 1024 ? 0 : shm;
 }
@@ -566,9 +569,11 @@ static int launch_sweep_part_t(hc_ctx* ctx, int axis, T* rho, const hc_grid* g, 
         if (xlds) launch_part_axis<0, 16, true, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap, s0);
         else launch_part_axis<0, 16, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap, s0);
     } else if (axis == 1) {
-        launch_part_axis<1, 4, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap, s0);
+        if (P == 8) launch_part_axis<1, 8, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap, s0);
+        else launch_part_axis<1, 4, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap, s0);
     } else {
-        launch_part_axis<2, 4, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap, s0);
+        if (P == 8) launch_part_axis<2, 8, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap, s0);
+        else launch_part_axis<2, 4, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap, s0);
     }
     return check_launch(ctx, "k_sweep_part");
 }

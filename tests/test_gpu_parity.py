@@ -63,6 +63,8 @@ GRIDS = [
     (34, 66, 67, (False, True)),    # segment lengths 8, 16 and 17 (window 16 / 32)
     (259, 12, 9, (True, True)),     # x: 16 segments of 17 (window 32), y/z short
     (530, 3, 12, (True, False)),    # x too long for the partitioned window -> serial
+    (6, 150, 140, (True, False)),   # FP64 y/z longer than 130: 8 segments (window 32)
+    (5, 300, 6, (False, True)),     # FP64 y: 8 segments of 37 (window 64)
 ]
 
 
