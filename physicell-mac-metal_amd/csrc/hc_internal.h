@@ -120,14 +120,14 @@ __device__ inline uint32_t xcd_chunk(uint32_t b, uint32_t nb) {
 // part_mt(nmax, P), then the two rows (left / right neighbour value) of the
 // inverse interface matrix, each of length 2P.  Blocks have a common stride.
 // Segments per line.  y and z: 4 (a wave solves 16 lines), or 8 (8 lines)
-// for FP64 lines longer than 130, whose 4-segment windows of up to 64 values
-// take 161 VGPRs (3 waves per SIMD) against about 100 (5 waves) for 8
-// segments of up to 32.  x lines are staged through LDS, a wave's lines at a
+// for lines longer than 130, whose 4-segment windows of up to 64 values take
+// 161 VGPRs in FP64 (3 waves per SIMD) and 92 in FP32 (5 waves) against 88
+// (5 waves) and 64 (8 waves) for 8 segments of up to 32.  x lines are staged through LDS, a wave's lines at a
 // time: 4 segments (16 lines per wave) while 64 lines of n values fit the
 // staging tile (n <= 130 in FP32, n <= 96 in FP64), else 16 segments (4
 // lines per wave).
 __host__ __device__ inline int part_p(int axis, uint32_t n, bool fp32) {
-    if (axis != 0) return (!fp32 && n > 130u) ? 8 : 4;
+    if (axis != 0) return n > 130u ? 8 : 4;
     return n <= (fp32 ? 130u : 96u) ? 4 : 16;
 }
 __host__ __device__ inline uint32_t part_mt(uint32_t nmax, int P) {
