@@ -354,8 +354,17 @@ __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict_
 // ---------------------------------------------------------------------------
 // one simulation: one workgroup per (line group, substrate)
 // ---------------------------------------------------------------------------
+// FP32 y / z sweeps of one simulation with windows of up to 32 values ask
+// for 8 waves per SIMD: they fit in 61-63 VGPRs without spills, against
+// 66-68 (7 waves) when the compiler chooses (mixed config-2 step -0.7%).
+// The ensemble's item kernels measured slower that way (y spills 12 bytes).
+template <int AXIS, int MM, typename T>
+constexpr int yz32_waves() {
+    return (AXIS != 0 && MM <= 32 && sizeof(T) == sizeof(float)) ? 8 : 1;
+}
+
 template <int AXIS, int P, int MM, bool XLDS, typename T>
-__global__ __launch_bounds__(256) void k_sweep_part(T* __restrict__ rho, const uint8_t* __restrict__ tables,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(yz32_waves<AXIS, MM, T>()))) void k_sweep_part(T* __restrict__ rho, const uint8_t* __restrict__ tables,
                                                     uint32_t nx, uint32_t ny, uint32_t nz, uint32_t nmax,
                                                     int fuse_dirichlet, uint32_t xmap, uint32_t s0, uint32_t ns) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
