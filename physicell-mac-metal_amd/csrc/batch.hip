@@ -80,7 +80,8 @@ template <bool FP32>
 __global__ __launch_bounds__(256) void k_b_gather(const EnsSim* __restrict__ sims, const AgentTile* __restrict__ tiles,
                                                   const uint32_t* __restrict__ kglob, const uint32_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ tmp, uint32_t* __restrict__ sorted,
-                                                  void* __restrict__ recs, float4* __restrict__ pos32) {
+                                                  const uint32_t* __restrict__ pos_of, void* __restrict__ recs,
+                                                  float4* __restrict__ pos32) {
     const AgentTile tl = tiles[blockIdx.x];
     const EnsSim m = sims[tl.sim];                       // one member per workgroup
     const uint32_t c = tl.c0 + threadIdx.x;
@@ -88,8 +89,13 @@ __global__ __launch_bounds__(256) void k_b_gather(const EnsSim* __restrict__ sim
     const uint32_t g = m.cell_off + c;
     const uint32_t b = kglob[g];
     const uint32_t beg = offsets[b], end = offsets[b + 1];
-    const uint32_t pos = beg + stable_rank(tmp, beg, end, g);
-    sorted[pos] = g;
+    uint32_t pos;
+    if (end - beg <= BIN_SORT_CAP) {
+        pos = pos_of[g];                                  // placed by k_bin_sort
+    } else {
+        pos = beg + stable_rank(tmp, beg, end, g);
+        sorted[pos] = g;
+    }
     const double px = sim_field(m, HC_F_POS_X)[c], py = sim_field(m, HC_F_POS_Y)[c], pz = sim_field(m, HC_F_POS_Z)[c];
     if (FP32) {
         const uint32_t lb = b - m.bin_off;
@@ -657,8 +663,8 @@ public:
 
 private:
     struct Workspace {
-        DeviceBuffer sims, bsims, tiles, work[NLIST], agent_sim, kglob, slot, tmp, sorted, recs, pos32, counts, offsets,
-            partial;
+        DeviceBuffer sims, bsims, tiles, work[NLIST], agent_sim, kglob, slot, tmp, sorted, pos_of, recs, pos32, counts,
+            offsets, partial;
         DeviceBuffer pcounts, poffsets, pkv, pslot, ptmp, psorted, pflag, pgsim, pgvox, pgbegin, pcoef, phdr, ppartial;
         uint32_t ntot = 0, nbins = 0, vtot = 0, ntiles = 0, nwork[NLIST] = {}, ns_max = 0, nmax_all = 0;
         uint32_t max_line[NLIST] = {};
@@ -788,6 +794,7 @@ private:
         put(w.agent_sim, agent_sim.data(), agent_sim.size() * sizeof(uint32_t));
         const size_t N = std::max<uint32_t>(1, w.ntot);
         alloc(w.kglob, N * 4); alloc(w.slot, N * 4); alloc(w.tmp, N * 4); alloc(w.sorted, N * 4);
+        alloc(w.pos_of, N * 4);
         alloc(w.recs, N * (mixed_ ? sizeof(hc::CellRec32) : sizeof(hc::CellRec)));
         if (!mixed_) alloc(w.pos32, N * sizeof(float4));
         alloc(w.counts, ((size_t)w.nbins + 1) * 4); alloc(w.offsets, ((size_t)w.nbins + 2) * 4);
@@ -887,9 +894,13 @@ private:
         uint32_t* slot = (uint32_t*)hc_device_ptr(w.slot.get());
         uint32_t* tmp = (uint32_t*)hc_device_ptr(w.tmp.get());
         uint32_t* sorted = (uint32_t*)hc_device_ptr(w.sorted.get());
+        uint32_t* pos_of = (uint32_t*)hc_device_ptr(w.pos_of.get());
         const uint32_t* asim = (const uint32_t*)hc_device_ptr(w.agent_sim.get());
         void* recs = hc_device_ptr(w.recs.get());
         float4* pos32 = mixed_ ? nullptr : (float4*)hc_device_ptr(w.pos32.get());
+        if (!sims || !tiles || !counts || !offsets || !kglob || !slot || !tmp || !sorted || !pos_of || !asim ||
+            !recs || (!mixed_ && !pos32) || !hc_device_ptr(w.partial.get()))
+            throw std::runtime_error("ensemble: mechanics workspace not allocated");
         {
             hc::KernelTimer tm(ctx_, HC_K_HASH);
             check(ctx_, hc::fill_u32(ctx_, counts, w.nbins, 0), "fill");
@@ -897,12 +908,13 @@ private:
             check(ctx_, hc::check_launch(ctx_, "k_b_mech_keys"), "keys");
             check(ctx_, hc::scan_exclusive(ctx_, counts, offsets, w.nbins, (uint32_t*)hc_device_ptr(w.partial.get())), "scan");
             check(ctx_, hc::launch_scatter(ctx_, kglob, slot, offsets, tmp, w.ntot), "scatter");
+            check(ctx_, hc::launch_bin_sort(ctx_, offsets, tmp, sorted, pos_of, w.nbins), "bin sort");
             if (mixed_)
                 hipLaunchKernelGGL(hc::k_b_gather<true>, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, kglob, offsets, tmp,
-                                   sorted, recs, nullptr);
+                                   sorted, pos_of, recs, nullptr);
             else
                 hipLaunchKernelGGL(hc::k_b_gather<false>, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, kglob, offsets,
-                                   tmp, sorted, recs, pos32);
+                                   tmp, sorted, pos_of, recs, pos32);
             check(ctx_, hc::check_launch(ctx_, "k_b_gather"), "gather");
         }
         {

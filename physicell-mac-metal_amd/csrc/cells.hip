@@ -170,19 +170,31 @@ __global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ ke
 }
 
 
+__global__ __launch_bounds__(256) void k_bin_sort(const uint32_t* __restrict__ offsets,
+                                                  const uint32_t* __restrict__ tmp, uint32_t* __restrict__ sorted,
+                                                  uint32_t* __restrict__ pos_of, uint32_t nbins) {
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b < nbins) bin_sort_one(offsets, tmp, sorted, pos_of, b);
+}
+
 __global__ __launch_bounds__(256) void k_stable_gather(CellPtrs cp, const uint32_t* __restrict__ key,
                                                        const uint32_t* __restrict__ offsets,
                                                        const uint32_t* __restrict__ tmp,
-                                                       uint32_t* __restrict__ sorted, CellRec* __restrict__ rec,
-                                                       float4* __restrict__ pos32, uint32_t n, double x_min,
-                                                       double y_min, double z_min, double voxel, uint32_t mnx,
-                                                       uint32_t mny) {
+                                                       uint32_t* __restrict__ sorted, const uint32_t* __restrict__ pos_of,
+                                                       CellRec* __restrict__ rec, float4* __restrict__ pos32,
+                                                       uint32_t n, double x_min, double y_min, double z_min,
+                                                       double voxel, uint32_t mnx, uint32_t mny) {
     const uint32_t c = blockIdx.x * 256 + threadIdx.x;
     if (c >= n) return;
     const uint32_t b = key[c];
     const uint32_t beg = offsets[b], end = offsets[b + 1];
-    const uint32_t pos = beg + stable_rank(tmp, beg, end, c);
-    sorted[pos] = c;
+    uint32_t pos;
+    if (end - beg <= BIN_SORT_CAP) {
+        pos = pos_of[c];                                  // placed by k_bin_sort
+    } else {
+        pos = beg + stable_rank(tmp, beg, end, c);
+        sorted[pos] = c;
+    }
     if (pos32) {
         const uint32_t bi = b % mnx, bj = (b / mnx) % mny, bk = b / (mnx * mny);
         pos32[pos] = make_float4((float)(cp.f[HC_F_POS_X][c] - (x_min + bi * voxel)),
@@ -441,6 +453,7 @@ HashLayout hash_layout(uint32_t n_mvox, uint32_t max_cells) {
     L.tmp = o;     o = align256(o + (size_t)max_cells * 4);
     L.sorted = o;  o = align256(o + (size_t)max_cells * 4);
     L.rec = o;     o = align256(o + (size_t)max_cells * sizeof(CellRec));
+    L.pos_of = o;  o = align256(o + (size_t)max_cells * 4);
     L.pos32 = o;   o = align256(o + (size_t)max_cells * sizeof(float4));
     L.partial = o; o = align256(o + scan_partial_bytes((uint64_t)n_mvox + 1));
     L.total = o;
@@ -527,15 +540,22 @@ __global__ __launch_bounds__(256) void k_stable_sort(const uint32_t* __restrict_
 __global__ __launch_bounds__(256) void k_stable_gather32(CellPtrs cp, const uint32_t* __restrict__ key,
                                                          const uint32_t* __restrict__ offsets,
                                                          const uint32_t* __restrict__ tmp,
-                                                         uint32_t* __restrict__ sorted, CellRec32* __restrict__ rec,
-                                                         uint32_t n, double x_min, double y_min, double z_min,
-                                                         double voxel, uint32_t mnx, uint32_t mny) {
+                                                         uint32_t* __restrict__ sorted,
+                                                         const uint32_t* __restrict__ pos_of,
+                                                         CellRec32* __restrict__ rec, uint32_t n, double x_min,
+                                                         double y_min, double z_min, double voxel, uint32_t mnx,
+                                                         uint32_t mny) {
     const uint32_t c = blockIdx.x * 256 + threadIdx.x;
     if (c >= n) return;
     const uint32_t b = key[c];
     const uint32_t beg = offsets[b], end = offsets[b + 1];
-    const uint32_t pos = beg + stable_rank(tmp, beg, end, c);
-    sorted[pos] = c;
+    uint32_t pos;
+    if (end - beg <= BIN_SORT_CAP) {
+        pos = pos_of[c];                                  // placed by k_bin_sort
+    } else {
+        pos = beg + stable_rank(tmp, beg, end, c);
+        sorted[pos] = c;
+    }
     const uint32_t bi = b % mnx, bj = (b / mnx) % mny, bk = b / (mnx * mny);
     CellRec32 r;
     r.x = (float)(cp.f[HC_F_POS_X][c] - (x_min + bi * voxel));
@@ -624,12 +644,21 @@ int probe_kernels_cells2(hc_ctx* ctx);
 int probe_kernels_cells2(hc_ctx* ctx) {
     hipFuncAttributes attr;
     const void* fns[] = {(const void*)k_stable_sort, (const void*)k_stable_gather32, (const void*)k_forces32,
+                         (const void*)k_bin_sort,
                          (const void*)k_totals<double>, (const void*)k_totals<float>, (const void*)k_totals_final};
     for (const void* f : fns) {
         hipError_t e = hipFuncGetAttributes(&attr, f);
         if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: cell kernel (mixed form)");
     }
     return 0;
+}
+
+int launch_bin_sort(hc_ctx* ctx, const uint32_t* offsets, const uint32_t* tmp, uint32_t* sorted, uint32_t* pos_of,
+                    uint32_t nbins) {
+    if (!nbins) return 0;
+    hipLaunchKernelGGL(k_bin_sort, dim3(blocks_for(nbins, 256)), dim3(256), 0, ctx->stream, offsets, tmp, sorted,
+                       pos_of, nbins);
+    return check_launch(ctx, "k_bin_sort");
 }
 
 int launch_scatter(hc_ctx* ctx, const uint32_t* key, const uint32_t* slot, const uint32_t* offsets, uint32_t* tmp,
@@ -654,7 +683,7 @@ int launch_sec_flags(hc_ctx* ctx, const uint32_t* key, const uint32_t* sorted, c
 }
 
 struct HashPtrs {
-    uint32_t *counts, *offsets, *slot, *tmp, *sorted, *partial;
+    uint32_t *counts, *offsets, *slot, *tmp, *sorted, *partial, *pos_of;
     CellRec* rec;
     float4* pos32;
 };
@@ -670,6 +699,7 @@ static HashPtrs hash_ptrs(hc_buf* hash, uint32_t n_mvox, uint32_t max_cells) {
     h.sorted = (uint32_t*)(P + L.sorted);
     h.partial = (uint32_t*)(P + L.partial);
     h.rec = (CellRec*)(P + L.rec);
+    h.pos_of = (uint32_t*)(P + L.pos_of);
     h.pos32 = (float4*)(P + L.pos32);
     return h;
 }
@@ -808,13 +838,14 @@ int hc_build_hash(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech,
     if (n) {
         const uint32_t* key = cp.u[HC_F_MECH_VOXEL - HC_NUM_F64_FIELDS];
         hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, ctx->stream, key, h.slot, h.offsets, h.tmp, n);
+        if ((r = launch_bin_sort(ctx, h.offsets, h.tmp, h.sorted, h.pos_of, n_mvox))) return r;
         if (mech->forces_fp32)
             hipLaunchKernelGGL(k_stable_gather32, dim3(nb), dim3(256), 0, ctx->stream, cp, key, h.offsets, h.tmp,
-                               h.sorted, reinterpret_cast<CellRec32*>(h.rec), n, mgrid->x_min, mgrid->y_min,
+                               h.sorted, h.pos_of, reinterpret_cast<CellRec32*>(h.rec), n, mgrid->x_min, mgrid->y_min,
                                mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny);
         else
             hipLaunchKernelGGL(k_stable_gather, dim3(nb), dim3(256), 0, ctx->stream, cp, key, h.offsets, h.tmp,
-                               h.sorted, h.rec, mark32_ok(mech) ? h.pos32 : nullptr, n, mgrid->x_min, mgrid->y_min,
+                               h.sorted, h.pos_of, h.rec, mark32_ok(mech) ? h.pos32 : nullptr, n, mgrid->x_min, mgrid->y_min,
                                mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny);
         if ((r = check_launch(ctx, "hash scatter"))) return r;
     }

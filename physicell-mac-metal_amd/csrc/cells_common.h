@@ -89,6 +89,42 @@ __device__ inline uint32_t stable_rank(const uint32_t* __restrict__ tmp, uint32_
 }
 
 
+// Bins of at most BIN_SORT_CAP members are put in ascending order by one
+// thread each (k_bin_sort: a 16-input bitonic network in registers, padded
+// with ~0u), which writes the bin's sorted list and every member's position;
+// the gather then reads that position instead of ranking the member against
+// its bin (stable_rank), which it still does for larger bins.
+constexpr uint32_t BIN_SORT_CAP = 16;
+
+__device__ inline void bin_sort_one(const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ tmp,
+                                    uint32_t* __restrict__ sorted, uint32_t* __restrict__ pos_of, uint32_t b) {
+    const uint32_t beg = offsets[b], cnt = offsets[b + 1] - beg;
+    if (cnt == 0 || cnt > BIN_SORT_CAP) return;
+    uint32_t a[BIN_SORT_CAP];
+#pragma unroll
+    for (uint32_t q = 0; q < BIN_SORT_CAP; q++) a[q] = q < cnt ? tmp[beg + q] : ~0u;
+#pragma unroll
+    for (uint32_t k = 2; k <= BIN_SORT_CAP; k <<= 1)
+#pragma unroll
+        for (uint32_t j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (uint32_t i = 0; i < BIN_SORT_CAP; i++) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const uint32_t lo = a[i] < a[l] ? a[i] : a[l], hi = a[i] < a[l] ? a[l] : a[i];
+                    const bool up = (i & k) == 0;
+                    a[i] = up ? lo : hi;
+                    a[l] = up ? hi : lo;
+                }
+            }
+#pragma unroll
+    for (uint32_t q = 0; q < BIN_SORT_CAP; q++)
+        if (q < cnt) {
+            sorted[beg + q] = a[q];
+            pos_of[a[q]] = beg + q;
+        }
+}
+
 // Counter-based uniform stream on [0, 1): the SplitMix64 finaliser (Steele,
 // Lea & Flood 2014) applied to (seed, step, cell, draw).  Stateless, so graph
 // replays and parallel lanes need no generator state.
@@ -133,6 +169,9 @@ inline MotPtrs mot_ptrs(void* base, uint32_t max_cells, uint32_t ns) {
 }
 
 int fill_u32(hc_ctx* ctx, uint32_t* p, uint64_t n, uint32_t v);
+// k_bin_sort over bins 0 .. nbins-1 (bin_sort_one)
+int launch_bin_sort(hc_ctx* ctx, const uint32_t* offsets, const uint32_t* tmp, uint32_t* sorted, uint32_t* pos_of,
+                    uint32_t nbins);
 int launch_scatter(hc_ctx* ctx, const uint32_t* key, const uint32_t* slot, const uint32_t* offsets, uint32_t* tmp,
                    uint32_t n);
 int launch_stable_sort(hc_ctx* ctx, const uint32_t* key, const uint32_t* offsets, const uint32_t* tmp,
