@@ -241,12 +241,16 @@ __global__ __launch_bounds__(256) void k_b_sec_keys(const EnsSim* __restrict__ s
     if (valid) slot[m.cell_off + c] = s;
 }
 
+// C: coefficient storage, the field's type (FP32 fields keep FP32 coefficients,
+// halving the bytes the per-substep apply reads; they are evaluated in FP64
+// and rounded once)
+template <typename C>
 __global__ __launch_bounds__(256) void k_b_sec_groups(const EnsSim* __restrict__ sims,
                                                       const uint32_t* __restrict__ agent_sim,
                                                       const uint32_t* __restrict__ kv, const uint32_t* __restrict__ sorted,
                                                       const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ gid,
                                                       uint32_t* __restrict__ gsim, uint32_t* __restrict__ gvox,
-                                                      uint32_t* __restrict__ gbegin, double* __restrict__ coef,
+                                                      uint32_t* __restrict__ gbegin, C* __restrict__ coef,
                                                       PlanHeader* __restrict__ hdr, uint32_t ntot, uint32_t vtot,
                                                       uint32_t ns_max) {
     const uint32_t t = blockIdx.x * 256 + threadIdx.x;
@@ -284,9 +288,9 @@ __global__ __launch_bounds__(256) void k_b_sec_groups(const EnsSim* __restrict__
         const double a = dt * net[off] / V_voxel;
         const double temp1 = internal_constant * sec[off] * sat[off];
         const double temp2 = 1.0 + internal_constant * (sec[off] + upt[off]);
-        coef[((size_t)s * 3 + 0) * ntot + t] = a;
-        coef[((size_t)s * 3 + 1) * ntot + t] = temp1;
-        coef[((size_t)s * 3 + 2) * ntot + t] = temp2;
+        coef[((size_t)s * 3 + 0) * ntot + t] = (C)a;
+        coef[((size_t)s * 3 + 1) * ntot + t] = (C)temp1;
+        coef[((size_t)s * 3 + 2) * ntot + t] = (C)temp2;
         any_net |= (a != 0.0);
     }
     (void)ns_max;
@@ -296,7 +300,7 @@ __global__ __launch_bounds__(256) void k_b_sec_groups(const EnsSim* __restrict__
 template <typename F>
 __global__ __launch_bounds__(256) void k_b_sec_apply(const EnsSim* __restrict__ sims, const uint32_t* __restrict__ gsim,
                                                      const uint32_t* __restrict__ gvox, const uint32_t* __restrict__ gbegin,
-                                                     const double* __restrict__ coef, const PlanHeader* __restrict__ hdr,
+                                                     const F* __restrict__ coef, const PlanHeader* __restrict__ hdr,
                                                      uint32_t ntot) {
     const uint32_t q = blockIdx.x * 256 + threadIdx.x;
     if (q >= hdr->ngroups) return;
@@ -306,13 +310,13 @@ __global__ __launch_bounds__(256) void k_b_sec_apply(const EnsSim* __restrict__ 
     const uint32_t v = gvox[q];
     const uint32_t b = gbegin[q], e = gbegin[q + 1];
     for (uint32_t s = 0; s < m.ns; s++) {
-        const double* ca = coef + ((size_t)s * 3 + 0) * ntot;
-        const double* c1 = coef + ((size_t)s * 3 + 1) * ntot;
-        const double* c2 = coef + ((size_t)s * 3 + 2) * ntot;
+        const F* ca = coef + ((size_t)s * 3 + 0) * ntot;
+        const F* c1 = coef + ((size_t)s * 3 + 1) * ntot;
+        const F* c2 = coef + ((size_t)s * 3 + 2) * ntot;
         double r = (double)rho[(uint64_t)s * m.V + v];
         for (uint32_t t = b; t < e; t++) {
-            if (has_net) r += ca[t];
-            r = (r + c1[t]) / c2[t];
+            if (has_net) r += (double)ca[t];
+            r = (r + (double)c1[t]) / (double)c2[t];
             if (r < 0.0) r = 0.0;
         }
         rho[(uint64_t)s * m.V + v] = (F)r;
@@ -323,7 +327,8 @@ int probe_kernels_batch(hc_ctx* ctx) {
     hipFuncAttributes attr;
     const void* fns[] = {(const void*)k_b_mech_keys, (const void*)k_b_gather<false>,
                          (const void*)k_b_gather<true>, (const void*)k_b_forces<false>, (const void*)k_b_forces<true>,
-                         (const void*)k_b_integrate, (const void*)k_b_sec_keys, (const void*)k_b_sec_groups,
+                         (const void*)k_b_integrate, (const void*)k_b_sec_keys, (const void*)k_b_sec_groups<double>,
+                         (const void*)k_b_sec_groups<float>,
                          (const void*)k_b_sec_apply<double>, (const void*)k_b_sec_apply<float>};
     for (const void* f : fns) {
         hipError_t e = hipFuncGetAttributes(&attr, f);
@@ -853,11 +858,19 @@ private:
         check(ctx_, hc::launch_stable_sort(ctx_, kv, offsets, tmp, sorted, w.ntot), "stable");
         check(ctx_, hc::launch_sec_flags(ctx_, kv, sorted, offsets, flag, w.ntot, w.vtot), "flags");
         check(ctx_, hc::scan_exclusive(ctx_, flag, flag, w.ntot, partial), "scan flags");
-        hipLaunchKernelGGL(hc::k_b_sec_groups, dim3(hc::blocks_for(std::max(1u, w.ntot), 256)), dim3(256), 0, st, sims,
-                           (const uint32_t*)hc_device_ptr(w.agent_sim.get()), kv, sorted, offsets, flag,
-                           (uint32_t*)hc_device_ptr(w.pgsim.get()), (uint32_t*)hc_device_ptr(w.pgvox.get()),
-                           (uint32_t*)hc_device_ptr(w.pgbegin.get()), (double*)hc_device_ptr(w.pcoef.get()), hdr,
-                           w.ntot, w.vtot, w.ns_max);
+        const dim3 gg(hc::blocks_for(std::max(1u, w.ntot), 256));
+        if (mixed_)
+            hipLaunchKernelGGL(hc::k_b_sec_groups<float>, gg, dim3(256), 0, st, sims,
+                               (const uint32_t*)hc_device_ptr(w.agent_sim.get()), kv, sorted, offsets, flag,
+                               (uint32_t*)hc_device_ptr(w.pgsim.get()), (uint32_t*)hc_device_ptr(w.pgvox.get()),
+                               (uint32_t*)hc_device_ptr(w.pgbegin.get()), (float*)hc_device_ptr(w.pcoef.get()), hdr,
+                               w.ntot, w.vtot, w.ns_max);
+        else
+            hipLaunchKernelGGL(hc::k_b_sec_groups<double>, gg, dim3(256), 0, st, sims,
+                               (const uint32_t*)hc_device_ptr(w.agent_sim.get()), kv, sorted, offsets, flag,
+                               (uint32_t*)hc_device_ptr(w.pgsim.get()), (uint32_t*)hc_device_ptr(w.pgvox.get()),
+                               (uint32_t*)hc_device_ptr(w.pgbegin.get()), (double*)hc_device_ptr(w.pcoef.get()), hdr,
+                               w.ntot, w.vtot, w.ns_max);
         check(ctx_, hc::check_launch(ctx_, "k_b_sec_groups"), "sec groups");
         plan_dirty_ = false;
     }
@@ -872,7 +885,7 @@ private:
         if (mixed_)
             hipLaunchKernelGGL(hc::k_b_sec_apply<float>, grid, dim3(256), 0, st, sims,
                                (const uint32_t*)hc_device_ptr(w.pgsim.get()), (const uint32_t*)hc_device_ptr(w.pgvox.get()),
-                               (const uint32_t*)hc_device_ptr(w.pgbegin.get()), (const double*)hc_device_ptr(w.pcoef.get()),
+                               (const uint32_t*)hc_device_ptr(w.pgbegin.get()), (const float*)hc_device_ptr(w.pcoef.get()),
                                (const hc::PlanHeader*)hc_device_ptr(w.phdr.get()), w.ntot);
         else
             hipLaunchKernelGGL(hc::k_b_sec_apply<double>, grid, dim3(256), 0, st, sims,

@@ -359,11 +359,15 @@ __global__ __launch_bounds__(256) void k_sec_flags(const uint32_t* __restrict__ 
     flag[t] = f;
 }
 
+// C: coefficient storage, the field's type (FP32 fields keep FP32 coefficients,
+// halving the bytes the per-substep apply reads; they are evaluated in FP64
+// and rounded once)
+template <typename C>
 __global__ __launch_bounds__(256) void k_sec_groups(CellPtrs cp, const double* __restrict__ rates,
                                                     const uint32_t* __restrict__ key, const uint32_t* __restrict__ sorted,
                                                     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ gid,
                                                     uint32_t* __restrict__ gvox, uint32_t* __restrict__ gbegin,
-                                                    double* __restrict__ coef, PlanHeader* __restrict__ hdr,
+                                                    C* __restrict__ coef, PlanHeader* __restrict__ hdr,
                                                     uint32_t n, uint32_t max_cells, uint32_t ns, uint32_t V,
                                                     double dt, double V_voxel) {
     const uint32_t t = blockIdx.x * 256 + threadIdx.x;
@@ -394,9 +398,9 @@ __global__ __launch_bounds__(256) void k_sec_groups(CellPtrs cp, const double* _
         const double a = dt * net[off] / V_voxel;
         const double temp1 = internal_constant * sec[off] * sat[off];
         const double temp2 = 1.0 + internal_constant * (sec[off] + upt[off]);
-        coef[((size_t)s * 3 + 0) * max_cells + t] = a;
-        coef[((size_t)s * 3 + 1) * max_cells + t] = temp1;
-        coef[((size_t)s * 3 + 2) * max_cells + t] = temp2;
+        coef[((size_t)s * 3 + 0) * max_cells + t] = (C)a;
+        coef[((size_t)s * 3 + 1) * max_cells + t] = (C)temp1;
+        coef[((size_t)s * 3 + 2) * max_cells + t] = (C)temp2;
         any_net |= (a != 0.0);
     }
     if (any_net) atomicOr(&hdr->has_net, 1u);
@@ -405,7 +409,7 @@ __global__ __launch_bounds__(256) void k_sec_groups(CellPtrs cp, const double* _
 template <typename F>
 __global__ __launch_bounds__(256) void k_sec_apply(F* __restrict__ rho, const uint32_t* __restrict__ gvox,
                                                    const uint32_t* __restrict__ gbegin,
-                                                   const double* __restrict__ coef,
+                                                   const F* __restrict__ coef,
                                                    const PlanHeader* __restrict__ hdr, uint32_t max_cells,
                                                    uint32_t ns, uint64_t V) {
     const uint32_t g = blockIdx.x * 256 + threadIdx.x;
@@ -414,13 +418,13 @@ __global__ __launch_bounds__(256) void k_sec_apply(F* __restrict__ rho, const ui
     const uint32_t v = gvox[g];
     const uint32_t b = gbegin[g], e = gbegin[g + 1];
     for (uint32_t s = 0; s < ns; s++) {
-        const double* ca = coef + ((size_t)s * 3 + 0) * max_cells;
-        const double* c1 = coef + ((size_t)s * 3 + 1) * max_cells;
-        const double* c2 = coef + ((size_t)s * 3 + 2) * max_cells;
+        const F* ca = coef + ((size_t)s * 3 + 0) * max_cells;
+        const F* c1 = coef + ((size_t)s * 3 + 1) * max_cells;
+        const F* c2 = coef + ((size_t)s * 3 + 2) * max_cells;
         double r = (double)rho[(uint64_t)s * V + v];
         for (uint32_t t = b; t < e; t++) {
-            if (has_net) r += ca[t];
-            r = (r + c1[t]) / c2[t];
+            if (has_net) r += (double)ca[t];
+            r = (r + (double)c1[t]) / (double)c2[t];
             if (r < 0.0) r = 0.0;
         }
         rho[(uint64_t)s * V + v] = (F)r;
@@ -433,7 +437,8 @@ int probe_kernels_cells(hc_ctx* ctx) {
                          (const void*)k_fill_u32, (const void*)k_voxel_keys, (const void*)k_mech_keys,
                          (const void*)k_scatter, (const void*)k_stable_gather, (const void*)k_forces,
                          (const void*)k_neighbors, (const void*)k_max_bin, (const void*)k_integrate,
-                         (const void*)k_sec_keys, (const void*)k_sec_flags, (const void*)k_sec_groups,
+                         (const void*)k_sec_keys, (const void*)k_sec_flags, (const void*)k_sec_groups<double>,
+                         (const void*)k_sec_groups<float>,
                          (const void*)k_sec_apply<double>, (const void*)k_sec_apply<float>};
     for (const void* f : fns) {
         hipError_t e = hipFuncGetAttributes(&attr, f);
@@ -760,9 +765,14 @@ int hc_secretion_plan(hc_ctx* ctx, hc_buf* cells, hc_buf* rates, const hc_grid* 
         hipError_t e = hipMemcpyAsync(tmp, key, (size_t)n * 4, hipMemcpyDeviceToDevice, ctx->stream);
         if (e != hipSuccess) return hip_fail(ctx, e, "hc_secretion_plan: key copy");
     }
-    hipLaunchKernelGGL(k_sec_groups, dim3(nb), dim3(256), 0, ctx->stream, cp, (const double*)rates->dptr,
-                       tmp, sorted, offsets, gflag, gvox, gbegin, coef, hdr, n, max_cells,
-                       g->n_substrates, (uint32_t)V, dt, g->dx * g->dy * g->dz);
+    if (g->field_fp32)
+        hipLaunchKernelGGL(k_sec_groups<float>, dim3(nb), dim3(256), 0, ctx->stream, cp, (const double*)rates->dptr,
+                           tmp, sorted, offsets, gflag, gvox, gbegin, (float*)coef, hdr, n, max_cells,
+                           g->n_substrates, (uint32_t)V, dt, g->dx * g->dy * g->dz);
+    else
+        hipLaunchKernelGGL(k_sec_groups<double>, dim3(nb), dim3(256), 0, ctx->stream, cp, (const double*)rates->dptr,
+                           tmp, sorted, offsets, gflag, gvox, gbegin, coef, hdr, n, max_cells,
+                           g->n_substrates, (uint32_t)V, dt, g->dx * g->dy * g->dz);
     return check_launch(ctx, "k_sec_groups");
 }
 
@@ -782,7 +792,7 @@ int hc_secretion_apply(hc_ctx* ctx, hc_buf* density, hc_buf* plan, const hc_grid
     if (g->field_fp32)
         hipLaunchKernelGGL(k_sec_apply<float>, dim3(blocks_for(max_cells, 256)), dim3(256), 0, ctx->stream,
                            (float*)density->dptr, (const uint32_t*)(P + L.gvox), (const uint32_t*)(P + L.gbegin),
-                           (const double*)(P + L.coef), (const PlanHeader*)(P + L.header), max_cells,
+                           (const float*)(P + L.coef), (const PlanHeader*)(P + L.header), max_cells,
                            g->n_substrates, V);
     else
         hipLaunchKernelGGL(k_sec_apply<double>, dim3(blocks_for(max_cells, 256)), dim3(256), 0, ctx->stream,
