@@ -84,6 +84,22 @@ def test_member_equals_single(mixed):
     ens.close()
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_crowded_member_equals_single(mixed):
+    """A member with bins of more than 16 agents (the hash ranks their agents
+    one by one) beside ordinary members."""
+    crowded = S.small3d(12, 10, 8, n_cells=4000, seed=11)
+    members = [small_member(230), crowded, small_member(231)]
+    P = 4
+    ens = hip.Ensemble(mixed=mixed)
+    ids = [ens.add(sc, c) for sc, c in members]
+    ens.run_periods(P, use_graph=False)
+    ens.wait()
+    for mid, (sc, c) in zip(ids, members):
+        assert_same(member_state(ens, mid), run_single(sc, c, P, mixed), f"member {mid}")
+    ens.close()
+
+
 def test_alone_and_in_mix_mixed():
     x = small_member(210)
     others = [small_member(s) for s in (211, 212, 213)]

@@ -154,9 +154,19 @@ def cells_buffer(ctx, spec, cells):
     return H.CellBuffer(ctx, cells, len(spec.substrates))
 
 
-@pytest.mark.parametrize("builder", ["config0", "small3d", "config1"])
+def crowded():
+    """About 12 agents per 30 um bin at random positions: bins of up to 16
+    members are sorted by one thread each, larger ones by the per-agent rank;
+    this population has both."""
+    return S.small3d(12, 10, 8, n_cells=4000, seed=11)
+
+
+MECH_BUILDERS = dict(S.BUILDERS, crowded=crowded)
+
+
+@pytest.mark.parametrize("builder", ["config0", "small3d", "config1", "crowded"])
 def test_voxel_keys_hash_neighbors_forces_ab2(ctx, builder):
-    spec, cells = S.BUILDERS[builder]()
+    spec, cells = MECH_BUILDERS[builder]()
     lib = H.load()
     n = cells.n
     cb = cells_buffer(ctx, spec, cells)
@@ -181,6 +191,9 @@ def test_voxel_keys_hash_neighbors_forces_ab2(ctx, builder):
     assert np.array_equal(off, off_ref)
     assert np.array_equal(srt, srt_ref)
     assert np.array_equal(np.diff(off_ref.astype(np.int64)), np.bincount(keys_ref, minlength=nb))
+    if builder == "crowded":
+        sizes = np.diff(off_ref.astype(np.int64))
+        assert sizes.max() > 16 and ((sizes > 0) & (sizes <= 16)).any(), "both bin paths"
     # neighbour lists (visit order) vs oracle
     cap = 128
     cnt_b = ctx.alloc(4 * n)
