@@ -514,6 +514,16 @@ int pe_add(pe_ensemble* e, const pc_config* cfg, uint32_t n, const double* x, co
            const double* saturation, const double* net_export, const uint32_t* alive, uint32_t* member_id);
 /* Removes a member and frees its device state (waits for queued work). */
 int pe_remove(pe_ensemble* e, uint32_t member_id);
+/* Motility for one member, as pc_sim_set_motility for a single simulation
+ * (updateMotility, src/motility.cpp:179-293, between the forces and AB2 of
+ * every mechanics step; NULL arrays take MotilityData's defaults,
+ * src/motility.cpp:137-157).  A member with motility steps exactly as a
+ * pc_sim with the same motility; members that have it run their mechanics
+ * after the substeps rather than beside them. */
+int pe_set_motility(pe_ensemble* e, uint32_t member_id, uint64_t seed, const double* persistence,
+                    const double* bias, const double* speed, const uint32_t* is_motile,
+                    const uint32_t* restrict_2d, const int32_t* chem_index, const int32_t* chem_dir,
+                    const double* sensitivity);
 /* Runs `periods` periods of every member; use_graph != 0 captures one period
  * as a hipGraph (re-captured after membership changes) and replays it. */
 int pe_run_periods(pe_ensemble* e, int periods, int use_graph);

@@ -14,7 +14,7 @@
 //   * the secretion plan: one counting sort over the concatenated voxel space.
 // A member's arithmetic does not depend on the other members: its results are
 // identical alone and in any mix (and identical to the single-simulation path).
-#include "cells_common.h"
+#include "motility_common.h"
 #include "host/physicell_host.h"
 
 #include <algorithm>
@@ -32,7 +32,7 @@ struct EnsSim {
     uint32_t nx, ny, nz, V;
     double x_min, y_min, z_min, dx, dy, dz, dt_diff;
     double mvoxel, max_dist, dt_mech;
-    uint32_t mnx, mny, mnz, _pad;
+    uint32_t mnx, mny, mnz, mot;   // mot: bit 0 motility on, bit 1 chemotaxis by sensitivities
     uint32_t cell_off, bin_off, vox_off, _pad2;
 };
 
@@ -323,13 +323,39 @@ __global__ __launch_bounds__(256) void k_b_sec_apply(const EnsSim* __restrict__ 
     }
 }
 
+// motility of every member that has it (src/motility.cpp:179-293), between
+// the forces and AB2, as the single simulation's k_motility: same per-agent
+// update, the member's own field, grid, step counter and seed
+template <typename F>
+__global__ __launch_bounds__(256) void k_b_motility(const EnsSim* __restrict__ sims, const AgentTile* __restrict__ tiles,
+                                                    const MotPtrs* __restrict__ mots) {
+    const AgentTile tl = tiles[blockIdx.x];
+    const EnsSim m = sims[tl.sim];
+    const uint32_t c = tl.c0 + threadIdx.x;
+    if (!(m.mot & 1u) || c >= m.n) return;
+    hc_grid g{};
+    g.nx = m.nx; g.ny = m.ny; g.nz = m.nz; g.n_substrates = m.ns;
+    g.dx = m.dx; g.dy = m.dy; g.dz = m.dz; g.x_min = m.x_min; g.y_min = m.y_min; g.z_min = m.z_min;
+    motility_one<F>(cell_ptrs(m.soa, m.max_cells), mots[tl.sim], reinterpret_cast<const F*>(m.rho), g, m.dt_mech, c,
+                    (m.mot >> 1) & 1u);
+}
+
+// each member's step counter advances after every motility call
+__global__ void k_b_motility_advance(const EnsSim* __restrict__ sims, const MotPtrs* __restrict__ mots,
+                                     uint32_t nsims) {
+    const uint32_t s = blockIdx.x * 256 + threadIdx.x;
+    if (s < nsims && (sims[s].mot & 1u)) mots[s].hdr[0] += 1;
+}
+
 int probe_kernels_batch(hc_ctx* ctx) {
     hipFuncAttributes attr;
     const void* fns[] = {(const void*)k_b_mech_keys, (const void*)k_b_gather<false>,
                          (const void*)k_b_gather<true>, (const void*)k_b_forces<false>, (const void*)k_b_forces<true>,
                          (const void*)k_b_integrate, (const void*)k_b_sec_keys, (const void*)k_b_sec_groups<double>,
                          (const void*)k_b_sec_groups<float>,
-                         (const void*)k_b_sec_apply<double>, (const void*)k_b_sec_apply<float>};
+                         (const void*)k_b_sec_apply<double>, (const void*)k_b_sec_apply<float>,
+                         (const void*)k_b_motility<double>, (const void*)k_b_motility<float>,
+                         (const void*)k_b_motility_advance};
     for (const void* f : fns) {
         hipError_t e = hipFuncGetAttributes(&attr, f);
         if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: ensemble kernel");
@@ -361,10 +387,12 @@ struct Member {
     hc_grid mgrid{};
     uint32_t n = 0, max_cells = 0;
     DeviceBuffer density, tables, soa, rates;
+    DeviceBuffer mot;                 // motility state (hc_motility_bytes), empty when off
+    bool mot_sens = false;            // chemotaxis by per-substrate sensitivities
     // the member's own schedule (the step loop of src/main.mm:275-415)
     double mech_acc = 0.0, pheno_acc = 0.0;
     int64_t substeps = 0, mech_steps = 0, pheno_steps = 0;
-    size_t bytes() const { return density.size() + tables.size() + soa.size() + rates.size(); }
+    size_t bytes() const { return density.size() + tables.size() + soa.size() + rates.size() + mot.size(); }
 };
 
 class Ensemble {
@@ -515,6 +543,47 @@ public:
         members_.erase(it);
         dirty_ = true;
         plan_dirty_ = true;
+    }
+
+    // Motility for one member (pc_sim_set_motility for a single simulation):
+    // defaults as MotilityData::setDefaults (src/motility.cpp:137-157) where no
+    // array is given; speed goes to the member's cell field.
+    void set_motility(uint32_t id, uint64_t seed, const double* persistence, const double* bias, const double* speed,
+                      const uint32_t* is_motile, const uint32_t* restrict_2d, const int32_t* chem_index,
+                      const int32_t* chem_dir, const double* sensitivity) {
+        Member& m = member(id);
+        check(ctx_, hc_wait(ctx_), "wait");
+        const uint32_t ns = m.grid.n_substrates, mc = m.max_cells, n = m.n;
+        m.mot.allocate(ctx_, hc_motility_bytes(mc, ns));
+        uint8_t* h = (uint8_t*)m.mot.host();
+        std::memset(h, 0, m.mot.size());
+        uint64_t* hdr = (uint64_t*)h;
+        hdr[0] = 0;
+        hdr[1] = seed;
+        auto F = [&](uint32_t f) { return (double*)(h + hc_motility_field_offset(f, mc, ns)); };
+        auto U = [&](uint32_t f) { return (uint32_t*)(h + hc_motility_field_offset(f, mc, ns)); };
+        auto I = [&](uint32_t f) { return (int32_t*)(h + hc_motility_field_offset(f, mc, ns)); };
+        for (uint32_t i = 0; i < n; i++) {
+            F(HC_M_PERSISTENCE)[i] = persistence ? persistence[i] : 15.0;
+            F(HC_M_BIAS)[i] = bias ? bias[i] : 0.5;
+            U(HC_M_MOTILE)[i] = is_motile ? is_motile[i] : 0u;
+            U(HC_M_RESTRICT_2D)[i] = restrict_2d ? restrict_2d[i] : (m.cfg.use_2D ? 1u : 0u);
+            I(HC_M_CHEM_INDEX)[i] = chem_index ? chem_index[i] : 0;
+            I(HC_M_CHEM_DIR)[i] = chem_dir ? chem_dir[i] : 1;
+            for (uint32_t q = 0; q < ns; q++)
+                F(HC_M_SENSITIVITY)[(size_t)i * ns + q] = sensitivity ? sensitivity[(size_t)i * ns + q] : 0.0;
+        }
+        m.mot.upload_all();
+        if (speed) {
+            // the member's cells have moved on the device: write the speed field alone
+            double* sp = (double*)((char*)m.soa.host() + hc_cells_field_offset(HC_F_MOT_SPEED, mc));
+            for (uint32_t i = 0; i < n; i++) sp[i] = speed[i];
+            check(ctx_, hc_upload(ctx_, m.soa.get(), hc_cells_field_offset(HC_F_MOT_SPEED, mc), (size_t)n * 8),
+                  "speed upload");
+        }
+        check(ctx_, hc_wait(ctx_), "wait");
+        m.mot_sens = sensitivity != nullptr;
+        dirty_ = true;
     }
 
     // one period: substeps_per_mech diffusion substeps, one mechanics step,
@@ -668,7 +737,7 @@ public:
 
 private:
     struct Workspace {
-        DeviceBuffer sims, bsims, tiles, work[NLIST], agent_sim, kglob, slot, tmp, sorted, pos_of, recs, pos32, counts,
+        DeviceBuffer sims, mots, bsims, tiles, work[NLIST], agent_sim, kglob, slot, tmp, sorted, pos_of, recs, pos32, counts,
             offsets, partial;
         DeviceBuffer pcounts, poffsets, pkv, pslot, ptmp, psorted, pflag, pgsim, pgvox, pgbegin, pcoef, phdr, ppartial;
         uint32_t ntot = 0, nbins = 0, vtot = 0, ntiles = 0, nwork[NLIST] = {}, ns_max = 0, nmax_all = 0;
@@ -724,6 +793,8 @@ private:
         std::vector<AgentTile> tiles;
         std::vector<BatchWork> work[NLIST], narrow[NLIST];
         std::vector<uint32_t> agent_sim;
+        std::vector<hc::MotPtrs> mots;
+        any_motility_ = false;
         uint32_t cell_off = 0, bin_off = 0, vox_off = 0, ns_max = 0, nmax_all = 0;
         uint32_t max_line[NLIST] = {};
         for (uint32_t si = 0; si < members_.size(); si++) {
@@ -741,6 +812,13 @@ private:
             e.mvoxel = m.mech.mechanics_voxel_size; e.max_dist = m.mech.max_interaction_distance; e.dt_mech = m.mech.dt;
             e.mnx = m.mech.mech_grid_nx; e.mny = m.mech.mech_grid_ny; e.mnz = m.mech.mech_grid_nz;
             e.cell_off = cell_off; e.bin_off = bin_off; e.vox_off = vox_off;
+            hc::MotPtrs mp{};
+            if (m.mot.get()) {
+                mp = hc::mot_ptrs(hc_device_ptr(m.mot.get()), m.max_cells, g.n_substrates);
+                e.mot = 1u | (m.mot_sens ? 2u : 0u);
+                any_motility_ = true;
+            }
+            mots.push_back(mp);
             sims.push_back(e);
             BatchSim b{};
             b.rho = e.rho;
@@ -793,6 +871,7 @@ private:
             bytes += b.size();
         };
         put(w.sims, sims.data(), sims.size() * sizeof(EnsSim));
+        put(w.mots, mots.data(), mots.size() * sizeof(hc::MotPtrs));
         put(w.bsims, bsims.data(), bsims.size() * sizeof(BatchSim));
         put(w.tiles, tiles.data(), tiles.size() * sizeof(AgentTile));
         for (int li = 0; li < NLIST; li++) put(w.work[li], work[li].data(), work[li].size() * sizeof(BatchWork));
@@ -941,6 +1020,18 @@ private:
                                    w.ntot);
             check(ctx_, hc::check_launch(ctx_, "k_b_forces"), "forces");
         }
+        if (any_motility_) {
+            const hc::MotPtrs* mots = (const hc::MotPtrs*)hc_device_ptr(w.mots.get());
+            hc::KernelTimer tm(ctx_, HC_K_MOTILITY);
+            if (mixed_)
+                hipLaunchKernelGGL(hc::k_b_motility<float>, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, mots);
+            else
+                hipLaunchKernelGGL(hc::k_b_motility<double>, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, mots);
+            const uint32_t nsims = (uint32_t)members_.size();
+            hipLaunchKernelGGL(hc::k_b_motility_advance, dim3(hc::blocks_for(nsims, 256)), dim3(256), 0, st, sims, mots,
+                               nsims);
+            check(ctx_, hc::check_launch(ctx_, "k_b_motility"), "motility");
+        }
         {
             hc::KernelTimer tm(ctx_, HC_K_INTEGRATE);
             hipLaunchKernelGGL(hc::k_b_integrate, dim3(w.ntiles), dim3(256), 0, st, sims, tiles);
@@ -969,7 +1060,7 @@ private:
     void period_kernels() {
         if (plan_dirty_) plan_secretion();
         const int k = substeps_per_mech();
-        if (!overlap_) {
+        if (!overlap_ || any_motility_) {            // motility reads the substeps' fields
             for (int s = 0; s < k; s++) {
                 sweeps();
                 secretion();
@@ -1053,7 +1144,7 @@ private:
     hipStream_t side_ = nullptr;
     hipEvent_t fork_ = nullptr, join_ = nullptr;
     bool overlap_ = true;
-    bool mixed_ = false, dirty_ = true, plan_dirty_ = true;
+    bool mixed_ = false, dirty_ = true, plan_dirty_ = true, any_motility_ = false;
     uint32_t next_id_ = 0;
     unsigned grid_blocks_[NLIST] = {};                 // 0: one workgroup per work item
     void* graph_ = nullptr;
@@ -1132,6 +1223,15 @@ int pe_add(pe_ensemble* p, const pc_config* cfg, uint32_t n, const double* x, co
     if (n && (!x || !y || !z || !radius || !volume || !rep || !adh || !rel)) return HC_ERR_ARG;
     return eguard(p, [&](Ensemble& e) {
         *member_id = e.add(to_simconfig(cfg), n, x, y, z, radius, volume, rep, adh, rel, sec, upt, sat, net, alive);
+    });
+}
+
+int pe_set_motility(pe_ensemble* p, uint32_t member_id, uint64_t seed, const double* persistence, const double* bias,
+                    const double* speed, const uint32_t* is_motile, const uint32_t* restrict_2d,
+                    const int32_t* chem_index, const int32_t* chem_dir, const double* sensitivity) {
+    return eguard(p, [&](Ensemble& e) {
+        e.set_motility(member_id, seed, persistence, bias, speed, is_motile, restrict_2d, chem_index, chem_dir,
+                       sensitivity);
     });
 }
 

@@ -18,7 +18,7 @@ struct CellPtrs {
     uint32_t* u[HC_NUM_U32_FIELDS];
 };
 
-inline CellPtrs cell_ptrs(void* base, uint32_t max_cells) {
+__host__ __device__ inline CellPtrs cell_ptrs(void* base, uint32_t max_cells) {
     CellPtrs p;
     for (int f = 0; f < HC_NUM_F64_FIELDS; f++) p.f[f] = (double*)base + (size_t)f * max_cells;
     uint32_t* ub = (uint32_t*)((double*)base + (size_t)HC_NUM_F64_FIELDS * max_cells);

@@ -182,7 +182,8 @@ def load(path=LIB_PATH):
         "pe_create": (c_i32, [P(c_vp), c_i32, c_u32]), "pe_destroy": (None, [c_vp]),
         "pe_last_error": (ctypes.c_char_p, [c_vp]),
         "pe_add": (c_i32, [c_vp, P(Config), c_u32] + [c_vp] * 13 + [P(c_u32)]),
-        "pe_remove": (c_i32, [c_vp, c_u32]), "pe_run_periods": (c_i32, [c_vp, c_i32, c_i32]),
+        "pe_remove": (c_i32, [c_vp, c_u32]),
+        "pe_set_motility": (c_i32, [c_vp, c_u32, ctypes.c_uint64] + [c_vp] * 8), "pe_run_periods": (c_i32, [c_vp, c_i32, c_i32]),
         "pe_wait": (c_i32, [c_vp]), "pe_get_density": (c_i32, [c_vp, c_u32, c_vp]),
         "pe_get_field": (c_i32, [c_vp, c_u32, c_u32, c_vp]),
         "pe_member_info": (c_i32, [c_vp, c_u32, P(ctypes.c_uint64), c_vp]),
@@ -546,6 +547,15 @@ class Ensemble:
         self._check(load().pe_remove(self.h, mid), "pe_remove")
         self.specs.pop(mid, None)
         self.ncells.pop(mid, None)
+
+    def set_motility(self, mid, seed, persistence=None, bias=None, speed=None, is_motile=None, restrict_2d=None,
+                     chem_index=None, chem_dir=None, sensitivity=None):
+        """pe_set_motility: Simulation.set_motility for one member."""
+        conv = [(persistence, np.float64), (bias, np.float64), (speed, np.float64), (is_motile, np.uint32),
+                (restrict_2d, np.uint32), (chem_index, np.int32), (chem_dir, np.int32), (sensitivity, np.float64)]
+        arrs = [None if a is None else np.ascontiguousarray(np.asarray(a, t).reshape(-1)) for a, t in conv]
+        self._keep_mot = getattr(self, "_keep_mot", []) + [arrs]
+        self._check(load().pe_set_motility(self.h, mid, seed, *[_ptr(a) for a in arrs]), "pe_set_motility")
 
     def run_periods(self, periods, use_graph=True):
         self._check(load().pe_run_periods(self.h, periods, 1 if use_graph else 0), "pe_run_periods")

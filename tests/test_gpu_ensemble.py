@@ -31,10 +31,13 @@ def substeps_until(dtd, dt):
     return 1000000
 
 
-def run_single(sc, c, periods, mixed, start=0):
+def run_single(sc, c, periods, mixed, start=0, motility=None):
     """pc_sim driven through the same schedule: per period k substeps + one
-    mechanics step, the phenotype step where the member's own clock fires."""
+    mechanics step, the phenotype step where the member's own clock fires.
+    motility: (seed, keyword arrays) for Simulation.set_motility."""
     sim = hip.Simulation(sc, c, mixed=mixed)
+    if motility is not None:
+        sim.set_motility(motility[0], **motility[1])
     k = substeps_until(sc.dt_diff, sc.dt_mech)
     kp = substeps_until(sc.dt_diff, sc.dt_pheno)
     for p in range(periods):
@@ -97,6 +100,44 @@ def test_crowded_member_equals_single(mixed):
     ens.wait()
     for mid, (sc, c) in zip(ids, members):
         assert_same(member_state(ens, mid), run_single(sc, c, P, mixed), f"member {mid}")
+    ens.close()
+
+
+def motility_arrays(n, ns, seed, sensitivities):
+    r = np.random.default_rng(seed)
+    kw = dict(persistence=r.uniform(0.05, 2.0, n), bias=r.uniform(0.0, 1.0, n), speed=r.uniform(0.5, 2.0, n),
+              is_motile=(r.uniform(size=n) < 0.8).astype(np.uint32), chem_index=r.integers(0, ns, n),
+              chem_dir=np.where(r.uniform(size=n) < 0.5, 1, -1))
+    if sensitivities:
+        kw["sensitivity"] = r.uniform(-1.0, 1.0, n * ns)
+    return kw
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_member_motility_equals_single(mixed):
+    """pe_set_motility: members with motility (chemotaxis by substrate index on
+    one, by sensitivities on another) beside a member without, over graph
+    replays (fresh draws every mechanics step), bit for bit against pc_sim
+    with the same motility."""
+    members = [small_member(240), small_member(241), small_member(242)]
+    P = 6
+    ens = hip.Ensemble(mixed=mixed)
+    ids = [ens.add(sc, c) for sc, c in members]
+    mot = {}
+    for j in (0, 2):
+        sc, c = members[j]
+        mot[j] = (1000 + j, motility_arrays(c.n, len(sc.substrates), 300 + j, sensitivities=(j == 2)))
+        ens.set_motility(ids[j], mot[j][0], **mot[j][1])
+    ens.run_periods(P, use_graph=True)
+    ens.wait()
+    for j, (mid, (sc, c)) in enumerate(zip(ids, members)):
+        got = member_state(ens, mid)
+        want = run_single(sc, c, P, mixed, motility=mot.get(j))
+        print(f"member {mid}: cells {c.n} motility={j in mot} mixed={mixed}", flush=True)
+        assert_same(got, want, f"member {mid}")
+    # motility moves the cells: the member without it differs from one with it
+    assert not np.array_equal(member_state(ens, ids[0])[hip.F_POS_X],
+                              run_single(*members[0], P, mixed)[hip.F_POS_X])
     ens.close()
 
 
