@@ -524,6 +524,18 @@ int pe_set_motility(pe_ensemble* e, uint32_t member_id, uint64_t seed, const dou
                     const double* bias, const double* speed, const uint32_t* is_motile,
                     const uint32_t* restrict_2d, const int32_t* chem_index, const int32_t* chem_dir,
                     const double* sensitivity);
+/* The device phenotype step for one member, as pc_sim_set_volume and
+ * pc_sim_update_phenotype for a single simulation.  pe_set_volume also
+ * makes room for max_cells cells (ignored when not larger than the member's
+ * capacity; the member's arrays move to the larger layout), which the
+ * divisions of pe_update_phenotype may fill.  pe_update_phenotype refreshes
+ * the member's voxel keys and re-plans secretion; *n_out is its new cell
+ * count. */
+int pe_set_volume(pe_ensemble* e, uint32_t member_id, uint64_t seed, const double* const* fields,
+                  uint32_t max_cells);
+int pe_update_phenotype(pe_ensemble* e, uint32_t member_id, double dt, int use_ratio, const uint32_t* mask,
+                        const uint32_t* parents, uint32_t n_parents, const uint32_t* removed, uint32_t n_removed,
+                        uint32_t* n_out);
 /* Runs `periods` periods of every member; use_graph != 0 captures one period
  * as a hipGraph (re-captured after membership changes) and replays it. */
 int pe_run_periods(pe_ensemble* e, int periods, int use_graph);

@@ -389,10 +389,13 @@ struct Member {
     DeviceBuffer density, tables, soa, rates;
     DeviceBuffer mot;                 // motility state (hc_motility_bytes), empty when off
     bool mot_sens = false;            // chemotaxis by per-substrate sensitivities
+    DeviceBuffer vol, mask;           // volume state (hc_volume_bytes) and phenotype mask, empty when off
     // the member's own schedule (the step loop of src/main.mm:275-415)
     double mech_acc = 0.0, pheno_acc = 0.0;
     int64_t substeps = 0, mech_steps = 0, pheno_steps = 0;
-    size_t bytes() const { return density.size() + tables.size() + soa.size() + rates.size() + mot.size(); }
+    size_t bytes() const {
+        return density.size() + tables.size() + soa.size() + rates.size() + mot.size() + vol.size() + mask.size();
+    }
 };
 
 class Ensemble {
@@ -505,7 +508,15 @@ public:
         auto F = [&](uint32_t f) { return (double*)(h + hc_cells_field_offset(f, m->max_cells)); };
         auto U = [&](uint32_t f) { return (uint32_t*)(h + hc_cells_field_offset(f, m->max_cells)); };
         std::memset(h, 0, m->soa.size());
+        // fresh-cell values the caller does not pass, as a single simulation's
+        // cell store writes them (src/cell_data.cpp:544-680): nuclear volume
+        // 540 um^3 and its equivalent-sphere radius, motility speed 1 and
+        // bias 0.5, oncoprotein 1
+        const double pi = 3.14159265358979323846;
+        const double nuc_rad = std::cbrt(3.0 * 540.0 / (4.0 * pi));
         for (uint32_t i = 0; i < n; i++) {
+            F(HC_F_NUCLEAR_VOL)[i] = 540.0; F(HC_F_NUCLEAR_RAD)[i] = nuc_rad;
+            F(HC_F_MOT_SPEED)[i] = 1.0; F(HC_F_MOT_BIAS)[i] = 0.5; F(HC_F_ONCOPROTEIN)[i] = 1.0;
             F(HC_F_POS_X)[i] = x[i]; F(HC_F_POS_Y)[i] = y[i]; F(HC_F_POS_Z)[i] = z[i];
             F(HC_F_RADIUS)[i] = radius[i]; F(HC_F_TOTAL_VOL)[i] = volume[i];
             F(HC_F_REPULSION)[i] = rep[i]; F(HC_F_ADHESION)[i] = adh[i]; F(HC_F_REL_MAX_ADH)[i] = rel[i];
@@ -584,6 +595,67 @@ public:
         check(ctx_, hc_wait(ctx_), "wait");
         m.mot_sens = sensitivity != nullptr;
         dirty_ = true;
+    }
+
+    // Volume state for one member's device phenotype step (pc_sim_set_volume
+    // for a single simulation: CellData::setDefaults, src/cell_data.cpp:
+    // 555-624, where fields[f] is NULL), with room for max_cells cells: the
+    // member's cell, rate and motility arrays move to the larger layout.
+    void set_volume(uint32_t id, uint64_t seed, const double* const* fields, uint32_t max_cells) {
+        Member& m = member(id);
+        check(ctx_, hc_wait(ctx_), "wait");
+        if (max_cells > m.max_cells) grow(m, max_cells);
+        const uint32_t mc = m.max_cells, n = m.n;
+        m.vol.allocate(ctx_, hc_volume_bytes(mc));
+        uint8_t* h = (uint8_t*)m.vol.host();
+        std::memset(h, 0, m.vol.size());
+        uint64_t* hdr = (uint64_t*)h;
+        hdr[0] = 0;
+        hdr[1] = seed;
+        const double V = 2494.0, Vn = 540.0, ff = 0.75, Vc = V - Vn;
+        const double Vns = Vn - ff * Vn, Vcs = Vc - ff * Vc;
+        const double fresh[HC_NUM_V_FIELDS] = {ff * V, Vns, Vcs, ff, 0.0, Vcs, Vns, ff, 3.62,
+                                               3.0 / 60.0, 0.33 / 60.0, 0.27 / 60.0, 0.0, 2.0 * V, 2.0};
+        for (uint32_t f = 0; f < HC_NUM_V_FIELDS; f++) {
+            double* a = (double*)(h + hc_volume_field_offset(f, mc));
+            for (uint32_t i = 0; i < mc; i++) a[i] = fresh[f];
+            if (fields && fields[f]) std::memcpy(a, fields[f], (size_t)n * sizeof(double));
+        }
+        m.vol.upload_all();
+        check(ctx_, hc_wait(ctx_), "wait");
+        dirty_ = true;
+    }
+
+    // One member's device phenotype step, as pc_sim_update_phenotype: the
+    // volume model on the masked cells, divisions, removals, then the
+    // member's voxel refresh and the secretion re-plan.
+    uint32_t update_phenotype(uint32_t id, double dt, int use_ratio, const uint32_t* mask, const uint32_t* parents,
+                              uint32_t n_parents, const uint32_t* removed, uint32_t n_removed) {
+        Member& m = member(id);
+        if (!m.vol.get()) throw std::runtime_error("ensemble: set the member's volume state first");
+        check(ctx_, hc_wait(ctx_), "wait");
+        const uint32_t mc = m.max_cells, ns = m.grid.n_substrates;
+        uint32_t n = m.n;
+        hc_buf* mbuf = nullptr;
+        if (mask && n) {
+            if (!m.mask.get() || m.mask.size() < (size_t)mc * 4) m.mask.allocate(ctx_, (size_t)mc * 4);
+            std::memcpy(m.mask.host(), mask, (size_t)n * 4);
+            m.mask.upload(0, (size_t)n * 4);
+            mbuf = m.mask.get();
+        }
+        check(ctx_, hc_volume_update(ctx_, m.soa.get(), m.vol.get(), mbuf, n, mc, dt, use_ratio), "volume");
+        hc_buf* mot = m.mot.get();
+        uint32_t n1 = n, n2 = n;
+        check(ctx_, hc_divide(ctx_, m.soa.get(), m.rates.get(), m.vol.get(), mot, ns, parents, n_parents, n, mc, &n1),
+              "divide");
+        check(ctx_, hc_remove(ctx_, m.soa.get(), m.rates.get(), m.vol.get(), mot, ns, removed, n_removed, n1, mc, &n2),
+              "remove");
+        m.n = n2;
+        check(ctx_, hc_voxel_keys(ctx_, m.soa.get(), &m.grid, m.n, m.max_cells), "voxel refresh");
+        check(ctx_, hc_wait(ctx_), "wait");
+        dirty_ = true;
+        plan_dirty_ = true;
+        return n2;
     }
 
     // one period: substeps_per_mech diffusion substeps, one mechanics step,
@@ -896,6 +968,50 @@ private:
     }
 
     hc_ctx* c() { return ctx_; }
+
+    // Moves a member's cell SoA, rate arrays and motility state to the layout
+    // of a larger capacity (every field keeps its first n values).
+    void grow(Member& m, uint32_t new_max) {
+        const uint32_t old_max = m.max_cells, n = m.n, ns = m.grid.n_substrates;
+        hipStream_t st = hc_stream_of(ctx_);
+        auto copy = [&](DeviceBuffer& to, size_t to_off, DeviceBuffer& from, size_t from_off, size_t nb) {
+            if (!nb) return;
+            hipError_t e = hipMemcpyAsync((char*)hc_device_ptr(to.get()) + to_off,
+                                          (const char*)hc_device_ptr(from.get()) + from_off, nb,
+                                          hipMemcpyDeviceToDevice, st);
+            if (e != hipSuccess) throw std::runtime_error(std::string("ensemble grow: ") + hipGetErrorString(e));
+        };
+        DeviceBuffer soa, rates, mot;
+        soa.allocate(ctx_, hc_cells_bytes(new_max));
+        for (uint32_t f = 0; f < HC_NUM_CELL_FIELDS; f++)
+            copy(soa, hc_cells_field_offset(f, new_max), m.soa, hc_cells_field_offset(f, old_max),
+                 (size_t)n * (f < HC_NUM_F64_FIELDS ? 8 : 4));
+        rates.allocate(ctx_, std::max<size_t>(8, hc_rates_bytes(new_max, ns)));
+        for (uint32_t q = 0; q < 4; q++)
+            copy(rates, (size_t)q * new_max * ns * 8, m.rates, (size_t)q * old_max * ns * 8, (size_t)n * ns * 8);
+        if (m.mot.get()) {
+            mot.allocate(ctx_, hc_motility_bytes(new_max, ns));
+            copy(mot, 0, m.mot, 0, 256);                       // step counter and seed
+            for (uint32_t f = 0; f < HC_M_SENSITIVITY; f++)
+                copy(mot, hc_motility_field_offset(f, new_max, ns), m.mot, hc_motility_field_offset(f, old_max, ns),
+                     (size_t)n * (f <= HC_M_VEC_Z ? 8 : 4));
+            copy(mot, hc_motility_field_offset(HC_M_SENSITIVITY, new_max, ns), m.mot,
+                 hc_motility_field_offset(HC_M_SENSITIVITY, old_max, ns), (size_t)n * ns * 8);
+        }
+        check(ctx_, hc_wait(ctx_), "wait");
+        m.soa = std::move(soa);
+        m.rates = std::move(rates);
+        if (mot.get()) m.mot = std::move(mot);
+        // host mirrors of the new buffers: the cell SoA's, which set_motility
+        // writes single fields through
+        m.soa.download_all();
+        check(ctx_, hc_wait(ctx_), "wait");
+        m.max_cells = new_max;
+        m.vol = DeviceBuffer();
+        m.mask = DeviceBuffer();
+        dirty_ = true;
+        plan_dirty_ = true;
+    }
 
     // x, y, [z] over every member; each member re-applies Dirichlet in its
     // own last sweep (z in 3-D, y in 2-D)
@@ -1232,6 +1348,21 @@ int pe_set_motility(pe_ensemble* p, uint32_t member_id, uint64_t seed, const dou
     return eguard(p, [&](Ensemble& e) {
         e.set_motility(member_id, seed, persistence, bias, speed, is_motile, restrict_2d, chem_index, chem_dir,
                        sensitivity);
+    });
+}
+
+int pe_set_volume(pe_ensemble* p, uint32_t member_id, uint64_t seed, const double* const* fields,
+                  uint32_t max_cells) {
+    return eguard(p, [&](Ensemble& e) { e.set_volume(member_id, seed, fields, max_cells); });
+}
+
+int pe_update_phenotype(pe_ensemble* p, uint32_t member_id, double dt, int use_ratio, const uint32_t* mask,
+                        const uint32_t* parents, uint32_t n_parents, const uint32_t* removed, uint32_t n_removed,
+                        uint32_t* n_out) {
+    if ((n_parents && !parents) || (n_removed && !removed)) return HC_ERR_ARG;
+    return eguard(p, [&](Ensemble& e) {
+        const uint32_t n = e.update_phenotype(member_id, dt, use_ratio, mask, parents, n_parents, removed, n_removed);
+        if (n_out) *n_out = n;
     });
 }
 

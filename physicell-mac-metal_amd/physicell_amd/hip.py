@@ -183,7 +183,9 @@ def load(path=LIB_PATH):
         "pe_last_error": (ctypes.c_char_p, [c_vp]),
         "pe_add": (c_i32, [c_vp, P(Config), c_u32] + [c_vp] * 13 + [P(c_u32)]),
         "pe_remove": (c_i32, [c_vp, c_u32]),
-        "pe_set_motility": (c_i32, [c_vp, c_u32, ctypes.c_uint64] + [c_vp] * 8), "pe_run_periods": (c_i32, [c_vp, c_i32, c_i32]),
+        "pe_set_motility": (c_i32, [c_vp, c_u32, ctypes.c_uint64] + [c_vp] * 8),
+        "pe_set_volume": (c_i32, [c_vp, c_u32, ctypes.c_uint64, c_vp, c_u32]),
+        "pe_update_phenotype": (c_i32, [c_vp, c_u32, c_f64, c_i32, c_vp, c_vp, c_u32, c_vp, c_u32, c_vp]), "pe_run_periods": (c_i32, [c_vp, c_i32, c_i32]),
         "pe_wait": (c_i32, [c_vp]), "pe_get_density": (c_i32, [c_vp, c_u32, c_vp]),
         "pe_get_field": (c_i32, [c_vp, c_u32, c_u32, c_vp]),
         "pe_member_info": (c_i32, [c_vp, c_u32, P(ctypes.c_uint64), c_vp]),
@@ -556,6 +558,26 @@ class Ensemble:
         arrs = [None if a is None else np.ascontiguousarray(np.asarray(a, t).reshape(-1)) for a, t in conv]
         self._keep_mot = getattr(self, "_keep_mot", []) + [arrs]
         self._check(load().pe_set_motility(self.h, mid, seed, *[_ptr(a) for a in arrs]), "pe_set_motility")
+
+    def set_volume(self, mid, seed, fields=None, max_cells=0):
+        """pe_set_volume: Simulation.set_volume for one member, with room for
+        max_cells cells."""
+        fields = fields or {}
+        arrs = [None if f not in fields else np.ascontiguousarray(fields[f], np.float64) for f in range(NUM_V_FIELDS)]
+        table = (c_vp * NUM_V_FIELDS)(*[None if a is None else a.ctypes.data for a in arrs])
+        self._check(load().pe_set_volume(self.h, mid, seed, table, max_cells), "pe_set_volume")
+
+    def update_phenotype(self, mid, dt, mask=None, parents=(), removed=(), use_ratio=True):
+        """pe_update_phenotype: Simulation.update_phenotype for one member;
+        returns its new cell count."""
+        m = None if mask is None else np.ascontiguousarray(mask, np.uint32)
+        pa = np.ascontiguousarray(parents, np.uint32)
+        rm = np.ascontiguousarray(removed, np.uint32)
+        out = ctypes.c_uint32()
+        self._check(load().pe_update_phenotype(self.h, mid, dt, 1 if use_ratio else 0, _ptr(m), _ptr(pa), len(pa),
+                                               _ptr(rm), len(rm), ctypes.byref(out)), "pe_update_phenotype")
+        self.ncells[mid] = out.value
+        return out.value
 
     def run_periods(self, periods, use_graph=True):
         self._check(load().pe_run_periods(self.h, periods, 1 if use_graph else 0), "pe_run_periods")

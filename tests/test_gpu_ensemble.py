@@ -12,6 +12,7 @@ pytestmark = pytest.mark.gpu
 
 POS = (hip.F_POS_X, hip.F_POS_Y, hip.F_POS_Z)
 VEL = (hip.F_VEL_X, hip.F_VEL_Y, hip.F_VEL_Z)
+VOLUMES = (hip.F_RADIUS, hip.F_NUCLEAR_RAD, hip.F_TOTAL_VOL, hip.F_NUCLEAR_VOL)
 
 
 def small_member(seed, dt_pheno=0.5, **kw):
@@ -62,10 +63,13 @@ def member_state(ens, mid):
 
 
 def assert_same(a, b, what):
+    bad = []
     for key in a:
         if not np.array_equal(a[key], b[key]):
-            d = np.max(np.abs(a[key] - b[key]))
-            raise AssertionError(f"{what}: {key} differs (max abs {d:.3e})")
+            diff = np.abs(a[key] - b[key])
+            bad.append(f"{key} (max abs {np.max(diff):.3e} at {int(np.argmax(diff))}, {int(np.count_nonzero(diff))} values)")
+    if bad:
+        raise AssertionError(f"{what}: " + "; ".join(bad) + " differ")
 
 
 @pytest.mark.parametrize("mixed", [False, True])
@@ -138,6 +142,62 @@ def test_member_motility_equals_single(mixed):
     # motility moves the cells: the member without it differs from one with it
     assert not np.array_equal(member_state(ens, ids[0])[hip.F_POS_X],
                               run_single(*members[0], P, mixed)[hip.F_POS_X])
+    ens.close()
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_member_phenotype_equals_single(mixed):
+    """pe_set_volume / pe_update_phenotype: the volume model, divisions (into
+    room made by pe_set_volume) and removals of one member, with motility on
+    it, between periods; bit for bit against pc_sim driven the same way."""
+    members = [small_member(250), small_member(251), small_member(252)]
+    sc, c = members[1]
+    n, ns = c.n, len(sc.substrates)
+    M = n + 64
+    mot = (77, motility_arrays(n, ns, 350, sensitivities=False))
+    r = np.random.default_rng(450)
+    fields = {hip.V_TARGET_SOLID_NUC: r.uniform(100, 200, n), hip.V_FLUID: r.uniform(1500, 2500, n)}
+    mask = (r.uniform(size=n) < 0.7).astype(np.uint32)
+    parents = r.choice(n, 40, replace=False)
+    removed = r.choice(n + 40, 25, replace=False)
+    k = substeps_until(sc.dt_diff, sc.dt_mech)
+    kp = substeps_until(sc.dt_diff, sc.dt_pheno)
+    P1, P2 = 3, 3
+
+    ens = hip.Ensemble(mixed=mixed)
+    ids = [ens.add(s_, c_) for s_, c_ in members]
+    ens.set_motility(ids[1], mot[0], **mot[1])
+    ens.set_volume(ids[1], 9, fields, max_cells=M)
+    ens.run_periods(P1, use_graph=True)
+    n_ens = ens.update_phenotype(ids[1], 0.1, mask=mask, parents=parents, removed=removed)
+    ens.run_periods(P2, use_graph=True)
+    ens.wait()
+
+    sim = hip.Simulation(sc, c, max_cells=M, mixed=mixed)
+    sim.set_motility(mot[0], **mot[1])
+    sim.set_volume(9, fields)
+    for p in range(P1 + P2):
+        if p == P1:
+            n_single = sim.update_phenotype(0.1, mask=mask, parents=parents, removed=removed)
+        for _ in range(k):
+            sim.substep()
+        sim.mech_step()
+        if ((p + 1) * k) % kp == 0:
+            sim.pheno_step()
+    sim.wait()
+    want = {"rho": sim.density()}
+    for f in POS + VEL + VOLUMES:
+        want[f] = sim.field(f)
+    sim.close()
+    print(f"phenotype member: n {n} -> {n_ens} (single {n_single}), mixed={mixed}", flush=True)
+    assert n_ens == n_single == n + 40 - 25
+    got = member_state(ens, ids[1])
+    for f in VOLUMES:
+        got[f] = ens.field(ids[1], f)
+    assert_same(got, want, "phenotype member")
+    # the other members are untouched by the resize and the phenotype step
+    for j in (0, 2):
+        assert_same(member_state(ens, ids[j]), run_single(*members[j], P1 + P2, mixed), f"member {ids[j]}")
     ens.close()
 
 
