@@ -251,9 +251,14 @@ int hc_neighbor_lists(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* m
  *     when volume_j >= 1e-15, 2-D distance^2 <= (2.5 r_i + r_j)^2 and
  *     distance <= 1.25 (r_i + r_j); visiting order kept.  Which cells query
  *     (death model, interaction rates) is the caller's.
+ *   HC_QUERY_ATTACH: the pair scan of tryFormAllAttachments
+ *     (src/cell_interactions.cpp:411-478, an O(N^2) loop there): for live
+ *     cells, the live cells j > i with sqrt(|p_j - p_i|^2) <= max(rel_i,
+ *     rel_j) (r_i + r_j), ascending j.  Rates, capacity and draws are the
+ *     caller's (pc_sim_form_attachments).  Same reach check as CONTACT.
  * idx may be NULL: then only offsets and *total are produced (size the idx
  * buffer and call again); an idx buffer smaller than *total fails. */
-enum { HC_QUERY_CONTACT = 0, HC_QUERY_INTERACT = 1 };
+enum { HC_QUERY_CONTACT = 0, HC_QUERY_INTERACT = 1, HC_QUERY_ATTACH = 2 };
 int hc_neighbor_query(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech, uint32_t n, uint32_t max_cells,
                       uint32_t n_mvox, int kind, hc_buf* offsets, hc_buf* idx, uint64_t* total);
 
@@ -352,6 +357,19 @@ int pc_sim_update_phenotype(pc_sim* sim, double dt, int use_ratio, const uint32_
  * *total, idx[*total] (host arrays).  *total is always set. */
 int pc_sim_neighbor_query(pc_sim* sim, int kind, uint32_t* offsets, uint32_t* idx, uint64_t capacity,
                           uint64_t* total);
+/* tryFormAllAttachments (src/cell_interactions.cpp:459-478) with the pair
+ * scan on the device (HC_QUERY_ATTACH over the hash built from the current
+ * positions) and the reference's greedy pass on the host: for i ascending
+ * with rate[i] > 0 and count[i] < 12, its candidates j ascending with
+ * rate[j] > 0; skip when either list is full or j is already i's partner,
+ * else draw u and attach both ways when u < max(rate_i, rate_j) dt.  The
+ * draws come from std::mt19937 seeded with seed (the reference's generator
+ * type, src/cell_interactions.cpp:293-298; pass a new seed per call).
+ * rate[n_cells], count[n_cells] and targets[n_cells * 12] are host arrays,
+ * count and targets updated in place; *formed (may be NULL) = the number of
+ * attachments made. */
+int pc_sim_form_attachments(pc_sim* sim, double dt, const double* rate, uint32_t* count, uint32_t* targets,
+                            uint64_t seed, uint32_t* formed);
 int pc_sim_wait(pc_sim* sim);
 /* Copies out (waits first).  density: ns*nx*ny*nz FP64 (converted from FP32
  * in the mixed form). */

@@ -2,7 +2,8 @@
 # Builds oracle/_ref/libref_path.so from the reference's own C++ sources where
 # they lie under /root/reference (nothing is copied into this repository):
 # the minimal set that links processSecretion and the .mat snapshot writer
-# (output_writer.cpp, matlab_writer.cpp) and updateMotility (motility.cpp)
+# (output_writer.cpp, matlab_writer.cpp), updateMotility (motility.cpp) and
+# tryFormAllAttachments (cell_interactions.cpp)
 # with no undefined symbols.
 # Test infrastructure only.  Exits 0 without building when the reference is
 # absent (the GPU box has no /root/reference).
@@ -21,7 +22,7 @@ g++ -std=c++17 -O2 -fPIC -shared -Wl,--no-undefined -w \
   "$REF/src/cell_data.cpp" "$REF/src/cell_phenotype.cpp" \
   "$REF/src/cell_definitions.cpp" "$REF/src/config_parser.cpp" \
   "$REF/src/output_writer.cpp" "$REF/src/matlab_writer.cpp" \
-  "$REF/src/motility.cpp" \
+  "$REF/src/motility.cpp" "$REF/src/cell_interactions.cpp" \
   "$REF/lib/pugixml/pugixml.cpp" \
   -o "$OUT/libref_path.so.tmp"
 mv "$OUT/libref_path.so.tmp" "$OUT/libref_path.so"

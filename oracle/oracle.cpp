@@ -12,6 +12,7 @@
 #include <cmath>
 #include <functional>
 #include <cstring>
+#include <random>
 #include <vector>
 #include <omp.h>
 
@@ -726,6 +727,41 @@ uint64_t or_contact_graph(const double* x, const double* y, const double* z, con
     }
     offsets[n] = (uint32_t)tot;
     return tot;
+}
+
+uint32_t or_form_attachments(const double* x, const double* y, const double* z, const double* radius,
+                             const double* rel, const uint32_t* alive, const double* rate, uint32_t n, double dt,
+                             uint64_t seed, uint32_t* count, uint32_t* targets) {
+    // tryFormAllAttachments (src/cell_interactions.cpp:459-478) calling
+    // tryFormAttachment (:411-455) on every pair j > i in order; the draw is
+    // a std::mt19937 double in [0, 1) (:293-298), seeded here
+    std::mt19937 gen((std::mt19937::result_type)seed);
+    std::uniform_real_distribution<double> uni(0.0, 1.0);
+    uint32_t formed = 0;
+    const uint32_t M = OR_MAX_ATTACHMENTS;
+    for (uint32_t i = 0; i < n; i++) {
+        if (!alive[i] || rate[i] <= 0.0 || count[i] >= M) continue;
+        for (uint32_t j = i + 1; j < n; j++) {
+            if (!alive[j] || rate[j] <= 0.0) continue;
+            // tryFormAttachment: capacity, already attached, distance, draw
+            if (count[i] < M && count[j] < M) {
+                bool have = false;
+                for (uint32_t a = 0; a < count[i]; a++) have = have || targets[(size_t)i * M + a] == j;
+                if (!have) {
+                    const double dx = x[j] - x[i], dy = y[j] - y[i], dz = z[j] - z[i];
+                    const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+                    const double md = std::max(rel[i], rel[j]) * (radius[i] + radius[j]);
+                    if (!(dist > md) && uni(gen) < std::max(rate[i], rate[j]) * dt) {
+                        targets[(size_t)i * M + count[i]++] = j;
+                        targets[(size_t)j * M + count[j]++] = i;
+                        formed++;
+                    }
+                }
+            }
+            if (count[i] >= M) break;
+        }
+    }
+    return formed;
 }
 
 uint64_t or_interaction_candidates(const double* x, const double* y, const double* radius, const double* volume,

@@ -198,6 +198,18 @@ uint32_t or_remove(or_cells* c, const uint32_t* indices, uint32_t count, uint32_
 uint64_t or_contact_graph(const double* x, const double* y, const double* z, const double* radius,
                           const double* volume, const double* rel, const uint32_t* alive, uint32_t n,
                           uint32_t* offsets, uint32_t* idx);
+/* tryFormAllAttachments (src/cell_interactions.cpp:411-478), the O(N^2)
+ * pair loop: for live i with rate > 0 and fewer than OR_MAX_ATTACHMENTS
+ * partners, every live j > i with rate > 0, ascending: skip when either is
+ * full or j is already i's partner; else when |d| <= max(rel_i, rel_j)
+ * (r_i + r_j) draw u (std::mt19937 seeded with seed, uniform [0, 1)) and
+ * attach both ways when u < max(rate_i, rate_j) dt.  count[n] and
+ * targets[n * OR_MAX_ATTACHMENTS] are updated in place; returns the number
+ * of attachments formed. */
+#define OR_MAX_ATTACHMENTS 12u
+uint32_t or_form_attachments(const double* x, const double* y, const double* z, const double* radius,
+                             const double* rel, const uint32_t* alive, const double* rate, uint32_t n, double dt,
+                             uint64_t seed, uint32_t* count, uint32_t* targets);
 /* The candidate scan of standardCellCellInteractions (src/cell_phenotype.cpp:
  * 893-967) over the hash (keys, off, srt of or_mech_keys / or_hash): the
  * first 512 cells j != i of the 3x3x3 bins around key[i] in visiting order,

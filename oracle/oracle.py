@@ -339,6 +339,23 @@ def contact_graph(x, y, z, radius, volume, rel, alive):
     return off, idx[:tot]
 
 
+MAX_ATTACHMENTS = 12
+
+
+def form_attachments(x, y, z, radius, rel, alive, rate, dt, seed, count=None, targets=None):
+    """or_form_attachments (src/cell_interactions.cpp:411-478): (count,
+    targets, formed), updated from the given state (default: none)."""
+    n = len(x)
+    count = np.zeros(n, np.uint32) if count is None else np.array(count, np.uint32)
+    targets = (np.zeros(max(1, n) * MAX_ATTACHMENTS, np.uint32) if targets is None
+               else np.array(targets, np.uint32).reshape(-1))
+    f = lib().or_form_attachments
+    f.restype = ctypes.c_uint32
+    formed = f(P(x), P(y), P(z), P(radius), P(rel), P(alive, u32p), P(np.asarray(rate, np.float64)),
+               ctypes.c_uint32(n), ctypes.c_double(dt), ctypes.c_uint64(seed), P(count, u32p), P(targets, u32p))
+    return count, targets[:n * MAX_ATTACHMENTS].reshape(n, MAX_ATTACHMENTS), formed
+
+
 def interaction_candidates(x, y, radius, volume, keys, off, srt, m):
     """or_interaction_candidates (src/cell_phenotype.cpp:893-967): (offsets, idx)."""
     n = len(x)

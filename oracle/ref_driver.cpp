@@ -1,7 +1,8 @@
 // ref_driver.cpp -- thin C entry points into the reference's own compiled
 // processSecretion (src/cell_phenotype.cpp:526-564), updateMotility
 // (src/motility.cpp:179-293), updateVolume / divideCell / removeCell
-// (src/cell_phenotype.cpp, src/cell_data.cpp) and .mat writers
+// (src/cell_phenotype.cpp, src/cell_data.cpp), tryFormAllAttachments
+// (src/cell_interactions.cpp:459-478) and .mat writers
 // (src/matlab_writer.cpp, src/output_writer.cpp:295-453), used only by tests
 // to pin the FP64 oracle's secretion restatement and the library's snapshot
 // writer against the reference code.
@@ -9,6 +10,7 @@
 // TEST INFRASTRUCTURE ONLY.  Built by oracle/build_ref.sh into oracle/_ref/
 // from the reference sources where they lie; nothing here is product code.
 #include "cell_data.h"
+#include "cell_interactions.h"
 #include "cell_phenotype.h"
 #include "types.h"
 
@@ -250,5 +252,36 @@ extern "C" int ref_write_neighbor_graph(const char* folder, uint32_t n, const fl
         cells.is_alive[i] = alive[i];
     }
     writeCellNeighborGraph(cells, folder);
+    return 0;
+}
+
+// The reference's tryFormAllAttachments (src/cell_interactions.cpp:459-478)
+// on n cells (FP32 positions, radii, adhesion scales and rates, as its store
+// holds them) from empty attachment lists; count[n] and targets[n * 12] out.
+// Its draws come from an unseeded generator, so only rates with rate * dt >= 1
+// (every draw accepted) give a repeatable result.
+extern "C" int ref_form_attachments(uint32_t n, const float* x, const float* y, const float* z,
+                                    const float* radius, const float* rel, const uint32_t* alive,
+                                    const float* rate, double dt, uint32_t* count, uint32_t* targets) {
+    CellData cells(n > 0 ? n : 1);
+    std::vector<float> gpu(cells.gpuBufferSize() / sizeof(float), 0.0f);
+    cells.setGPUBufferBase(gpu.data());
+    cells.allocateCPUBuffers();                 // the attachment lists (src/main.mm:170)
+    for (uint32_t i = 0; i < n; i++) cells.addCell();
+    for (uint32_t i = 0; i < n; i++) {
+        cells.position_x[i] = x[i]; cells.position_y[i] = y[i]; cells.position_z[i] = z[i];
+        cells.radius[i] = radius[i];
+        cells.relative_max_adhesion_distance[i] = rel[i];
+        cells.is_alive[i] = alive[i];
+        cells.attachment_rate[i] = rate[i];
+        cells.attachment_count[i] = 0;
+    }
+    tryFormAllAttachments(cells, dt);
+    for (uint32_t i = 0; i < n; i++) {
+        count[i] = cells.attachment_count[i];
+        for (uint32_t a = 0; a < CellData::MAX_ATTACHMENTS; a++)
+            targets[(size_t)i * CellData::MAX_ATTACHMENTS + a] =
+                a < count[i] ? cells.attachment_targets[(size_t)i * CellData::MAX_ATTACHMENTS + a] : 0u;
+    }
     return 0;
 }

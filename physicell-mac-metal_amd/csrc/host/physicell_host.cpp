@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <random>
 
 // the context's stream (internal export of context.hip; not part of the C-ABI header)
 extern "C" hipStream_t hc_stream_of(hc_ctx* ctx);
@@ -364,6 +365,20 @@ uint64_t CellMechanics::neighborQuery(CellData& cells, int kind, uint32_t* offse
     return total;
 }
 
+uint64_t CellMechanics::neighborQuery(CellData& cells, int kind, std::vector<uint32_t>& offsets,
+                                      std::vector<uint32_t>& idx) {
+    offsets.assign((size_t)cells.num_cells + 1, 0u);
+    const uint64_t total = neighborQuery(cells, kind, offsets.data(), nullptr, 0);
+    idx.resize(total);
+    if (total) {
+        // the fill pass above left the lists in q_idx_
+        q_idx_.download(0, total * 4);
+        check(ctx_, hc_wait(ctx_), "wait");
+        std::memcpy(idx.data(), q_idx_.host(), total * 4);
+    }
+    return total;
+}
+
 // ------------------------------------------------------------------ Simulation
 static hc_ctx* open_context(int device) {
     hc_ctx* ctx = nullptr;
@@ -547,6 +562,36 @@ uint32_t Simulation::updatePhenotypeSoA(double dt, int use_ratio, const uint32_t
         graph_exec_ = nullptr;
     }
     return n2;
+}
+
+uint32_t Simulation::formAttachments(double dt, const double* rate, uint32_t* count, uint32_t* targets,
+                                     uint64_t seed) {
+    constexpr uint32_t M = 12;    // CellData::MAX_ATTACHMENTS (src/cell_data.h:124)
+    const uint32_t n = cells.num_cells;
+    std::vector<uint32_t> off, idx;
+    mechanics.neighborQuery(cells, HC_QUERY_ATTACH, off, idx);
+    std::mt19937 gen((std::mt19937::result_type)seed);
+    std::uniform_real_distribution<double> uni(0.0, 1.0);
+    uint32_t formed = 0;
+    // the reference's i / j loops (src/cell_interactions.cpp:463-477), j over
+    // the candidates only: every other j fails the distance test, which comes
+    // before the draw in tryFormAttachment (:411-455)
+    for (uint32_t i = 0; i < n; i++) {
+        if (!(rate[i] > 0.0) || count[i] >= M) continue;
+        uint32_t* ti = targets + (size_t)i * M;
+        for (uint32_t u = off[i]; u < off[i + 1]; u++) {
+            const uint32_t j = idx[u];
+            if (!(rate[j] > 0.0)) continue;
+            if (count[i] < M && count[j] < M && std::find(ti, ti + count[i], j) == ti + count[i] &&
+                uni(gen) < std::max(rate[i], rate[j]) * dt) {
+                ti[count[i]++] = j;
+                targets[(size_t)j * M + count[j]++] = i;
+                formed++;
+            }
+            if (count[i] >= M) break;
+        }
+    }
+    return formed;
 }
 
 void Simulation::phenotypeStep() {
@@ -808,6 +853,15 @@ int pc_sim_neighbor_query(pc_sim* s, int kind, uint32_t* offsets, uint32_t* idx,
     if (!total) return HC_ERR_ARG;
     return guarded(s, [&](Simulation& m) {
         *total = m.mechanics.neighborQuery(m.cells, kind, offsets, idx, capacity);
+    });
+}
+
+int pc_sim_form_attachments(pc_sim* s, double dt, const double* rate, uint32_t* count, uint32_t* targets,
+                            uint64_t seed, uint32_t* formed) {
+    if (!rate || !count || !targets) return HC_ERR_ARG;
+    return guarded(s, [&](Simulation& m) {
+        const uint32_t f = m.formAttachments(dt, rate, count, targets, seed);
+        if (formed) *formed = f;
     });
 }
 

@@ -172,6 +172,8 @@ public:
     // HC_QUERY_*); copies out offsets[num_cells + 1] and, when idx is not null
     // and capacity suffices, the lists.  Returns the number of entries.
     uint64_t neighborQuery(CellData& cells, int kind, uint32_t* offsets, uint32_t* idx, uint64_t capacity);
+    // the same into host vectors sized to the result
+    uint64_t neighborQuery(CellData& cells, int kind, std::vector<uint32_t>& offsets, std::vector<uint32_t>& idx);
 
 private:
     hc_ctx* ctx_ = nullptr;
@@ -220,6 +222,12 @@ public:
     // new cell count.
     uint32_t updatePhenotypeSoA(double dt, int use_ratio, const uint32_t* mask, const uint32_t* parents,
                                 uint32_t n_parents, const uint32_t* removed, uint32_t n_removed);
+    // tryFormAllAttachments (src/cell_interactions.cpp:459-478): candidate
+    // pairs from the device hash (HC_QUERY_ATTACH), then the reference's
+    // greedy pass in its order on the host arrays rate / count / targets
+    // (num_cells, num_cells, num_cells x 12), drawing from std::mt19937(seed).
+    // Returns the number of attachments formed.
+    uint32_t formAttachments(double dt, const double* rate, uint32_t* count, uint32_t* targets, uint64_t seed);
     void wait();
     int substepsPerMechanicsStep() const;
 

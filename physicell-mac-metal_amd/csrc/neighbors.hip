@@ -1,16 +1,17 @@
 // neighbors.hip -- neighbour queries on the device hash for the phenotype
 // side: the contact graph of writeCellNeighborGraph
-// (src/output_writer.cpp:457-490, an O(N^2) pair loop in the reference) and
-// the interaction candidates of standardCellCellInteractions
+// (src/output_writer.cpp:457-490, an O(N^2) pair loop in the reference), the
+// interaction candidates of standardCellCellInteractions
 // (src/cell_phenotype.cpp:858-968, a Moore-neighbourhood scan of the
-// mechanics hash).
+// mechanics hash) and the attachment candidates of tryFormAllAttachments
+// (src/cell_interactions.cpp:411-478, another O(N^2) pair loop).
 //
 // Both visit the 27 (9 in 2-D) bins around a cell through the hash's
 // offsets and its ascending per-bin cell list, and read the current positions
 // and radii from the cell SoA in FP64.  Two passes give a CSR result: count
-// per cell, exclusive scan, fill.  Contact lists are sorted by neighbour
-// index (the order of the reference's j loop); interaction lists keep the
-// visiting order (dk, dj, di, then ascending index).
+// per cell, exclusive scan, fill.  Contact and attachment lists are sorted
+// by neighbour index (the order of the reference's j loops); interaction
+// lists keep the visiting order (dk, dj, di, then ascending index).
 #include "cells_common.h"
 
 #include <algorithm>
@@ -31,6 +32,9 @@ struct QueryArgs {
 // Moore neighbourhood in visiting order, then volume_j >= 1e-15, 2-D
 // distance^2 <= (2.5 r_i + r_j)^2 and distance <= 1.25 (r_i + r_j)
 // (src/cell_phenotype.cpp:893-967)
+// attach: j > i, both alive, sqrt(|d|^2) <= max(rel_i, rel_j) (r_i + r_j)
+// with d = p_j - p_i (tryFormAttachment's distance test,
+// src/cell_interactions.cpp:429-440); rates, capacity and draws are the host's
 template <int KIND, bool FILL>
 __global__ __launch_bounds__(256) void k_query(QueryArgs a, uint32_t* __restrict__ counts,
                                                const uint32_t* __restrict__ offsets, uint32_t* __restrict__ idx,
@@ -40,7 +44,7 @@ __global__ __launch_bounds__(256) void k_query(QueryArgs a, uint32_t* __restrict
     const uint32_t i = valid ? a.sorted[t] : 0u;
     const double* const* f = a.cp.f;
     const uint32_t* alive = a.cp.u[HC_F_IS_ALIVE - HC_NUM_F64_FIELDS];
-    if (!FILL && KIND == 0 && maxes) {
+    if (!FILL && KIND != 1 && maxes) {
         // largest radius and adhesion scale over the wave, then one atomic
         // per wave: the host checks that every contact distance fits in one
         // mechanics voxel (non-negative doubles compare as their bit patterns)
@@ -62,7 +66,7 @@ __global__ __launch_bounds__(256) void k_query(QueryArgs a, uint32_t* __restrict
     const double reli = f[HC_F_REL_MAX_ADH][i];
     uint32_t cnt = 0;
     const uint32_t base = FILL ? offsets[i] : 0u;
-    if (KIND == 0 && !(alive[i] && f[HC_F_TOTAL_VOL][i] > 1e-15)) {
+    if ((KIND == 0 && !(alive[i] && f[HC_F_TOTAL_VOL][i] > 1e-15)) || (KIND == 2 && !alive[i])) {
         if (!FILL) counts[i] = 0;
         return;
     }
@@ -90,6 +94,13 @@ __global__ __launch_bounds__(256) void k_query(QueryArgs a, uint32_t* __restrict
                         const double relj = f[HC_F_REL_MAX_ADH][j];
                         const double md = (reli > relj ? reli : relj) * (ri + f[HC_F_RADIUS][j]);
                         take = dx * dx + dy * dy + dz * dz <= md * md;
+                    } else if (KIND == 2) {
+                        if (j <= i || !alive[j]) continue;
+                        const double dx = f[HC_F_POS_X][j] - xi, dy = f[HC_F_POS_Y][j] - yi,
+                                     dz = f[HC_F_POS_Z][j] - zi;
+                        const double relj = f[HC_F_REL_MAX_ADH][j];
+                        const double md = (reli < relj ? relj : reli) * (ri + f[HC_F_RADIUS][j]);
+                        take = !(sqrt(dx * dx + dy * dy + dz * dz) > md);
                     } else {
                         if (j == i || j >= a.n) continue;
                         if (visited >= 512u) continue;                  // the reference's MAX_NEIGHBORS
@@ -110,7 +121,7 @@ __global__ __launch_bounds__(256) void k_query(QueryArgs a, uint32_t* __restrict
     }
     if (!FILL) {
         counts[i] = cnt;
-    } else if (KIND == 0) {
+    } else if (KIND != 1) {
         // ascending neighbour index (insertion sort of this cell's short list)
         for (uint32_t p = 1; p < cnt; p++) {
             const uint32_t v = idx[base + p];
@@ -127,7 +138,8 @@ __global__ __launch_bounds__(256) void k_query(QueryArgs a, uint32_t* __restrict
 int probe_kernels_neighbors(hc_ctx* ctx) {
     hipFuncAttributes attr;
     const void* fns[] = {(const void*)k_query<0, false>, (const void*)k_query<0, true>,
-                         (const void*)k_query<1, false>, (const void*)k_query<1, true>};
+                         (const void*)k_query<1, false>, (const void*)k_query<1, true>,
+                         (const void*)k_query<2, false>, (const void*)k_query<2, true>};
     for (const void* fn : fns) {
         hipError_t e = hipFuncGetAttributes(&attr, fn);
         if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: neighbour query kernel");
@@ -145,7 +157,7 @@ int hc_neighbor_query(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* m
                       uint32_t n_mvox, int kind, hc_buf* offsets, hc_buf* idx, uint64_t* total) {
     if (!ctx || !cells || !hash || !mech || !offsets || !total)
         return fail(ctx, HC_ERR_ARG, "hc_neighbor_query: null argument");
-    if (kind != HC_QUERY_CONTACT && kind != HC_QUERY_INTERACT)
+    if (kind != HC_QUERY_CONTACT && kind != HC_QUERY_INTERACT && kind != HC_QUERY_ATTACH)
         return fail(ctx, HC_ERR_ARG, "hc_neighbor_query: unknown kind");
     if (n > max_cells || hc_buf_size(cells) < hc_cells_bytes(max_cells))
         return fail(ctx, HC_ERR_ARG, "hc_neighbor_query: cell buffer");
@@ -183,6 +195,9 @@ int hc_neighbor_query(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* m
         if (kind == HC_QUERY_CONTACT)
             hipLaunchKernelGGL((k_query<0, false>), dim3(nb), dim3(256), 0, ctx->stream, a, off, nullptr, nullptr,
                                maxes);
+        else if (kind == HC_QUERY_ATTACH)
+            hipLaunchKernelGGL((k_query<2, false>), dim3(nb), dim3(256), 0, ctx->stream, a, off, nullptr, nullptr,
+                               maxes);
         else
             hipLaunchKernelGGL((k_query<1, false>), dim3(nb), dim3(256), 0, ctx->stream, a, off, nullptr, nullptr,
                                nullptr);
@@ -195,13 +210,13 @@ int hc_neighbor_query(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* m
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "hc_neighbor_query: read back");
     *total = host[0];
-    if (kind == HC_QUERY_CONTACT && n) {
+    if (kind != HC_QUERY_INTERACT && n) {
         double rmax, relmax;
         std::memcpy(&rmax, host + 1, 8);
         std::memcpy(&relmax, host + 3, 8);
         const double reach = (relmax > 0.0 ? relmax : 0.0) * 2.0 * rmax;
         if (reach > mech->mechanics_voxel_size)
-            return fail(ctx, HC_ERR_CAPACITY, "hc_neighbor_query: a contact distance (max scale x 2 max radius = " +
+            return fail(ctx, HC_ERR_CAPACITY, "hc_neighbor_query: a contact or attachment distance (max scale x 2 max radius = " +
                                                   std::to_string(reach) + ") exceeds the mechanics voxel");
     }
     if (!idx) return 0;
@@ -210,6 +225,9 @@ int hc_neighbor_query(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* m
     if (n && host[0]) {
         if (kind == HC_QUERY_CONTACT)
             hipLaunchKernelGGL((k_query<0, true>), dim3(nb), dim3(256), 0, ctx->stream, a, nullptr, off,
+                               (uint32_t*)idx->dptr, nullptr);
+        else if (kind == HC_QUERY_ATTACH)
+            hipLaunchKernelGGL((k_query<2, true>), dim3(nb), dim3(256), 0, ctx->stream, a, nullptr, off,
                                (uint32_t*)idx->dptr, nullptr);
         else
             hipLaunchKernelGGL((k_query<1, true>), dim3(nb), dim3(256), 0, ctx->stream, a, nullptr, off,

@@ -47,7 +47,8 @@ KERNEL_NAMES = ["sweep_x", "sweep_y", "sweep_z", "sweep_small", "dirichlet", "vo
  V_TARGET_FLUID_FRAC, V_TARGET_CN_RATIO, V_FLUID_RATE, V_NUC_RATE, V_CYTO_RATE, V_CALC_RATE, V_RUPTURE_VOL,
  V_REL_RUPTURE_VOL) = range(15)
 NUM_V_FIELDS = 15
-QUERY_CONTACT, QUERY_INTERACT = 0, 1
+QUERY_CONTACT, QUERY_INTERACT, QUERY_ATTACH = 0, 1, 2
+MAX_ATTACHMENTS = 12
 
 
 class HipError(RuntimeError):
@@ -173,6 +174,7 @@ def load(path=LIB_PATH):
         "pc_sim_set_motility": (c_i32, [c_vp, ctypes.c_uint64] + [c_vp] * 8),
         "pc_sim_set_volume": (c_i32, [c_vp, ctypes.c_uint64, c_vp]),
         "pc_sim_neighbor_query": (c_i32, [c_vp, c_i32, c_vp, c_vp, ctypes.c_uint64, c_vp]),
+        "pc_sim_form_attachments": (c_i32, [c_vp, ctypes.c_double, c_vp, c_vp, c_vp, ctypes.c_uint64, c_vp]),
         "hc_neighbor_query": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_u32, c_u32, c_u32, c_i32, c_vp, c_vp, c_vp]),
         "pc_write_neighbor_graph": (c_i32, [ctypes.c_char_p, c_u32, c_vp, c_vp]),
         "pc_sim_update_phenotype": (c_i32, [c_vp, c_f64, c_i32, c_vp, c_vp, c_u32, c_vp, c_u32, c_vp]),
@@ -447,8 +449,8 @@ class Simulation:
         return self.n
 
     def neighbor_query(self, kind):
-        """pc_sim_neighbor_query: (offsets[n + 1], idx) of QUERY_CONTACT or
-        QUERY_INTERACT lists over the current positions."""
+        """pc_sim_neighbor_query: (offsets[n + 1], idx) of QUERY_CONTACT,
+        QUERY_INTERACT or QUERY_ATTACH lists over the current positions."""
         off = np.zeros(self.n + 1, np.uint32)
         total = ctypes.c_uint64()
         self._check(load().pc_sim_neighbor_query(self.h, kind, _ptr(off), None, 0, ctypes.byref(total)),
@@ -457,6 +459,22 @@ class Simulation:
         self._check(load().pc_sim_neighbor_query(self.h, kind, _ptr(off), _ptr(idx), total.value,
                                                  ctypes.byref(total)), "pc_sim_neighbor_query")
         return off, idx[:total.value]
+
+    def form_attachments(self, dt, rate, seed, count=None, targets=None):
+        """pc_sim_form_attachments (tryFormAllAttachments): (count[n],
+        targets[n, 12], formed), updated from the given lists (default:
+        none)."""
+        n = self.n
+        rate = np.ascontiguousarray(rate, np.float64)
+        if len(rate) != n:
+            raise ValueError("form_attachments: rate needs one value per cell")
+        count = np.zeros(n, np.uint32) if count is None else np.array(count, np.uint32)
+        targets = (np.zeros((n, MAX_ATTACHMENTS), np.uint32) if targets is None
+                   else np.array(targets, np.uint32).reshape(n, MAX_ATTACHMENTS))
+        formed = c_u32()
+        self._check(load().pc_sim_form_attachments(self.h, float(dt), _ptr(rate), _ptr(count), _ptr(targets),
+                                                   int(seed), ctypes.byref(formed)), "pc_sim_form_attachments")
+        return count, targets, formed.value
 
     def density(self):
         out = np.empty(self.ns * self.spec.n_voxels)

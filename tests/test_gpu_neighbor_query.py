@@ -65,11 +65,72 @@ def test_empty_population():
     """No cells: empty lists, and the phenotype step and queries accept it."""
     spec, cells = S.small3d(12, 10, 8, n_cells=0, seed=74)
     sim = H.Simulation(spec, cells, max_cells=16)
-    for kind in (H.QUERY_CONTACT, H.QUERY_INTERACT):
+    for kind in (H.QUERY_CONTACT, H.QUERY_INTERACT, H.QUERY_ATTACH):
         off, idx = sim.neighbor_query(kind)
         assert off.tolist() == [0] and len(idx) == 0
+    cnt, tg, formed = sim.form_attachments(1.0, np.zeros(0), seed=1)
+    assert formed == 0 and len(cnt) == 0
     sim.set_volume(seed=1)
     assert sim.update_phenotype(6.0) == 0
     sim.run_periods(1)
     sim.wait()
+    sim.close()
+
+
+def attach_candidates_bruteforce(a):
+    """All pairs j > i, both alive, sqrt(|p_j - p_i|^2) <= max(rel) (r_i + r_j)
+    (tryFormAttachment's test, src/cell_interactions.cpp:429-440)."""
+    x, y, z = a["x"], a["y"], a["z"]
+    n = len(x)
+    off = np.zeros(n + 1, np.uint32)
+    idx = []
+    for i in range(n):
+        off[i] = len(idx)
+        if not a["alive"][i]:
+            continue
+        j = np.arange(i + 1, n)
+        d = np.sqrt((x[j] - x[i]) ** 2 + (y[j] - y[i]) ** 2 + (z[j] - z[i]) ** 2)
+        md = np.maximum(a["rel"][i], a["rel"][j]) * (a["radius"][i] + a["radius"][j])
+        idx.extend(j[(a["alive"][j] == 1) & ~(d > md)].tolist())
+    off[n] = len(idx)
+    return off, np.array(idx, np.uint32)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("case", ["small3d", "config0", "dense"])
+def test_attachments_match_oracle(case, mixed):
+    """HC_QUERY_ATTACH against the all-pairs test, and pc_sim_form_attachments
+    (device candidates + host greedy pass) against or_form_attachments (the
+    reference's O(N^2) loop restated, same std::mt19937 draws), bit for bit,
+    over two calls (the second starts from the first's lists)."""
+    if case == "small3d":
+        spec, cells = S.small3d(30, 24, 20, n_cells=2500, seed=81)
+    elif case == "config0":
+        spec, cells = S.config0()
+    else:
+        spec, cells = S.small3d(12, 10, 8, n_cells=1500, seed=82)    # lists fill to 12
+    r = np.random.default_rng(83)
+    cells.alive = (r.uniform(size=cells.n) < 0.93).astype(np.uint32)
+    sim = H.Simulation(spec, cells, mixed=mixed)
+    sim.run_periods(2)
+    sim.wait()
+    a = sim_arrays(sim)
+    print(f"attachments: {case} n={cells.n} mixed={mixed}", flush=True)
+    off, idx = sim.neighbor_query(H.QUERY_ATTACH)
+    woff, widx = attach_candidates_bruteforce(a)
+    assert np.array_equal(off, woff) and np.array_equal(idx, widx)
+    rate = r.uniform(0.0, 0.2, cells.n)
+    rate[r.uniform(size=cells.n) < 0.1] = 0.0
+    if case == "dense":
+        rate[:] = np.where(rate > 0, 1.0, 0.0)          # every draw accepts
+    cnt, tg, formed = sim.form_attachments(6.0 if case == "dense" else 1.0, rate, seed=5)
+    wc, wt, wf = O.form_attachments(a["x"], a["y"], a["z"], a["radius"], a["rel"], a["alive"], rate,
+                                    6.0 if case == "dense" else 1.0, 5)
+    assert formed == wf > 0 and np.array_equal(cnt, wc) and np.array_equal(tg, wt)
+    if case == "dense":
+        assert cnt.max() == H.MAX_ATTACHMENTS
+    cnt2, tg2, f2 = sim.form_attachments(1.0, rate, seed=6, count=cnt, targets=tg)
+    wc2, wt2, wf2 = O.form_attachments(a["x"], a["y"], a["z"], a["radius"], a["rel"], a["alive"], rate, 1.0, 6,
+                                       count=wc, targets=wt)
+    assert f2 == wf2 and np.array_equal(cnt2, wc2) and np.array_equal(tg2, wt2)
     sim.close()
