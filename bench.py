@@ -284,6 +284,14 @@ def bench_rows(args):
             qoff, qidx = sim.neighbor_query(kind)
             ms = 1e3 * (time.perf_counter() - t1)
         queries[name] = {"ms": ms, "entries": int(len(qidx)), "cells": int(len(qoff) - 1)}
+    # attachments (tryFormAllAttachments): device pair scan + host greedy pass,
+    # rates U[0, 0.2] per minute over dt 1 min; the second call is timed
+    arate = r.uniform(0.0, 0.2, sim.n)
+    for k in range(2):
+        t1 = time.perf_counter()
+        acnt, _, aformed = sim.form_attachments(1.0, arate, seed=20 + k)
+        ms = 1e3 * (time.perf_counter() - t1)
+    queries["form_attachments"] = {"ms": ms, "formed": int(aformed), "cells": int(len(acnt))}
     # the oracle beside it (one host thread), same operations and sizes
     cpu = {}
     sample = 20000
@@ -293,6 +301,10 @@ def bench_rows(args):
     t1 = time.perf_counter()
     O.contact_graph(a["x"], a["y"], a["z"], a["r"], a["v"], a["rel"], np.ascontiguousarray(sim.field(H.F_IS_ALIVE)[:sample]))
     cpu["contact_graph_ms_20k_cells"] = 1e3 * (time.perf_counter() - t1)
+    t1 = time.perf_counter()
+    O.form_attachments(a["x"], a["y"], a["z"], a["r"], a["rel"], np.ascontiguousarray(sim.field(H.F_IS_ALIVE)[:sample]),
+                       np.ascontiguousarray(arate[:sample]), 1.0, 21)
+    cpu["form_attachments_ms_20k_cells"] = 1e3 * (time.perf_counter() - t1)
     x, y, z = (np.ascontiguousarray(np.asarray(a, np.float64)) for a in (cells.x, cells.y, cells.z))
     vel = [np.zeros(n) for _ in range(3)]
     mv = [np.zeros(n) for _ in range(3)]
@@ -333,7 +345,8 @@ def bench_rows(args):
         "cpu_baseline": {"kind": "port", "cores": 1, **cpu,
                          "sample": f"the FP64 oracle on this host, one thread: motility over {n} agents, volume "
                                    f"model over {n}, {n // 100} divisions, {n // 100} removals; the contact "
-                                   f"graph's O(N^2) loop (the reference's) over the first {sample} agents only"},
+                                   f"graph's and the attachments' O(N^2) loops (the reference's) over the first {sample} "
+                                   f"agents only"},
     }
     print(json.dumps(line))
 
