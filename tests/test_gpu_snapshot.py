@@ -32,3 +32,58 @@ def test_sim_snapshot_holds_state(tmp_path, builder):
     x0 = np.float32(spec.x_min) + (np.arange(spec.nx, dtype=np.float32) + np.float32(0.5)) * np.float32(spec.dx)
     assert np.array_equal(me[matio.ME_ROW_X][:spec.nx], x0.astype(np.float64))
     sim.close()
+
+
+def oracle_snapshot(folder, frame, spec, o):
+    """The oracle's state through the library's host writer (pc_write_snapshot)."""
+    import ctypes
+    a = o.arr
+    n, ns = len(a["x"]), len(spec.substrates)
+    cols = {"x": a["x"], "y": a["y"], "z": a["z"], "total_volume": a["volume"], "radius": a["radius"],
+            "vx": a["vx"], "vy": a["vy"], "vz": a["vz"], "pressure": a["pressure"], "adhesion": a["adh"],
+            "repulsion": a["rep"], "rel_max_adhesion": a["rel_adh"], "nuclear_volume": np.full(n, 540.0),
+            "nuclear_radius": np.full(n, np.cbrt(3 * 540.0 / (4 * np.pi))), "cell_type": np.zeros(n, np.uint32),
+            "current_phase": np.zeros(n, np.uint32), "is_alive": a["alive"], "secretion": a["sec"],
+            "uptake": a["upt"], "saturation": a["sat"], "net_export": a["net"]}
+    snap = H.Snapshot()
+    snap.n_cells, snap.n_substrates = n, ns
+    keep = []
+    for k, v in cols.items():
+        v = np.ascontiguousarray(v)
+        keep.append(v)
+        setattr(snap, k, v.ctypes.data)
+    snap.nx, snap.ny, snap.nz = spec.nx, spec.ny, spec.nz
+    for k in ("dx", "dy", "dz", "x_min", "y_min", "z_min"):
+        setattr(snap, k, float(getattr(spec, k)))
+    rho = np.ascontiguousarray(a["rho"])
+    snap.density = rho.ctypes.data
+    assert H.load().pc_write_snapshot(str(folder).encode(), frame, ctypes.byref(snap)) == 0
+
+
+def test_config0_snapshots_pass_the_parity_diff(tmp_path):
+    """Config 0 over its full 60 min (6001 substeps, 600 mechanics steps) on
+    the device and in the oracle, both written as snapshots and compared with
+    the parity harness's semantics (physicell_amd.snapdiff) at the FP64 bar:
+    positions to 1e-10 um relative to the domain, volume exact, densities to
+    1e-10 relative."""
+    import oracle as O
+    from physicell_amd import snapdiff as D
+    spec, cells = S.config0()
+    sim = H.Simulation(spec, cells)
+    counts = sim.run(60.0)
+    sim.wait()
+    o = O.OracleSim(spec, cells)
+    assert o.run(60.0) == tuple(counts)
+    ours, ref = tmp_path / "ours", tmp_path / "ref"
+    ours.mkdir(); ref.mkdir()
+    sim.write_snapshot(ours, 5)
+    oracle_snapshot(ref, 5, spec, o)
+    sim.close()
+    scale = float(np.abs(o.arr["rho"]).max())
+    tol = {"position": 1e-10 * 500, "volume": 0.0, "phase": 0, "cell_count_abs": 0,
+           "substrate_linf": 1e-10 * scale, "substrate_rmse": 1e-10 * scale}
+    cells_d, micro_d = D.diff_frame(str(ref), str(ours), 5, tol)
+    ok, text = D.report(5, cells_d, micro_d)
+    print(text, flush=True)
+    assert ok and cells_d.matched == cells.n
+    assert max(f.rel for f in micro_d.fields) <= 1e-10
