@@ -554,6 +554,14 @@ int pe_set_volume(pe_ensemble* e, uint32_t member_id, uint64_t seed, const doubl
 int pe_update_phenotype(pe_ensemble* e, uint32_t member_id, double dt, int use_ratio, const uint32_t* mask,
                         const uint32_t* parents, uint32_t n_parents, const uint32_t* removed, uint32_t n_removed,
                         uint32_t* n_out);
+/* Neighbour queries and attachments for one member, as pc_sim_neighbor_query
+ * and pc_sim_form_attachments for a single simulation: a hash of the member
+ * alone is built from its current positions (waits for queued work first);
+ * host arrays sized by the member's cell count. */
+int pe_neighbor_query(pe_ensemble* e, uint32_t member_id, int kind, uint32_t* offsets, uint32_t* idx,
+                      uint64_t capacity, uint64_t* total);
+int pe_form_attachments(pe_ensemble* e, uint32_t member_id, double dt, const double* rate, uint32_t* count,
+                        uint32_t* targets, uint64_t seed, uint32_t* formed);
 /* Runs `periods` periods of every member; use_graph != 0 captures one period
  * as a hipGraph (re-captured after membership changes) and replays it. */
 int pe_run_periods(pe_ensemble* e, int periods, int use_graph);

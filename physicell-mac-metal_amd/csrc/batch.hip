@@ -435,6 +435,9 @@ public:
         hc_wait(ctx_);
         members_.clear();
         ws_ = Workspace();
+        q_hash_ = DeviceBuffer();
+        q_off_ = DeviceBuffer();
+        q_idx_ = DeviceBuffer();
         release_streams();
         hc_destroy(ctx_);
     }
@@ -658,6 +661,41 @@ public:
         return n2;
     }
 
+    // Neighbour lists of one member (hc_neighbor_query, kind HC_QUERY_*) over
+    // a hash of that member alone, built from its current positions into a
+    // query workspace; as pc_sim_neighbor_query for a single simulation.
+    uint64_t neighbor_query(uint32_t id, int kind, std::vector<uint32_t>& off, std::vector<uint32_t>& idx) {
+        Member& m = member(id);
+        check(ctx_, hc_wait(ctx_), "wait");
+        const uint32_t nmv = m.mech.mech_grid_nx * m.mech.mech_grid_ny * m.mech.mech_grid_nz;
+        const size_t hb = hc_hash_bytes(nmv, m.max_cells);
+        if (!q_hash_.get() || q_hash_.size() < hb) q_hash_.allocate(ctx_, hb);
+        if (!q_off_.get() || q_off_.size() < ((size_t)m.n + 1) * 4) q_off_.allocate(ctx_, ((size_t)m.n + 1) * 4);
+        check(ctx_, hc_build_hash(ctx_, m.soa.get(), q_hash_.get(), &m.mech, &m.mgrid, m.n, m.max_cells, nmv),
+              "member neighbour query: hash");
+        uint64_t total = 0;
+        check(ctx_, hc_neighbor_query(ctx_, m.soa.get(), q_hash_.get(), &m.mech, m.n, m.max_cells, nmv, kind,
+                                      q_off_.get(), nullptr, &total), "member neighbour query: count");
+        if (!q_idx_.get() || q_idx_.size() < std::max<uint64_t>(16, total * 4))
+            q_idx_.allocate(ctx_, std::max<uint64_t>(16, total * 4));
+        check(ctx_, hc_neighbor_query(ctx_, m.soa.get(), q_hash_.get(), &m.mech, m.n, m.max_cells, nmv, kind,
+                                      q_off_.get(), q_idx_.get(), &total), "member neighbour query");
+        q_off_.download(0, ((size_t)m.n + 1) * 4);
+        if (total) q_idx_.download(0, total * 4);
+        check(ctx_, hc_wait(ctx_), "wait");
+        off.assign((const uint32_t*)q_off_.host(), (const uint32_t*)q_off_.host() + m.n + 1);
+        idx.assign((const uint32_t*)q_idx_.host(), (const uint32_t*)q_idx_.host() + total);
+        return total;
+    }
+
+    // tryFormAllAttachments for one member (pc_sim_form_attachments)
+    uint32_t form_attachments(uint32_t id, double dt, const double* rate, uint32_t* count, uint32_t* targets,
+                              uint64_t seed) {
+        std::vector<uint32_t> off, idx;
+        neighbor_query(id, HC_QUERY_ATTACH, off, idx);
+        return attach_greedy(member(id).n, off, idx, dt, rate, count, targets, seed);
+    }
+
     // one period: substeps_per_mech diffusion substeps, one mechanics step,
     // then each member's own phenotype-cadence refresh when it fires
     void run_periods(int periods, bool use_graph) {
@@ -808,6 +846,7 @@ public:
     std::string last_error;
 
 private:
+    DeviceBuffer q_hash_, q_off_, q_idx_;     // member neighbour-query workspace
     struct Workspace {
         DeviceBuffer sims, mots, bsims, tiles, work[NLIST], agent_sim, kglob, slot, tmp, sorted, pos_of, recs, pos32, counts,
             offsets, partial;
@@ -1363,6 +1402,26 @@ int pe_update_phenotype(pe_ensemble* p, uint32_t member_id, double dt, int use_r
     return eguard(p, [&](Ensemble& e) {
         const uint32_t n = e.update_phenotype(member_id, dt, use_ratio, mask, parents, n_parents, removed, n_removed);
         if (n_out) *n_out = n;
+    });
+}
+
+int pe_neighbor_query(pe_ensemble* p, uint32_t member_id, int kind, uint32_t* offsets, uint32_t* idx,
+                      uint64_t capacity, uint64_t* total) {
+    if (!total) return HC_ERR_ARG;
+    return eguard(p, [&](Ensemble& e) {
+        std::vector<uint32_t> off, lst;
+        *total = e.neighbor_query(member_id, kind, off, lst);
+        if (offsets) std::memcpy(offsets, off.data(), off.size() * 4);
+        if (idx && capacity >= lst.size() && !lst.empty()) std::memcpy(idx, lst.data(), lst.size() * 4);
+    });
+}
+
+int pe_form_attachments(pe_ensemble* p, uint32_t member_id, double dt, const double* rate, uint32_t* count,
+                        uint32_t* targets, uint64_t seed, uint32_t* formed) {
+    if (!rate || !count || !targets) return HC_ERR_ARG;
+    return eguard(p, [&](Ensemble& e) {
+        const uint32_t f = e.form_attachments(member_id, dt, rate, count, targets, seed);
+        if (formed) *formed = f;
     });
 }
 

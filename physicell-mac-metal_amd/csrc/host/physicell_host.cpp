@@ -564,12 +564,9 @@ uint32_t Simulation::updatePhenotypeSoA(double dt, int use_ratio, const uint32_t
     return n2;
 }
 
-uint32_t Simulation::formAttachments(double dt, const double* rate, uint32_t* count, uint32_t* targets,
-                                     uint64_t seed) {
+uint32_t attach_greedy(uint32_t n, const std::vector<uint32_t>& off, const std::vector<uint32_t>& idx, double dt,
+                       const double* rate, uint32_t* count, uint32_t* targets, uint64_t seed) {
     constexpr uint32_t M = 12;    // CellData::MAX_ATTACHMENTS (src/cell_data.h:124)
-    const uint32_t n = cells.num_cells;
-    std::vector<uint32_t> off, idx;
-    mechanics.neighborQuery(cells, HC_QUERY_ATTACH, off, idx);
     std::mt19937 gen((std::mt19937::result_type)seed);
     std::uniform_real_distribution<double> uni(0.0, 1.0);
     uint32_t formed = 0;
@@ -592,6 +589,13 @@ uint32_t Simulation::formAttachments(double dt, const double* rate, uint32_t* co
         }
     }
     return formed;
+}
+
+uint32_t Simulation::formAttachments(double dt, const double* rate, uint32_t* count, uint32_t* targets,
+                                     uint64_t seed) {
+    std::vector<uint32_t> off, idx;
+    mechanics.neighborQuery(cells, HC_QUERY_ATTACH, off, idx);
+    return attach_greedy(cells.num_cells, off, idx, dt, rate, count, targets, seed);
 }
 
 void Simulation::phenotypeStep() {

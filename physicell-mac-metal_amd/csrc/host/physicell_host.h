@@ -38,6 +38,16 @@ struct SimConfig {
 void check(hc_ctx* ctx, int rc, const char* what);
 
 // RAII owner of one hc_buf.
+// The greedy pass of tryFormAllAttachments (src/cell_interactions.cpp:
+// 411-478) over attachment candidates (HC_QUERY_ATTACH lists off[n + 1],
+// idx): i ascending with rate[i] > 0 and fewer than 12 partners, its
+// candidates j ascending with rate[j] > 0; skip when either list is full or
+// j is already i's partner, else draw u (std::mt19937(seed), uniform [0, 1))
+// and attach both ways when u < max(rate_i, rate_j) dt.  Returns the number
+// of attachments formed.
+uint32_t attach_greedy(uint32_t n, const std::vector<uint32_t>& off, const std::vector<uint32_t>& idx, double dt,
+                       const double* rate, uint32_t* count, uint32_t* targets, uint64_t seed);
+
 class DeviceBuffer {
 public:
     DeviceBuffer() = default;

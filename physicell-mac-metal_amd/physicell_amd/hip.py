@@ -175,6 +175,8 @@ def load(path=LIB_PATH):
         "pc_sim_set_volume": (c_i32, [c_vp, ctypes.c_uint64, c_vp]),
         "pc_sim_neighbor_query": (c_i32, [c_vp, c_i32, c_vp, c_vp, ctypes.c_uint64, c_vp]),
         "pc_sim_form_attachments": (c_i32, [c_vp, ctypes.c_double, c_vp, c_vp, c_vp, ctypes.c_uint64, c_vp]),
+        "pe_neighbor_query": (c_i32, [c_vp, c_u32, c_i32, c_vp, c_vp, ctypes.c_uint64, c_vp]),
+        "pe_form_attachments": (c_i32, [c_vp, c_u32, ctypes.c_double, c_vp, c_vp, c_vp, ctypes.c_uint64, c_vp]),
         "hc_neighbor_query": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_u32, c_u32, c_u32, c_i32, c_vp, c_vp, c_vp]),
         "pc_write_neighbor_graph": (c_i32, [ctypes.c_char_p, c_u32, c_vp, c_vp]),
         "pc_sim_update_phenotype": (c_i32, [c_vp, c_f64, c_i32, c_vp, c_vp, c_u32, c_vp, c_u32, c_vp]),
@@ -613,6 +615,31 @@ class Ensemble:
         out = np.empty(max(1, self.ncells[mid]), np.float64 if f < NUM_F64_FIELDS else np.uint32)
         self._check(load().pe_get_field(self.h, mid, f, _ptr(out)), "pe_get_field")
         return out[:self.ncells[mid]]
+
+    def neighbor_query(self, mid, kind):
+        """pe_neighbor_query: (offsets[n + 1], idx) for one member."""
+        off = np.zeros(self.ncells[mid] + 1, np.uint32)
+        total = ctypes.c_uint64()
+        self._check(load().pe_neighbor_query(self.h, mid, kind, _ptr(off), None, 0, ctypes.byref(total)),
+                    "pe_neighbor_query")
+        idx = np.zeros(max(1, total.value), np.uint32)
+        self._check(load().pe_neighbor_query(self.h, mid, kind, _ptr(off), _ptr(idx), total.value,
+                                             ctypes.byref(total)), "pe_neighbor_query")
+        return off, idx[:total.value]
+
+    def form_attachments(self, mid, dt, rate, seed, count=None, targets=None):
+        """pe_form_attachments: (count[n], targets[n, 12], formed) for one member."""
+        n = self.ncells[mid]
+        rate = np.ascontiguousarray(rate, np.float64)
+        if len(rate) != n:
+            raise ValueError("form_attachments: rate needs one value per cell")
+        count = np.zeros(n, np.uint32) if count is None else np.array(count, np.uint32)
+        targets = (np.zeros((n, MAX_ATTACHMENTS), np.uint32) if targets is None
+                   else np.array(targets, np.uint32).reshape(n, MAX_ATTACHMENTS))
+        formed = c_u32()
+        self._check(load().pe_form_attachments(self.h, mid, float(dt), _ptr(rate), _ptr(count), _ptr(targets),
+                                               int(seed), ctypes.byref(formed)), "pe_form_attachments")
+        return count, targets, formed.value
 
     def member_info(self, mid):
         b = ctypes.c_uint64()

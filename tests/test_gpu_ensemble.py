@@ -294,3 +294,51 @@ def test_mixed_ensemble_error_vs_fp64():
         print(f"mixed vs FP64: density max rel {rel:.3e}, positions max abs {pos:.3e} um", flush=True)
         assert rel <= 1e-4
         assert pos <= 1e-3
+
+
+def single_after(sc, c, periods, mixed):
+    """A pc_sim driven through the member schedule, left open for queries."""
+    sim = hip.Simulation(sc, c, mixed=mixed)
+    k = substeps_until(sc.dt_diff, sc.dt_mech)
+    kp = substeps_until(sc.dt_diff, sc.dt_pheno)
+    for p in range(periods):
+        for _ in range(k):
+            sim.substep()
+        sim.mech_step()
+        if ((p + 1) * k) % kp == 0:
+            sim.pheno_step()
+    sim.wait()
+    return sim
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_member_queries_equal_single(mixed):
+    """pe_neighbor_query (contact, interaction, attachment lists) and
+    pe_form_attachments on one member equal pc_sim's on the same schedule,
+    bit for bit, beside other members."""
+    members = [small_member(270, cells=(1500, 3000)), small_member(271), small_member(272, cells=(1500, 3000))]
+    P = 4
+    ens = hip.Ensemble(mixed=mixed)
+    ids = [ens.add(s_, c_) for s_, c_ in members]
+    ens.run_periods(P, use_graph=True)
+    ens.wait()
+    for j in (0, 2):
+        sc, c = members[j]
+        sim = single_after(sc, c, P, mixed)
+        print(f"member queries: member {j} n={c.n} mixed={mixed}", flush=True)
+        for kind in (hip.QUERY_CONTACT, hip.QUERY_INTERACT, hip.QUERY_ATTACH):
+            mo, mi = ens.neighbor_query(ids[j], kind)
+            so, si = sim.neighbor_query(kind)
+            assert np.array_equal(mo, so) and np.array_equal(mi, si), kind
+            assert len(mi) > 0
+        rate = np.random.default_rng(273 + j).uniform(0.0, 0.3, c.n)
+        got = ens.form_attachments(ids[j], 1.0, rate, seed=9)
+        want = sim.form_attachments(1.0, rate, seed=9)
+        assert got[2] == want[2] > 0 and np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+        sim.close()
+    # the queries leave the members' steps unchanged
+    ens.run_periods(1, use_graph=True)
+    ens.wait()
+    for j in (0, 1):
+        assert_same(member_state(ens, ids[j]), run_single(*members[j], P + 1, mixed), f"member {j} after queries")
+    ens.close()
