@@ -567,6 +567,9 @@ uint32_t Simulation::updatePhenotypeSoA(double dt, int use_ratio, const uint32_t
 uint32_t attach_greedy(uint32_t n, const std::vector<uint32_t>& off, const std::vector<uint32_t>& idx, double dt,
                        const double* rate, uint32_t* count, uint32_t* targets, uint64_t seed) {
     constexpr uint32_t M = 12;    // CellData::MAX_ATTACHMENTS (src/cell_data.h:124)
+    if (off.size() != (size_t)n + 1) throw std::runtime_error("form_attachments: candidate lists do not match the cells");
+    for (uint32_t i = 0; i < n; i++)
+        if (count[i] > M) throw std::runtime_error("form_attachments: count[" + std::to_string(i) + "] exceeds 12");
     std::mt19937 gen((std::mt19937::result_type)seed);
     std::uniform_real_distribution<double> uni(0.0, 1.0);
     uint32_t formed = 0;

@@ -77,6 +77,22 @@ def test_empty_population():
     sim.close()
 
 
+def test_attachment_input_checks():
+    """Lists longer than 12 partners and rates of the wrong length are
+    rejected before any work."""
+    spec, cells = S.small3d(12, 10, 8, n_cells=50, seed=75)
+    sim = H.Simulation(spec, cells)
+    bad = np.zeros(cells.n, np.uint32)
+    bad[3] = 13
+    with pytest.raises(H.HipError, match="exceeds 12"):
+        sim.form_attachments(1.0, np.full(cells.n, 0.1), seed=1, count=bad)
+    with pytest.raises(ValueError):
+        sim.form_attachments(1.0, np.full(cells.n - 1, 0.1), seed=1)
+    cnt, tg, formed = sim.form_attachments(1.0, np.full(cells.n, 0.1), seed=1)
+    assert cnt.max(initial=0) <= H.MAX_ATTACHMENTS
+    sim.close()
+
+
 def attach_candidates_bruteforce(a):
     """All pairs j > i, both alive, sqrt(|p_j - p_i|^2) <= max(rel) (r_i + r_j)
     (tryFormAttachment's test, src/cell_interactions.cpp:429-440)."""
