@@ -69,17 +69,33 @@ def cpu_baseline(spec, cells, periods, substeps):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     threads = max(1, min(16, os.cpu_count() or 1))
-    sim = O.OracleSim(spec, cells)
-    t0 = time.perf_counter()
-    for _ in range(periods):
-        for _ in range(substeps):
-            sim.substep(nthreads=threads)
-        sim.mech_step(nthreads=threads)
-    dt = time.perf_counter() - t0
+
+    def timed(nper, nthreads):
+        sim = O.OracleSim(spec, cells)
+        t0 = time.perf_counter()
+        for _ in range(nper):
+            for _ in range(substeps):
+                sim.substep(nthreads=nthreads)
+            sim.mech_step(nthreads=nthreads)
+        return time.perf_counter() - t0
+
+    dt = timed(periods, threads)
+    dt1 = timed(1, 1)                        # one period on one host thread
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": cells.n * periods / dt, "unit": "agent-steps/s", "cores": threads, "kind": "port",
             "voxel_substeps_per_s": spec.n_voxels * len(spec.substrates) * substeps * periods / dt,
+            "one_core": {"value": cells.n / dt1,
+                         "voxel_substeps_per_s": spec.n_voxels * len(spec.substrates) * substeps / dt1,
+                         "seconds": dt1},
+            "host": {"cpu_model": model, "cpu_count": os.cpu_count()},
             "sample": f"{periods} period(s) of {substeps} diffusion substeps + 1 mechanics step of the "
-                      f"same workload, FP64 oracle, OpenMP {threads} threads, {dt:.1f} s"}
+                      f"same workload, FP64 oracle, OpenMP {threads} threads, {dt:.1f} s; and 1 period on "
+                      f"1 thread, {dt1:.1f} s"}
 
 
 def bench_ensemble(args):
