@@ -325,7 +325,11 @@ int pc_sim_phenotype_step(pc_sim* sim);
  * counts may be NULL (int64[3]: substeps, mechanics, phenotype). */
 int pc_sim_run(pc_sim* sim, double max_time, int64_t* counts);
 /* Run `periods` x (steps_per_mech diffusion substeps + 1 mechanics step);
- * use_graph != 0 captures one period as a hipGraph and replays it. */
+ * use_graph != 0 captures one period as a hipGraph and replays it.  The
+ * phenotype step (voxel refresh + secretion re-plan) fires every
+ * dt_phenotype in the order of src/main.mm:394-412, outside the graph, on a
+ * clock that persists across calls and restarts at pc_sim_set_cells; so
+ * P periods from fresh cells equal pc_sim_run(sim, (P*k - 1)*dt_diffusion). */
 int pc_sim_run_periods(pc_sim* sim, int periods, int use_graph);
 /* on != 0 (default): in pc_sim_run_periods, each period's mechanics step
  * runs on a second stream concurrently with its diffusion substeps (they

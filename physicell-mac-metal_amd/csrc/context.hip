@@ -201,7 +201,8 @@ void* hc_device_ptr(hc_buf* b) { return b ? b->dptr : nullptr; }
 size_t hc_buf_size(const hc_buf* b) { return b ? b->bytes : 0; }
 
 int hc_upload(hc_ctx* ctx, hc_buf* b, size_t off, size_t n) {
-    if (!ctx || !b || off + n > b->bytes) return fail(ctx, HC_ERR_ARG, "hc_upload: range outside buffer");
+    if (!ctx || !b || off > b->bytes || n > b->bytes - off)
+        return fail(ctx, HC_ERR_ARG, "hc_upload: range outside buffer");
     if (!n) return 0;
     if (!hc_host_mirror(b)) return fail(ctx, HC_ERR_NOMEM, "hc_upload: no host mirror");
     hipError_t e = hipMemcpyAsync((char*)b->dptr + off, (char*)b->host + off, n,
@@ -210,7 +211,8 @@ int hc_upload(hc_ctx* ctx, hc_buf* b, size_t off, size_t n) {
 }
 
 int hc_download(hc_ctx* ctx, hc_buf* b, size_t off, size_t n) {
-    if (!ctx || !b || off + n > b->bytes) return fail(ctx, HC_ERR_ARG, "hc_download: range outside buffer");
+    if (!ctx || !b || off > b->bytes || n > b->bytes - off)
+        return fail(ctx, HC_ERR_ARG, "hc_download: range outside buffer");
     if (!n) return 0;
     if (!hc_host_mirror(b)) return fail(ctx, HC_ERR_NOMEM, "hc_download: no host mirror");
     hipError_t e = hipMemcpyAsync((char*)b->host + off, (char*)b->dptr + off, n,

@@ -426,6 +426,7 @@ void Simulation::setCells(uint32_t n, const double* x, const double* y, const do
                           const double* sec, const double* upt, const double* sat, const double* net,
                           const uint32_t* alive) {
     if (n > cells.max_cells) throw std::runtime_error("setCells: more cells than max_cells");
+    pheno_acc_ = 0.0;
     cells.num_cells = 0;
     const uint32_t ns = cells.n_substrates;
     for (uint32_t i = 0; i < n; i++) {
@@ -691,11 +692,59 @@ void Simulation::periodBody() {
     hip_check(hipStreamWaitEvent(main, (hipEvent_t)join_, 0), "join wait");
 }
 
-void Simulation::runPeriods(int periods, bool use_graph) {
-    if (!use_graph) {
-        for (int p = 0; p < periods; p++) periodBody();
-        return;
+// The phenotype accumulator of src/main.mm:394-412 over the k substeps of the
+// next period: the substep indices (0..k-1) after which the phenotype step
+// fires, in order.  The mechanics step of a period comes at substep k-1, and
+// the reference runs it before that iteration's phenotype step.
+std::vector<int> Simulation::advancePhenotypeClock(int k) {
+    std::vector<int> fires;
+    const double dtd = config.dt_diffusion;
+    for (int s = 0; s < k; s++) {
+        pheno_acc_ += dtd;
+        if (pheno_acc_ >= config.dt_phenotype - 0.5 * dtd) {
+            fires.push_back(s);
+            pheno_acc_ = 0.0;
+        }
     }
+    return fires;
+}
+
+// One period in the exact order of the step loop when the phenotype step
+// fires inside it: substeps, the phenotype step after each substep where it
+// fires, the mechanics step after the last substep, then a phenotype step
+// that fires at the last substep.
+void Simulation::periodWithPhenotype(const std::vector<int>& fires) {
+    const int k = substepsPerMechanicsStep();
+    size_t f = 0;
+    for (int s = 0; s < k; s++) {
+        diffusionSubstep();
+        if (s == k - 1) mechanicsStep();
+        while (f < fires.size() && fires[f] == s) {
+            phenotypeStep();
+            f++;
+        }
+    }
+}
+
+void Simulation::runPeriods(int periods, bool use_graph) {
+    const int k = substepsPerMechanicsStep();
+    for (int p = 0; p < periods; p++) {
+        const std::vector<int> fires = advancePhenotypeClock(k);
+        if (fires.empty() || (fires.size() == 1 && fires[0] == k - 1)) {
+            // the phenotype step, if due, follows the whole period: the period
+            // runs as usual (graph replay or eager) and the phenotype step is
+            // issued after it, outside any graph.  Its voxel refresh and
+            // secretion re-plan write buffers whose addresses the graph keeps.
+            if (use_graph) launchPeriodGraph();
+            else periodBody();
+            if (!fires.empty()) phenotypeStep();
+        } else {
+            periodWithPhenotype(fires);
+        }
+    }
+}
+
+void Simulation::launchPeriodGraph() {
     hipStream_t stream = hc_stream_of(ctx);
     if (!graph_exec_) {
         check(ctx, hc_wait(ctx), "wait");
@@ -717,10 +766,8 @@ void Simulation::runPeriods(int periods, bool use_graph) {
         if (e != hipSuccess) throw std::runtime_error(std::string("graph instantiate: ") + hipGetErrorString(e));
         graph_exec_ = (void*)exec;
     }
-    for (int p = 0; p < periods; p++) {
-        hipError_t e = hipGraphLaunch((hipGraphExec_t)graph_exec_, stream);
-        if (e != hipSuccess) throw std::runtime_error(std::string("graph launch: ") + hipGetErrorString(e));
-    }
+    hipError_t e = hipGraphLaunch((hipGraphExec_t)graph_exec_, stream);
+    if (e != hipSuccess) throw std::runtime_error(std::string("graph launch: ") + hipGetErrorString(e));
 }
 
 void Simulation::wait() { check(ctx, hc_wait(ctx), "wait"); }

@@ -212,7 +212,10 @@ public:
     void run(double max_time, int64_t counts[3]);
     // periods x (substeps + one mechanics step); with overlap on (default),
     // the mechanics step of a period runs on a second stream concurrently
-    // with its substeps (they touch disjoint data; same results)
+    // with its substeps (they touch disjoint data; same results).  The
+    // phenotype step (voxel refresh + secretion re-plan) fires every
+    // dt_phenotype as in run(), on a clock that persists across calls and
+    // restarts when the cells are set.
     void runPeriods(int periods, bool use_graph);
     void setOverlap(bool on);
     // motility in the mechanics step (forces -> motility -> AB2); per-cell
@@ -250,6 +253,10 @@ public:
 
 private:
     void periodBody();
+    void launchPeriodGraph();
+    std::vector<int> advancePhenotypeClock(int k);
+    void periodWithPhenotype(const std::vector<int>& fires);
+    double pheno_acc_ = 0.0;       // runPeriods' phenotype accumulator (src/main.mm:394-412)
     void* graph_exec_ = nullptr;   // hipGraphExec_t of one period
     void* side_ = nullptr;         // hipStream_t for the overlapped mechanics step
     void* fork_ = nullptr;         // hipEvent_t

@@ -244,8 +244,16 @@ class Buffer:
         ctx._check(load().hc_alloc(ctx.h, self.nbytes, ctypes.byref(h)), "hc_alloc")
         self.h = h
 
+    def _check_range(self, offset, nbytes, what):
+        """The pinned host mirror is exactly self.nbytes long: a copy into or out of
+        it must be range-checked here, before any memmove touches it."""
+        offset, nbytes = int(offset), int(nbytes)
+        if offset < 0 or nbytes < 0 or offset + nbytes > self.nbytes:
+            raise HipError(f"{what}: range [{offset}, {offset + nbytes}) outside the {self.nbytes}-byte buffer")
+
     def upload(self, arr, offset=0):
         arr = np.ascontiguousarray(arr)
+        self._check_range(offset, arr.nbytes, "upload")
         lib = load()
         host = lib.hc_host_mirror(self.h)
         if not host:
@@ -255,8 +263,9 @@ class Buffer:
         self.ctx.wait()
 
     def download(self, dtype, count, offset=0):
+        nb = np.dtype(dtype).itemsize * int(count)
+        self._check_range(offset, nb, "download")
         lib = load()
-        nb = np.dtype(dtype).itemsize * count
         self.ctx._check(lib.hc_download(self.ctx.h, self.h, offset, nb), "hc_download")
         self.ctx.wait()
         host = lib.hc_host_mirror(self.h)

@@ -5,6 +5,7 @@ Bar (north_star): substrate fields within 1e-4 relative and positions within
 positions in both forms.  Each test prints max-abs and max-relative error.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -64,14 +65,14 @@ def test_fp32_substep_vs_fp64(ctx, nx, ny, nz):
     d.free(); tb.free()
 
 
-@pytest.mark.parametrize("builder,max_time", [("config0", 60.0), ("config1", 6.0)])
+@pytest.mark.parametrize("builder,max_time", [("config0", 60.0), ("config1", 60.0)])
 def test_mixed_whole_run_vs_fp64(builder, max_time):
     spec, cells = S.BUILDERS[builder]()
     print(f"mixed run {builder} {max_time} min", flush=True)
     sim = H.Simulation(spec, cells, mixed=True)
     counts = sim.run(max_time)
     ref = O.OracleSim(spec, cells)
-    assert counts == ref.run(max_time, nthreads=8)
+    assert counts == ref.run(max_time, nthreads=max(1, min(16, os.cpu_count() or 1)))
     ea, er = errs(sim.density(), ref.arr["rho"])
     print(f"  density: max abs {ea:.3e} max rel {er:.3e}")
     assert er <= 1e-4

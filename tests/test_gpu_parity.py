@@ -5,6 +5,8 @@ FP64 fields, positions, velocities within 1e-10 relative (the kernels and the
 oracle evaluate the same operation sequences with -ffp-contract=off, so the
 results are expected to be bit-identical and the check also reports that).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -227,6 +229,57 @@ def test_voxel_keys_hash_neighbors_forces_ab2(ctx, builder):
         b.free()
 
 
+def test_hash_overflow_report_keeps_every_agent(ctx):
+    """The reference's hash keeps at most 64 agents per bucket and drops the
+    rest (shaders/mechanics.metal:79-84).  This build drops none and reports
+    the largest bin: one 30 um bin holding 150 agents plus a sparse background;
+    hc_hash_max_bin returns the true count, every agent appears exactly once
+    in the sorted list, the bins equal the oracle's, and the forces on the
+    crowded bin's members equal the oracle's."""
+    spec, cells = S.small3d(12, 10, 8, n_cells=400, seed=21)
+    rng = np.random.default_rng(22)
+    crowd = 150
+    # bin (3, 2, 1) of the 30 um grid spans [90, 120) x [60, 90) x [30, 60)
+    lo = np.array([90.0, 60.0, 30.0])
+    p = lo + 0.5 + rng.random((crowd, 3)) * 29.0
+    cells.x[:crowd], cells.y[:crowd], cells.z[:crowd] = p[:, 0], p[:, 1], p[:, 2]
+    # keep the background out of that bin
+    bg = slice(crowd, None)
+    inside = ((cells.x[bg] >= 90) & (cells.x[bg] < 120) & (cells.y[bg] >= 60) & (cells.y[bg] < 90)
+              & (cells.z[bg] >= 30) & (cells.z[bg] < 60))
+    cells.x[bg][inside] += 60.0
+    lib = H.load()
+    n = cells.n
+    cb = cells_buffer(ctx, spec, cells)
+    m, mg = H.mech_of(spec), H.mech_grid_of(spec)
+    nb = spec.n_mech_voxels
+    hb = ctx.alloc(lib.hc_hash_bytes(nb, cb.max_cells))
+    ctx._check(lib.hc_build_hash(ctx.h, cb.soa.h, hb.h, m, mg, n, cb.max_cells, nb), "hash")
+    mx = H.c_u32()
+    ctx._check(lib.hc_hash_max_bin(ctx.h, hb.h, nb, H.ctypes.byref(mx)), "max bin")
+    om = O.mech(spec)
+    keys_ref = O.mech_keys(cells.x, cells.y, cells.z, om)
+    sizes = np.bincount(keys_ref, minlength=nb)
+    print(f"overflow: max bin {mx.value}, oracle {sizes.max()}", flush=True)
+    assert sizes.max() == crowd and mx.value == crowd
+    off = np.empty(nb + 1, np.uint32)
+    srt = np.empty(n, np.uint32)
+    ctx._check(lib.hc_hash_download(ctx.h, hb.h, nb, cb.max_cells, n, H._ptr(off), H._ptr(srt)), "hash dl")
+    off_ref, srt_ref = O.hash_sort(keys_ref, nb)
+    assert np.array_equal(off, off_ref) and np.array_equal(srt, srt_ref)
+    assert np.array_equal(np.sort(srt), np.arange(n, dtype=np.uint32)), "every agent exactly once"
+    big = int(np.argmax(sizes))
+    assert np.array_equal(srt[off[big]:off[big + 1]], np.arange(crowd, dtype=np.uint32))
+    ctx._check(lib.hc_forces(ctx.h, cb.soa.h, hb.h, m, n, cb.max_cells, nb), "forces")
+    vx_r, vy_r, vz_r, p_r = O.forces(cells.x, cells.y, cells.z, cells.radius, cells.rep, cells.adh,
+                                     cells.rel_adh, keys_ref, off_ref, srt_ref, om)
+    for f, r, nm in ((H.F_VEL_X, vx_r, "vx"), (H.F_VEL_Y, vy_r, "vy"), (H.F_VEL_Z, vz_r, "vz"),
+                     (H.F_PRESSURE, p_r, "pressure")):
+        assert_close(cb.field(f), r, f"overflow {nm}")
+    for b in (hb, cb.soa, cb.rates):
+        b.free()
+
+
 def test_secretion_crowded_voxels(ctx):
     """Many cells per voxel, dead cells, net export, out-of-range voxels."""
     spec, cells = S.small3d(6, 5, 4, n_cells=2000, seed=9)
@@ -261,14 +314,14 @@ def test_secretion_crowded_voxels(ctx):
         b.free()
 
 
-@pytest.mark.parametrize("builder,max_time", [("config0", 60.0), ("small3d", 2.0), ("config1", 3.0)])
+@pytest.mark.parametrize("builder,max_time", [("config0", 60.0), ("small3d", 2.0), ("config1", 60.0)])
 def test_whole_run_vs_oracle(builder, max_time):
     spec, cells = S.BUILDERS[builder]()
     print(f"whole run {builder} {max_time} min", flush=True)
     sim = H.Simulation(spec, cells)
     counts = sim.run(max_time)
     ref = O.OracleSim(spec, cells)
-    counts_r = ref.run(max_time, nthreads=8)
+    counts_r = ref.run(max_time, nthreads=max(1, min(16, os.cpu_count() or 1)))
     assert counts == counts_r
     assert_close(sim.density(), ref.arr["rho"], f"{builder} density after {counts[0]} substeps")
     for f, k in ((H.F_POS_X, "x"), (H.F_POS_Y, "y"), (H.F_POS_Z, "z"), (H.F_VEL_X, "vx"), (H.F_PRESSURE, "pressure")):
@@ -293,6 +346,44 @@ def test_graph_replay_matches_eager():
         assert np.array_equal(da, db), f"replay {rep}"
         assert np.array_equal(a.field(H.F_POS_X), b.field(H.F_POS_X))
     a.close(); b.close()
+
+
+@pytest.mark.parametrize("dt_pheno,periods", [(6.0, 70), (0.25, 12)])
+def test_run_periods_phenotype_clock_matches_run(dt_pheno, periods):
+    """pc_sim_run_periods fires the phenotype step (voxel refresh + secretion
+    re-plan) where the step loop of src/main.mm:394-412 fires it: P periods
+    equal pc_sim_run over the same P*k substeps bit for bit, eager and graph,
+    serial and overlapped, with the periods split over two calls.  dt_pheno
+    6.0 fires after the mechanics step of period 60; 0.25 fires inside
+    periods (substeps 25, 50, 75, ...)."""
+    import dataclasses
+    spec, cells = S.small3d(16, 14, 12, n_cells=3000, seed=4)
+    spec = dataclasses.replace(spec, dt_pheno=dt_pheno)
+    k = 10
+    print(f"run_periods phenotype clock dt_pheno={dt_pheno} periods={periods}", flush=True)
+    ref = H.Simulation(spec, cells)
+    keys0 = ref.field(H.F_VOXEL_IDX)
+    counts = ref.run((periods * k - 1) * spec.dt_diff)
+    assert counts[0] == periods * k and counts[1] == periods
+    assert counts[2] == int(periods * k * spec.dt_diff / dt_pheno + 1e-9)
+    keys = ref.field(H.F_VOXEL_IDX)
+    assert not np.array_equal(keys, keys0), "the voxel refresh must change keys for this case to test anything"
+    rho, pos = ref.density(), [ref.field(f) for f in (H.F_POS_X, H.F_POS_Y, H.F_POS_Z)]
+    ref.close()
+    for use_graph in (False, True):
+        for overlap in (False, True):
+            sim = H.Simulation(spec, cells)
+            sim.set_overlap(overlap)
+            first = periods // 2 + 1
+            sim.run_periods(first, use_graph=use_graph)
+            sim.run_periods(periods - first, use_graph=use_graph)
+            sim.wait()
+            what = f"graph={use_graph} overlap={overlap}"
+            assert np.array_equal(sim.field(H.F_VOXEL_IDX), keys), what
+            assert np.array_equal(sim.density(), rho), what
+            for f, p in zip((H.F_POS_X, H.F_POS_Y, H.F_POS_Z), pos):
+                assert np.array_equal(sim.field(f), p), what
+            sim.close()
 
 
 @pytest.mark.parametrize("mixed", [False, True])
