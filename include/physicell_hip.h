@@ -266,6 +266,18 @@ int hc_neighbor_query(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* m
  * summation in a fixed order (deterministic); out[n_substrates]. */
 int hc_substrate_totals(hc_ctx* ctx, hc_buf* density, const hc_grid* grid, double* out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): the same
+ * totals as hc_substrate_totals, bit for bit, with the partial sums and
+ * results in a file-scope device array of 256.5 KiB instead of a per-call
+ * hipMalloc/hipFree.  Each GPU the library runs on holds its own copy of the
+ * array until the process ends.  n_substrates above
+ * HC_TOTALS_NOALLOC_MAX_SUBSTRATES fails with HC_ERR_CAPACITY.  Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (from
+ * several host threads or streams) race. */
+#define HC_TOTALS_NOALLOC_MAX_SUBSTRATES 64
+int hc_substrate_totals_noalloc(hc_ctx* ctx, hc_buf* density, const hc_grid* grid, double* out);
+
 /* ---- timing ------------------------------------------------------------- */
 /* HIP events on the context stream around every launch of the selected
  * kernels (bit mask of hc_kernel_id); accumulated per kernel. */

@@ -18,6 +18,9 @@
 // Forces (shaders/mechanics.metal:91-216) run one agent per lane in
 // hash-sorted order and visit the 27 (9 in 2-D) bins in the reference's
 // (dk, dj, di) order, agents of a bin in ascending index.
+#include <map>
+#include <mutex>
+
 #include "cells_common.h"
 
 namespace hc {
@@ -645,6 +648,41 @@ __global__ void k_totals_final(const Neu* __restrict__ partial, uint32_t nb, uin
     out[s] = a.s + a.c;
 }
 
+}  // namespace hc
+
+// Scratch of hc_substrate_totals_noalloc: per-block partial sums of up to
+// HC_TOTALS_NOALLOC_MAX_SUBSTRATES substrates, then their results.  Each GPU
+// holds its own copy from module load until the process ends; every call on
+// that GPU shares it (so calls that overlap on one GPU race).  Every partial
+// and result a call reads is written by that call first, so nothing needs
+// clearing between calls.
+static constexpr size_t TOTALS_PARTIAL_BYTES =
+    (size_t)HC_TOTALS_NOALLOC_MAX_SUBSTRATES * 256 /* TOT_BLOCKS */ * 16 /* sizeof(Neu) */;
+static constexpr size_t TOTALS_SCRATCH_BYTES = TOTALS_PARTIAL_BYTES + HC_TOTALS_NOALLOC_MAX_SUBSTRATES * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[TOTALS_SCRATCH_BYTES];
+
+namespace hc {
+
+static_assert(TOT_BLOCKS == 256 && sizeof(Neu) == 16, "TOTALS_PARTIAL_BYTES assumes 256 blocks of 16-byte partials");
+
+// The device address of `scratch` on each GPU, looked up once per device with
+// that device current (the host-side name is only a stand-in for the symbol).
+int totals_scratch_init(hc_ctx* ctx) {
+    static std::mutex mu;
+    static std::map<int, void*> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(ctx->device);
+    if (it == addr.end()) {
+        hipError_t e = hipSetDevice(ctx->device);
+        void* p = nullptr;
+        if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(ctx, e, "totals scratch: hipGetSymbolAddress");
+        it = addr.emplace(ctx->device, p).first;
+    }
+    ctx->totals_scratch = it->second;
+    return 0;
+}
+
 int probe_kernels_cells2(hc_ctx* ctx);
 int probe_kernels_cells2(hc_ctx* ctx) {
     hipFuncAttributes attr;
@@ -966,12 +1004,12 @@ int hc_substrate_totals(hc_ctx* ctx, hc_buf* density, const hc_grid* g, double* 
     if (!ns) return 0;
     if (ns > 1024) return fail(ctx, HC_ERR_ARG, "hc_substrate_totals: more than 1024 substrates");
     if (hc_buf_size(density) < V * ns * field_elem(g)) return fail(ctx, HC_ERR_ARG, "hc_substrate_totals: buffer too small");
-    void* scratch = nullptr;
+    void* mem = nullptr;
     const size_t pbytes = (size_t)ns * TOT_BLOCKS * sizeof(Neu);
-    hipError_t e = hipMalloc(&scratch, pbytes + ns * sizeof(double));
+    hipError_t e = hipMalloc(&mem, pbytes + ns * sizeof(double));
     if (e != hipSuccess) return hip_fail(ctx, e, "hc_substrate_totals: scratch");
-    Neu* part = (Neu*)scratch;
-    double* dout = (double*)((char*)scratch + pbytes);
+    Neu* part = (Neu*)mem;
+    double* dout = (double*)((char*)mem + pbytes);
     if (g->field_fp32)
         hipLaunchKernelGGL(k_totals<float>, dim3(TOT_BLOCKS, ns), dim3(256), 0, ctx->stream, (const float*)density->dptr, V, part);
     else
@@ -984,8 +1022,38 @@ int hc_substrate_totals(hc_ctx* ctx, hc_buf* density, const hc_grid* g, double* 
         if (e != hipSuccess) r = hip_fail(ctx, e, "hc_substrate_totals");
     }
     (void)hipStreamSynchronize(ctx->stream);        // cleanup: r holds the first failure
-    (void)hipFree(scratch);
+    (void)hipFree(mem);
     return r;
+}
+
+int hc_substrate_totals_noalloc(hc_ctx* ctx, hc_buf* density, const hc_grid* g, double* out) {
+    if (!ctx || !density || !g || !out) return fail(ctx, HC_ERR_ARG, "hc_substrate_totals_noalloc: null argument");
+    const uint64_t V = (uint64_t)g->nx * g->ny * g->nz;
+    const uint32_t ns = g->n_substrates;
+    if (!ns) return 0;
+    if (ns > HC_TOTALS_NOALLOC_MAX_SUBSTRATES)
+        return fail(ctx, HC_ERR_CAPACITY, "hc_substrate_totals_noalloc: " + std::to_string(ns) +
+                                              " substrates exceed the static scratch's maximum of " +
+                                              std::to_string(HC_TOTALS_NOALLOC_MAX_SUBSTRATES));
+    if (hc_buf_size(density) < V * ns * field_elem(g))
+        return fail(ctx, HC_ERR_ARG, "hc_substrate_totals_noalloc: buffer too small");
+    if (!ctx->totals_scratch) {
+        int r = totals_scratch_init(ctx);
+        if (r) return r;
+    }
+    // the device address of this GPU's copy of `scratch` (never the host-side name)
+    Neu* part = (Neu*)ctx->totals_scratch;
+    double* dout = (double*)((char*)ctx->totals_scratch + TOTALS_PARTIAL_BYTES);
+    if (g->field_fp32)
+        hipLaunchKernelGGL(k_totals<float>, dim3(TOT_BLOCKS, ns), dim3(256), 0, ctx->stream, (const float*)density->dptr, V, part);
+    else
+        hipLaunchKernelGGL(k_totals<double>, dim3(TOT_BLOCKS, ns), dim3(256), 0, ctx->stream, (const double*)density->dptr, V, part);
+    hipLaunchKernelGGL(k_totals_final, dim3(1), dim3(1024), 0, ctx->stream, part, (uint32_t)TOT_BLOCKS, ns, dout);
+    int r = check_launch(ctx, "k_totals (static scratch)");
+    if (r) return r;
+    hipError_t e = hipMemcpyAsync(out, dout, ns * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? 0 : hip_fail(ctx, e, "hc_substrate_totals_noalloc");
 }
 
 }  // extern "C"

@@ -133,6 +133,8 @@ int hc_load_kernels(hc_ctx* ctx, const char* code_object_or_null) {
     if (r) return r;
     r = probe_kernels_cells2(ctx);
     if (r) return r;
+    r = totals_scratch_init(ctx);
+    if (r) return r;
     r = probe_kernels_batch(ctx);
     if (r) return r;
     r = probe_kernels_motility(ctx);

@@ -39,6 +39,9 @@ struct hc_ctx {
     // neighbour queries: two maxima and the scan's partials
     void* d_query = nullptr;
     size_t d_query_bytes = 0;
+    // this device's address of the static scratch of hc_substrate_totals_noalloc
+    // (set by hc_load_kernels, or by that entry point's first call)
+    void* totals_scratch = nullptr;
 };
 
 struct hc_buf {
@@ -256,5 +259,8 @@ PlanLayout plan_layout(const hc_grid* g, uint32_t max_cells);
 // out[n] = total; partial is scratch of scan_partial_bytes(n).
 size_t scan_partial_bytes(uint64_t n);
 int scan_exclusive(hc_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* partial);
+// looks up this device's address of hc_substrate_totals_noalloc's static
+// scratch (once per device) and keeps it in ctx->totals_scratch
+int totals_scratch_init(hc_ctx* ctx);
 
 }  // namespace hc

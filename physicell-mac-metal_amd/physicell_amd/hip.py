@@ -148,6 +148,7 @@ def load(path=LIB_PATH):
         "hc_neighbor_lists": (c_i32, [c_vp, c_vp, c_vp, P(Mech), c_u32, c_u32, c_u32, c_u32, c_vp, c_vp]),
         "hc_set_sweep_mode": (c_i32, [c_vp, c_i32]), "hc_set_substep_order": (c_i32, [c_vp, c_i32]),
         "hc_substrate_totals": (c_i32, [c_vp, c_vp, P(Grid), c_vp]),
+        "hc_substrate_totals_noalloc": (c_i32, [c_vp, c_vp, P(Grid), c_vp]),
         "hc_timing_enable": (c_i32, [c_vp, c_u32]),
         "hc_timing_get": (c_i32, [c_vp, c_i32, P(c_f64), P(ctypes.c_int64)]),
         "hc_timing_reset": (c_i32, [c_vp]),
