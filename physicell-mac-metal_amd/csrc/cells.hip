@@ -173,6 +173,17 @@ __global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ ke
 }
 
 
+// the largest of nb block maxima (one block)
+__global__ __launch_bounds__(256) void k_rmax(const float* __restrict__ rpart, uint32_t nb, float* __restrict__ rmax) {
+    float r = 0.0f;
+    for (uint32_t b = threadIdx.x; b < nb; b += 256) r = fmaxf(r, rpart[b]);
+    for (int d = 32; d >= 1; d >>= 1) r = fmaxf(r, __shfl_xor(r, d, 64));
+    __shared__ float wmax[4];
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) rmax[0] = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+}
+
 __global__ __launch_bounds__(256) void k_bin_sort(const uint32_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ tmp, uint32_t* __restrict__ sorted,
                                                   uint32_t* __restrict__ pos_of, uint32_t nbins) {
@@ -186,8 +197,19 @@ __global__ __launch_bounds__(256) void k_stable_gather(CellPtrs cp, const uint32
                                                        uint32_t* __restrict__ sorted, const uint32_t* __restrict__ pos_of,
                                                        CellRec* __restrict__ rec, float4* __restrict__ pos32,
                                                        uint32_t n, double x_min, double y_min, double z_min,
-                                                       double voxel, uint32_t mnx, uint32_t mny) {
+                                                       double voxel, uint32_t mnx, uint32_t mny, int x_grid,
+                                                       float* __restrict__ rpart) {
     const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    if (rpart) {
+        // this block's largest radius, rounded up to FP32 (no lane returns
+        // before the barrier)
+        __shared__ float wmax[4];
+        float r = c < n ? __double2float_ru(cp.f[HC_F_RADIUS][c]) : 0.0f;
+        for (int d = 32; d >= 1; d >>= 1) r = fmaxf(r, __shfl_xor(r, d, 64));
+        if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = r;
+        __syncthreads();
+        if (threadIdx.x == 0) rpart[blockIdx.x] = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+    }
     if (c >= n) return;
     const uint32_t b = key[c];
     const uint32_t beg = offsets[b], end = offsets[b + 1];
@@ -199,8 +221,10 @@ __global__ __launch_bounds__(256) void k_stable_gather(CellPtrs cp, const uint32
         sorted[pos] = c;
     }
     if (pos32) {
+        // x relative to the grid's x_min (force_agent64x) or to the bin corner
+        // (force_range64m); y and z relative to the bin corner
         const uint32_t bi = b % mnx, bj = (b / mnx) % mny, bk = b / (mnx * mny);
-        pos32[pos] = make_float4((float)(cp.f[HC_F_POS_X][c] - (x_min + bi * voxel)),
+        pos32[pos] = make_float4((float)(cp.f[HC_F_POS_X][c] - (x_grid ? x_min : x_min + bi * voxel)),
                                  (float)(cp.f[HC_F_POS_Y][c] - (y_min + bj * voxel)),
                                  (float)(cp.f[HC_F_POS_Z][c] - (z_min + bk * voxel)), (float)cp.f[HC_F_RADIUS][c]);
     }
@@ -224,12 +248,17 @@ __global__ __launch_bounds__(256) void k_stable_gather(CellPtrs cp, const uint32
 // pos32 is null unless the bin size and the cutoff allow the FP32 mark pass
 // (force_range64m); agents more than two bins from their bin corner take the
 // FP64 mark pass either way.
+// x_grid: pos32 holds x relative to the grid's x_min (force_agent64x, with
+// its list of marked candidates in LDS), else relative to the bin corner
+// (force_range64m).
 __global__ __launch_bounds__(256) void k_forces(const CellRec* __restrict__ rec, const float4* __restrict__ pos32,
                                                 const uint32_t* __restrict__ offsets,
                                                 double* __restrict__ vx, double* __restrict__ vy,
                                                 double* __restrict__ vz, double* __restrict__ pressure,
                                                 uint32_t n, uint32_t mnx, uint32_t mny, uint32_t mnz,
-                                                double max_dist, float voxel) {
+                                                double max_dist, float voxel, int x_grid,
+                                                const float* __restrict__ rmax) {
+    __shared__ uint32_t marks[4 * MARKX_CAP * 64];
     const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (t >= n) return;
     const CellRec me = rec[t];
@@ -240,7 +269,19 @@ __global__ __launch_bounds__(256) void k_forces(const CellRec* __restrict__ rec,
     const double cut2 = max_dist * max_dist;
     const float4 m = pos32 ? pos32[t] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float near = 2.0f * voxel;
-    const bool mark32 = pos32 && fabsf(m.x) < near && fabsf(m.y) < near && fabsf(m.z) < near;
+    const bool yz_near = fabsf(m.y) < near && fabsf(m.z) < near;
+    if (pos32 && x_grid && yz_near && m.x > -near && m.x < (float)mnx * voxel + near) {
+        uint32_t* list = marks + (threadIdx.x >> 6) * (MARKX_CAP * 64) + (threadIdx.x & 63);
+        force_agent64x(rec, pos32, offsets, list, 0u, vi, vj, vk, mnx, mny, mnz, m, voxel, rmax[0], px, py, pz,
+                       radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz, local_pressure);
+        const uint32_t c = me.idx;
+        vx[c] = fx;
+        vy[c] = fy;
+        vz[c] = fz;
+        pressure[c] = local_pressure;
+        return;
+    }
+    const bool mark32 = pos32 && !x_grid && fabsf(m.x) < near && yz_near;
 #pragma unroll 1
     for (int q = 0; q < 9; q++) {
         const RowRange rr = neighbor_row(offsets, 0u, vi, vj, vk, mnx, mny, mnz, q);
@@ -438,7 +479,7 @@ int probe_kernels_cells(hc_ctx* ctx) {
     hipFuncAttributes attr;
     const void* fns[] = {(const void*)k_scan_local, (const void*)k_scan_partials, (const void*)k_scan_add,
                          (const void*)k_fill_u32, (const void*)k_voxel_keys, (const void*)k_mech_keys,
-                         (const void*)k_scatter, (const void*)k_stable_gather, (const void*)k_forces,
+                         (const void*)k_scatter, (const void*)k_stable_gather, (const void*)k_rmax, (const void*)k_forces,
                          (const void*)k_neighbors, (const void*)k_max_bin, (const void*)k_integrate,
                          (const void*)k_sec_keys, (const void*)k_sec_flags, (const void*)k_sec_groups<double>,
                          (const void*)k_sec_groups<float>,
@@ -462,7 +503,12 @@ HashLayout hash_layout(uint32_t n_mvox, uint32_t max_cells) {
     L.sorted = o;  o = align256(o + (size_t)max_cells * 4);
     L.rec = o;     o = align256(o + (size_t)max_cells * sizeof(CellRec));
     L.pos_of = o;  o = align256(o + (size_t)max_cells * 4);
-    L.pos32 = o;   o = align256(o + (size_t)max_cells * sizeof(float4));
+    // pos32 carries MARKX_STEP entries of padding: the mark pass loads whole
+    // steps past the end of a range and masks the extra entries
+    L.pos32 = o;   o = align256(o + ((size_t)max_cells + MARKX_STEP) * sizeof(float4));
+    // largest radius: one partial per gather block, then the maximum
+    L.rpart = o;   o = align256(o + (size_t)blocks_for(max_cells ? max_cells : 1, 256) * 4);
+    L.rmax = o;    o = align256(o + 4);
     L.partial = o; o = align256(o + scan_partial_bytes((uint64_t)n_mvox + 1));
     L.total = o;
     return L;
@@ -729,6 +775,7 @@ struct HashPtrs {
     uint32_t *counts, *offsets, *slot, *tmp, *sorted, *partial, *pos_of;
     CellRec* rec;
     float4* pos32;
+    float *rpart, *rmax;
 };
 
 static HashPtrs hash_ptrs(hc_buf* hash, uint32_t n_mvox, uint32_t max_cells) {
@@ -744,6 +791,8 @@ static HashPtrs hash_ptrs(hc_buf* hash, uint32_t n_mvox, uint32_t max_cells) {
     h.rec = (CellRec*)(P + L.rec);
     h.pos_of = (uint32_t*)(P + L.pos_of);
     h.pos32 = (float4*)(P + L.pos32);
+    h.rpart = (float*)(P + L.rpart);
+    h.rmax = (float*)(P + L.rmax);
     return h;
 }
 
@@ -855,6 +904,11 @@ static bool mark32_ok(const hc_mech* mech) {
     return mech->mechanics_voxel_size > 0.0 && mech->mechanics_voxel_size <= 32.0 &&
            mech->max_interaction_distance <= 32.0;
 }
+// ... and its x relative to the grid's x_min (force_agent64x) when the grid's
+// x extent keeps every FP32 x of a candidate in range below 4096 um
+static int markx_ok(const hc_mech* mech) {
+    return mark32_ok(mech) && (double)mech->mech_grid_nx * mech->mechanics_voxel_size <= MARKX_MAX_EXTENT ? 1 : 0;
+}
 
 static int check_hash(hc_ctx* ctx, hc_buf* hash, uint32_t n_mvox, uint32_t max_cells, const char* who) {
     if (!hash || n_mvox == 0) return fail(ctx, HC_ERR_ARG, std::string(who) + ": bad hash argument");
@@ -894,7 +948,10 @@ int hc_build_hash(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech,
         else
             hipLaunchKernelGGL(k_stable_gather, dim3(nb), dim3(256), 0, ctx->stream, cp, key, h.offsets, h.tmp,
                                h.sorted, h.pos_of, h.rec, mark32_ok(mech) ? h.pos32 : nullptr, n, mgrid->x_min, mgrid->y_min,
-                               mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny);
+                               mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny,
+                               markx_ok(mech), markx_ok(mech) ? h.rpart : nullptr);
+        if (!mech->forces_fp32 && markx_ok(mech))
+            hipLaunchKernelGGL(k_rmax, dim3(1), dim3(256), 0, ctx->stream, h.rpart, nb, h.rmax);
         if ((r = check_launch(ctx, "hash scatter"))) return r;
     }
     return 0;
@@ -919,7 +976,8 @@ int hc_forces(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech, uin
         hipLaunchKernelGGL(k_forces, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, h.rec,
                            mark32_ok(mech) ? h.pos32 : nullptr, h.offsets, cp.f[HC_F_VEL_X], cp.f[HC_F_VEL_Y],
                            cp.f[HC_F_VEL_Z], cp.f[HC_F_PRESSURE], n, mech->mech_grid_nx, mech->mech_grid_ny,
-                           mech->mech_grid_nz, mech->max_interaction_distance, (float)mech->mechanics_voxel_size);
+                           mech->mech_grid_nz, mech->max_interaction_distance, (float)mech->mechanics_voxel_size,
+                           markx_ok(mech), h.rmax);
     return check_launch(ctx, "k_forces");
 }
 

@@ -373,6 +373,115 @@ __device__ __attribute__((always_inline)) inline void force_range64m(
     }
 }
 
+// FP64 forces of one agent with an FP32 mark pass over all its neighbour
+// rows and a per-lane list of the marked candidates in LDS.
+//
+// pos32 holds each hash-sorted agent's x relative to the grid's x_min (valid
+// while the grid's x extent is at most MARKX_MAX_EXTENT), y and z relative to
+// the lower corner of its own bin, and its radius, in FP32.  A member of row
+// (dj, dk) then differs from the agent by (x_a - x_m, y_a - dj*v - y_m,
+// z_a - dk*v - z_m) with no per-member column offset.  Every candidate truly
+// within the cutoff lies within 4096 um of x_min, so its FP32 x is within
+// 1.22e-4 um and x differences within 2.5e-4 um; y and z differences are
+// within 3e-5 um (bin-relative values below 64 um); d^2 of a pair within
+// 32 um is then within 0.025 um^2 of the exact value, interaction radius and
+// cutoff included, and the threshold carries MARKX_SLACK = 0.05 um^2.
+//
+// A candidate passing the FP32 test is appended to the lane's list (list
+// entry k at list[k * 64]); the FP64 law then walks the list in order with
+// the exact predicates and expressions of force_range64, so the pairs that
+// contribute, their order (rows q = 0..8, ascending sorted position) and
+// every bit of the sums are force_range64's.  The agent itself passes the
+// FP32 test and is dropped by the exact test (d^2 < 1e-12).  The list is
+// walked whenever the next step could overflow it, and at the end, so the
+// wave runs the FP64 law about as often as its busiest lane has neighbours,
+// not once per chunk of candidates.
+constexpr double MARKX_MAX_EXTENT = 3968.0;   // x extent of the grid; plus 2 bins + cutoff < 4096
+constexpr float MARKX_SLACK = 0.05f;
+constexpr uint32_t MARKX_STEP = 8;            // candidates per step (their loads in flight together)
+constexpr uint32_t MARKX_CAP = 24;            // list entries per lane
+
+__device__ __attribute__((always_inline)) inline void force_list64(
+    const CellRec* __restrict__ rec, const uint32_t* __restrict__ list, uint32_t cnt, double px, double py,
+    double pz, double radius_i, double rep_i, double adh_i, double rel_max_adh, double cut2, double& fx, double& fy,
+    double& fz, double& local_pressure) {
+    const double relf = rel_max_adh > 1.0 ? rel_max_adh : 1.0;
+    constexpr double margin = 1.0 + 0x1p-40;
+    for (uint32_t k = 0; k < cnt; k++) {
+        const uint32_t u = list[k * 64];
+        const double2 xy = *reinterpret_cast<const double2*>(&rec[u].x);
+        const double2 zr = *reinterpret_cast<const double2*>(&rec[u].z);
+        const double ddx = px - xy.x;
+        const double ddy = py - xy.y;
+        const double ddz = pz - zr.x;
+        const double dist_sq = ddx * ddx + ddy * ddy + ddz * ddz;
+        const double Lx = (radius_i + zr.y) * relf;
+        if (dist_sq >= cut2 || dist_sq < 1e-12 || !(dist_sq < Lx * Lx * margin)) continue;
+        const double dist = sqrt(dist_sq);
+        const double inv_dist = 1.0 / dist;
+        const double nxv = ddx * inv_dist;
+        const double nyv = ddy * inv_dist;
+        const double nzv = ddz * inv_dist;
+        const double R_sum = radius_i + zr.y;
+        double force_mag = 0.0;
+        if (dist < R_sum) {
+            const double overlap = 1.0 - dist / R_sum;
+            force_mag += overlap * overlap * sqrt(rep_i * rec[u].rep);
+            local_pressure += overlap * overlap / 0.027288820670331;
+        }
+        const double adhesion_distance = rel_max_adh * R_sum;
+        if (dist < adhesion_distance) {
+            const double t2 = 1.0 - dist / adhesion_distance;
+            force_mag -= t2 * t2 * sqrt(adh_i * rec[u].adh);
+        }
+        fx += force_mag * nxv;
+        fy += force_mag * nyv;
+        fz += force_mag * nzv;
+    }
+}
+
+// m: the agent's own pos32 entry; rmax: the largest radius of any agent,
+// rounded up to FP32; list: this lane's first entry in LDS.  The FP32 test
+// uses the agent's largest interaction radius, (r_i + rmax) * max(1, rel),
+// for every candidate: no smaller than the pair's, so no pair is lost, and
+// the same for all candidates, so the test is a comparison with one
+// threshold.  pos32 is padded by MARKX_STEP entries: a step loads past the
+// end of its range and masks the extra entries.
+__device__ __attribute__((always_inline)) inline void force_agent64x(
+    const CellRec* __restrict__ rec, const float4* __restrict__ pos32, const uint32_t* __restrict__ offsets,
+    uint32_t* __restrict__ list, uint32_t bin0, uint32_t vi, uint32_t vj, uint32_t vk, uint32_t mnx, uint32_t mny,
+    uint32_t mnz, float4 m, float voxel, float rmax, double px, double py, double pz, double radius_i, double rep_i,
+    double adh_i, double rel_max_adh, double cut2, double& fx, double& fy, double& fz, double& local_pressure) {
+    const float relf32 = (float)(rel_max_adh > 1.0 ? rel_max_adh : 1.0);
+    const float L = ((float)radius_i + rmax) * relf32;
+    const float lim = fminf(L * L * (1.0f + 0x1p-16f), (float)cut2) + MARKX_SLACK;
+    uint32_t cnt = 0;
+#pragma unroll 1
+    for (int q = 0; q < 9; q++) {
+        const RowRange rr = neighbor_row(offsets, bin0, vi, vj, vk, mnx, mny, mnz, q);
+        const float my = m.y - (float)rr.dj * voxel, mz = m.z - (float)rr.dk * voxel;
+        for (uint32_t u0 = rr.beg; u0 < rr.end; u0 += MARKX_STEP) {
+            const float4* p = pos32 + u0;
+            float4 o[MARKX_STEP];
+#pragma unroll
+            for (uint32_t j = 0; j < MARKX_STEP; j++) o[j] = p[j];
+#pragma unroll
+            for (uint32_t j = 0; j < MARKX_STEP; j++) {
+                const float ddx = m.x - o[j].x, ddy = my - o[j].y, ddz = mz - o[j].z;
+                const float d2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(ddy, ddy, ddx * ddx));
+                list[cnt * 64] = u0 + j;
+                cnt += (u0 + j < rr.end && d2 < lim) ? 1u : 0u;
+            }
+            if (cnt > MARKX_CAP - MARKX_STEP) {
+                force_list64(rec, list, cnt, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz,
+                             local_pressure);
+                cnt = 0;
+            }
+        }
+    }
+    force_list64(rec, list, cnt, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz, local_pressure);
+}
+
 // FP32 form: positions are relative to each agent's own bin corner.  A
 // member's bin corner relative to the agent's is (di, dj, dk) * voxel: dj, dk
 // are the row's (oy, oz), and di = -1, 0, +1 for members before mid0 (the

@@ -2,9 +2,12 @@
 
     python -m physicell_amd.profsummary trace  <results.db>
     python -m physicell_amd.profsummary traffic <fetch.db> <write.db> <copy_fetch.db> <copy_write.db> [--json out]
+    python -m physicell_amd.profsummary counters <pmc.db> [<pmc.db> ...] [--kernels a,b,...]
 
 `trace` prints per-kernel calls / total / average duration (the
-`--kernel-trace --stats` view).  `traffic` turns per-dispatch FETCH_SIZE and
+`--kernel-trace --stats` view).  `counters` prints the per-dispatch average
+of every counter found in the databases, per kernel (optionally only the
+kernels whose names contain one of the given substrings).  `traffic` turns per-dispatch FETCH_SIZE and
 WRITE_SIZE readings (KiB) into bytes per launch, scaled by what the same
 counters read for a plain device copy of known size (1 GiB read + 1 GiB
 written, `bench.py --calibrate-copy`): scale = known bytes / counter bytes.
@@ -71,9 +74,36 @@ def traffic(fetch_db, write_db, cfetch_db, cwrite_db):
     return res
 
 
+def counters(dbs, kernels=None):
+    """{kernel: {counter: per-dispatch average}} over the given databases."""
+    out = {}
+    for db in dbs:
+        c = sqlite3.connect(db)
+        rows = c.execute("select kernel_name, counter_name, value from counters_collection")
+        acc = {}
+        for name, cn, val in rows:
+            acc.setdefault((name, cn), []).append(val)
+        for (name, cn), vals in acc.items():
+            if kernels and not any(k in name for k in kernels):
+                continue
+            out.setdefault(re.sub(r"[(].*", "", name), {})[cn] = sum(vals) / len(vals)
+    return out
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "trace":
         print(trace(sys.argv[2]))
+    elif sys.argv[1] == "counters":
+        args = sys.argv[2:]
+        ks = None
+        if "--kernels" in args:
+            i = args.index("--kernels")
+            ks = args[i + 1].split(",")
+            args = args[:i] + args[i + 2:]
+        for k, v in sorted(counters(args, ks).items()):
+            print(k)
+            for cn, val in sorted(v.items()):
+                print(f"    {cn:<28s} {val:16.1f}")
     elif sys.argv[1] == "traffic":
         r = traffic(*sys.argv[2:6])
         if "--json" in sys.argv:
