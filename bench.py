@@ -14,8 +14,11 @@ Prints ONE JSON line (rank 0) with agent-steps/s as `value`, voxel-substeps/s,
 p50/p99 step latency, the roofline of the dominant kernel (its algorithmic
 bytes over its average launch time, from HIP events on the context stream
 around each launch in an eager serial pass of the same periods right after
-the timed region) and the CPU baseline (the FP64 oracle on this host's cores,
-bounded sample).
+the timed region), the whole step's roofline (`step_frac`: the step's
+algorithmic bytes over ms_per_step), the CPU baseline (the FP64 oracle on
+this host's cores, bounded sample) and, under `ensemble`, the 64-member
+ensemble timed in its three forms (batched FP32-mixed, batched FP64, one
+plain FP64 simulation per member) with the batched forms' speedups.
 """
 import argparse
 import ctypes
@@ -63,6 +66,25 @@ def algorithmic_bytes(kernel, spec, n_cells, mixed=False):
     return None
 
 
+def step_bytes(spec, n_cells, substeps, mixed=False):
+    """Algorithmic HBM bytes of one step (substeps diffusion substeps + one
+    mechanics step), by unit of work: a voxel-substep (VS) moves 16 B per
+    sweep in FP64 (8 B in FP32); an agent-substep of secretion (ASS) reads a
+    4-byte key and per substrate two folded coefficients and the density RMW
+    (4 + 32 S B; 4 + 24 S B mixed); an agent-step (AS) of mechanics moves
+    248 B (180 B mixed): key, sort, forces and AB2 traffic."""
+    ns = len(spec.substrates)
+    sweeps = 2 if spec.nz == 1 else 3
+    vs = spec.n_voxels * ns * substeps
+    fields = vs * (8.0 if mixed else 16.0) * sweeps
+    secretion = n_cells * substeps * (4.0 + (24.0 if mixed else 32.0) * ns)
+    mechanics = n_cells * (180.0 if mixed else 248.0)
+    return {"fields": fields, "secretion": secretion, "mechanics": mechanics,
+            "total": fields + secretion + mechanics,
+            "basis": f"VS {vs} x {(8 if mixed else 16) * sweeps} B, ASS {n_cells * substeps} x "
+                     f"{4 + (24 if mixed else 32) * ns} B, AS {n_cells} x {180 if mixed else 248} B"}
+
+
 def cpu_baseline(spec, cells, periods, substeps):
     """FP64 oracle (the reference's algorithm restated in C++, OpenMP) on this
     host: `periods` x (substeps diffusion substeps + 1 mechanics step)."""
@@ -98,34 +120,30 @@ def cpu_baseline(spec, cells, periods, substeps):
                       f"1 thread, {dt1:.1f} s"}
 
 
-def bench_ensemble(args):
-    """64-simulation ensemble (members ensemble_member(100..163)): the batched
-    FP32-mixed form against the batched FP64 form and against the plain FP64
-    single-simulation path (one pc_sim per member, each replaying its own
-    period graph), all on the same members.  One step = one period of every
-    member (10 diffusion substeps + 1 mechanics step)."""
-    t_setup = time.perf_counter()
-    members = S.config3(n_sims=args.members)
+def ensemble_forms(members, steps, warmup, overlap=True):
+    """The same ensemble members in three forms, each timed over `steps`
+    periods after `warmup`: the batched FP32-mixed ensemble, the batched FP64
+    ensemble, and the plain FP64 single-simulation path (one pc_sim per
+    member, each replaying its own period graph on its own stream).  One step
+    = one period of every member (10 diffusion substeps + 1 mechanics step)."""
     n_cells = sum(c.n for _, c in members)
     n_vals = sum(sc.n_voxels * len(sc.substrates) for sc, _ in members)
     n_sub = 10
-    print(f"[bench] ensemble: {len(members)} members, {n_cells} agents, {n_vals} voxel values; "
-          f"setup {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
 
     def time_ensemble(mixed):
         ens = H.Ensemble(mixed=mixed)
         for sc, c in members:
             ens.add(sc, c)
-        ens.set_overlap(not args.no_overlap)
+        ens.set_overlap(overlap)
         blocks, us = ens.autotune(reps=3)
-        ens.run_periods(args.warmup)
+        ens.run_periods(warmup)
         ens.wait()
         t0 = time.perf_counter()
-        ens.run_periods(args.steps)
+        ens.run_periods(steps)
         ens.wait()
         T = time.perf_counter() - t0
         lat = []
-        for _ in range(max(10, min(args.steps, 50))):
+        for _ in range(max(10, min(steps, 50))):
             t1 = time.perf_counter()
             ens.run_periods(1)
             ens.wait()
@@ -141,8 +159,8 @@ def bench_ensemble(args):
                 kern[name] = {"total_ms": ms, "launches": n, "avg_us": 1e3 * ms / n}
         ens.timing(0)
         ens.close()
-        return {"ms_per_step": 1e3 * T / args.steps, "agent_steps_per_s": n_cells * args.steps / T,
-                "voxel_substeps_per_s": n_vals * n_sub * args.steps / T,
+        return {"ms_per_step": 1e3 * T / steps, "agent_steps_per_s": n_cells * steps / T,
+                "voxel_substeps_per_s": n_vals * n_sub * steps / T,
                 "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
                 "autotune": {"grid_blocks": blocks, "sweep_us": us}, "kernels": kern}
 
@@ -154,25 +172,51 @@ def bench_ensemble(args):
     # streams the device is already busy, and a second stream per member
     # for the mechanics step (overlap) measured slower (15.0 vs 11.3 ms/step)
     sims = [H.Simulation(sc, c) for sc, c in members]
-    for s in sims:
-        s.set_overlap(False)
-        s.run_periods(args.warmup, use_graph=True)
-    for s in sims:
-        s.wait()
+    for sim in sims:
+        sim.set_overlap(False)
+        sim.run_periods(warmup, use_graph=True)
+    for sim in sims:
+        sim.wait()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        for s in sims:
-            s.run_periods(1, use_graph=True)
-    for s in sims:
-        s.wait()
+    for _ in range(steps):
+        for sim in sims:
+            sim.run_periods(1, use_graph=True)
+    for sim in sims:
+        sim.wait()
     T = time.perf_counter() - t0
-    res["plain_fp64_per_sim"] = {"ms_per_step": 1e3 * T / args.steps, "agent_steps_per_s": n_cells * args.steps / T,
-                                 "voxel_substeps_per_s": n_vals * n_sub * args.steps / T,
+    res["plain_fp64_per_sim"] = {"ms_per_step": 1e3 * T / steps, "agent_steps_per_s": n_cells * steps / T,
+                                 "voxel_substeps_per_s": n_vals * n_sub * steps / T,
                                  "setting": "one pc_sim per member, own stream, period graph, no overlap"}
-    for s in sims:
-        s.close()
+    for sim in sims:
+        sim.close()
     print(f"[bench] plain FP64 per-sim: {res['plain_fp64_per_sim']['ms_per_step']:.2f} ms/step",
           file=sys.stderr, flush=True)
+    return res
+
+
+def ensemble_summary(res):
+    """ms/step of the three forms and the ratios of the batched forms to the
+    plain FP64 single-simulation path (agent-steps/s and voxel-substeps/s
+    ratios are equal: every form runs the same members and periods)."""
+    pl = res["plain_fp64_per_sim"]["ms_per_step"]
+    return {"batched_mixed_ms_per_step": res["batched_mixed"]["ms_per_step"],
+            "batched_fp64_ms_per_step": res["batched_fp64"]["ms_per_step"],
+            "plain_fp64_per_sim_ms_per_step": pl,
+            "speedup_batched_mixed_vs_plain_fp64": pl / res["batched_mixed"]["ms_per_step"],
+            "speedup_batched_fp64_vs_plain_fp64": pl / res["batched_fp64"]["ms_per_step"]}
+
+
+def bench_ensemble(args):
+    """64-simulation ensemble (members ensemble_member(100..163)): the three
+    forms of ensemble_forms, with the batched FP32-mixed form as the line."""
+    t_setup = time.perf_counter()
+    members = S.config3(n_sims=args.members)
+    n_cells = sum(c.n for _, c in members)
+    n_vals = sum(sc.n_voxels * len(sc.substrates) for sc, _ in members)
+    n_sub = 10
+    print(f"[bench] ensemble: {len(members)} members, {n_cells} agents, {n_vals} voxel values; "
+          f"setup {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
+    res = ensemble_forms(members, args.steps, args.warmup, overlap=not args.no_overlap)
     bm, pl = res["batched_mixed"], res["plain_fp64_per_sim"]
     kern = bm["kernels"]
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
@@ -185,6 +229,12 @@ def bench_ensemble(args):
             "avg_launch_us": kern[dom]["avg_us"], "algorithmic_bytes_per_launch": ab,
             "achieved": ab / (kern[dom]["avg_us"] * 1e-6) / 1e9 if ab else None}
     roof["frac"] = roof["achieved"] / HBM_PEAK_GBS if ab else None
+    # the whole step: every member's algorithmic bytes (mixed form) over the step time
+    sb = [step_bytes(sc, c.n, n_sub, mixed=True) for sc, c in members]
+    tot = {k: sum(b[k] for b in sb) for k in ("fields", "secretion", "mechanics", "total")}
+    roof["step_achieved"] = tot["total"] / (bm["ms_per_step"] * 1e-3) / 1e9
+    roof["step_frac"] = roof["step_achieved"] / HBM_PEAK_GBS
+    roof["step_bytes"] = tot
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -215,8 +265,7 @@ def bench_ensemble(args):
                                f"({n_sub} substeps + 1 mechanics step) of every member", "graph": True},
         "voxel_substeps_per_s": bm["voxel_substeps_per_s"], "p50_step_ms": bm["p50_step_ms"],
         "p99_step_ms": bm["p99_step_ms"], "roofline": roof, "cpu_baseline": cpu,
-        "speedup_vs_plain_fp64_path": bm["agent_steps_per_s"] / pl["agent_steps_per_s"],
-        "speedup_batched_fp64_vs_plain": res["batched_fp64"]["agent_steps_per_s"] / pl["agent_steps_per_s"],
+        "ensemble": ensemble_summary(res),
         "forms": res,
     }
     print(json.dumps(out))
@@ -392,6 +441,10 @@ def main():
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_config2.json"))
     ap.add_argument("--members", type=int, default=64, help="ensemble size (workload config3)")
     ap.add_argument("--cpu-members", type=int, default=64, help="members in the CPU sample (workload config3)")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="leave out the ensemble leg of the default line (config3 members in the batched mixed, "
+                         "batched FP64 and per-member FP64 forms)")
+    ap.add_argument("--ensemble-steps", type=int, default=20, help="timed steps per form in the ensemble leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain FP64 device copy of 2^27 elements (1 GiB read + 1 GiB "
                          "written) five times, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -498,6 +551,23 @@ def main():
     n_sub = k_sub or 10
     value = cells.n * args.steps / T
     vss = spec.n_voxels * len(spec.substrates) * n_sub * args.steps / T
+    # the whole step's roofline: algorithmic bytes of one step over its time
+    sb = step_bytes(spec, cells.n, n_sub, args.mixed)
+    roof["step_achieved"] = sb["total"] / (ms_per_step * 1e-3) / 1e9
+    roof["step_frac"] = roof["step_achieved"] / HBM_PEAK_GBS
+    roof["step_bytes"] = sb
+    sim.close()
+    ens = None
+    if not args.no_ensemble and args.workload == "config2":
+        t1 = time.perf_counter()
+        members = S.config3(n_sims=args.members)
+        print(f"[bench] ensemble leg: {len(members)} members, {sum(c.n for _, c in members)} agents; setup "
+              f"{time.perf_counter() - t1:.1f} s", file=sys.stderr, flush=True)
+        ens = ensemble_summary(ensemble_forms(members, args.ensemble_steps, 3))
+        ens["workload"] = (f"config3 ensemble: {len(members)} members (seeds 100..{99 + len(members)}), "
+                           f"{sum(c.n for _, c in members)} agents, "
+                           f"{sum(sc.n_voxels * len(sc.substrates) for sc, _ in members)} voxel values; "
+                           f"step = one period of every member; {args.ensemble_steps} timed steps per form")
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
@@ -515,9 +585,8 @@ def main():
                    "substep_order": args.substep_order},
         "voxel_substeps_per_s": vss,
         "p50_step_ms": float(np.percentile(lat, 50)), "p99_step_ms": float(np.percentile(lat, 99)),
-        "roofline": roof, "kernels": kern, "cpu_baseline": cpu,
+        "roofline": roof, "kernels": kern, "cpu_baseline": cpu, "ensemble": ens,
     }
-    sim.close()
     print(json.dumps(out))
 
 
