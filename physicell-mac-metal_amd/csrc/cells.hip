@@ -200,16 +200,9 @@ __global__ __launch_bounds__(256) void k_stable_gather(CellPtrs cp, const uint32
                                                        double voxel, uint32_t mnx, uint32_t mny, int x_grid,
                                                        float* __restrict__ rpart) {
     const uint32_t c = blockIdx.x * 256 + threadIdx.x;
-    if (rpart) {
-        // this block's largest radius, rounded up to FP32 (no lane returns
-        // before the barrier)
-        __shared__ float wmax[4];
-        float r = c < n ? __double2float_ru(cp.f[HC_F_RADIUS][c]) : 0.0f;
-        for (int d = 32; d >= 1; d >>= 1) r = fmaxf(r, __shfl_xor(r, d, 64));
-        if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = r;
-        __syncthreads();
-        if (threadIdx.x == 0) rpart[blockIdx.x] = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
-    }
+    // this block's largest radius, rounded up to FP32 (no lane returns before
+    // the barrier)
+    if (rpart) block_max_to(c < n ? __double2float_ru(cp.f[HC_F_RADIUS][c]) : 0.0f, rpart);
     if (c >= n) return;
     const uint32_t b = key[c];
     const uint32_t beg = offsets[b], end = offsets[b + 1];
@@ -598,8 +591,10 @@ __global__ __launch_bounds__(256) void k_stable_gather32(CellPtrs cp, const uint
                                                          const uint32_t* __restrict__ pos_of,
                                                          CellRec32* __restrict__ rec, uint32_t n, double x_min,
                                                          double y_min, double z_min, double voxel, uint32_t mnx,
-                                                         uint32_t mny) {
+                                                         uint32_t mny, float* __restrict__ rpart) {
     const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    // this block's largest FP32 radius (the values the records hold)
+    block_max_to(c < n ? (float)cp.f[HC_F_RADIUS][c] : 0.0f, rpart);
     if (c >= n) return;
     const uint32_t b = key[c];
     const uint32_t beg = offsets[b], end = offsets[b + 1];
@@ -623,11 +618,18 @@ __global__ __launch_bounds__(256) void k_stable_gather32(CellPtrs cp, const uint
     rec[pos] = r;
 }
 
+// LIST: the mark pass with a list of marked candidates (force_agent32x; rmax
+// is the largest FP32 radius); else the chunked pass (force_range32), which
+// populations of 2^27 agents or more take.  Two kernels, so that each gets
+// the registers its own path needs.
+template <bool LIST>
 __global__ __launch_bounds__(256) void k_forces32(const CellRec32* __restrict__ rec, const uint32_t* __restrict__ offsets,
                                                   const double* __restrict__ rel_adh, double* __restrict__ vx,
                                                   double* __restrict__ vy, double* __restrict__ vz,
                                                   double* __restrict__ pressure, uint32_t n, uint32_t mnx, uint32_t mny,
-                                                  uint32_t mnz, float max_dist, float voxel) {
+                                                  uint32_t mnz, float max_dist, float voxel,
+                                                  const float* __restrict__ rmax) {
+    __shared__ uint32_t marks[LIST ? 4 * MARKX_CAP * 64 : 1];
     const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (t >= n) return;
     const CellRec32 me = rec[t];
@@ -636,12 +638,18 @@ __global__ __launch_bounds__(256) void k_forces32(const CellRec32* __restrict__ 
     float fx = 0.f, fy = 0.f, fz = 0.f, local_pressure = 0.f;
     const uint32_t vi = me.key % mnx, vj = (me.key / mnx) % mny, vk = me.key / (mnx * mny);
     const float cut2 = max_dist * max_dist;
+    if (LIST) {
+        uint32_t* list = marks + (threadIdx.x >> 6) * (MARKX_CAP * 64) + (threadIdx.x & 63);
+        force_agent32x(rec, offsets, list, t, 0u, vi, vj, vk, mnx, mny, mnz, me.x, me.y, me.z, voxel, rmax[0],
+                       radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz, local_pressure);
+    } else {
 #pragma unroll 1
-    for (int q = 0; q < 9; q++) {
-        const RowRange rr = neighbor_row(offsets, 0u, vi, vj, vk, mnx, mny, mnz, q);
-        force_range32(rec, rr.beg, rr.end, t, rr.mid0, rr.mid1, me.x, me.y, me.z, (float)rr.dj * voxel,
-                      (float)rr.dk * voxel, voxel, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz,
-                      local_pressure);
+        for (int q = 0; q < 9; q++) {
+            const RowRange rr = neighbor_row(offsets, 0u, vi, vj, vk, mnx, mny, mnz, q);
+            force_range32(rec, rr.beg, rr.end, t, rr.mid0, rr.mid1, me.x, me.y, me.z, (float)rr.dj * voxel,
+                          (float)rr.dk * voxel, voxel, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz,
+                          local_pressure);
+        }
     }
     const uint32_t c = me.idx;
     vx[c] = fx;
@@ -732,7 +740,7 @@ int totals_scratch_init(hc_ctx* ctx) {
 int probe_kernels_cells2(hc_ctx* ctx);
 int probe_kernels_cells2(hc_ctx* ctx) {
     hipFuncAttributes attr;
-    const void* fns[] = {(const void*)k_stable_sort, (const void*)k_stable_gather32, (const void*)k_forces32,
+    const void* fns[] = {(const void*)k_stable_sort, (const void*)k_stable_gather32, (const void*)k_forces32<true>, (const void*)k_forces32<false>,
                          (const void*)k_bin_sort,
                          (const void*)k_totals<double>, (const void*)k_totals<float>, (const void*)k_totals_final};
     for (const void* f : fns) {
@@ -944,13 +952,14 @@ int hc_build_hash(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech,
         if (mech->forces_fp32)
             hipLaunchKernelGGL(k_stable_gather32, dim3(nb), dim3(256), 0, ctx->stream, cp, key, h.offsets, h.tmp,
                                h.sorted, h.pos_of, reinterpret_cast<CellRec32*>(h.rec), n, mgrid->x_min, mgrid->y_min,
-                               mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny);
+                               mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny,
+                               h.rpart);
         else
             hipLaunchKernelGGL(k_stable_gather, dim3(nb), dim3(256), 0, ctx->stream, cp, key, h.offsets, h.tmp,
                                h.sorted, h.pos_of, h.rec, mark32_ok(mech) ? h.pos32 : nullptr, n, mgrid->x_min, mgrid->y_min,
                                mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny,
                                markx_ok(mech), markx_ok(mech) ? h.rpart : nullptr);
-        if (!mech->forces_fp32 && markx_ok(mech))
+        if (mech->forces_fp32 || markx_ok(mech))
             hipLaunchKernelGGL(k_rmax, dim3(1), dim3(256), 0, ctx->stream, h.rpart, nb, h.rmax);
         if ((r = check_launch(ctx, "hash scatter"))) return r;
     }
@@ -967,11 +976,12 @@ int hc_forces(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech, uin
     CellPtrs cp = cell_ptrs(cells->dptr, max_cells);
     KernelTimer tm(ctx, HC_K_FORCES);
     if (mech->forces_fp32)
-        hipLaunchKernelGGL(k_forces32, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(n < (1u << MARK32_POS_BITS) ? k_forces32<true> : k_forces32<false>,
+                           dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream,
                            reinterpret_cast<const CellRec32*>(h.rec), h.offsets, cp.f[HC_F_REL_MAX_ADH],
                            cp.f[HC_F_VEL_X], cp.f[HC_F_VEL_Y], cp.f[HC_F_VEL_Z], cp.f[HC_F_PRESSURE], n,
                            mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz,
-                           (float)mech->max_interaction_distance, (float)mech->mechanics_voxel_size);
+                           (float)mech->max_interaction_distance, (float)mech->mechanics_voxel_size, h.rmax);
     else
         hipLaunchKernelGGL(k_forces, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, h.rec,
                            mark32_ok(mech) ? h.pos32 : nullptr, h.offsets, cp.f[HC_F_VEL_X], cp.f[HC_F_VEL_Y],

@@ -125,6 +125,17 @@ __device__ inline void bin_sort_one(const uint32_t* __restrict__ offsets, const 
         }
 }
 
+// The largest r of a 256-thread block into part[blockIdx.x]; every thread of
+// the block must call it (it holds a barrier).
+__device__ inline void block_max_to(float r, float* __restrict__ part) {
+    __shared__ float wmax[4];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) r = fmaxf(r, __shfl_xor(r, d, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+}
+
 // Counter-based uniform stream on [0, 1): the SplitMix64 finaliser (Steele,
 // Lea & Flood 2014) applied to (seed, step, cell, draw).  Stateless, so graph
 // replays and parallel lanes need no generator state.
@@ -480,6 +491,110 @@ __device__ __attribute__((always_inline)) inline void force_agent64x(
         }
     }
     force_list64(rec, list, cnt, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz, local_pressure);
+}
+
+// FP32 forces of one agent with a mark pass and a per-lane list of the
+// marked candidates in LDS (the mixed form; force_range32's result bit for
+// bit).
+//
+// Mark: one threshold per agent, min(L^2 (1 + 2^-16), cut^2) + MARK32_SLACK
+// with L = (r_i + rmax) * max(1, rel) in FP32, rmax the largest FP32 radius of
+// any agent.  FP32 addition and multiplication are monotonic, so L is no
+// smaller than any pair's (r_i + r_j) * max(1, rel) and the threshold no
+// smaller than any pair's bound in force_range32; the mark's d^2 (by fma,
+// with the column offset folded into the agent's coordinate) is within
+// 1e-3 um^2 of force_range32's for pairs within 32 um, far below the slack.
+// Every pair force_range32 keeps is therefore marked.  A marked candidate is
+// listed with its bin offset code (di + 1) + 3 (dj + 1) + 9 (dk + 1) in the
+// top 5 bits of the entry, and the law walks the list in visit order (rows
+// q = 0..8, ascending sorted position), re-testing each pair with
+// force_range32's exact predicates and expressions, so the pairs that
+// contribute, their order and every bit of the sums are force_range32's.
+// Sorted positions must stay below 2^27 (callers check).
+constexpr float MARK32_SLACK = 0.01f;
+constexpr uint32_t MARK32_POS_BITS = 27;
+constexpr uint32_t MARK32_STEP = 8;           // candidates per step (their loads in flight together)
+
+__device__ __attribute__((always_inline)) inline void force_list32(
+    const CellRec32* __restrict__ rec, const uint32_t* __restrict__ list, uint32_t cnt, uint32_t t, float mx,
+    float my, float mz, float voxel, float radius_i, float rep_i, float adh_i, float rel_max_adh, float cut2,
+    float& fx, float& fy, float& fz, float& local_pressure) {
+    const float relf = rel_max_adh > 1.0f ? rel_max_adh : 1.0f;
+    constexpr float margin = 1.0f + 0x1p-16f;
+    for (uint32_t k = 0; k < cnt; k++) {
+        const uint32_t e = list[k * 64];
+        const uint32_t u = e & ((1u << MARK32_POS_BITS) - 1u), code = e >> MARK32_POS_BITS;
+        if (u == t) continue;
+        const uint32_t cj = code / 3u;
+        const float ox = (float)((int)(code - 3u * cj) - 1) * voxel;
+        const float oy = (float)((int)(cj % 3u) - 1) * voxel, oz = (float)((int)(cj / 3u) - 1) * voxel;
+        const float4 o = *reinterpret_cast<const float4*>(&rec[u].x);
+        const float ddx = mx - o.x - ox, ddy = my - o.y - oy, ddz = mz - o.z - oz;
+        const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        const float L = (radius_i + o.w) * relf;
+        if (d2 >= cut2 || d2 < 1e-12f || !(d2 < L * L * margin)) continue;
+        // the hardware square root and reciprocal (about 1 ulp), as force_range32
+        const float dist = __builtin_amdgcn_sqrtf(d2);
+        const float inv_dist = __builtin_amdgcn_rcpf(dist);
+        const float R_sum = radius_i + o.w;
+        float force_mag = 0.f;
+        if (dist < R_sum) {
+            const float overlap = 1.0f - dist * __builtin_amdgcn_rcpf(R_sum);
+            force_mag += overlap * overlap * __builtin_amdgcn_sqrtf(rep_i * rec[u].rep);
+            local_pressure += overlap * overlap * (1.0f / 0.027288820670331f);
+        }
+        const float adhesion_distance = rel_max_adh * R_sum;
+        if (dist < adhesion_distance) {
+            const float t2 = 1.0f - dist * __builtin_amdgcn_rcpf(adhesion_distance);
+            force_mag -= t2 * t2 * __builtin_amdgcn_sqrtf(adh_i * rec[u].adh);
+        }
+        fx += force_mag * (ddx * inv_dist);
+        fy += force_mag * (ddy * inv_dist);
+        fz += force_mag * (ddz * inv_dist);
+    }
+}
+
+// mx, my, mz: the agent's position relative to its own bin corner; t its
+// sorted position; bin0 the key of the simulation's bin 0; rmax as above;
+// list: this lane's first entry in LDS (entry k at list[k * 64]).
+__device__ __attribute__((always_inline)) inline void force_agent32x(
+    const CellRec32* __restrict__ rec, const uint32_t* __restrict__ offsets, uint32_t* __restrict__ list,
+    uint32_t t, uint32_t bin0, uint32_t vi, uint32_t vj, uint32_t vk, uint32_t mnx, uint32_t mny, uint32_t mnz,
+    float mx, float my, float mz, float voxel, float rmax, float radius_i, float rep_i, float adh_i,
+    float rel_max_adh, float cut2, float& fx, float& fy, float& fz, float& local_pressure) {
+    const float relf = rel_max_adh > 1.0f ? rel_max_adh : 1.0f;
+    const float L = (radius_i + rmax) * relf;
+    const float lim = fminf(L * L * (1.0f + 0x1p-16f), cut2) + MARK32_SLACK;
+    uint32_t cnt = 0;
+#pragma unroll 1
+    for (int q = 0; q < 9; q++) {
+        const RowRange rr = neighbor_row(offsets, bin0, vi, vj, vk, mnx, mny, mnz, q);
+        const float myr = my - (float)rr.dj * voxel, mzr = mz - (float)rr.dk * voxel;
+        const float mxm = mx + voxel, mxp = mx - voxel;     // di = -1 / +1 folded into the agent's x
+        // entry code of the row's di = -1 column; +1 per column to the right
+        const uint32_t code0 = ((uint32_t)(3 * (rr.dj + 1) + 9 * (rr.dk + 1))) << MARK32_POS_BITS;
+        for (uint32_t u0 = rr.beg; u0 < rr.end; u0 += MARK32_STEP) {
+            float4 o[MARK32_STEP];
+#pragma unroll
+            for (uint32_t j = 0; j < MARK32_STEP; j++) o[j] = *reinterpret_cast<const float4*>(&rec[u0 + j].x);
+#pragma unroll
+            for (uint32_t j = 0; j < MARK32_STEP; j++) {
+                const uint32_t u = u0 + j;
+                const bool a = u < rr.mid0, b = u < rr.mid1;
+                const float ddx = (a ? mxm : (b ? mx : mxp)) - o[j].x, ddy = myr - o[j].y, ddz = mzr - o[j].z;
+                const float d2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(ddy, ddy, ddx * ddx));
+                list[cnt * 64] = (u | code0) + (a ? 0u : (b ? 1u : 2u) << MARK32_POS_BITS);
+                cnt += (u < rr.end && d2 < lim) ? 1u : 0u;
+            }
+            if (cnt > MARKX_CAP - MARK32_STEP) {
+                force_list32(rec, list, cnt, t, mx, my, mz, voxel, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy,
+                             fz, local_pressure);
+                cnt = 0;
+            }
+        }
+    }
+    force_list32(rec, list, cnt, t, mx, my, mz, voxel, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz,
+                 local_pressure);
 }
 
 // FP32 form: positions are relative to each agent's own bin corner.  A
