@@ -124,6 +124,7 @@ __host__ __device__ inline uint32_t xtile_elems(uint32_t lpw, uint32_t n) {
     return (lpw * n + 2 * E - 1) / E * E;
 }
 
+
 // Each wave owns its LDS tile: lanes of one wave exchange data through it, so
 // ordering the wave's own LDS accesses is enough (no workgroup barrier).
 __device__ inline void wave_sync() {
@@ -315,13 +316,46 @@ __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict_
         if (t == (int)mk - 1) yE = (double)v[t];
     // interface values of the line (its P lanes are l, l+LPW, l+2*LPW, ...), FP64
     double uL = 0.0, uR = 0.0;
+    if (XLDS && P == 16) {
+        // x: every lane publishes its segment's two end values once, in the
+        // first KiB of the wave's staging tile; each lane then forms uL and uR
+        // as dot products with its two rows of the inverse interface matrix,
+        // in two independent FMA chains each, instead of 2P shuffles of FP64
+        // values and one chain of 2P dependent adds.  (With 4 segments the
+        // shuffles are few and cost fewer registers.)  The tile's interior
+        // values are all in registers by now; its lines' end nodes are not
+        // rewritten without Dirichlet, so the lanes owning them keep them in
+        // registers across the exchange.  A 16-segment tile (n > 96) holds
+        // more than 1 KiB.
+        const T keep0 = tile[l * n], keep1 = tile[l * n + n - 1];
+        double2* ex = reinterpret_cast<double2*>(tile_base + (size_t)w * xtile_elems<T>(LPW, n));
+        wave_sync();
+        ex[lane] = make_double2(yS, yE);
+        wave_sync();
+        double aL0 = 0.0, aL1 = 0.0, aR0 = 0.0, aR1 = 0.0;
 #pragma unroll
-    for (int j = 0; j < P; j++) {
-        const int src = tr ? (int)l * P + j : j * LPW + (int)l;
-        const double a = __shfl(yS, src, 64);
-        const double b = __shfl(yE, src, 64);
-        uL += gl[2 * j] * a + gl[2 * j + 1] * b;
-        uR += gr[2 * j] * a + gr[2 * j + 1] * b;
+        for (int j = 0; j < P; j++) {
+            const int src = tr ? (int)l * P + j : j * LPW + (int)l;
+            const double2 ab = ex[src];
+            aL0 = __builtin_fma(gl[2 * j], ab.x, aL0);
+            aR0 = __builtin_fma(gr[2 * j], ab.x, aR0);
+            aL1 = __builtin_fma(gl[2 * j + 1], ab.y, aL1);
+            aR1 = __builtin_fma(gr[2 * j + 1], ab.y, aR1);
+        }
+        uL = aL0 + aL1;
+        uR = aR0 + aR1;
+        wave_sync();
+        if (k == 0) tile[l * n] = keep0;
+        if (k == P - 1) tile[l * n + n - 1] = keep1;
+    } else {
+#pragma unroll
+        for (int j = 0; j < P; j++) {
+            const int src = tr ? (int)l * P + j : j * LPW + (int)l;
+            const double a = __shfl(yS, src, 64);
+            const double b = __shfl(yE, src, 64);
+            uL += gl[2 * j] * a + gl[2 * j + 1] * b;
+            uR += gr[2 * j] * a + gr[2 * j + 1] * b;
+        }
     }
     const T tuL = (T)uL, tuR = (T)uR;
 #pragma unroll
