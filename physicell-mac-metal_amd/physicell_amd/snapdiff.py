@@ -18,7 +18,20 @@ Every field also reports its normwise relative error (largest absolute
 error over the largest reference magnitude), the measure of the FP64 bar
 (1e-10).  The arithmetic is numpy's; the reference walks Python lists.
 
-    python -m physicell_amd.snapdiff REF_DIR OURS_DIR [--frames 0 3] [--tol position=1e-9 ...]
+Exact mode (--exact): the integer outputs of the path, written beside the
+snapshot by write_exact as Level-4 matrices (integers below 2^53 are exact
+in them), must be equal, value for value:
+- output%08d_keys.mat: per cell (one column each) its ID, cell->voxel key
+  and mechanics-voxel key;
+- output%08d_bins.mat: the hash's bin offsets (one row, bins + 1 values) and
+  output%08d_sorted.mat: the cells in hash order;
+- output%08d_neighbors.mat: the neighbour lists of the force predicates
+  (same or adjacent mechanics voxel, d^2 < cutoff^2, d^2 >= 1e-12), two rows
+  (cell, neighbour) in visiting order; compared as sorted sets per cell and
+  in visiting order.
+A file missing on both sides is skipped; on one side it fails.
+
+    python -m physicell_amd.snapdiff REF_DIR OURS_DIR [--frames 0 3] [--tol position=1e-9 ...] [--exact]
 """
 import argparse
 import glob
@@ -194,6 +207,101 @@ def report(frame, cells, micro):
     return ok, "\n".join(lines)
 
 
+EXACT_FILES = ("keys", "bins", "sorted", "neighbors")
+
+
+def write_exact(folder, frame, cell_ids, voxel_keys, mech_keys, offsets, sorted_cells, nbr_offsets, nbr_idx):
+    """The exact-mode files of one frame: keys, bins, sorted list and the
+    neighbour lists (CSR: nbr_offsets[n + 1], nbr_idx in visiting order)."""
+    label = os.path.join(str(folder), f"output{frame:08d}")
+    ids = np.asarray(cell_ids, np.float64)
+    matio.write_mat(label + "_keys.mat", "keys",
+                    np.vstack([ids, np.asarray(voxel_keys, np.float64), np.asarray(mech_keys, np.float64)]))
+    matio.write_mat(label + "_bins.mat", "bins", np.asarray(offsets, np.float64)[None, :])
+    matio.write_mat(label + "_sorted.mat", "sorted", np.asarray(sorted_cells, np.float64)[None, :])
+    nbr_offsets = np.asarray(nbr_offsets, np.int64)
+    owner = np.repeat(np.arange(len(nbr_offsets) - 1), np.diff(nbr_offsets))
+    matio.write_mat(label + "_neighbors.mat", "neighbors",
+                    np.vstack([owner.astype(np.float64), np.asarray(nbr_idx, np.float64)]))
+
+
+@dataclass
+class ExactStat:
+    name: str
+    ok: bool = True
+    n: int = 0
+    note: str = ""
+
+    def line(self):
+        return f"  {self.name:<14s} n={self.n:<10d} {'EXACT' if self.ok else 'DIFFER'} {self.note}"
+
+
+def _first_diff(a, b):
+    if a.shape != b.shape:
+        return f"shape {a.shape} vs {b.shape}"
+    idx = np.argwhere(a != b)
+    return f"{len(idx)} values differ, first at {tuple(int(v) for v in idx[0])}: {a[tuple(idx[0])]!r} vs " \
+           f"{b[tuple(idx[0])]!r}" if len(idx) else ""
+
+
+def diff_exact(ref_dir, ours_dir, frame):
+    """[ExactStat] for the exact-mode files of one frame."""
+    label = f"output{frame:08d}"
+    out = []
+    for what in EXACT_FILES:
+        pa = os.path.join(ref_dir, f"{label}_{what}.mat")
+        pb = os.path.join(ours_dir, f"{label}_{what}.mat")
+        ha, hb = os.path.isfile(pa), os.path.isfile(pb)
+        if not ha and not hb:
+            continue
+        if ha != hb:
+            out.append(ExactStat(what, False, 0, f"present only in {'ref' if ha else 'ours'}"))
+            continue
+        a, b = matio.read_mat(pa)[1], matio.read_mat(pb)[1]
+        if what == "neighbors":
+            st = ExactStat("neighbors", True, a.shape[1])
+            if a.shape != b.shape or not np.array_equal(a[0], b[0]):
+                st.ok, st.note = False, "neighbour counts differ: " + _first_diff(
+                    np.bincount(a[0].astype(np.int64)), np.bincount(b[0].astype(np.int64)))
+            else:
+                # sorted sets per cell, then the visiting order
+                ka = np.lexsort((a[1], a[0]))
+                kb = np.lexsort((b[1], b[0]))
+                if not np.array_equal(a[1][ka], b[1][kb]):
+                    st.ok, st.note = False, "sets differ: " + _first_diff(a[:, ka], b[:, kb])
+                elif not np.array_equal(a[1], b[1]):
+                    st.ok, st.note = False, "same sets, visiting order differs: " + _first_diff(a, b)
+                else:
+                    st.note = "(sets and visiting order)"
+            out.append(st)
+        else:
+            d = _first_diff(a, b)
+            out.append(ExactStat(what, not d, a.size, d))
+    return out
+
+
+def compare(ref_dir, ours_dir, frames=None, tol=None, exact=False):
+    """(all passed, report text) over the frames present in both folders (or
+    the given ones): the snapshot rules, plus the exact-mode files."""
+    frames = frames if frames else sorted(set(frames_in(ref_dir)) & set(frames_in(ours_dir)))
+    if not frames:
+        return False, "no frame present in both folders"
+    all_ok, texts = True, []
+    for f in frames:
+        ok, text = report(f, *diff_frame(ref_dir, ours_dir, f, tol))
+        if exact:
+            ex = diff_exact(ref_dir, ours_dir, f)
+            if not ex:
+                ok, text = False, text + "\n exact: no exact-mode files"
+            else:
+                ok = ok and all(e.ok for e in ex)
+                text += "\n exact:\n" + "\n".join(e.line() for e in ex)
+            text = text.replace("[PASS]", "[FAIL]", 1) if not ok else text
+        texts.append(text)
+        all_ok = all_ok and ok
+    return all_ok, "\n".join(texts)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("ref_dir")
@@ -201,18 +309,13 @@ def main(argv=None):
     ap.add_argument("--frames", type=int, nargs="*", help="default: every frame present in both folders")
     ap.add_argument("--tol", nargs="*", default=[], help="key=value (position, volume, phase, cell_count_abs, "
                                                           "voxel_coord, substrate_linf, substrate_rmse)")
+    ap.add_argument("--exact", action="store_true",
+                    help="also require the exact-mode files (keys, bins, sorted list, neighbour lists) to be equal")
     a = ap.parse_args(argv)
     tol = {k: float(v) for k, v in (t.split("=", 1) for t in a.tol)}
-    frames = a.frames if a.frames else sorted(set(frames_in(a.ref_dir)) & set(frames_in(a.ours_dir)))
-    if not frames:
-        print("no frame present in both folders")
-        return 1
-    all_ok = True
-    for f in frames:
-        ok, text = report(f, *diff_frame(a.ref_dir, a.ours_dir, f, tol))
-        print(text)
-        all_ok = all_ok and ok
-    return 0 if all_ok else 1
+    ok, text = compare(a.ref_dir, a.ours_dir, a.frames, tol, a.exact)
+    print(text)
+    return 0 if ok else 1
 
 
 if __name__ == "__main__":

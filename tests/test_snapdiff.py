@@ -114,3 +114,69 @@ def test_microenvironment_gates(tmp_path):
     assert not micro.ok and not micro.fields
     micro = D.diff_frame(a, write(tmp_path, "ours", 4, micro=me[:, :59]), 4)[1]
     assert not micro.ok and "voxel count" in micro.notes[0]
+
+
+def exact_state(n=50, nb=12, seed=3):
+    """Hand-built integer outputs: keys, bins of a counting sort, and CSR
+    neighbour lists with 0-4 entries per cell."""
+    r = np.random.default_rng(seed)
+    mech = r.integers(0, nb, n)
+    off = np.concatenate([[0], np.cumsum(np.bincount(mech, minlength=nb))])
+    srt = np.argsort(mech, kind="stable")
+    cnt = r.integers(0, 5, n)
+    noff = np.concatenate([[0], np.cumsum(cnt)])
+    nidx = r.integers(0, n, int(noff[-1]))
+    return dict(cell_ids=np.arange(n), voxel_keys=r.integers(0, 1000, n), mech_keys=mech, offsets=off,
+                sorted_cells=srt, nbr_offsets=noff, nbr_idx=nidx)
+
+
+def test_exact_mode(tmp_path):
+    """The exact-mode files: equal outputs pass; one changed key, bin, sorted
+    entry or neighbour fails and is named; the same neighbour sets in another
+    visiting order fail as an order difference; a file on one side only
+    fails.  The command line (parity/diff.py) returns the status."""
+    import os
+    import subprocess
+    import sys
+    st = exact_state()
+    ref = write(tmp_path, "ref", 2, cells_matrix(50), micro_matrix())
+    D.write_exact(ref, 2, **st)
+    for what, change in (("keys", lambda s: s["voxel_keys"].__setitem__(7, s["voxel_keys"][7] + 1)),
+                         ("bins", lambda s: s["offsets"].__setitem__(3, s["offsets"][3] + 1)),
+                         ("sorted", lambda s: s["sorted_cells"].__setitem__(slice(0, 2), s["sorted_cells"][1::-1])),
+                         ("neighbors", lambda s: s["nbr_idx"].__setitem__(0, (s["nbr_idx"][0] + 1) % 50)),
+                         (None, None)):
+        ours = write(tmp_path, f"ours_{what}", 2, cells_matrix(50), micro_matrix())
+        s2 = {k: np.array(v) for k, v in st.items()}
+        if change:
+            change(s2)
+        D.write_exact(ours, 2, **s2)
+        ok, text = D.compare(ref, ours, exact=True)
+        stats = {e.name: e for e in D.diff_exact(ref, ours, 2)}
+        if what is None:
+            assert ok and all(e.ok for e in stats.values()), text
+        else:
+            assert not ok and not stats[what].ok, text
+            assert all(e.ok for k, e in stats.items() if k != what), text
+    # same sets, another visiting order
+    ours = write(tmp_path, "ours_order", 2, cells_matrix(50), micro_matrix())
+    s2 = {k: np.array(v) for k, v in st.items()}
+    i = int(np.argmax(np.diff(s2["nbr_offsets"]) >= 2))
+    a = s2["nbr_offsets"][i]
+    s2["nbr_idx"][a], s2["nbr_idx"][a + 1] = s2["nbr_idx"][a + 1], s2["nbr_idx"][a]
+    if s2["nbr_idx"][a] != s2["nbr_idx"][a + 1]:
+        D.write_exact(ours, 2, **s2)
+        e = {x.name: x for x in D.diff_exact(ref, ours, 2)}["neighbors"]
+        assert not e.ok and "visiting order" in e.note
+    # a file on one side only
+    ours = write(tmp_path, "ours_missing", 2, cells_matrix(50), micro_matrix())
+    D.write_exact(ours, 2, **st)
+    os.remove(os.path.join(ours, "output00000002_bins.mat"))
+    ok, text = D.compare(ref, ours, exact=True)
+    assert not ok and "present only in ref" in text
+    # the command line
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    good = write(tmp_path, "ours_None", 2)
+    cli = [sys.executable, os.path.join(root, "parity", "diff.py"), ref]
+    assert subprocess.run(cli + [good, "--exact"], capture_output=True).returncode == 0
+    assert subprocess.run(cli + [ours, "--exact"], capture_output=True).returncode == 1
