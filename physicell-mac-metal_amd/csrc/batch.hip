@@ -76,15 +76,27 @@ __global__ __launch_bounds__(256) void k_b_mech_keys(const EnsSim* __restrict__ 
 }
 
 // stable placement + hash-sorted records (FP64 or bin-relative FP32)
+// A member whose mechanics grid spans at most MARKX_MAX_EXTENT in x takes the
+// FP64 mark pass of force_agent64x (pos32 x relative to its x_min); the
+// gather and the force kernel decide this alike.
+__device__ inline bool member_markx(const EnsSim& m) {
+    return m.mvoxel > 0.0 && m.mvoxel <= 32.0 && m.max_dist <= 32.0 &&
+           (double)m.mnx * m.mvoxel <= MARKX_MAX_EXTENT;
+}
+
 template <bool FP32>
 __global__ __launch_bounds__(256) void k_b_gather(const EnsSim* __restrict__ sims, const AgentTile* __restrict__ tiles,
                                                   const uint32_t* __restrict__ kglob, const uint32_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ tmp, uint32_t* __restrict__ sorted,
                                                   const uint32_t* __restrict__ pos_of, void* __restrict__ recs,
-                                                  float4* __restrict__ pos32) {
+                                                  float4* __restrict__ pos32, float* __restrict__ rpart) {
     const AgentTile tl = tiles[blockIdx.x];
     const EnsSim m = sims[tl.sim];                       // one member per workgroup
     const uint32_t c = tl.c0 + threadIdx.x;
+    // the tile's largest radius (FP32 as the records hold it; rounded up for
+    // the FP64 mark pass); no lane returns before the barrier
+    const double rc = c < m.n ? sim_field(m, HC_F_RADIUS)[c] : 0.0;
+    block_max_to(FP32 ? (float)rc : __double2float_ru(rc), rpart);
     if (c >= m.n) return;
     const uint32_t g = m.cell_off + c;
     const uint32_t b = kglob[g];
@@ -112,9 +124,12 @@ __global__ __launch_bounds__(256) void k_b_gather(const EnsSim* __restrict__ sim
         reinterpret_cast<CellRec32*>(recs)[pos] = r;
     } else {
         if (pos32) {
+            // x relative to the member's x_min (force_agent64x) or to the bin
+            // corner (force_range64m); y and z relative to the bin corner
             const uint32_t lb = b - m.bin_off;
             const uint32_t bi = lb % m.mnx, bj = (lb / m.mnx) % m.mny, bk = lb / (m.mnx * m.mny);
-            pos32[pos] = make_float4((float)(px - (m.x_min + bi * m.mvoxel)), (float)(py - (m.y_min + bj * m.mvoxel)),
+            pos32[pos] = make_float4((float)(px - (member_markx(m) ? m.x_min : m.x_min + bi * m.mvoxel)),
+                                     (float)(py - (m.y_min + bj * m.mvoxel)),
                                      (float)(pz - (m.z_min + bk * m.mvoxel)), (float)sim_field(m, HC_F_RADIUS)[c]);
         }
         CellRec r;
@@ -131,10 +146,17 @@ __global__ __launch_bounds__(256) void k_b_gather(const EnsSim* __restrict__ sim
     }
 }
 
-template <bool FP32>
+// The single simulation's force passes over the member's bins: the mark
+// passes with a list of marked candidates (FP32: LIST, force_agent32x; FP64:
+// force_agent64x where member_markx holds), else the chunked ones.  rmax: the
+// largest FP32 radius over all members.
+template <bool FP32, bool LIST = true>
 __global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sims, const uint32_t* __restrict__ agent_sim,
                                                   const void* __restrict__ recs, const float4* __restrict__ pos32,
-                                                  const uint32_t* __restrict__ offsets, uint32_t ntot) {
+                                                  const uint32_t* __restrict__ offsets, uint32_t ntot,
+                                                  const float* __restrict__ rmax) {
+    __shared__ uint32_t marks[LIST ? 4 * MARKX_CAP * 64 : 1];
+    uint32_t* list = marks + (LIST ? (threadIdx.x >> 6) * (MARKX_CAP * 64) + (threadIdx.x & 63) : 0);
     const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (t >= ntot) return;
     uint32_t idx, key;
@@ -158,11 +180,16 @@ __global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sim
         const float rel = (float)sim_field(m, HC_F_REL_MAX_ADH)[c];
         const float cut = (float)m.max_dist, vox = (float)m.mvoxel;
         float fx = 0.f, fy = 0.f, fz = 0.f, pr = 0.f;
+        if constexpr (LIST) {
+            force_agent32x(rec, offsets, list, t, m.bin_off, vi, vj, vk, m.mnx, m.mny, m.mnz, me.x, me.y, me.z, vox,
+                           rmax[0], me.r, me.rep, me.adh, rel, cut * cut, fx, fy, fz, pr);
+        } else {
 #pragma unroll 1
-        for (int q = 0; q < 9; q++) {
-            const RowRange rr = neighbor_row(offsets, m.bin_off, vi, vj, vk, m.mnx, m.mny, m.mnz, q);
-            force_range32(rec, rr.beg, rr.end, t, rr.mid0, rr.mid1, me.x, me.y, me.z, (float)rr.dj * vox,
-                          (float)rr.dk * vox, vox, me.r, me.rep, me.adh, rel, cut * cut, fx, fy, fz, pr);
+            for (int q = 0; q < 9; q++) {
+                const RowRange rr = neighbor_row(offsets, m.bin_off, vi, vj, vk, m.mnx, m.mny, m.mnz, q);
+                force_range32(rec, rr.beg, rr.end, t, rr.mid0, rr.mid1, me.x, me.y, me.z, (float)rr.dj * vox,
+                              (float)rr.dk * vox, vox, me.r, me.rep, me.adh, rel, cut * cut, fx, fy, fz, pr);
+            }
         }
         ofx = fx; ofy = fy; ofz = fz; opr = pr;
     } else {
@@ -173,10 +200,16 @@ __global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sim
         // FP32 mark pass where its error bound holds (force_range64m)
         const float vox = (float)m.mvoxel, near = 2.0f * vox;
         const float4 m32 = pos32 ? pos32[t] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const bool mark32 = pos32 && m.mvoxel > 0.0 && m.mvoxel <= 32.0 && m.max_dist <= 32.0 &&
-                            fabsf(m32.x) < near && fabsf(m32.y) < near && fabsf(m32.z) < near;
+        const bool xgrid = member_markx(m);
+        const bool yz_near = fabsf(m32.y) < near && fabsf(m32.z) < near;
+        const bool markx = pos32 && rmax && xgrid && yz_near && m32.x > -near && m32.x < (float)m.mnx * vox + near;
+        const bool mark32 = pos32 && !xgrid && m.mvoxel > 0.0 && m.mvoxel <= 32.0 && m.max_dist <= 32.0 &&
+                            fabsf(m32.x) < near && yz_near;
+        if (markx)
+            force_agent64x(rec, pos32, offsets, list, m.bin_off, vi, vj, vk, m.mnx, m.mny, m.mnz, m32, vox, rmax[0],
+                           me.x, me.y, me.z, me.r, me.rep, me.adh, me.rel, cut2, fx, fy, fz, pr);
 #pragma unroll 1
-        for (int q = 0; q < 9; q++) {
+        for (int q = 0; q < (markx ? 0 : 9); q++) {
             const RowRange rr = neighbor_row(offsets, m.bin_off, vi, vj, vk, m.mnx, m.mny, m.mnz, q);
             if (mark32)
                 force_range64m(rec, pos32, rr.beg, rr.end, t, rr.mid0, rr.mid1, m32.x, m32.y, m32.z,
@@ -350,7 +383,7 @@ __global__ void k_b_motility_advance(const EnsSim* __restrict__ sims, const MotP
 int probe_kernels_batch(hc_ctx* ctx) {
     hipFuncAttributes attr;
     const void* fns[] = {(const void*)k_b_mech_keys, (const void*)k_b_gather<false>,
-                         (const void*)k_b_gather<true>, (const void*)k_b_forces<false>, (const void*)k_b_forces<true>,
+                         (const void*)k_b_gather<true>, (const void*)k_b_forces<false>, (const void*)k_b_forces<true, true>, (const void*)k_b_forces<true, false>,
                          (const void*)k_b_integrate, (const void*)k_b_sec_keys, (const void*)k_b_sec_groups<double>,
                          (const void*)k_b_sec_groups<float>,
                          (const void*)k_b_sec_apply<double>, (const void*)k_b_sec_apply<float>,
@@ -849,7 +882,7 @@ private:
     DeviceBuffer q_hash_, q_off_, q_idx_;     // member neighbour-query workspace
     struct Workspace {
         DeviceBuffer sims, mots, bsims, tiles, work[NLIST], agent_sim, kglob, slot, tmp, sorted, pos_of, recs, pos32, counts,
-            offsets, partial;
+            offsets, partial, rpart, rmax;
         DeviceBuffer pcounts, poffsets, pkv, pslot, ptmp, psorted, pflag, pgsim, pgvox, pgbegin, pcoef, phdr, ppartial;
         uint32_t ntot = 0, nbins = 0, vtot = 0, ntiles = 0, nwork[NLIST] = {}, ns_max = 0, nmax_all = 0;
         uint32_t max_line[NLIST] = {};
@@ -990,8 +1023,12 @@ private:
         const size_t N = std::max<uint32_t>(1, w.ntot);
         alloc(w.kglob, N * 4); alloc(w.slot, N * 4); alloc(w.tmp, N * 4); alloc(w.sorted, N * 4);
         alloc(w.pos_of, N * 4);
-        alloc(w.recs, N * (mixed_ ? sizeof(hc::CellRec32) : sizeof(hc::CellRec)));
-        if (!mixed_) alloc(w.pos32, N * sizeof(float4));
+        // records and pos32 carry one mark step of padding: the force kernels'
+        // mark passes load whole steps past the end of a range
+        alloc(w.recs, (N + hc::MARK32_STEP) * (mixed_ ? sizeof(hc::CellRec32) : sizeof(hc::CellRec)));
+        if (!mixed_) alloc(w.pos32, (N + hc::MARKX_STEP) * sizeof(float4));
+        alloc(w.rpart, (size_t)std::max(1u, w.ntiles) * 4);
+        alloc(w.rmax, 256);
         alloc(w.counts, ((size_t)w.nbins + 1) * 4); alloc(w.offsets, ((size_t)w.nbins + 2) * 4);
         alloc(w.partial, hc::scan_partial_bytes((uint64_t)w.nbins + 1));
         alloc(w.pcounts, ((size_t)w.vtot + 2) * 4); alloc(w.poffsets, ((size_t)w.vtot + 3) * 4);
@@ -1156,23 +1193,28 @@ private:
             check(ctx_, hc::scan_exclusive(ctx_, counts, offsets, w.nbins, (uint32_t*)hc_device_ptr(w.partial.get())), "scan");
             check(ctx_, hc::launch_scatter(ctx_, kglob, slot, offsets, tmp, w.ntot), "scatter");
             check(ctx_, hc::launch_bin_sort(ctx_, offsets, tmp, sorted, pos_of, w.nbins), "bin sort");
+            float* rpart = (float*)hc_device_ptr(w.rpart.get());
             if (mixed_)
                 hipLaunchKernelGGL(hc::k_b_gather<true>, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, kglob, offsets, tmp,
-                                   sorted, pos_of, recs, nullptr);
+                                   sorted, pos_of, recs, nullptr, rpart);
             else
                 hipLaunchKernelGGL(hc::k_b_gather<false>, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, kglob, offsets,
-                                   tmp, sorted, pos_of, recs, pos32);
+                                   tmp, sorted, pos_of, recs, pos32, rpart);
             check(ctx_, hc::check_launch(ctx_, "k_b_gather"), "gather");
+            check(ctx_, hc::launch_rmax(ctx_, rpart, w.ntiles, (float*)hc_device_ptr(w.rmax.get())), "largest radius");
         }
         {
             hc::KernelTimer tm(ctx_, HC_K_FORCES);
             const dim3 g(hc::blocks_for(w.ntot, 256));
+            // positions in the records' list entries take 27 bits (force_agent32x)
+            const float* rmax = (const float*)hc_device_ptr(w.rmax.get());
             if (mixed_)
-                hipLaunchKernelGGL(hc::k_b_forces<true>, g, dim3(256), 0, st, sims, asim, recs, nullptr, offsets,
-                                   w.ntot);
+                hipLaunchKernelGGL((w.ntot < (1u << hc::MARK32_POS_BITS) ? hc::k_b_forces<true, true>
+                                                                         : hc::k_b_forces<true, false>),
+                                   g, dim3(256), 0, st, sims, asim, recs, nullptr, offsets, w.ntot, rmax);
             else
                 hipLaunchKernelGGL(hc::k_b_forces<false>, g, dim3(256), 0, st, sims, asim, recs, pos32, offsets,
-                                   w.ntot);
+                                   w.ntot, rmax);
             check(ctx_, hc::check_launch(ctx_, "k_b_forces"), "forces");
         }
         if (any_motility_) {

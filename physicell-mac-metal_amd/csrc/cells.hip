@@ -750,6 +750,11 @@ int probe_kernels_cells2(hc_ctx* ctx) {
     return 0;
 }
 
+int launch_rmax(hc_ctx* ctx, const float* rpart, uint32_t nb, float* rmax) {
+    hipLaunchKernelGGL(k_rmax, dim3(1), dim3(256), 0, ctx->stream, rpart, nb, rmax);
+    return check_launch(ctx, "k_rmax");
+}
+
 int launch_bin_sort(hc_ctx* ctx, const uint32_t* offsets, const uint32_t* tmp, uint32_t* sorted, uint32_t* pos_of,
                     uint32_t nbins) {
     if (!nbins) return 0;

@@ -180,6 +180,8 @@ inline MotPtrs mot_ptrs(void* base, uint32_t max_cells, uint32_t ns) {
 }
 
 int fill_u32(hc_ctx* ctx, uint32_t* p, uint64_t n, uint32_t v);
+// the largest of nb block maxima rpart[] into rmax[0] (one block)
+int launch_rmax(hc_ctx* ctx, const float* rpart, uint32_t nb, float* rmax);
 // k_bin_sort over bins 0 .. nbins-1 (bin_sort_one)
 int launch_bin_sort(hc_ctx* ctx, const uint32_t* offsets, const uint32_t* tmp, uint32_t* sorted, uint32_t* pos_of,
                     uint32_t nbins);
