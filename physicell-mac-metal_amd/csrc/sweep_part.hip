@@ -34,6 +34,7 @@
 #include "hc_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 
 namespace hc {
@@ -245,13 +246,15 @@ __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict_
     if (AXIS == 0) {
         stride = 1; lines = (uint64_t)ny * nz; base = L * nx;
     } else if (AXIS == 1) {
+        // a substrate's y or z lines number nz*nx or ny*nx < 2^32: 32-bit
+        // division (a 64-bit one costs about 60 VALU instructions)
         stride = nx; lines = (uint64_t)nz * nx;
-        const uint64_t kk = L / nx, i = L % nx;
-        base = kk * nx * ny + i;
+        const uint32_t kk = (uint32_t)L / nx, i = (uint32_t)L - kk * nx;
+        base = (uint64_t)kk * nx * ny + i;
         face = (i == 0 || i == nx - 1);
     } else {
         stride = (uint64_t)nx * ny; lines = (uint64_t)ny * nx;
-        const uint64_t j = L / nx, i = L % nx;
+        const uint32_t j = (uint32_t)L / nx, i = (uint32_t)L - j * nx;
         base = L;
         face = (i == 0 || i == nx - 1 || j == 0 || j == ny - 1);
     }
@@ -265,12 +268,21 @@ __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict_
     const uint64_t rem = lines > Lw0 ? lines - Lw0 : 0;
     const uint32_t nval = (uint32_t)((rem < (uint64_t)LPW ? rem : (uint64_t)LPW) * n);
 
+    // Dirichlet term c*T on the line's first and last unknown, added in FP64
+    const double ct = c * T0;
     T v[MM];
     if (XLDS) {
         // the wave's LPW consecutive x lines are LPW*n contiguous values
         stage_in<T, RV>(grho + Lw0 * nx, tile, nval, lane);
         mid();
         wave_sync();
+        if (dir) {
+            // in the tile, by the lanes whose segments hold those unknowns
+            // (at most one of the two per lane: P >= 4)
+            const uint32_t at = k == 0 ? l * n + 1 : l * n + n - 2;
+            if (k == 0 || k == P - 1) tile[at] = (T)((double)tile[at] + ct);
+            wave_sync();
+        }
 #pragma unroll
         for (int t = 0; t < MM; t++) v[t] = (t < (int)mk) ? tile[l * n + sk + 1 + t] : (T)0;
     } else {
@@ -286,12 +298,19 @@ __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict_
     const double* gl = tG + k * PartLds<P, T>::GS;
     const double* gr = gl + 2 * P;
 
-    if (dir) {
-        const double ct = c * T0;                           // Dirichlet term in FP64
+    if (!XLDS && dir) {
         if (k == 0) v[0] = (T)((double)v[0] + ct);
+        // the last segment's length is the same in every lane, so its last
+        // unknown is one register position for the whole wave
+        const uint32_t jl = __builtin_amdgcn_readfirstlane(N - ((P - 1) * N) / P - 1);
+        const bool last = k == P - 1;
 #pragma unroll
         for (int t = 0; t < MM; t++)
-            if (k == P - 1 && t == (int)mk - 1) v[t] = (T)((double)v[t] + ct);
+            if (t == (int)jl) {                            // uniform: one position is taken
+                // a scalar branch, not evaluated at every position and selected
+                __asm__ volatile("");
+                if (last) v[t] = (T)((double)v[t] + ct);
+            }
     }
     // segment solve with zero coupling
     const T cc = (T)c;
@@ -302,6 +321,9 @@ __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict_
             dp = (v[t] + cc * dp) * idn[t];
             v[t] = dp;
         }
+    // the segment's last value: the backward pass starts there with no
+    // right neighbour, so it keeps the forward pass's last value
+    const double yE = (double)dp;
     T y = 0;
 #pragma unroll
     for (int t = MM - 1; t >= 0; t--)
@@ -310,10 +332,6 @@ __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict_
             v[t] = y;
         }
     const double yS = (double)v[0];
-    double yE = 0.0;
-#pragma unroll
-    for (int t = 0; t < MM; t++)
-        if (t == (int)mk - 1) yE = (double)v[t];
     // interface values of the line (its P lanes are l, l+LPW, l+2*LPW, ...), FP64
     double uL = 0.0, uR = 0.0;
     if (XLDS && P == 16) {
@@ -333,15 +351,22 @@ __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict_
         ex[lane] = make_double2(yS, yE);
         wave_sync();
         double aL0 = 0.0, aL1 = 0.0, aR0 = 0.0, aR1 = 0.0;
+        // the line's P entries sit STEP apart from its first; a uniform
+        // branch per lane order, so every read has an immediate offset
+        auto dots = [&](const double2* e, auto step) {
 #pragma unroll
-        for (int j = 0; j < P; j++) {
-            const int src = tr ? (int)l * P + j : j * LPW + (int)l;
-            const double2 ab = ex[src];
-            aL0 = __builtin_fma(gl[2 * j], ab.x, aL0);
-            aR0 = __builtin_fma(gr[2 * j], ab.x, aR0);
-            aL1 = __builtin_fma(gl[2 * j + 1], ab.y, aL1);
-            aR1 = __builtin_fma(gr[2 * j + 1], ab.y, aR1);
-        }
+            for (int j = 0; j < P; j++) {
+                const double2 ab = e[j * decltype(step)::value];
+                aL0 = __builtin_fma(gl[2 * j], ab.x, aL0);
+                aR0 = __builtin_fma(gr[2 * j], ab.x, aR0);
+                aL1 = __builtin_fma(gl[2 * j + 1], ab.y, aL1);
+                aR1 = __builtin_fma(gr[2 * j + 1], ab.y, aR1);
+            }
+        };
+        if (tr)
+            dots(ex + l * P, std::integral_constant<int, 1>());
+        else
+            dots(ex + l, std::integral_constant<int, LPW>());
         uL = aL0 + aL1;
         uR = aR0 + aR1;
         wave_sync();
