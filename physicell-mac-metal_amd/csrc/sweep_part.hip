@@ -361,6 +361,9 @@ __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict_
                 aR0 = __builtin_fma(gr[2 * j], ab.x, aR0);
                 aL1 = __builtin_fma(gl[2 * j + 1], ab.y, aL1);
                 aR1 = __builtin_fma(gr[2 * j + 1], ab.y, aR1);
+                // one step's reads at a time: hoisting all P steps' reads
+                // would hold 12*P registers
+                __builtin_amdgcn_sched_barrier(0);
             }
         };
         if (tr)
