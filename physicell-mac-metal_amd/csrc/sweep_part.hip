@@ -144,8 +144,10 @@ struct PartLds {
     static constexpr uint32_t GS = 4 * P + 1;                 // per segment: left row, right row, pad
     static __host__ __device__ uint32_t arr_stride(uint32_t MT) { return 4 * MT + 1; }
     static __host__ __device__ size_t g_count() { return (size_t)P * GS; }
+    // plus 4 elements after the arrays: seg_solve / seg_join read table
+    // positions in batches of 4, up to 3 past the last segment's end
     static __host__ __device__ size_t bytes(uint32_t nmax) {
-        const size_t b = g_count() * sizeof(double) + (size_t)P * arr_stride(part_mt(nmax, P)) * sizeof(T);
+        const size_t b = g_count() * sizeof(double) + ((size_t)P * arr_stride(part_mt(nmax, P)) + 4) * sizeof(T);
         return (b + 15) / 16 * 16;
     }
 };
@@ -221,6 +223,85 @@ struct TableRegs {
 struct NoTables {
     __device__ inline void operator()() const {}
 };
+
+// The recurrences of one lane's segment (mk <= MM unknowns in v[0..mk)),
+// written without a branch per element: the table values are read from LDS
+// SB at a time (so a batch costs one LDS latency instead of one per element;
+// the reads past the segment's end stay inside the table block or the LDS
+// that follows it, and their values are not used), positions t >= mk are
+// kept out of the chains by selects, and batches at or beyond Mu (the largest
+// mk of the wave, uniform) are skipped.  Same operations in the same order as
+// the element-by-element form.
+//   forward: dp = (v[t] + c*dp) / den[t] (times the reciprocal), v[t] = dp;
+//   backward: y = v[t] - c'[t]*y, v[t] = y (c' = 0 at the segment's end);
+//   yE = the forward pass's last value, yS = v[0] after the backward pass.
+template <int MM, typename T, int SB = 4>
+__device__ __attribute__((always_inline)) inline void seg_solve(T (&v)[MM], uint32_t mk, uint32_t Mu, const T* idn,
+                                                                const T* cpl, T cc, double& yS, double& yE) {
+    T dp = 0;
+#pragma unroll
+    for (int c = 0; c < MM / SB; c++) {
+        if ((uint32_t)(SB * c) < Mu) {
+            T d[SB];
+#pragma unroll
+            for (int q = 0; q < SB; q++) {
+                const uint32_t t = SB * c + q;
+                d[q] = idn[t];
+            }
+#pragma unroll
+            for (int q = 0; q < SB; q++) {
+                const int t = SB * c + q;
+                const T nd = (v[t] + cc * dp) * d[q];
+                dp = (uint32_t)t < mk ? nd : dp;
+                v[t] = nd;                                 // positions t >= mk are never used
+            }
+        }
+    }
+    yE = (double)dp;
+    T y = 0;
+#pragma unroll
+    for (int c = MM / SB - 1; c >= 0; c--) {
+        if ((uint32_t)(SB * c) < Mu) {
+            T d[SB];
+#pragma unroll
+            for (int q = 0; q < SB; q++) {
+                const uint32_t t = SB * c + q;
+                d[q] = cpl[t];
+            }
+#pragma unroll
+            for (int q = SB - 1; q >= 0; q--) {
+                const int t = SB * c + q;
+                const T ny = v[t] - d[q] * y;
+                y = (uint32_t)t < mk ? ny : y;
+                v[t] = ny;
+            }
+        }
+    }
+    yS = (double)v[0];
+}
+
+// u = y + uL*alpha + uR*beta over the segment (same batching)
+template <int MM, typename T, int SB = 4>
+__device__ __attribute__((always_inline)) inline void seg_join(T (&v)[MM], uint32_t mk, uint32_t Mu, const T* al,
+                                                               const T* be, T tuL, T tuR) {
+#pragma unroll
+    for (int c = 0; c < MM / SB; c++) {
+        if ((uint32_t)(SB * c) < Mu) {
+            T a[SB], b[SB];
+#pragma unroll
+            for (int q = 0; q < SB; q++) {
+                const uint32_t t = SB * c + q;
+                a[q] = al[t];
+                b[q] = be[t];
+            }
+#pragma unroll
+            for (int q = 0; q < SB; q++) {
+                const int t = SB * c + q;
+                v[t] = v[t] + a[q] * tuL + b[q] * tuR;
+            }
+        }
+    }
+}
 
 // One group of WPB*LPW lines (a workgroup of WPB waves) of one substrate of
 // one simulation.  rho points at that substrate's field.
@@ -312,26 +393,12 @@ __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict_
                 if (last) v[t] = (T)((double)v[t] + ct);
             }
     }
-    // segment solve with zero coupling
-    const T cc = (T)c;
-    T dp = 0;
-#pragma unroll
-    for (int t = 0; t < MM; t++)
-        if (t < (int)mk) {
-            dp = (v[t] + cc * dp) * idn[t];
-            v[t] = dp;
-        }
-    // the segment's last value: the backward pass starts there with no
-    // right neighbour, so it keeps the forward pass's last value
-    const double yE = (double)dp;
-    T y = 0;
-#pragma unroll
-    for (int t = MM - 1; t >= 0; t--)
-        if (t < (int)mk) {
-            y = v[t] - cpl[t] * y;
-            v[t] = y;
-        }
-    const double yS = (double)v[0];
+    // segment solve with zero coupling; the segment's last value yE: the
+    // backward pass starts there with no right neighbour, so it keeps the
+    // forward pass's last value
+    const uint32_t Mu = __builtin_amdgcn_readfirstlane((N + P - 1) / P);
+    double yS, yE;
+    seg_solve<MM, T>(v, mk, Mu, idn, cpl, (T)c, yS, yE);
     // interface values of the line (its P lanes are l, l+LPW, l+2*LPW, ...), FP64
     double uL = 0.0, uR = 0.0;
     if (XLDS && P == 16) {
@@ -385,10 +452,7 @@ __device__ __attribute__((always_inline)) inline void sweep_group(T* __restrict_
             uR += gr[2 * j] * a + gr[2 * j + 1] * b;
         }
     }
-    const T tuL = (T)uL, tuR = (T)uR;
-#pragma unroll
-    for (int t = 0; t < MM; t++)
-        if (t < (int)mk) v[t] = v[t] + al[t] * tuL + be[t] * tuR;
+    seg_join<MM, T>(v, mk, Mu, al, be, (T)uL, (T)uR);
 
     const T tT = (T)T0;
     if (XLDS) {
