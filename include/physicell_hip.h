@@ -348,6 +348,17 @@ int pc_sim_run_periods(pc_sim* sim, int periods, int use_graph);
  * touch disjoint data; the results are the same).  pc_sim_run keeps the
  * reference's serial order. */
 int pc_sim_set_overlap(pc_sim* sim, int on);
+/* No counterpart in the reference (whose cell buffer is the only copy).
+ * on != 0 (default): the FP64 mechanics step without motility runs on a
+ * hash-ordered mirror of the agents' mechanics state (records and previous
+ * velocities in the order of the last hash build), so its per-agent passes
+ * read and write in spatial order; positions, velocities, previous
+ * velocities, pressure and mechanics voxels are written back into the cell
+ * buffer before anything else reads them (the phenotype step, field reads,
+ * snapshots, neighbour queries, attachments, pc_sim_wait).  Code that reads
+ * the cell buffer through hc_* directly calls pc_sim_wait first.  The results
+ * are the same bit for bit with the mirror on or off. */
+int pc_sim_set_mechanics_mirror(pc_sim* sim, int on);
 /* Enables motility in the mechanics step (forces -> motility -> AB2, as
  * src/main.mm:376-387).  Per-cell arrays as MotilityData (src/motility.h);
  * sensitivity [cell*ns + s] may be NULL (then chem_index / chem_dir bias).

@@ -244,13 +244,15 @@ __global__ __launch_bounds__(256) void k_stable_gather(CellPtrs cp, const uint32
 // x_grid: pos32 holds x relative to the grid's x_min (force_agent64x, with
 // its list of marked candidates in LDS), else relative to the bin corner
 // (force_range64m).
+// vout non-null: velocity and pressure go to vout[t] in hash-sorted order
+// (the hash-ordered mirror), else to the cell fields at the agent's index.
 __global__ __launch_bounds__(256) void k_forces(const CellRec* __restrict__ rec, const float4* __restrict__ pos32,
                                                 const uint32_t* __restrict__ offsets,
                                                 double* __restrict__ vx, double* __restrict__ vy,
                                                 double* __restrict__ vz, double* __restrict__ pressure,
                                                 uint32_t n, uint32_t mnx, uint32_t mny, uint32_t mnz,
                                                 double max_dist, float voxel, int x_grid,
-                                                const float* __restrict__ rmax) {
+                                                const float* __restrict__ rmax, double4* __restrict__ vout) {
     __shared__ uint32_t marks[4 * MARKX_CAP * 64];
     const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (t >= n) return;
@@ -267,6 +269,10 @@ __global__ __launch_bounds__(256) void k_forces(const CellRec* __restrict__ rec,
         uint32_t* list = marks + (threadIdx.x >> 6) * (MARKX_CAP * 64) + (threadIdx.x & 63);
         force_agent64x(rec, pos32, offsets, list, 0u, vi, vj, vk, mnx, mny, mnz, m, voxel, rmax[0], px, py, pz,
                        radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz, local_pressure);
+        if (vout) {
+            vout[t] = make_double4(fx, fy, fz, local_pressure);
+            return;
+        }
         const uint32_t c = me.idx;
         vx[c] = fx;
         vy[c] = fy;
@@ -285,6 +291,10 @@ __global__ __launch_bounds__(256) void k_forces(const CellRec* __restrict__ rec,
         else
             force_range64(rec, rr.beg, rr.end, t, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy,
                           fz, local_pressure);
+    }
+    if (vout) {
+        vout[t] = make_double4(fx, fy, fz, local_pressure);
+        return;
     }
     const uint32_t c = me.idx;
     vx[c] = fx;
@@ -992,7 +1002,7 @@ int hc_forces(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech, uin
                            mark32_ok(mech) ? h.pos32 : nullptr, h.offsets, cp.f[HC_F_VEL_X], cp.f[HC_F_VEL_Y],
                            cp.f[HC_F_VEL_Z], cp.f[HC_F_PRESSURE], n, mech->mech_grid_nx, mech->mech_grid_ny,
                            mech->mech_grid_nz, mech->max_interaction_distance, (float)mech->mechanics_voxel_size,
-                           markx_ok(mech), h.rmax);
+                           markx_ok(mech), h.rmax, (double4*)nullptr);
     return check_launch(ctx, "k_forces");
 }
 
@@ -1130,3 +1140,306 @@ int hc_substrate_totals_noalloc(hc_ctx* ctx, hc_buf* density, const hc_grid* g, 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Hash-ordered mechanics mirror of one simulation (FP64 forces).
+//
+// The agents' order in the cell SoA (their index) need not follow space, so
+// every per-agent pass of the mechanics step in index order scatters: the
+// ballot of the key pass finds about one key per lane (one atomic each), the
+// gather reads each agent's fields from unrelated cache lines, and the force
+// pass stores velocity and pressure at the agent's index.  The mirror keeps
+// the mechanics state -- the 64-byte records (position, radius, repulsion,
+// adhesion, adhesion reach, index, key) and the previous velocities -- in the
+// order of the last hash build, where it stays after AB2.  The next step's
+// keys are then computed in nearly sorted order, the new hash order is a
+// near-identity permutation of the mirror, the forces are written in hash
+// order, and AB2 reads and writes in that order too.  The cell fields
+// (position, velocity, previous velocity, pressure, mechanics voxel) are
+// written back, by agent index, only when something else reads them
+// (mirror_sync).  Keys, bins, the in-bin order (ascending agent index), the
+// records and every arithmetic step are those of the direct path, so the
+// results are the same bit for bit.
+// ---------------------------------------------------------------------------
+namespace hc {
+
+MirrorLayout mirror_layout(uint32_t max_cells) {
+    MirrorLayout L;
+    size_t o = 0;
+    L.recA = o;  o = align256(o + (size_t)max_cells * sizeof(CellRec));
+    L.prevA = o; o = align256(o + (size_t)max_cells * sizeof(double4));
+    L.prevB = o; o = align256(o + (size_t)max_cells * sizeof(double4));
+    L.vp = o;    o = align256(o + (size_t)max_cells * sizeof(double4));
+    L.mkey = o;  o = align256(o + (size_t)max_cells * 4);
+    L.total = o;
+    return L;
+}
+
+// mechanics keys (shaders/mechanics.metal:61-72) of the mirror's agents
+__global__ __launch_bounds__(256) void k_m_keys(const CellRec* __restrict__ recA, uint32_t* __restrict__ mkey,
+                                                uint32_t* __restrict__ counts, uint32_t* __restrict__ slot, uint32_t n,
+                                                double x_min, double y_min, double z_min, double voxel, uint32_t mnx,
+                                                uint32_t mny, uint32_t mnz) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = t < n;
+    uint32_t key = 0;
+    if (valid) {
+        const double2 xy = *reinterpret_cast<const double2*>(&recA[t].x);
+        const double z = recA[t].z;
+        int vi = (int)((xy.x - x_min) / voxel);
+        int vj = (int)((xy.y - y_min) / voxel);
+        int vk = (int)((z - z_min) / voxel);
+        vi = vi < 0 ? 0 : (vi > (int)mnx - 1 ? (int)mnx - 1 : vi);
+        vj = vj < 0 ? 0 : (vj > (int)mny - 1 ? (int)mny - 1 : vj);
+        vk = vk < 0 ? 0 : (vk > (int)mnz - 1 ? (int)mnz - 1 : vk);
+        key = (uint32_t)vk * mny * mnx + (uint32_t)vj * mnx + (uint32_t)vi;
+        mkey[t] = key;
+    }
+    const uint32_t s = ballot_bin(valid, key, counts);
+    if (valid) slot[t] = s;
+}
+
+// Bins of up to BIN_SORT_CAP members: the members (mirror slots) ordered by
+// agent index (a bitonic network over (index, slot) pairs); writes the bin's
+// agent list and each slot's new position.
+__global__ __launch_bounds__(256) void k_m_bin_sort(const uint32_t* __restrict__ offsets,
+                                                    const uint32_t* __restrict__ tmp,
+                                                    const CellRec* __restrict__ recA, uint32_t* __restrict__ sorted,
+                                                    uint32_t* __restrict__ pos_of, uint32_t nbins) {
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= nbins) return;
+    const uint32_t beg = offsets[b], cnt = offsets[b + 1] - beg;
+    if (cnt == 0 || cnt > BIN_SORT_CAP) return;
+    uint64_t a[BIN_SORT_CAP];
+#pragma unroll
+    for (uint32_t q = 0; q < BIN_SORT_CAP; q++) {
+        const uint32_t m = q < cnt ? tmp[beg + q] : 0u;
+        a[q] = q < cnt ? ((uint64_t)recA[m].idx << 32) | m : ~0ull;
+    }
+#pragma unroll
+    for (uint32_t k = 2; k <= BIN_SORT_CAP; k <<= 1)
+#pragma unroll
+        for (uint32_t j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (uint32_t i = 0; i < BIN_SORT_CAP; i++) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const uint64_t lo = a[i] < a[l] ? a[i] : a[l], hi = a[i] < a[l] ? a[l] : a[i];
+                    const bool up = (i & k) == 0;
+                    a[i] = up ? lo : hi;
+                    a[l] = up ? hi : lo;
+                }
+            }
+#pragma unroll
+    for (uint32_t q = 0; q < BIN_SORT_CAP; q++)
+        if (q < cnt) {
+            sorted[beg + q] = (uint32_t)(a[q] >> 32);
+            pos_of[(uint32_t)a[q]] = beg + q;
+        }
+}
+
+// The mirror in the new hash order: records (with their new key), previous
+// velocities and the FP32 mark positions, as k_stable_gather builds them from
+// the cell fields.  Members of bins above BIN_SORT_CAP are ranked by agent
+// index here.
+__global__ __launch_bounds__(256) void k_m_gather(const CellRec* __restrict__ recA, const double4* __restrict__ prevA,
+                                                  const uint32_t* __restrict__ mkey,
+                                                  const uint32_t* __restrict__ offsets,
+                                                  const uint32_t* __restrict__ tmp,
+                                                  const uint32_t* __restrict__ pos_of, uint32_t* __restrict__ sorted,
+                                                  CellRec* __restrict__ recB, double4* __restrict__ prevB,
+                                                  float4* __restrict__ pos32, uint32_t n, double x_min, double y_min,
+                                                  double z_min, double voxel, uint32_t mnx, uint32_t mny, int x_grid,
+                                                  float* __restrict__ rpart) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (rpart) block_max_to(t < n ? __double2float_ru(recA[t].r) : 0.0f, rpart);
+    if (t >= n) return;
+    CellRec r = recA[t];
+    const uint32_t b = mkey[t];
+    const uint32_t beg = offsets[b], end = offsets[b + 1];
+    uint32_t pos;
+    if (end - beg <= BIN_SORT_CAP) {
+        pos = pos_of[t];                                  // placed by k_m_bin_sort
+    } else {
+        uint32_t rank = 0;
+        for (uint32_t u0 = beg; u0 < end; u0 += 8) {
+            uint32_t v[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q] = recA[tmp[u0 + q < end ? u0 + q : end - 1]].idx;
+#pragma unroll
+            for (int q = 0; q < 8; q++) rank += (u0 + q < end && v[q] < r.idx) ? 1u : 0u;
+        }
+        pos = beg + rank;
+        sorted[pos] = r.idx;
+    }
+    if (pos32) {
+        const uint32_t bi = b % mnx, bj = (b / mnx) % mny, bk = b / (mnx * mny);
+        pos32[pos] = make_float4((float)(r.x - (x_grid ? x_min : x_min + bi * voxel)),
+                                 (float)(r.y - (y_min + bj * voxel)), (float)(r.z - (z_min + bk * voxel)),
+                                 (float)r.r);
+    }
+    r.key = b;
+    recB[pos] = r;
+    prevB[pos] = prevA[t];
+}
+
+// AB2 (shaders/integrate.metal:56-79) in hash order: this step's records
+// (recB), forces (vp) and previous velocities (prevB) to the mirror (recA,
+// prevA), in the same order.
+__global__ __launch_bounds__(256) void k_m_integrate(const CellRec* __restrict__ recB,
+                                                     const double4* __restrict__ prevB,
+                                                     const double4* __restrict__ vp, CellRec* __restrict__ recA,
+                                                     double4* __restrict__ prevA, uint32_t n, double dt,
+                                                     double x_min, double y_min, double z_min, double x_max,
+                                                     double y_max, double z_max) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    CellRec r = recB[t];
+    const double4 v = vp[t], pv = prevB[t];
+    double px = r.x, py = r.y, pz = r.z;
+    px += dt * (1.5 * v.x - 0.5 * pv.x);
+    py += dt * (1.5 * v.y - 0.5 * pv.y);
+    pz += dt * (1.5 * v.z - 0.5 * pv.z);
+    r.x = px < x_min ? x_min : (px > x_max ? x_max : px);
+    r.y = py < y_min ? y_min : (py > y_max ? y_max : py);
+    r.z = pz < z_min ? z_min : (pz > z_max ? z_max : pz);
+    recA[t] = r;
+    prevA[t] = make_double4(v.x, v.y, v.z, 0.0);
+}
+
+// the cell fields from the mirror, by agent index
+__global__ __launch_bounds__(256) void k_m_sync(const CellRec* __restrict__ recA, const double4* __restrict__ prevA,
+                                                const double4* __restrict__ vp, CellPtrs cp, uint32_t n) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    const CellRec r = recA[t];
+    const double4 v = vp[t], pv = prevA[t];
+    const uint32_t c = r.idx;
+    cp.f[HC_F_POS_X][c] = r.x;
+    cp.f[HC_F_POS_Y][c] = r.y;
+    cp.f[HC_F_POS_Z][c] = r.z;
+    cp.f[HC_F_VEL_X][c] = v.x;
+    cp.f[HC_F_VEL_Y][c] = v.y;
+    cp.f[HC_F_VEL_Z][c] = v.z;
+    cp.f[HC_F_PRESSURE][c] = v.w;
+    cp.f[HC_F_PREV_VX][c] = pv.x;
+    cp.f[HC_F_PREV_VY][c] = pv.y;
+    cp.f[HC_F_PREV_VZ][c] = pv.z;
+    cp.u[HC_F_MECH_VOXEL - HC_NUM_F64_FIELDS][c] = r.key;
+}
+
+// the mirror from a hash built on the cell fields: the hash's records and
+// each agent's previous velocity, in hash order
+__global__ __launch_bounds__(256) void k_m_load(const CellRec* __restrict__ recB, CellPtrs cp,
+                                                CellRec* __restrict__ recA, double4* __restrict__ prevA, uint32_t n) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    const CellRec r = recB[t];
+    recA[t] = r;
+    prevA[t] = make_double4(cp.f[HC_F_PREV_VX][r.idx], cp.f[HC_F_PREV_VY][r.idx], cp.f[HC_F_PREV_VZ][r.idx], 0.0);
+}
+
+int probe_kernels_mirror(hc_ctx* ctx) {
+    hipFuncAttributes attr;
+    const void* fns[] = {(const void*)k_m_keys, (const void*)k_m_bin_sort, (const void*)k_m_gather,
+                         (const void*)k_m_integrate, (const void*)k_m_sync, (const void*)k_m_load};
+    for (const void* f : fns) {
+        hipError_t e = hipFuncGetAttributes(&attr, f);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: mechanics mirror kernel");
+    }
+    return 0;
+}
+
+static int mirror_check(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, uint32_t n,
+                        uint32_t max_cells, uint32_t n_mvox, const char* who) {
+    if (!ctx || !hash || !mirror || !mech) return fail(ctx, HC_ERR_ARG, std::string(who) + ": null argument");
+    if (n > max_cells) return fail(ctx, HC_ERR_ARG, std::string(who) + ": n > max_cells");
+    if (mech->forces_fp32) return fail(ctx, HC_ERR_ARG, std::string(who) + ": the mirror holds FP64 records");
+    if (hc_buf_size(hash) < hc_hash_bytes(n_mvox, max_cells) || hc_buf_size(mirror) < mirror_layout(max_cells).total)
+        return fail(ctx, HC_ERR_ARG, std::string(who) + ": buffer too small");
+    return 0;
+}
+
+struct MirrorPtrs {
+    CellRec* recA;
+    double4 *prevA, *prevB, *vp;
+    uint32_t* mkey;
+};
+static MirrorPtrs mirror_ptrs(hc_buf* mirror, uint32_t max_cells) {
+    const MirrorLayout L = mirror_layout(max_cells);
+    uint8_t* P = (uint8_t*)mirror->dptr;
+    return MirrorPtrs{(CellRec*)(P + L.recA), (double4*)(P + L.prevA), (double4*)(P + L.prevB), (double4*)(P + L.vp),
+                      (uint32_t*)(P + L.mkey)};
+}
+
+int mirror_load(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, const hc_grid* mgrid,
+                uint32_t n, uint32_t max_cells, uint32_t n_mvox) {
+    int r = mirror_check(ctx, hash, mirror, mech, n, max_cells, n_mvox, "mirror_load");
+    if (r || (r = hc_build_hash(ctx, cells, hash, mech, mgrid, n, max_cells, n_mvox))) return r;
+    if (!n) return 0;
+    const HashPtrs h = hash_ptrs(hash, n_mvox, max_cells);
+    const MirrorPtrs m = mirror_ptrs(mirror, max_cells);
+    hipLaunchKernelGGL(k_m_load, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, h.rec,
+                       cell_ptrs(cells->dptr, max_cells), m.recA, m.prevA, n);
+    return check_launch(ctx, "k_m_load");
+}
+
+int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, const hc_grid* mgrid, double dt,
+                uint32_t n, uint32_t max_cells, uint32_t n_mvox) {
+    int r = mirror_check(ctx, hash, mirror, mech, n, max_cells, n_mvox, "mirror_step");
+    if (r) return r;
+    if (!mgrid || (uint64_t)mech->mech_grid_nx * mech->mech_grid_ny * mech->mech_grid_nz != n_mvox)
+        return fail(ctx, HC_ERR_ARG, "mirror_step: n_mvox != mech grid size");
+    if (!n) return 0;
+    const HashPtrs h = hash_ptrs(hash, n_mvox, max_cells);
+    const MirrorPtrs m = mirror_ptrs(mirror, max_cells);
+    const unsigned nb = blocks_for(n, 256);
+    const int xg = markx_ok(mech);
+    {
+        KernelTimer tm(ctx, HC_K_HASH);
+        if ((r = fill_u32(ctx, h.counts, n_mvox, 0))) return r;
+        hipLaunchKernelGGL(k_m_keys, dim3(nb), dim3(256), 0, ctx->stream, m.recA, m.mkey, h.counts, h.slot, n,
+                           mgrid->x_min, mgrid->y_min, mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx,
+                           mech->mech_grid_ny, mech->mech_grid_nz);
+        if ((r = check_launch(ctx, "k_m_keys"))) return r;
+        if ((r = scan_exclusive(ctx, h.counts, h.offsets, n_mvox, h.partial))) return r;
+        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, ctx->stream, m.mkey, h.slot, h.offsets, h.tmp, n);
+        hipLaunchKernelGGL(k_m_bin_sort, dim3(blocks_for(n_mvox, 256)), dim3(256), 0, ctx->stream, h.offsets, h.tmp,
+                           m.recA, h.sorted, h.pos_of, n_mvox);
+        hipLaunchKernelGGL(k_m_gather, dim3(nb), dim3(256), 0, ctx->stream, m.recA, m.prevA, m.mkey, h.offsets, h.tmp,
+                           h.pos_of, h.sorted, h.rec, m.prevB, mark32_ok(mech) ? h.pos32 : nullptr, n, mgrid->x_min,
+                           mgrid->y_min, mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx,
+                           mech->mech_grid_ny, xg, xg ? h.rpart : nullptr);
+        if (xg) hipLaunchKernelGGL(k_rmax, dim3(1), dim3(256), 0, ctx->stream, h.rpart, nb, h.rmax);
+        if ((r = check_launch(ctx, "mirror hash"))) return r;
+    }
+    {
+        KernelTimer tm(ctx, HC_K_FORCES);
+        hipLaunchKernelGGL(k_forces, dim3(nb), dim3(256), 0, ctx->stream, h.rec, mark32_ok(mech) ? h.pos32 : nullptr,
+                           h.offsets, (double*)nullptr, (double*)nullptr, (double*)nullptr, (double*)nullptr, n,
+                           mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz, mech->max_interaction_distance,
+                           (float)mech->mechanics_voxel_size, xg, h.rmax, m.vp);
+        if ((r = check_launch(ctx, "k_forces (mirror)"))) return r;
+    }
+    KernelTimer tm(ctx, HC_K_INTEGRATE);
+    const double x_max = mgrid->x_min + (double)mgrid->nx * mgrid->dx;
+    const double y_max = mgrid->y_min + (double)mgrid->ny * mgrid->dy;
+    const double z_max = mgrid->z_min + (double)mgrid->nz * mgrid->dz;
+    hipLaunchKernelGGL(k_m_integrate, dim3(nb), dim3(256), 0, ctx->stream, h.rec, m.prevB, m.vp, m.recA, m.prevA, n,
+                       dt, mgrid->x_min, mgrid->y_min, mgrid->z_min, x_max, y_max, z_max);
+    return check_launch(ctx, "k_m_integrate");
+}
+
+int mirror_sync(hc_ctx* ctx, hc_buf* cells, hc_buf* mirror, uint32_t n, uint32_t max_cells) {
+    int r = check_cells(ctx, cells, n, max_cells, "mirror_sync");
+    if (r) return r;
+    if (!mirror || hc_buf_size(mirror) < mirror_layout(max_cells).total)
+        return fail(ctx, HC_ERR_ARG, "mirror_sync: mirror buffer too small");
+    if (!n) return 0;
+    const MirrorPtrs m = mirror_ptrs(mirror, max_cells);
+    hipLaunchKernelGGL(k_m_sync, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, m.recA, m.prevA, m.vp,
+                       cell_ptrs(cells->dptr, max_cells), n);
+    return check_launch(ctx, "k_m_sync");
+}
+
+}  // namespace hc

@@ -59,6 +59,7 @@ KernelTimer::~KernelTimer() {
 int probe_kernels_diffusion(hc_ctx* ctx);
 int probe_kernels_cells(hc_ctx* ctx);
 int probe_kernels_cells2(hc_ctx* ctx);
+int probe_kernels_mirror(hc_ctx* ctx);
 int probe_kernels_batch(hc_ctx* ctx);
 int probe_kernels_motility(hc_ctx* ctx);
 int probe_kernels_phenotype(hc_ctx* ctx);
@@ -132,6 +133,8 @@ int hc_load_kernels(hc_ctx* ctx, const char* code_object_or_null) {
     r = probe_kernels_cells(ctx);
     if (r) return r;
     r = probe_kernels_cells2(ctx);
+    if (r) return r;
+    r = probe_kernels_mirror(ctx);
     if (r) return r;
     r = totals_scratch_init(ctx);
     if (r) return r;

@@ -250,6 +250,25 @@ struct HashLayout {
 };
 HashLayout hash_layout(uint32_t n_mvox, uint32_t max_cells);
 
+// Hash-ordered mechanics mirror of one simulation (cells.hip): the
+// post-AB2 records and previous velocities in the order of the last hash
+// build, the previous velocities in this step's order, the forces in hash
+// order and each mirror slot's new key.
+struct MirrorLayout {
+    size_t recA, prevA, prevB, vp, mkey, total;
+};
+MirrorLayout mirror_layout(uint32_t max_cells);
+// builds the hash from the cell fields and the mirror from it
+int mirror_load(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, const hc_grid* mgrid,
+                uint32_t n, uint32_t max_cells, uint32_t n_mvox);
+// one mechanics step (hash -> forces -> AB2) on the mirror; the cell fields
+// are not touched
+int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, const hc_grid* mgrid, double dt,
+                uint32_t n, uint32_t max_cells, uint32_t n_mvox);
+// position, velocity, previous velocity, pressure and mechanics voxel of
+// every agent from the mirror into the cell fields
+int mirror_sync(hc_ctx* ctx, hc_buf* cells, hc_buf* mirror, uint32_t n, uint32_t max_cells);
+
 struct PlanLayout {
     size_t header, counts, offsets, slot, tmp, sorted, gflag, gvox, gbegin, coef, partial, total;
 };

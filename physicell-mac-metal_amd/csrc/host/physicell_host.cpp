@@ -1,6 +1,7 @@
 // physicell_host.cpp -- host classes (Microenvironment, CellMechanics,
 // CellData, Simulation) over the hc_* C-ABI, and the pc_* C entry points.
 #include "physicell_host.h"
+#include "hc_internal.h"
 
 #include <hip/hip_runtime.h>
 
@@ -365,6 +366,25 @@ uint64_t CellMechanics::neighborQuery(CellData& cells, int kind, uint32_t* offse
     return total;
 }
 
+void CellMechanics::loadMirror(CellData& cells) {
+    const size_t bytes = hc::mirror_layout(max_cells_).total;
+    if (!mirror_.get() || mirror_.size() < bytes) mirror_.allocate(ctx_, bytes);
+    check(ctx_, hc::mirror_load(ctx_, cells.gpu_buffer(), hash_.get(), mirror_.get(), &params_, &mgrid_,
+                                cells.num_cells, cells.max_cells, total_mech_voxels_),
+          "mechanics mirror: load");
+}
+
+void CellMechanics::mirrorStep(CellData& cells, double dt) {
+    check(ctx_, hc::mirror_step(ctx_, hash_.get(), mirror_.get(), &params_, &mgrid_, dt, cells.num_cells,
+                                cells.max_cells, total_mech_voxels_),
+          "mechanics mirror: step");
+}
+
+void CellMechanics::syncMirror(CellData& cells) {
+    check(ctx_, hc::mirror_sync(ctx_, cells.gpu_buffer(), mirror_.get(), cells.num_cells, cells.max_cells),
+          "mechanics mirror: sync");
+}
+
 uint64_t CellMechanics::neighborQuery(CellData& cells, int kind, std::vector<uint32_t>& offsets,
                                       std::vector<uint32_t>& idx) {
     offsets.assign((size_t)cells.num_cells + 1, 0u);
@@ -451,6 +471,7 @@ void Simulation::setCells(uint32_t n, const double* x, const double* y, const do
         }
     }
     cells.syncToDevice();
+    mirror_state_ = 0;
     microenv.planSecretion(cells, config.dt_diffusion);
     if (graph_exec_) {
         (void)hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
@@ -463,7 +484,35 @@ void Simulation::diffusionSubstep() {
     microenv.applySecretion();
 }
 
+bool Simulation::useMirror() const {
+    return mirror_on_ && !motility_on_ && !config.mixed_precision && cells.num_cells > 0;
+}
+
+void Simulation::setMechanicsMirror(bool on) {
+    syncMirror();
+    mirror_on_ = on;
+    if (!on) mirror_state_ = 0;
+    if (graph_exec_) {
+        (void)hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
+        graph_exec_ = nullptr;
+    }
+}
+
+void Simulation::syncMirror() {
+    if (mirror_state_ != 1) return;
+    mechanics.syncMirror(cells);
+    mirror_state_ = 2;
+}
+
 void Simulation::mechanicsStep() {
+    if (useMirror()) {
+        if (mirror_state_ == 0) mechanics.loadMirror(cells);
+        mechanics.mirrorStep(cells, config.dt_mechanics);
+        mirror_state_ = 1;
+        return;
+    }
+    syncMirror();
+    mirror_state_ = 0;                           // the fields move on; the mirror is rebuilt when next used
     mechanics.computeForces(cells, config.dt_mechanics);
     if (motility_on_) {
         // updateMotility between forces and AB2 (src/main.mm:382-384)
@@ -478,6 +527,8 @@ void Simulation::mechanicsStep() {
 void Simulation::setMotility(uint64_t seed, const double* persistence, const double* bias, const double* speed,
                              const uint32_t* is_motile, const uint32_t* restrict_2d, const int32_t* chem_index,
                              const int32_t* chem_dir, const double* sensitivity) {
+    syncMirror();
+    mirror_state_ = 0;
     const uint32_t ns = cells.n_substrates, mc = cells.max_cells, n = cells.num_cells;
     motility_.allocate(ctx, hc_motility_bytes(mc, ns));
     uint8_t* h = (uint8_t*)motility_.host();
@@ -536,6 +587,8 @@ void Simulation::setVolume(uint64_t seed, const double* const* fields) {
 uint32_t Simulation::updatePhenotypeSoA(double dt, int use_ratio, const uint32_t* mask, const uint32_t* parents,
                                         uint32_t n_parents, const uint32_t* removed, uint32_t n_removed) {
     if (!volume_.get()) throw std::runtime_error("updatePhenotypeSoA: call setVolume first");
+    syncMirror();
+    mirror_state_ = 0;                           // divisions and removals move agents
     const uint32_t mc = cells.max_cells, ns = cells.n_substrates;
     uint32_t n = cells.num_cells;
     hc_buf* mbuf = nullptr;
@@ -597,12 +650,14 @@ uint32_t attach_greedy(uint32_t n, const std::vector<uint32_t>& off, const std::
 
 uint32_t Simulation::formAttachments(double dt, const double* rate, uint32_t* count, uint32_t* targets,
                                      uint64_t seed) {
+    syncMirror();
     std::vector<uint32_t> off, idx;
     mechanics.neighborQuery(cells, HC_QUERY_ATTACH, off, idx);
     return attach_greedy(cells.num_cells, off, idx, dt, rate, count, targets, seed);
 }
 
 void Simulation::phenotypeStep() {
+    syncMirror();                                // the voxel refresh reads positions
     microenv.refreshVoxelIndices(cells);
     microenv.planSecretion(cells, config.dt_diffusion);
 }
@@ -746,6 +801,11 @@ void Simulation::runPeriods(int periods, bool use_graph) {
 
 void Simulation::launchPeriodGraph() {
     hipStream_t stream = hc_stream_of(ctx);
+    if (useMirror() && mirror_state_ == 0) {
+        // the captured period steps the mirror; build it first, outside the graph
+        mechanics.loadMirror(cells);
+        mirror_state_ = 2;
+    }
     if (!graph_exec_) {
         check(ctx, hc_wait(ctx), "wait");
         hipGraph_t graph = nullptr;
@@ -768,9 +828,13 @@ void Simulation::launchPeriodGraph() {
     }
     hipError_t e = hipGraphLaunch((hipGraphExec_t)graph_exec_, stream);
     if (e != hipSuccess) throw std::runtime_error(std::string("graph launch: ") + hipGetErrorString(e));
+    if (useMirror()) mirror_state_ = 1;          // the replayed mechanics step moved the mirror on
 }
 
-void Simulation::wait() { check(ctx, hc_wait(ctx), "wait"); }
+void Simulation::wait() {
+    syncMirror();
+    check(ctx, hc_wait(ctx), "wait");
+}
 
 }  // namespace physicell
 
@@ -877,6 +941,7 @@ int pc_sim_write_snapshot(pc_sim* s, const char* folder, int frame) {
     if (!folder) return HC_ERR_ARG;
     int rc = 0;
     const int g = guarded(s, [&](Simulation& m) {
+        m.syncMirror();
         m.cells.syncToHost();
         CellData& c = m.cells;
         const hc_grid gr = m.microenv.getGridParams();
@@ -906,6 +971,7 @@ int pc_sim_neighbor_query(pc_sim* s, int kind, uint32_t* offsets, uint32_t* idx,
                           uint64_t* total) {
     if (!total) return HC_ERR_ARG;
     return guarded(s, [&](Simulation& m) {
+        m.syncMirror();
         *total = m.mechanics.neighborQuery(m.cells, kind, offsets, idx, capacity);
     });
 }
@@ -944,6 +1010,10 @@ int pc_sim_set_overlap(pc_sim* s, int on) {
     return guarded(s, [&](Simulation& m) { m.setOverlap(on != 0); });
 }
 
+int pc_sim_set_mechanics_mirror(pc_sim* s, int on) {
+    return guarded(s, [&](Simulation& m) { m.setMechanicsMirror(on != 0); });
+}
+
 int pc_sim_substrate_totals(pc_sim* s, double* out) {
     if (!out) return HC_ERR_ARG;
     return guarded(s, [&](Simulation& m) { m.microenv.totals(out); });
@@ -952,6 +1022,7 @@ int pc_sim_substrate_totals(pc_sim* s, double* out) {
 int pc_sim_get_field(pc_sim* s, uint32_t field, void* out) {
     if (!out || field >= HC_NUM_CELL_FIELDS) return HC_ERR_ARG;
     return guarded(s, [&](Simulation& m) {
+        m.syncMirror();
         m.cells.syncFieldToHost(field);
         if (field < HC_NUM_F64_FIELDS)
             std::memcpy(out, m.cells.field(field), (size_t)m.cells.num_cells * sizeof(double));

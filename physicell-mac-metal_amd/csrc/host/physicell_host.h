@@ -184,13 +184,21 @@ public:
     uint64_t neighborQuery(CellData& cells, int kind, uint32_t* offsets, uint32_t* idx, uint64_t capacity);
     // the same into host vectors sized to the result
     uint64_t neighborQuery(CellData& cells, int kind, std::vector<uint32_t>& offsets, std::vector<uint32_t>& idx);
+    // Hash-ordered mechanics mirror (FP64 forces; hc::mirror_*): loadMirror
+    // builds it from the cell fields, mirrorStep runs one mechanics step on it
+    // (hash -> forces -> AB2, the cell fields untouched), syncMirror writes
+    // positions, velocities, previous velocities, pressure and mechanics
+    // voxels back into the cell fields.
+    void loadMirror(CellData& cells);
+    void mirrorStep(CellData& cells, double dt);
+    void syncMirror(CellData& cells);
 
 private:
     hc_ctx* ctx_ = nullptr;
     hc_mech params_{};
     hc_grid mgrid_{};
     uint32_t total_mech_voxels_ = 0, max_cells_ = 0;
-    DeviceBuffer hash_, q_off_, q_idx_;
+    DeviceBuffer hash_, q_off_, q_idx_, mirror_;
     std::vector<uint32_t> host_offsets_, host_cells_;
 };
 
@@ -241,6 +249,14 @@ public:
     // (num_cells, num_cells, num_cells x 12), drawing from std::mt19937(seed).
     // Returns the number of attachments formed.
     uint32_t formAttachments(double dt, const double* rate, uint32_t* count, uint32_t* targets, uint64_t seed);
+    // The FP64 mechanics step without motility runs on the hash-ordered
+    // mirror (on by default); the cell fields it changes are written back
+    // before anything else reads them (the phenotype step, field reads,
+    // snapshots, neighbour queries, wait).  Same results either way.
+    void setMechanicsMirror(bool on);
+    // writes the mirror back into the cell fields when they are behind it
+    void syncMirror();
+    // waits for queued work; the cell fields are current afterwards
     void wait();
     int substepsPerMechanicsStep() const;
 
@@ -262,6 +278,11 @@ private:
     void* fork_ = nullptr;         // hipEvent_t
     void* join_ = nullptr;         // hipEvent_t
     bool overlap_ = true;
+    bool useMirror() const;
+    // 0: the cell fields hold the mechanics state (the mirror is not built);
+    // 1: the mirror does and the fields are behind; 2: both agree
+    int mirror_state_ = 0;
+    bool mirror_on_ = true;
     DeviceBuffer motility_;
     DeviceBuffer volume_, mask_;
     bool motility_on_ = false;

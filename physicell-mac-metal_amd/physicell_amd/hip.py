@@ -159,7 +159,7 @@ def load(path=LIB_PATH):
         "pc_sim_phenotype_step": (c_i32, [c_vp]),
         "pc_sim_run": (c_i32, [c_vp, c_f64, c_vp]), "pc_sim_run_periods": (c_i32, [c_vp, c_i32, c_i32]),
         "pc_sim_wait": (c_i32, [c_vp]), "pc_sim_get_density": (c_i32, [c_vp, c_vp]),
-        "pc_sim_set_overlap": (c_i32, [c_vp, c_i32]), "pe_set_overlap": (c_i32, [c_vp, c_i32]),
+        "pc_sim_set_overlap": (c_i32, [c_vp, c_i32]), "pc_sim_set_mechanics_mirror": (c_i32, [c_vp, c_i32]), "pe_set_overlap": (c_i32, [c_vp, c_i32]),
         "pc_sim_set_density": (c_i32, [c_vp, c_vp]), "pc_sim_get_field": (c_i32, [c_vp, c_u32, c_vp]),
         "pc_sim_get_hash": (c_i32, [c_vp, c_vp, c_vp]),
         "pc_sim_substrate_totals": (c_i32, [c_vp, c_vp]),
@@ -431,6 +431,10 @@ class Simulation:
 
     def set_overlap(self, on):
         self._check(load().pc_sim_set_overlap(self.h, 1 if on else 0), "pc_sim_set_overlap")
+
+    def set_mechanics_mirror(self, on):
+        """FP64 mechanics on the hash-ordered mirror (default on; same results)."""
+        self._check(load().pc_sim_set_mechanics_mirror(self.h, 1 if on else 0), "pc_sim_set_mechanics_mirror")
 
     def set_motility(self, seed, persistence=None, bias=None, speed=None, is_motile=None, restrict_2d=None,
                      chem_index=None, chem_dir=None, sensitivity=None):
