@@ -354,10 +354,9 @@ int pc_sim_set_overlap(pc_sim* sim, int on);
  * velocities in the order of the last hash build), so its per-agent passes
  * read and write in spatial order; positions, velocities, previous
  * velocities, pressure and mechanics voxels are written back into the cell
- * buffer before anything else reads them (the phenotype step, field reads,
- * snapshots, neighbour queries, attachments, pc_sim_wait).  Code that reads
- * the cell buffer through hc_* directly calls pc_sim_wait first.  The results
- * are the same bit for bit with the mirror on or off. */
+ * buffer before anything in pc_sim_* reads them (the phenotype step, field
+ * reads, snapshots, neighbour queries, attachments, phenotype updates).  The
+ * results are the same bit for bit with the mirror on or off. */
 int pc_sim_set_mechanics_mirror(pc_sim* sim, int on);
 /* Enables motility in the mechanics step (forces -> motility -> AB2, as
  * src/main.mm:376-387).  Per-cell arrays as MotilityData (src/motility.h);

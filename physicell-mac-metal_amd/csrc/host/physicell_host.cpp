@@ -831,10 +831,7 @@ void Simulation::launchPeriodGraph() {
     if (useMirror()) mirror_state_ = 1;          // the replayed mechanics step moved the mirror on
 }
 
-void Simulation::wait() {
-    syncMirror();
-    check(ctx, hc_wait(ctx), "wait");
-}
+void Simulation::wait() { check(ctx, hc_wait(ctx), "wait"); }
 
 }  // namespace physicell
 
@@ -1013,6 +1010,7 @@ int pc_sim_set_overlap(pc_sim* s, int on) {
 int pc_sim_set_mechanics_mirror(pc_sim* s, int on) {
     return guarded(s, [&](Simulation& m) { m.setMechanicsMirror(on != 0); });
 }
+
 
 int pc_sim_substrate_totals(pc_sim* s, double* out) {
     if (!out) return HC_ERR_ARG;

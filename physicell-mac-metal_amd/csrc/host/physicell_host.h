@@ -251,12 +251,12 @@ public:
     uint32_t formAttachments(double dt, const double* rate, uint32_t* count, uint32_t* targets, uint64_t seed);
     // The FP64 mechanics step without motility runs on the hash-ordered
     // mirror (on by default); the cell fields it changes are written back
-    // before anything else reads them (the phenotype step, field reads,
-    // snapshots, neighbour queries, wait).  Same results either way.
+    // before anything else here reads them (the phenotype step, field reads,
+    // snapshots, neighbour queries, attachments) and by syncMirror.  Same
+    // results either way.
     void setMechanicsMirror(bool on);
     // writes the mirror back into the cell fields when they are behind it
     void syncMirror();
-    // waits for queued work; the cell fields are current afterwards
     void wait();
     int substepsPerMechanicsStep() const;
 

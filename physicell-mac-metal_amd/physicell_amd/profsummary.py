@@ -22,6 +22,7 @@ KERNEL_ALIASES = [
     ("k_sweep_lane<2", "sweep_z"), ("k_sweep_part<0", "sweep_x"), ("k_sweep_part<1", "sweep_y"),
     ("k_sweep_part<2", "sweep_z"), ("k_forces", "forces"), ("k_sec_apply", "secretion_apply"),
     ("k_integrate", "integrate"), ("k_stable_gather", "hash_gather"), ("k_mech_keys", "hash_keys"),
+    ("k_m_integrate", "integrate"), ("k_m_gather", "hash_gather"), ("k_m_keys", "hash_keys"),
 ]
 
 
