@@ -433,14 +433,15 @@ struct Member {
 
 class Ensemble {
     // sweep work lists, one per (axis, segments per line), in launch order:
-    // x (4 segments), x (16 segments), y, z.  The 4-segment lists hold members
-    // of both register windows (16 or 32 values per lane, each member's own:
-    // BatchSim::wide), those with the longer lines first; the 16-segment x
-    // list holds lines of at most 16 values per lane.
-    static constexpr int NLIST = 4;
-    static constexpr int LIST_AXIS[NLIST] = {0, 0, 1, 2};
-    static constexpr int LIST_P[NLIST] = {4, 16, 4, 4};
-    static constexpr int LIST_MM[NLIST] = {0, 16, 0, 0};
+    // x (4 segments), x (16 segments), x (8 segments: FP32 lines of 131..258),
+    // y, z.  The 4-segment lists hold members of both register windows (16 or
+    // 32 values per lane, each member's own: BatchSim::wide), those with the
+    // longer lines first; the 16-segment x list holds lines of at most 16
+    // values per lane, the 8-segment one lines of at most 32.
+    static constexpr int NLIST = 5;
+    static constexpr int LIST_AXIS[NLIST] = {0, 0, 0, 1, 2};
+    static constexpr int LIST_P[NLIST] = {4, 16, 8, 4, 4};
+    static constexpr int LIST_MM[NLIST] = {0, 16, 32, 0, 0};
     static int list_of(int axis, int P) {
         for (int li = 0; li < NLIST; li++)
             if (LIST_AXIS[li] == axis && LIST_P[li] == P) return li;

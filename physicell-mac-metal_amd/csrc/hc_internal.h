@@ -127,11 +127,15 @@ __device__ inline uint32_t xcd_chunk(uint32_t b, uint32_t nb) {
 // 161 VGPRs in FP64 (3 waves per SIMD) and 92 in FP32 (5 waves) against 88
 // (5 waves) and 64 (8 waves) for 8 segments of up to 32.  x lines are staged through LDS, a wave's lines at a
 // time: 4 segments (16 lines per wave) while 64 lines of n values fit the
-// staging tile (n <= 130 in FP32, n <= 96 in FP64), else 16 segments (4
-// lines per wave).
+// staging tile (n <= 130 in FP32, n <= 96 in FP64); longer FP32 lines (up to
+// 258) take 8 segments (8 lines per wave, windows of up to 32: half the
+// interface work per line of 16 segments; the 200-node x sweep of one FP32
+// simulation 57.1 -> 45.0 us), everything else 16 segments (4 lines per
+// wave; for FP64 200-node lines 8 segments measured 87 -> 92 us).
 __host__ __device__ inline int part_p(int axis, uint32_t n, bool fp32) {
     if (axis != 0) return n > 130u ? 8 : 4;
-    return n <= (fp32 ? 130u : 96u) ? 4 : 16;
+    if (n <= (fp32 ? 130u : 96u)) return 4;
+    return (fp32 && n <= 258u) ? 8 : 16;
 }
 __host__ __device__ inline uint32_t part_mt(uint32_t nmax, int P) {
     return nmax > 2 ? (nmax - 2 + P - 1) / P : 1u;

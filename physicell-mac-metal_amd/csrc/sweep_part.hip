@@ -93,11 +93,9 @@ __device__ inline void stage_in(const global_t<T>* __restrict__ src, T* tile, ui
     if (lane < nval - done) tile[done + lane] = src[done + lane];
 }
 
-// The bulk stores are non-temporal: a sweep writes each value once and the
-// next sweep reads it only after the whole field has streamed through.  On
-// one box, alternating builds: FP64 config-2 x sweep 107.5 -> 106.8 us (FP32
-// unchanged); the strided y/z stores measured slower with it (FP32 y 60 ->
-// 72 us) and keep ordinary stores.
+// Ordinary stores: the next sweep (y) reads the values soon after.  Non-
+// temporal stores here left FP64 x unchanged and, with 8-segment FP32 x
+// lines, slowed the following y sweep from 47 to 78 us (config 2).
 template <typename T, int RV>
 __device__ inline void stage_out(global_t<T>* __restrict__ dst, const T* tile, uint32_t nval, uint32_t lane) {
     constexpr uint32_t E = 16 / sizeof(T);
@@ -110,7 +108,7 @@ __device__ inline void stage_out(global_t<T>* __restrict__ dst, const T* tile, u
 #pragma unroll
     for (int r = 0; r < RV; r++) {
         const uint32_t e = lane + 64u * r;
-        if (e < nvec) __builtin_nontemporal_store(t2[e], &d2[e]);
+        if (e < nvec) d2[e] = t2[e];
     }
     const uint32_t done = head + E * nvec;
     if (lane < nval - done) dst[done + lane] = tile[done + lane];
@@ -627,6 +625,9 @@ int probe_kernels_part(hc_ctx* ctx) {
         (const void*)k_sweep_part<0, 4, 16, false, double>, (const void*)k_sweep_part<0, 4, 32, false, double>,
         (const void*)k_sweep_part<0, 4, 16, true, float>, (const void*)k_sweep_part<0, 4, 32, true, float>,
         (const void*)k_sweep_part<0, 4, 16, false, float>, (const void*)k_sweep_part<0, 4, 32, false, float>,
+        (const void*)k_sweep_part<0, 8, 16, true, float>, (const void*)k_sweep_part<0, 8, 32, true, float>,
+        (const void*)k_sweep_part<0, 8, 16, false, float>, (const void*)k_sweep_part<0, 8, 32, false, float>,
+        (const void*)k_sweep_part_batch<0, 8, 32, true, float>, (const void*)k_sweep_part_item<0, 8, 32, true, float>,
         (const void*)k_sweep_part_batch<0, 16, 16, true, double>, (const void*)k_sweep_part_item<0, 16, 16, true, double>,
         (const void*)k_sweep_part_batch<0, 4, 0, true, double>, (const void*)k_sweep_part_item<0, 4, 0, true, double>,
         (const void*)k_sweep_part_batch<1, 4, 0, false, double>, (const void*)k_sweep_part_item<1, 4, 0, false, double>,
@@ -700,6 +701,9 @@ static int launch_sweep_part_t(hc_ctx* ctx, int axis, T* rho, const hc_grid* g, 
     if (axis == 0 && P == 4) {
         if (xlds) launch_part_axis<0, 4, true, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap, s0);
         else launch_part_axis<0, 4, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap, s0);
+    } else if (axis == 0 && P == 8) {
+        if (xlds) launch_part_axis<0, 8, true, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap, s0);
+        else launch_part_axis<0, 8, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap, s0);
     } else if (axis == 0) {
         if (xlds) launch_part_axis<0, 16, true, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap, s0);
         else launch_part_axis<0, 16, false, T>(ctx, mm, grid, shm, rho, tables, g, nmax, fuse_dirichlet, xmap, s0);
@@ -732,20 +736,28 @@ static void launch_batch_kernel(hc_ctx* ctx, unsigned grid_blocks, size_t shm, c
 }
 
 template <typename T>
+constexpr bool fp32_t() {
+    return sizeof(T) == sizeof(float);
+}
+
+template <typename T>
 static int launch_batch_t(hc_ctx* ctx, int axis, int P, int mm, const BatchSim* sims, const BatchWork* work,
                           uint32_t nwork, uint32_t nmax_all, uint32_t max_line, unsigned grid_blocks) {
     if (nwork == 0) return 0;
     if (axis != 0 && P != 4) return fail(ctx, HC_ERR_ARG, "batched sweep: y and z use 4 segments per line");
-    // mm: 16 (every member's lines fit 16 values per lane), or 0 (each
-    // member's own window, 16 or 32; P = 4 only)
-    if ((mm != 0 && mm != 16) || (P == 16 && mm != 16) || part_window(max_line, P) > (mm ? mm : 32))
+    // mm: 16 (every member's lines fit 16 values per lane; P = 16), 32
+    // (P = 8, FP32 x lines), or 0 (each member's own window, 16 or 32; P = 4)
+    const bool ok = (P == 4 && (mm == 0 || mm == 16)) || (P == 16 && mm == 16) || (P == 8 && axis == 0 && fp32_t<T>() && mm == 32);
+    if (!ok || part_window(max_line, P) > (mm ? mm : 32))
         return fail(ctx, HC_ERR_CAPACITY, "batched sweep: line length outside the list's register window");
     const int LPW = 64 / P;
-    const size_t tab_bytes = P == 16 ? PartLds<16, T>::bytes(nmax_all) : PartLds<4, T>::bytes(nmax_all);
+    const size_t tab_bytes = P == 16 ? PartLds<16, T>::bytes(nmax_all)
+                                     : (P == 8 ? PartLds<8, T>::bytes(nmax_all) : PartLds<4, T>::bytes(nmax_all));
     const size_t shm = tab_bytes + (axis == 0 ? (size_t)4 * xtile_elems<T>(LPW, max_line) * sizeof(T) : 0);
     if (shm > 64 * 1024) return fail(ctx, HC_ERR_CAPACITY, "batched sweep: LDS");
     KernelTimer tm(ctx, HC_K_SWEEP_X + axis);
     if (axis == 0 && P == 16) launch_batch_kernel<0, 16, 16, true, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
+    else if (axis == 0 && P == 8) launch_batch_kernel<0, 8, 32, true, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
     else if (axis == 0) launch_batch_kernel<0, 4, 0, true, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
     else if (axis == 1) launch_batch_kernel<1, 4, 0, false, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);
     else launch_batch_kernel<2, 4, 0, false, T>(ctx, grid_blocks, shm, sims, work, nwork, nmax_all);

@@ -342,3 +342,32 @@ def test_member_queries_equal_single(mixed):
     for j in (0, 1):
         assert_same(member_state(ens, ids[j]), run_single(*members[j], P + 1, mixed), f"member {j} after queries")
     ens.close()
+
+
+def box_member(nx, ny, nz, n_cells, seed):
+    """A member on an nx x ny x nz grid of 20 um voxels with uniform random agents."""
+    dx = 20.0
+    subs = [S.Substrate("oxygen", 1e5, 0.1, 38.0, True, 38.0), S.Substrate("factor", 1000.0, 0.01, 0.0, False, 0.0)]
+    sc = S.Scenario(f"box{nx}x{ny}x{nz}", 0.0, nx * dx, 0.0, ny * dx, 0.0, nz * dx, dx, dx, dx, False, subs,
+                    max_time=10.0)
+    sc.dt_pheno = 0.5
+    p = np.random.default_rng(seed).random((n_cells, 3)) * np.array([nx * dx, ny * dx, nz * dx])
+    c = S.uniform_cells(p[:, 0], p[:, 1], p[:, 2], 2)
+    c.upt[:, 0] = 10.0
+    c.sec[:, 1] = 1.0
+    c.sat[:, 1] = 1.0
+    return sc, c
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_long_x_lines_members_equal_single(mixed):
+    """x edges of 131..258 voxels: 8 segments per line in the FP32-mixed form
+    (16 in FP64) in their own work list, beside a member of short lines."""
+    members = [box_member(150, 12, 10, 1500, 61), box_member(258, 8, 7, 1500, 62), small_member(263)]
+    P = 7
+    ens = hip.Ensemble(mixed=mixed)
+    ids = [ens.add(sc, c) for sc, c in members]
+    ens.run_periods(P, use_graph=True)
+    for mid, (sc, c) in zip(ids, members):
+        assert_same(member_state(ens, mid), run_single(sc, c, P, mixed), f"member {sc.name} mixed={mixed}")
+    ens.close()
