@@ -150,15 +150,21 @@ __global__ __launch_bounds__(256) void k_b_gather(const EnsSim* __restrict__ sim
 // passes with a list of marked candidates (FP32: LIST, force_agent32x; FP64:
 // force_agent64x where member_markx holds), else the chunked ones.  rmax: the
 // largest FP32 radius over all members.
+// A member's agents hold the sorted positions [cell_off, cell_off + n) (its
+// bins follow the earlier members' bins), so the agent tiles of a member,
+// taken as sorted positions, give each workgroup one member: its descriptor
+// is uniform (scalar registers) instead of a per-lane gather.
 template <bool FP32, bool LIST = true>
-__global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sims, const uint32_t* __restrict__ agent_sim,
+__global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sims, const AgentTile* __restrict__ tiles,
                                                   const void* __restrict__ recs, const float4* __restrict__ pos32,
-                                                  const uint32_t* __restrict__ offsets, uint32_t ntot,
+                                                  const uint32_t* __restrict__ offsets,
                                                   const float* __restrict__ rmax) {
     __shared__ uint32_t marks[LIST ? 4 * MARKX_CAP * 64 : 1];
     uint32_t* list = marks + (LIST ? (threadIdx.x >> 6) * (MARKX_CAP * 64) + (threadIdx.x & 63) : 0);
-    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    if (t >= ntot) return;
+    const AgentTile tl = tiles[xcd_block(blockIdx.x, gridDim.x)];
+    const EnsSim m = sims[tl.sim];
+    if (tl.c0 + threadIdx.x >= m.n) return;
+    const uint32_t t = m.cell_off + tl.c0 + threadIdx.x;   // sorted position
     uint32_t idx, key;
     if (FP32) {
         const CellRec32 me = reinterpret_cast<const CellRec32*>(recs)[t];
@@ -169,7 +175,6 @@ __global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sim
         idx = me.idx;
         key = me.key;
     }
-    const EnsSim m = sims[agent_sim[idx]];
     const uint32_t c = idx - m.cell_off;
     const uint32_t lk = key - m.bin_off;
     const uint32_t vi = lk % m.mnx, vj = (lk / m.mnx) % m.mny, vk = lk / (m.mnx * m.mny);
@@ -1206,16 +1211,16 @@ private:
         }
         {
             hc::KernelTimer tm(ctx_, HC_K_FORCES);
-            const dim3 g(hc::blocks_for(w.ntot, 256));
+            const dim3 g(w.ntiles);
             // positions in the records' list entries take 27 bits (force_agent32x)
             const float* rmax = (const float*)hc_device_ptr(w.rmax.get());
             if (mixed_)
                 hipLaunchKernelGGL((w.ntot < (1u << hc::MARK32_POS_BITS) ? hc::k_b_forces<true, true>
                                                                          : hc::k_b_forces<true, false>),
-                                   g, dim3(256), 0, st, sims, asim, recs, nullptr, offsets, w.ntot, rmax);
+                                   g, dim3(256), 0, st, sims, tiles, recs, nullptr, offsets, rmax);
             else
-                hipLaunchKernelGGL(hc::k_b_forces<false>, g, dim3(256), 0, st, sims, asim, recs, pos32, offsets,
-                                   w.ntot, rmax);
+                hipLaunchKernelGGL(hc::k_b_forces<false>, g, dim3(256), 0, st, sims, tiles, recs, pos32, offsets,
+                                   rmax);
             check(ctx_, hc::check_launch(ctx_, "k_b_forces"), "forces");
         }
         if (any_motility_) {
