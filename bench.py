@@ -6,9 +6,9 @@ is stated).  One "step" = one mechanics period: 10 diffusion substeps (x, y, z
 Thomas sweeps with the Dirichlet re-apply fused into the z sweep, then
 secretion/uptake) followed by one mechanics step (hash, forces, AB2).  By
 default the periods are launched eagerly and serially (measured as fast as
-hipGraph replay with the mechanics step overlapped: 4.03-4.04 vs 4.03 ms per
-config-2 step); --graph and --overlap select those forms.  Inputs are
-resident in HBM before the timed region.
+hipGraph replay, and overlapping the mechanics step with the substeps on a
+second stream measured no faster); --graph and --overlap select those forms.
+Inputs are resident in HBM before the timed region.
 
 Prints ONE JSON line (rank 0) with agent-steps/s as `value`, voxel-substeps/s,
 p50/p99 step latency, the roofline of the dominant kernel (its algorithmic
@@ -120,7 +120,7 @@ def cpu_baseline(spec, cells, periods, substeps):
                       f"1 thread, {dt1:.1f} s"}
 
 
-def ensemble_forms(members, steps, warmup, overlap=True):
+def ensemble_forms(members, steps, warmup, overlap=False):
     """The same ensemble members in three forms, each timed over `steps`
     periods after `warmup`: the batched FP32-mixed ensemble, the batched FP64
     ensemble, and the plain FP64 single-simulation path (one pc_sim per
@@ -216,7 +216,7 @@ def bench_ensemble(args):
     n_sub = 10
     print(f"[bench] ensemble: {len(members)} members, {n_cells} agents, {n_vals} voxel values; "
           f"setup {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
-    res = ensemble_forms(members, args.steps, args.warmup, overlap=not args.no_overlap)
+    res = ensemble_forms(members, args.steps, args.warmup, overlap=args.overlap)
     bm, pl = res["batched_mixed"], res["plain_fp64_per_sim"]
     kern = bm["kernels"]
     dom = max(kern, key=lambda k: kern[k]["total_ms"])

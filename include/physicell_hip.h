@@ -343,7 +343,7 @@ int pc_sim_run(pc_sim* sim, double max_time, int64_t* counts);
  * clock that persists across calls and restarts at pc_sim_set_cells; so
  * P periods from fresh cells equal pc_sim_run(sim, (P*k - 1)*dt_diffusion). */
 int pc_sim_run_periods(pc_sim* sim, int periods, int use_graph);
-/* on != 0 (default): in pc_sim_run_periods, each period's mechanics step
+/* on != 0 (default off): in pc_sim_run_periods, each period's mechanics step
  * runs on a second stream concurrently with its diffusion substeps (they
  * touch disjoint data; the results are the same).  pc_sim_run keeps the
  * reference's serial order. */
@@ -610,8 +610,9 @@ int pe_set_grid_blocks(pe_ensemble* e, int axis, uint32_t blocks);
  * afterwards) and keeps the fastest; writes, per axis, the grid of its
  * busiest list and the summed mean launch times (us). */
 int pe_autotune(pe_ensemble* e, int reps, uint32_t* blocks_out, double* us_out);
-/* on != 0 (default): each period's batched mechanics step runs on a second
- * stream concurrently with the batched diffusion substeps (same results). */
+/* on != 0 (default off): each period's batched mechanics step runs on a
+ * second stream concurrently with the batched diffusion substeps (same
+ * results). */
 int pe_set_overlap(pe_ensemble* e, int on);
 hc_ctx* pe_context(pe_ensemble* e);
 

@@ -1346,7 +1346,7 @@ private:
     hc_ctx* ctx_ = nullptr;
     hipStream_t side_ = nullptr;
     hipEvent_t fork_ = nullptr, join_ = nullptr;
-    bool overlap_ = true;
+    bool overlap_ = false;
     bool mixed_ = false, dirty_ = true, plan_dirty_ = true, any_motility_ = false;
     uint32_t next_id_ = 0;
     unsigned grid_blocks_[NLIST] = {};                 // 0: one workgroup per work item

@@ -218,9 +218,9 @@ public:
     void mechanicsStep();
     void phenotypeStep();
     void run(double max_time, int64_t counts[3]);
-    // periods x (substeps + one mechanics step); with overlap on (default),
-    // the mechanics step of a period runs on a second stream concurrently
-    // with its substeps (they touch disjoint data; same results).  The
+    // periods x (substeps + one mechanics step); with overlap on (default
+    // off), the mechanics step of a period runs on a second stream
+    // concurrently with its substeps (they touch disjoint data; same results).  The
     // phenotype step (voxel refresh + secretion re-plan) fires every
     // dt_phenotype as in run(), on a clock that persists across calls and
     // restarts when the cells are set.
@@ -277,7 +277,7 @@ private:
     void* side_ = nullptr;         // hipStream_t for the overlapped mechanics step
     void* fork_ = nullptr;         // hipEvent_t
     void* join_ = nullptr;         // hipEvent_t
-    bool overlap_ = true;
+    bool overlap_ = false;
     bool useMirror() const;
     // 0: the cell fields hold the mechanics state (the mirror is not built);
     // 1: the mirror does and the fields are behind; 2: both agree

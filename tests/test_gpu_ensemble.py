@@ -371,3 +371,21 @@ def test_long_x_lines_members_equal_single(mixed):
     for mid, (sc, c) in zip(ids, members):
         assert_same(member_state(ens, mid), run_single(sc, c, P, mixed), f"member {sc.name} mixed={mixed}")
     ens.close()
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_overlapped_periods_equal_serial(mixed):
+    """pe_set_overlap(1): the mechanics step on a second stream beside the
+    substeps gives the serial periods' results bit for bit (graph and eager)."""
+    members = [small_member(280), small_member(281), small_member(282)]
+    out = []
+    for overlap in (False, True):
+        ens = hip.Ensemble(mixed=mixed)
+        ens.set_overlap(overlap)
+        ids = [ens.add(sc, c) for sc, c in members]
+        ens.run_periods(4, use_graph=True)
+        ens.run_periods(3, use_graph=False)
+        out.append([member_state(ens, i) for i in ids])
+        ens.close()
+    for a, b, (sc, _) in zip(out[0], out[1], members):
+        assert_same(b, a, f"member {sc.name} overlapped, mixed={mixed}")
