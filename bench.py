@@ -85,11 +85,38 @@ def step_bytes(spec, n_cells, substeps, mixed=False):
                      f"{4 + (24 if mixed else 32) * ns} B, AS {n_cells} x {180 if mixed else 248} B"}
 
 
+def load_oracle():
+    """The FP64 oracle module, bound to a copy of liboracle compiled for this
+    host's CPU (-O3 -march=native, otherwise the flags of oracle/Makefile) in
+    a temporary directory when g++ is available; else to the shipped build.
+    Returns (module, description of the build)."""
+    import shutil
+    import subprocess
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    build = "shipped oracle/liboracle.so (-O3, generic x86-64)"
+    cxx = shutil.which("g++")
+    if cxx and O._lib is None:
+        tmp = tempfile.mkdtemp(prefix="oracle_native_")
+        import atexit
+        atexit.register(shutil.rmtree, tmp, True)
+        out = os.path.join(tmp, "liboracle.so")
+        cmd = [cxx, "-std=c++17", "-O3", "-march=native", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",
+               "-shared", os.path.join(ROOT, "oracle", "oracle.cpp"), "-o", out]
+        try:
+            subprocess.run(cmd, check=True, capture_output=True, timeout=180)
+            O.LIB_PATH = out
+            build = "oracle.cpp compiled on this host: g++ -O3 -march=native -fopenmp -ffp-contract=off"
+        except (OSError, subprocess.SubprocessError):
+            pass
+    return O, build
+
+
 def cpu_baseline(spec, cells, periods, substeps):
     """FP64 oracle (the reference's algorithm restated in C++, OpenMP) on this
     host: `periods` x (substeps diffusion substeps + 1 mechanics step)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
+    O, build = load_oracle()
     threads = max(1, min(16, os.cpu_count() or 1))
 
     def timed(nper, nthreads):
@@ -114,7 +141,7 @@ def cpu_baseline(spec, cells, periods, substeps):
             "one_core": {"value": cells.n / dt1,
                          "voxel_substeps_per_s": spec.n_voxels * len(spec.substrates) * substeps / dt1,
                          "seconds": dt1},
-            "host": {"cpu_model": model, "cpu_count": os.cpu_count()},
+            "host": {"cpu_model": model, "cpu_count": os.cpu_count()}, "build": build,
             "sample": f"{periods} period(s) of {substeps} diffusion substeps + 1 mechanics step of the "
                       f"same workload, FP64 oracle, OpenMP {threads} threads, {dt:.1f} s; and 1 period on "
                       f"1 thread, {dt1:.1f} s"}
@@ -237,8 +264,7 @@ def bench_ensemble(args):
     roof["step_bytes"] = tot
     cpu = None
     if not args.no_cpu_baseline:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
+        O, build = load_oracle()
         threads = max(1, min(16, os.cpu_count() or 1))
         sample = members[:args.cpu_members]
         dt = 0.0
@@ -252,7 +278,7 @@ def bench_ensemble(args):
             dt += time.perf_counter() - t1
         P = args.cpu_periods
         cpu = {"value": sum(c.n for _, c in sample) * P / dt, "unit": "agent-steps/s", "cores": threads,
-               "kind": "port",
+               "kind": "port", "build": build,
                "voxel_substeps_per_s": sum(sc.n_voxels * len(sc.substrates) for sc, _ in sample) * n_sub * P / dt,
                "sample": f"{P} period(s) of members 0..{len(sample) - 1}, FP64 oracle, OpenMP {threads} threads, "
                          f"{dt:.1f} s"}
