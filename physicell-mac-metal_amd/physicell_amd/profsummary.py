@@ -22,7 +22,6 @@ KERNEL_ALIASES = [
     ("k_sweep_lane<2", "sweep_z"), ("k_sweep_part<0", "sweep_x"), ("k_sweep_part<1", "sweep_y"),
     ("k_sweep_part<2", "sweep_z"), ("k_forces", "forces"), ("k_sec_apply", "secretion_apply"),
     ("k_integrate", "integrate"), ("k_stable_gather", "hash_gather"), ("k_mech_keys", "hash_keys"),
-    ("k_m_integrate", "integrate"), ("k_m_gather", "hash_gather"), ("k_m_keys", "hash_keys"),
 ]
 
 
@@ -70,8 +69,11 @@ def traffic(fetch_db, write_db, cfetch_db, cwrite_db):
         wk = sum(w.get(name, [0])) / max(1, len(w.get(name, [1])))
         fetch_b = fk * 1024.0 * (sf or 1.0)
         write_b = wk * 1024.0 * (sw or 1.0)
-        res[short(name)] = {"fetch_bytes": fetch_b, "write_bytes": write_b,
-                            "bytes_per_launch": fetch_b + write_b, "dispatches": len(f.get(name, []))}
+        key = short(name)
+        if key in res:                        # two kernels with one short name: keep both apart
+            key = re.sub(r"\(.*", "", name)
+        res[key] = {"fetch_bytes": fetch_b, "write_bytes": write_b,
+                    "bytes_per_launch": fetch_b + write_b, "dispatches": len(f.get(name, []))}
     return res
 
 
