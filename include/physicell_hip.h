@@ -79,7 +79,9 @@ typedef struct hc_mech {
     uint32_t mech_grid_nx, mech_grid_ny, mech_grid_nz;
     /* 0: FP64 forces (plain form, 64-byte records); 1: FP32 force arithmetic
      * over 32-byte records holding positions relative to their bin's corner
-     * (mixed form).  Keys, bins and AB2 positions are FP64 in both. */
+     * (mixed form); 2: as 1, with every member of the neighbour rows visited
+     * (no mark pass; the same results, for testing).  Keys, bins and AB2
+     * positions are FP64 in all. */
     uint32_t forces_fp32;
 } hc_mech;
 

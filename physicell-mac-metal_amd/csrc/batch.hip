@@ -50,7 +50,7 @@ __device__ inline uint32_t* sim_ufield(const EnsSim& m, int f) {
 __global__ __launch_bounds__(256) void k_b_mech_keys(const EnsSim* __restrict__ sims,
                                                      const AgentTile* __restrict__ tiles,
                                                      uint32_t* __restrict__ counts, uint32_t* __restrict__ kglob,
-                                                     uint32_t* __restrict__ slot) {
+                                                     uint32_t* __restrict__ slot, uint8_t* __restrict__ xslab) {
     const AgentTile tl = tiles[blockIdx.x];
     const EnsSim m = sims[tl.sim];
     const uint32_t c = tl.c0 + threadIdx.x;
@@ -60,7 +60,8 @@ __global__ __launch_bounds__(256) void k_b_mech_keys(const EnsSim* __restrict__ 
         const double* x = sim_field(m, HC_F_POS_X);
         const double* y = sim_field(m, HC_F_POS_Y);
         const double* z = sim_field(m, HC_F_POS_Z);
-        int vi = (int)((x[c] - m.x_min) / m.mvoxel);
+        const double px = x[c];
+        int vi = (int)((px - m.x_min) / m.mvoxel);
         int vj = (int)((y[c] - m.y_min) / m.mvoxel);
         int vk = (int)((z[c] - m.z_min) / m.mvoxel);
         vi = vi < 0 ? 0 : (vi > (int)m.mnx - 1 ? (int)m.mnx - 1 : vi);
@@ -70,6 +71,10 @@ __global__ __launch_bounds__(256) void k_b_mech_keys(const EnsSim* __restrict__ 
         sim_ufield(m, HC_F_MECH_VOXEL)[c] = local;
         key = m.bin_off + local;
         kglob[m.cell_off + c] = key;
+        // FP32 records: the x slab of the record's bin-relative x
+        if (xslab)
+            xslab[m.cell_off + c] =
+                (uint8_t)xslab_of(bin_rel32(px, m.x_min, (uint32_t)vi, m.mvoxel), xslab_scale(m.mvoxel));
     }
     const uint32_t s = ballot_bin(valid, key, counts);
     if (valid) slot[m.cell_off + c] = s;
@@ -84,12 +89,15 @@ __device__ inline bool member_markx(const EnsSim& m) {
            (double)m.mnx * m.mvoxel <= MARKX_MAX_EXTENT;
 }
 
+// FP32: records in each bin by x slab, then index (positions from
+// k_bin_sort32, or slab_rank in bins above BIN_SORT_CAP).
 template <bool FP32>
 __global__ __launch_bounds__(256) void k_b_gather(const EnsSim* __restrict__ sims, const AgentTile* __restrict__ tiles,
                                                   const uint32_t* __restrict__ kglob, const uint32_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ tmp, uint32_t* __restrict__ sorted,
-                                                  const uint32_t* __restrict__ pos_of, void* __restrict__ recs,
-                                                  float4* __restrict__ pos32, float* __restrict__ rpart) {
+                                                  const uint32_t* __restrict__ pos_of, const uint8_t* __restrict__ xslab,
+                                                  void* __restrict__ recs, float4* __restrict__ pos32,
+                                                  float* __restrict__ rpart) {
     const AgentTile tl = tiles[blockIdx.x];
     const EnsSim m = sims[tl.sim];                       // one member per workgroup
     const uint32_t c = tl.c0 + threadIdx.x;
@@ -103,7 +111,10 @@ __global__ __launch_bounds__(256) void k_b_gather(const EnsSim* __restrict__ sim
     const uint32_t beg = offsets[b], end = offsets[b + 1];
     uint32_t pos;
     if (end - beg <= BIN_SORT_CAP) {
-        pos = pos_of[g];                                  // placed by k_bin_sort
+        pos = pos_of[g];                                  // placed by k_bin_sort / k_bin_sort32
+    } else if (FP32) {
+        sorted[beg + stable_rank(tmp, beg, end, g)] = g;
+        pos = beg + slab_rank(tmp, xslab, beg, end, g, xslab[g]);
     } else {
         pos = beg + stable_rank(tmp, beg, end, g);
         sorted[pos] = g;
@@ -113,14 +124,14 @@ __global__ __launch_bounds__(256) void k_b_gather(const EnsSim* __restrict__ sim
         const uint32_t lb = b - m.bin_off;
         const uint32_t bi = lb % m.mnx, bj = (lb / m.mnx) % m.mny, bk = lb / (m.mnx * m.mny);
         CellRec32 r;
-        r.x = (float)(px - (m.x_min + bi * m.mvoxel));
-        r.y = (float)(py - (m.y_min + bj * m.mvoxel));
-        r.z = (float)(pz - (m.z_min + bk * m.mvoxel));
+        r.xg = (float)(px - m.x_min);
+        r.y = bin_rel32(py, m.y_min, bj, m.mvoxel);
+        r.z = bin_rel32(pz, m.z_min, bk, m.mvoxel);
+        r.x = bin_rel32(px, m.x_min, bi, m.mvoxel);
         r.r = (float)sim_field(m, HC_F_RADIUS)[c];
         r.rep = (float)sim_field(m, HC_F_REPULSION)[c];
         r.adh = (float)sim_field(m, HC_F_ADHESION)[c];
         r.idx = g;
-        r.key = b;
         reinterpret_cast<CellRec32*>(recs)[pos] = r;
     } else {
         if (pos32) {
@@ -154,10 +165,15 @@ __global__ __launch_bounds__(256) void k_b_gather(const EnsSim* __restrict__ sim
 // bins follow the earlier members' bins), so the agent tiles of a member,
 // taken as sorted positions, give each workgroup one member: its descriptor
 // is uniform (scalar registers) instead of a per-lane gather.
+// FP32 members whose mechanics grid keeps the FP32 mark's error bound
+// (member_markx, bins of at least 0.5 um) take the x-trimmed mark pass with a
+// list; the others the chunked pass.
 template <bool FP32, bool LIST = true>
 __global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sims, const AgentTile* __restrict__ tiles,
                                                   const void* __restrict__ recs, const float4* __restrict__ pos32,
                                                   const uint32_t* __restrict__ offsets,
+                                                  const uint32_t* __restrict__ xoff,
+                                                  const uint32_t* __restrict__ kglob,
                                                   const float* __restrict__ rmax) {
     __shared__ uint32_t marks[LIST ? 4 * MARKX_CAP * 64 : 1];
     uint32_t* list = marks + (LIST ? (threadIdx.x >> 6) * (MARKX_CAP * 64) + (threadIdx.x & 63) : 0);
@@ -169,7 +185,7 @@ __global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sim
     if (FP32) {
         const CellRec32 me = reinterpret_cast<const CellRec32*>(recs)[t];
         idx = me.idx;
-        key = me.key;
+        key = kglob[idx];
     } else {
         const CellRec me = reinterpret_cast<const CellRec*>(recs)[t];
         idx = me.idx;
@@ -185,17 +201,13 @@ __global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sim
         const float rel = (float)sim_field(m, HC_F_REL_MAX_ADH)[c];
         const float cut = (float)m.max_dist, vox = (float)m.mvoxel;
         float fx = 0.f, fy = 0.f, fz = 0.f, pr = 0.f;
-        if constexpr (LIST) {
-            force_agent32x(rec, offsets, list, t, m.bin_off, vi, vj, vk, m.mnx, m.mny, m.mnz, me.x, me.y, me.z, vox,
-                           rmax[0], me.r, me.rep, me.adh, rel, cut * cut, fx, fy, fz, pr);
-        } else {
-#pragma unroll 1
-            for (int q = 0; q < 9; q++) {
-                const RowRange rr = neighbor_row(offsets, m.bin_off, vi, vj, vk, m.mnx, m.mny, m.mnz, q);
-                force_range32(rec, rr.beg, rr.end, t, rr.mid0, rr.mid1, me.x, me.y, me.z, (float)rr.dj * vox,
-                              (float)rr.dk * vox, vox, me.r, me.rep, me.adh, rel, cut * cut, fx, fy, fz, pr);
-            }
-        }
+        if (LIST && member_markx(m) && m.mvoxel >= 0.5)
+            force_agent32x(rec, xoff, list, t, m.bin_off, vi, vj, vk, m.mnx, m.mny, m.mnz, me.xg, me.x, me.y, me.z,
+                           vox, xslab_scale(m.mvoxel), (float)(1.0 / m.mvoxel), rmax[0], me.r, me.rep, me.adh, rel,
+                           cut * cut, fx, fy, fz, pr);
+        else
+            force_rows32(rec, offsets, t, m.bin_off, vi, vj, vk, m.mnx, m.mny, m.mnz, me.x, me.y, me.z, vox, me.r,
+                         me.rep, me.adh, rel, cut * cut, fx, fy, fz, pr);
         ofx = fx; ofy = fy; ofz = fz; opr = pr;
     } else {
         const CellRec me = reinterpret_cast<const CellRec*>(recs)[t];
@@ -887,7 +899,7 @@ public:
 private:
     DeviceBuffer q_hash_, q_off_, q_idx_;     // member neighbour-query workspace
     struct Workspace {
-        DeviceBuffer sims, mots, bsims, tiles, work[NLIST], agent_sim, kglob, slot, tmp, sorted, pos_of, recs, pos32, counts,
+        DeviceBuffer sims, mots, bsims, tiles, work[NLIST], agent_sim, kglob, slot, tmp, sorted, pos_of, recs, pos32, xoff, xslab, counts,
             offsets, partial, rpart, rmax;
         DeviceBuffer pcounts, poffsets, pkv, pslot, ptmp, psorted, pflag, pgsim, pgvox, pgbegin, pcoef, phdr, ppartial;
         uint32_t ntot = 0, nbins = 0, vtot = 0, ntiles = 0, nwork[NLIST] = {}, ns_max = 0, nmax_all = 0;
@@ -1033,6 +1045,10 @@ private:
         // mark passes load whole steps past the end of a range
         alloc(w.recs, (N + hc::MARK32_STEP) * (mixed_ ? sizeof(hc::CellRec32) : sizeof(hc::CellRec)));
         if (!mixed_) alloc(w.pos32, (N + hc::MARKX_STEP) * sizeof(float4));
+        if (mixed_) {
+            alloc(w.xoff, ((size_t)w.nbins + 1) * hc::XSLABS * 4);
+            alloc(w.xslab, N);
+        }
         alloc(w.rpart, (size_t)std::max(1u, w.ntiles) * 4);
         alloc(w.rmax, 256);
         alloc(w.counts, ((size_t)w.nbins + 1) * 4); alloc(w.offsets, ((size_t)w.nbins + 2) * 4);
@@ -1188,39 +1204,46 @@ private:
         const uint32_t* asim = (const uint32_t*)hc_device_ptr(w.agent_sim.get());
         void* recs = hc_device_ptr(w.recs.get());
         float4* pos32 = mixed_ ? nullptr : (float4*)hc_device_ptr(w.pos32.get());
+        uint32_t* xoff = mixed_ ? (uint32_t*)hc_device_ptr(w.xoff.get()) : nullptr;
+        uint8_t* xslab = mixed_ ? (uint8_t*)hc_device_ptr(w.xslab.get()) : nullptr;
         if (!sims || !tiles || !counts || !offsets || !kglob || !slot || !tmp || !sorted || !pos_of || !asim ||
-            !recs || (!mixed_ && !pos32) || !hc_device_ptr(w.partial.get()))
+            !recs || (!mixed_ && !pos32) || (mixed_ && (!xoff || !xslab)) || !hc_device_ptr(w.partial.get()))
             throw std::runtime_error("ensemble: mechanics workspace not allocated");
         {
             hc::KernelTimer tm(ctx_, HC_K_HASH);
             check(ctx_, hc::fill_u32(ctx_, counts, w.nbins, 0), "fill");
-            hipLaunchKernelGGL(hc::k_b_mech_keys, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, counts, kglob, slot);
+            hipLaunchKernelGGL(hc::k_b_mech_keys, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, counts, kglob, slot,
+                               xslab);
             check(ctx_, hc::check_launch(ctx_, "k_b_mech_keys"), "keys");
             check(ctx_, hc::scan_exclusive(ctx_, counts, offsets, w.nbins, (uint32_t*)hc_device_ptr(w.partial.get())), "scan");
             check(ctx_, hc::launch_scatter(ctx_, kglob, slot, offsets, tmp, w.ntot), "scatter");
-            check(ctx_, hc::launch_bin_sort(ctx_, offsets, tmp, sorted, pos_of, w.nbins), "bin sort");
+            if (mixed_)
+                check(ctx_, hc::launch_bin_sort32(ctx_, offsets, tmp, xslab, sorted, pos_of, xoff, w.nbins), "bin sort");
+            else
+                check(ctx_, hc::launch_bin_sort(ctx_, offsets, tmp, sorted, pos_of, w.nbins), "bin sort");
             float* rpart = (float*)hc_device_ptr(w.rpart.get());
             if (mixed_)
                 hipLaunchKernelGGL(hc::k_b_gather<true>, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, kglob, offsets, tmp,
-                                   sorted, pos_of, recs, nullptr, rpart);
+                                   sorted, pos_of, xslab, recs, nullptr, rpart);
             else
                 hipLaunchKernelGGL(hc::k_b_gather<false>, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, kglob, offsets,
-                                   tmp, sorted, pos_of, recs, pos32, rpart);
+                                   tmp, sorted, pos_of, nullptr, recs, pos32, rpart);
             check(ctx_, hc::check_launch(ctx_, "k_b_gather"), "gather");
             check(ctx_, hc::launch_rmax(ctx_, rpart, w.ntiles, (float*)hc_device_ptr(w.rmax.get())), "largest radius");
         }
         {
             hc::KernelTimer tm(ctx_, HC_K_FORCES);
             const dim3 g(w.ntiles);
-            // positions in the records' list entries take 27 bits (force_agent32x)
+            // positions in the records' list entries take 27 bits and slab
+            // indices 32 (force_agent32x)
             const float* rmax = (const float*)hc_device_ptr(w.rmax.get());
+            const bool list = w.ntot < (1u << hc::MARK32_POS_BITS) && w.nbins < (1u << 30);
             if (mixed_)
-                hipLaunchKernelGGL((w.ntot < (1u << hc::MARK32_POS_BITS) ? hc::k_b_forces<true, true>
-                                                                         : hc::k_b_forces<true, false>),
-                                   g, dim3(256), 0, st, sims, tiles, recs, nullptr, offsets, rmax);
+                hipLaunchKernelGGL((list ? hc::k_b_forces<true, true> : hc::k_b_forces<true, false>), g, dim3(256), 0,
+                                   st, sims, tiles, recs, nullptr, offsets, xoff, kglob, rmax);
             else
                 hipLaunchKernelGGL(hc::k_b_forces<false>, g, dim3(256), 0, st, sims, tiles, recs, pos32, offsets,
-                                   rmax);
+                                   nullptr, nullptr, rmax);
             check(ctx_, hc::check_launch(ctx_, "k_b_forces"), "forces");
         }
         if (any_motility_) {

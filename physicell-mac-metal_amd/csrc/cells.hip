@@ -145,13 +145,15 @@ __global__ __launch_bounds__(256) void k_mech_keys(const double* __restrict__ x,
                                                    const double* __restrict__ z, uint32_t* __restrict__ key_out,
                                                    uint32_t* __restrict__ counts, uint32_t* __restrict__ slot,
                                                    uint32_t n, double x_min, double y_min, double z_min,
-                                                   double voxel, uint32_t mnx, uint32_t mny, uint32_t mnz) {
+                                                   double voxel, uint32_t mnx, uint32_t mny, uint32_t mnz,
+                                                   uint8_t* __restrict__ xslab) {
     const uint32_t c = blockIdx.x * 256 + threadIdx.x;
     const bool valid = c < n;
     uint32_t key = 0;
     if (valid) {
         // shaders/mechanics.metal:61-72
-        int vi = (int)((x[c] - x_min) / voxel);
+        const double px = x[c];
+        int vi = (int)((px - x_min) / voxel);
         int vj = (int)((y[c] - y_min) / voxel);
         int vk = (int)((z[c] - z_min) / voxel);
         vi = vi < 0 ? 0 : (vi > (int)mnx - 1 ? (int)mnx - 1 : vi);
@@ -159,6 +161,8 @@ __global__ __launch_bounds__(256) void k_mech_keys(const double* __restrict__ x,
         vk = vk < 0 ? 0 : (vk > (int)mnz - 1 ? (int)mnz - 1 : vk);
         key = (uint32_t)vk * mny * mnx + (uint32_t)vj * mnx + (uint32_t)vi;
         key_out[c] = key;
+        // FP32 records: the x slab of the record's bin-relative x
+        if (xslab) xslab[c] = (uint8_t)xslab_of(bin_rel32(px, x_min, (uint32_t)vi, voxel), xslab_scale(voxel));
     }
     const uint32_t s = ballot_bin(valid, key, counts);
     if (valid) slot[c] = s;
@@ -189,6 +193,17 @@ __global__ __launch_bounds__(256) void k_bin_sort(const uint32_t* __restrict__ o
                                                   uint32_t* __restrict__ pos_of, uint32_t nbins) {
     const uint32_t b = blockIdx.x * 256 + threadIdx.x;
     if (b < nbins) bin_sort_one(offsets, tmp, sorted, pos_of, b);
+}
+
+// FP32 records: the ascending lists, record positions (rpos) and slab starts
+// (xoff) of bins 0 .. nbins-1, and the entries past the last bin
+__global__ __launch_bounds__(256) void k_bin_sort32(const uint32_t* __restrict__ offsets,
+                                                    const uint32_t* __restrict__ tmp,
+                                                    const uint8_t* __restrict__ xslab, uint32_t* __restrict__ sorted,
+                                                    uint32_t* __restrict__ rpos, uint32_t* __restrict__ xoff,
+                                                    uint32_t nbins) {
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b <= nbins) bin_sort_slabs(offsets, tmp, xslab, sorted, rpos, xoff, b, nbins);
 }
 
 __global__ __launch_bounds__(256) void k_stable_gather(CellPtrs cp, const uint32_t* __restrict__ key,
@@ -513,6 +528,9 @@ HashLayout hash_layout(uint32_t n_mvox, uint32_t max_cells) {
     L.rpart = o;   o = align256(o + (size_t)blocks_for(max_cells ? max_cells : 1, 256) * 4);
     L.rmax = o;    o = align256(o + 4);
     L.partial = o; o = align256(o + scan_partial_bytes((uint64_t)n_mvox + 1));
+    // FP32 records: slab starts per bin (and past the last bin), slab per agent
+    L.xoff = o;    o = align256(o + ((size_t)n_mvox + 1) * XSLABS * 4);
+    L.xslab = o;   o = align256(o + (size_t)max_cells);
     L.total = o;
     return L;
 }
@@ -589,16 +607,16 @@ __global__ __launch_bounds__(256) void k_stable_sort(const uint32_t* __restrict_
 // ---------------------------------------------------------------------------
 // mixed form: 32-byte records, FP32 force arithmetic
 // ---------------------------------------------------------------------------
-// Positions are stored relative to the lower corner of the agent's own bin,
-// so a neighbour difference is (own offset - other offset) - bin delta * 30:
-// values below ~60 um in FP32 keep ~4e-6 um of resolution wherever the
-// domain sits.  Own relative adhesion distance is read from the SoA.
-
+// Records (CellRec32) in each bin by x slab, then agent index; positions
+// from k_bin_sort32 (rpos, in pos_of) or, in bins above BIN_SORT_CAP, by
+// slab_rank here, which also places those bins' ascending lists.  Own
+// relative adhesion distance is read from the SoA.
 __global__ __launch_bounds__(256) void k_stable_gather32(CellPtrs cp, const uint32_t* __restrict__ key,
                                                          const uint32_t* __restrict__ offsets,
                                                          const uint32_t* __restrict__ tmp,
                                                          uint32_t* __restrict__ sorted,
                                                          const uint32_t* __restrict__ pos_of,
+                                                         const uint8_t* __restrict__ xslab,
                                                          CellRec32* __restrict__ rec, uint32_t n, double x_min,
                                                          double y_min, double z_min, double voxel, uint32_t mnx,
                                                          uint32_t mny, float* __restrict__ rpart) {
@@ -610,56 +628,56 @@ __global__ __launch_bounds__(256) void k_stable_gather32(CellPtrs cp, const uint
     const uint32_t beg = offsets[b], end = offsets[b + 1];
     uint32_t pos;
     if (end - beg <= BIN_SORT_CAP) {
-        pos = pos_of[c];                                  // placed by k_bin_sort
+        pos = pos_of[c];                                  // placed by k_bin_sort32
     } else {
-        pos = beg + stable_rank(tmp, beg, end, c);
-        sorted[pos] = c;
+        sorted[beg + stable_rank(tmp, beg, end, c)] = c;
+        pos = beg + slab_rank(tmp, xslab, beg, end, c, xslab[c]);
     }
     const uint32_t bi = b % mnx, bj = (b / mnx) % mny, bk = b / (mnx * mny);
+    const double px = cp.f[HC_F_POS_X][c];
     CellRec32 r;
-    r.x = (float)(cp.f[HC_F_POS_X][c] - (x_min + bi * voxel));
-    r.y = (float)(cp.f[HC_F_POS_Y][c] - (y_min + bj * voxel));
-    r.z = (float)(cp.f[HC_F_POS_Z][c] - (z_min + bk * voxel));
+    r.xg = (float)(px - x_min);
+    r.y = bin_rel32(cp.f[HC_F_POS_Y][c], y_min, bj, voxel);
+    r.z = bin_rel32(cp.f[HC_F_POS_Z][c], z_min, bk, voxel);
+    r.x = bin_rel32(px, x_min, bi, voxel);
     r.r = (float)cp.f[HC_F_RADIUS][c];
     r.rep = (float)cp.f[HC_F_REPULSION][c];
     r.adh = (float)cp.f[HC_F_ADHESION][c];
     r.idx = c;
-    r.key = b;
     rec[pos] = r;
 }
 
-// LIST: the mark pass with a list of marked candidates (force_agent32x; rmax
-// is the largest FP32 radius); else the chunked pass (force_range32), which
-// populations of 2^27 agents or more take.  Two kernels, so that each gets
-// the registers its own path needs.
+// LIST: the mark pass over x-trimmed rows with a list of marked candidates
+// (force_agent32x; rmax is the largest FP32 radius); else the chunked pass
+// (force_rows32), which populations of 2^27 agents or more, grids wider than
+// MARKX_MAX_EXTENT in x and forces_fp32 = 2 take.  Two kernels, so that each
+// gets the registers its own path needs.
 template <bool LIST>
 __global__ __launch_bounds__(256) void k_forces32(const CellRec32* __restrict__ rec, const uint32_t* __restrict__ offsets,
+                                                  const uint32_t* __restrict__ xoff,
+                                                  const uint32_t* __restrict__ mkey,
                                                   const double* __restrict__ rel_adh, double* __restrict__ vx,
                                                   double* __restrict__ vy, double* __restrict__ vz,
                                                   double* __restrict__ pressure, uint32_t n, uint32_t mnx, uint32_t mny,
-                                                  uint32_t mnz, float max_dist, float voxel,
+                                                  uint32_t mnz, float max_dist, double voxel,
                                                   const float* __restrict__ rmax) {
     __shared__ uint32_t marks[LIST ? 4 * MARKX_CAP * 64 : 1];
     const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (t >= n) return;
     const CellRec32 me = rec[t];
-    const float radius_i = me.r, rep_i = me.rep, adh_i = me.adh;
+    const uint32_t key = mkey[me.idx];
     const float rel_max_adh = (float)rel_adh[me.idx];
     float fx = 0.f, fy = 0.f, fz = 0.f, local_pressure = 0.f;
-    const uint32_t vi = me.key % mnx, vj = (me.key / mnx) % mny, vk = me.key / (mnx * mny);
-    const float cut2 = max_dist * max_dist;
+    const uint32_t vi = key % mnx, vj = (key / mnx) % mny, vk = key / (mnx * mny);
+    const float cut2 = max_dist * max_dist, vox = (float)voxel;
     if (LIST) {
         uint32_t* list = marks + (threadIdx.x >> 6) * (MARKX_CAP * 64) + (threadIdx.x & 63);
-        force_agent32x(rec, offsets, list, t, 0u, vi, vj, vk, mnx, mny, mnz, me.x, me.y, me.z, voxel, rmax[0],
-                       radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz, local_pressure);
+        force_agent32x(rec, xoff, list, t, 0u, vi, vj, vk, mnx, mny, mnz, me.xg, me.x, me.y, me.z, vox,
+                       xslab_scale(voxel), (float)(1.0 / voxel), rmax[0], me.r, me.rep, me.adh, rel_max_adh, cut2,
+                       fx, fy, fz, local_pressure);
     } else {
-#pragma unroll 1
-        for (int q = 0; q < 9; q++) {
-            const RowRange rr = neighbor_row(offsets, 0u, vi, vj, vk, mnx, mny, mnz, q);
-            force_range32(rec, rr.beg, rr.end, t, rr.mid0, rr.mid1, me.x, me.y, me.z, (float)rr.dj * voxel,
-                          (float)rr.dk * voxel, voxel, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz,
-                          local_pressure);
-        }
+        force_rows32(rec, offsets, t, 0u, vi, vj, vk, mnx, mny, mnz, me.x, me.y, me.z, vox, me.r, me.rep, me.adh,
+                     rel_max_adh, cut2, fx, fy, fz, local_pressure);
     }
     const uint32_t c = me.idx;
     vx[c] = fx;
@@ -751,7 +769,7 @@ int probe_kernels_cells2(hc_ctx* ctx);
 int probe_kernels_cells2(hc_ctx* ctx) {
     hipFuncAttributes attr;
     const void* fns[] = {(const void*)k_stable_sort, (const void*)k_stable_gather32, (const void*)k_forces32<true>, (const void*)k_forces32<false>,
-                         (const void*)k_bin_sort,
+                         (const void*)k_bin_sort, (const void*)k_bin_sort32,
                          (const void*)k_totals<double>, (const void*)k_totals<float>, (const void*)k_totals_final};
     for (const void* f : fns) {
         hipError_t e = hipFuncGetAttributes(&attr, f);
@@ -771,6 +789,13 @@ int launch_bin_sort(hc_ctx* ctx, const uint32_t* offsets, const uint32_t* tmp, u
     hipLaunchKernelGGL(k_bin_sort, dim3(blocks_for(nbins, 256)), dim3(256), 0, ctx->stream, offsets, tmp, sorted,
                        pos_of, nbins);
     return check_launch(ctx, "k_bin_sort");
+}
+
+int launch_bin_sort32(hc_ctx* ctx, const uint32_t* offsets, const uint32_t* tmp, const uint8_t* xslab,
+                      uint32_t* sorted, uint32_t* rpos, uint32_t* xoff, uint32_t nbins) {
+    hipLaunchKernelGGL(k_bin_sort32, dim3(blocks_for(nbins + 1, 256)), dim3(256), 0, ctx->stream, offsets, tmp, xslab,
+                       sorted, rpos, xoff, nbins);
+    return check_launch(ctx, "k_bin_sort32");
 }
 
 int launch_scatter(hc_ctx* ctx, const uint32_t* key, const uint32_t* slot, const uint32_t* offsets, uint32_t* tmp,
@@ -799,6 +824,8 @@ struct HashPtrs {
     CellRec* rec;
     float4* pos32;
     float *rpart, *rmax;
+    uint32_t* xoff;
+    uint8_t* xslab;
 };
 
 static HashPtrs hash_ptrs(hc_buf* hash, uint32_t n_mvox, uint32_t max_cells) {
@@ -814,6 +841,8 @@ static HashPtrs hash_ptrs(hc_buf* hash, uint32_t n_mvox, uint32_t max_cells) {
     h.rec = (CellRec*)(P + L.rec);
     h.pos_of = (uint32_t*)(P + L.pos_of);
     h.pos32 = (float4*)(P + L.pos32);
+    h.xoff = (uint32_t*)(P + L.xoff);
+    h.xslab = (uint8_t*)(P + L.xslab);
     h.rpart = (float*)(P + L.rpart);
     h.rmax = (float*)(P + L.rmax);
     return h;
@@ -933,6 +962,15 @@ static int markx_ok(const hc_mech* mech) {
     return mark32_ok(mech) && (double)mech->mech_grid_nx * mech->mechanics_voxel_size <= MARKX_MAX_EXTENT ? 1 : 0;
 }
 
+// FP32 forces take the x-trimmed mark pass with a list (force_agent32x) when
+// the mark's FP32 x relative to x_min keeps its error bound (markx_ok), the
+// record positions fit the list entries, and forces_fp32 is not 2 (the
+// chunked pass on request)
+static bool list32_ok(const hc_mech* mech, uint32_t n, uint32_t n_mvox) {
+    return mech->forces_fp32 == 1 && markx_ok(mech) && mech->mechanics_voxel_size >= 0.5 &&
+           n < (1u << MARK32_POS_BITS) && n_mvox < (1u << 30);
+}
+
 static int check_hash(hc_ctx* ctx, hc_buf* hash, uint32_t n_mvox, uint32_t max_cells, const char* who) {
     if (!hash || n_mvox == 0) return fail(ctx, HC_ERR_ARG, std::string(who) + ": bad hash argument");
     if (hc_buf_size(hash) < hc_hash_bytes(n_mvox, max_cells))
@@ -956,24 +994,27 @@ int hc_build_hash(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech,
         hipLaunchKernelGGL(k_mech_keys, dim3(nb), dim3(256), 0, ctx->stream, cp.f[HC_F_POS_X], cp.f[HC_F_POS_Y],
                            cp.f[HC_F_POS_Z], cp.u[HC_F_MECH_VOXEL - HC_NUM_F64_FIELDS], h.counts, h.slot, n,
                            mgrid->x_min, mgrid->y_min, mgrid->z_min, mech->mechanics_voxel_size,
-                           mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz);
+                           mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz,
+                           mech->forces_fp32 ? h.xslab : nullptr);
         if ((r = check_launch(ctx, "k_mech_keys"))) return r;
     }
     if ((r = scan_exclusive(ctx, h.counts, h.offsets, n_mvox, h.partial))) return r;
     if (n) {
         const uint32_t* key = cp.u[HC_F_MECH_VOXEL - HC_NUM_F64_FIELDS];
         hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, ctx->stream, key, h.slot, h.offsets, h.tmp, n);
-        if ((r = launch_bin_sort(ctx, h.offsets, h.tmp, h.sorted, h.pos_of, n_mvox))) return r;
-        if (mech->forces_fp32)
+        if (mech->forces_fp32) {
+            if ((r = launch_bin_sort32(ctx, h.offsets, h.tmp, h.xslab, h.sorted, h.pos_of, h.xoff, n_mvox))) return r;
             hipLaunchKernelGGL(k_stable_gather32, dim3(nb), dim3(256), 0, ctx->stream, cp, key, h.offsets, h.tmp,
-                               h.sorted, h.pos_of, reinterpret_cast<CellRec32*>(h.rec), n, mgrid->x_min, mgrid->y_min,
-                               mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny,
-                               h.rpart);
-        else
+                               h.sorted, h.pos_of, h.xslab, reinterpret_cast<CellRec32*>(h.rec), n, mgrid->x_min,
+                               mgrid->y_min, mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx,
+                               mech->mech_grid_ny, h.rpart);
+        } else {
+            if ((r = launch_bin_sort(ctx, h.offsets, h.tmp, h.sorted, h.pos_of, n_mvox))) return r;
             hipLaunchKernelGGL(k_stable_gather, dim3(nb), dim3(256), 0, ctx->stream, cp, key, h.offsets, h.tmp,
                                h.sorted, h.pos_of, h.rec, mark32_ok(mech) ? h.pos32 : nullptr, n, mgrid->x_min, mgrid->y_min,
                                mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny,
                                markx_ok(mech), markx_ok(mech) ? h.rpart : nullptr);
+        }
         if (mech->forces_fp32 || markx_ok(mech))
             hipLaunchKernelGGL(k_rmax, dim3(1), dim3(256), 0, ctx->stream, h.rpart, nb, h.rmax);
         if ((r = check_launch(ctx, "hash scatter"))) return r;
@@ -991,12 +1032,13 @@ int hc_forces(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech, uin
     CellPtrs cp = cell_ptrs(cells->dptr, max_cells);
     KernelTimer tm(ctx, HC_K_FORCES);
     if (mech->forces_fp32)
-        hipLaunchKernelGGL(n < (1u << MARK32_POS_BITS) ? k_forces32<true> : k_forces32<false>,
+        hipLaunchKernelGGL(list32_ok(mech, n, n_mvox) ? k_forces32<true> : k_forces32<false>,
                            dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream,
-                           reinterpret_cast<const CellRec32*>(h.rec), h.offsets, cp.f[HC_F_REL_MAX_ADH],
+                           reinterpret_cast<const CellRec32*>(h.rec), h.offsets, h.xoff,
+                           cp.u[HC_F_MECH_VOXEL - HC_NUM_F64_FIELDS], cp.f[HC_F_REL_MAX_ADH],
                            cp.f[HC_F_VEL_X], cp.f[HC_F_VEL_Y], cp.f[HC_F_VEL_Z], cp.f[HC_F_PRESSURE], n,
                            mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz,
-                           (float)mech->max_interaction_distance, (float)mech->mechanics_voxel_size, h.rmax);
+                           (float)mech->max_interaction_distance, mech->mechanics_voxel_size, h.rmax);
     else
         hipLaunchKernelGGL(k_forces, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, h.rec,
                            mark32_ok(mech) ? h.pos32 : nullptr, h.offsets, cp.f[HC_F_VEL_X], cp.f[HC_F_VEL_Y],

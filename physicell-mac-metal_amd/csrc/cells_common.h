@@ -26,11 +26,33 @@ __host__ __device__ inline CellPtrs cell_ptrs(void* base, uint32_t max_cells) {
     return p;
 }
 
+// 32-byte hash-sorted record (FP32 forces, the mixed form): xg = x relative
+// to the grid's x_min (the mark pass's coordinate), y and z relative to the
+// lower corner of the agent's own bin, x relative to the same corner (the
+// force law's coordinates: values below ~60 um keep ~4e-6 um of resolution
+// wherever the domain sits), radius, repulsion, adhesion, agent index.
 struct __attribute__((aligned(16))) CellRec32 {
-    float x, y, z, r, rep, adh;
-    uint32_t idx, key;
+    float xg, y, z, x, r, rep, adh;
+    uint32_t idx;
 };
 static_assert(sizeof(CellRec32) == 32, "mixed record must be 32 bytes");
+
+// FP32 records of a bin are ordered by x slab, then by agent index: XSLABS
+// slabs of voxel / XSLABS along x, by the record's bin-relative x.
+// xoff[XSLABS * b + s] is the position of the first record of slab s of bin
+// b, so xoff[XSLABS * b] = offsets[b], and the XSLABS entries past the last
+// bin hold n.  The members of a row of bins whose x lies in an interval are
+// then one contiguous range of records, found with two loads.
+constexpr uint32_t XSLABS = 4;
+__host__ __device__ inline float xslab_scale(double voxel) { return (float)((double)XSLABS / voxel); }
+// bin-relative FP32 coordinate, the same expression wherever it is formed
+__host__ __device__ inline float bin_rel32(double p, double origin, uint32_t bin, double voxel) {
+    return (float)(p - (origin + bin * voxel));
+}
+__device__ inline uint32_t xslab_of(float xb, float scale) {
+    const float s = floorf(xb * scale);
+    return !(s > 0.0f) ? 0u : (s >= (float)(XSLABS - 1) ? XSLABS - 1 : (uint32_t)s);
+}
 
 // ---------------------------------------------------------------------------
 // wavefront primitives
@@ -185,6 +207,9 @@ int launch_rmax(hc_ctx* ctx, const float* rpart, uint32_t nb, float* rmax);
 // k_bin_sort over bins 0 .. nbins-1 (bin_sort_one)
 int launch_bin_sort(hc_ctx* ctx, const uint32_t* offsets, const uint32_t* tmp, uint32_t* sorted, uint32_t* pos_of,
                     uint32_t nbins);
+// k_bin_sort32 (bin_sort_slabs) over bins 0 .. nbins-1 and the entries past the last
+int launch_bin_sort32(hc_ctx* ctx, const uint32_t* offsets, const uint32_t* tmp, const uint8_t* xslab,
+                      uint32_t* sorted, uint32_t* rpos, uint32_t* xoff, uint32_t nbins);
 int launch_scatter(hc_ctx* ctx, const uint32_t* key, const uint32_t* slot, const uint32_t* offsets, uint32_t* tmp,
                    uint32_t n);
 int launch_stable_sort(hc_ctx* ctx, const uint32_t* key, const uint32_t* offsets, const uint32_t* tmp,
@@ -495,173 +520,274 @@ __device__ __attribute__((always_inline)) inline void force_agent64x(
     force_list64(rec, list, cnt, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz, local_pressure);
 }
 
-// FP32 forces of one agent with a mark pass and a per-lane list of the
-// marked candidates in LDS (the mixed form; force_range32's result bit for
-// bit).
-//
-// Mark: one threshold per agent, min(L^2 (1 + 2^-16), cut^2) + MARK32_SLACK
-// with L = (r_i + rmax) * max(1, rel) in FP32, rmax the largest FP32 radius of
-// any agent.  FP32 addition and multiplication are monotonic, so L is no
-// smaller than any pair's (r_i + r_j) * max(1, rel) and the threshold no
-// smaller than any pair's bound in force_range32; the mark's d^2 (by fma,
-// with the column offset folded into the agent's coordinate) is within
-// 1e-3 um^2 of force_range32's for pairs within 32 um, far below the slack.
-// Every pair force_range32 keeps is therefore marked.  A marked candidate is
-// listed with its bin offset code (di + 1) + 3 (dj + 1) + 9 (dk + 1) in the
-// top 5 bits of the entry, and the law walks the list in visit order (rows
-// q = 0..8, ascending sorted position), re-testing each pair with
-// force_range32's exact predicates and expressions, so the pairs that
-// contribute, their order and every bit of the sums are force_range32's.
-// Sorted positions must stay below 2^27 (callers check).
-constexpr float MARK32_SLACK = 0.01f;
+// ---------------------------------------------------------------------------
+// FP32 forces (mixed form)
+// ---------------------------------------------------------------------------
+// record position bits of a mark-list entry (the row code above them); mark
+// step; margin of the walked x interval beyond the mark's reach (um)
 constexpr uint32_t MARK32_POS_BITS = 27;
 constexpr uint32_t MARK32_STEP = 8;           // candidates per step (their loads in flight together)
-
-__device__ __attribute__((always_inline)) inline void force_list32(
-    const CellRec32* __restrict__ rec, const uint32_t* __restrict__ list, uint32_t cnt, uint32_t t, float mx,
-    float my, float mz, float voxel, float radius_i, float rep_i, float adh_i, float rel_max_adh, float cut2,
-    float& fx, float& fy, float& fz, float& local_pressure) {
-    const float relf = rel_max_adh > 1.0f ? rel_max_adh : 1.0f;
+constexpr float XTRIM_MARGIN = 0.05f;
+// The force law of one FP32 pair (shaders/mechanics.metal:164-204 in FP32):
+// o = the candidate's record, (ox, oy, oz) its bin corner relative to the
+// agent's, (mx, my, mz) the agent's bin-relative position.  Exact
+// predicates first; the hardware square root and reciprocal (about 1 ulp)
+// instead of the correctly rounded sequences.
+__device__ __attribute__((always_inline)) inline void force_pair32(
+    const float4 a, const float4 b, float ox, float oy, float oz, float mx, float my, float mz, float radius_i,
+    float rep_i, float adh_i, float rel_max_adh, float relf, float cut2, float& fx, float& fy, float& fz,
+    float& local_pressure) {
     constexpr float margin = 1.0f + 0x1p-16f;
-    for (uint32_t k = 0; k < cnt; k++) {
-        const uint32_t e = list[k * 64];
-        const uint32_t u = e & ((1u << MARK32_POS_BITS) - 1u), code = e >> MARK32_POS_BITS;
-        if (u == t) continue;
-        const uint32_t cj = code / 3u;
-        const float ox = (float)((int)(code - 3u * cj) - 1) * voxel;
-        const float oy = (float)((int)(cj % 3u) - 1) * voxel, oz = (float)((int)(cj / 3u) - 1) * voxel;
-        const float4 o = *reinterpret_cast<const float4*>(&rec[u].x);
-        const float ddx = mx - o.x - ox, ddy = my - o.y - oy, ddz = mz - o.z - oz;
-        const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-        const float L = (radius_i + o.w) * relf;
-        if (d2 >= cut2 || d2 < 1e-12f || !(d2 < L * L * margin)) continue;
-        // the hardware square root and reciprocal (about 1 ulp), as force_range32
-        const float dist = __builtin_amdgcn_sqrtf(d2);
-        const float inv_dist = __builtin_amdgcn_rcpf(dist);
-        const float R_sum = radius_i + o.w;
-        float force_mag = 0.f;
-        if (dist < R_sum) {
-            const float overlap = 1.0f - dist * __builtin_amdgcn_rcpf(R_sum);
-            force_mag += overlap * overlap * __builtin_amdgcn_sqrtf(rep_i * rec[u].rep);
-            local_pressure += overlap * overlap * (1.0f / 0.027288820670331f);
-        }
-        const float adhesion_distance = rel_max_adh * R_sum;
-        if (dist < adhesion_distance) {
-            const float t2 = 1.0f - dist * __builtin_amdgcn_rcpf(adhesion_distance);
-            force_mag -= t2 * t2 * __builtin_amdgcn_sqrtf(adh_i * rec[u].adh);
-        }
-        fx += force_mag * (ddx * inv_dist);
-        fy += force_mag * (ddy * inv_dist);
-        fz += force_mag * (ddz * inv_dist);
+    // a = (xg, y, z, x), b = (r, rep, adh, idx)
+    const float ddx = mx - a.w - ox, ddy = my - a.y - oy, ddz = mz - a.z - oz;
+    const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+    const float L = (radius_i + b.x) * relf;
+    if (d2 >= cut2 || d2 < 1e-12f || !(d2 < L * L * margin)) return;
+    const float dist = __builtin_amdgcn_sqrtf(d2);
+    const float inv_dist = __builtin_amdgcn_rcpf(dist);
+    const float R_sum = radius_i + b.x;
+    float force_mag = 0.f;
+    if (dist < R_sum) {
+        const float overlap = 1.0f - dist * __builtin_amdgcn_rcpf(R_sum);
+        force_mag += overlap * overlap * __builtin_amdgcn_sqrtf(rep_i * b.y);
+        local_pressure += overlap * overlap * (1.0f / 0.027288820670331f);
     }
+    const float adhesion_distance = rel_max_adh * R_sum;
+    if (dist < adhesion_distance) {
+        const float t2 = 1.0f - dist * __builtin_amdgcn_rcpf(adhesion_distance);
+        force_mag -= t2 * t2 * __builtin_amdgcn_sqrtf(adh_i * b.z);
+    }
+    fx += force_mag * (ddx * inv_dist);
+    fy += force_mag * (ddy * inv_dist);
+    fz += force_mag * (ddz * inv_dist);
 }
 
-// mx, my, mz: the agent's position relative to its own bin corner; t its
-// sorted position; bin0 the key of the simulation's bin 0; rmax as above;
-// list: this lane's first entry in LDS (entry k at list[k * 64]).
-__device__ __attribute__((always_inline)) inline void force_agent32x(
-    const CellRec32* __restrict__ rec, const uint32_t* __restrict__ offsets, uint32_t* __restrict__ list,
-    uint32_t t, uint32_t bin0, uint32_t vi, uint32_t vj, uint32_t vk, uint32_t mnx, uint32_t mny, uint32_t mnz,
-    float mx, float my, float mz, float voxel, float rmax, float radius_i, float rep_i, float adh_i,
-    float rel_max_adh, float cut2, float& fx, float& fy, float& fz, float& local_pressure) {
-    const float relf = rel_max_adh > 1.0f ? rel_max_adh : 1.0f;
-    const float L = (radius_i + rmax) * relf;
-    const float lim = fminf(L * L * (1.0f + 0x1p-16f), cut2) + MARK32_SLACK;
-    uint32_t cnt = 0;
-#pragma unroll 1
-    for (int q = 0; q < 9; q++) {
-        const RowRange rr = neighbor_row(offsets, bin0, vi, vj, vk, mnx, mny, mnz, q);
-        const float myr = my - (float)rr.dj * voxel, mzr = mz - (float)rr.dk * voxel;
-        const float mxm = mx + voxel, mxp = mx - voxel;     // di = -1 / +1 folded into the agent's x
-        // entry code of the row's di = -1 column; +1 per column to the right
-        const uint32_t code0 = ((uint32_t)(3 * (rr.dj + 1) + 9 * (rr.dk + 1))) << MARK32_POS_BITS;
-        for (uint32_t u0 = rr.beg; u0 < rr.end; u0 += MARK32_STEP) {
-            float4 o[MARK32_STEP];
-#pragma unroll
-            for (uint32_t j = 0; j < MARK32_STEP; j++) o[j] = *reinterpret_cast<const float4*>(&rec[u0 + j].x);
-#pragma unroll
-            for (uint32_t j = 0; j < MARK32_STEP; j++) {
-                const uint32_t u = u0 + j;
-                const bool a = u < rr.mid0, b = u < rr.mid1;
-                const float ddx = (a ? mxm : (b ? mx : mxp)) - o[j].x, ddy = myr - o[j].y, ddz = mzr - o[j].z;
-                const float d2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(ddy, ddy, ddx * ddx));
-                list[cnt * 64] = (u | code0) + (a ? 0u : (b ? 1u : 2u) << MARK32_POS_BITS);
-                cnt += (u < rr.end && d2 < lim) ? 1u : 0u;
-            }
-            if (cnt > MARKX_CAP - MARK32_STEP) {
-                force_list32(rec, list, cnt, t, mx, my, mz, voxel, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy,
-                             fz, local_pressure);
-                cnt = 0;
-            }
-        }
-    }
-    force_list32(rec, list, cnt, t, mx, my, mz, voxel, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz,
-                 local_pressure);
-}
-
-// FP32 form: positions are relative to each agent's own bin corner.  A
-// member's bin corner relative to the agent's is (di, dj, dk) * voxel: dj, dk
-// are the row's (oy, oz), and di = -1, 0, +1 for members before mid0 (the
-// first member of the agent's own column), before mid1 (the first of the
-// next column), and after.
+// FP32 forces of one agent over one row of bins (the chunked pass): every
+// member of [beg, end) in storage order.  A member's bin corner relative to
+// the agent's is (di, dj, dk) * voxel: dj, dk are the row's (oy, oz), and
+// di = -1, 0, +1 for members before mid0 (the first record of the agent's
+// own column), before mid1 (the first of the next column), and after.  A
+// pair outside the interaction range adds +0 to the sums, which leaves them
+// unchanged (they start at +0, so they are never -0): the sums equal those
+// over the pairs the mark pass lists, in the same order.
 __device__ __attribute__((always_inline)) inline void force_range32(
     const CellRec32* __restrict__ rec, uint32_t beg, uint32_t end, uint32_t t, uint32_t mid0, uint32_t mid1, float mx,
     float my, float mz, float oy, float oz, float voxel, float radius_i, float rep_i, float adh_i, float rel_max_adh,
     float cut2, float& fx, float& fy, float& fz, float& local_pressure) {
-    const float oxm = -1.0f * voxel, ox0 = 0.0f * voxel, oxp = 1.0f * voxel;
     const float relf = rel_max_adh > 1.0f ? rel_max_adh : 1.0f;
-    constexpr float margin = 1.0f + 0x1p-16f;
-    constexpr uint32_t FB = 4;                     // candidate loads in flight per lane (8 measured slower)
-    for (uint32_t u0 = beg; u0 < end; u0 += 32) {
-        const uint32_t cnt = end - u0 < 32u ? end - u0 : 32u;
-        uint32_t mask = 0;
-        for (uint32_t j0 = 0; j0 < cnt; j0 += FB) {
-            float4 o[FB];
+    for (uint32_t u = beg; u < end; u++) {
+        if (u == t) continue;
+        const float4 a = reinterpret_cast<const float4*>(rec + u)[0];
+        const float4 b = reinterpret_cast<const float4*>(rec + u)[1];
+        const float ox = (float)(u < mid0 ? -1 : (u < mid1 ? 0 : 1)) * voxel;
+        force_pair32(a, b, ox, oy, oz, mx, my, mz, radius_i, rep_i, adh_i, rel_max_adh, relf, cut2, fx, fy, fz,
+                     local_pressure);
+    }
+}
+
+// The law over a lane's list of marked candidates, in list order.  Entry:
+// the record position in the low MARK32_POS_BITS bits and the row code
+// (dk + 1) * 3 + (dj + 1) above them.  The candidate's column comes from its
+// record: xg - x is its bin corner's distance from x_min, a whole number of
+// bins (to within 3e-4 um while the grid's x extent is at most
+// MARKX_MAX_EXTENT, far below half a bin).
+__device__ __attribute__((always_inline)) inline void force_list32(
+    const CellRec32* __restrict__ rec, const uint32_t* __restrict__ list, uint32_t cnt, uint32_t t, int vi, float mx,
+    float my, float mz, float voxel, float inv_voxel, float radius_i, float rep_i, float adh_i, float rel_max_adh,
+    float cut2, float& fx, float& fy, float& fz, float& local_pressure) {
+    const float relf = rel_max_adh > 1.0f ? rel_max_adh : 1.0f;
+    for (uint32_t k = 0; k < cnt; k++) {
+        const uint32_t e = list[k * 64];
+        const uint32_t u = e & ((1u << MARK32_POS_BITS) - 1u), code = e >> MARK32_POS_BITS;
+        if (u == t) continue;
+        const float4 a = reinterpret_cast<const float4*>(rec + u)[0];
+        const float4 b = reinterpret_cast<const float4*>(rec + u)[1];
+        const int di = (int)rintf((a.x - a.w) * inv_voxel) - vi;
+        const float ox = (float)di * voxel;
+        const float oy = (float)((int)(code % 3u) - 1) * voxel, oz = (float)((int)(code / 3u) - 1) * voxel;
+        force_pair32(a, b, ox, oy, oz, mx, my, mz, radius_i, rep_i, adh_i, rel_max_adh, relf, cut2, fx, fy, fz,
+                     local_pressure);
+    }
+}
+
+// FP32 forces of one agent with a mark pass over x-trimmed rows and a
+// per-lane list of the marked candidates in LDS (the mixed form; the chunked
+// pass's result bit for bit).
+//
+// Mark: one threshold per agent, lim = min(L^2 (1 + 2^-16), cut^2) +
+// MARKX_SLACK with L = (r_i + rmax) * max(1, rel) in FP32, rmax the largest
+// FP32 radius of any agent: FP32 addition and multiplication are monotonic,
+// so lim is no smaller than any pair's bound in force_pair32.  The mark's
+// d^2 uses xg (x relative to x_min: within 1.2e-4 um while the grid's x
+// extent is at most MARKX_MAX_EXTENT) and the bin-relative y, z; for pairs
+// within 32 um it is within 0.025 um^2 of the law's d^2, below the slack, so
+// every pair the law keeps is marked.
+//
+// Rows: a candidate marked has |dx| < sqrt(lim) + 3e-4 um, so only the x
+// interval [mx - Lx, mx + Lx], Lx = sqrt(lim) + XTRIM_MARGIN, of each row is
+// walked: from the slab holding mx - Lx (in the column left of the agent's
+// when the interval starts there) to the slab holding mx + Lx.  Slabs come
+// from the same FP32 products and floors as the records' (xslab_of, a
+// monotonic function of x), so no record with x inside the interval falls
+// outside the range.  The range is the same for all 9 rows and is one
+// contiguous run of records per row (two loads).
+//
+// A marked candidate is listed with its row code, and the law walks the
+// list in visit order (rows, ascending record position) with the exact
+// predicates, so the pairs that contribute, their order and every bit of the
+// sums are the chunked pass's.  Record positions stay below 2^27 (callers
+// check).
+
+__device__ __attribute__((always_inline)) inline void force_agent32x(
+    const CellRec32* __restrict__ rec, const uint32_t* __restrict__ xoff, uint32_t* __restrict__ list, uint32_t t,
+    uint32_t bin0, uint32_t vi, uint32_t vj, uint32_t vk, uint32_t mnx, uint32_t mny, uint32_t mnz, float mxg,
+    float mx, float my, float mz, float voxel, float scale, float inv_voxel, float rmax, float radius_i, float rep_i,
+    float adh_i, float rel_max_adh, float cut2, float& fx, float& fy, float& fz, float& local_pressure) {
+    const float relf = rel_max_adh > 1.0f ? rel_max_adh : 1.0f;
+    const float L = (radius_i + rmax) * relf;
+    const float lim = fminf(L * L * (1.0f + 0x1p-16f), cut2) + MARKX_SLACK;
+    const float Lx = sqrtf(lim) + XTRIM_MARGIN;
+    // the walked x interval, as slab indices of the row (column vi - 1 .. vi + 1)
+    const float lo = mx - Lx, hi = mx + Lx;
+    const bool left = lo < 0.0f && vi > 0, right = hi >= voxel && vi + 1 < mnx;
+    const uint32_t jlo = XSLABS * (left ? vi - 1 : vi) + xslab_of(left ? lo + voxel : lo, scale);
+    const uint32_t jhi = XSLABS * (right ? vi + 1 : vi) + xslab_of(right ? hi - voxel : hi, scale) + 1;
+    const bool three_d = mnz > 1;
+    uint32_t cnt = 0;
+#pragma unroll 1
+    for (int q = 0; q < 9; q++) {
+        const int dk = three_d ? q / 3 - 1 : 0, dj = q % 3 - 1;
+        const int nk = (int)vk + dk, nj = (int)vj + dj;
+        const bool ok = (three_d || q < 3) && nk >= 0 && nk < (int)mnz && nj >= 0 && nj < (int)mny;
+        const uint32_t rb = XSLABS * (bin0 + (ok ? (uint32_t)nk * mny * mnx + (uint32_t)nj * mnx : 0u));
+        const uint32_t beg = ok ? xoff[rb + jlo] : 0u, end = ok ? xoff[rb + jhi] : 0u;
+        const float myr = my - (float)dj * voxel, mzr = mz - (float)dk * voxel;
+        const uint32_t qc = (uint32_t)((dk + 1) * 3 + (dj + 1)) << MARK32_POS_BITS;
+        const CellRec32* p = rec + beg;
+        for (uint32_t u0 = beg; u0 < end; u0 += MARK32_STEP, p += MARK32_STEP) {
+            float4 o[MARK32_STEP];
 #pragma unroll
-            for (int q = 0; q < FB; q++) {
-                const uint32_t u = u0 + (j0 + q < cnt ? j0 + q : cnt - 1);
-                o[q] = *reinterpret_cast<const float4*>(&rec[u].x);
-            }
+            for (uint32_t j = 0; j < MARK32_STEP; j++) o[j] = *reinterpret_cast<const float4*>(p + j);
 #pragma unroll
-            for (int q = 0; q < FB; q++) {
-                const uint32_t j = j0 + q, u = u0 + j;
-                const float ox = u < mid0 ? oxm : (u < mid1 ? ox0 : oxp);
-                const float ddx = mx - o[q].x - ox, ddy = my - o[q].y - oy, ddz = mz - o[q].z - oz;
-                const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                const float L = (radius_i + o[q].w) * relf;
-                const bool inside = j < cnt && u != t && !(d2 >= cut2) && !(d2 < 1e-12f) && d2 < L * L * margin;
-                mask |= inside ? (1u << j) : 0u;
+            for (uint32_t j = 0; j < MARK32_STEP; j++) {
+                const float ddx = mxg - o[j].x, ddy = myr - o[j].y, ddz = mzr - o[j].z;
+                const float d2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(ddy, ddy, ddx * ddx));
+                list[cnt * 64] = (u0 + j) | qc;
+                cnt += (u0 + j < end && d2 < lim) ? 1u : 0u;
             }
-        }
-        while (mask) {
-            const uint32_t j = __builtin_ctz(mask);
-            mask &= mask - 1u;
-            const uint32_t u = u0 + j;
-            const float4 o = *reinterpret_cast<const float4*>(&rec[u].x);
-            const float ox = u < mid0 ? oxm : (u < mid1 ? ox0 : oxp);
-            const float ddx = mx - o.x - ox, ddy = my - o.y - oy, ddz = mz - o.z - oz;
-            const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            // FP32 form: the hardware square root and reciprocal (about 1 ulp)
-            // instead of the correctly rounded sequences
-            const float dist = __builtin_amdgcn_sqrtf(d2);
-            const float inv_dist = __builtin_amdgcn_rcpf(dist);
-            const float R_sum = radius_i + o.w;
-            float force_mag = 0.f;
-            if (dist < R_sum) {
-                const float overlap = 1.0f - dist * __builtin_amdgcn_rcpf(R_sum);
-                force_mag += overlap * overlap * __builtin_amdgcn_sqrtf(rep_i * rec[u].rep);
-                local_pressure += overlap * overlap * (1.0f / 0.027288820670331f);
+            if (cnt > MARKX_CAP - MARK32_STEP) {
+                force_list32(rec, list, cnt, t, (int)vi, mx, my, mz, voxel, inv_voxel, radius_i, rep_i, adh_i,
+                             rel_max_adh, cut2, fx, fy, fz, local_pressure);
+                cnt = 0;
             }
-            const float adhesion_distance = rel_max_adh * R_sum;
-            if (dist < adhesion_distance) {
-                const float t2 = 1.0f - dist * __builtin_amdgcn_rcpf(adhesion_distance);
-                force_mag -= t2 * t2 * __builtin_amdgcn_sqrtf(adh_i * rec[u].adh);
-            }
-            fx += force_mag * (ddx * inv_dist);
-            fy += force_mag * (ddy * inv_dist);
-            fz += force_mag * (ddz * inv_dist);
         }
     }
+    force_list32(rec, list, cnt, t, (int)vi, mx, my, mz, voxel, inv_voxel, radius_i, rep_i, adh_i, rel_max_adh, cut2,
+                 fx, fy, fz, local_pressure);
+}
+
+// The chunked pass over the 9 (3 in 2-D) neighbour rows of an agent.
+__device__ __attribute__((always_inline)) inline void force_rows32(
+    const CellRec32* __restrict__ rec, const uint32_t* __restrict__ offsets, uint32_t t, uint32_t bin0, uint32_t vi,
+    uint32_t vj, uint32_t vk, uint32_t mnx, uint32_t mny, uint32_t mnz, float mx, float my, float mz, float voxel,
+    float radius_i, float rep_i, float adh_i, float rel_max_adh, float cut2, float& fx, float& fy, float& fz,
+    float& local_pressure) {
+#pragma unroll 1
+    for (int q = 0; q < 9; q++) {
+        const RowRange rr = neighbor_row(offsets, bin0, vi, vj, vk, mnx, mny, mnz, q);
+        force_range32(rec, rr.beg, rr.end, t, rr.mid0, rr.mid1, mx, my, mz, (float)rr.dj * voxel,
+                      (float)rr.dk * voxel, voxel, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz,
+                      local_pressure);
+    }
+}
+
+// Bins of at most BIN_SORT_CAP members: the hash's ascending list (as
+// bin_sort_one) and each member's FP32 record position by (x slab, index),
+// into rpos; every bin: its slab starts into xoff (xoff[XSLABS * b + s]).
+// Larger bins leave their members' positions to the gather (slab_rank).
+// Thread nbins writes the XSLABS entries past the last bin.
+__device__ inline void bin_sort_slabs(const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ tmp,
+                                      const uint8_t* __restrict__ xslab, uint32_t* __restrict__ sorted,
+                                      uint32_t* __restrict__ rpos, uint32_t* __restrict__ xoff, uint32_t b,
+                                      uint32_t nbins) {
+    if (b == nbins) {
+        const uint32_t n = offsets[nbins];
+        *reinterpret_cast<uint4*>(xoff + (size_t)XSLABS * nbins) = make_uint4(n, n, n, n);
+        return;
+    }
+    const uint32_t beg = offsets[b], cnt = offsets[b + 1] - beg;
+    uint32_t c0 = 0, c1 = 0, c2 = 0;
+    if (cnt > BIN_SORT_CAP) {
+        for (uint32_t p = beg; p < beg + cnt; p++) {
+            const uint32_t s = xslab[tmp[p]];
+            c0 += s == 0 ? 1u : 0u;
+            c1 += s == 1 ? 1u : 0u;
+            c2 += s == 2 ? 1u : 0u;
+        }
+        *reinterpret_cast<uint4*>(xoff + (size_t)XSLABS * b) =
+            make_uint4(beg, beg + c0, beg + c0 + c1, beg + c0 + c1 + c2);
+        return;
+    }
+    uint32_t a[BIN_SORT_CAP];
+#pragma unroll
+    for (uint32_t q = 0; q < BIN_SORT_CAP; q++) a[q] = q < cnt ? tmp[beg + q] : ~0u;
+#pragma unroll
+    for (uint32_t k = 2; k <= BIN_SORT_CAP; k <<= 1)
+#pragma unroll
+        for (uint32_t j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (uint32_t i = 0; i < BIN_SORT_CAP; i++) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const uint32_t lo = a[i] < a[l] ? a[i] : a[l], hi = a[i] < a[l] ? a[l] : a[i];
+                    const bool up = (i & k) == 0;
+                    a[i] = up ? lo : hi;
+                    a[l] = up ? hi : lo;
+                }
+            }
+    uint32_t s[BIN_SORT_CAP];
+#pragma unroll
+    for (uint32_t q = 0; q < BIN_SORT_CAP; q++) s[q] = q < cnt ? xslab[a[q]] : XSLABS;
+#pragma unroll
+    for (uint32_t q = 0; q < BIN_SORT_CAP; q++) {
+        c0 += s[q] == 0 ? 1u : 0u;
+        c1 += s[q] == 1 ? 1u : 0u;
+        c2 += s[q] == 2 ? 1u : 0u;
+    }
+    const uint32_t x1 = beg + c0, x2 = x1 + c1, x3 = x2 + c2;
+    *reinterpret_cast<uint4*>(xoff + (size_t)XSLABS * b) = make_uint4(beg, x1, x2, x3);
+    // members in ascending index order: each takes the next position of its slab
+    uint32_t n0 = beg, n1 = x1, n2 = x2, n3 = x3;
+#pragma unroll
+    for (uint32_t q = 0; q < BIN_SORT_CAP; q++)
+        if (q < cnt) {
+            sorted[beg + q] = a[q];
+            const uint32_t sq = s[q];
+            rpos[a[q]] = sq == 0 ? n0 : (sq == 1 ? n1 : (sq == 2 ? n2 : n3));
+            n0 += sq == 0 ? 1u : 0u;
+            n1 += sq == 1 ? 1u : 0u;
+            n2 += sq == 2 ? 1u : 0u;
+            n3 += sq == 3 ? 1u : 0u;
+        }
+}
+
+// FP32 record position of member c (slab sc) of a bin of more than
+// BIN_SORT_CAP members: the members of a smaller slab, or of the same slab
+// and a smaller index, come first.
+__device__ inline uint32_t slab_rank(const uint32_t* __restrict__ tmp, const uint8_t* __restrict__ xslab,
+                                     uint32_t beg, uint32_t end, uint32_t c, uint32_t sc) {
+    uint32_t r = 0;
+    for (uint32_t t0 = beg; t0 < end; t0 += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = tmp[t0 + q < end ? t0 + q : end - 1];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const uint32_t sv = xslab[v[q]];
+            r += (t0 + q < end && (sv < sc || (sv == sc && v[q] < c))) ? 1u : 0u;
+        }
+    }
+    return r;
 }
 
 }  // namespace hc

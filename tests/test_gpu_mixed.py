@@ -114,3 +114,49 @@ def test_mixed_first_hash_matches_fp64(ctx):
     assert er < 1e-4
     for b in (hb, cb.soa, cb.rates):
         b.free()
+
+
+def _fp32_forces(ctx, spec, cells, mode):
+    """FP32 forces of one hash build: mode 1 = x-trimmed mark pass with a
+    list, mode 2 = the chunked pass over whole rows."""
+    lib = H.load()
+    cb = H.CellBuffer(ctx, cells, len(spec.substrates))
+    nb = spec.n_mech_voxels
+    hb = ctx.alloc(lib.hc_hash_bytes(nb, cb.max_cells))
+    mech = H.mech_of(spec, fp32=True)
+    mech.forces_fp32 = mode
+    ctx._check(lib.hc_build_hash(ctx.h, cb.soa.h, hb.h, mech, H.mech_grid_of(spec), cells.n, cb.max_cells, nb),
+               "hash")
+    ctx._check(lib.hc_forces(ctx.h, cb.soa.h, hb.h, mech, cells.n, cb.max_cells, nb), "forces")
+    out = [cb.field(f) for f in (H.F_VEL_X, H.F_VEL_Y, H.F_VEL_Z, H.F_PRESSURE)]
+    for b in (hb, cb.soa, cb.rates):
+        b.free()
+    return out
+
+
+def _crowded(seed=7):
+    """small3d with a knot of 300 agents in a few 30 um bins (bins above the
+    16-member sort cap, slabs holding many members) beside sparse agents."""
+    spec, cells = S.small3d(nx=16, ny=12, nz=9, n_cells=600, seed=seed)
+    rng = np.random.default_rng(seed)
+    k = 300
+    cells.x[:k] = 95.0 + rng.uniform(-20.0, 20.0, k)
+    cells.y[:k] = 118.0 + rng.uniform(-12.0, 12.0, k)
+    cells.z[:k] = 61.0 + rng.uniform(-6.0, 6.0, k)
+    return spec, cells
+
+
+@pytest.mark.parametrize("builder", ["config1", "small3d", "crowded", "config0"])
+def test_fp32_trimmed_list_equals_chunked_pass(ctx, builder):
+    """The mark pass walks only the x interval of each row that can interact
+    (records ordered by x slab inside their bins); the chunked pass walks
+    every member of the 9 rows.  Pairs outside the interval add +0, so the
+    two must agree bit for bit."""
+    spec, cells = {"config1": S.config1, "small3d": S.small3d, "crowded": _crowded, "config0": S.config0}[builder]()
+    a = _fp32_forces(ctx, spec, cells, 1)
+    b = _fp32_forces(ctx, spec, cells, 2)
+    nz = int(np.count_nonzero(a[3]))
+    print(f"{builder}: {cells.n} agents, {nz} under pressure", flush=True)
+    assert nz > cells.n // 10
+    for x, y in zip(a, b):
+        assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
