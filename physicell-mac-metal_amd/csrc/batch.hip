@@ -168,13 +168,17 @@ __global__ __launch_bounds__(256) void k_b_gather(const EnsSim* __restrict__ sim
 // FP32 members whose mechanics grid keeps the FP32 mark's error bound
 // (member_markx, bins of at least 0.5 um) take the x-trimmed mark pass with a
 // list; the others the chunked pass.
-template <bool FP32, bool LIST = true>
+// MIRROR (FP32 only): the records' index field is the agent's mirror slot,
+// kglob its bin (the mirror's key array), the relative adhesion distance
+// comes from the mirror and the forces go to vrec in record order.
+template <bool FP32, bool LIST = true, bool MIRROR = false>
 __global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sims, const AgentTile* __restrict__ tiles,
                                                   const void* __restrict__ recs, const float4* __restrict__ pos32,
                                                   const uint32_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ xoff,
                                                   const uint32_t* __restrict__ kglob,
-                                                  const float* __restrict__ rmax) {
+                                                  const float* __restrict__ rmax,
+                                                  const float4* __restrict__ mprop, float4* __restrict__ vrec) {
     __shared__ uint32_t marks[LIST ? 4 * MARKX_CAP * 64 : 1];
     uint32_t* list = marks + (LIST ? (threadIdx.x >> 6) * (MARKX_CAP * 64) + (threadIdx.x & 63) : 0);
     const AgentTile tl = tiles[xcd_block(blockIdx.x, gridDim.x)];
@@ -198,7 +202,7 @@ __global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sim
     if (FP32) {
         const CellRec32 me = reinterpret_cast<const CellRec32*>(recs)[t];
         const CellRec32* rec = reinterpret_cast<const CellRec32*>(recs);
-        const float rel = (float)sim_field(m, HC_F_REL_MAX_ADH)[c];
+        const float rel = MIRROR ? mprop[idx].w : (float)sim_field(m, HC_F_REL_MAX_ADH)[c];
         const float cut = (float)m.max_dist, vox = (float)m.mvoxel;
         float fx = 0.f, fy = 0.f, fz = 0.f, pr = 0.f;
         if (LIST && member_markx(m) && m.mvoxel >= 0.5)
@@ -208,6 +212,10 @@ __global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sim
         else
             force_rows32(rec, offsets, t, m.bin_off, vi, vj, vk, m.mnx, m.mny, m.mnz, me.x, me.y, me.z, vox, me.r,
                          me.rep, me.adh, rel, cut * cut, fx, fy, fz, pr);
+        if (MIRROR) {
+            vrec[t] = make_float4(fx, fy, fz, pr);
+            return;
+        }
         ofx = fx; ofy = fy; ofz = fz; opr = pr;
     } else {
         const CellRec me = reinterpret_cast<const CellRec*>(recs)[t];
@@ -397,10 +405,312 @@ __global__ void k_b_motility_advance(const EnsSim* __restrict__ sims, const MotP
     if (s < nsims && (sims[s].mot & 1u)) mots[s].hdr[0] += 1;
 }
 
+// ---------------------------------------------------------------------------
+// Agent mirror of the batched FP32-mixed ensemble
+// ---------------------------------------------------------------------------
+// Agents in index order are not in spatial order, so every per-agent pass of
+// the mechanics step in index order scatters: the key pass's ballots find
+// about one bin per lane (one atomic each), the gather writes its records
+// and the forces their results at unrelated addresses.  The mirror keeps each
+// agent's mechanics state -- FP64 position and previous velocity, FP32
+// radius, repulsion, adhesion and relative adhesion distance, index -- in
+// slots ordered like the records of a recent step.  Slots keep their order
+// from step to step (a periodic re-sort restores it as agents change bins),
+// the next step's bin and slot in the bin are computed by AB2 itself, and
+// the forces are written in record order; each slot knows its record
+// position of the step (rpos).  Records, their order (bin, x slab, index),
+// every arithmetic step and so every result are those of the direct path.
+// The cell fields (position, velocity, previous velocity, pressure,
+// mechanics voxel) are written back by index only when something reads them.
+struct BMirror {
+    double4* pos;     // x, y, z: the FP64 AB2 accumulator
+    double4* prev;    // previous velocity
+    float4* prop;     // radius, repulsion, adhesion, relative max adhesion distance (FP32)
+    uint32_t* idx;    // global agent index (member cell_off + cell)
+    uint32_t* key;    // global bin of the coming step
+    uint32_t* slot;   // slot inside that bin
+    uint8_t* xslab;   // x slab of the agent's FP32 record in that bin
+    uint32_t* rpos;   // record position in the last step
+    uint32_t* kstep;  // bin of the last step (the cell's mechanics voxel)
+    float4* v;        // forces of the last step by record position: vx, vy, vz, pressure
+};
+
+// bin and x slab of a position (k_b_mech_keys's expressions)
+__device__ inline uint32_t bm_key(const EnsSim& m, double px, double py, double pz, uint8_t& xs) {
+    int vi = (int)((px - m.x_min) / m.mvoxel);
+    int vj = (int)((py - m.y_min) / m.mvoxel);
+    int vk = (int)((pz - m.z_min) / m.mvoxel);
+    vi = vi < 0 ? 0 : (vi > (int)m.mnx - 1 ? (int)m.mnx - 1 : vi);
+    vj = vj < 0 ? 0 : (vj > (int)m.mny - 1 ? (int)m.mny - 1 : vj);
+    vk = vk < 0 ? 0 : (vk > (int)m.mnz - 1 ? (int)m.mnz - 1 : vk);
+    xs = (uint8_t)xslab_of(bin_rel32(px, m.x_min, (uint32_t)vi, m.mvoxel), xslab_scale(m.mvoxel));
+    return m.bin_off + (uint32_t)vk * m.mny * m.mnx + (uint32_t)vj * m.mnx + (uint32_t)vi;
+}
+
+// the mirror from the cell fields, slot = index; bins and slots of the
+// current positions (counts must be zero)
+__global__ __launch_bounds__(256) void k_bm_load(const EnsSim* __restrict__ sims, const AgentTile* __restrict__ tiles,
+                                                 BMirror mr, uint32_t* __restrict__ counts) {
+    const AgentTile tl = tiles[blockIdx.x];
+    const EnsSim m = sims[tl.sim];
+    const uint32_t c = tl.c0 + threadIdx.x;
+    const bool valid = c < m.n;
+    const uint32_t p = m.cell_off + c;
+    uint32_t key = 0;
+    uint8_t xs = 0;
+    if (valid) {
+        const double px = sim_field(m, HC_F_POS_X)[c], py = sim_field(m, HC_F_POS_Y)[c],
+                     pz = sim_field(m, HC_F_POS_Z)[c];
+        mr.pos[p] = make_double4(px, py, pz, 0.0);
+        mr.prev[p] = make_double4(sim_field(m, HC_F_PREV_VX)[c], sim_field(m, HC_F_PREV_VY)[c],
+                                  sim_field(m, HC_F_PREV_VZ)[c], 0.0);
+        mr.prop[p] = make_float4((float)sim_field(m, HC_F_RADIUS)[c], (float)sim_field(m, HC_F_REPULSION)[c],
+                                 (float)sim_field(m, HC_F_ADHESION)[c], (float)sim_field(m, HC_F_REL_MAX_ADH)[c]);
+        mr.idx[p] = p;
+        key = bm_key(m, px, py, pz, xs);
+        mr.key[p] = key;
+        mr.xslab[p] = xs;
+    }
+    const uint32_t sl = ballot_bin(valid, key, counts);
+    if (valid) mr.slot[p] = sl;
+}
+
+// Bins of at most BIN_SORT_CAP members (mirror slots in tmp): the ascending
+// list of their agent indices and each slot's record position by (x slab,
+// index); every bin: its slab starts (xoff).  Slot indices are below 2^27.
+__global__ __launch_bounds__(256) void k_bm_bin_sort(const uint32_t* __restrict__ offsets,
+                                                     const uint32_t* __restrict__ tmp, BMirror mr,
+                                                     uint32_t* __restrict__ sorted, uint32_t* __restrict__ xoff,
+                                                     uint32_t nbins) {
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b > nbins) return;
+    if (b == nbins) {
+        const uint32_t n = offsets[nbins];
+        *reinterpret_cast<uint4*>(xoff + (size_t)XSLABS * nbins) = make_uint4(n, n, n, n);
+        return;
+    }
+    const uint32_t beg = offsets[b], cnt = offsets[b + 1] - beg;
+    uint32_t c0 = 0, c1 = 0, c2 = 0;
+    if (cnt == 0) {
+        *reinterpret_cast<uint4*>(xoff + (size_t)XSLABS * b) = make_uint4(beg, beg, beg, beg);
+        return;
+    }
+    if (cnt > BIN_SORT_CAP) {
+        for (uint32_t q = beg; q < beg + cnt; q++) {
+            const uint32_t sq = mr.xslab[tmp[q]];
+            c0 += sq == 0 ? 1u : 0u;
+            c1 += sq == 1 ? 1u : 0u;
+            c2 += sq == 2 ? 1u : 0u;
+        }
+        *reinterpret_cast<uint4*>(xoff + (size_t)XSLABS * b) =
+            make_uint4(beg, beg + c0, beg + c0 + c1, beg + c0 + c1 + c2);
+        return;
+    }
+    // (index << 5 | member number) sorted: indices are below 2^27
+    uint32_t a[BIN_SORT_CAP];
+#pragma unroll
+    for (uint32_t q = 0; q < BIN_SORT_CAP; q++) a[q] = q < cnt ? (mr.idx[tmp[beg + q]] << 5) | q : ~0u;
+#pragma unroll
+    for (uint32_t k = 2; k <= BIN_SORT_CAP; k <<= 1)
+#pragma unroll
+        for (uint32_t j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (uint32_t i = 0; i < BIN_SORT_CAP; i++) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const uint32_t lo = a[i] < a[l] ? a[i] : a[l], hi = a[i] < a[l] ? a[l] : a[i];
+                    const bool up = (i & k) == 0;
+                    a[i] = up ? lo : hi;
+                    a[l] = up ? hi : lo;
+                }
+            }
+    uint32_t ps[BIN_SORT_CAP], s[BIN_SORT_CAP];
+#pragma unroll
+    for (uint32_t q = 0; q < BIN_SORT_CAP; q++) {
+        ps[q] = q < cnt ? tmp[beg + (a[q] & 31u)] : 0u;
+        s[q] = q < cnt ? mr.xslab[ps[q]] : XSLABS;
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < BIN_SORT_CAP; q++) {
+        c0 += s[q] == 0 ? 1u : 0u;
+        c1 += s[q] == 1 ? 1u : 0u;
+        c2 += s[q] == 2 ? 1u : 0u;
+    }
+    const uint32_t x1 = beg + c0, x2 = x1 + c1, x3 = x2 + c2;
+    *reinterpret_cast<uint4*>(xoff + (size_t)XSLABS * b) = make_uint4(beg, x1, x2, x3);
+    uint32_t n0 = beg, n1 = x1, n2 = x2, n3 = x3;
+#pragma unroll
+    for (uint32_t q = 0; q < BIN_SORT_CAP; q++)
+        if (q < cnt) {
+            sorted[beg + q] = a[q] >> 5;
+            const uint32_t sq = s[q];
+            mr.rpos[ps[q]] = sq == 0 ? n0 : (sq == 1 ? n1 : (sq == 2 ? n2 : n3));
+            n0 += sq == 0 ? 1u : 0u;
+            n1 += sq == 1 ? 1u : 0u;
+            n2 += sq == 2 ? 1u : 0u;
+            n3 += sq == 3 ? 1u : 0u;
+        }
+}
+
+// FP32 records from the mirror (k_b_gather<true>'s values); slots of bins
+// above BIN_SORT_CAP are ranked here.  The record's index field holds the slot.
+__global__ __launch_bounds__(256) void k_bm_gather(const EnsSim* __restrict__ sims,
+                                                   const AgentTile* __restrict__ tiles, BMirror mr,
+                                                   const uint32_t* __restrict__ offsets,
+                                                   const uint32_t* __restrict__ tmp, uint32_t* __restrict__ sorted,
+                                                   CellRec32* __restrict__ recs, float* __restrict__ rpart) {
+    const AgentTile tl = tiles[blockIdx.x];
+    const EnsSim m = sims[tl.sim];
+    const uint32_t c = tl.c0 + threadIdx.x;
+    const uint32_t p = m.cell_off + c;
+    const float4 pr = c < m.n ? mr.prop[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+    block_max_to(pr.x, rpart);
+    if (c >= m.n) return;
+    const uint32_t b = mr.key[p];
+    const uint32_t beg = offsets[b], end = offsets[b + 1];
+    uint32_t pos;
+    if (end - beg <= BIN_SORT_CAP) {
+        pos = mr.rpos[p];
+    } else {
+        const uint32_t id = mr.idx[p], sc = mr.xslab[p];
+        uint32_t ri = 0, rs = 0;
+        for (uint32_t q = beg; q < end; q++) {
+            const uint32_t o = tmp[q];
+            const uint32_t io = mr.idx[o], so = mr.xslab[o];
+            ri += io < id ? 1u : 0u;
+            rs += (so < sc || (so == sc && io < id)) ? 1u : 0u;
+        }
+        sorted[beg + ri] = id;
+        pos = beg + rs;
+        mr.rpos[p] = pos;
+    }
+    mr.kstep[p] = b;
+    const double4 x = mr.pos[p];
+    const uint32_t lb = b - m.bin_off;
+    const uint32_t bi = lb % m.mnx, bj = (lb / m.mnx) % m.mny, bk = lb / (m.mnx * m.mny);
+    CellRec32 r;
+    r.xg = (float)(x.x - m.x_min);
+    r.y = bin_rel32(x.y, m.y_min, bj, m.mvoxel);
+    r.z = bin_rel32(x.z, m.z_min, bk, m.mvoxel);
+    r.x = bin_rel32(x.x, m.x_min, bi, m.mvoxel);
+    r.r = pr.x;
+    r.rep = pr.y;
+    r.adh = pr.z;
+    r.idx = p;
+    recs[pos] = r;
+}
+
+// AB2 (shaders/integrate.metal:56-79, k_b_integrate's expressions) on the
+// mirror, then the bin and slot of the new position for the coming step
+// (counts must be zero)
+__global__ __launch_bounds__(256) void k_bm_integrate(const EnsSim* __restrict__ sims,
+                                                      const AgentTile* __restrict__ tiles, BMirror mr,
+                                                      uint32_t* __restrict__ counts) {
+    const AgentTile tl = tiles[blockIdx.x];
+    const EnsSim m = sims[tl.sim];
+    const uint32_t c = tl.c0 + threadIdx.x;
+    const bool valid = c < m.n;
+    const uint32_t p = m.cell_off + c;
+    uint32_t key = 0;
+    uint8_t xs = 0;
+    if (valid) {
+        const double x_max = m.x_min + (double)m.mnx * m.mvoxel;
+        const double y_max = m.y_min + (double)m.mny * m.mvoxel;
+        const double z_max = m.z_min + (double)m.mnz * m.mvoxel;
+        const double dt = m.dt_mech;
+        const float4 f = mr.v[mr.rpos[p]];
+        const double vx = (double)f.x, vy = (double)f.y, vz = (double)f.z;
+        const double4 pv = mr.prev[p];
+        const double4 x = mr.pos[p];
+        double px = x.x, py = x.y, pz = x.z;
+        px += dt * (1.5 * vx - 0.5 * pv.x);
+        py += dt * (1.5 * vy - 0.5 * pv.y);
+        pz += dt * (1.5 * vz - 0.5 * pv.z);
+        px = px < m.x_min ? m.x_min : (px > x_max ? x_max : px);
+        py = py < m.y_min ? m.y_min : (py > y_max ? y_max : py);
+        pz = pz < m.z_min ? m.z_min : (pz > z_max ? z_max : pz);
+        mr.pos[p] = make_double4(px, py, pz, 0.0);
+        mr.prev[p] = make_double4(vx, vy, vz, 0.0);
+        key = bm_key(m, px, py, pz, xs);
+        mr.key[p] = key;
+        mr.xslab[p] = xs;
+    }
+    const uint32_t sl = ballot_bin(valid, key, counts);
+    if (valid) mr.slot[p] = sl;
+}
+
+// the cell fields from the mirror, by agent index: position, velocity
+// (forces of the last step), previous velocity, pressure, mechanics voxel
+__global__ __launch_bounds__(256) void k_bm_sync(const EnsSim* __restrict__ sims, const AgentTile* __restrict__ tiles,
+                                                 BMirror mr) {
+    const AgentTile tl = tiles[blockIdx.x];
+    const EnsSim m = sims[tl.sim];
+    const uint32_t c0 = tl.c0 + threadIdx.x;
+    if (c0 >= m.n) return;
+    const uint32_t p = m.cell_off + c0;
+    const uint32_t c = mr.idx[p] - m.cell_off;
+    const double4 x = mr.pos[p], pv = mr.prev[p];
+    const float4 f = mr.v[mr.rpos[p]];
+    sim_field(m, HC_F_POS_X)[c] = x.x;
+    sim_field(m, HC_F_POS_Y)[c] = x.y;
+    sim_field(m, HC_F_POS_Z)[c] = x.z;
+    sim_field(m, HC_F_VEL_X)[c] = (double)f.x;
+    sim_field(m, HC_F_VEL_Y)[c] = (double)f.y;
+    sim_field(m, HC_F_VEL_Z)[c] = (double)f.z;
+    sim_field(m, HC_F_PRESSURE)[c] = (double)f.w;
+    sim_field(m, HC_F_PREV_VX)[c] = pv.x;
+    sim_field(m, HC_F_PREV_VY)[c] = pv.y;
+    sim_field(m, HC_F_PREV_VZ)[c] = pv.z;
+    sim_ufield(m, HC_F_MECH_VOXEL)[c] = mr.kstep[p] - m.bin_off;
+}
+
+// Re-sort: every slot's state moves to its record position of the last
+// step (into the second set), which becomes its slot.
+__global__ __launch_bounds__(256) void k_bm_permute(const EnsSim* __restrict__ sims,
+                                                    const AgentTile* __restrict__ tiles, BMirror a, BMirror b) {
+    const AgentTile tl = tiles[blockIdx.x];
+    const EnsSim m = sims[tl.sim];
+    const uint32_t c = tl.c0 + threadIdx.x;
+    if (c >= m.n) return;
+    const uint32_t p = m.cell_off + c;
+    const uint32_t q = a.rpos[p];
+    b.pos[q] = a.pos[p];
+    b.prev[q] = a.prev[p];
+    b.prop[q] = a.prop[p];
+    b.idx[q] = a.idx[p];
+    b.key[q] = a.key[p];
+    b.slot[q] = a.slot[p];
+    b.xslab[q] = a.xslab[p];
+    b.kstep[q] = a.kstep[p];
+    b.rpos[q] = q;
+}
+
+// the second set back into the first, slot by slot
+__global__ __launch_bounds__(256) void k_bm_copy(const EnsSim* __restrict__ sims, const AgentTile* __restrict__ tiles,
+                                                 BMirror a, BMirror b) {
+    const AgentTile tl = tiles[blockIdx.x];
+    const EnsSim m = sims[tl.sim];
+    const uint32_t c = tl.c0 + threadIdx.x;
+    if (c >= m.n) return;
+    const uint32_t p = m.cell_off + c;
+    a.pos[p] = b.pos[p];
+    a.prev[p] = b.prev[p];
+    a.prop[p] = b.prop[p];
+    a.idx[p] = b.idx[p];
+    a.key[p] = b.key[p];
+    a.slot[p] = b.slot[p];
+    a.xslab[p] = b.xslab[p];
+    a.kstep[p] = b.kstep[p];
+    a.rpos[p] = b.rpos[p];
+}
+
 int probe_kernels_batch(hc_ctx* ctx) {
     hipFuncAttributes attr;
     const void* fns[] = {(const void*)k_b_mech_keys, (const void*)k_b_gather<false>,
                          (const void*)k_b_gather<true>, (const void*)k_b_forces<false>, (const void*)k_b_forces<true, true>, (const void*)k_b_forces<true, false>,
+                         (const void*)k_b_forces<true, true, true>, (const void*)k_bm_load, (const void*)k_bm_bin_sort,
+                         (const void*)k_bm_gather, (const void*)k_bm_integrate, (const void*)k_bm_sync,
+                         (const void*)k_bm_permute, (const void*)k_bm_copy,
                          (const void*)k_b_integrate, (const void*)k_b_sec_keys, (const void*)k_b_sec_groups<double>,
                          (const void*)k_b_sec_groups<float>,
                          (const void*)k_b_sec_apply<double>, (const void*)k_b_sec_apply<float>,
@@ -501,6 +811,7 @@ public:
     uint32_t add(const SimConfig& c_in, uint32_t n, const double* x, const double* y, const double* z,
                  const double* radius, const double* volume, const double* rep, const double* adh, const double* rel,
                  const double* sec, const double* upt, const double* sat, const double* net, const uint32_t* alive) {
+        sync_soa();
         auto m = std::make_unique<Member>();
         m->id = next_id_++;
         m->cfg = c_in;
@@ -602,6 +913,7 @@ public:
     }
 
     void remove(uint32_t id) {
+        sync_soa();
         auto it = std::find_if(members_.begin(), members_.end(), [&](const std::unique_ptr<Member>& m) { return m->id == id; });
         if (it == members_.end()) throw std::runtime_error("ensemble: unknown member");
         check(ctx_, hc_wait(ctx_), "wait");
@@ -616,6 +928,7 @@ public:
     void set_motility(uint32_t id, uint64_t seed, const double* persistence, const double* bias, const double* speed,
                       const uint32_t* is_motile, const uint32_t* restrict_2d, const int32_t* chem_index,
                       const int32_t* chem_dir, const double* sensitivity) {
+        sync_soa();
         Member& m = member(id);
         check(ctx_, hc_wait(ctx_), "wait");
         const uint32_t ns = m.grid.n_substrates, mc = m.max_cells, n = m.n;
@@ -656,6 +969,7 @@ public:
     // 555-624, where fields[f] is NULL), with room for max_cells cells: the
     // member's cell, rate and motility arrays move to the larger layout.
     void set_volume(uint32_t id, uint64_t seed, const double* const* fields, uint32_t max_cells) {
+        sync_soa();
         Member& m = member(id);
         check(ctx_, hc_wait(ctx_), "wait");
         if (max_cells > m.max_cells) grow(m, max_cells);
@@ -685,6 +999,7 @@ public:
     // member's voxel refresh and the secretion re-plan.
     uint32_t update_phenotype(uint32_t id, double dt, int use_ratio, const uint32_t* mask, const uint32_t* parents,
                               uint32_t n_parents, const uint32_t* removed, uint32_t n_removed) {
+        sync_soa();
         Member& m = member(id);
         if (!m.vol.get()) throw std::runtime_error("ensemble: set the member's volume state first");
         check(ctx_, hc_wait(ctx_), "wait");
@@ -716,6 +1031,7 @@ public:
     // a hash of that member alone, built from its current positions into a
     // query workspace; as pc_sim_neighbor_query for a single simulation.
     uint64_t neighbor_query(uint32_t id, int kind, std::vector<uint32_t>& off, std::vector<uint32_t>& idx) {
+        sync_soa();
         Member& m = member(id);
         check(ctx_, hc_wait(ctx_), "wait");
         const uint32_t nmv = m.mech.mech_grid_nx * m.mech.mech_grid_ny * m.mech.mech_grid_nz;
@@ -753,12 +1069,22 @@ public:
         rebuild();
         if (members_.empty()) return;
         for (int p = 0; p < periods; p++) {
+            const bool mir = mirror_on();
+            if (mir) {
+                // eager, before the period (graph replays assume a loaded mirror)
+                if (!mirror_valid_) mirror_load();
+                else if (steps_since_sort_ >= MIRROR_RESORT_STEPS) mirror_resort();
+            }
             if (use_graph) {
                 if (!graph_) capture();
                 hipError_t e = hipGraphLaunch((hipGraphExec_t)graph_, hc_stream_of(ctx_));
                 if (e != hipSuccess) throw std::runtime_error(std::string("ensemble graph launch: ") + hipGetErrorString(e));
             } else {
                 period_kernels();
+            }
+            if (mir) {
+                soa_stale_ = true;
+                steps_since_sort_++;
             }
             advance_clocks();
         }
@@ -787,6 +1113,7 @@ public:
     }
 
     void get_field(uint32_t id, uint32_t field, void* out) {
+        sync_soa();
         Member& m = member(id);
         wait();
         const size_t w = field < HC_NUM_F64_FIELDS ? 8 : 4;
@@ -902,6 +1229,8 @@ private:
         DeviceBuffer sims, mots, bsims, tiles, work[NLIST], agent_sim, kglob, slot, tmp, sorted, pos_of, recs, pos32, xoff, xslab, counts,
             offsets, partial, rpart, rmax;
         DeviceBuffer pcounts, poffsets, pkv, pslot, ptmp, psorted, pflag, pgsim, pgvox, pgbegin, pcoef, phdr, ppartial;
+        // agent mirror (mixed form without motility): two sets, forces by record
+        DeviceBuffer mpos[2], mprev[2], mprop[2], midx[2], mkey[2], mslot[2], mxslab[2], mrpos[2], mkstep[2], mv;
         uint32_t ntot = 0, nbins = 0, vtot = 0, ntiles = 0, nwork[NLIST] = {}, ns_max = 0, nmax_all = 0;
         uint32_t max_line[NLIST] = {};
         size_t bytes = 0;
@@ -947,6 +1276,7 @@ private:
 
     void rebuild() {
         if (!dirty_) return;
+        sync_soa();                                  // (every mutator has synced already)
         drop_graph();
         check(ctx_, hc_wait(ctx_), "wait");        // queued work may still read the old workspace
         Workspace& w = ws_;
@@ -1049,6 +1379,16 @@ private:
             alloc(w.xoff, ((size_t)w.nbins + 1) * hc::XSLABS * 4);
             alloc(w.xslab, N);
         }
+        if (mirror_on()) {
+            for (int k = 0; k < 2; k++) {
+                alloc(w.mpos[k], N * 32); alloc(w.mprev[k], N * 32); alloc(w.mprop[k], N * 16);
+                alloc(w.midx[k], N * 4); alloc(w.mkey[k], N * 4); alloc(w.mslot[k], N * 4);
+                alloc(w.mxslab[k], N); alloc(w.mrpos[k], N * 4); alloc(w.mkstep[k], N * 4);
+            }
+            alloc(w.mv, N * 16);
+        }
+        mirror_valid_ = false;
+        soa_stale_ = false;
         alloc(w.rpart, (size_t)std::max(1u, w.ntiles) * 4);
         alloc(w.rmax, 256);
         alloc(w.counts, ((size_t)w.nbins + 1) * 4); alloc(w.offsets, ((size_t)w.nbins + 2) * 4);
@@ -1188,9 +1528,134 @@ private:
         check(ctx_, hc::check_launch(ctx_, "k_b_sec_apply"), "secretion");
     }
 
+    // The agent mirror runs the mixed form's mechanics when no member is
+    // motile (motility rewrites the velocities between forces and AB2) and
+    // the records take the mark pass with a list.
+    bool mirror_on() const {
+        return mixed_ && !any_motility_ && ws_.ntot > 0 && ws_.ntot < (1u << hc::MARK32_POS_BITS) &&
+               ws_.nbins < (1u << 30);
+    }
+
+    hc::BMirror mirror_set(int k) {
+        Workspace& w = ws_;
+        hc::BMirror m;
+        m.pos = (double4*)hc_device_ptr(w.mpos[k].get());
+        m.prev = (double4*)hc_device_ptr(w.mprev[k].get());
+        m.prop = (float4*)hc_device_ptr(w.mprop[k].get());
+        m.idx = (uint32_t*)hc_device_ptr(w.midx[k].get());
+        m.key = (uint32_t*)hc_device_ptr(w.mkey[k].get());
+        m.slot = (uint32_t*)hc_device_ptr(w.mslot[k].get());
+        m.xslab = (uint8_t*)hc_device_ptr(w.mxslab[k].get());
+        m.rpos = (uint32_t*)hc_device_ptr(w.mrpos[k].get());
+        m.kstep = (uint32_t*)hc_device_ptr(w.mkstep[k].get());
+        m.v = (float4*)hc_device_ptr(w.mv.get());
+        if (!m.pos || !m.prev || !m.prop || !m.idx || !m.key || !m.slot || !m.xslab || !m.rpos || !m.kstep || !m.v)
+            throw std::runtime_error("ensemble: agent mirror not allocated");
+        return m;
+    }
+
+    // hash of the mirror's coming step: lists, record positions, slab starts
+    // and the FP32 records (counts hold the histogram of the mirror's bins)
+    void mirror_hash(const hc::BMirror& a) {
+        Workspace& w = ws_;
+        hipStream_t st = hc_stream_of(ctx_);
+        const EnsSim* sims = (const EnsSim*)hc_device_ptr(w.sims.get());
+        const AgentTile* tiles = (const AgentTile*)hc_device_ptr(w.tiles.get());
+        uint32_t* counts = (uint32_t*)hc_device_ptr(w.counts.get());
+        uint32_t* offsets = (uint32_t*)hc_device_ptr(w.offsets.get());
+        uint32_t* tmp = (uint32_t*)hc_device_ptr(w.tmp.get());
+        uint32_t* sorted = (uint32_t*)hc_device_ptr(w.sorted.get());
+        uint32_t* xoff = (uint32_t*)hc_device_ptr(w.xoff.get());
+        float* rpart = (float*)hc_device_ptr(w.rpart.get());
+        check(ctx_, hc::scan_exclusive(ctx_, counts, offsets, w.nbins, (uint32_t*)hc_device_ptr(w.partial.get())),
+              "scan");
+        check(ctx_, hc::launch_scatter(ctx_, a.key, a.slot, offsets, tmp, w.ntot), "scatter");
+        hipLaunchKernelGGL(hc::k_bm_bin_sort, dim3(hc::blocks_for(w.nbins + 1, 256)), dim3(256), 0, st, offsets, tmp,
+                           a, sorted, xoff, w.nbins);
+        hipLaunchKernelGGL(hc::k_bm_gather, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, a, offsets, tmp, sorted,
+                           (hc::CellRec32*)hc_device_ptr(w.recs.get()), rpart);
+        check(ctx_, hc::check_launch(ctx_, "mirror hash"), "mirror hash");
+        check(ctx_, hc::launch_rmax(ctx_, rpart, w.ntiles, (float*)hc_device_ptr(w.rmax.get())), "largest radius");
+    }
+
+    // every slot's state to its record position of the last step
+    void mirror_resort() {
+        Workspace& w = ws_;
+        hipStream_t st = hc_stream_of(ctx_);
+        const EnsSim* sims = (const EnsSim*)hc_device_ptr(w.sims.get());
+        const AgentTile* tiles = (const AgentTile*)hc_device_ptr(w.tiles.get());
+        const hc::BMirror a = mirror_set(0), b = mirror_set(1);
+        hipLaunchKernelGGL(hc::k_bm_permute, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, a, b);
+        hipLaunchKernelGGL(hc::k_bm_copy, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, a, b);
+        check(ctx_, hc::check_launch(ctx_, "mirror re-sort"), "mirror re-sort");
+        steps_since_sort_ = 0;
+    }
+
+    // the mirror from the cell fields, in the record order of the current
+    // positions; bins and slots for the coming step in counts
+    void mirror_load() {
+        Workspace& w = ws_;
+        hipStream_t st = hc_stream_of(ctx_);
+        const EnsSim* sims = (const EnsSim*)hc_device_ptr(w.sims.get());
+        const AgentTile* tiles = (const AgentTile*)hc_device_ptr(w.tiles.get());
+        uint32_t* counts = (uint32_t*)hc_device_ptr(w.counts.get());
+        const hc::BMirror a = mirror_set(0);
+        check(ctx_, hc::fill_u32(ctx_, counts, w.nbins, 0), "fill");
+        hipLaunchKernelGGL(hc::k_bm_load, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, a, counts);
+        check(ctx_, hc::check_launch(ctx_, "k_bm_load"), "mirror load");
+        mirror_hash(a);
+        mirror_resort();
+        mirror_valid_ = true;
+        soa_stale_ = false;
+    }
+
+    // the cell fields from the mirror, when a step has run since they were
+    // last written
+    void sync_soa() {
+        if (!soa_stale_) return;
+        soa_stale_ = false;
+        Workspace& w = ws_;
+        hipLaunchKernelGGL(hc::k_bm_sync, dim3(w.ntiles), dim3(256), 0, hc_stream_of(ctx_),
+                           (const EnsSim*)hc_device_ptr(w.sims.get()), (const AgentTile*)hc_device_ptr(w.tiles.get()),
+                           mirror_set(0));
+        check(ctx_, hc::check_launch(ctx_, "k_bm_sync"), "mirror write-back");
+    }
+
+    // one mechanics step on the mirror (bins and slots from the last AB2)
+    void mechanics_mirror() {
+        Workspace& w = ws_;
+        hipStream_t st = hc_stream_of(ctx_);
+        const EnsSim* sims = (const EnsSim*)hc_device_ptr(w.sims.get());
+        const AgentTile* tiles = (const AgentTile*)hc_device_ptr(w.tiles.get());
+        uint32_t* counts = (uint32_t*)hc_device_ptr(w.counts.get());
+        const hc::BMirror a = mirror_set(0);
+        {
+            hc::KernelTimer tm(ctx_, HC_K_HASH);
+            mirror_hash(a);
+        }
+        {
+            hc::KernelTimer tm(ctx_, HC_K_FORCES);
+            hipLaunchKernelGGL((hc::k_b_forces<true, true, true>), dim3(w.ntiles), dim3(256), 0, st, sims, tiles,
+                               hc_device_ptr(w.recs.get()), nullptr, (const uint32_t*)hc_device_ptr(w.offsets.get()),
+                               (const uint32_t*)hc_device_ptr(w.xoff.get()), a.key,
+                               (const float*)hc_device_ptr(w.rmax.get()), a.prop, a.v);
+            check(ctx_, hc::check_launch(ctx_, "k_b_forces (mirror)"), "forces");
+        }
+        {
+            hc::KernelTimer tm(ctx_, HC_K_INTEGRATE);
+            check(ctx_, hc::fill_u32(ctx_, counts, w.nbins, 0), "fill");
+            hipLaunchKernelGGL(hc::k_bm_integrate, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, a, counts);
+            check(ctx_, hc::check_launch(ctx_, "k_bm_integrate"), "integrate");
+        }
+    }
+
     void mechanics() {
         Workspace& w = ws_;
         if (!w.ntot) return;
+        if (mirror_on()) {
+            mechanics_mirror();
+            return;
+        }
         hipStream_t st = hc_stream_of(ctx_);
         const EnsSim* sims = (const EnsSim*)hc_device_ptr(w.sims.get());
         const AgentTile* tiles = (const AgentTile*)hc_device_ptr(w.tiles.get());
@@ -1240,10 +1705,10 @@ private:
             const bool list = w.ntot < (1u << hc::MARK32_POS_BITS) && w.nbins < (1u << 30);
             if (mixed_)
                 hipLaunchKernelGGL((list ? hc::k_b_forces<true, true> : hc::k_b_forces<true, false>), g, dim3(256), 0,
-                                   st, sims, tiles, recs, nullptr, offsets, xoff, kglob, rmax);
+                                   st, sims, tiles, recs, nullptr, offsets, xoff, kglob, rmax, nullptr, nullptr);
             else
                 hipLaunchKernelGGL(hc::k_b_forces<false>, g, dim3(256), 0, st, sims, tiles, recs, pos32, offsets,
-                                   nullptr, nullptr, rmax);
+                                   nullptr, nullptr, rmax, nullptr, nullptr);
             check(ctx_, hc::check_launch(ctx_, "k_b_forces"), "forces");
         }
         if (any_motility_) {
@@ -1354,6 +1819,7 @@ private:
                 if (m.pheno_acc >= m.cfg.dt_phenotype - 0.5 * dtd) {
                     m.pheno_steps++;
                     m.pheno_acc = 0.0;
+                    sync_soa();
                     check(ctx_, hc_voxel_keys(ctx_, m.soa.get(), &m.grid, m.n, m.max_cells), "voxel refresh");
                     refreshed = true;
                 }
@@ -1371,6 +1837,12 @@ private:
     hipEvent_t fork_ = nullptr, join_ = nullptr;
     bool overlap_ = false;
     bool mixed_ = false, dirty_ = true, plan_dirty_ = true, any_motility_ = false;
+    // agent mirror state: loaded, cell fields behind it, steps since its
+    // slots were last put in record order (re-sorted every
+    // MIRROR_RESORT_STEPS mechanics steps)
+    static constexpr int MIRROR_RESORT_STEPS = 64;
+    bool mirror_valid_ = false, soa_stale_ = false;
+    int steps_since_sort_ = 0;
     uint32_t next_id_ = 0;
     unsigned grid_blocks_[NLIST] = {};                 // 0: one workgroup per work item
     void* graph_ = nullptr;

@@ -92,12 +92,102 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan_add(uint32_t* __restrict__ out
     if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = partial[nb];
 }
 
+// Two-kernel scan for up to SCAN2_MAX_TILES tiles of 4096 values: tile sums,
+// then each tile adds the sum of the tiles before it (read by the whole
+// block from the partial sums) to its own exclusive scan.  Counts are read
+// twice and offsets written once, with no serial pass over the partials.
+static constexpr int SCAN2_PER = 16, SCAN2_TILE = SCAN_BS * SCAN2_PER;
+static constexpr uint32_t SCAN2_MAX_TILES = 2048;
+
+__device__ inline uint32_t block_sum(uint32_t v) {
+    __shared__ uint32_t ws[SCAN_BS / 64];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t t = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_BS / 64; k++) t += ws[k];
+    __syncthreads();
+    return t;
+}
+
+__device__ inline void scan2_load(const uint32_t* in, uint64_t base, uint64_t n, uint32_t* v) {
+    if (base + SCAN2_PER <= n) {
+#pragma unroll
+        for (int q = 0; q < SCAN2_PER / 4; q++) {
+            const uint4 w = *reinterpret_cast<const uint4*>(in + base + 4 * q);
+            v[4 * q] = w.x;
+            v[4 * q + 1] = w.y;
+            v[4 * q + 2] = w.z;
+            v[4 * q + 3] = w.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < SCAN2_PER; k++) v[k] = base + k < n ? in[base + k] : 0u;
+    }
+}
+
+__global__ __launch_bounds__(SCAN_BS) void k_scan2_reduce(const uint32_t* __restrict__ in, uint64_t n,
+                                                          uint32_t* __restrict__ partial) {
+    uint32_t v[SCAN2_PER];
+    scan2_load(in, (uint64_t)blockIdx.x * SCAN2_TILE + (uint64_t)threadIdx.x * SCAN2_PER, n, v);
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN2_PER; k++) sum += v[k];
+    const uint32_t tot = block_sum(sum);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// in and out may be the same array (in-place scans)
+__global__ __launch_bounds__(SCAN_BS) void k_scan2_down(const uint32_t* in, uint32_t* out,
+                                                        uint64_t n, const uint32_t* __restrict__ partial,
+                                                        uint32_t nb) {
+    uint32_t pre = 0;
+    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += SCAN_BS) pre += partial[i];
+    const uint32_t tile_base = block_sum(pre);
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN2_TILE + (uint64_t)threadIdx.x * SCAN2_PER;
+    uint32_t v[SCAN2_PER];
+    scan2_load(in, base, n, v);
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN2_PER; k++) sum += v[k];
+    uint32_t total;
+    uint32_t run = tile_base + block_exclusive_scan(sum, &total);
+    if (base + SCAN2_PER <= n) {
+#pragma unroll
+        for (int q = 0; q < SCAN2_PER / 4; q++) {
+            uint4 w;
+            w.x = run; run += v[4 * q];
+            w.y = run; run += v[4 * q + 1];
+            w.z = run; run += v[4 * q + 2];
+            w.w = run; run += v[4 * q + 3];
+            *reinterpret_cast<uint4*>(out + base + 4 * q) = w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < SCAN2_PER; k++) {
+            if (base + k < n) out[base + k] = run;
+            run += v[k];
+        }
+    }
+    if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[n] = tile_base + total;
+}
+
 size_t scan_partial_bytes(uint64_t n) {
     const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     return (size_t)(nb + 1) * sizeof(uint32_t);
 }
 
+// in and out hold n (+1 for out) values from 16-byte aligned addresses
 int scan_exclusive(hc_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* partial) {
+    const uint64_t nb2 = (n + SCAN2_TILE - 1) / SCAN2_TILE;
+    if (nb2 > 0 && nb2 <= SCAN2_MAX_TILES && ((uintptr_t)in & 15) == 0 && ((uintptr_t)out & 15) == 0) {
+        hipLaunchKernelGGL(k_scan2_reduce, dim3((uint32_t)nb2), dim3(SCAN_BS), 0, ctx->stream, in, n, partial);
+        hipLaunchKernelGGL(k_scan2_down, dim3((uint32_t)nb2), dim3(SCAN_BS), 0, ctx->stream, in, out, n, partial,
+                           (uint32_t)nb2);
+        return check_launch(ctx, "scan");
+    }
     const uint32_t nb = (uint32_t)((n + SCAN_TILE - 1) / SCAN_TILE);
     if (nb == 0) {
         hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(SCAN_BS), 0, ctx->stream, partial, 0u);
@@ -496,6 +586,7 @@ __global__ __launch_bounds__(256) void k_sec_apply(F* __restrict__ rho, const ui
 int probe_kernels_cells(hc_ctx* ctx) {
     hipFuncAttributes attr;
     const void* fns[] = {(const void*)k_scan_local, (const void*)k_scan_partials, (const void*)k_scan_add,
+                         (const void*)k_scan2_reduce, (const void*)k_scan2_down,
                          (const void*)k_fill_u32, (const void*)k_voxel_keys, (const void*)k_mech_keys,
                          (const void*)k_scatter, (const void*)k_stable_gather, (const void*)k_rmax, (const void*)k_forces,
                          (const void*)k_neighbors, (const void*)k_max_bin, (const void*)k_integrate,

@@ -434,47 +434,74 @@ __device__ __attribute__((always_inline)) inline void force_range64m(
 // walked whenever the next step could overflow it, and at the end, so the
 // wave runs the FP64 law about as often as its busiest lane has neighbours,
 // not once per chunk of candidates.
+// a 32-bit address in LDS (the per-lane mark lists)
+using lds_u32 = __attribute__((address_space(3))) uint32_t;
 constexpr double MARKX_MAX_EXTENT = 3968.0;   // x extent of the grid; plus 2 bins + cutoff < 4096
 constexpr float MARKX_SLACK = 0.05f;
 constexpr uint32_t MARKX_STEP = 8;            // candidates per step (their loads in flight together)
 constexpr uint32_t MARKX_CAP = 24;            // list entries per lane
 
+// The FP64 law of one listed pair (the expressions of force_range64):
+// xy = (x, y), zr = (z, r), ra = (rep, adh) of the candidate's record.
+__device__ __attribute__((always_inline)) inline void force_pair64(
+    const double2 xy, const double2 zr, const double2 ra, double px, double py, double pz, double radius_i,
+    double rep_i, double adh_i, double rel_max_adh, double relf, double cut2, double& fx, double& fy, double& fz,
+    double& local_pressure) {
+    constexpr double margin = 1.0 + 0x1p-40;
+    const double ddx = px - xy.x;
+    const double ddy = py - xy.y;
+    const double ddz = pz - zr.x;
+    const double dist_sq = ddx * ddx + ddy * ddy + ddz * ddz;
+    const double Lx = (radius_i + zr.y) * relf;
+    if (dist_sq >= cut2 || dist_sq < 1e-12 || !(dist_sq < Lx * Lx * margin)) return;
+    const double dist = sqrt(dist_sq);
+    const double inv_dist = 1.0 / dist;
+    const double nxv = ddx * inv_dist;
+    const double nyv = ddy * inv_dist;
+    const double nzv = ddz * inv_dist;
+    const double R_sum = radius_i + zr.y;
+    double force_mag = 0.0;
+    if (dist < R_sum) {
+        const double overlap = 1.0 - dist / R_sum;
+        force_mag += overlap * overlap * sqrt(rep_i * ra.x);
+        local_pressure += overlap * overlap / 0.027288820670331;
+    }
+    const double adhesion_distance = rel_max_adh * R_sum;
+    if (dist < adhesion_distance) {
+        const double t2 = 1.0 - dist / adhesion_distance;
+        force_mag -= t2 * t2 * sqrt(adh_i * ra.y);
+    }
+    fx += force_mag * nxv;
+    fy += force_mag * nyv;
+    fz += force_mag * nzv;
+}
+
+// The law over a lane's list of marked candidates (record positions), in
+// list order, LAW_BATCH entries at a time: the records of a batch (48 of
+// their 64 bytes each) are loaded together, then the pairs run in order.
+constexpr uint32_t LAW_BATCH = 4;
 __device__ __attribute__((always_inline)) inline void force_list64(
     const CellRec* __restrict__ rec, const uint32_t* __restrict__ list, uint32_t cnt, double px, double py,
     double pz, double radius_i, double rep_i, double adh_i, double rel_max_adh, double cut2, double& fx, double& fy,
     double& fz, double& local_pressure) {
     const double relf = rel_max_adh > 1.0 ? rel_max_adh : 1.0;
-    constexpr double margin = 1.0 + 0x1p-40;
-    for (uint32_t k = 0; k < cnt; k++) {
-        const uint32_t u = list[k * 64];
-        const double2 xy = *reinterpret_cast<const double2*>(&rec[u].x);
-        const double2 zr = *reinterpret_cast<const double2*>(&rec[u].z);
-        const double ddx = px - xy.x;
-        const double ddy = py - xy.y;
-        const double ddz = pz - zr.x;
-        const double dist_sq = ddx * ddx + ddy * ddy + ddz * ddz;
-        const double Lx = (radius_i + zr.y) * relf;
-        if (dist_sq >= cut2 || dist_sq < 1e-12 || !(dist_sq < Lx * Lx * margin)) continue;
-        const double dist = sqrt(dist_sq);
-        const double inv_dist = 1.0 / dist;
-        const double nxv = ddx * inv_dist;
-        const double nyv = ddy * inv_dist;
-        const double nzv = ddz * inv_dist;
-        const double R_sum = radius_i + zr.y;
-        double force_mag = 0.0;
-        if (dist < R_sum) {
-            const double overlap = 1.0 - dist / R_sum;
-            force_mag += overlap * overlap * sqrt(rep_i * rec[u].rep);
-            local_pressure += overlap * overlap / 0.027288820670331;
+    for (uint32_t k0 = 0; k0 < cnt; k0 += LAW_BATCH) {
+        uint32_t u[LAW_BATCH];
+        u[0] = list[k0 * 64];
+#pragma unroll
+        for (uint32_t j = 1; j < LAW_BATCH; j++) u[j] = k0 + j < cnt ? list[(k0 + j) * 64] : u[0];
+        double2 xy[LAW_BATCH], zr[LAW_BATCH], ra[LAW_BATCH];
+#pragma unroll
+        for (uint32_t j = 0; j < LAW_BATCH; j++) {
+            xy[j] = *reinterpret_cast<const double2*>(&rec[u[j]].x);
+            zr[j] = *reinterpret_cast<const double2*>(&rec[u[j]].z);
+            ra[j] = *reinterpret_cast<const double2*>(&rec[u[j]].rep);
         }
-        const double adhesion_distance = rel_max_adh * R_sum;
-        if (dist < adhesion_distance) {
-            const double t2 = 1.0 - dist / adhesion_distance;
-            force_mag -= t2 * t2 * sqrt(adh_i * rec[u].adh);
-        }
-        fx += force_mag * nxv;
-        fy += force_mag * nyv;
-        fz += force_mag * nzv;
+#pragma unroll
+        for (uint32_t j = 0; j < LAW_BATCH; j++)
+            if (k0 + j < cnt)
+                force_pair64(xy[j], zr[j], ra[j], px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, relf, cut2, fx, fy,
+                             fz, local_pressure);
     }
 }
 
@@ -493,13 +520,18 @@ __device__ __attribute__((always_inline)) inline void force_agent64x(
     const float relf32 = (float)(rel_max_adh > 1.0 ? rel_max_adh : 1.0);
     const float L = ((float)radius_i + rmax) * relf32;
     const float lim = fminf(L * L * (1.0f + 0x1p-16f), (float)cut2) + MARKX_SLACK;
-    uint32_t cnt = 0;
+    // w: the next list entry (an LDS address: 32-bit arithmetic); every
+    // candidate is written there and w moves on past the marked ones
+    lds_u32* const l0 = (lds_u32*)list;
+    lds_u32* w = l0;
+    lds_u32* const wflush = l0 + (MARKX_CAP - MARKX_STEP) * 64;
 #pragma unroll 1
     for (int q = 0; q < 9; q++) {
         const RowRange rr = neighbor_row(offsets, bin0, vi, vj, vk, mnx, mny, mnz, q);
         const float my = m.y - (float)rr.dj * voxel, mz = m.z - (float)rr.dk * voxel;
         for (uint32_t u0 = rr.beg; u0 < rr.end; u0 += MARKX_STEP) {
             const float4* p = pos32 + u0;
+            const uint32_t rem = rr.end - u0;
             float4 o[MARKX_STEP];
 #pragma unroll
             for (uint32_t j = 0; j < MARKX_STEP; j++) o[j] = p[j];
@@ -507,17 +539,18 @@ __device__ __attribute__((always_inline)) inline void force_agent64x(
             for (uint32_t j = 0; j < MARKX_STEP; j++) {
                 const float ddx = m.x - o[j].x, ddy = my - o[j].y, ddz = mz - o[j].z;
                 const float d2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(ddy, ddy, ddx * ddx));
-                list[cnt * 64] = u0 + j;
-                cnt += (u0 + j < rr.end && d2 < lim) ? 1u : 0u;
+                *w = u0 + j;
+                w += (j < rem && d2 < lim) ? 64 : 0;
             }
-            if (cnt > MARKX_CAP - MARKX_STEP) {
-                force_list64(rec, list, cnt, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz,
-                             local_pressure);
-                cnt = 0;
+            if (w > wflush) {
+                force_list64(rec, list, (uint32_t)(w - l0) / 64u, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh,
+                             cut2, fx, fy, fz, local_pressure);
+                w = l0;
             }
         }
     }
-    force_list64(rec, list, cnt, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz, local_pressure);
+    force_list64(rec, list, (uint32_t)(w - l0) / 64u, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx,
+                 fy, fz, local_pressure);
 }
 
 // ---------------------------------------------------------------------------
@@ -585,7 +618,9 @@ __device__ __attribute__((always_inline)) inline void force_range32(
     }
 }
 
-// The law over a lane's list of marked candidates, in list order.  Entry:
+// The law over a lane's list of marked candidates, in list order (loading
+// a batch of records together, as force_list64 does, measured slower
+// here).  Entry:
 // the record position in the low MARK32_POS_BITS bits and the row code
 // (dk + 1) * 3 + (dj + 1) above them.  The candidate's column comes from its
 // record: xg - x is its bin corner's distance from x_min, a whole number of
@@ -600,8 +635,8 @@ __device__ __attribute__((always_inline)) inline void force_list32(
         const uint32_t e = list[k * 64];
         const uint32_t u = e & ((1u << MARK32_POS_BITS) - 1u), code = e >> MARK32_POS_BITS;
         if (u == t) continue;
-        const float4 a = reinterpret_cast<const float4*>(rec + u)[0];
-        const float4 b = reinterpret_cast<const float4*>(rec + u)[1];
+        const float4 a = reinterpret_cast<const float4*>(rec + u)[0];   // xg, y, z, x
+        const float4 b = reinterpret_cast<const float4*>(rec + u)[1];   // r, rep, adh, idx
         const int di = (int)rintf((a.x - a.w) * inv_voxel) - vi;
         const float ox = (float)di * voxel;
         const float oy = (float)((int)(code % 3u) - 1) * voxel, oz = (float)((int)(code / 3u) - 1) * voxel;
@@ -653,7 +688,11 @@ __device__ __attribute__((always_inline)) inline void force_agent32x(
     const uint32_t jlo = XSLABS * (left ? vi - 1 : vi) + xslab_of(left ? lo + voxel : lo, scale);
     const uint32_t jhi = XSLABS * (right ? vi + 1 : vi) + xslab_of(right ? hi - voxel : hi, scale) + 1;
     const bool three_d = mnz > 1;
-    uint32_t cnt = 0;
+    // w: the next list entry (an LDS address: 32-bit arithmetic); every
+    // candidate is written there and w moves on past the marked ones
+    lds_u32* const l0 = (lds_u32*)list;
+    lds_u32* w = l0;
+    lds_u32* const wflush = l0 + (MARKX_CAP - MARK32_STEP) * 64;
 #pragma unroll 1
     for (int q = 0; q < 9; q++) {
         const int dk = three_d ? q / 3 - 1 : 0, dj = q % 3 - 1;
@@ -665,6 +704,8 @@ __device__ __attribute__((always_inline)) inline void force_agent32x(
         const uint32_t qc = (uint32_t)((dk + 1) * 3 + (dj + 1)) << MARK32_POS_BITS;
         const CellRec32* p = rec + beg;
         for (uint32_t u0 = beg; u0 < end; u0 += MARK32_STEP, p += MARK32_STEP) {
+            // entries u0 + j with the row code above them (u0 + j < 2^27)
+            const uint32_t eb = u0 + qc, rem = end - u0;
             float4 o[MARK32_STEP];
 #pragma unroll
             for (uint32_t j = 0; j < MARK32_STEP; j++) o[j] = *reinterpret_cast<const float4*>(p + j);
@@ -672,18 +713,18 @@ __device__ __attribute__((always_inline)) inline void force_agent32x(
             for (uint32_t j = 0; j < MARK32_STEP; j++) {
                 const float ddx = mxg - o[j].x, ddy = myr - o[j].y, ddz = mzr - o[j].z;
                 const float d2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(ddy, ddy, ddx * ddx));
-                list[cnt * 64] = (u0 + j) | qc;
-                cnt += (u0 + j < end && d2 < lim) ? 1u : 0u;
+                *w = eb + j;
+                w += (j < rem && d2 < lim) ? 64 : 0;
             }
-            if (cnt > MARKX_CAP - MARK32_STEP) {
-                force_list32(rec, list, cnt, t, (int)vi, mx, my, mz, voxel, inv_voxel, radius_i, rep_i, adh_i,
-                             rel_max_adh, cut2, fx, fy, fz, local_pressure);
-                cnt = 0;
+            if (w > wflush) {
+                force_list32(rec, list, (uint32_t)(w - l0) / 64u, t, (int)vi, mx, my, mz, voxel, inv_voxel,
+                             radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz, local_pressure);
+                w = l0;
             }
         }
     }
-    force_list32(rec, list, cnt, t, (int)vi, mx, my, mz, voxel, inv_voxel, radius_i, rep_i, adh_i, rel_max_adh, cut2,
-                 fx, fy, fz, local_pressure);
+    force_list32(rec, list, (uint32_t)(w - l0) / 64u, t, (int)vi, mx, my, mz, voxel, inv_voxel, radius_i, rep_i,
+                 adh_i, rel_max_adh, cut2, fx, fy, fz, local_pressure);
 }
 
 // The chunked pass over the 9 (3 in 2-D) neighbour rows of an agent.
@@ -717,6 +758,10 @@ __device__ inline void bin_sort_slabs(const uint32_t* __restrict__ offsets, cons
     }
     const uint32_t beg = offsets[b], cnt = offsets[b + 1] - beg;
     uint32_t c0 = 0, c1 = 0, c2 = 0;
+    if (cnt == 0) {
+        *reinterpret_cast<uint4*>(xoff + (size_t)XSLABS * b) = make_uint4(beg, beg, beg, beg);
+        return;
+    }
     if (cnt > BIN_SORT_CAP) {
         for (uint32_t p = beg; p < beg + cnt; p++) {
             const uint32_t s = xslab[tmp[p]];
