@@ -389,3 +389,55 @@ def test_overlapped_periods_equal_serial(mixed):
         ens.close()
     for a, b, (sc, _) in zip(out[0], out[1], members):
         assert_same(b, a, f"member {sc.name} overlapped, mixed={mixed}")
+
+
+def test_mixed_mirror_resort_and_writeback_equal_single():
+    """The mixed ensemble keeps its mechanics state in an agent mirror in
+    record order, re-sorted every 64 mechanics steps, and writes the cell
+    fields back only when they are read.  Reads in the middle of the run
+    (fields after 50 periods, a neighbour query after 90), phenotype
+    refreshes and two re-sorts must leave every member equal to its single
+    simulation at each point, bit for bit, previous velocities, pressure and
+    the mechanics voxel included."""
+    extra = (hip.F_PREV_VX, hip.F_PREV_VY, hip.F_PREV_VZ, hip.F_PRESSURE, hip.F_MECH_VOXEL)
+    members = [small_member(260, dt_pheno=1.0), small_member(261, dt_pheno=2.0)]
+    ens = hip.Ensemble(mixed=True)
+    ids = [ens.add(sc, c) for sc, c in members]
+    sims = [hip.Simulation(sc, c, mixed=True) for sc, c in members]
+
+    def advance(periods, use_graph):
+        ens.run_periods(periods, use_graph=use_graph)
+        ens.wait()
+        for sim, (sc, c) in zip(sims, members):
+            k = substeps_until(sc.dt_diff, sc.dt_mech)
+            kp = substeps_until(sc.dt_diff, sc.dt_pheno)
+            for _ in range(periods):
+                for _ in range(k):
+                    sim.substep()
+                sim.mech_step()
+                sim.mech_periods = getattr(sim, "mech_periods", 0) + 1
+                if (sim.mech_periods * k) % kp == 0:
+                    sim.pheno_step()
+            sim.wait()
+
+    def compare(when):
+        for mid, sim in zip(ids, sims):
+            a = {"rho": ens.density(mid)}
+            b = {"rho": sim.density()}
+            for f in POS + VEL + extra:
+                a[f] = ens.field(mid, f)
+                b[f] = sim.field(f)
+            assert_same(a, b, f"member {mid} {when}")
+
+    advance(50, True)
+    compare("after 50 periods")
+    advance(40, True)
+    for mid, sim in zip(ids, sims):
+        qa = ens.neighbor_query(mid, hip.QUERY_CONTACT)
+        qb = sim.neighbor_query(hip.QUERY_CONTACT)
+        assert all(np.array_equal(x, y) for x, y in zip(qa, qb)), f"member {mid}: contact lists after 90 periods"
+    advance(50, False)
+    compare("after 140 periods")
+    for sim in sims:
+        sim.close()
+    ens.close()
