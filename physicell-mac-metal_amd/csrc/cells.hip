@@ -1275,79 +1275,86 @@ int hc_substrate_totals_noalloc(hc_ctx* ctx, hc_buf* density, const hc_grid* g, 
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// Hash-ordered mechanics mirror of one simulation (FP64 forces).
+// Mechanics mirror of one simulation (FP64 forces).
 //
 // The agents' order in the cell SoA (their index) need not follow space, so
 // every per-agent pass of the mechanics step in index order scatters: the
 // ballot of the key pass finds about one key per lane (one atomic each), the
 // gather reads each agent's fields from unrelated cache lines, and the force
 // pass stores velocity and pressure at the agent's index.  The mirror keeps
-// the mechanics state -- the 64-byte records (position, radius, repulsion,
-// adhesion, adhesion reach, index, key) and the previous velocities -- in the
-// order of the last hash build, where it stays after AB2.  The next step's
-// keys are then computed in nearly sorted order, the new hash order is a
-// near-identity permutation of the mirror, the forces are written in hash
-// order, and AB2 reads and writes in that order too.  The cell fields
-// (position, velocity, previous velocity, pressure, mechanics voxel) are
-// written back, by agent index, only when something else reads them
-// (mirror_sync).  Keys, bins, the in-bin order (ascending agent index), the
-// records and every arithmetic step are those of the direct path, so the
-// results are the same bit for bit.
+// each agent's mechanics state -- position, previous velocity, radius,
+// repulsion, adhesion, relative adhesion distance (FP64), index -- in slots
+// ordered like the records of a recent hash build.  Slots keep their order
+// from step to step (mirror_resort puts them back in record order as agents
+// change bins); each slot's bin and place in the bin for the coming step are
+// computed by AB2 itself, with coherent ballots, and each slot knows its
+// record position of the step (rpos), through which AB2 reads its forces.
+// Per step the gather reads 68 bytes of slot state and writes the 64-byte
+// record, AB2 reads and writes 64 bytes of state beside its 32-byte forces.
+// The cell fields (position, velocity, previous velocity, pressure,
+// mechanics voxel) are written back, by agent index, only when something
+// else reads them (mirror_sync).  Keys, bins, the in-bin order (ascending
+// agent index), the records and every arithmetic step are those of the
+// direct path, so the results are the same bit for bit.
 // ---------------------------------------------------------------------------
 namespace hc {
 
 MirrorLayout mirror_layout(uint32_t max_cells) {
     MirrorLayout L;
     size_t o = 0;
-    L.recA = o;  o = align256(o + (size_t)max_cells * sizeof(CellRec));
-    L.prevA = o; o = align256(o + (size_t)max_cells * sizeof(double4));
-    L.prevB = o; o = align256(o + (size_t)max_cells * sizeof(double4));
-    L.vp = o;    o = align256(o + (size_t)max_cells * sizeof(double4));
-    L.mkey = o;  o = align256(o + (size_t)max_cells * 4);
+    for (int k = 0; k < 2; k++) {
+        L.pos[k] = o;   o = align256(o + (size_t)max_cells * sizeof(double4));
+        L.prev[k] = o;  o = align256(o + (size_t)max_cells * sizeof(double4));
+        L.prop[k] = o;  o = align256(o + (size_t)max_cells * sizeof(double4));
+        L.idx[k] = o;   o = align256(o + (size_t)max_cells * 4);
+        L.key[k] = o;   o = align256(o + (size_t)max_cells * 4);
+        L.slot[k] = o;  o = align256(o + (size_t)max_cells * 4);
+        L.kstep[k] = o; o = align256(o + (size_t)max_cells * 4);
+        L.rpos[k] = o;  o = align256(o + (size_t)max_cells * 4);
+    }
+    L.vp = o;      o = align256(o + (size_t)max_cells * sizeof(double4));
     L.total = o;
     return L;
 }
 
-// mechanics keys (shaders/mechanics.metal:61-72) of the mirror's agents
-__global__ __launch_bounds__(256) void k_m_keys(const CellRec* __restrict__ recA, uint32_t* __restrict__ mkey,
-                                                uint32_t* __restrict__ counts, uint32_t* __restrict__ slot, uint32_t n,
-                                                double x_min, double y_min, double z_min, double voxel, uint32_t mnx,
-                                                uint32_t mny, uint32_t mnz) {
-    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
-    const bool valid = t < n;
-    uint32_t key = 0;
-    if (valid) {
-        const double2 xy = *reinterpret_cast<const double2*>(&recA[t].x);
-        const double z = recA[t].z;
-        int vi = (int)((xy.x - x_min) / voxel);
-        int vj = (int)((xy.y - y_min) / voxel);
-        int vk = (int)((z - z_min) / voxel);
-        vi = vi < 0 ? 0 : (vi > (int)mnx - 1 ? (int)mnx - 1 : vi);
-        vj = vj < 0 ? 0 : (vj > (int)mny - 1 ? (int)mny - 1 : vj);
-        vk = vk < 0 ? 0 : (vk > (int)mnz - 1 ? (int)mnz - 1 : vk);
-        key = (uint32_t)vk * mny * mnx + (uint32_t)vj * mnx + (uint32_t)vi;
-        mkey[t] = key;
-    }
-    const uint32_t s = ballot_bin(valid, key, counts);
-    if (valid) slot[t] = s;
+// one set of slot arrays (set 1 receives a re-sort)
+struct MSet {
+    double4 *pos, *prev, *prop;     // prop: radius, repulsion, adhesion, relative max adhesion distance
+    uint32_t *idx, *key, *slot, *kstep, *rpos;
+};
+
+// the slots from a hash built on the cell fields, in its record order: state,
+// bin and place in the bin (the hash's counts hold the histogram)
+__global__ __launch_bounds__(256) void k_m_load(const CellRec* __restrict__ rec, const uint32_t* __restrict__ offsets,
+                                                CellPtrs cp, MSet a, uint32_t n) {
+    const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= n) return;
+    const CellRec r = rec[q];
+    a.pos[q] = make_double4(r.x, r.y, r.z, 0.0);
+    a.prev[q] = make_double4(cp.f[HC_F_PREV_VX][r.idx], cp.f[HC_F_PREV_VY][r.idx], cp.f[HC_F_PREV_VZ][r.idx], 0.0);
+    a.prop[q] = make_double4(r.r, r.rep, r.adh, r.rel);
+    a.idx[q] = r.idx;
+    a.key[q] = r.key;
+    a.slot[q] = q - offsets[r.key];
+    a.kstep[q] = r.key;
+    a.rpos[q] = q;
 }
 
-// Bins of up to BIN_SORT_CAP members: the members (mirror slots) ordered by
+// Bins of up to BIN_SORT_CAP members (slots in tmp): the members ordered by
 // agent index (a bitonic network over (index, slot) pairs); writes the bin's
-// agent list and each slot's new position.
+// agent list and each slot's record position.
 __global__ __launch_bounds__(256) void k_m_bin_sort(const uint32_t* __restrict__ offsets,
-                                                    const uint32_t* __restrict__ tmp,
-                                                    const CellRec* __restrict__ recA, uint32_t* __restrict__ sorted,
-                                                    uint32_t* __restrict__ pos_of, uint32_t nbins) {
+                                                    const uint32_t* __restrict__ tmp, MSet a,
+                                                    uint32_t* __restrict__ sorted, uint32_t nbins) {
     const uint32_t b = blockIdx.x * 256 + threadIdx.x;
     if (b >= nbins) return;
     const uint32_t beg = offsets[b], cnt = offsets[b + 1] - beg;
     if (cnt == 0 || cnt > BIN_SORT_CAP) return;
-    uint64_t a[BIN_SORT_CAP];
+    uint64_t v[BIN_SORT_CAP];
 #pragma unroll
     for (uint32_t q = 0; q < BIN_SORT_CAP; q++) {
         const uint32_t m = q < cnt ? tmp[beg + q] : 0u;
-        a[q] = q < cnt ? ((uint64_t)recA[m].idx << 32) | m : ~0ull;
+        v[q] = q < cnt ? ((uint64_t)a.idx[m] << 32) | m : ~0ull;
     }
 #pragma unroll
     for (uint32_t k = 2; k <= BIN_SORT_CAP; k <<= 1)
@@ -1357,100 +1364,116 @@ __global__ __launch_bounds__(256) void k_m_bin_sort(const uint32_t* __restrict__
             for (uint32_t i = 0; i < BIN_SORT_CAP; i++) {
                 const uint32_t l = i ^ j;
                 if (l > i) {
-                    const uint64_t lo = a[i] < a[l] ? a[i] : a[l], hi = a[i] < a[l] ? a[l] : a[i];
+                    const uint64_t lo = v[i] < v[l] ? v[i] : v[l], hi = v[i] < v[l] ? v[l] : v[i];
                     const bool up = (i & k) == 0;
-                    a[i] = up ? lo : hi;
-                    a[l] = up ? hi : lo;
+                    v[i] = up ? lo : hi;
+                    v[l] = up ? hi : lo;
                 }
             }
 #pragma unroll
     for (uint32_t q = 0; q < BIN_SORT_CAP; q++)
         if (q < cnt) {
-            sorted[beg + q] = (uint32_t)(a[q] >> 32);
-            pos_of[(uint32_t)a[q]] = beg + q;
+            sorted[beg + q] = (uint32_t)(v[q] >> 32);
+            a.rpos[(uint32_t)v[q]] = beg + q;
         }
 }
 
-// The mirror in the new hash order: records (with their new key), previous
-// velocities and the FP32 mark positions, as k_stable_gather builds them from
-// the cell fields.  Members of bins above BIN_SORT_CAP are ranked by agent
-// index here.
-__global__ __launch_bounds__(256) void k_m_gather(const CellRec* __restrict__ recA, const double4* __restrict__ prevA,
-                                                  const uint32_t* __restrict__ mkey,
-                                                  const uint32_t* __restrict__ offsets,
-                                                  const uint32_t* __restrict__ tmp,
-                                                  const uint32_t* __restrict__ pos_of, uint32_t* __restrict__ sorted,
-                                                  CellRec* __restrict__ recB, double4* __restrict__ prevB,
-                                                  float4* __restrict__ pos32, uint32_t n, double x_min, double y_min,
-                                                  double z_min, double voxel, uint32_t mnx, uint32_t mny, int x_grid,
-                                                  float* __restrict__ rpart) {
-    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
-    if (rpart) block_max_to(t < n ? __double2float_ru(recA[t].r) : 0.0f, rpart);
-    if (t >= n) return;
-    CellRec r = recA[t];
-    const uint32_t b = mkey[t];
+// This step's records (k_stable_gather's values) and FP32 mark positions
+// from the slots; slots of bins above BIN_SORT_CAP are ranked here by agent
+// index.
+__global__ __launch_bounds__(256) void k_m_gather(MSet a, const uint32_t* __restrict__ offsets,
+                                                  const uint32_t* __restrict__ tmp, uint32_t* __restrict__ sorted,
+                                                  CellRec* __restrict__ rec, float4* __restrict__ pos32, uint32_t n,
+                                                  double x_min, double y_min, double z_min, double voxel,
+                                                  uint32_t mnx, uint32_t mny, int x_grid, float* __restrict__ rpart) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    const double4 pr = p < n ? a.prop[p] : make_double4(0.0, 0.0, 0.0, 0.0);
+    if (rpart) block_max_to(__double2float_ru(pr.x), rpart);
+    if (p >= n) return;
+    const uint32_t b = a.key[p], id = a.idx[p];
     const uint32_t beg = offsets[b], end = offsets[b + 1];
     uint32_t pos;
     if (end - beg <= BIN_SORT_CAP) {
-        pos = pos_of[t];                                  // placed by k_m_bin_sort
+        pos = a.rpos[p];                                  // placed by k_m_bin_sort
     } else {
         uint32_t rank = 0;
         for (uint32_t u0 = beg; u0 < end; u0 += 8) {
-            uint32_t v[8];
+            uint32_t w[8];
 #pragma unroll
-            for (int q = 0; q < 8; q++) v[q] = recA[tmp[u0 + q < end ? u0 + q : end - 1]].idx;
+            for (int q = 0; q < 8; q++) w[q] = a.idx[tmp[u0 + q < end ? u0 + q : end - 1]];
 #pragma unroll
-            for (int q = 0; q < 8; q++) rank += (u0 + q < end && v[q] < r.idx) ? 1u : 0u;
+            for (int q = 0; q < 8; q++) rank += (u0 + q < end && w[q] < id) ? 1u : 0u;
         }
         pos = beg + rank;
-        sorted[pos] = r.idx;
+        sorted[pos] = id;
+        a.rpos[p] = pos;
     }
+    a.kstep[p] = b;
+    const double4 x = a.pos[p];
     if (pos32) {
         const uint32_t bi = b % mnx, bj = (b / mnx) % mny, bk = b / (mnx * mny);
-        pos32[pos] = make_float4((float)(r.x - (x_grid ? x_min : x_min + bi * voxel)),
-                                 (float)(r.y - (y_min + bj * voxel)), (float)(r.z - (z_min + bk * voxel)),
-                                 (float)r.r);
+        pos32[pos] = make_float4((float)(x.x - (x_grid ? x_min : x_min + bi * voxel)),
+                                 (float)(x.y - (y_min + bj * voxel)), (float)(x.z - (z_min + bk * voxel)),
+                                 (float)pr.x);
     }
+    CellRec r;
+    r.x = x.x;
+    r.y = x.y;
+    r.z = x.z;
+    r.r = pr.x;
+    r.rep = pr.y;
+    r.adh = pr.z;
+    r.rel = pr.w;
+    r.idx = id;
     r.key = b;
-    recB[pos] = r;
-    prevB[pos] = prevA[t];
+    rec[pos] = r;
 }
 
-// AB2 (shaders/integrate.metal:56-79) in hash order: this step's records
-// (recB), forces (vp) and previous velocities (prevB) to the mirror (recA,
-// prevA), in the same order.
-__global__ __launch_bounds__(256) void k_m_integrate(const CellRec* __restrict__ recB,
-                                                     const double4* __restrict__ prevB,
-                                                     const double4* __restrict__ vp, CellRec* __restrict__ recA,
-                                                     double4* __restrict__ prevA, uint32_t n, double dt,
+// AB2 (shaders/integrate.metal:56-79) on the slots, with this step's forces
+// by record position; then the bin (shaders/mechanics.metal:61-72) and the
+// place in it of the new position for the coming step (counts must be zero)
+__global__ __launch_bounds__(256) void k_m_integrate(MSet a, const double4* __restrict__ vp,
+                                                     uint32_t* __restrict__ counts, uint32_t n, double dt,
                                                      double x_min, double y_min, double z_min, double x_max,
-                                                     double y_max, double z_max) {
-    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= n) return;
-    CellRec r = recB[t];
-    const double4 v = vp[t], pv = prevB[t];
-    double px = r.x, py = r.y, pz = r.z;
-    px += dt * (1.5 * v.x - 0.5 * pv.x);
-    py += dt * (1.5 * v.y - 0.5 * pv.y);
-    pz += dt * (1.5 * v.z - 0.5 * pv.z);
-    r.x = px < x_min ? x_min : (px > x_max ? x_max : px);
-    r.y = py < y_min ? y_min : (py > y_max ? y_max : py);
-    r.z = pz < z_min ? z_min : (pz > z_max ? z_max : pz);
-    recA[t] = r;
-    prevA[t] = make_double4(v.x, v.y, v.z, 0.0);
+                                                     double y_max, double z_max, double mx_min, double my_min,
+                                                     double mz_min, double voxel, uint32_t mnx, uint32_t mny,
+                                                     uint32_t mnz) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = p < n;
+    uint32_t key = 0;
+    if (valid) {
+        const double4 v = vp[a.rpos[p]], pv = a.prev[p], x = a.pos[p];
+        double px = x.x, py = x.y, pz = x.z;
+        px += dt * (1.5 * v.x - 0.5 * pv.x);
+        py += dt * (1.5 * v.y - 0.5 * pv.y);
+        pz += dt * (1.5 * v.z - 0.5 * pv.z);
+        px = px < x_min ? x_min : (px > x_max ? x_max : px);
+        py = py < y_min ? y_min : (py > y_max ? y_max : py);
+        pz = pz < z_min ? z_min : (pz > z_max ? z_max : pz);
+        a.pos[p] = make_double4(px, py, pz, 0.0);
+        a.prev[p] = make_double4(v.x, v.y, v.z, 0.0);
+        int vi = (int)((px - mx_min) / voxel);
+        int vj = (int)((py - my_min) / voxel);
+        int vk = (int)((pz - mz_min) / voxel);
+        vi = vi < 0 ? 0 : (vi > (int)mnx - 1 ? (int)mnx - 1 : vi);
+        vj = vj < 0 ? 0 : (vj > (int)mny - 1 ? (int)mny - 1 : vj);
+        vk = vk < 0 ? 0 : (vk > (int)mnz - 1 ? (int)mnz - 1 : vk);
+        key = (uint32_t)vk * mny * mnx + (uint32_t)vj * mnx + (uint32_t)vi;
+        a.key[p] = key;
+    }
+    const uint32_t s = ballot_bin(valid, key, counts);
+    if (valid) a.slot[p] = s;
 }
 
-// the cell fields from the mirror, by agent index
-__global__ __launch_bounds__(256) void k_m_sync(const CellRec* __restrict__ recA, const double4* __restrict__ prevA,
-                                                const double4* __restrict__ vp, CellPtrs cp, uint32_t n) {
-    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= n) return;
-    const CellRec r = recA[t];
-    const double4 v = vp[t], pv = prevA[t];
-    const uint32_t c = r.idx;
-    cp.f[HC_F_POS_X][c] = r.x;
-    cp.f[HC_F_POS_Y][c] = r.y;
-    cp.f[HC_F_POS_Z][c] = r.z;
+// the cell fields from the slots, by agent index
+__global__ __launch_bounds__(256) void k_m_sync(MSet a, const double4* __restrict__ vp, CellPtrs cp, uint32_t n) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t c = a.idx[p];
+    const double4 x = a.pos[p], pv = a.prev[p], v = vp[a.rpos[p]];
+    cp.f[HC_F_POS_X][c] = x.x;
+    cp.f[HC_F_POS_Y][c] = x.y;
+    cp.f[HC_F_POS_Z][c] = x.z;
     cp.f[HC_F_VEL_X][c] = v.x;
     cp.f[HC_F_VEL_Y][c] = v.y;
     cp.f[HC_F_VEL_Z][c] = v.z;
@@ -1458,24 +1481,42 @@ __global__ __launch_bounds__(256) void k_m_sync(const CellRec* __restrict__ recA
     cp.f[HC_F_PREV_VX][c] = pv.x;
     cp.f[HC_F_PREV_VY][c] = pv.y;
     cp.f[HC_F_PREV_VZ][c] = pv.z;
-    cp.u[HC_F_MECH_VOXEL - HC_NUM_F64_FIELDS][c] = r.key;
+    cp.u[HC_F_MECH_VOXEL - HC_NUM_F64_FIELDS][c] = a.kstep[p];
 }
 
-// the mirror from a hash built on the cell fields: the hash's records and
-// each agent's previous velocity, in hash order
-__global__ __launch_bounds__(256) void k_m_load(const CellRec* __restrict__ recB, CellPtrs cp,
-                                                CellRec* __restrict__ recA, double4* __restrict__ prevA, uint32_t n) {
-    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= n) return;
-    const CellRec r = recB[t];
-    recA[t] = r;
-    prevA[t] = make_double4(cp.f[HC_F_PREV_VX][r.idx], cp.f[HC_F_PREV_VY][r.idx], cp.f[HC_F_PREV_VZ][r.idx], 0.0);
+// re-sort: each slot's state to its record position of the last step (set
+// b), then back into set a slot by slot
+__global__ __launch_bounds__(256) void k_m_permute(MSet a, MSet b, uint32_t n) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t q = a.rpos[p];
+    b.pos[q] = a.pos[p];
+    b.prev[q] = a.prev[p];
+    b.prop[q] = a.prop[p];
+    b.idx[q] = a.idx[p];
+    b.key[q] = a.key[p];
+    b.slot[q] = a.slot[p];
+    b.kstep[q] = a.kstep[p];
+    b.rpos[q] = q;
+}
+__global__ __launch_bounds__(256) void k_m_copy(MSet a, MSet b, uint32_t n) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    a.pos[p] = b.pos[p];
+    a.prev[p] = b.prev[p];
+    a.prop[p] = b.prop[p];
+    a.idx[p] = b.idx[p];
+    a.key[p] = b.key[p];
+    a.slot[p] = b.slot[p];
+    a.kstep[p] = b.kstep[p];
+    a.rpos[p] = b.rpos[p];
 }
 
 int probe_kernels_mirror(hc_ctx* ctx) {
     hipFuncAttributes attr;
-    const void* fns[] = {(const void*)k_m_keys, (const void*)k_m_bin_sort, (const void*)k_m_gather,
-                         (const void*)k_m_integrate, (const void*)k_m_sync, (const void*)k_m_load};
+    const void* fns[] = {(const void*)k_m_load, (const void*)k_m_bin_sort, (const void*)k_m_gather,
+                         (const void*)k_m_integrate, (const void*)k_m_sync, (const void*)k_m_permute,
+                         (const void*)k_m_copy};
     for (const void* f : fns) {
         hipError_t e = hipFuncGetAttributes(&attr, f);
         if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: mechanics mirror kernel");
@@ -1493,16 +1534,22 @@ static int mirror_check(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech
     return 0;
 }
 
-struct MirrorPtrs {
-    CellRec* recA;
-    double4 *prevA, *prevB, *vp;
-    uint32_t* mkey;
-};
-static MirrorPtrs mirror_ptrs(hc_buf* mirror, uint32_t max_cells) {
+static MSet mirror_set(hc_buf* mirror, uint32_t max_cells, int k) {
     const MirrorLayout L = mirror_layout(max_cells);
     uint8_t* P = (uint8_t*)mirror->dptr;
-    return MirrorPtrs{(CellRec*)(P + L.recA), (double4*)(P + L.prevA), (double4*)(P + L.prevB), (double4*)(P + L.vp),
-                      (uint32_t*)(P + L.mkey)};
+    MSet m;
+    m.pos = (double4*)(P + L.pos[k]);
+    m.prev = (double4*)(P + L.prev[k]);
+    m.prop = (double4*)(P + L.prop[k]);
+    m.idx = (uint32_t*)(P + L.idx[k]);
+    m.key = (uint32_t*)(P + L.key[k]);
+    m.slot = (uint32_t*)(P + L.slot[k]);
+    m.kstep = (uint32_t*)(P + L.kstep[k]);
+    m.rpos = (uint32_t*)(P + L.rpos[k]);
+    return m;
+}
+static double4* mirror_vp(hc_buf* mirror, uint32_t max_cells) {
+    return (double4*)((uint8_t*)mirror->dptr + mirror_layout(max_cells).vp);
 }
 
 int mirror_load(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, const hc_grid* mgrid,
@@ -1511,9 +1558,8 @@ int mirror_load(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, hc_buf* mirror, const 
     if (r || (r = hc_build_hash(ctx, cells, hash, mech, mgrid, n, max_cells, n_mvox))) return r;
     if (!n) return 0;
     const HashPtrs h = hash_ptrs(hash, n_mvox, max_cells);
-    const MirrorPtrs m = mirror_ptrs(mirror, max_cells);
-    hipLaunchKernelGGL(k_m_load, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, h.rec,
-                       cell_ptrs(cells->dptr, max_cells), m.recA, m.prevA, n);
+    hipLaunchKernelGGL(k_m_load, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, h.rec, h.offsets,
+                       cell_ptrs(cells->dptr, max_cells), mirror_set(mirror, max_cells, 0), n);
     return check_launch(ctx, "k_m_load");
 }
 
@@ -1525,24 +1571,21 @@ int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, 
         return fail(ctx, HC_ERR_ARG, "mirror_step: n_mvox != mech grid size");
     if (!n) return 0;
     const HashPtrs h = hash_ptrs(hash, n_mvox, max_cells);
-    const MirrorPtrs m = mirror_ptrs(mirror, max_cells);
+    const MSet a = mirror_set(mirror, max_cells, 0);
+    double4* vp = mirror_vp(mirror, max_cells);
     const unsigned nb = blocks_for(n, 256);
     const int xg = markx_ok(mech);
     {
+        // the slots' bins and places in them come from the last AB2 (or load)
         KernelTimer tm(ctx, HC_K_HASH);
-        if ((r = fill_u32(ctx, h.counts, n_mvox, 0))) return r;
-        hipLaunchKernelGGL(k_m_keys, dim3(nb), dim3(256), 0, ctx->stream, m.recA, m.mkey, h.counts, h.slot, n,
-                           mgrid->x_min, mgrid->y_min, mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx,
-                           mech->mech_grid_ny, mech->mech_grid_nz);
-        if ((r = check_launch(ctx, "k_m_keys"))) return r;
         if ((r = scan_exclusive(ctx, h.counts, h.offsets, n_mvox, h.partial))) return r;
-        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, ctx->stream, m.mkey, h.slot, h.offsets, h.tmp, n);
+        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, ctx->stream, a.key, a.slot, h.offsets, h.tmp, n);
         hipLaunchKernelGGL(k_m_bin_sort, dim3(blocks_for(n_mvox, 256)), dim3(256), 0, ctx->stream, h.offsets, h.tmp,
-                           m.recA, h.sorted, h.pos_of, n_mvox);
-        hipLaunchKernelGGL(k_m_gather, dim3(nb), dim3(256), 0, ctx->stream, m.recA, m.prevA, m.mkey, h.offsets, h.tmp,
-                           h.pos_of, h.sorted, h.rec, m.prevB, mark32_ok(mech) ? h.pos32 : nullptr, n, mgrid->x_min,
-                           mgrid->y_min, mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx,
-                           mech->mech_grid_ny, xg, xg ? h.rpart : nullptr);
+                           a, h.sorted, n_mvox);
+        hipLaunchKernelGGL(k_m_gather, dim3(nb), dim3(256), 0, ctx->stream, a, h.offsets, h.tmp, h.sorted, h.rec,
+                           mark32_ok(mech) ? h.pos32 : nullptr, n, mgrid->x_min, mgrid->y_min, mgrid->z_min,
+                           mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny, xg,
+                           xg ? h.rpart : nullptr);
         if (xg) hipLaunchKernelGGL(k_rmax, dim3(1), dim3(256), 0, ctx->stream, h.rpart, nb, h.rmax);
         if ((r = check_launch(ctx, "mirror hash"))) return r;
     }
@@ -1551,16 +1594,29 @@ int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, 
         hipLaunchKernelGGL(k_forces, dim3(nb), dim3(256), 0, ctx->stream, h.rec, mark32_ok(mech) ? h.pos32 : nullptr,
                            h.offsets, (double*)nullptr, (double*)nullptr, (double*)nullptr, (double*)nullptr, n,
                            mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz, mech->max_interaction_distance,
-                           (float)mech->mechanics_voxel_size, xg, h.rmax, m.vp);
+                           (float)mech->mechanics_voxel_size, xg, h.rmax, vp);
         if ((r = check_launch(ctx, "k_forces (mirror)"))) return r;
     }
     KernelTimer tm(ctx, HC_K_INTEGRATE);
+    if ((r = fill_u32(ctx, h.counts, n_mvox, 0))) return r;
     const double x_max = mgrid->x_min + (double)mgrid->nx * mgrid->dx;
     const double y_max = mgrid->y_min + (double)mgrid->ny * mgrid->dy;
     const double z_max = mgrid->z_min + (double)mgrid->nz * mgrid->dz;
-    hipLaunchKernelGGL(k_m_integrate, dim3(nb), dim3(256), 0, ctx->stream, h.rec, m.prevB, m.vp, m.recA, m.prevA, n,
-                       dt, mgrid->x_min, mgrid->y_min, mgrid->z_min, x_max, y_max, z_max);
+    hipLaunchKernelGGL(k_m_integrate, dim3(nb), dim3(256), 0, ctx->stream, a, (const double4*)vp, h.counts, n, dt,
+                       mgrid->x_min, mgrid->y_min, mgrid->z_min, x_max, y_max, z_max, mgrid->x_min, mgrid->y_min,
+                       mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny,
+                       mech->mech_grid_nz);
     return check_launch(ctx, "k_m_integrate");
+}
+
+int mirror_resort(hc_ctx* ctx, hc_buf* mirror, uint32_t n, uint32_t max_cells) {
+    if (!ctx || !mirror || n > max_cells || hc_buf_size(mirror) < mirror_layout(max_cells).total)
+        return fail(ctx, HC_ERR_ARG, "mirror_resort: bad argument");
+    if (!n) return 0;
+    const MSet a = mirror_set(mirror, max_cells, 0), b = mirror_set(mirror, max_cells, 1);
+    hipLaunchKernelGGL(k_m_permute, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, a, b, n);
+    hipLaunchKernelGGL(k_m_copy, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, a, b, n);
+    return check_launch(ctx, "mirror re-sort");
 }
 
 int mirror_sync(hc_ctx* ctx, hc_buf* cells, hc_buf* mirror, uint32_t n, uint32_t max_cells) {
@@ -1569,8 +1625,8 @@ int mirror_sync(hc_ctx* ctx, hc_buf* cells, hc_buf* mirror, uint32_t n, uint32_t
     if (!mirror || hc_buf_size(mirror) < mirror_layout(max_cells).total)
         return fail(ctx, HC_ERR_ARG, "mirror_sync: mirror buffer too small");
     if (!n) return 0;
-    const MirrorPtrs m = mirror_ptrs(mirror, max_cells);
-    hipLaunchKernelGGL(k_m_sync, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, m.recA, m.prevA, m.vp,
+    hipLaunchKernelGGL(k_m_sync, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream,
+                       mirror_set(mirror, max_cells, 0), (const double4*)mirror_vp(mirror, max_cells),
                        cell_ptrs(cells->dptr, max_cells), n);
     return check_launch(ctx, "k_m_sync");
 }

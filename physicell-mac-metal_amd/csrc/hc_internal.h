@@ -254,12 +254,12 @@ struct HashLayout {
 };
 HashLayout hash_layout(uint32_t n_mvox, uint32_t max_cells);
 
-// Hash-ordered mechanics mirror of one simulation (cells.hip): the
-// post-AB2 records and previous velocities in the order of the last hash
-// build, the previous velocities in this step's order, the forces in hash
-// order and each mirror slot's new key.
+// Mechanics mirror of one simulation (cells.hip): each agent's mechanics
+// state in slots ordered like a recent step's records (two sets: the second
+// receives a re-sort), the coming step's bin and place in the bin per slot,
+// the step's forces by record position.
 struct MirrorLayout {
-    size_t recA, prevA, prevB, vp, mkey, total;
+    size_t pos[2], prev[2], prop[2], idx[2], key[2], slot[2], kstep[2], rpos[2], vp, total;
 };
 MirrorLayout mirror_layout(uint32_t max_cells);
 // builds the hash from the cell fields and the mirror from it
@@ -269,6 +269,8 @@ int mirror_load(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, hc_buf* mirror, const 
 // are not touched
 int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, const hc_grid* mgrid, double dt,
                 uint32_t n, uint32_t max_cells, uint32_t n_mvox);
+// puts the slots back in the record order of the last step
+int mirror_resort(hc_ctx* ctx, hc_buf* mirror, uint32_t n, uint32_t max_cells);
 // position, velocity, previous velocity, pressure and mechanics voxel of
 // every agent from the mirror into the cell fields
 int mirror_sync(hc_ctx* ctx, hc_buf* cells, hc_buf* mirror, uint32_t n, uint32_t max_cells);

@@ -380,6 +380,10 @@ void CellMechanics::mirrorStep(CellData& cells, double dt) {
           "mechanics mirror: step");
 }
 
+void CellMechanics::resortMirror(CellData& cells) {
+    check(ctx_, hc::mirror_resort(ctx_, mirror_.get(), cells.num_cells, cells.max_cells), "mechanics mirror: re-sort");
+}
+
 void CellMechanics::syncMirror(CellData& cells) {
     check(ctx_, hc::mirror_sync(ctx_, cells.gpu_buffer(), mirror_.get(), cells.num_cells, cells.max_cells),
           "mechanics mirror: sync");
@@ -504,11 +508,31 @@ void Simulation::syncMirror() {
     mirror_state_ = 2;
 }
 
+// the mirror's slots back in record order every MIRROR_RESORT_STEPS steps
+// (eager, outside any graph)
+void Simulation::mirrorResortIfDue() {
+    if (mirror_state_ != 0 && mirror_steps_ >= MIRROR_RESORT_STEPS) {
+        mechanics.resortMirror(cells);
+        mirror_steps_ = 0;
+    }
+}
+
 void Simulation::mechanicsStep() {
     if (useMirror()) {
-        if (mirror_state_ == 0) mechanics.loadMirror(cells);
+        // while a period graph is being captured the step is recorded, not
+        // run: launchPeriodGraph loads, re-sorts and counts around its replays
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(hc_stream_of(ctx), &cs) != hipSuccess)
+            throw std::runtime_error("mechanics step: stream capture status");
+        const bool eager = cs == hipStreamCaptureStatusNone;
+        if (mirror_state_ == 0) {
+            mechanics.loadMirror(cells);
+            mirror_steps_ = 0;
+        }
+        if (eager) mirrorResortIfDue();
         mechanics.mirrorStep(cells, config.dt_mechanics);
         mirror_state_ = 1;
+        if (eager) mirror_steps_++;
         return;
     }
     syncMirror();
@@ -805,7 +829,9 @@ void Simulation::launchPeriodGraph() {
         // the captured period steps the mirror; build it first, outside the graph
         mechanics.loadMirror(cells);
         mirror_state_ = 2;
+        mirror_steps_ = 0;
     }
+    if (useMirror()) mirrorResortIfDue();
     if (!graph_exec_) {
         check(ctx, hc_wait(ctx), "wait");
         hipGraph_t graph = nullptr;
@@ -828,7 +854,10 @@ void Simulation::launchPeriodGraph() {
     }
     hipError_t e = hipGraphLaunch((hipGraphExec_t)graph_exec_, stream);
     if (e != hipSuccess) throw std::runtime_error(std::string("graph launch: ") + hipGetErrorString(e));
-    if (useMirror()) mirror_state_ = 1;          // the replayed mechanics step moved the mirror on
+    if (useMirror()) {
+        mirror_state_ = 1;                       // the replayed mechanics step moved the mirror on
+        mirror_steps_++;
+    }
 }
 
 void Simulation::wait() { check(ctx, hc_wait(ctx), "wait"); }

@@ -192,6 +192,7 @@ public:
     void loadMirror(CellData& cells);
     void mirrorStep(CellData& cells, double dt);
     void syncMirror(CellData& cells);
+    void resortMirror(CellData& cells);
 
 private:
     hc_ctx* ctx_ = nullptr;
@@ -282,6 +283,11 @@ private:
     // 0: the cell fields hold the mechanics state (the mirror is not built);
     // 1: the mirror does and the fields are behind; 2: both agree
     int mirror_state_ = 0;
+    // mechanics steps since the mirror's slots were put in record order
+    // (re-sorted every MIRROR_RESORT_STEPS steps, between periods)
+    int mirror_steps_ = 0;
+    static constexpr int MIRROR_RESORT_STEPS = 64;
+    void mirrorResortIfDue();
     bool mirror_on_ = true;
     DeviceBuffer motility_;
     DeviceBuffer volume_, mask_;

@@ -102,3 +102,26 @@ def test_mirror_switch_mid_run():
     sim.wait()
     assert_same(state(sim), a, "switching")
     sim.close()
+
+
+@pytest.mark.parametrize("builder", ["config1", "crowded"])
+def test_mirror_resort_long_run(builder):
+    """The mirror's slots keep their order between steps and are put back in
+    record order every 64 mechanics steps; over 140 periods (two re-sorts,
+    graph replays and eager periods, a field read and a neighbour query in
+    between) the run equals the direct one bit for bit."""
+    out = []
+    for mirror in (True, False):
+        sim = make(builder, mirror)
+        sim.run_periods(70, use_graph=True)
+        mid = {f: sim.field(f) for f in FIELDS}
+        sim.run_periods(30, use_graph=False)
+        q = sim.neighbor_query(H.QUERY_CONTACT)
+        sim.run_periods(40, use_graph=True)
+        sim.wait()
+        st = state(sim)
+        st.update({f"mid{f}": v for f, v in mid.items()})
+        st["q_off"], st["q_idx"] = q
+        out.append(st)
+        sim.close()
+    assert_same(out[0], out[1], f"{builder} 140 periods")
