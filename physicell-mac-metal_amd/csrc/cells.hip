@@ -743,6 +743,10 @@ __global__ __launch_bounds__(256) void k_stable_gather32(CellPtrs cp, const uint
 // (force_rows32), which populations of 2^27 agents or more, grids wider than
 // MARKX_MAX_EXTENT in x and forces_fp32 = 2 take.  Two kernels, so that each
 // gets the registers its own path needs.
+// vout non-null (the mechanics mirror): the records' index field is the
+// agent's mirror slot, mkey the slots' bins, rel_adh the slots' properties
+// (double4, relative adhesion distance in w) and the forces go to vout in
+// record order.
 template <bool LIST>
 __global__ __launch_bounds__(256) void k_forces32(const CellRec32* __restrict__ rec, const uint32_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ xoff,
@@ -751,13 +755,13 @@ __global__ __launch_bounds__(256) void k_forces32(const CellRec32* __restrict__ 
                                                   double* __restrict__ vy, double* __restrict__ vz,
                                                   double* __restrict__ pressure, uint32_t n, uint32_t mnx, uint32_t mny,
                                                   uint32_t mnz, float max_dist, double voxel,
-                                                  const float* __restrict__ rmax) {
+                                                  const float* __restrict__ rmax, double4* __restrict__ vout) {
     __shared__ uint32_t marks[LIST ? 4 * MARKX_CAP * 64 : 1];
     const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (t >= n) return;
     const CellRec32 me = rec[t];
     const uint32_t key = mkey[me.idx];
-    const float rel_max_adh = (float)rel_adh[me.idx];
+    const float rel_max_adh = (float)(vout ? rel_adh[4 * (size_t)me.idx + 3] : rel_adh[me.idx]);
     float fx = 0.f, fy = 0.f, fz = 0.f, local_pressure = 0.f;
     const uint32_t vi = key % mnx, vj = (key / mnx) % mny, vk = key / (mnx * mny);
     const float cut2 = max_dist * max_dist, vox = (float)voxel;
@@ -769,6 +773,10 @@ __global__ __launch_bounds__(256) void k_forces32(const CellRec32* __restrict__ 
     } else {
         force_rows32(rec, offsets, t, 0u, vi, vj, vk, mnx, mny, mnz, me.x, me.y, me.z, vox, me.r, me.rep, me.adh,
                      rel_max_adh, cut2, fx, fy, fz, local_pressure);
+    }
+    if (vout) {
+        vout[t] = make_double4(fx, fy, fz, local_pressure);
+        return;
     }
     const uint32_t c = me.idx;
     vx[c] = fx;
@@ -1129,7 +1137,8 @@ int hc_forces(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech, uin
                            cp.u[HC_F_MECH_VOXEL - HC_NUM_F64_FIELDS], cp.f[HC_F_REL_MAX_ADH],
                            cp.f[HC_F_VEL_X], cp.f[HC_F_VEL_Y], cp.f[HC_F_VEL_Z], cp.f[HC_F_PRESSURE], n,
                            mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz,
-                           (float)mech->max_interaction_distance, mech->mechanics_voxel_size, h.rmax);
+                           (float)mech->max_interaction_distance, mech->mechanics_voxel_size, h.rmax,
+                           (double4*)nullptr);
     else
         hipLaunchKernelGGL(k_forces, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, h.rec,
                            mark32_ok(mech) ? h.pos32 : nullptr, h.offsets, cp.f[HC_F_VEL_X], cp.f[HC_F_VEL_Y],
@@ -1311,6 +1320,7 @@ MirrorLayout mirror_layout(uint32_t max_cells) {
         L.slot[k] = o;  o = align256(o + (size_t)max_cells * 4);
         L.kstep[k] = o; o = align256(o + (size_t)max_cells * 4);
         L.rpos[k] = o;  o = align256(o + (size_t)max_cells * 4);
+        L.xslab[k] = o; o = align256(o + (size_t)max_cells);
     }
     L.vp = o;      o = align256(o + (size_t)max_cells * sizeof(double4));
     L.total = o;
@@ -1321,6 +1331,7 @@ MirrorLayout mirror_layout(uint32_t max_cells) {
 struct MSet {
     double4 *pos, *prev, *prop;     // prop: radius, repulsion, adhesion, relative max adhesion distance
     uint32_t *idx, *key, *slot, *kstep, *rpos;
+    uint8_t* xslab;                 // mixed form: the x slab of the slot's FP32 record in its coming bin
 };
 
 // the slots from a hash built on the cell fields, in its record order: state,
@@ -1429,6 +1440,146 @@ __global__ __launch_bounds__(256) void k_m_gather(MSet a, const uint32_t* __rest
     rec[pos] = r;
 }
 
+// Mixed form: the slots from an FP32 hash built on the cell fields, in its
+// record order (state from the cell fields; bins, places and slabs as the
+// hash build computed them)
+__global__ __launch_bounds__(256) void k_m_load32(const CellRec32* __restrict__ rec,
+                                                  const uint32_t* __restrict__ offsets,
+                                                  const uint8_t* __restrict__ xslab, CellPtrs cp, MSet a, uint32_t n) {
+    const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t c = rec[q].idx;
+    const uint32_t key = cp.u[HC_F_MECH_VOXEL - HC_NUM_F64_FIELDS][c];
+    a.pos[q] = make_double4(cp.f[HC_F_POS_X][c], cp.f[HC_F_POS_Y][c], cp.f[HC_F_POS_Z][c], 0.0);
+    a.prev[q] = make_double4(cp.f[HC_F_PREV_VX][c], cp.f[HC_F_PREV_VY][c], cp.f[HC_F_PREV_VZ][c], 0.0);
+    a.prop[q] = make_double4(cp.f[HC_F_RADIUS][c], cp.f[HC_F_REPULSION][c], cp.f[HC_F_ADHESION][c],
+                             cp.f[HC_F_REL_MAX_ADH][c]);
+    a.idx[q] = c;
+    a.key[q] = key;
+    a.slot[q] = q - offsets[key];
+    a.kstep[q] = key;
+    a.rpos[q] = q;
+    a.xslab[q] = xslab[c];
+}
+
+// Mixed form: bins of up to BIN_SORT_CAP members (slots in tmp) in ascending
+// agent index (the bin's list) and each slot's FP32 record position by (x
+// slab, index); every bin's slab starts (xoff) and the entries past the last
+// bin (grid of nbins + 1 threads).  Larger bins: slab starts only (the
+// gather ranks their slots).
+__global__ __launch_bounds__(256) void k_m_bin_sort32(const uint32_t* __restrict__ offsets,
+                                                      const uint32_t* __restrict__ tmp, MSet a,
+                                                      uint32_t* __restrict__ sorted, uint32_t* __restrict__ xoff,
+                                                      uint32_t nbins) {
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b > nbins) return;
+    if (b == nbins) {
+        const uint32_t n = offsets[nbins];
+        *reinterpret_cast<uint4*>(xoff + (size_t)XSLABS * nbins) = make_uint4(n, n, n, n);
+        return;
+    }
+    const uint32_t beg = offsets[b], cnt = offsets[b + 1] - beg;
+    uint32_t c0 = 0, c1 = 0, c2 = 0;
+    if (cnt == 0 || cnt > BIN_SORT_CAP) {
+        for (uint32_t q = beg; q < beg + cnt; q++) {
+            const uint32_t sq = a.xslab[tmp[q]];
+            c0 += sq == 0 ? 1u : 0u;
+            c1 += sq == 1 ? 1u : 0u;
+            c2 += sq == 2 ? 1u : 0u;
+        }
+        *reinterpret_cast<uint4*>(xoff + (size_t)XSLABS * b) =
+            make_uint4(beg, beg + c0, beg + c0 + c1, beg + c0 + c1 + c2);
+        return;
+    }
+    uint64_t v[BIN_SORT_CAP];
+#pragma unroll
+    for (uint32_t q = 0; q < BIN_SORT_CAP; q++) {
+        const uint32_t m = q < cnt ? tmp[beg + q] : 0u;
+        v[q] = q < cnt ? ((uint64_t)a.idx[m] << 32) | m : ~0ull;
+    }
+#pragma unroll
+    for (uint32_t k = 2; k <= BIN_SORT_CAP; k <<= 1)
+#pragma unroll
+        for (uint32_t j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (uint32_t i = 0; i < BIN_SORT_CAP; i++) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const uint64_t lo = v[i] < v[l] ? v[i] : v[l], hi = v[i] < v[l] ? v[l] : v[i];
+                    const bool up = (i & k) == 0;
+                    v[i] = up ? lo : hi;
+                    v[l] = up ? hi : lo;
+                }
+            }
+    uint32_t sl[BIN_SORT_CAP];
+#pragma unroll
+    for (uint32_t q = 0; q < BIN_SORT_CAP; q++) {
+        sl[q] = q < cnt ? a.xslab[(uint32_t)v[q]] : XSLABS;
+        c0 += sl[q] == 0 ? 1u : 0u;
+        c1 += sl[q] == 1 ? 1u : 0u;
+        c2 += sl[q] == 2 ? 1u : 0u;
+    }
+    const uint32_t x1 = beg + c0, x2 = x1 + c1, x3 = x2 + c2;
+    *reinterpret_cast<uint4*>(xoff + (size_t)XSLABS * b) = make_uint4(beg, x1, x2, x3);
+    uint32_t n0 = beg, n1 = x1, n2 = x2, n3 = x3;
+#pragma unroll
+    for (uint32_t q = 0; q < BIN_SORT_CAP; q++)
+        if (q < cnt) {
+            sorted[beg + q] = (uint32_t)(v[q] >> 32);
+            const uint32_t sq = sl[q];
+            a.rpos[(uint32_t)v[q]] = sq == 0 ? n0 : (sq == 1 ? n1 : (sq == 2 ? n2 : n3));
+            n0 += sq == 0 ? 1u : 0u;
+            n1 += sq == 1 ? 1u : 0u;
+            n2 += sq == 2 ? 1u : 0u;
+            n3 += sq == 3 ? 1u : 0u;
+        }
+}
+
+// Mixed form: this step's FP32 records (k_stable_gather32's values) from the
+// slots; the records' index field holds the slot.  Slots of bins above
+// BIN_SORT_CAP are ranked here.
+__global__ __launch_bounds__(256) void k_m_gather32(MSet a, const uint32_t* __restrict__ offsets,
+                                                    const uint32_t* __restrict__ tmp, uint32_t* __restrict__ sorted,
+                                                    CellRec32* __restrict__ rec, uint32_t n, double x_min,
+                                                    double y_min, double z_min, double voxel, uint32_t mnx,
+                                                    uint32_t mny, float* __restrict__ rpart) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    const double4 pr = p < n ? a.prop[p] : make_double4(0.0, 0.0, 0.0, 0.0);
+    block_max_to((float)pr.x, rpart);
+    if (p >= n) return;
+    const uint32_t b = a.key[p];
+    const uint32_t beg = offsets[b], end = offsets[b + 1];
+    uint32_t pos;
+    if (end - beg <= BIN_SORT_CAP) {
+        pos = a.rpos[p];                                  // placed by k_m_bin_sort32
+    } else {
+        const uint32_t id = a.idx[p], sc = a.xslab[p];
+        uint32_t ri = 0, rs = 0;
+        for (uint32_t u = beg; u < end; u++) {
+            const uint32_t o = tmp[u];
+            const uint32_t io = a.idx[o], so = a.xslab[o];
+            ri += io < id ? 1u : 0u;
+            rs += (so < sc || (so == sc && io < id)) ? 1u : 0u;
+        }
+        sorted[beg + ri] = id;
+        pos = beg + rs;
+        a.rpos[p] = pos;
+    }
+    a.kstep[p] = b;
+    const double4 x = a.pos[p];
+    const uint32_t bi = b % mnx, bj = (b / mnx) % mny, bk = b / (mnx * mny);
+    CellRec32 r;
+    r.xg = (float)(x.x - x_min);
+    r.y = bin_rel32(x.y, y_min, bj, voxel);
+    r.z = bin_rel32(x.z, z_min, bk, voxel);
+    r.x = bin_rel32(x.x, x_min, bi, voxel);
+    r.r = (float)pr.x;
+    r.rep = (float)pr.y;
+    r.adh = (float)pr.z;
+    r.idx = p;
+    rec[pos] = r;
+}
+
 // AB2 (shaders/integrate.metal:56-79) on the slots, with this step's forces
 // by record position; then the bin (shaders/mechanics.metal:61-72) and the
 // place in it of the new position for the coming step (counts must be zero)
@@ -1437,7 +1588,7 @@ __global__ __launch_bounds__(256) void k_m_integrate(MSet a, const double4* __re
                                                      double x_min, double y_min, double z_min, double x_max,
                                                      double y_max, double z_max, double mx_min, double my_min,
                                                      double mz_min, double voxel, uint32_t mnx, uint32_t mny,
-                                                     uint32_t mnz) {
+                                                     uint32_t mnz, int fp32) {
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
     const bool valid = p < n;
     uint32_t key = 0;
@@ -1460,6 +1611,7 @@ __global__ __launch_bounds__(256) void k_m_integrate(MSet a, const double4* __re
         vk = vk < 0 ? 0 : (vk > (int)mnz - 1 ? (int)mnz - 1 : vk);
         key = (uint32_t)vk * mny * mnx + (uint32_t)vj * mnx + (uint32_t)vi;
         a.key[p] = key;
+        if (fp32) a.xslab[p] = (uint8_t)xslab_of(bin_rel32(px, mx_min, (uint32_t)vi, voxel), xslab_scale(voxel));
     }
     const uint32_t s = ballot_bin(valid, key, counts);
     if (valid) a.slot[p] = s;
@@ -1497,6 +1649,7 @@ __global__ __launch_bounds__(256) void k_m_permute(MSet a, MSet b, uint32_t n) {
     b.key[q] = a.key[p];
     b.slot[q] = a.slot[p];
     b.kstep[q] = a.kstep[p];
+    b.xslab[q] = a.xslab[p];
     b.rpos[q] = q;
 }
 __global__ __launch_bounds__(256) void k_m_copy(MSet a, MSet b, uint32_t n) {
@@ -1509,6 +1662,7 @@ __global__ __launch_bounds__(256) void k_m_copy(MSet a, MSet b, uint32_t n) {
     a.key[p] = b.key[p];
     a.slot[p] = b.slot[p];
     a.kstep[p] = b.kstep[p];
+    a.xslab[p] = b.xslab[p];
     a.rpos[p] = b.rpos[p];
 }
 
@@ -1516,7 +1670,8 @@ int probe_kernels_mirror(hc_ctx* ctx) {
     hipFuncAttributes attr;
     const void* fns[] = {(const void*)k_m_load, (const void*)k_m_bin_sort, (const void*)k_m_gather,
                          (const void*)k_m_integrate, (const void*)k_m_sync, (const void*)k_m_permute,
-                         (const void*)k_m_copy};
+                         (const void*)k_m_copy, (const void*)k_m_load32, (const void*)k_m_bin_sort32,
+                         (const void*)k_m_gather32};
     for (const void* f : fns) {
         hipError_t e = hipFuncGetAttributes(&attr, f);
         if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: mechanics mirror kernel");
@@ -1528,7 +1683,6 @@ static int mirror_check(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech
                         uint32_t max_cells, uint32_t n_mvox, const char* who) {
     if (!ctx || !hash || !mirror || !mech) return fail(ctx, HC_ERR_ARG, std::string(who) + ": null argument");
     if (n > max_cells) return fail(ctx, HC_ERR_ARG, std::string(who) + ": n > max_cells");
-    if (mech->forces_fp32) return fail(ctx, HC_ERR_ARG, std::string(who) + ": the mirror holds FP64 records");
     if (hc_buf_size(hash) < hc_hash_bytes(n_mvox, max_cells) || hc_buf_size(mirror) < mirror_layout(max_cells).total)
         return fail(ctx, HC_ERR_ARG, std::string(who) + ": buffer too small");
     return 0;
@@ -1546,6 +1700,7 @@ static MSet mirror_set(hc_buf* mirror, uint32_t max_cells, int k) {
     m.slot = (uint32_t*)(P + L.slot[k]);
     m.kstep = (uint32_t*)(P + L.kstep[k]);
     m.rpos = (uint32_t*)(P + L.rpos[k]);
+    m.xslab = (uint8_t*)(P + L.xslab[k]);
     return m;
 }
 static double4* mirror_vp(hc_buf* mirror, uint32_t max_cells) {
@@ -1558,8 +1713,13 @@ int mirror_load(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, hc_buf* mirror, const 
     if (r || (r = hc_build_hash(ctx, cells, hash, mech, mgrid, n, max_cells, n_mvox))) return r;
     if (!n) return 0;
     const HashPtrs h = hash_ptrs(hash, n_mvox, max_cells);
-    hipLaunchKernelGGL(k_m_load, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, h.rec, h.offsets,
-                       cell_ptrs(cells->dptr, max_cells), mirror_set(mirror, max_cells, 0), n);
+    if (mech->forces_fp32)
+        hipLaunchKernelGGL(k_m_load32, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream,
+                           reinterpret_cast<const CellRec32*>(h.rec), h.offsets, h.xslab,
+                           cell_ptrs(cells->dptr, max_cells), mirror_set(mirror, max_cells, 0), n);
+    else
+        hipLaunchKernelGGL(k_m_load, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, h.rec, h.offsets,
+                           cell_ptrs(cells->dptr, max_cells), mirror_set(mirror, max_cells, 0), n);
     return check_launch(ctx, "k_m_load");
 }
 
@@ -1575,7 +1735,29 @@ int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, 
     double4* vp = mirror_vp(mirror, max_cells);
     const unsigned nb = blocks_for(n, 256);
     const int xg = markx_ok(mech);
-    {
+    const int fp32 = mech->forces_fp32 ? 1 : 0;
+    if (fp32) {
+        // the mixed form: FP32 records in (bin, x slab, index) order
+        {
+            KernelTimer tm(ctx, HC_K_HASH);
+            if ((r = scan_exclusive(ctx, h.counts, h.offsets, n_mvox, h.partial))) return r;
+            hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, ctx->stream, a.key, a.slot, h.offsets, h.tmp, n);
+            hipLaunchKernelGGL(k_m_bin_sort32, dim3(blocks_for(n_mvox + 1, 256)), dim3(256), 0, ctx->stream,
+                               h.offsets, h.tmp, a, h.sorted, h.xoff, n_mvox);
+            hipLaunchKernelGGL(k_m_gather32, dim3(nb), dim3(256), 0, ctx->stream, a, h.offsets, h.tmp, h.sorted,
+                               reinterpret_cast<CellRec32*>(h.rec), n, mgrid->x_min, mgrid->y_min, mgrid->z_min,
+                               mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny, h.rpart);
+            hipLaunchKernelGGL(k_rmax, dim3(1), dim3(256), 0, ctx->stream, h.rpart, nb, h.rmax);
+            if ((r = check_launch(ctx, "mirror hash (mixed)"))) return r;
+        }
+        KernelTimer tm(ctx, HC_K_FORCES);
+        hipLaunchKernelGGL(list32_ok(mech, n, n_mvox) ? k_forces32<true> : k_forces32<false>, dim3(nb), dim3(256), 0,
+                           ctx->stream, reinterpret_cast<const CellRec32*>(h.rec), h.offsets, h.xoff, a.key,
+                           (const double*)a.prop, (double*)nullptr, (double*)nullptr, (double*)nullptr,
+                           (double*)nullptr, n, mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz,
+                           (float)mech->max_interaction_distance, mech->mechanics_voxel_size, h.rmax, vp);
+        if ((r = check_launch(ctx, "k_forces32 (mirror)"))) return r;
+    } else {
         // the slots' bins and places in them come from the last AB2 (or load)
         KernelTimer tm(ctx, HC_K_HASH);
         if ((r = scan_exclusive(ctx, h.counts, h.offsets, n_mvox, h.partial))) return r;
@@ -1589,7 +1771,7 @@ int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, 
         if (xg) hipLaunchKernelGGL(k_rmax, dim3(1), dim3(256), 0, ctx->stream, h.rpart, nb, h.rmax);
         if ((r = check_launch(ctx, "mirror hash"))) return r;
     }
-    {
+    if (!fp32) {
         KernelTimer tm(ctx, HC_K_FORCES);
         hipLaunchKernelGGL(k_forces, dim3(nb), dim3(256), 0, ctx->stream, h.rec, mark32_ok(mech) ? h.pos32 : nullptr,
                            h.offsets, (double*)nullptr, (double*)nullptr, (double*)nullptr, (double*)nullptr, n,
@@ -1605,7 +1787,7 @@ int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, 
     hipLaunchKernelGGL(k_m_integrate, dim3(nb), dim3(256), 0, ctx->stream, a, (const double4*)vp, h.counts, n, dt,
                        mgrid->x_min, mgrid->y_min, mgrid->z_min, x_max, y_max, z_max, mgrid->x_min, mgrid->y_min,
                        mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny,
-                       mech->mech_grid_nz);
+                       mech->mech_grid_nz, fp32);
     return check_launch(ctx, "k_m_integrate");
 }
 

@@ -259,7 +259,7 @@ HashLayout hash_layout(uint32_t n_mvox, uint32_t max_cells);
 // receives a re-sort), the coming step's bin and place in the bin per slot,
 // the step's forces by record position.
 struct MirrorLayout {
-    size_t pos[2], prev[2], prop[2], idx[2], key[2], slot[2], kstep[2], rpos[2], vp, total;
+    size_t pos[2], prev[2], prop[2], idx[2], key[2], slot[2], kstep[2], rpos[2], xslab[2], vp, total;
 };
 MirrorLayout mirror_layout(uint32_t max_cells);
 // builds the hash from the cell fields and the mirror from it

@@ -489,7 +489,7 @@ void Simulation::diffusionSubstep() {
 }
 
 bool Simulation::useMirror() const {
-    return mirror_on_ && !motility_on_ && !config.mixed_precision && cells.num_cells > 0;
+    return mirror_on_ && !motility_on_ && cells.num_cells > 0;
 }
 
 void Simulation::setMechanicsMirror(bool on) {

@@ -1,5 +1,5 @@
-"""The hash-ordered mechanics mirror of the FP64 single simulation
-(pc_sim_set_mechanics_mirror): a simulation stepping on the mirror equals the
+"""The mechanics mirror of the single simulation (pc_sim_set_mechanics_mirror),
+plain FP64 and FP32-mixed: a simulation stepping on the mirror equals the
 same simulation stepping on the cell fields directly, bit for bit, in every
 cell field, the density and the hash, through eager and graph periods, the
 phenotype clock (which writes the mirror back before the voxel refresh),
@@ -41,24 +41,25 @@ def assert_same(a, b, what):
             raise AssertionError(f"{what} {k}: {int(np.count_nonzero(d))} values differ, max abs {d.max():.3e}")
 
 
-def make(builder, mirror, dt_pheno=None):
+def make(builder, mirror, dt_pheno=None, mixed=False):
     spec, cells = BUILDERS[builder]()
     if dt_pheno is not None:
         spec.dt_pheno = dt_pheno
-    sim = H.Simulation(spec, cells)
+    sim = H.Simulation(spec, cells, mixed=mixed)
     sim.set_mechanics_mirror(mirror)
     return sim
 
 
+@pytest.mark.parametrize("mixed", [False, True])
 @pytest.mark.parametrize("builder", ["config0", "small3d", "config1", "crowded"])
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_mirror_equals_direct_periods(builder, use_graph):
-    print(f"mirror {builder} graph={use_graph}", flush=True)
+def test_mirror_equals_direct_periods(builder, use_graph, mixed):
+    print(f"mirror {builder} graph={use_graph} mixed={mixed}", flush=True)
     out = []
     for mirror in (True, False):
         # phenotype clock every 3 periods: the mirror is written back before
         # each voxel refresh and the run continues on it
-        sim = make(builder, mirror, dt_pheno=0.3)
+        sim = make(builder, mirror, dt_pheno=0.3, mixed=mixed)
         sim.run_periods(4, use_graph=use_graph)
         mid = {f: sim.field(f) for f in FIELDS}        # a read in the middle of the run
         sim.run_periods(5, use_graph=use_graph)
@@ -67,7 +68,7 @@ def test_mirror_equals_direct_periods(builder, use_graph):
         st.update({f"mid{f}": v for f, v in mid.items()})
         out.append(st)
         sim.close()
-    assert_same(out[0], out[1], f"{builder} graph={use_graph}")
+    assert_same(out[0], out[1], f"{builder} graph={use_graph} mixed={mixed}")
     off = out[0]["offsets"].astype(np.int64)
     if builder == "crowded":
         sizes = np.diff(off)
@@ -104,15 +105,16 @@ def test_mirror_switch_mid_run():
     sim.close()
 
 
+@pytest.mark.parametrize("mixed", [False, True])
 @pytest.mark.parametrize("builder", ["config1", "crowded"])
-def test_mirror_resort_long_run(builder):
+def test_mirror_resort_long_run(builder, mixed):
     """The mirror's slots keep their order between steps and are put back in
     record order every 64 mechanics steps; over 140 periods (two re-sorts,
     graph replays and eager periods, a field read and a neighbour query in
     between) the run equals the direct one bit for bit."""
     out = []
     for mirror in (True, False):
-        sim = make(builder, mirror)
+        sim = make(builder, mirror, mixed=mixed)
         sim.run_periods(70, use_graph=True)
         mid = {f: sim.field(f) for f in FIELDS}
         sim.run_periods(30, use_graph=False)
@@ -124,4 +126,4 @@ def test_mirror_resort_long_run(builder):
         st["q_off"], st["q_idx"] = q
         out.append(st)
         sim.close()
-    assert_same(out[0], out[1], f"{builder} 140 periods")
+    assert_same(out[0], out[1], f"{builder} 140 periods mixed={mixed}")
