@@ -475,11 +475,27 @@ __global__ __launch_bounds__(256) void k_bm_load(const EnsSim* __restrict__ sims
     if (valid) mr.slot[p] = sl;
 }
 
-// Bins of at most BIN_SORT_CAP members (mirror slots in tmp): the ascending
-// list of their agent indices and each slot's record position by (x slab,
-// index); every bin: its slab starts (xoff).  Slot indices are below 2^27.
+// Provisional placement of the mirror's slots in their bins, as one 64-bit
+// entry each: agent index << 32 | x slab << 28 | slot (indices and slots
+// below 2^27), so that the bin sort and the ranks of large bins need no
+// further loads.
+__global__ __launch_bounds__(256) void k_bm_scatter(const EnsSim* __restrict__ sims,
+                                                    const AgentTile* __restrict__ tiles, BMirror mr,
+                                                    const uint32_t* __restrict__ offsets,
+                                                    uint64_t* __restrict__ tmp64) {
+    const AgentTile tl = tiles[blockIdx.x];
+    const EnsSim m = sims[tl.sim];
+    const uint32_t c = tl.c0 + threadIdx.x;
+    if (c >= m.n) return;
+    const uint32_t p = m.cell_off + c;
+    tmp64[offsets[mr.key[p]] + mr.slot[p]] = ((uint64_t)mr.idx[p] << 32) | ((uint64_t)mr.xslab[p] << 28) | p;
+}
+
+// Bins of at most BIN_SORT_CAP members: the entries in ascending agent index
+// (the bin's list) and each slot's record position by (x slab, index);
+// every bin: its slab starts (xoff).  Larger bins: slab starts only.
 __global__ __launch_bounds__(256) void k_bm_bin_sort(const uint32_t* __restrict__ offsets,
-                                                     const uint32_t* __restrict__ tmp, BMirror mr,
+                                                     const uint64_t* __restrict__ tmp64, BMirror mr,
                                                      uint32_t* __restrict__ sorted, uint32_t* __restrict__ xoff,
                                                      uint32_t nbins) {
     const uint32_t b = blockIdx.x * 256 + threadIdx.x;
@@ -497,7 +513,7 @@ __global__ __launch_bounds__(256) void k_bm_bin_sort(const uint32_t* __restrict_
     }
     if (cnt > BIN_SORT_CAP) {
         for (uint32_t q = beg; q < beg + cnt; q++) {
-            const uint32_t sq = mr.xslab[tmp[q]];
+            const uint32_t sq = (uint32_t)(tmp64[q] >> 28) & 3u;
             c0 += sq == 0 ? 1u : 0u;
             c1 += sq == 1 ? 1u : 0u;
             c2 += sq == 2 ? 1u : 0u;
@@ -506,10 +522,9 @@ __global__ __launch_bounds__(256) void k_bm_bin_sort(const uint32_t* __restrict_
             make_uint4(beg, beg + c0, beg + c0 + c1, beg + c0 + c1 + c2);
         return;
     }
-    // (index << 5 | member number) sorted: indices are below 2^27
-    uint32_t a[BIN_SORT_CAP];
+    uint64_t a[BIN_SORT_CAP];
 #pragma unroll
-    for (uint32_t q = 0; q < BIN_SORT_CAP; q++) a[q] = q < cnt ? (mr.idx[tmp[beg + q]] << 5) | q : ~0u;
+    for (uint32_t q = 0; q < BIN_SORT_CAP; q++) a[q] = q < cnt ? tmp64[beg + q] : ~0ull;
 #pragma unroll
     for (uint32_t k = 2; k <= BIN_SORT_CAP; k <<= 1)
 #pragma unroll
@@ -518,23 +533,18 @@ __global__ __launch_bounds__(256) void k_bm_bin_sort(const uint32_t* __restrict_
             for (uint32_t i = 0; i < BIN_SORT_CAP; i++) {
                 const uint32_t l = i ^ j;
                 if (l > i) {
-                    const uint32_t lo = a[i] < a[l] ? a[i] : a[l], hi = a[i] < a[l] ? a[l] : a[i];
+                    const uint64_t lo = a[i] < a[l] ? a[i] : a[l], hi = a[i] < a[l] ? a[l] : a[i];
                     const bool up = (i & k) == 0;
                     a[i] = up ? lo : hi;
                     a[l] = up ? hi : lo;
                 }
             }
-    uint32_t ps[BIN_SORT_CAP], s[BIN_SORT_CAP];
 #pragma unroll
     for (uint32_t q = 0; q < BIN_SORT_CAP; q++) {
-        ps[q] = q < cnt ? tmp[beg + (a[q] & 31u)] : 0u;
-        s[q] = q < cnt ? mr.xslab[ps[q]] : XSLABS;
-    }
-#pragma unroll
-    for (uint32_t q = 0; q < BIN_SORT_CAP; q++) {
-        c0 += s[q] == 0 ? 1u : 0u;
-        c1 += s[q] == 1 ? 1u : 0u;
-        c2 += s[q] == 2 ? 1u : 0u;
+        const uint32_t sq = q < cnt ? (uint32_t)(a[q] >> 28) & 3u : XSLABS;
+        c0 += sq == 0 ? 1u : 0u;
+        c1 += sq == 1 ? 1u : 0u;
+        c2 += sq == 2 ? 1u : 0u;
     }
     const uint32_t x1 = beg + c0, x2 = x1 + c1, x3 = x2 + c2;
     *reinterpret_cast<uint4*>(xoff + (size_t)XSLABS * b) = make_uint4(beg, x1, x2, x3);
@@ -542,9 +552,9 @@ __global__ __launch_bounds__(256) void k_bm_bin_sort(const uint32_t* __restrict_
 #pragma unroll
     for (uint32_t q = 0; q < BIN_SORT_CAP; q++)
         if (q < cnt) {
-            sorted[beg + q] = a[q] >> 5;
-            const uint32_t sq = s[q];
-            mr.rpos[ps[q]] = sq == 0 ? n0 : (sq == 1 ? n1 : (sq == 2 ? n2 : n3));
+            sorted[beg + q] = (uint32_t)(a[q] >> 32);
+            const uint32_t sq = (uint32_t)(a[q] >> 28) & 3u;
+            mr.rpos[(uint32_t)a[q] & ((1u << 28) - 1u)] = sq == 0 ? n0 : (sq == 1 ? n1 : (sq == 2 ? n2 : n3));
             n0 += sq == 0 ? 1u : 0u;
             n1 += sq == 1 ? 1u : 0u;
             n2 += sq == 2 ? 1u : 0u;
@@ -557,7 +567,7 @@ __global__ __launch_bounds__(256) void k_bm_bin_sort(const uint32_t* __restrict_
 __global__ __launch_bounds__(256) void k_bm_gather(const EnsSim* __restrict__ sims,
                                                    const AgentTile* __restrict__ tiles, BMirror mr,
                                                    const uint32_t* __restrict__ offsets,
-                                                   const uint32_t* __restrict__ tmp, uint32_t* __restrict__ sorted,
+                                                   const uint64_t* __restrict__ tmp64, uint32_t* __restrict__ sorted,
                                                    CellRec32* __restrict__ recs, float* __restrict__ rpart) {
     const AgentTile tl = tiles[blockIdx.x];
     const EnsSim m = sims[tl.sim];
@@ -575,8 +585,8 @@ __global__ __launch_bounds__(256) void k_bm_gather(const EnsSim* __restrict__ si
         const uint32_t id = mr.idx[p], sc = mr.xslab[p];
         uint32_t ri = 0, rs = 0;
         for (uint32_t q = beg; q < end; q++) {
-            const uint32_t o = tmp[q];
-            const uint32_t io = mr.idx[o], so = mr.xslab[o];
+            const uint64_t e = tmp64[q];
+            const uint32_t io = (uint32_t)(e >> 32), so = (uint32_t)(e >> 28) & 3u;
             ri += io < id ? 1u : 0u;
             rs += (so < sc || (so == sc && io < id)) ? 1u : 0u;
         }
@@ -710,7 +720,7 @@ int probe_kernels_batch(hc_ctx* ctx) {
                          (const void*)k_b_gather<true>, (const void*)k_b_forces<false>, (const void*)k_b_forces<true, true>, (const void*)k_b_forces<true, false>,
                          (const void*)k_b_forces<true, true, true>, (const void*)k_bm_load, (const void*)k_bm_bin_sort,
                          (const void*)k_bm_gather, (const void*)k_bm_integrate, (const void*)k_bm_sync,
-                         (const void*)k_bm_permute, (const void*)k_bm_copy,
+                         (const void*)k_bm_permute, (const void*)k_bm_copy, (const void*)k_bm_scatter,
                          (const void*)k_b_integrate, (const void*)k_b_sec_keys, (const void*)k_b_sec_groups<double>,
                          (const void*)k_b_sec_groups<float>,
                          (const void*)k_b_sec_apply<double>, (const void*)k_b_sec_apply<float>,
@@ -1230,7 +1240,7 @@ private:
             offsets, partial, rpart, rmax;
         DeviceBuffer pcounts, poffsets, pkv, pslot, ptmp, psorted, pflag, pgsim, pgvox, pgbegin, pcoef, phdr, ppartial;
         // agent mirror (mixed form without motility): two sets, forces by record
-        DeviceBuffer mpos[2], mprev[2], mprop[2], midx[2], mkey[2], mslot[2], mxslab[2], mrpos[2], mkstep[2], mv;
+        DeviceBuffer mpos[2], mprev[2], mprop[2], midx[2], mkey[2], mslot[2], mxslab[2], mrpos[2], mkstep[2], mv, mtmp64;
         uint32_t ntot = 0, nbins = 0, vtot = 0, ntiles = 0, nwork[NLIST] = {}, ns_max = 0, nmax_all = 0;
         uint32_t max_line[NLIST] = {};
         size_t bytes = 0;
@@ -1386,6 +1396,7 @@ private:
                 alloc(w.mxslab[k], N); alloc(w.mrpos[k], N * 4); alloc(w.mkstep[k], N * 4);
             }
             alloc(w.mv, N * 16);
+            alloc(w.mtmp64, N * 8);
         }
         mirror_valid_ = false;
         soa_stale_ = false;
@@ -1563,16 +1574,17 @@ private:
         const AgentTile* tiles = (const AgentTile*)hc_device_ptr(w.tiles.get());
         uint32_t* counts = (uint32_t*)hc_device_ptr(w.counts.get());
         uint32_t* offsets = (uint32_t*)hc_device_ptr(w.offsets.get());
-        uint32_t* tmp = (uint32_t*)hc_device_ptr(w.tmp.get());
+        uint64_t* tmp64 = (uint64_t*)hc_device_ptr(w.mtmp64.get());
         uint32_t* sorted = (uint32_t*)hc_device_ptr(w.sorted.get());
         uint32_t* xoff = (uint32_t*)hc_device_ptr(w.xoff.get());
         float* rpart = (float*)hc_device_ptr(w.rpart.get());
+        if (!tmp64) throw std::runtime_error("ensemble: agent mirror not allocated");
         check(ctx_, hc::scan_exclusive(ctx_, counts, offsets, w.nbins, (uint32_t*)hc_device_ptr(w.partial.get())),
               "scan");
-        check(ctx_, hc::launch_scatter(ctx_, a.key, a.slot, offsets, tmp, w.ntot), "scatter");
-        hipLaunchKernelGGL(hc::k_bm_bin_sort, dim3(hc::blocks_for(w.nbins + 1, 256)), dim3(256), 0, st, offsets, tmp,
-                           a, sorted, xoff, w.nbins);
-        hipLaunchKernelGGL(hc::k_bm_gather, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, a, offsets, tmp, sorted,
+        hipLaunchKernelGGL(hc::k_bm_scatter, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, a, offsets, tmp64);
+        hipLaunchKernelGGL(hc::k_bm_bin_sort, dim3(hc::blocks_for(w.nbins + 1, 256)), dim3(256), 0, st, offsets,
+                           tmp64, a, sorted, xoff, w.nbins);
+        hipLaunchKernelGGL(hc::k_bm_gather, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, a, offsets, tmp64, sorted,
                            (hc::CellRec32*)hc_device_ptr(w.recs.get()), rpart);
         check(ctx_, hc::check_launch(ctx_, "mirror hash"), "mirror hash");
         check(ctx_, hc::launch_rmax(ctx_, rpart, w.ntiles, (float*)hc_device_ptr(w.rmax.get())), "largest radius");
