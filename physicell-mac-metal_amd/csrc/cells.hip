@@ -408,6 +408,46 @@ __global__ __launch_bounds__(256) void k_forces(const CellRec* __restrict__ rec,
     pressure[c] = local_pressure;
 }
 
+// FP64 forces on the mirror with the x-trimmed mark pass (force_agent64t):
+// pos32 in (bin, x slab, index) order with record positions, xoff the slab
+// starts; velocity and pressure to vout[t] in record order.  The agent's own
+// FP32 mark coordinates come from its FP64 record by the gather's
+// expressions.  Agents more than two bins from their bin corner take the
+// exact FP64 pass over the rows.
+__global__ __launch_bounds__(256) void k_forces_xt(const CellRec* __restrict__ rec, const float4* __restrict__ pos32,
+                                                   const uint32_t* __restrict__ offsets,
+                                                   const uint32_t* __restrict__ xoff, uint32_t n, uint32_t mnx,
+                                                   uint32_t mny, uint32_t mnz, double max_dist, double voxel,
+                                                   double x_min, double y_min, double z_min,
+                                                   const float* __restrict__ rmax, double4* __restrict__ vout) {
+    __shared__ uint32_t marks[4 * MARKX_CAP * 64];
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (t >= n) return;
+    const CellRec me = rec[t];
+    const double px = me.x, py = me.y, pz = me.z;
+    const double radius_i = me.r, rep_i = me.rep, adh_i = me.adh, rel_max_adh = me.rel;
+    double fx = 0.0, fy = 0.0, fz = 0.0, local_pressure = 0.0;
+    const uint32_t vi = me.key % mnx, vj = (me.key / mnx) % mny, vk = me.key / (mnx * mny);
+    const double cut2 = max_dist * max_dist;
+    const float mxg = (float)(px - x_min), my = (float)(py - (y_min + vj * voxel)),
+                mz = (float)(pz - (z_min + vk * voxel));
+    const float vox = (float)voxel, near = 2.0f * vox;
+    if (fabsf(my) < near && fabsf(mz) < near && mxg > -near && mxg < (float)mnx * vox + near) {
+        uint32_t* list = marks + (threadIdx.x >> 6) * (MARKX_CAP * 64) + (threadIdx.x & 63);
+        force_agent64t(rec, pos32, xoff, list, vi, vj, vk, mnx, mny, mnz, mxg, bin_rel32(px, x_min, vi, voxel), my,
+                       mz, vox, xslab_scale(voxel), rmax[0], px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2,
+                       fx, fy, fz, local_pressure);
+    } else {
+#pragma unroll 1
+        for (int q = 0; q < 9; q++) {
+            const RowRange rr = neighbor_row(offsets, 0u, vi, vj, vk, mnx, mny, mnz, q);
+            force_range64(rec, rr.beg, rr.end, t, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz,
+                          local_pressure);
+        }
+    }
+    vout[t] = make_double4(fx, fy, fz, local_pressure);
+}
+
 __global__ __launch_bounds__(256) void k_neighbors(const CellRec* __restrict__ rec, const uint32_t* __restrict__ offsets,
                                                    uint32_t* __restrict__ counts, uint32_t* __restrict__ idx,
                                                    uint32_t n, uint32_t mnx, uint32_t mny, uint32_t mnz,
@@ -1070,6 +1110,12 @@ static bool list32_ok(const hc_mech* mech, uint32_t n, uint32_t n_mvox) {
            n < (1u << MARK32_POS_BITS) && n_mvox < (1u << 30);
 }
 
+// FP64 forces on the mirror take the x-trimmed mark pass (force_agent64t)
+// under the same conditions, less the list entries' row code
+static bool xtrim64_ok(const hc_mech* mech, uint32_t n_mvox) {
+    return !mech->forces_fp32 && markx_ok(mech) && mech->mechanics_voxel_size >= 0.5 && n_mvox < (1u << 30);
+}
+
 static int check_hash(hc_ctx* ctx, hc_buf* hash, uint32_t n_mvox, uint32_t max_cells, const char* who) {
     if (!hash || n_mvox == 0) return fail(ctx, HC_ERR_ARG, std::string(who) + ": bad hash argument");
     if (hc_buf_size(hash) < hc_hash_bytes(n_mvox, max_cells))
@@ -1323,6 +1369,7 @@ MirrorLayout mirror_layout(uint32_t max_cells) {
         L.xslab[k] = o; o = align256(o + (size_t)max_cells);
     }
     L.vp = o;      o = align256(o + (size_t)max_cells * sizeof(double4));
+    L.spos = o;    o = align256(o + (size_t)max_cells * 4);
     L.total = o;
     return L;
 }
@@ -1331,16 +1378,19 @@ MirrorLayout mirror_layout(uint32_t max_cells) {
 struct MSet {
     double4 *pos, *prev, *prop;     // prop: radius, repulsion, adhesion, relative max adhesion distance
     uint32_t *idx, *key, *slot, *kstep, *rpos;
-    uint8_t* xslab;                 // mixed form: the x slab of the slot's FP32 record in its coming bin
+    uint8_t* xslab;                 // the x slab of the slot's FP32 record (or mark position) in its coming bin
+    uint32_t* spos;                 // plain form, x-trimmed marks: the slot's mark position this step
 };
 
 // the slots from a hash built on the cell fields, in its record order: state,
 // bin and place in the bin (the hash's counts hold the histogram)
 __global__ __launch_bounds__(256) void k_m_load(const CellRec* __restrict__ rec, const uint32_t* __restrict__ offsets,
-                                                CellPtrs cp, MSet a, uint32_t n) {
+                                                CellPtrs cp, MSet a, uint32_t n, double x_min, double voxel,
+                                                uint32_t mnx, int slabs) {
     const uint32_t q = blockIdx.x * 256 + threadIdx.x;
     if (q >= n) return;
     const CellRec r = rec[q];
+    if (slabs) a.xslab[q] = (uint8_t)xslab_of(bin_rel32(r.x, x_min, r.key % mnx, voxel), xslab_scale(voxel));
     a.pos[q] = make_double4(r.x, r.y, r.z, 0.0);
     a.prev[q] = make_double4(cp.f[HC_F_PREV_VX][r.idx], cp.f[HC_F_PREV_VY][r.idx], cp.f[HC_F_PREV_VZ][r.idx], 0.0);
     a.prop[q] = make_double4(r.r, r.rep, r.adh, r.rel);
@@ -1391,21 +1441,38 @@ __global__ __launch_bounds__(256) void k_m_bin_sort(const uint32_t* __restrict__
 
 // This step's records (k_stable_gather's values) and FP32 mark positions
 // from the slots; slots of bins above BIN_SORT_CAP are ranked here by agent
-// index.
+// index.  xt: the mark positions go in (bin, x slab, index) order, each with
+// its record position in w (force_agent64t), else in record order with the
+// radius in w.
 __global__ __launch_bounds__(256) void k_m_gather(MSet a, const uint32_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ tmp, uint32_t* __restrict__ sorted,
                                                   CellRec* __restrict__ rec, float4* __restrict__ pos32, uint32_t n,
                                                   double x_min, double y_min, double z_min, double voxel,
-                                                  uint32_t mnx, uint32_t mny, int x_grid, float* __restrict__ rpart) {
+                                                  uint32_t mnx, uint32_t mny, int x_grid, int xt,
+                                                  float* __restrict__ rpart) {
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
     const double4 pr = p < n ? a.prop[p] : make_double4(0.0, 0.0, 0.0, 0.0);
     if (rpart) block_max_to(__double2float_ru(pr.x), rpart);
     if (p >= n) return;
     const uint32_t b = a.key[p], id = a.idx[p];
     const uint32_t beg = offsets[b], end = offsets[b + 1];
-    uint32_t pos;
+    uint32_t pos, sp = 0;
     if (end - beg <= BIN_SORT_CAP) {
-        pos = a.rpos[p];                                  // placed by k_m_bin_sort
+        pos = a.rpos[p];                                  // placed by k_m_bin_sort (k_m_bin_sort32<true>)
+        if (xt) sp = a.spos[p];
+    } else if (xt) {
+        const uint32_t sc = a.xslab[p];
+        uint32_t ri = 0, rs = 0;
+        for (uint32_t u = beg; u < end; u++) {
+            const uint32_t o = tmp[u];
+            const uint32_t io = a.idx[o], so = a.xslab[o];
+            ri += io < id ? 1u : 0u;
+            rs += (so < sc || (so == sc && io < id)) ? 1u : 0u;
+        }
+        pos = beg + ri;
+        sp = beg + rs;
+        sorted[pos] = id;
+        a.rpos[p] = pos;
     } else {
         uint32_t rank = 0;
         for (uint32_t u0 = beg; u0 < end; u0 += 8) {
@@ -1423,9 +1490,9 @@ __global__ __launch_bounds__(256) void k_m_gather(MSet a, const uint32_t* __rest
     const double4 x = a.pos[p];
     if (pos32) {
         const uint32_t bi = b % mnx, bj = (b / mnx) % mny, bk = b / (mnx * mny);
-        pos32[pos] = make_float4((float)(x.x - (x_grid ? x_min : x_min + bi * voxel)),
-                                 (float)(x.y - (y_min + bj * voxel)), (float)(x.z - (z_min + bk * voxel)),
-                                 (float)pr.x);
+        pos32[xt ? sp : pos] = make_float4((float)(x.x - (x_grid ? x_min : x_min + bi * voxel)),
+                                           (float)(x.y - (y_min + bj * voxel)), (float)(x.z - (z_min + bk * voxel)),
+                                           xt ? __uint_as_float(pos) : (float)pr.x);
     }
     CellRec r;
     r.x = x.x;
@@ -1462,11 +1529,13 @@ __global__ __launch_bounds__(256) void k_m_load32(const CellRec32* __restrict__ 
     a.xslab[q] = xslab[c];
 }
 
-// Mixed form: bins of up to BIN_SORT_CAP members (slots in tmp) in ascending
-// agent index (the bin's list) and each slot's FP32 record position by (x
-// slab, index); every bin's slab starts (xoff) and the entries past the last
-// bin (grid of nbins + 1 threads).  Larger bins: slab starts only (the
-// gather ranks their slots).
+// Bins of up to BIN_SORT_CAP members (slots in tmp) in ascending agent index
+// (the bin's list) and each slot's place by (x slab, index): the mixed
+// form's FP32 record position (rpos), or, REC64, the plain form's FP32 mark
+// position (spos) beside its record position by index (rpos); every bin's
+// slab starts (xoff) and the entries past the last bin (grid of nbins + 1
+// threads).  Larger bins: slab starts only (the gather ranks their slots).
+template <bool REC64>
 __global__ __launch_bounds__(256) void k_m_bin_sort32(const uint32_t* __restrict__ offsets,
                                                       const uint32_t* __restrict__ tmp, MSet a,
                                                       uint32_t* __restrict__ sorted, uint32_t* __restrict__ xoff,
@@ -1527,7 +1596,13 @@ __global__ __launch_bounds__(256) void k_m_bin_sort32(const uint32_t* __restrict
         if (q < cnt) {
             sorted[beg + q] = (uint32_t)(v[q] >> 32);
             const uint32_t sq = sl[q];
-            a.rpos[(uint32_t)v[q]] = sq == 0 ? n0 : (sq == 1 ? n1 : (sq == 2 ? n2 : n3));
+            const uint32_t sp = sq == 0 ? n0 : (sq == 1 ? n1 : (sq == 2 ? n2 : n3));
+            if (REC64) {
+                a.rpos[(uint32_t)v[q]] = beg + q;
+                a.spos[(uint32_t)v[q]] = sp;
+            } else {
+                a.rpos[(uint32_t)v[q]] = sp;
+            }
             n0 += sq == 0 ? 1u : 0u;
             n1 += sq == 1 ? 1u : 0u;
             n2 += sq == 2 ? 1u : 0u;
@@ -1588,7 +1663,7 @@ __global__ __launch_bounds__(256) void k_m_integrate(MSet a, const double4* __re
                                                      double x_min, double y_min, double z_min, double x_max,
                                                      double y_max, double z_max, double mx_min, double my_min,
                                                      double mz_min, double voxel, uint32_t mnx, uint32_t mny,
-                                                     uint32_t mnz, int fp32) {
+                                                     uint32_t mnz, int slabs) {
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
     const bool valid = p < n;
     uint32_t key = 0;
@@ -1611,7 +1686,7 @@ __global__ __launch_bounds__(256) void k_m_integrate(MSet a, const double4* __re
         vk = vk < 0 ? 0 : (vk > (int)mnz - 1 ? (int)mnz - 1 : vk);
         key = (uint32_t)vk * mny * mnx + (uint32_t)vj * mnx + (uint32_t)vi;
         a.key[p] = key;
-        if (fp32) a.xslab[p] = (uint8_t)xslab_of(bin_rel32(px, mx_min, (uint32_t)vi, voxel), xslab_scale(voxel));
+        if (slabs) a.xslab[p] = (uint8_t)xslab_of(bin_rel32(px, mx_min, (uint32_t)vi, voxel), xslab_scale(voxel));
     }
     const uint32_t s = ballot_bin(valid, key, counts);
     if (valid) a.slot[p] = s;
@@ -1670,8 +1745,8 @@ int probe_kernels_mirror(hc_ctx* ctx) {
     hipFuncAttributes attr;
     const void* fns[] = {(const void*)k_m_load, (const void*)k_m_bin_sort, (const void*)k_m_gather,
                          (const void*)k_m_integrate, (const void*)k_m_sync, (const void*)k_m_permute,
-                         (const void*)k_m_copy, (const void*)k_m_load32, (const void*)k_m_bin_sort32,
-                         (const void*)k_m_gather32};
+                         (const void*)k_m_copy, (const void*)k_m_load32, (const void*)k_m_bin_sort32<false>,
+                         (const void*)k_m_bin_sort32<true>, (const void*)k_m_gather32, (const void*)k_forces_xt};
     for (const void* f : fns) {
         hipError_t e = hipFuncGetAttributes(&attr, f);
         if (e != hipSuccess) return hip_fail(ctx, e, "hc_load_kernels: mechanics mirror kernel");
@@ -1701,6 +1776,7 @@ static MSet mirror_set(hc_buf* mirror, uint32_t max_cells, int k) {
     m.kstep = (uint32_t*)(P + L.kstep[k]);
     m.rpos = (uint32_t*)(P + L.rpos[k]);
     m.xslab = (uint8_t*)(P + L.xslab[k]);
+    m.spos = (uint32_t*)(P + L.spos);
     return m;
 }
 static double4* mirror_vp(hc_buf* mirror, uint32_t max_cells) {
@@ -1719,7 +1795,8 @@ int mirror_load(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, hc_buf* mirror, const 
                            cell_ptrs(cells->dptr, max_cells), mirror_set(mirror, max_cells, 0), n);
     else
         hipLaunchKernelGGL(k_m_load, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, h.rec, h.offsets,
-                           cell_ptrs(cells->dptr, max_cells), mirror_set(mirror, max_cells, 0), n);
+                           cell_ptrs(cells->dptr, max_cells), mirror_set(mirror, max_cells, 0), n, mgrid->x_min,
+                           mech->mechanics_voxel_size, mech->mech_grid_nx, xtrim64_ok(mech, n_mvox) ? 1 : 0);
     return check_launch(ctx, "k_m_load");
 }
 
@@ -1736,13 +1813,14 @@ int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, 
     const unsigned nb = blocks_for(n, 256);
     const int xg = markx_ok(mech);
     const int fp32 = mech->forces_fp32 ? 1 : 0;
+    const int xt = xtrim64_ok(mech, n_mvox) ? 1 : 0;
     if (fp32) {
         // the mixed form: FP32 records in (bin, x slab, index) order
         {
             KernelTimer tm(ctx, HC_K_HASH);
             if ((r = scan_exclusive(ctx, h.counts, h.offsets, n_mvox, h.partial))) return r;
             hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, ctx->stream, a.key, a.slot, h.offsets, h.tmp, n);
-            hipLaunchKernelGGL(k_m_bin_sort32, dim3(blocks_for(n_mvox + 1, 256)), dim3(256), 0, ctx->stream,
+            hipLaunchKernelGGL(k_m_bin_sort32<false>, dim3(blocks_for(n_mvox + 1, 256)), dim3(256), 0, ctx->stream,
                                h.offsets, h.tmp, a, h.sorted, h.xoff, n_mvox);
             hipLaunchKernelGGL(k_m_gather32, dim3(nb), dim3(256), 0, ctx->stream, a, h.offsets, h.tmp, h.sorted,
                                reinterpret_cast<CellRec32*>(h.rec), n, mgrid->x_min, mgrid->y_min, mgrid->z_min,
@@ -1762,21 +1840,32 @@ int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, 
         KernelTimer tm(ctx, HC_K_HASH);
         if ((r = scan_exclusive(ctx, h.counts, h.offsets, n_mvox, h.partial))) return r;
         hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, ctx->stream, a.key, a.slot, h.offsets, h.tmp, n);
-        hipLaunchKernelGGL(k_m_bin_sort, dim3(blocks_for(n_mvox, 256)), dim3(256), 0, ctx->stream, h.offsets, h.tmp,
-                           a, h.sorted, n_mvox);
+        if (xt)
+            hipLaunchKernelGGL(k_m_bin_sort32<true>, dim3(blocks_for(n_mvox + 1, 256)), dim3(256), 0, ctx->stream,
+                               h.offsets, h.tmp, a, h.sorted, h.xoff, n_mvox);
+        else
+            hipLaunchKernelGGL(k_m_bin_sort, dim3(blocks_for(n_mvox, 256)), dim3(256), 0, ctx->stream, h.offsets,
+                               h.tmp, a, h.sorted, n_mvox);
         hipLaunchKernelGGL(k_m_gather, dim3(nb), dim3(256), 0, ctx->stream, a, h.offsets, h.tmp, h.sorted, h.rec,
                            mark32_ok(mech) ? h.pos32 : nullptr, n, mgrid->x_min, mgrid->y_min, mgrid->z_min,
-                           mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny, xg,
+                           mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny, xg, xt,
                            xg ? h.rpart : nullptr);
         if (xg) hipLaunchKernelGGL(k_rmax, dim3(1), dim3(256), 0, ctx->stream, h.rpart, nb, h.rmax);
         if ((r = check_launch(ctx, "mirror hash"))) return r;
     }
     if (!fp32) {
         KernelTimer tm(ctx, HC_K_FORCES);
-        hipLaunchKernelGGL(k_forces, dim3(nb), dim3(256), 0, ctx->stream, h.rec, mark32_ok(mech) ? h.pos32 : nullptr,
-                           h.offsets, (double*)nullptr, (double*)nullptr, (double*)nullptr, (double*)nullptr, n,
-                           mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz, mech->max_interaction_distance,
-                           (float)mech->mechanics_voxel_size, xg, h.rmax, vp);
+        if (xt)
+            hipLaunchKernelGGL(k_forces_xt, dim3(nb), dim3(256), 0, ctx->stream, h.rec, h.pos32, h.offsets, h.xoff,
+                               n, mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz,
+                               mech->max_interaction_distance, mech->mechanics_voxel_size, mgrid->x_min,
+                               mgrid->y_min, mgrid->z_min, h.rmax, vp);
+        else
+            hipLaunchKernelGGL(k_forces, dim3(nb), dim3(256), 0, ctx->stream, h.rec,
+                               mark32_ok(mech) ? h.pos32 : nullptr, h.offsets, (double*)nullptr, (double*)nullptr,
+                               (double*)nullptr, (double*)nullptr, n, mech->mech_grid_nx, mech->mech_grid_ny,
+                               mech->mech_grid_nz, mech->max_interaction_distance,
+                               (float)mech->mechanics_voxel_size, xg, h.rmax, vp);
         if ((r = check_launch(ctx, "k_forces (mirror)"))) return r;
     }
     KernelTimer tm(ctx, HC_K_INTEGRATE);
@@ -1787,7 +1876,7 @@ int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, 
     hipLaunchKernelGGL(k_m_integrate, dim3(nb), dim3(256), 0, ctx->stream, a, (const double4*)vp, h.counts, n, dt,
                        mgrid->x_min, mgrid->y_min, mgrid->z_min, x_max, y_max, z_max, mgrid->x_min, mgrid->y_min,
                        mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny,
-                       mech->mech_grid_nz, fp32);
+                       mech->mech_grid_nz, fp32 || xt ? 1 : 0);
     return check_launch(ctx, "k_m_integrate");
 }
 

@@ -727,6 +727,82 @@ __device__ __attribute__((always_inline)) inline void force_agent32x(
                  adh_i, rel_max_adh, cut2, fx, fy, fz, local_pressure);
 }
 
+// FP64 forces of one agent with an x-trimmed FP32 mark pass.  The records
+// stay in (bin, index) order; pos32 holds the same members per bin in (x
+// slab, index) order, each entry's w the member's record position (the bits
+// of a uint32), with the slab starts in xoff as for the FP32 records.
+//
+// Mark: force_agent64x's threshold and coordinates (x relative to x_min, y
+// and z relative to the agent's bin corner), so every pair the law keeps is
+// marked; rows: force_agent32x's x interval and slab range, with mxb the
+// agent's bin-relative x from the same expression as the slabs'.  A marked
+// candidate sets the bit of its record position relative to the row's first
+// record in a 64-bit mask (a row of more than 64 records takes one walk per
+// 64 of them), and the set bits go to the LDS list in ascending order: the
+// law sees the marked candidates in the reference's visit order (rows, then
+// ascending record position) without a sort, re-tests each with the exact
+// FP64 predicates, and the sums are force_agent64x's bit for bit.
+__device__ __attribute__((always_inline)) inline void force_agent64t(
+    const CellRec* __restrict__ rec, const float4* __restrict__ pos32, const uint32_t* __restrict__ xoff,
+    uint32_t* __restrict__ list, uint32_t vi, uint32_t vj, uint32_t vk, uint32_t mnx, uint32_t mny, uint32_t mnz,
+    float mxg, float mxb, float my, float mz, float voxel, float scale, float rmax, double px, double py, double pz,
+    double radius_i, double rep_i, double adh_i, double rel_max_adh, double cut2, double& fx, double& fy,
+    double& fz, double& local_pressure) {
+    const float relf32 = (float)(rel_max_adh > 1.0 ? rel_max_adh : 1.0);
+    const float L = ((float)radius_i + rmax) * relf32;
+    const float lim = fminf(L * L * (1.0f + 0x1p-16f), (float)cut2) + MARKX_SLACK;
+    const float Lx = sqrtf(lim) + XTRIM_MARGIN;
+    const float lo = mxb - Lx, hi = mxb + Lx;
+    const bool left = lo < 0.0f && vi > 0, right = hi >= voxel && vi + 1 < mnx;
+    const uint32_t blo = left ? vi - 1 : vi, bhi = right ? vi + 1 : vi;
+    const uint32_t jlo = XSLABS * blo + xslab_of(left ? lo + voxel : lo, scale);
+    const uint32_t jhi = XSLABS * bhi + xslab_of(right ? hi - voxel : hi, scale) + 1;
+    const bool three_d = mnz > 1;
+    lds_u32* const l0 = (lds_u32*)list;
+    lds_u32* w = l0;
+    lds_u32* const wend = l0 + MARKX_CAP * 64;
+#pragma unroll 1
+    for (int q = 0; q < 9; q++) {
+        const int dk = three_d ? q / 3 - 1 : 0, dj = q % 3 - 1;
+        const int nk = (int)vk + dk, nj = (int)vj + dj;
+        const bool ok = (three_d || q < 3) && nk >= 0 && nk < (int)mnz && nj >= 0 && nj < (int)mny;
+        const uint32_t rb = XSLABS * (ok ? (uint32_t)nk * mny * mnx + (uint32_t)nj * mnx : 0u);
+        // walked slab range [sb, se); records of the walked bins [r0, r1)
+        const uint32_t sb = ok ? xoff[rb + jlo] : 0u, se = ok ? xoff[rb + jhi] : 0u;
+        const uint32_t r0 = ok ? xoff[rb + XSLABS * blo] : 0u, r1 = ok ? xoff[rb + XSLABS * (bhi + 1)] : 0u;
+        const float myr = my - (float)dj * voxel, mzr = mz - (float)dk * voxel;
+        for (uint32_t wb = r0; sb < se && wb < r1; wb += 64) {
+            uint64_t mask = 0;
+            for (uint32_t u0 = sb; u0 < se; u0 += MARKX_STEP) {
+                const float4* p = pos32 + u0;
+                const uint32_t rem = se - u0;
+                float4 o[MARKX_STEP];
+#pragma unroll
+                for (uint32_t j = 0; j < MARKX_STEP; j++) o[j] = p[j];
+#pragma unroll
+                for (uint32_t j = 0; j < MARKX_STEP; j++) {
+                    const float ddx = mxg - o[j].x, ddy = myr - o[j].y, ddz = mzr - o[j].z;
+                    const float d2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(ddy, ddy, ddx * ddx));
+                    const uint32_t rp = __float_as_uint(o[j].w) - wb;
+                    mask |= (j < rem && d2 < lim && rp < 64u) ? (1ull << (rp & 63u)) : 0ull;
+                }
+            }
+            while (mask) {
+                if (w == wend) {
+                    force_list64(rec, list, MARKX_CAP, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy,
+                                 fz, local_pressure);
+                    w = l0;
+                }
+                *w = wb + (uint32_t)__builtin_ctzll(mask);
+                w += 64;
+                mask &= mask - 1;
+            }
+        }
+    }
+    force_list64(rec, list, (uint32_t)(w - l0) / 64u, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy,
+                 fz, local_pressure);
+}
+
 // The chunked pass over the 9 (3 in 2-D) neighbour rows of an agent.
 __device__ __attribute__((always_inline)) inline void force_rows32(
     const CellRec32* __restrict__ rec, const uint32_t* __restrict__ offsets, uint32_t t, uint32_t bin0, uint32_t vi,

@@ -257,9 +257,10 @@ HashLayout hash_layout(uint32_t n_mvox, uint32_t max_cells);
 // Mechanics mirror of one simulation (cells.hip): each agent's mechanics
 // state in slots ordered like a recent step's records (two sets: the second
 // receives a re-sort), the coming step's bin and place in the bin per slot,
-// the step's forces by record position.
+// the step's forces by record position; spos: each slot's place in the
+// step's (bin, x slab, index) order of the FP32 mark positions (plain form).
 struct MirrorLayout {
-    size_t pos[2], prev[2], prop[2], idx[2], key[2], slot[2], kstep[2], rpos[2], xslab[2], vp, total;
+    size_t pos[2], prev[2], prop[2], idx[2], key[2], slot[2], kstep[2], rpos[2], xslab[2], vp, spos, total;
 };
 MirrorLayout mirror_layout(uint32_t max_cells);
 // builds the hash from the cell fields and the mirror from it

@@ -22,7 +22,14 @@ def crowded():
     return S.small3d(12, 10, 8, n_cells=4000, seed=11)
 
 
-BUILDERS = dict(S.BUILDERS, crowded=crowded)
+def packed():
+    """About 33 agents per 30 um bin: rows of three bins above 64 records (the
+    x-trimmed mark pass walks them 64 records at a time) and more marked
+    candidates per agent than its LDS list holds (the list is flushed)."""
+    return S.small3d(12, 10, 8, n_cells=8000, seed=13)
+
+
+BUILDERS = dict(S.BUILDERS, crowded=crowded, packed=packed)
 
 
 def state(sim):
@@ -51,7 +58,7 @@ def make(builder, mirror, dt_pheno=None, mixed=False):
 
 
 @pytest.mark.parametrize("mixed", [False, True])
-@pytest.mark.parametrize("builder", ["config0", "small3d", "config1", "crowded"])
+@pytest.mark.parametrize("builder", ["config0", "small3d", "config1", "crowded", "packed"])
 @pytest.mark.parametrize("use_graph", [False, True])
 def test_mirror_equals_direct_periods(builder, use_graph, mixed):
     print(f"mirror {builder} graph={use_graph} mixed={mixed}", flush=True)
@@ -73,6 +80,9 @@ def test_mirror_equals_direct_periods(builder, use_graph, mixed):
     if builder == "crowded":
         sizes = np.diff(off)
         assert sizes.max() > 16 and ((sizes > 0) & (sizes <= 16)).any(), "both bin paths"
+    if builder == "packed":
+        sizes = np.diff(off)
+        assert (sizes[:-2] + sizes[1:-1] + sizes[2:]).max() > 64, "rows above 64 records"
 
 
 @pytest.mark.parametrize("builder", ["config0", "config1"])
