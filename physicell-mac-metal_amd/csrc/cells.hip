@@ -409,14 +409,15 @@ __global__ __launch_bounds__(256) void k_forces(const CellRec* __restrict__ rec,
 }
 
 // FP64 forces on the mirror with the x-trimmed mark pass (force_agent64t):
-// pos32 in (bin, x slab, index) order with record positions, xoff the slab
-// starts; velocity and pressure to vout[t] in record order.  The agent's own
-// FP32 mark coordinates come from its FP64 record by the gather's
-// expressions.  Agents more than two bins from their bin corner take the
-// exact FP64 pass over the rows.
-__global__ __launch_bounds__(256) void k_forces_xt(const CellRec* __restrict__ rec, const float4* __restrict__ pos32,
+// 8-byte mark entries in (bin, x slab, index) order, xoff the slab starts;
+// velocity and pressure to vout[t] in record order.  The agent's own mark
+// coordinates come from its FP64 record.  Agents more than two bins from
+// their bin corner, and every agent of a step whose gather set xtflag, take
+// the exact FP64 pass over the rows.
+__global__ __launch_bounds__(256) void k_forces_xt(const CellRec* __restrict__ rec, const uint2* __restrict__ mark,
                                                    const uint32_t* __restrict__ offsets,
-                                                   const uint32_t* __restrict__ xoff, uint32_t n, uint32_t mnx,
+                                                   const uint32_t* __restrict__ xoff,
+                                                   const uint32_t* __restrict__ xtflag, uint32_t n, uint32_t mnx,
                                                    uint32_t mny, uint32_t mnz, double max_dist, double voxel,
                                                    double x_min, double y_min, double z_min,
                                                    const float* __restrict__ rmax, double4* __restrict__ vout) {
@@ -429,14 +430,15 @@ __global__ __launch_bounds__(256) void k_forces_xt(const CellRec* __restrict__ r
     double fx = 0.0, fy = 0.0, fz = 0.0, local_pressure = 0.0;
     const uint32_t vi = me.key % mnx, vj = (me.key / mnx) % mny, vk = me.key / (mnx * mny);
     const double cut2 = max_dist * max_dist;
-    const float mxg = (float)(px - x_min), my = (float)(py - (y_min + vj * voxel)),
-                mz = (float)(pz - (z_min + vk * voxel));
+    const double yb = py - (y_min + vj * voxel), zb = pz - (z_min + vk * voxel);
+    const float mxg = (float)(px - x_min);
     const float vox = (float)voxel, near = 2.0f * vox;
-    if (fabsf(my) < near && fabsf(mz) < near && mxg > -near && mxg < (float)mnx * vox + near) {
+    if (xtflag[0] == 0u && fabs(yb) < (double)near && fabs(zb) < (double)near && mxg > -near &&
+        mxg < (float)mnx * vox + near) {
         uint32_t* list = marks + (threadIdx.x >> 6) * (MARKX_CAP * 64) + (threadIdx.x & 63);
-        force_agent64t(rec, pos32, xoff, list, vi, vj, vk, mnx, mny, mnz, mxg, bin_rel32(px, x_min, vi, voxel), my,
-                       mz, vox, xslab_scale(voxel), rmax[0], px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2,
-                       fx, fy, fz, local_pressure);
+        force_agent64t(rec, mark, xoff, list, vi, vj, vk, mnx, mny, mnz, mxg, bin_rel32(px, x_min, vi, voxel),
+                       (float)(yb + 64.0), (float)(zb + 64.0), vox, xslab_scale(voxel), rmax[0], px, py, pz, radius_i,
+                       rep_i, adh_i, rel_max_adh, cut2, fx, fy, fz, local_pressure);
     } else {
 #pragma unroll 1
         for (int q = 0; q < 9; q++) {
@@ -662,6 +664,8 @@ HashLayout hash_layout(uint32_t n_mvox, uint32_t max_cells) {
     // FP32 records: slab starts per bin (and past the last bin), slab per agent
     L.xoff = o;    o = align256(o + ((size_t)n_mvox + 1) * XSLABS * 4);
     L.xslab = o;   o = align256(o + (size_t)max_cells);
+    // set by the gather when a member does not fit its 8-byte mark entry
+    L.xtflag = o;  o = align256(o + 4);
     L.total = o;
     return L;
 }
@@ -965,6 +969,7 @@ struct HashPtrs {
     float *rpart, *rmax;
     uint32_t* xoff;
     uint8_t* xslab;
+    uint32_t* xtflag;
 };
 
 static HashPtrs hash_ptrs(hc_buf* hash, uint32_t n_mvox, uint32_t max_cells) {
@@ -982,6 +987,7 @@ static HashPtrs hash_ptrs(hc_buf* hash, uint32_t n_mvox, uint32_t max_cells) {
     h.pos32 = (float4*)(P + L.pos32);
     h.xoff = (uint32_t*)(P + L.xoff);
     h.xslab = (uint8_t*)(P + L.xslab);
+    h.xtflag = (uint32_t*)(P + L.xtflag);
     h.rpart = (float*)(P + L.rpart);
     h.rmax = (float*)(P + L.rmax);
     return h;
@@ -1441,15 +1447,16 @@ __global__ __launch_bounds__(256) void k_m_bin_sort(const uint32_t* __restrict__
 
 // This step's records (k_stable_gather's values) and FP32 mark positions
 // from the slots; slots of bins above BIN_SORT_CAP are ranked here by agent
-// index.  xt: the mark positions go in (bin, x slab, index) order, each with
-// its record position in w (force_agent64t), else in record order with the
-// radius in w.
+// index.  xt: 8-byte mark entries (mark_entry16) in (bin, x slab, index)
+// order into pos32's memory (force_agent64t), a member that does not fit
+// setting xtflag; else FP32 mark positions in record order with the radius
+// in w.
 __global__ __launch_bounds__(256) void k_m_gather(MSet a, const uint32_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ tmp, uint32_t* __restrict__ sorted,
                                                   CellRec* __restrict__ rec, float4* __restrict__ pos32, uint32_t n,
                                                   double x_min, double y_min, double z_min, double voxel,
                                                   uint32_t mnx, uint32_t mny, int x_grid, int xt,
-                                                  float* __restrict__ rpart) {
+                                                  uint32_t* __restrict__ xtflag, float* __restrict__ rpart) {
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
     const double4 pr = p < n ? a.prop[p] : make_double4(0.0, 0.0, 0.0, 0.0);
     if (rpart) block_max_to(__double2float_ru(pr.x), rpart);
@@ -1488,11 +1495,17 @@ __global__ __launch_bounds__(256) void k_m_gather(MSet a, const uint32_t* __rest
     }
     a.kstep[p] = b;
     const double4 x = a.pos[p];
-    if (pos32) {
+    if (xt) {
+        const uint32_t bj = (b / mnx) % mny, bk = b / (mnx * mny);
+        bool fits;
+        reinterpret_cast<uint2*>(pos32)[sp] = mark_entry16(x.x - x_min, x.y - (y_min + bj * voxel),
+                                                           x.z - (z_min + bk * voxel), (int)pos - (int)sp, fits);
+        if (!fits) xtflag[0] = 1u;
+    } else if (pos32) {
         const uint32_t bi = b % mnx, bj = (b / mnx) % mny, bk = b / (mnx * mny);
-        pos32[xt ? sp : pos] = make_float4((float)(x.x - (x_grid ? x_min : x_min + bi * voxel)),
-                                           (float)(x.y - (y_min + bj * voxel)), (float)(x.z - (z_min + bk * voxel)),
-                                           xt ? __uint_as_float(pos) : (float)pr.x);
+        pos32[pos] = make_float4((float)(x.x - (x_grid ? x_min : x_min + bi * voxel)),
+                                 (float)(x.y - (y_min + bj * voxel)), (float)(x.z - (z_min + bk * voxel)),
+                                 (float)pr.x);
     }
     CellRec r;
     r.x = x.x;
@@ -1531,20 +1544,22 @@ __global__ __launch_bounds__(256) void k_m_load32(const CellRec32* __restrict__ 
 
 // Bins of up to BIN_SORT_CAP members (slots in tmp) in ascending agent index
 // (the bin's list) and each slot's place by (x slab, index): the mixed
-// form's FP32 record position (rpos), or, REC64, the plain form's FP32 mark
+// form's FP32 record position (rpos), or, REC64, the plain form's mark entry
 // position (spos) beside its record position by index (rpos); every bin's
 // slab starts (xoff) and the entries past the last bin (grid of nbins + 1
-// threads).  Larger bins: slab starts only (the gather ranks their slots).
+// threads; REC64: that thread also clears xtflag).  Larger bins: slab starts
+// only (the gather ranks their slots).
 template <bool REC64>
 __global__ __launch_bounds__(256) void k_m_bin_sort32(const uint32_t* __restrict__ offsets,
                                                       const uint32_t* __restrict__ tmp, MSet a,
                                                       uint32_t* __restrict__ sorted, uint32_t* __restrict__ xoff,
-                                                      uint32_t nbins) {
+                                                      uint32_t nbins, uint32_t* __restrict__ xtflag) {
     const uint32_t b = blockIdx.x * 256 + threadIdx.x;
     if (b > nbins) return;
     if (b == nbins) {
         const uint32_t n = offsets[nbins];
         *reinterpret_cast<uint4*>(xoff + (size_t)XSLABS * nbins) = make_uint4(n, n, n, n);
+        if (REC64) xtflag[0] = 0u;                       // the gather that follows sets it
         return;
     }
     const uint32_t beg = offsets[b], cnt = offsets[b + 1] - beg;
@@ -1821,7 +1836,7 @@ int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, 
             if ((r = scan_exclusive(ctx, h.counts, h.offsets, n_mvox, h.partial))) return r;
             hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, ctx->stream, a.key, a.slot, h.offsets, h.tmp, n);
             hipLaunchKernelGGL(k_m_bin_sort32<false>, dim3(blocks_for(n_mvox + 1, 256)), dim3(256), 0, ctx->stream,
-                               h.offsets, h.tmp, a, h.sorted, h.xoff, n_mvox);
+                               h.offsets, h.tmp, a, h.sorted, h.xoff, n_mvox, (uint32_t*)nullptr);
             hipLaunchKernelGGL(k_m_gather32, dim3(nb), dim3(256), 0, ctx->stream, a, h.offsets, h.tmp, h.sorted,
                                reinterpret_cast<CellRec32*>(h.rec), n, mgrid->x_min, mgrid->y_min, mgrid->z_min,
                                mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny, h.rpart);
@@ -1842,13 +1857,13 @@ int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, 
         hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, ctx->stream, a.key, a.slot, h.offsets, h.tmp, n);
         if (xt)
             hipLaunchKernelGGL(k_m_bin_sort32<true>, dim3(blocks_for(n_mvox + 1, 256)), dim3(256), 0, ctx->stream,
-                               h.offsets, h.tmp, a, h.sorted, h.xoff, n_mvox);
+                               h.offsets, h.tmp, a, h.sorted, h.xoff, n_mvox, h.xtflag);
         else
             hipLaunchKernelGGL(k_m_bin_sort, dim3(blocks_for(n_mvox, 256)), dim3(256), 0, ctx->stream, h.offsets,
                                h.tmp, a, h.sorted, n_mvox);
         hipLaunchKernelGGL(k_m_gather, dim3(nb), dim3(256), 0, ctx->stream, a, h.offsets, h.tmp, h.sorted, h.rec,
                            mark32_ok(mech) ? h.pos32 : nullptr, n, mgrid->x_min, mgrid->y_min, mgrid->z_min,
-                           mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny, xg, xt,
+                           mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny, xg, xt, h.xtflag,
                            xg ? h.rpart : nullptr);
         if (xg) hipLaunchKernelGGL(k_rmax, dim3(1), dim3(256), 0, ctx->stream, h.rpart, nb, h.rmax);
         if ((r = check_launch(ctx, "mirror hash"))) return r;
@@ -1856,8 +1871,9 @@ int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, 
     if (!fp32) {
         KernelTimer tm(ctx, HC_K_FORCES);
         if (xt)
-            hipLaunchKernelGGL(k_forces_xt, dim3(nb), dim3(256), 0, ctx->stream, h.rec, h.pos32, h.offsets, h.xoff,
-                               n, mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz,
+            hipLaunchKernelGGL(k_forces_xt, dim3(nb), dim3(256), 0, ctx->stream, h.rec,
+                               reinterpret_cast<const uint2*>(h.pos32), h.offsets, h.xoff, h.xtflag, n,
+                               mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz,
                                mech->max_interaction_distance, mech->mechanics_voxel_size, mgrid->x_min,
                                mgrid->y_min, mgrid->z_min, h.rmax, vp);
         else

@@ -480,25 +480,26 @@ __device__ __attribute__((always_inline)) inline void force_pair64(
 // list order, LAW_BATCH entries at a time: the records of a batch (48 of
 // their 64 bytes each) are loaded together, then the pairs run in order.
 constexpr uint32_t LAW_BATCH = 4;
+template <uint32_t BATCH = LAW_BATCH>
 __device__ __attribute__((always_inline)) inline void force_list64(
     const CellRec* __restrict__ rec, const uint32_t* __restrict__ list, uint32_t cnt, double px, double py,
     double pz, double radius_i, double rep_i, double adh_i, double rel_max_adh, double cut2, double& fx, double& fy,
     double& fz, double& local_pressure) {
     const double relf = rel_max_adh > 1.0 ? rel_max_adh : 1.0;
-    for (uint32_t k0 = 0; k0 < cnt; k0 += LAW_BATCH) {
-        uint32_t u[LAW_BATCH];
+    for (uint32_t k0 = 0; k0 < cnt; k0 += BATCH) {
+        uint32_t u[BATCH];
         u[0] = list[k0 * 64];
 #pragma unroll
-        for (uint32_t j = 1; j < LAW_BATCH; j++) u[j] = k0 + j < cnt ? list[(k0 + j) * 64] : u[0];
-        double2 xy[LAW_BATCH], zr[LAW_BATCH], ra[LAW_BATCH];
+        for (uint32_t j = 1; j < BATCH; j++) u[j] = k0 + j < cnt ? list[(k0 + j) * 64] : u[0];
+        double2 xy[BATCH], zr[BATCH], ra[BATCH];
 #pragma unroll
-        for (uint32_t j = 0; j < LAW_BATCH; j++) {
+        for (uint32_t j = 0; j < BATCH; j++) {
             xy[j] = *reinterpret_cast<const double2*>(&rec[u[j]].x);
             zr[j] = *reinterpret_cast<const double2*>(&rec[u[j]].z);
             ra[j] = *reinterpret_cast<const double2*>(&rec[u[j]].rep);
         }
 #pragma unroll
-        for (uint32_t j = 0; j < LAW_BATCH; j++)
+        for (uint32_t j = 0; j < BATCH; j++)
             if (k0 + j < cnt)
                 force_pair64(xy[j], zr[j], ra[j], px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, relf, cut2, fx, fy,
                              fz, local_pressure);
@@ -727,30 +728,52 @@ __device__ __attribute__((always_inline)) inline void force_agent32x(
                  adh_i, rel_max_adh, cut2, fx, fy, fz, local_pressure);
 }
 
-// FP64 forces of one agent with an x-trimmed FP32 mark pass.  The records
-// stay in (bin, index) order; pos32 holds the same members per bin in (x
-// slab, index) order, each entry's w the member's record position (the bits
-// of a uint32), with the slab starts in xoff as for the FP32 records.
+// FP64 forces of one agent with an x-trimmed mark pass over 8-byte mark
+// entries.  The records stay in (bin, index) order; the mark entries hold the
+// same members per bin in (x slab, index) order (slab starts in xoff, as for
+// the FP32 records), each a uint2 (MarkEntry16):
+//   x: x - x_min in 1/16 um (16 bits) | y - the bin's lower y in 1/512 um,
+//      plus 64 um (16 bits);
+//   y: z as y (16 bits) | record position - mark position (signed 16 bits).
+// The gather sets *flag when a member does not fit (then every agent takes
+// the exact pass).  Quantisation moves x by at most 1/32 um and y, z by at
+// most 1/1024 um, the agent's FP32 coordinates and the FMA differences add
+// below 2.5e-4 um, so for candidates within 32 um the mark's d^2 is within
+// 2.14 um^2 of the exact one: the threshold carries MARK16_SLACK = 2.25 um^2
+// above force_agent64x's bound, and every pair the law keeps is marked.  A
+// marked candidate has |dx| < sqrt(lim) + 0.032 um, inside the walked x
+// interval (force_agent32x's, with XTRIM_MARGIN = 0.05 um).
 //
-// Mark: force_agent64x's threshold and coordinates (x relative to x_min, y
-// and z relative to the agent's bin corner), so every pair the law keeps is
-// marked; rows: force_agent32x's x interval and slab range, with mxb the
-// agent's bin-relative x from the same expression as the slabs'.  A marked
-// candidate sets the bit of its record position relative to the row's first
-// record in a 64-bit mask (a row of more than 64 records takes one walk per
-// 64 of them), and the set bits go to the LDS list in ascending order: the
-// law sees the marked candidates in the reference's visit order (rows, then
-// ascending record position) without a sort, re-tests each with the exact
-// FP64 predicates, and the sums are force_agent64x's bit for bit.
+// A marked candidate sets the bit of its record position relative to a
+// 64-record window of the row (the row's first record, then 64 on: a row of
+// more than 64 records takes one walk per window), and the set bits go to
+// the LDS list in ascending order: the law sees the marked candidates in the
+// reference's visit order (rows, then ascending record position) without a
+// sort, re-tests each with the exact FP64 predicates, and the sums are
+// force_agent64x's bit for bit.
+constexpr float MARK16_SLACK = 2.25f;
+constexpr uint32_t MARK16_STEP = 4;   // entries per step (their loads in flight together)
+constexpr uint32_t LAW16_BATCH = 2;   // records per law batch (fewer registers: 5 waves per SIMD)
+__host__ __device__ inline uint2 mark_entry16(double xg, double yb, double zb, int disp, bool& ok) {
+    const double qx = rint(xg * 16.0), qy = rint((yb + 64.0) * 512.0), qz = rint((zb + 64.0) * 512.0);
+    ok = qx >= 0.0 && qx <= 65535.0 && qy >= 0.0 && qy <= 65535.0 && qz >= 0.0 && qz <= 65535.0 &&
+         disp >= -32768 && disp <= 32767;
+    if (!ok) return make_uint2(0u, 0u);
+    return make_uint2((uint32_t)qx | ((uint32_t)qy << 16), (uint32_t)qz | ((uint32_t)disp << 16));
+}
+
+// mxg: the agent's x - x_min; my64, mz64: its y, z relative to its bin's
+// lower corner plus 64 um (FP32 of the FP64 values); mxb: its bin-relative x
+// (bin_rel32, the slabs' expression)
 __device__ __attribute__((always_inline)) inline void force_agent64t(
-    const CellRec* __restrict__ rec, const float4* __restrict__ pos32, const uint32_t* __restrict__ xoff,
+    const CellRec* __restrict__ rec, const uint2* __restrict__ mark, const uint32_t* __restrict__ xoff,
     uint32_t* __restrict__ list, uint32_t vi, uint32_t vj, uint32_t vk, uint32_t mnx, uint32_t mny, uint32_t mnz,
-    float mxg, float mxb, float my, float mz, float voxel, float scale, float rmax, double px, double py, double pz,
-    double radius_i, double rep_i, double adh_i, double rel_max_adh, double cut2, double& fx, double& fy,
+    float mxg, float mxb, float my64, float mz64, float voxel, float scale, float rmax, double px, double py,
+    double pz, double radius_i, double rep_i, double adh_i, double rel_max_adh, double cut2, double& fx, double& fy,
     double& fz, double& local_pressure) {
     const float relf32 = (float)(rel_max_adh > 1.0 ? rel_max_adh : 1.0);
     const float L = ((float)radius_i + rmax) * relf32;
-    const float lim = fminf(L * L * (1.0f + 0x1p-16f), (float)cut2) + MARKX_SLACK;
+    const float lim = fminf(L * L * (1.0f + 0x1p-16f), (float)cut2) + MARK16_SLACK;
     const float Lx = sqrtf(lim) + XTRIM_MARGIN;
     const float lo = mxb - Lx, hi = mxb + Lx;
     const bool left = lo < 0.0f && vi > 0, right = hi >= voxel && vi + 1 < mnx;
@@ -770,27 +793,29 @@ __device__ __attribute__((always_inline)) inline void force_agent64t(
         // walked slab range [sb, se); records of the walked bins [r0, r1)
         const uint32_t sb = ok ? xoff[rb + jlo] : 0u, se = ok ? xoff[rb + jhi] : 0u;
         const uint32_t r0 = ok ? xoff[rb + XSLABS * blo] : 0u, r1 = ok ? xoff[rb + XSLABS * (bhi + 1)] : 0u;
-        const float myr = my - (float)dj * voxel, mzr = mz - (float)dk * voxel;
+        const float myr = my64 - (float)dj * voxel, mzr = mz64 - (float)dk * voxel;
         for (uint32_t wb = r0; sb < se && wb < r1; wb += 64) {
             uint64_t mask = 0;
-            for (uint32_t u0 = sb; u0 < se; u0 += MARKX_STEP) {
-                const float4* p = pos32 + u0;
-                const uint32_t rem = se - u0;
-                float4 o[MARKX_STEP];
+            for (uint32_t u0 = sb; u0 < se; u0 += MARK16_STEP) {
+                const uint2* p = mark + u0;
+                const uint32_t rem = se - u0, base = u0 - wb;
+                uint2 o[MARK16_STEP];
 #pragma unroll
-                for (uint32_t j = 0; j < MARKX_STEP; j++) o[j] = p[j];
+                for (uint32_t j = 0; j < MARK16_STEP; j++) o[j] = p[j];
 #pragma unroll
-                for (uint32_t j = 0; j < MARKX_STEP; j++) {
-                    const float ddx = mxg - o[j].x, ddy = myr - o[j].y, ddz = mzr - o[j].z;
+                for (uint32_t j = 0; j < MARK16_STEP; j++) {
+                    const float ddx = __builtin_fmaf((float)(o[j].x & 0xffffu), -0.0625f, mxg);
+                    const float ddy = __builtin_fmaf((float)(o[j].x >> 16), -0x1p-9f, myr);
+                    const float ddz = __builtin_fmaf((float)(o[j].y & 0xffffu), -0x1p-9f, mzr);
                     const float d2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(ddy, ddy, ddx * ddx));
-                    const uint32_t rp = __float_as_uint(o[j].w) - wb;
+                    const uint32_t rp = base + j + (uint32_t)((int32_t)o[j].y >> 16);
                     mask |= (j < rem && d2 < lim && rp < 64u) ? (1ull << (rp & 63u)) : 0ull;
                 }
             }
             while (mask) {
                 if (w == wend) {
-                    force_list64(rec, list, MARKX_CAP, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy,
-                                 fz, local_pressure);
+                    force_list64<LAW16_BATCH>(rec, list, MARKX_CAP, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh,
+                                              cut2, fx, fy, fz, local_pressure);
                     w = l0;
                 }
                 *w = wb + (uint32_t)__builtin_ctzll(mask);
@@ -799,8 +824,8 @@ __device__ __attribute__((always_inline)) inline void force_agent64t(
             }
         }
     }
-    force_list64(rec, list, (uint32_t)(w - l0) / 64u, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh, cut2, fx, fy,
-                 fz, local_pressure);
+    force_list64<LAW16_BATCH>(rec, list, (uint32_t)(w - l0) / 64u, px, py, pz, radius_i, rep_i, adh_i, rel_max_adh,
+                              cut2, fx, fy, fz, local_pressure);
 }
 
 // The chunked pass over the 9 (3 in 2-D) neighbour rows of an agent.

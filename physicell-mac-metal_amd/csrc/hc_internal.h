@@ -250,7 +250,7 @@ int launch_dirichlet(hc_ctx* ctx, void* rho, const hc_grid* g, const uint8_t* ta
 
 // Hash workspace layout (byte offsets inside the hash buffer).
 struct HashLayout {
-    size_t counts, offsets, slot, tmp, sorted, rec, pos_of, pos32, rpart, rmax, partial, xoff, xslab, total;
+    size_t counts, offsets, slot, tmp, sorted, rec, pos_of, pos32, rpart, rmax, partial, xoff, xslab, xtflag, total;
 };
 HashLayout hash_layout(uint32_t n_mvox, uint32_t max_cells);
 

@@ -137,3 +137,22 @@ def test_mirror_resort_long_run(builder, mixed):
         out.append(st)
         sim.close()
     assert_same(out[0], out[1], f"{builder} 140 periods mixed={mixed}")
+
+
+def test_mirror_giant_bin_takes_exact_pass():
+    """One 30 um bin holding 48000 agents: their record and mark positions lie
+    more than 32767 apart, beyond the 16-bit field of the plain form's 8-byte
+    mark entries, so the gather flags the step and every agent takes the
+    exact FP64 pass; the run still equals the direct one bit for bit."""
+    spec, cells = S.small3d(2, 2, 1, n_cells=48000, seed=3)
+    out = []
+    for mirror in (True, False):
+        sim = H.Simulation(spec, cells)
+        sim.set_mechanics_mirror(mirror)
+        sim.run_periods(2)
+        sim.wait()
+        out.append(state(sim))
+        sim.close()
+    sizes = np.diff(out[0]["offsets"].astype(np.int64))
+    assert sizes.max() > 43690, "one bin large enough to overflow the entry"
+    assert_same(out[0], out[1], "giant bin")
