@@ -574,7 +574,7 @@ __global__ __launch_bounds__(256) void k_bm_gather(const EnsSim* __restrict__ si
     const uint32_t c = tl.c0 + threadIdx.x;
     const uint32_t p = m.cell_off + c;
     const float4 pr = c < m.n ? mr.prop[p] : make_float4(0.f, 0.f, 0.f, 0.f);
-    block_max_to(pr.x, rpart);
+    if (rpart) block_max_to(pr.x, rpart);
     if (c >= m.n) return;
     const uint32_t b = mr.key[p];
     const uint32_t beg = offsets[b], end = offsets[b + 1];
@@ -1567,7 +1567,10 @@ private:
 
     // hash of the mirror's coming step: lists, record positions, slab starts
     // and the FP32 records (counts hold the histogram of the mirror's bins)
-    void mirror_hash(const hc::BMirror& a) {
+    // the hash of the slots' coming bins.  at_load: also the largest radius
+    // (the slots' radii change only by a reload), and the counts stay for
+    // the first step's hash; else the scan clears them for AB2's histogram.
+    void mirror_hash(const hc::BMirror& a, bool at_load) {
         Workspace& w = ws_;
         hipStream_t st = hc_stream_of(ctx_);
         const EnsSim* sims = (const EnsSim*)hc_device_ptr(w.sims.get());
@@ -1579,15 +1582,17 @@ private:
         uint32_t* xoff = (uint32_t*)hc_device_ptr(w.xoff.get());
         float* rpart = (float*)hc_device_ptr(w.rpart.get());
         if (!tmp64) throw std::runtime_error("ensemble: agent mirror not allocated");
-        check(ctx_, hc::scan_exclusive(ctx_, counts, offsets, w.nbins, (uint32_t*)hc_device_ptr(w.partial.get())),
+        check(ctx_, hc::scan_exclusive(ctx_, counts, offsets, w.nbins, (uint32_t*)hc_device_ptr(w.partial.get()),
+                                       at_load ? nullptr : counts),
               "scan");
         hipLaunchKernelGGL(hc::k_bm_scatter, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, a, offsets, tmp64);
         hipLaunchKernelGGL(hc::k_bm_bin_sort, dim3(hc::blocks_for(w.nbins + 1, 256)), dim3(256), 0, st, offsets,
                            tmp64, a, sorted, xoff, w.nbins);
         hipLaunchKernelGGL(hc::k_bm_gather, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, a, offsets, tmp64, sorted,
-                           (hc::CellRec32*)hc_device_ptr(w.recs.get()), rpart);
+                           (hc::CellRec32*)hc_device_ptr(w.recs.get()), at_load ? rpart : nullptr);
         check(ctx_, hc::check_launch(ctx_, "mirror hash"), "mirror hash");
-        check(ctx_, hc::launch_rmax(ctx_, rpart, w.ntiles, (float*)hc_device_ptr(w.rmax.get())), "largest radius");
+        if (at_load)
+            check(ctx_, hc::launch_rmax(ctx_, rpart, w.ntiles, (float*)hc_device_ptr(w.rmax.get())), "largest radius");
     }
 
     // every slot's state to its record position of the last step
@@ -1615,7 +1620,7 @@ private:
         check(ctx_, hc::fill_u32(ctx_, counts, w.nbins, 0), "fill");
         hipLaunchKernelGGL(hc::k_bm_load, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, a, counts);
         check(ctx_, hc::check_launch(ctx_, "k_bm_load"), "mirror load");
-        mirror_hash(a);
+        mirror_hash(a, true);
         mirror_resort();
         mirror_valid_ = true;
         soa_stale_ = false;
@@ -1643,7 +1648,7 @@ private:
         const hc::BMirror a = mirror_set(0);
         {
             hc::KernelTimer tm(ctx_, HC_K_HASH);
-            mirror_hash(a);
+            mirror_hash(a, false);
         }
         {
             hc::KernelTimer tm(ctx_, HC_K_FORCES);
@@ -1655,7 +1660,6 @@ private:
         }
         {
             hc::KernelTimer tm(ctx_, HC_K_INTEGRATE);
-            check(ctx_, hc::fill_u32(ctx_, counts, w.nbins, 0), "fill");
             hipLaunchKernelGGL(hc::k_bm_integrate, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, a, counts);
             check(ctx_, hc::check_launch(ctx_, "k_bm_integrate"), "integrate");
         }

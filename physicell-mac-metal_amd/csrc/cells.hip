@@ -139,16 +139,27 @@ __global__ __launch_bounds__(SCAN_BS) void k_scan2_reduce(const uint32_t* __rest
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-// in and out may be the same array (in-place scans)
+// in and out may be the same array (in-place scans); clear: null, or in's
+// address when out is another array (each thread zeroes what it read)
 __global__ __launch_bounds__(SCAN_BS) void k_scan2_down(const uint32_t* in, uint32_t* out,
                                                         uint64_t n, const uint32_t* __restrict__ partial,
-                                                        uint32_t nb) {
+                                                        uint32_t nb, uint32_t* clear) {
     uint32_t pre = 0;
     for (uint32_t i = threadIdx.x; i < blockIdx.x; i += SCAN_BS) pre += partial[i];
     const uint32_t tile_base = block_sum(pre);
     const uint64_t base = (uint64_t)blockIdx.x * SCAN2_TILE + (uint64_t)threadIdx.x * SCAN2_PER;
     uint32_t v[SCAN2_PER];
     scan2_load(in, base, n, v);
+    if (clear) {
+        if (base + SCAN2_PER <= n) {
+#pragma unroll
+            for (int q = 0; q < SCAN2_PER / 4; q++)
+                *reinterpret_cast<uint4*>(clear + base + 4 * q) = make_uint4(0u, 0u, 0u, 0u);
+        } else {
+            for (int k = 0; k < SCAN2_PER; k++)
+                if (base + k < n) clear[base + k] = 0u;
+        }
+    }
     uint32_t sum = 0;
 #pragma unroll
     for (int k = 0; k < SCAN2_PER; k++) sum += v[k];
@@ -180,13 +191,18 @@ size_t scan_partial_bytes(uint64_t n) {
 }
 
 // in and out hold n (+1 for out) values from 16-byte aligned addresses
-int scan_exclusive(hc_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* partial) {
+int scan_exclusive(hc_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* partial,
+                   uint32_t* clear_in) {
     const uint64_t nb2 = (n + SCAN2_TILE - 1) / SCAN2_TILE;
     if (nb2 > 0 && nb2 <= SCAN2_MAX_TILES && ((uintptr_t)in & 15) == 0 && ((uintptr_t)out & 15) == 0) {
         hipLaunchKernelGGL(k_scan2_reduce, dim3((uint32_t)nb2), dim3(SCAN_BS), 0, ctx->stream, in, n, partial);
         hipLaunchKernelGGL(k_scan2_down, dim3((uint32_t)nb2), dim3(SCAN_BS), 0, ctx->stream, in, out, n, partial,
-                           (uint32_t)nb2);
+                           (uint32_t)nb2, clear_in);
         return check_launch(ctx, "scan");
+    }
+    if (clear_in) {
+        const int r = scan_exclusive(ctx, in, out, n, partial);
+        return r ? r : fill_u32(ctx, clear_in, n, 0u);
     }
     const uint32_t nb = (uint32_t)((n + SCAN_TILE - 1) / SCAN_TILE);
     if (nb == 0) {
@@ -1356,7 +1372,10 @@ int hc_substrate_totals_noalloc(hc_ctx* ctx, hc_buf* density, const hc_grid* g, 
 // mechanics voxel) are written back, by agent index, only when something
 // else reads them (mirror_sync).  Keys, bins, the in-bin order (ascending
 // agent index), the records and every arithmetic step are those of the
-// direct path, so the results are the same bit for bit.
+// direct path, so the results are the same bit for bit.  The slots' radii
+// change only by a reload (mirror_load's hash build), so the largest radius
+// that build computes serves every step until the next load; the scan clears
+// the bin counts it reads, for AB2's histogram of the coming step.
 // ---------------------------------------------------------------------------
 namespace hc {
 
@@ -1575,11 +1594,13 @@ __global__ __launch_bounds__(256) void k_m_bin_sort32(const uint32_t* __restrict
             make_uint4(beg, beg + c0, beg + c0 + c1, beg + c0 + c1 + c2);
         return;
     }
+    // (index, slab, slot): the slab rides along with the sort (slots below
+    // 2^30), so it is loaded with the index instead of after the sort
     uint64_t v[BIN_SORT_CAP];
 #pragma unroll
     for (uint32_t q = 0; q < BIN_SORT_CAP; q++) {
         const uint32_t m = q < cnt ? tmp[beg + q] : 0u;
-        v[q] = q < cnt ? ((uint64_t)a.idx[m] << 32) | m : ~0ull;
+        v[q] = q < cnt ? ((uint64_t)a.idx[m] << 32) | ((uint64_t)a.xslab[m] << 30) | m : ~0ull;
     }
 #pragma unroll
     for (uint32_t k = 2; k <= BIN_SORT_CAP; k <<= 1)
@@ -1598,7 +1619,7 @@ __global__ __launch_bounds__(256) void k_m_bin_sort32(const uint32_t* __restrict
     uint32_t sl[BIN_SORT_CAP];
 #pragma unroll
     for (uint32_t q = 0; q < BIN_SORT_CAP; q++) {
-        sl[q] = q < cnt ? a.xslab[(uint32_t)v[q]] : XSLABS;
+        sl[q] = q < cnt ? ((uint32_t)v[q] >> 30) : XSLABS;
         c0 += sl[q] == 0 ? 1u : 0u;
         c1 += sl[q] == 1 ? 1u : 0u;
         c2 += sl[q] == 2 ? 1u : 0u;
@@ -1612,11 +1633,12 @@ __global__ __launch_bounds__(256) void k_m_bin_sort32(const uint32_t* __restrict
             sorted[beg + q] = (uint32_t)(v[q] >> 32);
             const uint32_t sq = sl[q];
             const uint32_t sp = sq == 0 ? n0 : (sq == 1 ? n1 : (sq == 2 ? n2 : n3));
+            const uint32_t m = (uint32_t)v[q] & 0x3fffffffu;
             if (REC64) {
-                a.rpos[(uint32_t)v[q]] = beg + q;
-                a.spos[(uint32_t)v[q]] = sp;
+                a.rpos[m] = beg + q;
+                a.spos[m] = sp;
             } else {
-                a.rpos[(uint32_t)v[q]] = sp;
+                a.rpos[m] = sp;
             }
             n0 += sq == 0 ? 1u : 0u;
             n1 += sq == 1 ? 1u : 0u;
@@ -1635,7 +1657,7 @@ __global__ __launch_bounds__(256) void k_m_gather32(MSet a, const uint32_t* __re
                                                     uint32_t mny, float* __restrict__ rpart) {
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
     const double4 pr = p < n ? a.prop[p] : make_double4(0.0, 0.0, 0.0, 0.0);
-    block_max_to((float)pr.x, rpart);
+    if (rpart) block_max_to((float)pr.x, rpart);
     if (p >= n) return;
     const uint32_t b = a.key[p];
     const uint32_t beg = offsets[b], end = offsets[b + 1];
@@ -1773,6 +1795,8 @@ static int mirror_check(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech
                         uint32_t max_cells, uint32_t n_mvox, const char* who) {
     if (!ctx || !hash || !mirror || !mech) return fail(ctx, HC_ERR_ARG, std::string(who) + ": null argument");
     if (n > max_cells) return fail(ctx, HC_ERR_ARG, std::string(who) + ": n > max_cells");
+    // slots take 30 bits beside the x slab in the bin sort's values
+    if (max_cells > (1u << 30)) return fail(ctx, HC_ERR_ARG, std::string(who) + ": max_cells above 2^30");
     if (hc_buf_size(hash) < hc_hash_bytes(n_mvox, max_cells) || hc_buf_size(mirror) < mirror_layout(max_cells).total)
         return fail(ctx, HC_ERR_ARG, std::string(who) + ": buffer too small");
     return 0;
@@ -1833,14 +1857,13 @@ int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, 
         // the mixed form: FP32 records in (bin, x slab, index) order
         {
             KernelTimer tm(ctx, HC_K_HASH);
-            if ((r = scan_exclusive(ctx, h.counts, h.offsets, n_mvox, h.partial))) return r;
+            if ((r = scan_exclusive(ctx, h.counts, h.offsets, n_mvox, h.partial, h.counts))) return r;
             hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, ctx->stream, a.key, a.slot, h.offsets, h.tmp, n);
             hipLaunchKernelGGL(k_m_bin_sort32<false>, dim3(blocks_for(n_mvox + 1, 256)), dim3(256), 0, ctx->stream,
                                h.offsets, h.tmp, a, h.sorted, h.xoff, n_mvox, (uint32_t*)nullptr);
             hipLaunchKernelGGL(k_m_gather32, dim3(nb), dim3(256), 0, ctx->stream, a, h.offsets, h.tmp, h.sorted,
                                reinterpret_cast<CellRec32*>(h.rec), n, mgrid->x_min, mgrid->y_min, mgrid->z_min,
-                               mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny, h.rpart);
-            hipLaunchKernelGGL(k_rmax, dim3(1), dim3(256), 0, ctx->stream, h.rpart, nb, h.rmax);
+                               mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny, (float*)nullptr);
             if ((r = check_launch(ctx, "mirror hash (mixed)"))) return r;
         }
         KernelTimer tm(ctx, HC_K_FORCES);
@@ -1853,7 +1876,7 @@ int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, 
     } else {
         // the slots' bins and places in them come from the last AB2 (or load)
         KernelTimer tm(ctx, HC_K_HASH);
-        if ((r = scan_exclusive(ctx, h.counts, h.offsets, n_mvox, h.partial))) return r;
+        if ((r = scan_exclusive(ctx, h.counts, h.offsets, n_mvox, h.partial, h.counts))) return r;
         hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, ctx->stream, a.key, a.slot, h.offsets, h.tmp, n);
         if (xt)
             hipLaunchKernelGGL(k_m_bin_sort32<true>, dim3(blocks_for(n_mvox + 1, 256)), dim3(256), 0, ctx->stream,
@@ -1864,8 +1887,7 @@ int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, 
         hipLaunchKernelGGL(k_m_gather, dim3(nb), dim3(256), 0, ctx->stream, a, h.offsets, h.tmp, h.sorted, h.rec,
                            mark32_ok(mech) ? h.pos32 : nullptr, n, mgrid->x_min, mgrid->y_min, mgrid->z_min,
                            mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny, xg, xt, h.xtflag,
-                           xg ? h.rpart : nullptr);
-        if (xg) hipLaunchKernelGGL(k_rmax, dim3(1), dim3(256), 0, ctx->stream, h.rpart, nb, h.rmax);
+                           (float*)nullptr);
         if ((r = check_launch(ctx, "mirror hash"))) return r;
     }
     if (!fp32) {
@@ -1885,7 +1907,6 @@ int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, 
         if ((r = check_launch(ctx, "k_forces (mirror)"))) return r;
     }
     KernelTimer tm(ctx, HC_K_INTEGRATE);
-    if ((r = fill_u32(ctx, h.counts, n_mvox, 0))) return r;
     const double x_max = mgrid->x_min + (double)mgrid->nx * mgrid->dx;
     const double y_max = mgrid->y_min + (double)mgrid->ny * mgrid->dy;
     const double z_max = mgrid->z_min + (double)mgrid->nz * mgrid->dz;

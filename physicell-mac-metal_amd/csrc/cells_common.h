@@ -752,7 +752,7 @@ __device__ __attribute__((always_inline)) inline void force_agent32x(
 // sort, re-tests each with the exact FP64 predicates, and the sums are
 // force_agent64x's bit for bit.
 constexpr float MARK16_SLACK = 2.25f;
-constexpr uint32_t MARK16_STEP = 4;   // entries per step (their loads in flight together)
+constexpr uint32_t MARK16_STEP = 8;   // entries per step (their loads in flight together)
 constexpr uint32_t LAW16_BATCH = 2;   // records per law batch (fewer registers: 5 waves per SIMD)
 __host__ __device__ inline uint2 mark_entry16(double xg, double yb, double zb, int disp, bool& ok) {
     const double qx = rint(xg * 16.0), qy = rint((yb + 64.0) * 512.0), qz = rint((zb + 64.0) * 512.0);

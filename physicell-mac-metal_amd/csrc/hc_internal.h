@@ -282,9 +282,12 @@ struct PlanLayout {
 PlanLayout plan_layout(const hc_grid* g, uint32_t max_cells);
 
 // exclusive scan of n uint32 values in place into out (out may equal in),
-// out[n] = total; partial is scratch of scan_partial_bytes(n).
+// out[n] = total; partial is scratch of scan_partial_bytes(n).  clear_in
+// (in's address, out elsewhere): in is left zero (histogram counters for the
+// next pass), by the scan's last kernel where it can, else by a fill.
 size_t scan_partial_bytes(uint64_t n);
-int scan_exclusive(hc_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* partial);
+int scan_exclusive(hc_ctx* ctx, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* partial,
+                   uint32_t* clear_in = nullptr);
 // looks up this device's address of hc_substrate_totals_noalloc's static
 // scratch (once per device) and keeps it in ctx->totals_scratch
 int totals_scratch_init(hc_ctx* ctx);
