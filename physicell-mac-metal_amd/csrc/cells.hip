@@ -1366,8 +1366,9 @@ int hc_substrate_totals_noalloc(hc_ctx* ctx, hc_buf* density, const hc_grid* g, 
 // change bins); each slot's bin and place in the bin for the coming step are
 // computed by AB2 itself, with coherent ballots, and each slot knows its
 // record position of the step (rpos), through which AB2 reads its forces.
-// Per step the gather reads 68 bytes of slot state and writes the 64-byte
-// record, AB2 reads and writes 64 bytes of state beside its 32-byte forces.
+// Per step the gather reads 72 bytes of slot state (positions packed in 24
+// bytes) and writes the 64-byte record and the mark entry, AB2 reads and
+// writes 48 bytes of state beside its 32-byte forces.
 // The cell fields (position, velocity, previous velocity, pressure,
 // mechanics voxel) are written back, by agent index, only when something
 // else reads them (mirror_sync).  Keys, bins, the in-bin order (ascending
@@ -1379,12 +1380,26 @@ int hc_substrate_totals_noalloc(hc_ctx* ctx, hc_buf* density, const hc_grid* g, 
 // ---------------------------------------------------------------------------
 namespace hc {
 
+// three packed doubles (24 bytes): the slots' positions and previous
+// velocities, so AB2 and the gather move no padding
+struct Dbl3 {
+    double x, y, z;
+};
+static_assert(sizeof(Dbl3) == 24, "packed triple");
+__device__ inline Dbl3 dbl3(double x, double y, double z) {
+    Dbl3 r;
+    r.x = x;
+    r.y = y;
+    r.z = z;
+    return r;
+}
+
 MirrorLayout mirror_layout(uint32_t max_cells) {
     MirrorLayout L;
     size_t o = 0;
     for (int k = 0; k < 2; k++) {
-        L.pos[k] = o;   o = align256(o + (size_t)max_cells * sizeof(double4));
-        L.prev[k] = o;  o = align256(o + (size_t)max_cells * sizeof(double4));
+        L.pos[k] = o;   o = align256(o + (size_t)max_cells * sizeof(Dbl3));
+        L.prev[k] = o;  o = align256(o + (size_t)max_cells * sizeof(Dbl3));
         L.prop[k] = o;  o = align256(o + (size_t)max_cells * sizeof(double4));
         L.idx[k] = o;   o = align256(o + (size_t)max_cells * 4);
         L.key[k] = o;   o = align256(o + (size_t)max_cells * 4);
@@ -1401,7 +1416,8 @@ MirrorLayout mirror_layout(uint32_t max_cells) {
 
 // one set of slot arrays (set 1 receives a re-sort)
 struct MSet {
-    double4 *pos, *prev, *prop;     // prop: radius, repulsion, adhesion, relative max adhesion distance
+    Dbl3 *pos, *prev;
+    double4* prop;                  // radius, repulsion, adhesion, relative max adhesion distance
     uint32_t *idx, *key, *slot, *kstep, *rpos;
     uint8_t* xslab;                 // the x slab of the slot's FP32 record (or mark position) in its coming bin
     uint32_t* spos;                 // plain form, x-trimmed marks: the slot's mark position this step
@@ -1416,8 +1432,8 @@ __global__ __launch_bounds__(256) void k_m_load(const CellRec* __restrict__ rec,
     if (q >= n) return;
     const CellRec r = rec[q];
     if (slabs) a.xslab[q] = (uint8_t)xslab_of(bin_rel32(r.x, x_min, r.key % mnx, voxel), xslab_scale(voxel));
-    a.pos[q] = make_double4(r.x, r.y, r.z, 0.0);
-    a.prev[q] = make_double4(cp.f[HC_F_PREV_VX][r.idx], cp.f[HC_F_PREV_VY][r.idx], cp.f[HC_F_PREV_VZ][r.idx], 0.0);
+    a.pos[q] = dbl3(r.x, r.y, r.z);
+    a.prev[q] = dbl3(cp.f[HC_F_PREV_VX][r.idx], cp.f[HC_F_PREV_VY][r.idx], cp.f[HC_F_PREV_VZ][r.idx]);
     a.prop[q] = make_double4(r.r, r.rep, r.adh, r.rel);
     a.idx[q] = r.idx;
     a.key[q] = r.key;
@@ -1513,7 +1529,7 @@ __global__ __launch_bounds__(256) void k_m_gather(MSet a, const uint32_t* __rest
         a.rpos[p] = pos;
     }
     a.kstep[p] = b;
-    const double4 x = a.pos[p];
+    const Dbl3 x = a.pos[p];
     if (xt) {
         const uint32_t bj = (b / mnx) % mny, bk = b / (mnx * mny);
         bool fits;
@@ -1549,8 +1565,8 @@ __global__ __launch_bounds__(256) void k_m_load32(const CellRec32* __restrict__ 
     if (q >= n) return;
     const uint32_t c = rec[q].idx;
     const uint32_t key = cp.u[HC_F_MECH_VOXEL - HC_NUM_F64_FIELDS][c];
-    a.pos[q] = make_double4(cp.f[HC_F_POS_X][c], cp.f[HC_F_POS_Y][c], cp.f[HC_F_POS_Z][c], 0.0);
-    a.prev[q] = make_double4(cp.f[HC_F_PREV_VX][c], cp.f[HC_F_PREV_VY][c], cp.f[HC_F_PREV_VZ][c], 0.0);
+    a.pos[q] = dbl3(cp.f[HC_F_POS_X][c], cp.f[HC_F_POS_Y][c], cp.f[HC_F_POS_Z][c]);
+    a.prev[q] = dbl3(cp.f[HC_F_PREV_VX][c], cp.f[HC_F_PREV_VY][c], cp.f[HC_F_PREV_VZ][c]);
     a.prop[q] = make_double4(cp.f[HC_F_RADIUS][c], cp.f[HC_F_REPULSION][c], cp.f[HC_F_ADHESION][c],
                              cp.f[HC_F_REL_MAX_ADH][c]);
     a.idx[q] = c;
@@ -1678,7 +1694,7 @@ __global__ __launch_bounds__(256) void k_m_gather32(MSet a, const uint32_t* __re
         a.rpos[p] = pos;
     }
     a.kstep[p] = b;
-    const double4 x = a.pos[p];
+    const Dbl3 x = a.pos[p];
     const uint32_t bi = b % mnx, bj = (b / mnx) % mny, bk = b / (mnx * mny);
     CellRec32 r;
     r.xg = (float)(x.x - x_min);
@@ -1705,7 +1721,8 @@ __global__ __launch_bounds__(256) void k_m_integrate(MSet a, const double4* __re
     const bool valid = p < n;
     uint32_t key = 0;
     if (valid) {
-        const double4 v = vp[a.rpos[p]], pv = a.prev[p], x = a.pos[p];
+        const double4 v = vp[a.rpos[p]];
+        const Dbl3 pv = a.prev[p], x = a.pos[p];
         double px = x.x, py = x.y, pz = x.z;
         px += dt * (1.5 * v.x - 0.5 * pv.x);
         py += dt * (1.5 * v.y - 0.5 * pv.y);
@@ -1713,8 +1730,8 @@ __global__ __launch_bounds__(256) void k_m_integrate(MSet a, const double4* __re
         px = px < x_min ? x_min : (px > x_max ? x_max : px);
         py = py < y_min ? y_min : (py > y_max ? y_max : py);
         pz = pz < z_min ? z_min : (pz > z_max ? z_max : pz);
-        a.pos[p] = make_double4(px, py, pz, 0.0);
-        a.prev[p] = make_double4(v.x, v.y, v.z, 0.0);
+        a.pos[p] = dbl3(px, py, pz);
+        a.prev[p] = dbl3(v.x, v.y, v.z);
         int vi = (int)((px - mx_min) / voxel);
         int vj = (int)((py - my_min) / voxel);
         int vk = (int)((pz - mz_min) / voxel);
@@ -1734,7 +1751,8 @@ __global__ __launch_bounds__(256) void k_m_sync(MSet a, const double4* __restric
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
     if (p >= n) return;
     const uint32_t c = a.idx[p];
-    const double4 x = a.pos[p], pv = a.prev[p], v = vp[a.rpos[p]];
+    const Dbl3 x = a.pos[p], pv = a.prev[p];
+    const double4 v = vp[a.rpos[p]];
     cp.f[HC_F_POS_X][c] = x.x;
     cp.f[HC_F_POS_Y][c] = x.y;
     cp.f[HC_F_POS_Z][c] = x.z;
@@ -1806,8 +1824,8 @@ static MSet mirror_set(hc_buf* mirror, uint32_t max_cells, int k) {
     const MirrorLayout L = mirror_layout(max_cells);
     uint8_t* P = (uint8_t*)mirror->dptr;
     MSet m;
-    m.pos = (double4*)(P + L.pos[k]);
-    m.prev = (double4*)(P + L.prev[k]);
+    m.pos = (Dbl3*)(P + L.pos[k]);
+    m.prev = (Dbl3*)(P + L.prev[k]);
     m.prop = (double4*)(P + L.prop[k]);
     m.idx = (uint32_t*)(P + L.idx[k]);
     m.key = (uint32_t*)(P + L.key[k]);
