@@ -423,8 +423,8 @@ __global__ void k_b_motility_advance(const EnsSim* __restrict__ sims, const MotP
 // The cell fields (position, velocity, previous velocity, pressure,
 // mechanics voxel) are written back by index only when something reads them.
 struct BMirror {
-    double4* pos;     // x, y, z: the FP64 AB2 accumulator
-    double4* prev;    // previous velocity
+    Dbl3* pos;        // x, y, z: the FP64 AB2 accumulator
+    Dbl3* prev;       // previous velocity
     float4* prop;     // radius, repulsion, adhesion, relative max adhesion distance (FP32)
     uint32_t* idx;    // global agent index (member cell_off + cell)
     uint32_t* key;    // global bin of the coming step
@@ -461,9 +461,9 @@ __global__ __launch_bounds__(256) void k_bm_load(const EnsSim* __restrict__ sims
     if (valid) {
         const double px = sim_field(m, HC_F_POS_X)[c], py = sim_field(m, HC_F_POS_Y)[c],
                      pz = sim_field(m, HC_F_POS_Z)[c];
-        mr.pos[p] = make_double4(px, py, pz, 0.0);
-        mr.prev[p] = make_double4(sim_field(m, HC_F_PREV_VX)[c], sim_field(m, HC_F_PREV_VY)[c],
-                                  sim_field(m, HC_F_PREV_VZ)[c], 0.0);
+        mr.pos[p] = dbl3(px, py, pz);
+        mr.prev[p] = dbl3(sim_field(m, HC_F_PREV_VX)[c], sim_field(m, HC_F_PREV_VY)[c],
+                          sim_field(m, HC_F_PREV_VZ)[c]);
         mr.prop[p] = make_float4((float)sim_field(m, HC_F_RADIUS)[c], (float)sim_field(m, HC_F_REPULSION)[c],
                                  (float)sim_field(m, HC_F_ADHESION)[c], (float)sim_field(m, HC_F_REL_MAX_ADH)[c]);
         mr.idx[p] = p;
@@ -595,7 +595,7 @@ __global__ __launch_bounds__(256) void k_bm_gather(const EnsSim* __restrict__ si
         mr.rpos[p] = pos;
     }
     mr.kstep[p] = b;
-    const double4 x = mr.pos[p];
+    const Dbl3 x = mr.pos[p];
     const uint32_t lb = b - m.bin_off;
     const uint32_t bi = lb % m.mnx, bj = (lb / m.mnx) % m.mny, bk = lb / (m.mnx * m.mny);
     CellRec32 r;
@@ -630,8 +630,8 @@ __global__ __launch_bounds__(256) void k_bm_integrate(const EnsSim* __restrict__
         const double dt = m.dt_mech;
         const float4 f = mr.v[mr.rpos[p]];
         const double vx = (double)f.x, vy = (double)f.y, vz = (double)f.z;
-        const double4 pv = mr.prev[p];
-        const double4 x = mr.pos[p];
+        const Dbl3 pv = mr.prev[p];
+        const Dbl3 x = mr.pos[p];
         double px = x.x, py = x.y, pz = x.z;
         px += dt * (1.5 * vx - 0.5 * pv.x);
         py += dt * (1.5 * vy - 0.5 * pv.y);
@@ -639,8 +639,8 @@ __global__ __launch_bounds__(256) void k_bm_integrate(const EnsSim* __restrict__
         px = px < m.x_min ? m.x_min : (px > x_max ? x_max : px);
         py = py < m.y_min ? m.y_min : (py > y_max ? y_max : py);
         pz = pz < m.z_min ? m.z_min : (pz > z_max ? z_max : pz);
-        mr.pos[p] = make_double4(px, py, pz, 0.0);
-        mr.prev[p] = make_double4(vx, vy, vz, 0.0);
+        mr.pos[p] = dbl3(px, py, pz);
+        mr.prev[p] = dbl3(vx, vy, vz);
         key = bm_key(m, px, py, pz, xs);
         mr.key[p] = key;
         mr.xslab[p] = xs;
@@ -659,7 +659,7 @@ __global__ __launch_bounds__(256) void k_bm_sync(const EnsSim* __restrict__ sims
     if (c0 >= m.n) return;
     const uint32_t p = m.cell_off + c0;
     const uint32_t c = mr.idx[p] - m.cell_off;
-    const double4 x = mr.pos[p], pv = mr.prev[p];
+    const Dbl3 x = mr.pos[p], pv = mr.prev[p];
     const float4 f = mr.v[mr.rpos[p]];
     sim_field(m, HC_F_POS_X)[c] = x.x;
     sim_field(m, HC_F_POS_Y)[c] = x.y;
@@ -1391,7 +1391,8 @@ private:
         }
         if (mirror_on()) {
             for (int k = 0; k < 2; k++) {
-                alloc(w.mpos[k], N * 32); alloc(w.mprev[k], N * 32); alloc(w.mprop[k], N * 16);
+                alloc(w.mpos[k], N * sizeof(hc::Dbl3)); alloc(w.mprev[k], N * sizeof(hc::Dbl3));
+                alloc(w.mprop[k], N * 16);
                 alloc(w.midx[k], N * 4); alloc(w.mkey[k], N * 4); alloc(w.mslot[k], N * 4);
                 alloc(w.mxslab[k], N); alloc(w.mrpos[k], N * 4); alloc(w.mkstep[k], N * 4);
             }
@@ -1550,8 +1551,8 @@ private:
     hc::BMirror mirror_set(int k) {
         Workspace& w = ws_;
         hc::BMirror m;
-        m.pos = (double4*)hc_device_ptr(w.mpos[k].get());
-        m.prev = (double4*)hc_device_ptr(w.mprev[k].get());
+        m.pos = (hc::Dbl3*)hc_device_ptr(w.mpos[k].get());
+        m.prev = (hc::Dbl3*)hc_device_ptr(w.mprev[k].get());
         m.prop = (float4*)hc_device_ptr(w.mprop[k].get());
         m.idx = (uint32_t*)hc_device_ptr(w.midx[k].get());
         m.key = (uint32_t*)hc_device_ptr(w.mkey[k].get());

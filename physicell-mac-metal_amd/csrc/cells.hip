@@ -1380,20 +1380,6 @@ int hc_substrate_totals_noalloc(hc_ctx* ctx, hc_buf* density, const hc_grid* g, 
 // ---------------------------------------------------------------------------
 namespace hc {
 
-// three packed doubles (24 bytes): the slots' positions and previous
-// velocities, so AB2 and the gather move no padding
-struct Dbl3 {
-    double x, y, z;
-};
-static_assert(sizeof(Dbl3) == 24, "packed triple");
-__device__ inline Dbl3 dbl3(double x, double y, double z) {
-    Dbl3 r;
-    r.x = x;
-    r.y = y;
-    r.z = z;
-    return r;
-}
-
 MirrorLayout mirror_layout(uint32_t max_cells) {
     MirrorLayout L;
     size_t o = 0;

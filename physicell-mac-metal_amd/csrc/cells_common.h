@@ -37,6 +37,20 @@ struct __attribute__((aligned(16))) CellRec32 {
 };
 static_assert(sizeof(CellRec32) == 32, "mixed record must be 32 bytes");
 
+// three packed doubles (24 bytes): the mechanics mirrors' positions and
+// previous velocities, so AB2 and the gathers move no padding
+struct Dbl3 {
+    double x, y, z;
+};
+static_assert(sizeof(Dbl3) == 24, "packed triple");
+__device__ inline Dbl3 dbl3(double x, double y, double z) {
+    Dbl3 r;
+    r.x = x;
+    r.y = y;
+    r.z = z;
+    return r;
+}
+
 // FP32 records of a bin are ordered by x slab, then by agent index: XSLABS
 // slabs of voxel / XSLABS along x, by the record's bin-relative x.
 // xoff[XSLABS * b + s] is the position of the first record of slab s of bin
