@@ -807,7 +807,10 @@ __global__ __launch_bounds__(256) void k_stable_gather32(CellPtrs cp, const uint
 // agent's mirror slot, mkey the slots' bins, rel_adh the slots' properties
 // (double4, relative adhesion distance in w) and the forces go to vout in
 // record order.
-template <bool LIST>
+// M16 (the mirror): the mark pass reads the 8-byte entries in mark[],
+// unless the gather set *xtflag (a member that does not fit its entry):
+// then every agent takes the chunked pass.
+template <bool LIST, bool M16 = false>
 __global__ __launch_bounds__(256) void k_forces32(const CellRec32* __restrict__ rec, const uint32_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ xoff,
                                                   const uint32_t* __restrict__ mkey,
@@ -815,7 +818,9 @@ __global__ __launch_bounds__(256) void k_forces32(const CellRec32* __restrict__ 
                                                   double* __restrict__ vy, double* __restrict__ vz,
                                                   double* __restrict__ pressure, uint32_t n, uint32_t mnx, uint32_t mny,
                                                   uint32_t mnz, float max_dist, double voxel,
-                                                  const float* __restrict__ rmax, double4* __restrict__ vout) {
+                                                  const float* __restrict__ rmax, double4* __restrict__ vout,
+                                                  const uint2* __restrict__ mark = nullptr,
+                                                  const uint32_t* __restrict__ xtflag = nullptr) {
     __shared__ uint32_t marks[LIST ? 4 * MARKX_CAP * 64 : 1];
     const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (t >= n) return;
@@ -825,11 +830,11 @@ __global__ __launch_bounds__(256) void k_forces32(const CellRec32* __restrict__ 
     float fx = 0.f, fy = 0.f, fz = 0.f, local_pressure = 0.f;
     const uint32_t vi = key % mnx, vj = (key / mnx) % mny, vk = key / (mnx * mny);
     const float cut2 = max_dist * max_dist, vox = (float)voxel;
-    if (LIST) {
+    if (LIST && (!M16 || xtflag[0] == 0u)) {
         uint32_t* list = marks + (threadIdx.x >> 6) * (MARKX_CAP * 64) + (threadIdx.x & 63);
-        force_agent32x(rec, xoff, list, t, 0u, vi, vj, vk, mnx, mny, mnz, me.xg, me.x, me.y, me.z, vox,
-                       xslab_scale(voxel), (float)(1.0 / voxel), rmax[0], me.r, me.rep, me.adh, rel_max_adh, cut2,
-                       fx, fy, fz, local_pressure);
+        force_agent32x<M16>(rec, xoff, list, t, 0u, vi, vj, vk, mnx, mny, mnz, me.xg, me.x, me.y, me.z, vox,
+                            xslab_scale(voxel), (float)(1.0 / voxel), rmax[0], me.r, me.rep, me.adh, rel_max_adh,
+                            cut2, fx, fy, fz, local_pressure, mark);
     } else {
         force_rows32(rec, offsets, t, 0u, vi, vj, vk, mnx, mny, mnz, me.x, me.y, me.z, vox, me.r, me.rep, me.adh,
                      rel_max_adh, cut2, fx, fy, fz, local_pressure);
@@ -927,7 +932,7 @@ int totals_scratch_init(hc_ctx* ctx) {
 int probe_kernels_cells2(hc_ctx* ctx);
 int probe_kernels_cells2(hc_ctx* ctx) {
     hipFuncAttributes attr;
-    const void* fns[] = {(const void*)k_stable_sort, (const void*)k_stable_gather32, (const void*)k_forces32<true>, (const void*)k_forces32<false>,
+    const void* fns[] = {(const void*)k_stable_sort, (const void*)k_stable_gather32, (const void*)k_forces32<true>, (const void*)k_forces32<false>, (const void*)k_forces32<true, true>,
                          (const void*)k_bin_sort, (const void*)k_bin_sort32,
                          (const void*)k_totals<double>, (const void*)k_totals<float>, (const void*)k_totals_final};
     for (const void* f : fns) {
@@ -1206,7 +1211,7 @@ int hc_forces(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech, uin
                            cp.f[HC_F_VEL_X], cp.f[HC_F_VEL_Y], cp.f[HC_F_VEL_Z], cp.f[HC_F_PRESSURE], n,
                            mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz,
                            (float)mech->max_interaction_distance, mech->mechanics_voxel_size, h.rmax,
-                           (double4*)nullptr);
+                           (double4*)nullptr, (const uint2*)nullptr, (const uint32_t*)nullptr);
     else
         hipLaunchKernelGGL(k_forces, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, h.rec,
                            mark32_ok(mech) ? h.pos32 : nullptr, h.offsets, cp.f[HC_F_VEL_X], cp.f[HC_F_VEL_Y],
@@ -1580,7 +1585,7 @@ __global__ __launch_bounds__(256) void k_m_bin_sort32(const uint32_t* __restrict
     if (b == nbins) {
         const uint32_t n = offsets[nbins];
         *reinterpret_cast<uint4*>(xoff + (size_t)XSLABS * nbins) = make_uint4(n, n, n, n);
-        if (REC64) xtflag[0] = 0u;                       // the gather that follows sets it
+        if (xtflag) xtflag[0] = 0u;                      // the gather that follows sets it
         return;
     }
     const uint32_t beg = offsets[b], cnt = offsets[b + 1] - beg;
@@ -1656,7 +1661,8 @@ __global__ __launch_bounds__(256) void k_m_gather32(MSet a, const uint32_t* __re
                                                     const uint32_t* __restrict__ tmp, uint32_t* __restrict__ sorted,
                                                     CellRec32* __restrict__ rec, uint32_t n, double x_min,
                                                     double y_min, double z_min, double voxel, uint32_t mnx,
-                                                    uint32_t mny, float* __restrict__ rpart) {
+                                                    uint32_t mny, float* __restrict__ rpart, uint2* __restrict__ mark,
+                                                    uint32_t* __restrict__ xtflag) {
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
     const double4 pr = p < n ? a.prop[p] : make_double4(0.0, 0.0, 0.0, 0.0);
     if (rpart) block_max_to((float)pr.x, rpart);
@@ -1692,6 +1698,11 @@ __global__ __launch_bounds__(256) void k_m_gather32(MSet a, const uint32_t* __re
     r.adh = (float)pr.z;
     r.idx = p;
     rec[pos] = r;
+    if (mark) {
+        bool fits;
+        mark[pos] = mark_entry16(x.x - x_min, x.y - (y_min + bj * voxel), x.z - (z_min + bk * voxel), 0, fits);
+        if (!fits) xtflag[0] = 1u;
+    }
 }
 
 // AB2 (shaders/integrate.metal:56-79) on the slots, with this step's forces
@@ -1857,6 +1868,7 @@ int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, 
     const int xg = markx_ok(mech);
     const int fp32 = mech->forces_fp32 ? 1 : 0;
     const int xt = xtrim64_ok(mech, n_mvox) ? 1 : 0;
+    const bool list32 = fp32 && list32_ok(mech, n, n_mvox);
     if (fp32) {
         // the mixed form: FP32 records in (bin, x slab, index) order
         {
@@ -1864,18 +1876,20 @@ int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, 
             if ((r = scan_exclusive(ctx, h.counts, h.offsets, n_mvox, h.partial, h.counts))) return r;
             hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, ctx->stream, a.key, a.slot, h.offsets, h.tmp, n);
             hipLaunchKernelGGL(k_m_bin_sort32<false>, dim3(blocks_for(n_mvox + 1, 256)), dim3(256), 0, ctx->stream,
-                               h.offsets, h.tmp, a, h.sorted, h.xoff, n_mvox, (uint32_t*)nullptr);
+                               h.offsets, h.tmp, a, h.sorted, h.xoff, n_mvox, list32 ? h.xtflag : nullptr);
             hipLaunchKernelGGL(k_m_gather32, dim3(nb), dim3(256), 0, ctx->stream, a, h.offsets, h.tmp, h.sorted,
                                reinterpret_cast<CellRec32*>(h.rec), n, mgrid->x_min, mgrid->y_min, mgrid->z_min,
-                               mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny, (float*)nullptr);
+                               mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny, (float*)nullptr,
+                               list32 ? reinterpret_cast<uint2*>(h.pos32) : nullptr, h.xtflag);
             if ((r = check_launch(ctx, "mirror hash (mixed)"))) return r;
         }
         KernelTimer tm(ctx, HC_K_FORCES);
-        hipLaunchKernelGGL(list32_ok(mech, n, n_mvox) ? k_forces32<true> : k_forces32<false>, dim3(nb), dim3(256), 0,
+        hipLaunchKernelGGL((list32 ? k_forces32<true, true> : k_forces32<false, false>), dim3(nb), dim3(256), 0,
                            ctx->stream, reinterpret_cast<const CellRec32*>(h.rec), h.offsets, h.xoff, a.key,
                            (const double*)a.prop, (double*)nullptr, (double*)nullptr, (double*)nullptr,
                            (double*)nullptr, n, mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz,
-                           (float)mech->max_interaction_distance, mech->mechanics_voxel_size, h.rmax, vp);
+                           (float)mech->max_interaction_distance, mech->mechanics_voxel_size, h.rmax, vp,
+                           reinterpret_cast<const uint2*>(h.pos32), h.xtflag);
         if ((r = check_launch(ctx, "k_forces32 (mirror)"))) return r;
     } else {
         // the slots' bins and places in them come from the last AB2 (or load)

@@ -660,6 +660,22 @@ __device__ __attribute__((always_inline)) inline void force_list32(
     }
 }
 
+// 8-byte mark entries (force_agent64t, and force_agent32x<true>): x - x_min
+// in 1/16 um, y and z relative to the bin's lower corner plus 64 um in
+// 1/512 um, and a signed 16-bit displacement from the entry's position to
+// the member's record position (0 where the records are in the entries'
+// order).  ok is false when a member does not fit.
+constexpr float MARK16_SLACK = 2.25f;
+constexpr uint32_t MARK16_STEP = 8;   // entries per step (their loads in flight together)
+constexpr uint32_t LAW16_BATCH = 2;   // records per law batch (fewer registers: 5 waves per SIMD)
+__host__ __device__ inline uint2 mark_entry16(double xg, double yb, double zb, int disp, bool& ok) {
+    const double qx = rint(xg * 16.0), qy = rint((yb + 64.0) * 512.0), qz = rint((zb + 64.0) * 512.0);
+    ok = qx >= 0.0 && qx <= 65535.0 && qy >= 0.0 && qy <= 65535.0 && qz >= 0.0 && qz <= 65535.0 &&
+         disp >= -32768 && disp <= 32767;
+    if (!ok) return make_uint2(0u, 0u);
+    return make_uint2((uint32_t)qx | ((uint32_t)qy << 16), (uint32_t)qz | ((uint32_t)disp << 16));
+}
+
 // FP32 forces of one agent with a mark pass over x-trimmed rows and a
 // per-lane list of the marked candidates in LDS (the mixed form; the chunked
 // pass's result bit for bit).
@@ -688,14 +704,21 @@ __device__ __attribute__((always_inline)) inline void force_list32(
 // sums are the chunked pass's.  Record positions stay below 2^27 (callers
 // check).
 
+//
+// M16: the mark reads the 8-byte entries in mark[] (the records' order)
+// instead of the records' first 16 bytes, with MARK16_SLACK (the entries'
+// quantisation bound of force_agent64t plus the FP32 law's own rounding,
+// below 0.01 um^2 for pairs within 32 um).
+template <bool M16 = false>
 __device__ __attribute__((always_inline)) inline void force_agent32x(
     const CellRec32* __restrict__ rec, const uint32_t* __restrict__ xoff, uint32_t* __restrict__ list, uint32_t t,
     uint32_t bin0, uint32_t vi, uint32_t vj, uint32_t vk, uint32_t mnx, uint32_t mny, uint32_t mnz, float mxg,
     float mx, float my, float mz, float voxel, float scale, float inv_voxel, float rmax, float radius_i, float rep_i,
-    float adh_i, float rel_max_adh, float cut2, float& fx, float& fy, float& fz, float& local_pressure) {
+    float adh_i, float rel_max_adh, float cut2, float& fx, float& fy, float& fz, float& local_pressure,
+    const uint2* __restrict__ mark = nullptr) {
     const float relf = rel_max_adh > 1.0f ? rel_max_adh : 1.0f;
     const float L = (radius_i + rmax) * relf;
-    const float lim = fminf(L * L * (1.0f + 0x1p-16f), cut2) + MARKX_SLACK;
+    const float lim = fminf(L * L * (1.0f + 0x1p-16f), cut2) + (M16 ? MARK16_SLACK : MARKX_SLACK);
     const float Lx = sqrtf(lim) + XTRIM_MARGIN;
     // the walked x interval, as slab indices of the row (column vi - 1 .. vi + 1)
     const float lo = mx - Lx, hi = mx + Lx;
@@ -717,19 +740,36 @@ __device__ __attribute__((always_inline)) inline void force_agent32x(
         const uint32_t beg = ok ? xoff[rb + jlo] : 0u, end = ok ? xoff[rb + jhi] : 0u;
         const float myr = my - (float)dj * voxel, mzr = mz - (float)dk * voxel;
         const uint32_t qc = (uint32_t)((dk + 1) * 3 + (dj + 1)) << MARK32_POS_BITS;
-        const CellRec32* p = rec + beg;
-        for (uint32_t u0 = beg; u0 < end; u0 += MARK32_STEP, p += MARK32_STEP) {
+        const float myr64 = myr + 64.0f, mzr64 = mzr + 64.0f;
+        for (uint32_t u0 = beg; u0 < end; u0 += MARK32_STEP) {
             // entries u0 + j with the row code above them (u0 + j < 2^27)
             const uint32_t eb = u0 + qc, rem = end - u0;
-            float4 o[MARK32_STEP];
+            float d2[MARK32_STEP];
+            if (M16) {
+                uint2 o[MARK32_STEP];
 #pragma unroll
-            for (uint32_t j = 0; j < MARK32_STEP; j++) o[j] = *reinterpret_cast<const float4*>(p + j);
+                for (uint32_t j = 0; j < MARK32_STEP; j++) o[j] = mark[u0 + j];
+#pragma unroll
+                for (uint32_t j = 0; j < MARK32_STEP; j++) {
+                    const float ddx = __builtin_fmaf((float)(o[j].x & 0xffffu), -0.0625f, mxg);
+                    const float ddy = __builtin_fmaf((float)(o[j].x >> 16), -0x1p-9f, myr64);
+                    const float ddz = __builtin_fmaf((float)(o[j].y & 0xffffu), -0x1p-9f, mzr64);
+                    d2[j] = __builtin_fmaf(ddz, ddz, __builtin_fmaf(ddy, ddy, ddx * ddx));
+                }
+            } else {
+                float4 o[MARK32_STEP];
+#pragma unroll
+                for (uint32_t j = 0; j < MARK32_STEP; j++) o[j] = *reinterpret_cast<const float4*>(rec + u0 + j);
+#pragma unroll
+                for (uint32_t j = 0; j < MARK32_STEP; j++) {
+                    const float ddx = mxg - o[j].x, ddy = myr - o[j].y, ddz = mzr - o[j].z;
+                    d2[j] = __builtin_fmaf(ddz, ddz, __builtin_fmaf(ddy, ddy, ddx * ddx));
+                }
+            }
 #pragma unroll
             for (uint32_t j = 0; j < MARK32_STEP; j++) {
-                const float ddx = mxg - o[j].x, ddy = myr - o[j].y, ddz = mzr - o[j].z;
-                const float d2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(ddy, ddy, ddx * ddx));
                 *w = eb + j;
-                w += (j < rem && d2 < lim) ? 64 : 0;
+                w += (j < rem && d2[j] < lim) ? 64 : 0;
             }
             if (w > wflush) {
                 force_list32(rec, list, (uint32_t)(w - l0) / 64u, t, (int)vi, mx, my, mz, voxel, inv_voxel,
@@ -765,16 +805,6 @@ __device__ __attribute__((always_inline)) inline void force_agent32x(
 // reference's visit order (rows, then ascending record position) without a
 // sort, re-tests each with the exact FP64 predicates, and the sums are
 // force_agent64x's bit for bit.
-constexpr float MARK16_SLACK = 2.25f;
-constexpr uint32_t MARK16_STEP = 8;   // entries per step (their loads in flight together)
-constexpr uint32_t LAW16_BATCH = 2;   // records per law batch (fewer registers: 5 waves per SIMD)
-__host__ __device__ inline uint2 mark_entry16(double xg, double yb, double zb, int disp, bool& ok) {
-    const double qx = rint(xg * 16.0), qy = rint((yb + 64.0) * 512.0), qz = rint((zb + 64.0) * 512.0);
-    ok = qx >= 0.0 && qx <= 65535.0 && qy >= 0.0 && qy <= 65535.0 && qz >= 0.0 && qz <= 65535.0 &&
-         disp >= -32768 && disp <= 32767;
-    if (!ok) return make_uint2(0u, 0u);
-    return make_uint2((uint32_t)qx | ((uint32_t)qy << 16), (uint32_t)qz | ((uint32_t)disp << 16));
-}
 
 // mxg: the agent's x - x_min; my64, mz64: its y, z relative to its bin's
 // lower corner plus 64 um (FP32 of the FP64 values); mxb: its bin-relative x
