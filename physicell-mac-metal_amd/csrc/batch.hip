@@ -170,7 +170,9 @@ __global__ __launch_bounds__(256) void k_b_gather(const EnsSim* __restrict__ sim
 // list; the others the chunked pass.
 // MIRROR (FP32 only): the records' index field is the agent's mirror slot,
 // kglob its bin (the mirror's key array), the relative adhesion distance
-// comes from the mirror and the forces go to vrec in record order.
+// comes from the mirror, the mark pass reads the 8-byte entries in mark[]
+// (every agent takes the chunked pass when k_bm_gather set *xtflag) and the
+// forces go to vrec in record order.
 template <bool FP32, bool LIST = true, bool MIRROR = false>
 __global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sims, const AgentTile* __restrict__ tiles,
                                                   const void* __restrict__ recs, const float4* __restrict__ pos32,
@@ -178,7 +180,9 @@ __global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sim
                                                   const uint32_t* __restrict__ xoff,
                                                   const uint32_t* __restrict__ kglob,
                                                   const float* __restrict__ rmax,
-                                                  const float4* __restrict__ mprop, float4* __restrict__ vrec) {
+                                                  const float4* __restrict__ mprop, float4* __restrict__ vrec,
+                                                  const uint2* __restrict__ mark,
+                                                  const uint32_t* __restrict__ xtflag) {
     __shared__ uint32_t marks[LIST ? 4 * MARKX_CAP * 64 : 1];
     uint32_t* list = marks + (LIST ? (threadIdx.x >> 6) * (MARKX_CAP * 64) + (threadIdx.x & 63) : 0);
     const AgentTile tl = tiles[xcd_block(blockIdx.x, gridDim.x)];
@@ -205,10 +209,10 @@ __global__ __launch_bounds__(256) void k_b_forces(const EnsSim* __restrict__ sim
         const float rel = MIRROR ? mprop[idx].w : (float)sim_field(m, HC_F_REL_MAX_ADH)[c];
         const float cut = (float)m.max_dist, vox = (float)m.mvoxel;
         float fx = 0.f, fy = 0.f, fz = 0.f, pr = 0.f;
-        if (LIST && member_markx(m) && m.mvoxel >= 0.5)
-            force_agent32x(rec, xoff, list, t, m.bin_off, vi, vj, vk, m.mnx, m.mny, m.mnz, me.xg, me.x, me.y, me.z,
-                           vox, xslab_scale(m.mvoxel), (float)(1.0 / m.mvoxel), rmax[0], me.r, me.rep, me.adh, rel,
-                           cut * cut, fx, fy, fz, pr);
+        if (LIST && member_markx(m) && m.mvoxel >= 0.5 && (!MIRROR || xtflag[0] == 0u))
+            force_agent32x<MIRROR>(rec, xoff, list, t, m.bin_off, vi, vj, vk, m.mnx, m.mny, m.mnz, me.xg, me.x, me.y,
+                                   me.z, vox, xslab_scale(m.mvoxel), (float)(1.0 / m.mvoxel), rmax[0], me.r, me.rep,
+                                   me.adh, rel, cut * cut, fx, fy, fz, pr, mark);
         else
             force_rows32(rec, offsets, t, m.bin_off, vi, vj, vk, m.mnx, m.mny, m.mnz, me.x, me.y, me.z, vox, me.r,
                          me.rep, me.adh, rel, cut * cut, fx, fy, fz, pr);
@@ -497,12 +501,13 @@ __global__ __launch_bounds__(256) void k_bm_scatter(const EnsSim* __restrict__ s
 __global__ __launch_bounds__(256) void k_bm_bin_sort(const uint32_t* __restrict__ offsets,
                                                      const uint64_t* __restrict__ tmp64, BMirror mr,
                                                      uint32_t* __restrict__ sorted, uint32_t* __restrict__ xoff,
-                                                     uint32_t nbins) {
+                                                     uint32_t nbins, uint32_t* __restrict__ xtflag) {
     const uint32_t b = blockIdx.x * 256 + threadIdx.x;
     if (b > nbins) return;
     if (b == nbins) {
         const uint32_t n = offsets[nbins];
         *reinterpret_cast<uint4*>(xoff + (size_t)XSLABS * nbins) = make_uint4(n, n, n, n);
+        xtflag[0] = 0u;                                  // k_bm_gather sets it
         return;
     }
     const uint32_t beg = offsets[b], cnt = offsets[b + 1] - beg;
@@ -568,7 +573,8 @@ __global__ __launch_bounds__(256) void k_bm_gather(const EnsSim* __restrict__ si
                                                    const AgentTile* __restrict__ tiles, BMirror mr,
                                                    const uint32_t* __restrict__ offsets,
                                                    const uint64_t* __restrict__ tmp64, uint32_t* __restrict__ sorted,
-                                                   CellRec32* __restrict__ recs, float* __restrict__ rpart) {
+                                                   CellRec32* __restrict__ recs, float* __restrict__ rpart,
+                                                   uint2* __restrict__ mark, uint32_t* __restrict__ xtflag) {
     const AgentTile tl = tiles[blockIdx.x];
     const EnsSim m = sims[tl.sim];
     const uint32_t c = tl.c0 + threadIdx.x;
@@ -608,6 +614,9 @@ __global__ __launch_bounds__(256) void k_bm_gather(const EnsSim* __restrict__ si
     r.adh = pr.z;
     r.idx = p;
     recs[pos] = r;
+    bool fits;
+    mark[pos] = mark_entry16(x.x - m.x_min, x.y - (m.y_min + bj * m.mvoxel), x.z - (m.z_min + bk * m.mvoxel), 0, fits);
+    if (!fits) xtflag[0] = 1u;
 }
 
 // AB2 (shaders/integrate.metal:56-79, k_b_integrate's expressions) on the
@@ -1241,6 +1250,7 @@ private:
         DeviceBuffer pcounts, poffsets, pkv, pslot, ptmp, psorted, pflag, pgsim, pgvox, pgbegin, pcoef, phdr, ppartial;
         // agent mirror (mixed form without motility): two sets, forces by record
         DeviceBuffer mpos[2], mprev[2], mprop[2], midx[2], mkey[2], mslot[2], mxslab[2], mrpos[2], mkstep[2], mv, mtmp64;
+        DeviceBuffer mmark, mflag;          // 8-byte mark entries of the records (padded by MARK32_STEP) and their flag
         uint32_t ntot = 0, nbins = 0, vtot = 0, ntiles = 0, nwork[NLIST] = {}, ns_max = 0, nmax_all = 0;
         uint32_t max_line[NLIST] = {};
         size_t bytes = 0;
@@ -1398,6 +1408,8 @@ private:
             }
             alloc(w.mv, N * 16);
             alloc(w.mtmp64, N * 8);
+            alloc(w.mmark, (N + hc::MARK32_STEP) * 8);
+            alloc(w.mflag, 256);
         }
         mirror_valid_ = false;
         soa_stale_ = false;
@@ -1582,15 +1594,18 @@ private:
         uint32_t* sorted = (uint32_t*)hc_device_ptr(w.sorted.get());
         uint32_t* xoff = (uint32_t*)hc_device_ptr(w.xoff.get());
         float* rpart = (float*)hc_device_ptr(w.rpart.get());
-        if (!tmp64) throw std::runtime_error("ensemble: agent mirror not allocated");
+        if (!tmp64 || !hc_device_ptr(w.mmark.get()) || !hc_device_ptr(w.mflag.get()))
+            throw std::runtime_error("ensemble: agent mirror not allocated");
         check(ctx_, hc::scan_exclusive(ctx_, counts, offsets, w.nbins, (uint32_t*)hc_device_ptr(w.partial.get()),
                                        at_load ? nullptr : counts),
               "scan");
         hipLaunchKernelGGL(hc::k_bm_scatter, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, a, offsets, tmp64);
+        uint32_t* mflag = (uint32_t*)hc_device_ptr(w.mflag.get());
         hipLaunchKernelGGL(hc::k_bm_bin_sort, dim3(hc::blocks_for(w.nbins + 1, 256)), dim3(256), 0, st, offsets,
-                           tmp64, a, sorted, xoff, w.nbins);
+                           tmp64, a, sorted, xoff, w.nbins, mflag);
         hipLaunchKernelGGL(hc::k_bm_gather, dim3(w.ntiles), dim3(256), 0, st, sims, tiles, a, offsets, tmp64, sorted,
-                           (hc::CellRec32*)hc_device_ptr(w.recs.get()), at_load ? rpart : nullptr);
+                           (hc::CellRec32*)hc_device_ptr(w.recs.get()), at_load ? rpart : nullptr,
+                           (uint2*)hc_device_ptr(w.mmark.get()), mflag);
         check(ctx_, hc::check_launch(ctx_, "mirror hash"), "mirror hash");
         if (at_load)
             check(ctx_, hc::launch_rmax(ctx_, rpart, w.ntiles, (float*)hc_device_ptr(w.rmax.get())), "largest radius");
@@ -1656,7 +1671,8 @@ private:
             hipLaunchKernelGGL((hc::k_b_forces<true, true, true>), dim3(w.ntiles), dim3(256), 0, st, sims, tiles,
                                hc_device_ptr(w.recs.get()), nullptr, (const uint32_t*)hc_device_ptr(w.offsets.get()),
                                (const uint32_t*)hc_device_ptr(w.xoff.get()), a.key,
-                               (const float*)hc_device_ptr(w.rmax.get()), a.prop, a.v);
+                               (const float*)hc_device_ptr(w.rmax.get()), a.prop, a.v,
+                               (const uint2*)hc_device_ptr(w.mmark.get()), (const uint32_t*)hc_device_ptr(w.mflag.get()));
             check(ctx_, hc::check_launch(ctx_, "k_b_forces (mirror)"), "forces");
         }
         {
@@ -1722,10 +1738,11 @@ private:
             const bool list = w.ntot < (1u << hc::MARK32_POS_BITS) && w.nbins < (1u << 30);
             if (mixed_)
                 hipLaunchKernelGGL((list ? hc::k_b_forces<true, true> : hc::k_b_forces<true, false>), g, dim3(256), 0,
-                                   st, sims, tiles, recs, nullptr, offsets, xoff, kglob, rmax, nullptr, nullptr);
+                                   st, sims, tiles, recs, nullptr, offsets, xoff, kglob, rmax, nullptr, nullptr,
+                                   nullptr, nullptr);
             else
                 hipLaunchKernelGGL(hc::k_b_forces<false>, g, dim3(256), 0, st, sims, tiles, recs, pos32, offsets,
-                                   nullptr, nullptr, rmax, nullptr, nullptr);
+                                   nullptr, nullptr, rmax, nullptr, nullptr, nullptr, nullptr);
             check(ctx_, hc::check_launch(ctx_, "k_b_forces"), "forces");
         }
         if (any_motility_) {
