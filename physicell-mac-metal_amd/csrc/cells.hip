@@ -282,6 +282,18 @@ __global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ ke
     tmp[offsets[key[c]] + slot[c]] = c;
 }
 
+// k_scatter with each agent's x slab in the entry's low bits (agent << 2 |
+// slab; agents below 2^30), for the plain form's 8-byte mark entries
+__global__ __launch_bounds__(256) void k_scatter_slab(const uint32_t* __restrict__ key,
+                                                      const uint32_t* __restrict__ slot,
+                                                      const uint8_t* __restrict__ xslab,
+                                                      const uint32_t* __restrict__ offsets, uint32_t* __restrict__ tmp,
+                                                      uint32_t n) {
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    tmp[offsets[key[c]] + slot[c]] = (c << 2) | xslab[c];
+}
+
 
 // the largest of nb block maxima (one block)
 __global__ __launch_bounds__(256) void k_rmax(const float* __restrict__ rpart, uint32_t nb, float* __restrict__ rmax) {
@@ -307,9 +319,10 @@ __global__ __launch_bounds__(256) void k_bin_sort32(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ tmp,
                                                     const uint8_t* __restrict__ xslab, uint32_t* __restrict__ sorted,
                                                     uint32_t* __restrict__ rpos, uint32_t* __restrict__ xoff,
-                                                    uint32_t nbins) {
+                                                    uint32_t nbins, uint32_t* __restrict__ rec_pos,
+                                                    uint32_t* __restrict__ xtflag) {
     const uint32_t b = blockIdx.x * 256 + threadIdx.x;
-    if (b <= nbins) bin_sort_slabs(offsets, tmp, xslab, sorted, rpos, xoff, b, nbins);
+    if (b <= nbins) bin_sort_slabs(offsets, tmp, xslab, sorted, rpos, xoff, b, nbins, rec_pos, xtflag);
 }
 
 __global__ __launch_bounds__(256) void k_stable_gather(CellPtrs cp, const uint32_t* __restrict__ key,
@@ -319,7 +332,9 @@ __global__ __launch_bounds__(256) void k_stable_gather(CellPtrs cp, const uint32
                                                        CellRec* __restrict__ rec, float4* __restrict__ pos32,
                                                        uint32_t n, double x_min, double y_min, double z_min,
                                                        double voxel, uint32_t mnx, uint32_t mny, int x_grid,
-                                                       float* __restrict__ rpart) {
+                                                       float* __restrict__ rpart, const uint32_t* __restrict__ spos,
+                                                       const uint8_t* __restrict__ xslab,
+                                                       uint32_t* __restrict__ xtflag) {
     const uint32_t c = blockIdx.x * 256 + threadIdx.x;
     // this block's largest radius, rounded up to FP32 (no lane returns before
     // the barrier)
@@ -329,12 +344,38 @@ __global__ __launch_bounds__(256) void k_stable_gather(CellPtrs cp, const uint32
     const uint32_t beg = offsets[b], end = offsets[b + 1];
     uint32_t pos;
     if (end - beg <= BIN_SORT_CAP) {
-        pos = pos_of[c];                                  // placed by k_bin_sort
+        pos = pos_of[c];                                  // placed by k_bin_sort (k_bin_sort32)
     } else {
-        pos = beg + stable_rank(tmp, beg, end, c);
+        // spos: tmp holds (agent << 2 | slab), ordered as the agents are
+        pos = beg + stable_rank(tmp, beg, end, spos ? c << 2 : c);
         sorted[pos] = c;
     }
-    if (pos32) {
+    if (spos) {
+        if (c == 0) {
+            double* org = reinterpret_cast<double*>(xtflag + 2);
+            org[0] = x_min;
+            org[1] = y_min;
+            org[2] = z_min;
+        }
+        // 8-byte mark entry at its (x slab, index) position (force_agent64t)
+        uint32_t sp;
+        if (end - beg <= BIN_SORT_CAP) {
+            sp = spos[c];
+        } else {
+            const uint32_t sc = xslab[c];
+            sp = beg;
+            for (uint32_t u = beg; u < end; u++) {
+                const uint32_t v = tmp[u], sv = v & 3u, iv = v >> 2;
+                sp += (sv < sc || (sv == sc && iv < c)) ? 1u : 0u;
+            }
+        }
+        const uint32_t bj = (b / mnx) % mny, bk = b / (mnx * mny);
+        bool fits;
+        reinterpret_cast<uint2*>(pos32)[sp] =
+            mark_entry16(cp.f[HC_F_POS_X][c] - x_min, cp.f[HC_F_POS_Y][c] - (y_min + bj * voxel),
+                         cp.f[HC_F_POS_Z][c] - (z_min + bk * voxel), (int)pos - (int)sp, fits);
+        if (!fits) xtflag[0] = 1u;
+    } else if (pos32) {
         // x relative to the grid's x_min (force_agent64x) or to the bin corner
         // (force_range64m); y and z relative to the bin corner
         const uint32_t bi = b % mnx, bj = (b / mnx) % mny, bk = b / (mnx * mny);
@@ -435,8 +476,10 @@ __global__ __launch_bounds__(256) void k_forces_xt(const CellRec* __restrict__ r
                                                    const uint32_t* __restrict__ xoff,
                                                    const uint32_t* __restrict__ xtflag, uint32_t n, uint32_t mnx,
                                                    uint32_t mny, uint32_t mnz, double max_dist, double voxel,
-                                                   double x_min, double y_min, double z_min,
-                                                   const float* __restrict__ rmax, double4* __restrict__ vout) {
+                                                   const double* __restrict__ origin,
+                                                   const float* __restrict__ rmax, double4* __restrict__ vout,
+                                                   double* __restrict__ vx, double* __restrict__ vy,
+                                                   double* __restrict__ vz, double* __restrict__ pressure) {
     __shared__ uint32_t marks[4 * MARKX_CAP * 64];
     const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (t >= n) return;
@@ -446,6 +489,7 @@ __global__ __launch_bounds__(256) void k_forces_xt(const CellRec* __restrict__ r
     double fx = 0.0, fy = 0.0, fz = 0.0, local_pressure = 0.0;
     const uint32_t vi = me.key % mnx, vj = (me.key / mnx) % mny, vk = me.key / (mnx * mny);
     const double cut2 = max_dist * max_dist;
+    const double x_min = origin[0], y_min = origin[1], z_min = origin[2];
     const double yb = py - (y_min + vj * voxel), zb = pz - (z_min + vk * voxel);
     const float mxg = (float)(px - x_min);
     const float vox = (float)voxel, near = 2.0f * vox;
@@ -463,7 +507,15 @@ __global__ __launch_bounds__(256) void k_forces_xt(const CellRec* __restrict__ r
                           local_pressure);
         }
     }
-    vout[t] = make_double4(fx, fy, fz, local_pressure);
+    if (vout) {
+        vout[t] = make_double4(fx, fy, fz, local_pressure);
+        return;
+    }
+    const uint32_t c = me.idx;
+    vx[c] = fx;
+    vy[c] = fy;
+    vz[c] = fz;
+    pressure[c] = local_pressure;
 }
 
 __global__ __launch_bounds__(256) void k_neighbors(const CellRec* __restrict__ rec, const uint32_t* __restrict__ offsets,
@@ -646,7 +698,7 @@ int probe_kernels_cells(hc_ctx* ctx) {
     const void* fns[] = {(const void*)k_scan_local, (const void*)k_scan_partials, (const void*)k_scan_add,
                          (const void*)k_scan2_reduce, (const void*)k_scan2_down,
                          (const void*)k_fill_u32, (const void*)k_voxel_keys, (const void*)k_mech_keys,
-                         (const void*)k_scatter, (const void*)k_stable_gather, (const void*)k_rmax, (const void*)k_forces,
+                         (const void*)k_scatter, (const void*)k_scatter_slab, (const void*)k_stable_gather, (const void*)k_rmax, (const void*)k_forces,
                          (const void*)k_neighbors, (const void*)k_max_bin, (const void*)k_integrate,
                          (const void*)k_sec_keys, (const void*)k_sec_flags, (const void*)k_sec_groups<double>,
                          (const void*)k_sec_groups<float>,
@@ -680,8 +732,9 @@ HashLayout hash_layout(uint32_t n_mvox, uint32_t max_cells) {
     // FP32 records: slab starts per bin (and past the last bin), slab per agent
     L.xoff = o;    o = align256(o + ((size_t)n_mvox + 1) * XSLABS * 4);
     L.xslab = o;   o = align256(o + (size_t)max_cells);
-    // set by the gather when a member does not fit its 8-byte mark entry
-    L.xtflag = o;  o = align256(o + 4);
+    // set by the gather when a member does not fit its 8-byte mark entry;
+    // the grid origin (x_min, y_min, z_min) follows at byte 8
+    L.xtflag = o;  o = align256(o + 32);
     L.total = o;
     return L;
 }
@@ -958,7 +1011,7 @@ int launch_bin_sort(hc_ctx* ctx, const uint32_t* offsets, const uint32_t* tmp, u
 int launch_bin_sort32(hc_ctx* ctx, const uint32_t* offsets, const uint32_t* tmp, const uint8_t* xslab,
                       uint32_t* sorted, uint32_t* rpos, uint32_t* xoff, uint32_t nbins) {
     hipLaunchKernelGGL(k_bin_sort32, dim3(blocks_for(nbins + 1, 256)), dim3(256), 0, ctx->stream, offsets, tmp, xslab,
-                       sorted, rpos, xoff, nbins);
+                       sorted, rpos, xoff, nbins, (uint32_t*)nullptr, (uint32_t*)nullptr);
     return check_launch(ctx, "k_bin_sort32");
 }
 
@@ -1159,6 +1212,7 @@ int hc_build_hash(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech,
         return fail(ctx, HC_ERR_ARG, "hc_build_hash: n_mvox != mech grid size");
     HashPtrs h = hash_ptrs(hash, n_mvox, max_cells);
     CellPtrs cp = cell_ptrs(cells->dptr, max_cells);
+    const bool xt = xtrim64_ok(mech, n_mvox) && max_cells <= (1u << 30);
     KernelTimer tm(ctx, HC_K_HASH);
     if ((r = fill_u32(ctx, h.counts, n_mvox, 0))) return r;
     const unsigned nb = blocks_for(n ? n : 1, 256);
@@ -1167,25 +1221,40 @@ int hc_build_hash(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech,
                            cp.f[HC_F_POS_Z], cp.u[HC_F_MECH_VOXEL - HC_NUM_F64_FIELDS], h.counts, h.slot, n,
                            mgrid->x_min, mgrid->y_min, mgrid->z_min, mech->mechanics_voxel_size,
                            mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz,
-                           mech->forces_fp32 ? h.xslab : nullptr);
+                           mech->forces_fp32 || xt ? h.xslab : nullptr);
         if ((r = check_launch(ctx, "k_mech_keys"))) return r;
     }
     if ((r = scan_exclusive(ctx, h.counts, h.offsets, n_mvox, h.partial))) return r;
     if (n) {
         const uint32_t* key = cp.u[HC_F_MECH_VOXEL - HC_NUM_F64_FIELDS];
-        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, ctx->stream, key, h.slot, h.offsets, h.tmp, n);
+        if (xt)
+            hipLaunchKernelGGL(k_scatter_slab, dim3(nb), dim3(256), 0, ctx->stream, key, h.slot, h.xslab, h.offsets,
+                               h.tmp, n);
+        else
+            hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, ctx->stream, key, h.slot, h.offsets, h.tmp, n);
         if (mech->forces_fp32) {
             if ((r = launch_bin_sort32(ctx, h.offsets, h.tmp, h.xslab, h.sorted, h.pos_of, h.xoff, n_mvox))) return r;
             hipLaunchKernelGGL(k_stable_gather32, dim3(nb), dim3(256), 0, ctx->stream, cp, key, h.offsets, h.tmp,
                                h.sorted, h.pos_of, h.xslab, reinterpret_cast<CellRec32*>(h.rec), n, mgrid->x_min,
                                mgrid->y_min, mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx,
                                mech->mech_grid_ny, h.rpart);
+        } else if (xt) {
+            // the plain form's 8-byte mark entries: the bin sort also places
+            // each agent's entry by (x slab, index), into slot (dead after
+            // the scatter)
+            hipLaunchKernelGGL(k_bin_sort32, dim3(blocks_for(n_mvox + 1, 256)), dim3(256), 0, ctx->stream, h.offsets,
+                               h.tmp, h.xslab, h.sorted, h.slot, h.xoff, n_mvox, h.pos_of, h.xtflag);
+            hipLaunchKernelGGL(k_stable_gather, dim3(nb), dim3(256), 0, ctx->stream, cp, key, h.offsets, h.tmp,
+                               h.sorted, h.pos_of, h.rec, h.pos32, n, mgrid->x_min, mgrid->y_min, mgrid->z_min,
+                               mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny, 1, h.rpart,
+                               h.slot, h.xslab, h.xtflag);
         } else {
             if ((r = launch_bin_sort(ctx, h.offsets, h.tmp, h.sorted, h.pos_of, n_mvox))) return r;
             hipLaunchKernelGGL(k_stable_gather, dim3(nb), dim3(256), 0, ctx->stream, cp, key, h.offsets, h.tmp,
                                h.sorted, h.pos_of, h.rec, mark32_ok(mech) ? h.pos32 : nullptr, n, mgrid->x_min, mgrid->y_min,
                                mgrid->z_min, mech->mechanics_voxel_size, mech->mech_grid_nx, mech->mech_grid_ny,
-                               markx_ok(mech), markx_ok(mech) ? h.rpart : nullptr);
+                               markx_ok(mech), markx_ok(mech) ? h.rpart : nullptr, (const uint32_t*)nullptr,
+                               (const uint8_t*)nullptr, (uint32_t*)nullptr);
         }
         if (mech->forces_fp32 || markx_ok(mech))
             hipLaunchKernelGGL(k_rmax, dim3(1), dim3(256), 0, ctx->stream, h.rpart, nb, h.rmax);
@@ -1212,6 +1281,13 @@ int hc_forces(hc_ctx* ctx, hc_buf* cells, hc_buf* hash, const hc_mech* mech, uin
                            mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz,
                            (float)mech->max_interaction_distance, mech->mechanics_voxel_size, h.rmax,
                            (double4*)nullptr, (const uint2*)nullptr, (const uint32_t*)nullptr);
+    else if (xtrim64_ok(mech, n_mvox) && max_cells <= (1u << 30))
+        hipLaunchKernelGGL(k_forces_xt, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, h.rec,
+                           reinterpret_cast<const uint2*>(h.pos32), h.offsets, h.xoff, h.xtflag, n,
+                           mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz,
+                           mech->max_interaction_distance, mech->mechanics_voxel_size,
+                           reinterpret_cast<const double*>(h.xtflag + 2), h.rmax, (double4*)nullptr,
+                           cp.f[HC_F_VEL_X], cp.f[HC_F_VEL_Y], cp.f[HC_F_VEL_Z], cp.f[HC_F_PRESSURE]);
     else
         hipLaunchKernelGGL(k_forces, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, h.rec,
                            mark32_ok(mech) ? h.pos32 : nullptr, h.offsets, cp.f[HC_F_VEL_X], cp.f[HC_F_VEL_Y],
@@ -1522,6 +1598,12 @@ __global__ __launch_bounds__(256) void k_m_gather(MSet a, const uint32_t* __rest
     a.kstep[p] = b;
     const Dbl3 x = a.pos[p];
     if (xt) {
+        if (p == 0) {
+            double* org = reinterpret_cast<double*>(xtflag + 2);
+            org[0] = x_min;
+            org[1] = y_min;
+            org[2] = z_min;
+        }
         const uint32_t bj = (b / mnx) % mny, bk = b / (mnx * mny);
         bool fits;
         reinterpret_cast<uint2*>(pos32)[sp] = mark_entry16(x.x - x_min, x.y - (y_min + bj * voxel),
@@ -1914,8 +1996,9 @@ int mirror_step(hc_ctx* ctx, hc_buf* hash, hc_buf* mirror, const hc_mech* mech, 
             hipLaunchKernelGGL(k_forces_xt, dim3(nb), dim3(256), 0, ctx->stream, h.rec,
                                reinterpret_cast<const uint2*>(h.pos32), h.offsets, h.xoff, h.xtflag, n,
                                mech->mech_grid_nx, mech->mech_grid_ny, mech->mech_grid_nz,
-                               mech->max_interaction_distance, mech->mechanics_voxel_size, mgrid->x_min,
-                               mgrid->y_min, mgrid->z_min, h.rmax, vp);
+                               mech->max_interaction_distance, mech->mechanics_voxel_size,
+                               reinterpret_cast<const double*>(h.xtflag + 2), h.rmax, vp, (double*)nullptr,
+                               (double*)nullptr, (double*)nullptr, (double*)nullptr);
         else
             hipLaunchKernelGGL(k_forces, dim3(nb), dim3(256), 0, ctx->stream, h.rec,
                                mark32_ok(mech) ? h.pos32 : nullptr, h.offsets, (double*)nullptr, (double*)nullptr,

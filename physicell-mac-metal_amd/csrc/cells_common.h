@@ -892,13 +892,19 @@ __device__ __attribute__((always_inline)) inline void force_rows32(
 // into rpos; every bin: its slab starts into xoff (xoff[XSLABS * b + s]).
 // Larger bins leave their members' positions to the gather (slab_rank).
 // Thread nbins writes the XSLABS entries past the last bin.
+// rec_pos non-null (the plain form's 8-byte mark entries): tmp holds
+// (agent << 2 | x slab) (k_scatter_slab; sorting these values sorts by
+// agent), rpos receives the entry position by (x slab, index), rec_pos the
+// record position by index, and thread nbins clears *xtflag.
 __device__ inline void bin_sort_slabs(const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ tmp,
                                       const uint8_t* __restrict__ xslab, uint32_t* __restrict__ sorted,
                                       uint32_t* __restrict__ rpos, uint32_t* __restrict__ xoff, uint32_t b,
-                                      uint32_t nbins) {
+                                      uint32_t nbins, uint32_t* __restrict__ rec_pos = nullptr,
+                                      uint32_t* __restrict__ xtflag = nullptr) {
     if (b == nbins) {
         const uint32_t n = offsets[nbins];
         *reinterpret_cast<uint4*>(xoff + (size_t)XSLABS * nbins) = make_uint4(n, n, n, n);
+        if (xtflag) xtflag[0] = 0u;
         return;
     }
     const uint32_t beg = offsets[b], cnt = offsets[b + 1] - beg;
@@ -909,7 +915,7 @@ __device__ inline void bin_sort_slabs(const uint32_t* __restrict__ offsets, cons
     }
     if (cnt > BIN_SORT_CAP) {
         for (uint32_t p = beg; p < beg + cnt; p++) {
-            const uint32_t s = xslab[tmp[p]];
+            const uint32_t s = rec_pos ? (tmp[p] & 3u) : xslab[tmp[p]];
             c0 += s == 0 ? 1u : 0u;
             c1 += s == 1 ? 1u : 0u;
             c2 += s == 2 ? 1u : 0u;
@@ -937,7 +943,7 @@ __device__ inline void bin_sort_slabs(const uint32_t* __restrict__ offsets, cons
             }
     uint32_t s[BIN_SORT_CAP];
 #pragma unroll
-    for (uint32_t q = 0; q < BIN_SORT_CAP; q++) s[q] = q < cnt ? xslab[a[q]] : XSLABS;
+    for (uint32_t q = 0; q < BIN_SORT_CAP; q++) s[q] = q < cnt ? (rec_pos ? (a[q] & 3u) : xslab[a[q]]) : XSLABS;
 #pragma unroll
     for (uint32_t q = 0; q < BIN_SORT_CAP; q++) {
         c0 += s[q] == 0 ? 1u : 0u;
@@ -951,9 +957,11 @@ __device__ inline void bin_sort_slabs(const uint32_t* __restrict__ offsets, cons
 #pragma unroll
     for (uint32_t q = 0; q < BIN_SORT_CAP; q++)
         if (q < cnt) {
-            sorted[beg + q] = a[q];
+            const uint32_t ag = rec_pos ? a[q] >> 2 : a[q];
+            sorted[beg + q] = ag;
+            if (rec_pos) rec_pos[ag] = beg + q;
             const uint32_t sq = s[q];
-            rpos[a[q]] = sq == 0 ? n0 : (sq == 1 ? n1 : (sq == 2 ? n2 : n3));
+            rpos[ag] = sq == 0 ? n0 : (sq == 1 ? n1 : (sq == 2 ? n2 : n3));
             n0 += sq == 0 ? 1u : 0u;
             n1 += sq == 1 ? 1u : 0u;
             n2 += sq == 2 ? 1u : 0u;
